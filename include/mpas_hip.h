@@ -1,0 +1,209 @@
+/*
+ * C ABI of the MI355X (gfx950) split-explicit dynamics step.
+ *
+ * Every pointer is a device pointer owned by the caller (PyTorch allocates
+ * them); no entry point allocates or frees device memory, every entry point
+ * runs asynchronously on the stream it is given and returns 0 on success or a
+ * nonzero status with text available from mpas_hip_last_error().
+ *
+ * Layout matches the reference runtime tensors (mpas_pytorch docs/API.md
+ * "State and controls"): row-major (rows, levels) with levels innermost and a
+ * trailing padding row per entity.  Connectivity is 0-based int32 with -1 for
+ * "no neighbour" (the host converts the reference's 1-based values).
+ */
+#ifndef MPAS_HIP_H
+#define MPAS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Static mesh, vertical metrics and base state (read only).  Field names
+ * follow the reference mesh dict (mpas_pytorch/ic_loader.py:1243-1440). */
+typedef struct mpas_hip_mesh {
+    int32_t nCells, nEdges, nVertices, nLevels;
+    int32_t maxEdges, maxEdges2, vertexDegree, maxAdv;
+    int32_t flatTerrain; /* 1: zb_cell/zb3_cell/zxu are all zero */
+    int32_t reserved0;
+    const int32_t *cellsOnEdge, *verticesOnEdge, *nEdgesOnCell, *edgesOnCell;
+    const int32_t *cellsOnCell, *verticesOnCell, *kiteForCell, *edgesOnVertex;
+    const int32_t *nEdgesOnEdge, *edgesOnEdge, *nAdvCellsForEdge, *advCellsForEdge;
+    const double *edgesOnCell_sign, *edgesOnVertex_sign, *weightsOnEdge, *kiteAreasOnVertex;
+    const double *dcEdge, *dvEdge, *invDcEdge, *invDvEdge, *invAreaCell, *invAreaTriangle;
+    const double *fEdge, *fVertex, *meshScalingDel2, *meshScalingDel4;
+    const double *adv_coefs, *adv_coefs_3rd, *defc_a, *defc_b, *coeffs_reconstruct;
+    const double *latCell, *lonCell;
+    const double *zz, *zxu, *dss, *zb_cell, *zb3_cell;
+    const double *rdzw, *rdzu, *fzm, *fzp, *rdnw, *fnm, *fnp;
+    double cf1, cf2, cf3, reserved1;
+    const double *rho_base, *rtheta_base, *exner_base;
+} mpas_hip_mesh;
+
+/* Prognostic state and the diagnostics carried between large steps
+ * (atmosphere_model.py:157-201).  Updated in place by mpas_hip_srk3_step. */
+typedef struct mpas_hip_state {
+    double *u, *w, *theta_m, *rho_zz, *scalars, *rt_diabatic_tend;
+    double *rho_p, *rtheta_p, *exner, *pressure_p, *ru, *rw;
+    double *v, *divergence, *vorticity, *ke, *pv_edge, *rho_edge;
+    double *ur_cell, *vr_cell;
+    double *uReconstructX, *uReconstructY, *uReconstructZ;
+    double *uReconstructZonal, *uReconstructMeridional;
+    double *ruAvg, *wwAvg; /* split-averaged transport fluxes of the last step */
+} mpas_hip_state;
+
+/* Caller-owned scratch for one large step; sizes are given by
+ * mpas_hip_workspace_doubles() for the mesh dimensions. */
+typedef struct mpas_hip_workspace {
+    double *base;        /* one contiguous block carved by the library */
+    int64_t n_doubles;   /* its length */
+} mpas_hip_workspace;
+
+/* Step configuration (the reference config keys in brackets). */
+typedef struct mpas_hip_config {
+    double dt;           /* large time step */
+    double len_disp;     /* [config_len_disp] */
+    double epssm;        /* [config_epssm] 0.1 */
+    double smdiv;        /* [config_smdiv] 0.1 */
+    double apvm;         /* [config_apvm_upwinding] 0.5 */
+    double coef3;        /* [config_coef_3rd_order] 0.25 */
+    double c_s;          /* [config_smagorinsky_coef] 0.125 */
+    double visc4_2dsmag; /* [config_visc4_2dsmag] 0.05 */
+    double del4u_div_factor; /* [config_del4u_div_factor] 10 */
+    int32_t ns;          /* [config_number_of_sub_steps] 2 */
+    int32_t split;       /* [config_dynamics_split_steps] 3 */
+    int32_t monotonic;   /* [config_monotonic] 1 */
+    int32_t nScalars;
+    int32_t moist_start, moist_end; /* 1-based inclusive species range */
+    int32_t diagnostics_current; /* 1: skip the first-stage diagnostic refresh */
+    int32_t reserved;
+} mpas_hip_config;
+
+/* Number of doubles of workspace needed for one large step. */
+int64_t mpas_hip_workspace_doubles(int32_t nCells, int32_t nEdges, int32_t nVertices,
+                                   int32_t nLevels, int32_t nScalars);
+
+/* One split-explicit RK3 large step in place.  Replaces the reference's
+ * atm_srk3 (mpas_pytorch/dynamics/time_integration.py:4897-4956). */
+int mpas_hip_srk3_step(const mpas_hip_mesh *mesh, mpas_hip_state *state,
+                       const mpas_hip_workspace *work, const mpas_hip_config *cfg,
+                       void *stream);
+
+/* Sizes of the ABI structs, for binding checks. */
+int mpas_hip_struct_sizes(int32_t *mesh, int32_t *state, int32_t *workspace, int32_t *config);
+
+/* ---------------------------------------------------------------------
+ * Per-kernel entry points.  Each replaces one reference kernel function
+ * (cited below) with the same argument meaning; the parity tests and the
+ * per-kernel timings use them.  Every array has the reference shape with
+ * its padding row; "scratch" arguments are caller-owned device buffers of
+ * at least the size returned by mpas_hip_kernel_scratch_doubles().
+ * ------------------------------------------------------------------- */
+
+/* Scalars of atm_compute_dyn_tend_work evaluated on the host
+ * (dyn_tend.py:3367-3400, :9616-9626). */
+typedef struct mpas_hip_dyn_params {
+    int32_t rk_step;
+    double dt;
+    double coef_3rd_order;
+    double kdiff_coef;   /* (c_s * len_disp)**2 */
+    double kdiff_cap;    /* (0.01 * len_disp**2) * (1/dt) */
+    double h_mom_eddy_visc4;
+    double h_theta_eddy_visc4;
+    double del4u_div_factor;
+    double prandtl_inv;
+} mpas_hip_dyn_params;
+
+/* Scalars of atm_advance_acoustic_step_work (acoustic_step.py:1297-1340). */
+typedef struct mpas_hip_acoustic_params {
+    int32_t small_step;
+    double dts, epssm, resm, c2;
+} mpas_hip_acoustic_params;
+
+/* Scratch doubles needed by the kernels that take a scratch buffer. */
+int64_t mpas_hip_kernel_scratch_doubles(int32_t nCells, int32_t nEdges, int32_t nVertices,
+                                        int32_t nLevels);
+
+/* atm_compute_moist_coefficients, moist_coefficients.py:124 */
+int mpas_hip_moist_coefficients(const mpas_hip_mesh *m, const double *scalars, int32_t nScalars,
+                                int32_t moist_start, int32_t moist_end, double *cqw, double *cqu,
+                                double *qtot, void *stream);
+/* atm_compute_vert_imp_coefs_work, vert_imp_coefs.py:188 */
+int mpas_hip_vert_imp_coefs(const mpas_hip_mesh *m, double dts, double epssm, const double *cqw,
+                            const double *p, const double *t, const double *rt, const double *qtot,
+                            double *cofrz, double *cofwr, double *cofwz, double *coftz, double *cofwt,
+                            double *a_tri, double *alpha_tri, double *gamma_tri, void *stream);
+/* atm_compute_solve_diagnostics_work, solve_diagnostics.py:1052;
+ * rk_step <= 0 stands for the reference's rk_step=None */
+int mpas_hip_solve_diagnostics(const mpas_hip_mesh *m, double dt, double apvm, int32_t rk_step,
+                               const double *u, const double *h, const double *v_in, double *h_edge,
+                               double *v, double *divergence, double *vorticity, double *ke,
+                               double *pv_edge, double *scratch, void *stream);
+/* atm_compute_dyn_tend_work, dyn_tend.py:10836.  inputs: u, v, w, ru, rw,
+ * ru_save, rw_save, theta_m, theta_m_save, rho_zz, rho_edge, divergence,
+ * vorticity, ke, pv_edge, pp, rr_save, cqw, cqu, rt_diabatic_tend, qtot.
+ * carried (written at rk_step 1, read later): tend_rho, tend_u_euler,
+ * tend_w_euler, tend_theta_euler, kdiff. */
+int mpas_hip_dyn_tend(const mpas_hip_mesh *m, const mpas_hip_dyn_params *p,
+                      const double *u, const double *v, const double *w, const double *ru,
+                      const double *rw, const double *ru_save, const double *rw_save,
+                      const double *theta_m, const double *theta_m_save, const double *rho_zz,
+                      const double *rho_edge, const double *divergence, const double *vorticity,
+                      const double *ke, const double *pv_edge, const double *pp,
+                      const double *rr_save, const double *cqw, const double *cqu,
+                      const double *rt_diabatic_tend, const double *qtot,
+                      double *tend_rho, double *tend_u_euler, double *tend_w_euler,
+                      double *tend_theta_euler, double *kdiff,
+                      double *tend_u, double *tend_w, double *tend_theta, double *h_divergence,
+                      double *rthdynten, double *scratch, void *stream);
+/* atm_set_smlstep_pert_variables_work, smlstep_pert.py:373 (w_tend in place) */
+int mpas_hip_smlstep_pert(const mpas_hip_mesh *m, const double *u_tend, double *w_tend, void *stream);
+/* atm_advance_acoustic_step_work, acoustic_step.py:1297.  theta_m is the
+ * time-level-1 theta_m (theta_m_init). */
+int mpas_hip_acoustic_step(const mpas_hip_mesh *m, const mpas_hip_acoustic_params *p,
+                           const double *cofrz, const double *cofwr, const double *cofwz,
+                           const double *coftz, const double *cofwt, const double *a_tri,
+                           const double *alpha_tri, const double *gamma_tri, const double *rho_zz,
+                           const double *theta_m, const double *exner, const double *cqu,
+                           const double *tend_ru, const double *tend_rho, const double *tend_rt,
+                           const double *tend_rw, const double *w, const double *rw,
+                           const double *rw_save, double *ru_p, double *ruAvg, double *rho_pp,
+                           double *rtheta_pp, double *rtheta_pp_old, double *rw_p, double *wwAvg,
+                           void *stream);
+/* atm_divergence_damping_3d, divergence_damping.py:299 (ru_p in place) */
+int mpas_hip_divergence_damping(const mpas_hip_mesh *m, double dts, double smdiv, double len_disp,
+                                const double *theta_m, const double *rtheta_pp,
+                                const double *rtheta_pp_old, double *ru_p, void *stream);
+/* atm_recover_large_step_variables_work, recover_large_step.py:1090 */
+int mpas_hip_recover_large_step(const mpas_hip_mesh *m, int32_t rk_step, double dt, int32_t ns,
+                                const double *rho_p_save, const double *rho_pp,
+                                const double *rtheta_p_save, const double *rtheta_pp,
+                                const double *rt_diabatic_tend, const double *rw_save,
+                                const double *rw_p, const double *ru_save, const double *ru_p,
+                                double *wwAvg, double *ruAvg, double *rho_p, double *rho_zz,
+                                double *rtheta_p, double *theta_m, double *exner, double *pressure_p,
+                                double *rw, double *w, double *ru, double *u, void *stream);
+/* atm_advance_scalars_work, advance_scalars.py:1272 (weights per RK stage) */
+int mpas_hip_advance_scalars(const mpas_hip_mesh *m, int32_t nScalars, double dt, double coef3,
+                             double weight_old, double weight_new, const double *scalar_old,
+                             const double *scalar_new_in, const double *rho_zz_old,
+                             const double *rho_zz_new, const double *uhAvg, const double *wwAvg,
+                             double *scalar_new_out, double *horiz_flux, void *stream);
+/* atm_advance_scalars_mono_work, advance_scalars_mono.py:2878 */
+int mpas_hip_advance_scalars_mono(const mpas_hip_mesh *m, int32_t nScalars, double dt, double coef3,
+                                  const double *scalars_old, const double *scalars_new_in,
+                                  const double *rho_zz_old, const double *uhAvg,
+                                  const double *wwAvg, double *scalars_new_out,
+                                  double *rho_zz_int, double *scratch, void *stream);
+/* mpas_reconstruct_2d, vector_reconstruction.py:311 */
+int mpas_hip_reconstruct(const mpas_hip_mesh *m, const double *u, double *ux, double *uy,
+                         double *uz, double *uzonal, double *umerid, void *stream);
+
+const char *mpas_hip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPAS_HIP_H */

@@ -1,0 +1,126 @@
+// Shared definitions for the gfx950 split-explicit dynamics kernels.
+//
+// Arithmetic is IEEE binary64 with round-to-nearest and no contraction
+// (built with -ffp-contract=off): every expression keeps the association of
+// the reference statement it restates, so results match the reference CPU
+// float64 path bit for bit apart from libm calls (pow, sin, cos).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/mpas_hip.h"
+
+#define MPAS_GRAVITY 9.80616
+#define MPAS_RGAS 287.0
+#define MPAS_CP 1004.5
+#define MPAS_P0 100000.0
+
+namespace mpas {
+
+// Host-evaluated kernel scalars (layouts fixed by the C ABI header).
+using DynParams = mpas_hip_dyn_params;
+using AcousticParams = mpas_hip_acoustic_params;
+
+__device__ __forceinline__ double sign_pm(double x) { return (x >= 0.0) ? 1.0 : -1.0; }
+__device__ __forceinline__ double max0(double x) { return (x < 0.0) ? 0.0 : x; }
+__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double dmax(double a, double b) { return (b > a) ? b : a; }
+
+// flux4/flux3 statement functions (dyn_tend.py:839-890, advance_scalars.py:128-162)
+__device__ __forceinline__ double flux4(double qm2, double qm1, double q0, double qp1, double ua) {
+    return ua * (7.0 * (q0 + qm1) - (qp1 + qm2)) / 12.0;
+}
+__device__ __forceinline__ double flux3(double qm2, double qm1, double q0, double qp1, double ua,
+                                        double coef3) {
+    return flux4(qm2, qm1, q0, qp1, ua) + coef3 * fabs(ua) * ((qp1 - qm2) - 3.0 * (q0 - qm1)) / 12.0;
+}
+
+constexpr int kBlock = 256;
+
+inline unsigned blocks_for(long long n, int block = kBlock) {
+    return (unsigned)((n + block - 1) / block);
+}
+
+// Zero-fill by kernel rather than hipMemsetAsync so launch sequences stay
+// safe to capture in a HIP graph.
+void fill_zero(double *p, size_t n, hipStream_t s);
+
+// Error plumbing: every launcher records the first HIP error.
+void set_error(const std::string &msg);
+int check_launch(const char *what);
+
+}  // namespace mpas
+
+// Internal launchers (one per kernel family).
+namespace mpas {
+void launch_moist(const mpas_hip_mesh &m, const double *scalars, int S, int ms, int me, double *cqw,
+                  double *cqu, double *qtot, hipStream_t s);
+void launch_vimp(const mpas_hip_mesh &m, double dts, double epssm, const double *cqw, const double *p,
+                 const double *t, const double *rt, const double *qtot, double *cofrz, double *cofwr,
+                 double *cofwz, double *coftz, double *cofwt, double *a_tri, double *alpha_tri,
+                 double *gamma_tri, hipStream_t s);
+void launch_diagnostics(const mpas_hip_mesh &m, double dt, double apvm, int rk_step, const double *u,
+                        const double *h, const double *v_in, double *h_edge, double *v, double *div,
+                        double *vort, double *ke, double *pv_edge, double *scratch, hipStream_t s);
+size_t diagnostics_scratch_doubles(const mpas_hip_mesh &m);
+
+struct DynInputs {
+    const double *u, *v, *w, *ru, *rw, *ru_save, *rw_save, *theta_m, *theta_m_save, *rho_zz;
+    const double *rho_edge, *divergence, *vorticity, *ke, *pv_edge, *pp, *rr_save, *cqw, *cqu;
+    const double *rt_diabatic_tend, *qtot;
+};
+struct DynCarried {
+    double *tend_rho, *tend_u_euler, *tend_w_euler, *tend_theta_euler, *kdiff;
+};
+struct DynOutputs {
+    double *tend_u, *tend_w, *tend_theta, *h_divergence, *rthdynten;
+};
+void launch_dyn_tend(const mpas_hip_mesh &m, const DynParams &P, const DynInputs &in,
+                     const DynCarried &io, const DynOutputs &out, double *scratch, hipStream_t s);
+size_t dyn_tend_scratch_doubles(const mpas_hip_mesh &m);
+
+void launch_smlstep(const mpas_hip_mesh &m, const double *u_tend, double *w_tend, hipStream_t s);
+
+struct AcousticInputs {
+    const double *cofrz, *cofwr, *cofwz, *coftz, *cofwt, *a_tri, *alpha_tri, *gamma_tri;
+    const double *rho_zz, *theta_m, *exner, *cqu, *tend_ru, *tend_rho, *tend_rt, *tend_rw, *w, *rw,
+        *rw_save;
+};
+struct AcousticState {
+    double *ru_p, *ruAvg, *rho_pp, *rtheta_pp, *rtheta_pp_old, *rw_p, *wwAvg;
+};
+void launch_acoustic(const mpas_hip_mesh &m, const AcousticParams &P, const AcousticInputs &in,
+                     const AcousticState &io, hipStream_t s);
+void launch_divdamp(const mpas_hip_mesh &m, double dts, double smdiv, double len_disp,
+                    const double *theta_m, const double *rtheta_pp, const double *rtheta_pp_old,
+                    double *ru_p, hipStream_t s);
+
+struct RecoverInputs {
+    const double *rho_p_save, *rho_pp, *rtheta_p_save, *rtheta_pp, *rt_diabatic_tend, *rw_save, *rw_p,
+        *ru_save, *ru_p;
+};
+struct RecoverOutputs {
+    double *wwAvg, *ruAvg, *rho_p, *rho_zz, *rtheta_p, *theta_m, *exner, *pressure_p, *rw, *w, *ru, *u;
+};
+void launch_recover(const mpas_hip_mesh &m, int rk_step, double dt, int ns, const RecoverInputs &in,
+                    const RecoverOutputs &out, hipStream_t s);
+
+void launch_advance_scalars(const mpas_hip_mesh &m, int S, double dt, double coef3, double w_old,
+                            double w_new, const double *scalar_old, const double *scalar_new_in,
+                            const double *rho_zz_old, const double *rho_zz_new, const double *uhAvg,
+                            const double *wwAvg, double *scalar_new_out, double *horiz_flux,
+                            hipStream_t s);
+void launch_advance_scalars_mono(const mpas_hip_mesh &m, int S, double dt, double coef3,
+                                 const double *scalars_old, const double *scalars_new_in,
+                                 const double *rho_zz_old, const double *uhAvg, const double *wwAvg,
+                                 double *scalars_new_out, double *rho_zz_int, double *scratch,
+                                 hipStream_t s);
+size_t mono_scratch_doubles(const mpas_hip_mesh &m);
+
+void launch_reconstruct(const mpas_hip_mesh &m, const double *u, double *ux, double *uy, double *uz,
+                        double *uzonal, double *umerid, hipStream_t s);
+}  // namespace mpas

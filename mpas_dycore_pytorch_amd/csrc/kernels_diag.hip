@@ -1,0 +1,160 @@
+// atm_compute_solve_diagnostics_work (solve_diagnostics.py:531-756):
+// h_edge, kinetic energy with the 0.375 vertex blend, vorticity, divergence,
+// APVM-upwinded pv_edge and (rk_step 3 / first call) the TRiSK tangential
+// velocity.  Four gather passes, one thread per (entity, level); every slot
+// sum runs left to right in the reference's slot order.
+#include "common.hpp"
+
+namespace mpas {
+
+size_t diagnostics_scratch_doubles(const mpas_hip_mesh &m) {
+    const size_t L = m.nLevels;
+    return (size_t)m.nEdges * L + 2 * (size_t)m.nVertices * L + (size_t)m.nCells * L;
+}
+
+__global__ void k_diag_edges1(mpas_hip_mesh m, const double *__restrict__ u, const double *__restrict__ h,
+                              double *__restrict__ h_edge, double *__restrict__ ke_edge) {
+    const int L = m.nLevels, nE = m.nEdges;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (nE + 1) * L) return;
+    int e = i / L, k = i - e * L;
+    if (e == nE) {
+        h_edge[i] = 0.0;
+        return;
+    }
+    int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+    h_edge[i] = 0.5 * (h[c1 * L + k] + h[c2 * L + k]);
+    double efac = m.dcEdge[e] * m.dvEdge[e];
+    double uu = u[i];
+    ke_edge[i] = efac * (uu * uu);
+}
+
+__global__ void k_diag_vertices(mpas_hip_mesh m, const double *__restrict__ u,
+                                const double *__restrict__ ke_edge, double *__restrict__ vort,
+                                double *__restrict__ ke_vertex, double *__restrict__ pv_vertex) {
+    const int L = m.nLevels, nV = m.nVertices;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (nV + 1) * L) return;
+    int vx = i / L, k = i - vx * L;
+    if (vx == nV) {
+        vort[i] = 0.0;
+        return;
+    }
+    double s = 0.0, sk = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        int e = m.edgesOnVertex[3 * vx + j];
+        double w = m.edgesOnVertex_sign[3 * vx + j] * m.dcEdge[e];
+        double c = w * u[e * L + k];
+        double kk = ke_edge[e * L + k];
+        s = (j == 0) ? c : s + c;
+        sk = (j == 0) ? kk : sk + kk;
+    }
+    double vv = s * m.invAreaTriangle[vx];
+    vort[i] = vv;
+    ke_vertex[i] = 0.25 * m.invAreaTriangle[vx] * sk;
+    pv_vertex[i] = m.fVertex[vx] + vv;
+}
+
+__global__ void k_diag_cells(mpas_hip_mesh m, const double *__restrict__ u, const double *__restrict__ ke_edge,
+                             const double *__restrict__ ke_vertex, const double *__restrict__ pv_vertex,
+                             double *__restrict__ div, double *__restrict__ ke, double *__restrict__ pv_cell) {
+    const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (nC + 1) * L) return;
+    int c = i / L, k = i - c * L;
+    if (c == nC) {
+        div[i] = 0.0;
+        ke[i] = 0.0;
+        return;
+    }
+    const int n = m.nEdgesOnCell[c];
+    const double ia = m.invAreaCell[c];
+    double d = 0.0, kc = 0.0;
+    for (int j = 0; j < n; ++j) {
+        int e = m.edgesOnCell[c * ME + j];
+        double w = m.edgesOnCell_sign[c * ME + j] * m.dvEdge[e];
+        double cd = w * u[e * L + k];
+        double ck = 0.25 * ke_edge[e * L + k];
+        d = (j == 0) ? cd : d + cd;
+        kc = (j == 0) ? ck : kc + ck;
+    }
+    div[i] = d * ia;
+    const double ke_fact = 1.0 - 0.375;
+    double tot = ke_fact * (kc * ia);
+    double pc = 0.0;
+    for (int j = 0; j < n; ++j) {
+        int vx = m.verticesOnCell[c * ME + j];
+        double ka = m.kiteAreasOnVertex[3 * vx + m.kiteForCell[c * ME + j]];
+        double contrib = (1.0 - ke_fact) * ka;
+        contrib = contrib * ke_vertex[vx * L + k];
+        contrib = contrib * ia;
+        tot = tot + contrib;
+        double cp = ka * pv_vertex[vx * L + k] * ia;
+        pc = (j == 0) ? cp : pc + cp;
+    }
+    ke[i] = tot;
+    pv_cell[i] = pc;
+}
+
+__global__ void k_diag_edges2(mpas_hip_mesh m, double apvm_dt, int apvm_on, int reconstruct_v,
+                              const double *__restrict__ u, const double *__restrict__ v_in,
+                              const double *__restrict__ pv_vertex, const double *__restrict__ pv_cell,
+                              double *__restrict__ v, double *__restrict__ pv_edge) {
+    const int L = m.nLevels, nE = m.nEdges, ME2 = m.maxEdges2;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (nE + 1) * L) return;
+    int e = i / L, k = i - e * L;
+    if (e == nE) {
+        v[i] = 0.0;
+        pv_edge[i] = 0.0;
+        return;
+    }
+    double vv;
+    if (reconstruct_v) {
+        vv = 0.0;
+        const int ne = m.nEdgesOnEdge[e];
+        for (int j = 0; j < ne; ++j) {
+            int ee = m.edgesOnEdge[e * ME2 + j];
+            double c = m.weightsOnEdge[e * ME2 + j] * u[ee * L + k];
+            vv = (j == 0) ? c : vv + c;
+        }
+    } else {
+        vv = v_in[i];
+    }
+    v[i] = vv;
+    int v1 = m.verticesOnEdge[2 * e], v2 = m.verticesOnEdge[2 * e + 1];
+    double p1 = pv_vertex[v1 * L + k], p2 = pv_vertex[v2 * L + k];
+    double pe = 0.5 * (p1 + p2);
+    if (apvm_on) {
+        int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+        double grad_pvt = (p2 - p1) * m.invDvEdge[e];
+        double grad_pvn = (pv_cell[c2 * L + k] - pv_cell[c1 * L + k]) * m.invDcEdge[e];
+        double delta = apvm_dt * (vv * grad_pvt + u[i] * grad_pvn);
+        pe = pe - delta;
+    }
+    pv_edge[i] = pe;
+}
+
+void launch_diagnostics(const mpas_hip_mesh &m, double dt, double apvm, int rk_step, const double *u,
+                        const double *h, const double *v_in, double *h_edge, double *v, double *div,
+                        double *vort, double *ke, double *pv_edge, double *scratch, hipStream_t s) {
+    const int L = m.nLevels, nC = m.nCells, nE = m.nEdges, nV = m.nVertices;
+    double *ke_edge = scratch;
+    double *ke_vertex = ke_edge + (size_t)nE * L;
+    double *pv_vertex = ke_vertex + (size_t)nV * L;
+    double *pv_cell = pv_vertex + (size_t)nV * L;
+    const unsigned be = blocks_for((long long)(nE + 1) * L), bv = blocks_for((long long)(nV + 1) * L),
+                   bc = blocks_for((long long)(nC + 1) * L);
+    // ke_edge / ke_vertex / pv_vertex / pv_cell have no padding row: the
+    // padding-row threads of the pass that fills them only zero the outputs.
+    hipLaunchKernelGGL(k_diag_edges1, dim3(be), dim3(kBlock), 0, s, m, u, h, h_edge, ke_edge);
+    hipLaunchKernelGGL(k_diag_vertices, dim3(bv), dim3(kBlock), 0, s, m, u, ke_edge, vort, ke_vertex, pv_vertex);
+    hipLaunchKernelGGL(k_diag_cells, dim3(bc), dim3(kBlock), 0, s, m, u, ke_edge, ke_vertex, pv_vertex, div, ke,
+                       pv_cell);
+    const int reconstruct_v = (rk_step <= 0 || rk_step == 3) ? 1 : 0;
+    hipLaunchKernelGGL(k_diag_edges2, dim3(be), dim3(kBlock), 0, s, m, apvm * dt, apvm > 0.0 ? 1 : 0,
+                       reconstruct_v, u, v_in, pv_vertex, pv_cell, v, pv_edge);
+}
+
+}  // namespace mpas

@@ -1,0 +1,422 @@
+// atm_compute_dyn_tend_work (dyn_tend.py:9390-10833) as seven gather passes,
+// one thread per (entity, level).  Section references:
+//   Smagorinsky kdiff                :3299-3465
+//   density / h_divergence / dpdz    :1045-1094
+//   u: vertical flux, PV flux, KE grad, divergence coupling, PGF
+//                                    :1097-1131, :1455-1791, :2178-2282
+//   u: del2 / del4 mixing, final     :2508-2730, :2956-2999
+//   w/theta horizontal advection     :3627-3747, :4174-4380
+//   theta RK>1 perturbation flux     :5901-6014
+//   w/theta horizontal mixing        :5315-5900, ordered del4 :6485-6553
+//   w vertical flux, PGF, buoyancy   :4415-4657
+//   theta vertical flux, diabatic    :4803-5057
+// Active configuration: 2d_smagorinsky, no vertical eddy viscosity, no u
+// Rayleigh damping, no curvature term, no CAM damping, no physics tendencies.
+#include "common.hpp"
+
+namespace mpas {
+
+size_t dyn_tend_scratch_doubles(const mpas_hip_mesh &m) {
+    const size_t L = m.nLevels, C = m.nCells + 1, E = m.nEdges + 1, V = m.nVertices + 1;
+    // dpdz, delsq_u, delsq_div, delsq_vort, delsq_w, delsq_theta
+    return C * L + E * L + C * L + V * L + C * (L + 1) + C * L;
+}
+
+struct DynScratch {
+    double *dpdz, *delsq_u, *delsq_div, *delsq_vort, *delsq_w, *delsq_theta;
+};
+
+static DynScratch carve(const mpas_hip_mesh &m, double *base) {
+    const size_t L = m.nLevels, C = m.nCells + 1, E = m.nEdges + 1, V = m.nVertices + 1;
+    DynScratch s;
+    s.dpdz = base;
+    s.delsq_u = s.dpdz + C * L;
+    s.delsq_div = s.delsq_u + E * L;
+    s.delsq_vort = s.delsq_div + C * L;
+    s.delsq_w = s.delsq_vort + V * L;
+    s.delsq_theta = s.delsq_w + C * (L + 1);
+    return s;
+}
+
+// P1: Smagorinsky eddy viscosity (first RK stage only)
+__global__ void k_dyn_kdiff(mpas_hip_mesh m, DynParams P, const double *__restrict__ u,
+                            const double *__restrict__ v, double *__restrict__ kdiff) {
+    const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (nC + 1) * L) return;
+    int c = i / L, k = i - c * L;
+    if (c == nC) {
+        kdiff[i] = 0.0;
+        return;
+    }
+    const int n = m.nEdgesOnCell[c];
+    double dd = 0.0, dof = 0.0;
+    for (int j = 0; j < n; ++j) {
+        int e = m.edgesOnCell[c * ME + j];
+        double a = m.defc_a[c * ME + j], b = m.defc_b[c * ME + j];
+        double us = u[e * L + k], vs = v[e * L + k];
+        dd = (dd + a * us) - b * vs;
+        dof = (dof + b * us) + a * vs;
+    }
+    double def = sqrt(dd * dd + dof * dof);
+    kdiff[i] = dmin(P.kdiff_coef * def, P.kdiff_cap);
+}
+
+// P2: mass-flux divergence, density tendency, buoyancy dpdz; first stage also
+// the del2 w / theta mixing sums and their del2 fields for the del4 pass.
+__global__ void k_dyn_cells_a(mpas_hip_mesh m, DynParams P, const double *__restrict__ ru,
+                              const double *__restrict__ rw, const double *__restrict__ rr_save,
+                              const double *__restrict__ qtot, const double *__restrict__ w,
+                              const double *__restrict__ theta_m, const double *__restrict__ rho_edge,
+                              const double *__restrict__ kdiff, double *__restrict__ h_div,
+                              double *__restrict__ tend_rho, double *__restrict__ dpdz,
+                              double *__restrict__ delsq_w, double *__restrict__ tend_w_euler,
+                              double *__restrict__ delsq_theta, double *__restrict__ tend_theta_euler) {
+    const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (nC + 1) * Lw) return;
+    int c = i / Lw, k = i - c * Lw;
+    const bool rk1 = (P.rk_step == 1);
+    if (c == nC) {
+        if (k < L) h_div[c * L + k] = 0.0;
+        return;
+    }
+    const int n = m.nEdgesOnCell[c];
+    const double ia = m.invAreaCell[c];
+    if (k < L) {
+        double s = 0.0;
+        for (int j = 0; j < n; ++j) {
+            int e = m.edgesOnCell[c * ME + j];
+            double wgt = m.edgesOnCell_sign[c * ME + j] * m.dvEdge[e];
+            double cc = wgt * ru[e * L + k];
+            s = (j == 0) ? cc : s + cc;
+        }
+        double hd = s * ia;
+        h_div[c * L + k] = hd;
+        if (rk1) {
+            const int o = c * L + k;
+            tend_rho[o] = -hd - m.rdzw[k] * (rw[c * Lw + k + 1] - rw[c * Lw + k]);
+            dpdz[o] = -MPAS_GRAVITY * (m.rho_base[o] * qtot[o] + rr_save[o] * (1.0 + qtot[o]));
+            // theta del2 (dyn_tend.py:5563-5640)
+            double dsum = 0.0, tsum = 0.0;
+            for (int j = 0; j < n; ++j) {
+                int e = m.edgesOnCell[c * ME + j];
+                int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+                double es = ia * m.edgesOnCell_sign[c * ME + j] * m.dvEdge[e] * m.invDcEdge[e];
+                double prs = P.prandtl_inv * m.meshScalingDel2[e];
+                double bf = es * (theta_m[c2 * L + k] - theta_m[c1 * L + k]) * rho_edge[e * L + k];
+                double sf = bf * 0.5 * (kdiff[c1 * L + k] + kdiff[c2 * L + k]) * prs;
+                dsum = (j == 0) ? bf : dsum + bf;
+                tsum = (j == 0) ? sf : tsum + sf;
+            }
+            delsq_theta[o] = dsum;
+            tend_theta_euler[o] = tsum;
+        }
+    }
+    if (rk1) {
+        if (k >= 1 && k < L) {
+            // w del2 (dyn_tend.py:5315-5420)
+            double dsum = 0.0, tsum = 0.0;
+            for (int j = 0; j < n; ++j) {
+                int e = m.edgesOnCell[c * ME + j];
+                int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+                double es = 0.5 * ia * m.edgesOnCell_sign[c * ME + j] * m.dvEdge[e] * m.invDcEdge[e];
+                double bf = es * (rho_edge[e * L + k] + rho_edge[e * L + k - 1]) * (w[c2 * Lw + k] - w[c1 * Lw + k]);
+                double sf = bf * m.meshScalingDel2[e] * 0.25 *
+                            (kdiff[c1 * L + k] + kdiff[c2 * L + k] + kdiff[c1 * L + k - 1] + kdiff[c2 * L + k - 1]);
+                dsum = (j == 0) ? bf : dsum + bf;
+                tsum = (j == 0) ? sf : tsum + sf;
+            }
+            delsq_w[i] = dsum;
+            tend_w_euler[i] = tsum;
+        } else {
+            tend_w_euler[i] = 0.0;
+        }
+    }
+}
+
+// P4: normal-velocity tendency (vertical flux, PV flux, KE gradient,
+// divergence coupling); first stage also PGF and del2 mixing into tend_u_euler.
+__global__ void k_dyn_edges_a(mpas_hip_mesh m, DynParams P, const double *__restrict__ u,
+                              const double *__restrict__ rw, const double *__restrict__ pv_edge,
+                              const double *__restrict__ ke, const double *__restrict__ h_div,
+                              const double *__restrict__ rho_edge, const double *__restrict__ pp,
+                              const double *__restrict__ dpdz, const double *__restrict__ cqu,
+                              const double *__restrict__ divergence, const double *__restrict__ vorticity,
+                              const double *__restrict__ kdiff, double *__restrict__ tend_u,
+                              double *__restrict__ tend_u_euler, double *__restrict__ delsq_u) {
+    const int L = m.nLevels, nE = m.nEdges, ME2 = m.maxEdges2, Lw = L + 1;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (nE + 1) * L) return;
+    int e = i / L, k = i - e * L;
+    if (e == nE) {
+        tend_u[i] = 0.0;
+        return;
+    }
+    const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+    const double *ue = u + e * L;
+    const double *fzm = m.fzm, *fzp = m.fzp;
+    auto wduz = [&](int kk) -> double {
+        double erw = 0.5 * (rw[c1 * Lw + kk] + rw[c2 * Lw + kk]);
+        if (kk == 1 || kk == L - 1) return erw * (fzm[kk] * ue[kk] + fzp[kk] * ue[kk - 1]);
+        return flux3(ue[kk - 2], ue[kk - 1], ue[kk], ue[kk + 1], erw, 1.0);
+    };
+    double delta;
+    if (L == 1) delta = 0.0;
+    else if (k == 0) delta = wduz(1);
+    else if (k == L - 1) delta = -wduz(L - 1);
+    else delta = wduz(k + 1) - wduz(k);
+    double vert = -m.rdzw[k] * delta;
+    double q = 0.0;
+    const double pvs = pv_edge[i];
+    const int ne = m.nEdgesOnEdge[e];
+    for (int j = 0; j < ne; ++j) {
+        int ee = m.edgesOnEdge[e * ME2 + j];
+        double workpv = 0.5 * (pvs + pv_edge[ee * L + k]);
+        double cc = m.weightsOnEdge[e * ME2 + j] * u[ee * L + k] * workpv;
+        q = (j == 0) ? cc : q + cc;
+    }
+    double ke_delta = ke[c2 * L + k] - ke[c1 * L + k];
+    double hsum = h_div[c1 * L + k] + h_div[c2 * L + k];
+    double base = rho_edge[i] * (q - ke_delta * m.invDcEdge[e]) - ue[k] * 0.5 * hsum;
+    tend_u[i] = vert + base;
+    if (P.rk_step == 1) {
+        double zzs = m.zz[c2 * L + k] + m.zz[c1 * L + k];
+        double ppd = pp[c2 * L + k] - pp[c1 * L + k];
+        double dps = dpdz[c1 * L + k] + dpdz[c2 * L + k];
+        double eu = -cqu[i] * (ppd * m.invDcEdge[e] / (0.5 * zzs) - 0.5 * m.zxu[i] * dps);
+        const int v1 = m.verticesOnEdge[2 * e], v2 = m.verticesOnEdge[2 * e + 1];
+        double r_dc = m.invDcEdge[e];
+        double r_dv = dmin(m.invDvEdge[e], 4.0 * m.invDcEdge[e]);
+        double ud = (divergence[c2 * L + k] - divergence[c1 * L + k]) * r_dc -
+                    (vorticity[v2 * L + k] - vorticity[v1 * L + k]) * r_dv;
+        delsq_u[i] = ud;
+        double kdiffu = 0.5 * (kdiff[c1 * L + k] + kdiff[c2 * L + k]);
+        eu = eu + rho_edge[i] * kdiffu * ud * m.meshScalingDel2[e];
+        tend_u_euler[i] = eu;
+    }
+}
+
+// P5: del2 of the u Laplacian at vertices and cells (del4 first half)
+__global__ void k_dyn_del4_vc(mpas_hip_mesh m, const double *__restrict__ delsq_u,
+                              double *__restrict__ delsq_vort, double *__restrict__ delsq_div) {
+    const int L = m.nLevels, nC = m.nCells, nV = m.nVertices, ME = m.maxEdges;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nV * L) {
+        int vx = i / L, k = i - vx * L;
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            int e = m.edgesOnVertex[3 * vx + j];
+            double wgt = m.invAreaTriangle[vx] * m.dcEdge[e] * m.edgesOnVertex_sign[3 * vx + j];
+            double cc = wgt * delsq_u[e * L + k];
+            s = (j == 0) ? cc : s + cc;
+        }
+        delsq_vort[i] = s;
+    }
+    if (i < nC * L) {
+        int c = i / L, k = i - c * L;
+        const int n = m.nEdgesOnCell[c];
+        double s = 0.0;
+        for (int j = 0; j < n; ++j) {
+            int e = m.edgesOnCell[c * ME + j];
+            double wgt = m.invAreaCell[c] * m.dvEdge[e] * m.edgesOnCell_sign[c * ME + j];
+            double cc = wgt * delsq_u[e * L + k];
+            s = (j == 0) ? cc : s + cc;
+        }
+        delsq_div[i] = s;
+    }
+}
+
+// P6: del4 u mixing (first stage) and the final u tendency
+__global__ void k_dyn_edges_b(mpas_hip_mesh m, DynParams P, int do_del4, const double *__restrict__ rho_edge,
+                              const double *__restrict__ delsq_div, const double *__restrict__ delsq_vort,
+                              double *__restrict__ tend_u_euler, double *__restrict__ tend_u) {
+    const int L = m.nLevels, nE = m.nEdges;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nE * L) return;
+    int e = i / L, k = i - e * L;
+    double eu = tend_u_euler[i];
+    if (do_del4) {
+        const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+        const int v1 = m.verticesOnEdge[2 * e], v2 = m.verticesOnEdge[2 * e + 1];
+        double ums = m.meshScalingDel4[e] * P.h_mom_eddy_visc4;
+        double r_dc = ums * P.del4u_div_factor * m.invDcEdge[e];
+        double r_dv = ums * dmin(m.invDvEdge[e], 4.0 * m.invDcEdge[e]);
+        double ud = rho_edge[i] * ((delsq_div[c2 * L + k] - delsq_div[c1 * L + k]) * r_dc -
+                                   (delsq_vort[v2 * L + k] - delsq_vort[v1 * L + k]) * r_dv);
+        eu = eu - ud;
+        tend_u_euler[i] = eu;
+    }
+    tend_u[i] = tend_u[i] + eu;
+}
+
+// P7: w and theta tendencies (advection, mixing finish, PGF/buoyancy,
+// vertical transport, diabatic forcing), one thread per (cell, w-level).
+__global__ void k_dyn_cells_b(mpas_hip_mesh m, DynParams P, const double *__restrict__ ru,
+                              const double *__restrict__ rw, const double *__restrict__ ru_save,
+                              const double *__restrict__ rw_save, const double *__restrict__ w,
+                              const double *__restrict__ theta_m, const double *__restrict__ theta_m_save,
+                              const double *__restrict__ rho_zz, const double *__restrict__ pp,
+                              const double *__restrict__ cqw, const double *__restrict__ rt_diab,
+                              const double *__restrict__ dpdz, const double *__restrict__ delsq_w,
+                              const double *__restrict__ delsq_theta, const double *__restrict__ tend_rho,
+                              double *__restrict__ tend_w_euler, double *__restrict__ tend_theta_euler,
+                              double *__restrict__ tend_w, double *__restrict__ tend_theta,
+                              double *__restrict__ rthdynten) {
+    const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, MA = m.maxAdv, Lw = L + 1;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (nC + 1) * Lw) return;
+    int c = i / Lw, k = i - c * Lw;
+    if (c == nC) {
+        tend_w[i] = 0.0;
+        if (k < L) {
+            tend_theta[c * L + k] = 0.0;
+            rthdynten[c * L + k] = 0.0;
+        }
+        return;
+    }
+    const bool rk1 = (P.rk_step == 1);
+    const int n = m.nEdgesOnCell[c];
+    const double ia = m.invAreaCell[c];
+    const double *fzm = m.fzm, *fzp = m.fzp;
+    // ---------------- w at interface k
+    if (k >= 1 && k < L) {
+        double hadv = 0.0;
+        for (int j = 0; j < n; ++j) {
+            int e = m.edgesOnCell[c * ME + j];
+            double sgn = m.edgesOnCell_sign[c * ME + j];
+            double rue = fzm[k] * ru[e * L + k] + fzp[k] * ru[e * L + k - 1];
+            double sr = sign_pm(rue);
+            const int na = m.nAdvCellsForEdge[e];
+            double fs = 0.0;
+            for (int a = 0; a < na; ++a) {
+                int ac = m.advCellsForEdge[e * MA + a];
+                double wgt = m.adv_coefs[e * MA + a] + sr * m.adv_coefs_3rd[e * MA + a];
+                double cc = wgt * w[ac * Lw + k];
+                fs = (a == 0) ? cc : fs + cc;
+            }
+            double cc = -sgn * (rue * fs);
+            hadv = (j == 0) ? cc : hadv + cc;
+        }
+        double eu = tend_w_euler[i];
+        if (rk1) {
+            if (P.h_mom_eddy_visc4 > 0.0) {
+                double r_area = P.h_mom_eddy_visc4 * ia;
+                for (int j = 0; j < n; ++j) {
+                    int e = m.edgesOnCell[c * ME + j];
+                    int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+                    double es = m.meshScalingDel4[e] * r_area * m.dvEdge[e] * m.edgesOnCell_sign[c * ME + j] *
+                                m.invDcEdge[e];
+                    eu = eu - es * (delsq_w[c2 * Lw + k] - delsq_w[c1 * Lw + k]);
+                }
+            }
+            const int oc = c * L;
+            eu = eu - cqw[i] * (m.rdzu[k] * (pp[oc + k] - pp[oc + k - 1]) -
+                                (fzm[k] * dpdz[oc + k] + fzp[k] * dpdz[oc + k - 1]));
+            tend_w_euler[i] = eu;
+        }
+        const double *wc = w + c * Lw, *rwc = rw + c * Lw;
+        auto wdwz = [&](int jj) -> double {
+            if (jj == 0 || jj == L - 2) return 0.25 * (rwc[jj + 1] + rwc[jj]) * (wc[jj + 1] + wc[jj]);
+            return flux3(wc[jj - 1], wc[jj], wc[jj + 1], wc[jj + 2], 0.5 * (rwc[jj + 1] + rwc[jj]), 1.0);
+        };
+        double delta = (k == L - 1) ? -wdwz(L - 2) : wdwz(k) - wdwz(k - 1);
+        double inner = hadv * ia - m.rdzu[k] * delta;
+        tend_w[i] = inner + eu;
+    } else {
+        tend_w[i] = 0.0;
+    }
+    // ---------------- theta at level k
+    if (k < L) {
+        const int o = c * L + k;
+        double thadv = 0.0;
+        for (int j = 0; j < n; ++j) {
+            int e = m.edgesOnCell[c * ME + j];
+            double sgn = m.edgesOnCell_sign[c * ME + j];
+            double rue = ru[e * L + k];
+            double sr = sign_pm(rue);
+            const int na = m.nAdvCellsForEdge[e];
+            double fs = 0.0;
+            for (int a = 0; a < na; ++a) {
+                int ac = m.advCellsForEdge[e * MA + a];
+                double wgt = m.adv_coefs[e * MA + a] + sr * m.adv_coefs_3rd[e * MA + a];
+                double cc = wgt * theta_m[ac * L + k];
+                fs = (a == 0) ? cc : fs + cc;
+            }
+            double cc = -sgn * (rue * fs);
+            thadv = (j == 0) ? cc : thadv + cc;
+        }
+        if (!rk1) {
+            for (int j = 0; j < n; ++j) {
+                int e = m.edgesOnCell[c * ME + j];
+                int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+                double sdv = m.edgesOnCell_sign[c * ME + j] * m.dvEdge[e];
+                double face = 0.5 * (theta_m_save[c2 * L + k] + theta_m_save[c1 * L + k]);
+                double cc = sdv * (ru_save[e * L + k] - ru[e * L + k]) * face;
+                thadv = thadv - cc;
+            }
+        }
+        double tte = tend_theta_euler[o];
+        if (rk1 && P.h_theta_eddy_visc4 > 0.0) {
+            double r_area = P.h_theta_eddy_visc4 * P.prandtl_inv * ia;
+            for (int j = 0; j < n; ++j) {
+                int e = m.edgesOnCell[c * ME + j];
+                int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+                double es = m.meshScalingDel4[e] * r_area * m.dvEdge[e] * m.edgesOnCell_sign[c * ME + j] *
+                            m.invDcEdge[e];
+                tte = tte - es * (delsq_theta[c2 * L + k] - delsq_theta[c1 * L + k]);
+            }
+            tend_theta_euler[o] = tte;
+        }
+        const double *th = theta_m + c * L, *ths = theta_m_save + c * L;
+        const double *rwc = rw + c * Lw, *rws = rw_save + c * Lw;
+        auto wdtz = [&](int kk) -> double {
+            if (kk == 1)
+                return rwc[kk] * (fzm[kk] * th[kk] + fzp[kk] * th[kk - 1]) +
+                       (rws[kk] - rwc[kk]) * (fzm[kk] * ths[kk] + fzp[kk] * ths[kk - 1]);
+            if (kk == L - 1) return rws[kk] * (fzm[kk] * th[kk] + fzp[kk] * th[kk - 1]);
+            return flux3(th[kk - 2], th[kk - 1], th[kk], th[kk + 1], rwc[kk], P.coef_3rd_order) +
+                   (rws[kk] - rwc[kk]) * (fzm[kk] * ths[kk] + fzp[kk] * ths[kk - 1]);
+        };
+        double delta;
+        if (L == 1) delta = 0.0;
+        else if (k == 0) delta = wdtz(1);
+        else if (k == L - 1) delta = -wdtz(L - 1);
+        else delta = wdtz(k + 1) - wdtz(k);
+        double base = thadv * ia - m.rdzw[k] * delta;
+        rthdynten[o] = (base - tend_rho[o] * th[k]) / rho_zz[o];
+        double tt = base + rho_zz[o] * rt_diab[o];
+        tend_theta[o] = tt + tte;
+    }
+}
+
+void launch_dyn_tend(const mpas_hip_mesh &m, const DynParams &P, const DynInputs &in, const DynCarried &io,
+                     const DynOutputs &out, double *scratch, hipStream_t s) {
+    const int L = m.nLevels, nC = m.nCells, nE = m.nEdges, nV = m.nVertices;
+    DynScratch sc = carve(m, scratch);
+    const bool rk1 = (P.rk_step == 1);
+    if (rk1)
+        hipLaunchKernelGGL(k_dyn_kdiff, dim3(blocks_for((long long)(nC + 1) * L)), dim3(kBlock), 0, s, m, P, in.u,
+                           in.v, io.kdiff);
+    hipLaunchKernelGGL(k_dyn_cells_a, dim3(blocks_for((long long)(nC + 1) * (L + 1))), dim3(kBlock), 0, s, m, P,
+                       in.ru, in.rw, in.rr_save, in.qtot, in.w, in.theta_m, in.rho_edge, io.kdiff, out.h_divergence,
+                       io.tend_rho, sc.dpdz, sc.delsq_w, io.tend_w_euler, sc.delsq_theta, io.tend_theta_euler);
+    hipLaunchKernelGGL(k_dyn_edges_a, dim3(blocks_for((long long)(nE + 1) * L)), dim3(kBlock), 0, s, m, P, in.u,
+                       in.rw, in.pv_edge, in.ke, out.h_divergence, in.rho_edge, in.pp, sc.dpdz, in.cqu, in.divergence,
+                       in.vorticity, io.kdiff, out.tend_u, io.tend_u_euler, sc.delsq_u);
+    const int do_del4 = (rk1 && P.h_mom_eddy_visc4 > 0.0) ? 1 : 0;
+    if (do_del4) {
+        long long n = (long long)(nC > nV ? nC : nV) * L;
+        hipLaunchKernelGGL(k_dyn_del4_vc, dim3(blocks_for(n)), dim3(kBlock), 0, s, m, sc.delsq_u, sc.delsq_vort,
+                           sc.delsq_div);
+    }
+    hipLaunchKernelGGL(k_dyn_edges_b, dim3(blocks_for((long long)nE * L)), dim3(kBlock), 0, s, m, P, do_del4,
+                       in.rho_edge, sc.delsq_div, sc.delsq_vort, io.tend_u_euler, out.tend_u);
+    hipLaunchKernelGGL(k_dyn_cells_b, dim3(blocks_for((long long)(nC + 1) * (L + 1))), dim3(kBlock), 0, s, m, P,
+                       in.ru, in.rw, in.ru_save, in.rw_save, in.w, in.theta_m, in.theta_m_save, in.rho_zz, in.pp,
+                       in.cqw, in.rt_diabatic_tend, sc.dpdz, sc.delsq_w, sc.delsq_theta, io.tend_rho,
+                       io.tend_w_euler, io.tend_theta_euler, out.tend_w, out.tend_theta, out.rthdynten);
+}
+
+}  // namespace mpas

@@ -1,0 +1,465 @@
+// C ABI and the orchestration of one split-explicit large step
+// (atm_srk3 / _atm_srk3_impl, mpas_pytorch/dynamics/time_integration.py:
+// 3420-4894, stage body execute_rk_step :5000-9154).  Host code only
+// enqueues kernels on the caller's stream: no allocation, no host sync.
+#include <cstring>
+#include <mutex>
+
+#include "common.hpp"
+
+namespace mpas {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int check_launch(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string(what) + ": " + hipGetErrorString(e));
+        return 1;
+    }
+    return 0;
+}
+
+__global__ void k_fill_zero(double *p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+__global__ void k_copy(double *__restrict__ dst, const double *__restrict__ src, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+__global__ void k_add(double *__restrict__ dst, const double *__restrict__ src, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = dst[i] + src[i];
+}
+
+__global__ void k_scale(double *__restrict__ dst, const double *__restrict__ src, double f, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i] * f;
+}
+
+void fill_zero(double *p, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill_zero, dim3(blocks_for((long long)n)), dim3(kBlock), 0, s, p, n);
+}
+
+static void copy_d(double *dst, const double *src, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy, dim3(blocks_for((long long)n)), dim3(kBlock), 0, s, dst, src, n);
+}
+
+// ---------------------------------------------------------------------------
+// workspace layout
+// ---------------------------------------------------------------------------
+struct Work {
+    double *ru_save, *rw_save, *rtheta_p_save, *rho_p_save, *theta_m_init, *rho_zz_old_split;
+    double *scalars_old, *scalars_tmp, *ruAvg_split, *wwAvg_split;
+    double *cqw, *cqu, *qtot;
+    double *cofrz, *cofwr, *cofwz, *coftz, *cofwt, *a_tri, *alpha_tri, *gamma_tri;
+    double *tend_rho, *tend_u_euler, *tend_w_euler, *tend_theta_euler, *kdiff;
+    double *tend_u, *tend_w, *tend_theta, *h_divergence, *rthdynten;
+    double *ru_p, *ruAvg, *rho_pp, *rtheta_pp, *rtheta_pp_old, *rw_p, *wwAvg;
+    double *rho_zz_int, *horiz_flux, *scratch;
+    size_t total;
+};
+
+static size_t kernel_scratch(const mpas_hip_mesh &m) {
+    size_t a = dyn_tend_scratch_doubles(m), b = diagnostics_scratch_doubles(m), c = mono_scratch_doubles(m);
+    size_t r = a > b ? a : b;
+    return r > c ? r : c;
+}
+
+static Work carve_work(const mpas_hip_mesh &m, int S, double *base) {
+    const size_t L = m.nLevels, C = (size_t)(m.nCells + 1) * L, E = (size_t)(m.nEdges + 1) * L,
+                 W = (size_t)(m.nCells + 1) * (L + 1);
+    Work w;
+    double *p = base;
+    auto take = [&](size_t n) {
+        double *r = p;
+        p += (n + 31) & ~(size_t)31;  // 256-byte aligned slices
+        return r;
+    };
+    w.ru_save = take(E);
+    w.rw_save = take(W);
+    w.rtheta_p_save = take(C);
+    w.rho_p_save = take(C);
+    w.theta_m_init = take(C);
+    w.rho_zz_old_split = take(C);
+    w.scalars_old = take(C * S);
+    w.scalars_tmp = take(C * S);
+    w.ruAvg_split = take(E);
+    w.wwAvg_split = take(W);
+    w.cqw = take(W);
+    w.cqu = take(E);
+    w.qtot = take(C);
+    w.cofrz = take(L);
+    w.cofwr = take(C);
+    w.cofwz = take(C);
+    w.coftz = take(W);
+    w.cofwt = take(C);
+    w.a_tri = take(C);
+    w.alpha_tri = take(C);
+    w.gamma_tri = take(C);
+    w.tend_rho = take(C);
+    w.tend_u_euler = take(E);
+    w.tend_w_euler = take(W);
+    w.tend_theta_euler = take(C);
+    w.kdiff = take(C);
+    w.tend_u = take(E);
+    w.tend_w = take(W);
+    w.tend_theta = take(C);
+    w.h_divergence = take(C);
+    w.rthdynten = take(C);
+    w.ru_p = take(E);
+    w.ruAvg = take(E);
+    w.rho_pp = take(C);
+    w.rtheta_pp = take(C);
+    w.rtheta_pp_old = take(C);
+    w.rw_p = take(W);
+    w.wwAvg = take(W);
+    w.rho_zz_int = take(C);
+    w.horiz_flux = take(E * (S > 0 ? S : 1));
+    w.scratch = take(kernel_scratch(m));
+    w.total = (size_t)(p - base);
+    return w;
+}
+
+static int validate_mesh(const mpas_hip_mesh *m) {
+    if (!m) {
+        set_error("null mesh");
+        return 1;
+    }
+    if (m->nLevels < 3 || m->nLevels > 63) {
+        set_error("nLevels must be in [3, 63] (one wavefront per column)");
+        return 1;
+    }
+    long long big = (long long)(m->nEdges + 1) * (m->nLevels + 1) * 16;
+    if (big >= (1LL << 31)) {
+        set_error("mesh too large for 32-bit flat indexing");
+        return 1;
+    }
+    return 0;
+}
+
+static int validate_moist(int S, int ms, int me) {
+    if (S < 0 || ms < 1 || me > S) {
+        set_error("moist species range must satisfy 1 <= moist_start and moist_end <= nScalars");
+        return 1;
+    }
+    return 0;
+}
+
+}  // namespace mpas
+
+using namespace mpas;
+
+extern "C" {
+
+const char *mpas_hip_last_error(void) {
+    return g_last_error.c_str();
+}
+
+int mpas_hip_struct_sizes(int32_t *mesh, int32_t *state, int32_t *workspace, int32_t *config) {
+    *mesh = (int32_t)sizeof(mpas_hip_mesh);
+    *state = (int32_t)sizeof(mpas_hip_state);
+    *workspace = (int32_t)sizeof(mpas_hip_workspace);
+    *config = (int32_t)sizeof(mpas_hip_config);
+    return 0;
+}
+
+int64_t mpas_hip_kernel_scratch_doubles(int32_t nCells, int32_t nEdges, int32_t nVertices, int32_t nLevels) {
+    mpas_hip_mesh m;
+    std::memset(&m, 0, sizeof(m));
+    m.nCells = nCells;
+    m.nEdges = nEdges;
+    m.nVertices = nVertices;
+    m.nLevels = nLevels;
+    return (int64_t)kernel_scratch(m);
+}
+
+int64_t mpas_hip_workspace_doubles(int32_t nCells, int32_t nEdges, int32_t nVertices, int32_t nLevels,
+                                   int32_t nScalars) {
+    mpas_hip_mesh m;
+    std::memset(&m, 0, sizeof(m));
+    m.nCells = nCells;
+    m.nEdges = nEdges;
+    m.nVertices = nVertices;
+    m.nLevels = nLevels;
+    Work w = carve_work(m, nScalars, nullptr);
+    return (int64_t)w.total;
+}
+
+#define STREAM(s) ((hipStream_t)(s))
+
+int mpas_hip_moist_coefficients(const mpas_hip_mesh *m, const double *scalars, int32_t nScalars, int32_t moist_start,
+                                int32_t moist_end, double *cqw, double *cqu, double *qtot, void *stream) {
+    if (validate_mesh(m) || validate_moist(nScalars, moist_start, moist_end)) return 1;
+    launch_moist(*m, scalars, nScalars, moist_start, moist_end, cqw, cqu, qtot, STREAM(stream));
+    return check_launch("moist_coefficients");
+}
+
+int mpas_hip_vert_imp_coefs(const mpas_hip_mesh *m, double dts, double epssm, const double *cqw, const double *p,
+                            const double *t, const double *rt, const double *qtot, double *cofrz, double *cofwr,
+                            double *cofwz, double *coftz, double *cofwt, double *a_tri, double *alpha_tri,
+                            double *gamma_tri, void *stream) {
+    if (validate_mesh(m)) return 1;
+    launch_vimp(*m, dts, epssm, cqw, p, t, rt, qtot, cofrz, cofwr, cofwz, coftz, cofwt, a_tri, alpha_tri, gamma_tri,
+                STREAM(stream));
+    return check_launch("vert_imp_coefs");
+}
+
+int mpas_hip_solve_diagnostics(const mpas_hip_mesh *m, double dt, double apvm, int32_t rk_step, const double *u,
+                               const double *h, const double *v_in, double *h_edge, double *v, double *divergence,
+                               double *vorticity, double *ke, double *pv_edge, double *scratch, void *stream) {
+    if (validate_mesh(m)) return 1;
+    launch_diagnostics(*m, dt, apvm, rk_step, u, h, v_in, h_edge, v, divergence, vorticity, ke, pv_edge, scratch,
+                       STREAM(stream));
+    return check_launch("solve_diagnostics");
+}
+
+int mpas_hip_dyn_tend(const mpas_hip_mesh *m, const mpas_hip_dyn_params *p, const double *u, const double *v,
+                      const double *w, const double *ru, const double *rw, const double *ru_save, const double *rw_save,
+                      const double *theta_m, const double *theta_m_save, const double *rho_zz, const double *rho_edge,
+                      const double *divergence, const double *vorticity, const double *ke, const double *pv_edge,
+                      const double *pp, const double *rr_save, const double *cqw, const double *cqu,
+                      const double *rt_diabatic_tend, const double *qtot, double *tend_rho, double *tend_u_euler,
+                      double *tend_w_euler, double *tend_theta_euler, double *kdiff, double *tend_u, double *tend_w,
+                      double *tend_theta, double *h_divergence, double *rthdynten, double *scratch, void *stream) {
+    if (validate_mesh(m)) return 1;
+    DynInputs in{u, v, w, ru, rw, ru_save, rw_save, theta_m, theta_m_save, rho_zz, rho_edge, divergence, vorticity,
+                 ke, pv_edge, pp, rr_save, cqw, cqu, rt_diabatic_tend, qtot};
+    DynCarried io{tend_rho, tend_u_euler, tend_w_euler, tend_theta_euler, kdiff};
+    DynOutputs out{tend_u, tend_w, tend_theta, h_divergence, rthdynten};
+    launch_dyn_tend(*m, *p, in, io, out, scratch, STREAM(stream));
+    return check_launch("dyn_tend");
+}
+
+int mpas_hip_smlstep_pert(const mpas_hip_mesh *m, const double *u_tend, double *w_tend, void *stream) {
+    if (validate_mesh(m)) return 1;
+    launch_smlstep(*m, u_tend, w_tend, STREAM(stream));
+    return check_launch("smlstep_pert");
+}
+
+int mpas_hip_acoustic_step(const mpas_hip_mesh *m, const mpas_hip_acoustic_params *p, const double *cofrz,
+                           const double *cofwr, const double *cofwz, const double *coftz, const double *cofwt,
+                           const double *a_tri, const double *alpha_tri, const double *gamma_tri, const double *rho_zz,
+                           const double *theta_m, const double *exner, const double *cqu, const double *tend_ru,
+                           const double *tend_rho, const double *tend_rt, const double *tend_rw, const double *w,
+                           const double *rw, const double *rw_save, double *ru_p, double *ruAvg, double *rho_pp,
+                           double *rtheta_pp, double *rtheta_pp_old, double *rw_p, double *wwAvg, void *stream) {
+    if (validate_mesh(m)) return 1;
+    AcousticInputs in{cofrz, cofwr, cofwz, coftz, cofwt, a_tri, alpha_tri, gamma_tri, rho_zz, theta_m,
+                      exner, cqu,   tend_ru, tend_rho, tend_rt, tend_rw, w, rw, rw_save};
+    AcousticState io{ru_p, ruAvg, rho_pp, rtheta_pp, rtheta_pp_old, rw_p, wwAvg};
+    launch_acoustic(*m, *p, in, io, STREAM(stream));
+    return check_launch("acoustic_step");
+}
+
+int mpas_hip_divergence_damping(const mpas_hip_mesh *m, double dts, double smdiv, double len_disp,
+                                const double *theta_m, const double *rtheta_pp, const double *rtheta_pp_old,
+                                double *ru_p, void *stream) {
+    if (validate_mesh(m)) return 1;
+    launch_divdamp(*m, dts, smdiv, len_disp, theta_m, rtheta_pp, rtheta_pp_old, ru_p, STREAM(stream));
+    return check_launch("divergence_damping");
+}
+
+int mpas_hip_recover_large_step(const mpas_hip_mesh *m, int32_t rk_step, double dt, int32_t ns,
+                                const double *rho_p_save, const double *rho_pp, const double *rtheta_p_save,
+                                const double *rtheta_pp, const double *rt_diabatic_tend, const double *rw_save,
+                                const double *rw_p, const double *ru_save, const double *ru_p, double *wwAvg,
+                                double *ruAvg, double *rho_p, double *rho_zz, double *rtheta_p, double *theta_m,
+                                double *exner, double *pressure_p, double *rw, double *w, double *ru, double *u,
+                                void *stream) {
+    if (validate_mesh(m)) return 1;
+    RecoverInputs in{rho_p_save, rho_pp, rtheta_p_save, rtheta_pp, rt_diabatic_tend, rw_save, rw_p, ru_save, ru_p};
+    RecoverOutputs out{wwAvg, ruAvg, rho_p, rho_zz, rtheta_p, theta_m, exner, pressure_p, rw, w, ru, u};
+    launch_recover(*m, rk_step, dt, ns, in, out, STREAM(stream));
+    return check_launch("recover_large_step");
+}
+
+int mpas_hip_advance_scalars(const mpas_hip_mesh *m, int32_t nScalars, double dt, double coef3, double weight_old,
+                             double weight_new, const double *scalar_old, const double *scalar_new_in,
+                             const double *rho_zz_old, const double *rho_zz_new, const double *uhAvg,
+                             const double *wwAvg, double *scalar_new_out, double *horiz_flux, void *stream) {
+    if (validate_mesh(m)) return 1;
+    if (scalar_new_out == scalar_new_in) {
+        set_error("advance_scalars: output must not alias scalar_new_in");
+        return 1;
+    }
+    launch_advance_scalars(*m, nScalars, dt, coef3, weight_old, weight_new, scalar_old, scalar_new_in, rho_zz_old,
+                           rho_zz_new, uhAvg, wwAvg, scalar_new_out, horiz_flux, STREAM(stream));
+    return check_launch("advance_scalars");
+}
+
+int mpas_hip_advance_scalars_mono(const mpas_hip_mesh *m, int32_t nScalars, double dt, double coef3,
+                                  const double *scalars_old, const double *scalars_new_in, const double *rho_zz_old,
+                                  const double *uhAvg, const double *wwAvg, double *scalars_new_out,
+                                  double *rho_zz_int, double *scratch, void *stream) {
+    if (validate_mesh(m)) return 1;
+    launch_advance_scalars_mono(*m, nScalars, dt, coef3, scalars_old, scalars_new_in, rho_zz_old, uhAvg, wwAvg,
+                                scalars_new_out, rho_zz_int, scratch, STREAM(stream));
+    return check_launch("advance_scalars_mono");
+}
+
+int mpas_hip_reconstruct(const mpas_hip_mesh *m, const double *u, double *ux, double *uy, double *uz, double *uzonal,
+                         double *umerid, void *stream) {
+    if (validate_mesh(m)) return 1;
+    launch_reconstruct(*m, u, ux, uy, uz, uzonal, umerid, STREAM(stream));
+    return check_launch("reconstruct");
+}
+
+// ---------------------------------------------------------------------------
+// one large step
+// ---------------------------------------------------------------------------
+int mpas_hip_srk3_step(const mpas_hip_mesh *mesh, mpas_hip_state *st, const mpas_hip_workspace *work,
+                       const mpas_hip_config *cfg, void *stream) {
+    if (validate_mesh(mesh)) return 1;
+    const mpas_hip_mesh &m = *mesh;
+    hipStream_t s = STREAM(stream);
+    const int S = cfg->nScalars;
+    if (validate_moist(S, cfg->moist_start, cfg->moist_end)) return 1;
+    Work w = carve_work(m, S, work->base);
+    if ((int64_t)w.total > work->n_doubles) {
+        set_error("workspace too small");
+        return 1;
+    }
+    if (cfg->split < 1 || cfg->ns < 1) {
+        set_error("split and ns must be >= 1");
+        return 1;
+    }
+    const int L = m.nLevels, nC = m.nCells;
+    const size_t C = (size_t)(nC + 1) * L, E = (size_t)(m.nEdges + 1) * L, Wd = (size_t)(nC + 1) * (L + 1);
+    const double dt = cfg->dt;
+
+    // atm_rk_integration_setup (time_integration.py:2497-2785)
+    fill_zero(st->theta_m + (size_t)nC * L, L, s);
+    copy_d(w.theta_m_init, st->theta_m, C, s);
+    copy_d(w.rho_zz_old_split, st->rho_zz, C, s);
+    copy_d(w.scalars_old, st->scalars, C * S, s);
+    copy_d(w.ru_save, st->ru, E, s);
+    copy_d(w.rw_save, st->rw, Wd, s);
+    copy_d(w.rtheta_p_save, st->rtheta_p, C, s);
+    copy_d(w.rho_p_save, st->rho_p, C, s);
+
+    // compute_rk_weights, order 3 (time_integration.py:2787-2856)
+    const double dt_dyn = dt / (double)cfg->split;
+    const double rk_timestep[3] = {dt_dyn / 3.0, dt_dyn / 2.0, dt_dyn};
+    const double rk_sub[3] = {dt_dyn / 3.0, dt_dyn / (double)cfg->ns, dt_dyn / (double)cfg->ns};
+    const int nsub[3] = {1, cfg->ns / 2 > 1 ? cfg->ns / 2 : 1, cfg->ns};
+    const double resm = (1.0 - cfg->epssm) / (1.0 + cfg->epssm);
+    const double rcv = MPAS_RGAS / (MPAS_CP - MPAS_RGAS);
+    const double c2 = MPAS_CP * rcv;
+
+    // Dynamics-tendency scalars, evaluated as the reference evaluates them.
+    mpas_hip_dyn_params P;
+    P.dt = dt;
+    P.coef_3rd_order = cfg->coef3;
+    {
+        const double inv_dt = 1.0 / dt;
+        const double cl = cfg->c_s * cfg->len_disp;
+        P.kdiff_coef = cl * cl;
+        P.kdiff_cap = (0.01 * (cfg->len_disp * cfg->len_disp)) * inv_dt;
+        P.h_mom_eddy_visc4 = cfg->visc4_2dsmag * std::pow(cfg->len_disp, 3.0);
+        P.h_theta_eddy_visc4 = P.h_mom_eddy_visc4;
+        P.del4u_div_factor = cfg->del4u_div_factor;
+        P.prandtl_inv = 1.0 / 1.0;
+    }
+
+    for (int sub = 1; sub <= cfg->split; ++sub) {
+        for (int rk = 1; rk <= 3; ++rk) {
+            if (sub == 1 && rk == 1) {
+                launch_moist(m, st->scalars, S, cfg->moist_start, cfg->moist_end, w.cqw, w.cqu, w.qtot, s);
+                if (!cfg->diagnostics_current) {
+                    // pre-dynamics-tendency diagnostic refresh (time_integration.py:5687-5900)
+                    launch_diagnostics(m, dt, cfg->apvm, -1, st->u, st->rho_zz, st->v, st->rho_edge, st->v,
+                                       st->divergence, st->vorticity, st->ke, st->pv_edge, w.scratch, s);
+                }
+            }
+            if (rk == 1 || rk == 2)
+                launch_vimp(m, rk_sub[rk - 1], cfg->epssm, w.cqw, st->exner, st->theta_m, st->rtheta_p, w.qtot,
+                            w.cofrz, w.cofwr, w.cofwz, w.coftz, w.cofwt, w.a_tri, w.alpha_tri, w.gamma_tri, s);
+            P.rk_step = rk;
+            DynInputs din{st->u,          st->v,          st->w,          st->ru,      st->rw,
+                          w.ru_save,      w.rw_save,      st->theta_m,    w.theta_m_init, st->rho_zz,
+                          st->rho_edge,   st->divergence, st->vorticity,  st->ke,      st->pv_edge,
+                          st->pressure_p, w.rho_p_save,   w.cqw,          w.cqu,       st->rt_diabatic_tend,
+                          w.qtot};
+            DynCarried dio{w.tend_rho, w.tend_u_euler, w.tend_w_euler, w.tend_theta_euler, w.kdiff};
+            DynOutputs dout{w.tend_u, w.tend_w, w.tend_theta, w.h_divergence, w.rthdynten};
+            launch_dyn_tend(m, P, din, dio, dout, w.scratch, s);
+            launch_smlstep(m, w.tend_u, w.tend_w, s);
+            AcousticInputs ain{w.cofrz,  w.cofwr,        w.cofwz,    w.coftz,        w.cofwt, w.a_tri, w.alpha_tri,
+                               w.gamma_tri, st->rho_zz,  w.theta_m_init, st->exner, w.cqu,   w.tend_u, w.tend_rho,
+                               w.tend_theta, w.tend_w,   st->w,      st->rw,         w.rw_save};
+            AcousticState aio{w.ru_p, w.ruAvg, w.rho_pp, w.rtheta_pp, w.rtheta_pp_old, w.rw_p, w.wwAvg};
+            for (int small = 1; small <= nsub[rk - 1]; ++small) {
+                mpas_hip_acoustic_params AP;
+                AP.small_step = small;
+                AP.dts = rk_sub[rk - 1];
+                AP.epssm = cfg->epssm;
+                AP.resm = resm;
+                AP.c2 = c2;
+                launch_acoustic(m, AP, ain, aio, s);
+                launch_divdamp(m, rk_sub[rk - 1], cfg->smdiv, cfg->len_disp, w.theta_m_init, w.rtheta_pp,
+                               w.rtheta_pp_old, w.ru_p, s);
+            }
+            RecoverInputs rin{w.rho_p_save, w.rho_pp, w.rtheta_p_save, w.rtheta_pp, st->rt_diabatic_tend,
+                              w.rw_save,    w.rw_p,   w.ru_save,       w.ru_p};
+            RecoverOutputs rout{w.wwAvg,      w.ruAvg,       st->rho_p, st->rho_zz, st->rtheta_p, st->theta_m,
+                                st->exner,    st->pressure_p, st->rw,   st->w,      st->ru,       st->u};
+            launch_recover(m, rk, rk_timestep[rk - 1], nsub[rk - 1], rin, rout, s);
+            launch_diagnostics(m, dt, cfg->apvm, rk, st->u, st->rho_zz, st->v, st->rho_edge, st->v, st->divergence,
+                               st->vorticity, st->ke, st->pv_edge, w.scratch, s);
+        }
+        // split-flux accumulation and substep reset (time_integration.py:3820-3942)
+        if (sub == 1) {
+            copy_d(w.ruAvg_split, w.ruAvg, E, s);
+            copy_d(w.wwAvg_split, w.wwAvg, Wd, s);
+        } else {
+            hipLaunchKernelGGL(k_add, dim3(blocks_for((long long)E)), dim3(kBlock), 0, s, w.ruAvg_split, w.ruAvg, E);
+            hipLaunchKernelGGL(k_add, dim3(blocks_for((long long)Wd)), dim3(kBlock), 0, s, w.wwAvg_split, w.wwAvg, Wd);
+        }
+        if (sub < cfg->split) {
+            fill_zero(st->theta_m + (size_t)nC * L, L, s);
+            copy_d(w.ru_save, st->ru, E, s);
+            copy_d(w.rw_save, st->rw, Wd, s);
+            copy_d(w.rtheta_p_save, st->rtheta_p, C, s);
+            copy_d(w.rho_p_save, st->rho_p, C, s);
+            copy_d(w.theta_m_init, st->theta_m, C, s);
+        }
+    }
+    const double inv_split = 1.0 / (double)cfg->split;
+    hipLaunchKernelGGL(k_scale, dim3(blocks_for((long long)E)), dim3(kBlock), 0, s, st->ruAvg, w.ruAvg_split,
+                       inv_split, E);
+    hipLaunchKernelGGL(k_scale, dim3(blocks_for((long long)Wd)), dim3(kBlock), 0, s, st->wwAvg, w.wwAvg_split,
+                       inv_split, Wd);
+    // split transport RK3 (time_integration.py:3976-4675); rho_zz_new is the
+    // post-dynamics rho_zz already in the state
+    if (S > 0) {
+        const double dts[3] = {dt / 3.0, dt / 2.0, dt};
+        // stage 1: state.scalars -> tmp; stage 2: tmp -> state.scalars
+        launch_advance_scalars(m, S, dts[0], cfg->coef3, 1.0 - 1.0 / 3.0, 1.0 / 3.0, w.scalars_old, st->scalars,
+                               w.rho_zz_old_split, st->rho_zz, st->ruAvg, st->wwAvg, w.scalars_tmp, w.horiz_flux, s);
+        launch_advance_scalars(m, S, dts[1], cfg->coef3, 1.0 - 1.0 / 2.0, 1.0 / 2.0, w.scalars_old, w.scalars_tmp,
+                               w.rho_zz_old_split, st->rho_zz, st->ruAvg, st->wwAvg, st->scalars, w.horiz_flux, s);
+        if (cfg->monotonic) {
+            launch_advance_scalars_mono(m, S, dts[2], cfg->coef3, w.scalars_old, st->scalars, w.rho_zz_old_split,
+                                        st->ruAvg, st->wwAvg, st->scalars, w.rho_zz_int, w.scratch, s);
+        } else {
+            launch_advance_scalars(m, S, dts[2], cfg->coef3, 0.0, 1.0, w.scalars_old, st->scalars,
+                                   w.rho_zz_old_split, st->rho_zz, st->ruAvg, st->wwAvg, w.scalars_tmp,
+                                   w.horiz_flux, s);
+            copy_d(st->scalars, w.scalars_tmp, C * S, s);
+        }
+    }
+    launch_reconstruct(m, st->u, st->uReconstructX, st->uReconstructY, st->uReconstructZ, st->uReconstructZonal,
+                       st->uReconstructMeridional, s);
+    copy_d(st->ur_cell, st->uReconstructZonal, C, s);
+    copy_d(st->vr_cell, st->uReconstructMeridional, C, s);
+    return check_launch("srk3_step");
+}
+
+}  // extern "C"

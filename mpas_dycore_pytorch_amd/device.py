@@ -1,0 +1,202 @@
+"""Device-resident mesh and state for the gfx950 step.
+
+The public boundary keeps the reference layout (``mpas_pytorch`` docs/API.md
+"State and controls"): float64 tensors with one trailing padding row per
+entity, levels innermost, 1-based connectivity with 0 for "none".  On the
+device the connectivity is int32 and 0-based (-1 for "none"); every other
+array keeps the reference shape, so device results compare index for index.
+
+PyTorch owns every device allocation (mesh arrays, state, the per-mesh
+workspace); the HIP library only receives pointers.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from typing import Any, Mapping
+
+import numpy as np
+import torch
+
+from . import _native as N
+from .errors import MpasDeviceError, MpasMeshMismatchError, MpasMissingFieldError
+
+# 1-based connectivity -> 0-based; the entity each index points to
+INDEX_TARGET = {
+    "cellsOnEdge": "nCells", "verticesOnEdge": "nVertices", "edgesOnCell": "nEdges",
+    "cellsOnCell": "nCells", "verticesOnCell": "nVertices", "kiteForCell": 3,
+    "edgesOnVertex": "nEdges", "edgesOnEdge": "nEdges", "advCellsForEdge": "nCells",
+}
+COUNT_FIELDS = ("nEdgesOnCell", "nEdgesOnEdge", "nAdvCellsForEdge")
+BASE_FIELDS = ("rho_base", "rtheta_base", "exner_base")
+
+
+def to_numpy(v: Any) -> np.ndarray:
+    if isinstance(v, torch.Tensor):
+        return v.detach().cpu().numpy()
+    return np.asarray(v)
+
+
+def require_gpu(device: torch.device | str | None = None) -> torch.device:
+    if not torch.cuda.is_available():
+        raise MpasDeviceError(requested=str(device or "cuda"))
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    if dev.type != "cuda":
+        raise MpasDeviceError(f"the HIP step runs on a ROCm GPU, not {dev}", requested=str(dev))
+    if dev.index is None:
+        dev = torch.device("cuda", torch.cuda.current_device())
+    return dev
+
+
+def _get(mesh: Mapping, state: Mapping | None, name: str):
+    if name in mesh:
+        return mesh[name]
+    if state is not None and name in state:
+        return state[name]
+    raise MpasMissingFieldError(fields=[name], source="mesh")
+
+
+class DeviceMesh:
+    """The mesh of one ``atm_srk3`` call resident on one GPU.
+
+    Built once per (mesh mapping, device) and cached; connectivity is
+    validated on the host before anything reaches a kernel so that every
+    gather a kernel makes stays inside its array.
+    """
+
+    _cache: dict[tuple, "DeviceMesh"] = {}
+
+    def __init__(self, mesh: Mapping, device: torch.device, state: Mapping | None = None):
+        self.device = device
+        nc, ne, nv = (int(to_numpy(mesh[k])) for k in ("nCells", "nEdges", "nVertices"))
+        L = int(to_numpy(_get(mesh, state, "nVertLevels")))
+        self.nc, self.ne, self.nv, self.L = nc, ne, nv, L
+        if L < 3 or L > 63:
+            raise MpasMeshMismatchError(f"nVertLevels={L}: the column kernels take 3..63 levels")
+        self.t: dict[str, torch.Tensor] = {}
+        s = N.MeshStruct()
+        s.nCells, s.nEdges, s.nVertices, s.nLevels = nc, ne, nv, L
+        eoc = to_numpy(mesh["edgesOnCell"])
+        s.maxEdges = int(eoc.shape[1])
+        s.maxEdges2 = int(to_numpy(mesh["edgesOnEdge"]).shape[1])
+        s.vertexDegree = 3
+        s.maxAdv = int(to_numpy(mesh["advCellsForEdge"]).shape[1])
+        dims = {"nCells": nc, "nEdges": ne, "nVertices": nv}
+        rows = {"cellsOnEdge": ne, "verticesOnEdge": ne, "edgesOnCell": nc, "cellsOnCell": nc,
+                "verticesOnCell": nc, "kiteForCell": nc, "edgesOnVertex": nv, "edgesOnEdge": ne,
+                "advCellsForEdge": ne, "nEdgesOnCell": nc, "nEdgesOnEdge": ne, "nAdvCellsForEdge": ne}
+        counts = {}
+        for name in COUNT_FIELDS:
+            a = to_numpy(mesh[name]).astype(np.int64)
+            if a.shape[0] < rows[name]:
+                raise MpasMeshMismatchError(dim_name=name, init_value=a.shape[0], grid_value=rows[name])
+            counts[name] = a[: rows[name]]
+        width = {"edgesOnCell": "nEdgesOnCell", "cellsOnCell": "nEdgesOnCell", "verticesOnCell": "nEdgesOnCell",
+                 "kiteForCell": "nEdgesOnCell", "edgesOnEdge": "nEdgesOnEdge",
+                 "advCellsForEdge": "nAdvCellsForEdge"}
+        for name in N.MESH_INDEX_FIELDS:
+            a = to_numpy(mesh[name]).astype(np.int64)
+            if name in INDEX_TARGET:
+                a = a - 1
+                tgt = INDEX_TARGET[name]
+                hi = tgt if isinstance(tgt, int) else dims[tgt]
+                live = a[: rows[name]]
+                if live.ndim == 2 and name in width:
+                    cnt = counts[width[name]]
+                    if cnt.min() < 0 or cnt.max() > live.shape[1]:
+                        raise MpasMeshMismatchError(f"{width[name]} outside 0..{live.shape[1]}")
+                    used = np.arange(live.shape[1])[None, :] < cnt[:, None]
+                    bad = used & ((live < 0) | (live >= hi))
+                else:
+                    bad = (live < 0) | (live >= hi)
+                if bad.any():
+                    raise MpasMeshMismatchError(f"{name}: connectivity value outside 1..{hi}")
+            t = torch.as_tensor(np.ascontiguousarray(a.astype(np.int32)), device=device)
+            self.t[name] = t
+            setattr(s, name, t.data_ptr())
+        flat = True
+        for name in N.MESH_REAL_FIELDS_A:
+            if name == "zxu" and "zxu" not in mesh:
+                a = np.zeros((ne + 1, L))
+            elif name in ("zb_cell", "zb3_cell") and name not in mesh:
+                a = np.zeros((nc + 1, s.maxEdges, L + 1))
+            else:
+                a = to_numpy(_get(mesh, state, name)).astype(np.float64)
+            if name in ("zxu", "zb_cell", "zb3_cell") and np.any(a != 0.0):
+                flat = False
+            t = torch.as_tensor(np.ascontiguousarray(a), device=device)
+            self.t[name] = t
+            setattr(s, name, t.data_ptr())
+        s.flatTerrain = 1 if flat else 0
+        for name in ("cf1", "cf2", "cf3"):
+            setattr(s, name, float(to_numpy(_get(mesh, state, name))))
+        self.s = s
+        self._base_src: dict[str, int] = {}
+        self.set_base(mesh, state)
+        self._work: dict[int, torch.Tensor] = {}
+        self._kscratch: torch.Tensor | None = None
+
+    def set_base(self, mesh: Mapping, state: Mapping | None) -> None:
+        """(Re)bind the hydrostatic base state, looked up in the mesh then the state."""
+        for name in BASE_FIELDS:
+            src = _get(mesh, state, name)
+            if self._base_src.get(name) == id(src) and name in self.t:
+                continue
+            t = torch.as_tensor(np.ascontiguousarray(to_numpy(src).astype(np.float64)), device=self.device)
+            if tuple(t.shape) != (self.nc + 1, self.L):
+                raise MpasMeshMismatchError(f"{name} has shape {tuple(t.shape)}, expected {(self.nc + 1, self.L)}")
+            self.t[name] = t
+            self._base_src[name] = id(src)
+            setattr(self.s, name, t.data_ptr())
+
+    @classmethod
+    def get(cls, mesh: Mapping, device: torch.device, state: Mapping | None = None) -> "DeviceMesh":
+        key = (id(mesh), str(device))
+        dm = cls._cache.get(key)
+        if dm is None or dm._mesh_ref is not mesh:
+            dm = cls(mesh, device, state)
+            dm._mesh_ref = mesh
+            if len(cls._cache) > 8:
+                cls._cache.clear()
+            cls._cache[key] = dm
+        else:
+            dm.set_base(mesh, state)
+        return dm
+
+    _mesh_ref: Any = None
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+    # shapes of the state arrays
+    def shape_of(self, name: str, n_scalars: int = 0) -> tuple[int, ...]:
+        nc, ne, nv, L = self.nc, self.ne, self.nv, self.L
+        if name in ("u", "ru", "v", "pv_edge", "rho_edge", "ruAvg", "h_edge"):
+            return (ne + 1, L)
+        if name in ("w", "rw", "wwAvg"):
+            return (nc + 1, L + 1)
+        if name == "vorticity":
+            return (nv + 1, L)
+        if name == "scalars":
+            return (nc + 1, L, n_scalars)
+        return (nc + 1, L)
+
+    def workspace(self, n_scalars: int) -> torch.Tensor:
+        """Zero-initialised per-mesh workspace of one large step (kept for reuse)."""
+        w = self._work.get(n_scalars)
+        if w is None:
+            n = N.load().mpas_hip_workspace_doubles(self.nc, self.ne, self.nv, self.L, n_scalars)
+            w = torch.zeros(int(n), dtype=torch.float64, device=self.device)
+            self._work[n_scalars] = w
+        return w
+
+    def kernel_scratch(self) -> torch.Tensor:
+        if self._kscratch is None:
+            n = N.load().mpas_hip_kernel_scratch_doubles(self.nc, self.ne, self.nv, self.L)
+            self._kscratch = torch.zeros(int(n), dtype=torch.float64, device=self.device)
+        return self._kscratch
+
+
+def stream_handle(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream

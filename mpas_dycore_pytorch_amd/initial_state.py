@@ -184,3 +184,34 @@ def init_coupled_diagnostics(mesh: dict, raw: dict) -> dict:
     out["theta_m"][nc] = 0.0
     out["rho_zz"][nc] = 1.0  # padding sentinel, atmosphere_model.py:529-545
     return out
+
+
+DIAGNOSTIC_KEYS = ("h_edge", "v", "divergence", "vorticity", "ke", "pv_edge")
+RECONSTRUCT_KEYS = ("ur_cell", "vr_cell", "uReconstructX", "uReconstructY", "uReconstructZ",
+                    "uReconstructZonal", "uReconstructMeridional")
+
+
+def prepare_state(mesh: dict, raw: dict, *, diagnostics, reconstruct,
+                  coef_3rd_order: float = 0.25, c_s: float = 0.125) -> dict:
+    """Cold-start chain of the reference (``atmosphere_model.py:782-1339``).
+
+    ``diagnostics(u, rho_zz)`` must return the first diagnostic solve
+    (``rk_step=None``: h_edge, v, divergence, vorticity, ke, pv_edge) and
+    ``reconstruct(u)`` the cell vector reconstruction; the caller picks the
+    implementation (GPU kernels in production).  Returns one mapping that
+    holds the mesh, the raw fields and the prepared state, as the reference
+    state dict does.
+    """
+    st = dict(mesh)
+    st.update(raw)
+    st.update(init_coupled_diagnostics(mesh, raw))
+    st["coef_3rd_order"] = coef_3rd_order
+    st["c_s"] = c_s
+    d = diagnostics(st["u"], st["rho_zz"])
+    for k in DIAGNOSTIC_KEYS:
+        st[k] = np.asarray(d[k])
+    st["rho_edge"] = st["h_edge"]
+    r = reconstruct(st["u"])
+    for k in RECONSTRUCT_KEYS:
+        st[k] = np.asarray(r[k])
+    return st

@@ -295,37 +295,37 @@ def trisk_weights(hm: HorizontalMesh) -> tuple[np.ndarray, np.ndarray, np.ndarra
     ``sum_j w_j u_j`` the velocity component along ``k x n_e``.
     """
     n_e = hm.cells_on_edge.shape[0]
+    n_c = hm.x_cell.shape[0]
+    eoc, noc, voc = hm.edges_on_cell, hm.n_edges_on_cell, hm.vertices_on_cell
     n_eoe = np.zeros(n_e, dtype=np.int64)
     eoe = np.full((n_e, MAX_EDGES2), -1, dtype=np.int64)
     woe = np.zeros((n_e, MAX_EDGES2))
-    # kite area of (vertex, cell)
-    kite_lookup: dict[tuple[int, int], float] = {}
-    for v in range(hm.cells_on_vertex.shape[0]):
-        for i in range(3):
-            kite_lookup[(v, int(hm.cells_on_vertex[v, i]))] = hm.kite_areas_on_vertex[v, i]
-    for e in range(n_e):
-        k = 0
-        for side in (0, 1):
-            c = int(hm.cells_on_edge[e, side])
-            n = int(hm.n_edges_on_cell[c])
-            edges = [int(x) for x in hm.edges_on_cell[c, :n]]
-            j = edges.index(e)
-            r = 0.0
-            prev = e
-            for step in range(1, n):
-                ep = edges[(j + step) % n]
-                va = set(int(x) for x in hm.vertices_on_edge[prev])
-                vb = set(int(x) for x in hm.vertices_on_edge[ep])
-                (v,) = va & vb
-                r += kite_lookup[(v, c)] / hm.area_cell[c]
-                # outward sign of ep for c and of e for c
-                s_ep = 1.0 if int(hm.cells_on_edge[ep, 0]) == c else -1.0
-                s_e = 1.0 if int(hm.cells_on_edge[e, 0]) == c else -1.0
-                eoe[e, k] = ep
-                woe[e, k] = s_e * s_ep * (0.5 - r) * hm.dv_edge[ep] / hm.dc_edge[e]
-                k += 1
-                prev = ep
-        n_eoe[e] = k
+    # kite area of the vertex shared by slots i and i+1 of a cell, as a
+    # fraction of the cell area (verticesOnCell[i] is that vertex)
+    vv = np.clip(voc, 0, None)
+    corner = np.argmax(hm.cells_on_vertex[vv] == np.arange(n_c)[:, None, None], axis=-1)
+    frac = np.take_along_axis(hm.kite_areas_on_vertex[vv], corner[..., None], -1)[..., 0] / hm.area_cell[:, None]
+    e_all = np.arange(n_e)
+    k0 = np.zeros(n_e, dtype=np.int64)
+    for side in (0, 1):
+        c = hm.cells_on_edge[:, side]
+        n = noc[c]
+        j = np.argmax(eoc[c] == e_all[:, None], axis=1)
+        s_e = 1.0 if side == 0 else -1.0
+        r = np.zeros(n_e)
+        for step in range(1, MAX_EDGES):
+            live = step < n
+            ep = eoc[c, (j + step) % n]
+            # walk counter-clockwise; r accumulates the swept kite fraction
+            r = np.where(live, r + frac[c, (j + step - 1) % n], r)
+            s_ep = np.where(hm.cells_on_edge[ep, 0] == c, 1.0, -1.0)
+            w = s_e * s_ep * (0.5 - r) * hm.dv_edge[ep] / hm.dc_edge
+            slot = k0 + step - 1
+            idx = np.nonzero(live)[0]
+            eoe[idx, slot[idx]] = ep[idx]
+            woe[idx, slot[idx]] = w[idx]
+        k0 = k0 + (n - 1)
+    n_eoe[:] = k0
     return n_eoe, eoe, woe
 
 
@@ -346,81 +346,100 @@ def _trisk_sign_fix(hm: HorizontalMesh, eoe: np.ndarray, woe: np.ndarray) -> np.
 # least-squares helpers for deriv_two / reconstruction / deformation
 # ---------------------------------------------------------------------------
 def _local_plane(hm: HorizontalMesh, center: np.ndarray, pts: np.ndarray, xaxis: np.ndarray):
-    """Map points to a tangent-plane frame at ``center`` with x along ``xaxis``."""
+    """Map points to tangent-plane frames: ``center`` (m, 3), ``pts`` (m, p, 3),
+    ``xaxis`` (m, 3); returns x, y of shape (m, p) in metres."""
     k = center
-    ex = xaxis - np.dot(xaxis, k) * k
-    ex /= np.linalg.norm(ex)
+    ex = xaxis - np.sum(xaxis * k, -1, keepdims=True) * k
+    ex = ex / np.linalg.norm(ex, axis=-1, keepdims=True)
     ey = np.cross(k, ex)
-    ang = _arc(np.broadcast_to(k, pts.shape), pts)
-    d = pts - np.dot(pts, k)[:, None] * k[None, :]
-    dn = np.linalg.norm(d, axis=1)
+    kb = np.broadcast_to(k[:, None, :], pts.shape)
+    ang = _arc(kb, pts)
+    d = pts - np.sum(pts * kb, -1, keepdims=True) * kb
+    dn = np.linalg.norm(d, axis=-1)
     dn = np.where(dn > 0.0, dn, 1.0)
-    dirx = np.dot(d, ex) / dn
-    diry = np.dot(d, ey) / dn
+    dirx = np.sum(d * ex[:, None, :], -1) / dn
+    diry = np.sum(d * ey[:, None, :], -1) / dn
     r = hm.radius * ang
     return r * dirx, r * diry
 
 
 def deriv_two(hm: HorizontalMesh) -> np.ndarray:
-    """(E, 2, MAX_ADV) normal second-derivative weights for each edge cell."""
+    """(E, 2, MAX_ADV) normal second-derivative weights for each edge cell.
+
+    For edge e and its cell c, a quadratic is least-squares fitted over c and
+    its edge neighbours in the tangent plane of c with x along the normal of
+    e; the weights are 2x the x^2 row of the fit's pseudo-inverse.  The fit
+    runs in coordinates scaled by dcEdge (condition number O(10) instead of
+    O(1e12) in metres) and the x^2 row is scaled back by 1/dcEdge^2.
+    """
     n_e = hm.cells_on_edge.shape[0]
     out = np.zeros((n_e, 2, MAX_ADV))
-    for e in range(n_e):
-        for side in (0, 1):
-            c = int(hm.cells_on_edge[e, side])
-            n = int(hm.n_edges_on_cell[c])
-            ids = [c] + [int(x) for x in hm.cells_on_cell[c, :n]]
+    for side in (0, 1):
+        c_all = hm.cells_on_edge[:, side]
+        n_all = hm.n_edges_on_cell[c_all]
+        for n in np.unique(n_all):
+            e = np.nonzero(n_all == n)[0]
+            c = c_all[e]
+            ids = np.concatenate([c[:, None], hm.cells_on_cell[c, :n]], axis=1)
             x, y = _local_plane(hm, hm.x_cell[c], hm.x_cell[ids], hm.normal[e])
-            a = np.stack([np.ones_like(x), x, y, x * x, x * y, y * y], axis=1)
+            h = hm.dc_edge[e][:, None]
+            xs, ys = x / h, y / h
+            a = np.stack([np.ones_like(xs), xs, ys, xs * xs, xs * ys, ys * ys], axis=-1)
             pinv = np.linalg.pinv(a)
-            out[e, side, : n + 1] = 2.0 * pinv[3]
+            out[e, side, : n + 1] = 2.0 * pinv[:, 3, :] / (h * h)
     return out
 
 
 def advective_stencil(hm: HorizontalMesh, d2: np.ndarray, coef_3rd_order: float):
-    """advCellsForEdge construction of ``ic_loader.py:358-471`` (0-based ids)."""
+    """advCellsForEdge construction of ``ic_loader.py:358-471`` (0-based ids).
+
+    The cell list of edge e is c1, c2, then the neighbours of c1 and of c2
+    not yet listed, in slot order; the coefficients accumulate the two
+    deriv_two rows into those positions in the same order.
+    """
     n_e = hm.cells_on_edge.shape[0]
-    n_adv = np.zeros(n_e, dtype=np.int64)
+    coc, noc = hm.cells_on_cell, hm.n_edges_on_cell
+    c1, c2 = hm.cells_on_edge[:, 0], hm.cells_on_edge[:, 1]
     adv_cells = np.full((n_e, MAX_ADV), -1, dtype=np.int64)
+    adv_cells[:, 0], adv_cells[:, 1] = c1, c2
+    n_adv = np.full(n_e, 2, dtype=np.int64)
+    rows = np.arange(n_e)
+    # positions of the neighbour slots of c1 (pos1) and c2 (pos2)
+    pos1 = np.zeros((n_e, MAX_EDGES), dtype=np.int64)
+    pos2 = np.zeros((n_e, MAX_EDGES), dtype=np.int64)
+    for side, (cc, pos) in enumerate(((c1, pos1), (c2, pos2))):
+        for i in range(MAX_EDGES):
+            live = i < noc[cc]
+            nb = coc[cc, i]
+            hit = adv_cells == nb[:, None]
+            seen = hit.any(axis=1)
+            where = np.argmax(hit, axis=1)
+            add = live & ~seen
+            adv_cells[rows[add], n_adv[add]] = nb[add]
+            pos[:, i] = np.where(seen, where, n_adv)
+            n_adv = n_adv + add
     adv = np.zeros((n_e, MAX_ADV))
     adv3 = np.zeros((n_e, MAX_ADV))
-    for e in range(n_e):
-        c1, c2 = int(hm.cells_on_edge[e, 0]), int(hm.cells_on_edge[e, 1])
-        cl = [c1, c2]
-        pos = {c1: 0, c2: 1}
-        nb1 = [int(x) for x in hm.cells_on_cell[c1, : hm.n_edges_on_cell[c1]]]
-        nb2 = [int(x) for x in hm.cells_on_cell[c2, : hm.n_edges_on_cell[c2]]]
-        for nb in nb1:
-            if nb != c2 and nb not in pos:
-                pos[nb] = len(cl)
-                cl.append(nb)
-        for nb in nb2:
-            if nb not in pos:
-                pos[nb] = len(cl)
-                cl.append(nb)
-        n = len(cl)
-        n_adv[e] = n
-        adv_cells[e, :n] = cl
-        row, row3 = adv[e], adv3[e]
-        j = pos[c1]
-        row[j] += d2[e, 0, 0]
-        row3[j] += d2[e, 0, 0]
-        for i, nb in enumerate(nb1):
-            row[pos[nb]] += d2[e, 0, i + 1]
-            row3[pos[nb]] += d2[e, 0, i + 1]
-        j = pos[c2]
-        row[j] += d2[e, 1, 0]
-        row3[j] -= d2[e, 1, 0]
-        for i, nb in enumerate(nb2):
-            row[pos[nb]] += d2[e, 1, i + 1]
-            row3[pos[nb]] -= d2[e, 1, i + 1]
-        dcs = hm.dc_edge[e] ** 2
-        row[:n] = -(dcs) * row[:n] / 12.0
-        row3[:n] = -(dcs) * row3[:n] / 12.0
-        row[pos[c1]] += 0.5
-        row[pos[c2]] += 0.5
-        row[:n] = row[:n] * hm.dv_edge[e]
-        row3[:n] = row3[:n] * hm.dv_edge[e]
+    adv[rows, 0] += d2[:, 0, 0]
+    adv3[rows, 0] += d2[:, 0, 0]
+    for i in range(MAX_EDGES):
+        live = np.nonzero(i < noc[c1])[0]
+        adv[live, pos1[live, i]] += d2[live, 0, i + 1]
+        adv3[live, pos1[live, i]] += d2[live, 0, i + 1]
+    adv[rows, 1] += d2[:, 1, 0]
+    adv3[rows, 1] -= d2[:, 1, 0]
+    for i in range(MAX_EDGES):
+        live = np.nonzero(i < noc[c2])[0]
+        adv[live, pos2[live, i]] += d2[live, 1, i + 1]
+        adv3[live, pos2[live, i]] -= d2[live, 1, i + 1]
+    used = np.arange(MAX_ADV)[None, :] < n_adv[:, None]
+    dcs = (hm.dc_edge ** 2)[:, None]
+    adv = np.where(used, -(dcs) * adv / 12.0, 0.0)
+    adv3 = np.where(used, -(dcs) * adv3 / 12.0, 0.0)
+    adv[rows, 0] += 0.5
+    adv[rows, 1] += 0.5
+    adv = adv * hm.dv_edge[:, None]
+    adv3 = adv3 * hm.dv_edge[:, None]
     adv3 = adv3 * float(coef_3rd_order)
     return n_adv, adv_cells, adv, adv3
 

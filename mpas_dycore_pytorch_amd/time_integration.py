@@ -1,0 +1,220 @@
+"""``atm_srk3`` on the GPU: the drop-in for the reference's large step.
+
+``atm_srk3(state, mesh, config, dt, strict=True, differentiable=False)``
+keeps the signature, the dict contract and the error types of the reference
+(``mpas_pytorch/dynamics/time_integration.py:4897-4956``): it does not
+mutate the caller's mappings, it returns a new dict holding the advanced
+prognostics (``u, w, theta_m, rho_zz, scalars, rt_diabatic_tend``) and the
+diagnostics carried between steps (``atmosphere_model.py:157-201``).
+
+The whole large step (setup, 3 dynamics substeps x 3 RK stages with their
+acoustic substeps, split transport, reconstruction) is one call into
+``libmpas_dycore_hip.so`` on the current HIP stream; nothing runs on the CPU
+between kernels.  ``Srk3Stepper`` keeps the state resident across steps for
+callers that integrate many steps (``run_simulation`` style).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from typing import Any, Mapping
+
+import numpy as np
+import torch
+
+from . import _native as N
+from .device import DeviceMesh, require_gpu, stream_handle, to_numpy
+from .errors import MpasConfigError, MpasMissingFieldError, MpasUnsupportedCapabilityError
+
+# Reference defaults (mpas_pytorch/config_defaults.py:16-136)
+DEFAULTS = {
+    "config_time_integration_order": 2,
+    "config_split_dynamics_transport": True,
+    "config_dynamics_split_steps": 3,
+    "config_number_of_sub_steps": 2,
+    "config_epssm": 0.1,
+    "config_smdiv": 0.1,
+    "config_apvm_upwinding": 0.5,
+    "config_coef_3rd_order": 0.25,
+    "config_smagorinsky_coef": 0.125,
+    "config_visc4_2dsmag": 0.05,
+    "config_del4u_div_factor": 10.0,
+    "config_horiz_mixing": "2d_smagorinsky",
+    "config_monotonic": True,
+    "config_len_disp": 0.0,
+}
+# Short aliases the reference also accepts (config_defaults.py:149-160)
+ALIASES = {"smdiv": "config_smdiv", "len_disp": "config_len_disp", "monotonic": "config_monotonic",
+           "time_integration_order": "config_time_integration_order",
+           "del4u_div_factor": "config_del4u_div_factor", "visc4_2dsmag": "config_visc4_2dsmag",
+           "positive_definite": "config_positive_definite", "apply_lbcs": "config_apply_lbcs"}
+
+# Options whose non-default values select code the GPU step does not have.
+REQUIRED_VALUES = {
+    "config_time_integration_order": 3,
+    "config_split_dynamics_transport": True,
+    "config_horiz_mixing": "2d_smagorinsky",
+    "config_h_mom_eddy_visc2": 0.0,
+    "config_h_theta_eddy_visc2": 0.0,
+    "config_v_mom_eddy_visc2": 0.0,
+    "config_v_theta_eddy_visc2": 0.0,
+    "config_h_mom_eddy_visc4": 0.0,
+    "config_h_theta_eddy_visc4": 0.0,
+    "config_rayleigh_damp_u": False,
+    "config_mpas_cam_coef": 0.0,
+    "config_curvature_u": False,
+    "config_apply_lbcs": False,
+    "config_positive_definite": False,
+    "config_mix_full": True,
+    "config_scalar_advection": True,
+    "config_w_adv_order": 3, "config_theta_adv_order": 3, "config_scalar_adv_order": 3,
+    "config_u_vadv_order": 3, "config_w_vadv_order": 3, "config_theta_vadv_order": 3,
+    "config_scalar_vadv_order": 3,
+}
+
+PROGNOSTIC_KEYS = ("u", "w", "theta_m", "rho_zz", "scalars", "rt_diabatic_tend")
+CARRIED_KEYS = ("rho_p", "rtheta_p", "exner", "pressure_p", "ru", "rw")
+DIAGNOSTIC_KEYS = ("v", "divergence", "vorticity", "ke", "pv_edge", "rho_edge")
+OUTPUT_ONLY_KEYS = ("ur_cell", "vr_cell", "uReconstructX", "uReconstructY", "uReconstructZ",
+                    "uReconstructZonal", "uReconstructMeridional", "ruAvg", "wwAvg")
+
+
+def _cfg(config: Mapping[str, Any] | None) -> dict[str, Any]:
+    out = dict(DEFAULTS)
+    for k, v in (config or {}).items():
+        out[ALIASES.get(k, k)] = v
+    return out
+
+
+def _same(a: Any, b: Any) -> bool:
+    if isinstance(b, bool) or isinstance(b, str):
+        return a == b
+    try:
+        return float(a) == float(b)
+    except (TypeError, ValueError):
+        return False
+
+
+def resolve_config(config: Mapping[str, Any] | None, state: Mapping[str, Any], mesh: Mapping[str, Any],
+                   dt: float, n_scalars: int) -> N.ConfigStruct:
+    """Check the configuration against what the GPU step implements and pack it."""
+    c = _cfg(config)
+    for key, want in REQUIRED_VALUES.items():
+        if key in c and not _same(c[key], want):
+            raise MpasUnsupportedCapabilityError(capability=key, requested=c[key])
+    dt = float(dt)
+    if not dt > 0.0:
+        raise MpasConfigError(f"dt must be > 0, got {dt}")
+    ns, split = int(c["config_number_of_sub_steps"]), int(c["config_dynamics_split_steps"])
+    if ns < 1 or split < 1:
+        raise MpasConfigError("config_number_of_sub_steps and config_dynamics_split_steps must be >= 1")
+    len_disp = float(c["config_len_disp"])
+    if not len_disp > 0.0:
+        raise MpasConfigError("config_len_disp must be set to a positive length (e.g. the nominal dc)")
+    coef3 = float(c["config_coef_3rd_order"])
+    if "coef_3rd_order" in state and float(to_numpy(state["coef_3rd_order"])) != coef3:
+        raise MpasUnsupportedCapabilityError(
+            "state coef_3rd_order differs from config_coef_3rd_order", capability="coef_3rd_order",
+            requested=float(to_numpy(state["coef_3rd_order"])))
+    c_s = float(c["config_smagorinsky_coef"])
+    if "c_s" in state:
+        c_s = float(to_numpy(state["c_s"]))
+    s = N.ConfigStruct()
+    s.dt, s.len_disp, s.epssm, s.smdiv = dt, len_disp, float(c["config_epssm"]), float(c["config_smdiv"])
+    s.apvm, s.coef3, s.c_s = float(c["config_apvm_upwinding"]), coef3, c_s
+    s.visc4_2dsmag, s.del4u_div_factor = float(c["config_visc4_2dsmag"]), float(c["config_del4u_div_factor"])
+    s.ns, s.split, s.monotonic, s.nScalars = ns, split, 1 if bool(c["config_monotonic"]) else 0, n_scalars
+    s.moist_start = int(to_numpy(state.get("moist_start", 1)))
+    s.moist_end = int(to_numpy(state.get("moist_end", n_scalars)))
+    if s.moist_start < 1 or s.moist_end > n_scalars:
+        raise MpasConfigError(f"moist species range {s.moist_start}..{s.moist_end} outside 1..{n_scalars}")
+    s.diagnostics_current = 0
+    return s
+
+
+def _device_tensor(v: Any, device: torch.device, shape: tuple[int, ...], name: str) -> torch.Tensor:
+    if isinstance(v, torch.Tensor):
+        t = v.detach().to(device=device, dtype=torch.float64).clone()
+    else:
+        t = torch.as_tensor(np.array(v, dtype=np.float64), device=device)
+    if tuple(t.shape) != shape:
+        raise MpasMissingFieldError(f"{name} has shape {tuple(t.shape)}, expected {shape}", fields=[name],
+                                    source="state")
+    return t.contiguous()
+
+
+class Srk3Stepper:
+    """Device-resident state advanced in place by successive large steps."""
+
+    def __init__(self, state: Mapping[str, Any], mesh: Mapping[str, Any], config: Mapping[str, Any] | None,
+                 dt: float, device: torch.device | str | None = None, strict: bool = True):
+        self.device = require_gpu(device)
+        if "scalars" not in state:
+            raise MpasMissingFieldError(fields=["scalars"], source="state")
+        sc = state["scalars"]
+        n_scalars = int(sc.shape[2]) if len(sc.shape) == 3 else 0
+        self.n_scalars = n_scalars
+        self.mesh = DeviceMesh.get(mesh, self.device, state)
+        self.cfg = resolve_config(config, state, mesh, dt, n_scalars)
+        dm = self.mesh
+        missing = [k for k in PROGNOSTIC_KEYS[:5] + CARRIED_KEYS if k not in state]
+        if missing:
+            raise MpasMissingFieldError(fields=missing, source="state")
+        t: dict[str, torch.Tensor] = {}
+        with torch.cuda.device(self.device):
+            for k in PROGNOSTIC_KEYS + CARRIED_KEYS + DIAGNOSTIC_KEYS + OUTPUT_ONLY_KEYS:
+                shape = dm.shape_of(k, n_scalars)
+                src = state.get(k)
+                if src is None and k == "rho_edge":
+                    src = state.get("h_edge")
+                if src is None:
+                    t[k] = torch.zeros(shape, dtype=torch.float64, device=self.device)
+                else:
+                    t[k] = _device_tensor(src, self.device, shape, k)
+            self.work = dm.workspace(n_scalars)
+        self.t = t
+        self._diagnostics_current = False
+        self._st = N.StateStruct(*[t[k].data_ptr() for k in N.STATE_FIELDS])
+        self._ws = N.WorkspaceStruct(self.work.data_ptr(), self.work.numel())
+
+    def step(self, n: int = 1) -> None:
+        """Advance ``n`` large steps in place on the current stream (no host sync)."""
+        lib = N.load()
+        stream = stream_handle(self.device)
+        with torch.cuda.device(self.device):
+            for _ in range(n):
+                self.cfg.diagnostics_current = 1 if self._diagnostics_current else 0
+                N.check(lib.mpas_hip_srk3_step(self.mesh.ref, C.byref(self._st), C.byref(self._ws),
+                                               C.byref(self.cfg), C.c_void_p(stream)), "mpas_hip_srk3_step")
+                self._diagnostics_current = True
+
+    def state_dict(self) -> dict[str, torch.Tensor]:
+        out = dict(self.t)
+        out["h_edge"] = self.t["rho_edge"]
+        return out
+
+
+def atm_srk3(state: Mapping[str, Any], mesh: Mapping[str, Any], config: Mapping[str, Any], dt: float,
+             strict: bool = True, differentiable: bool = False) -> dict[str, Any]:
+    """One split-explicit RK3 large step on the GPU (reference ``atm_srk3``).
+
+    Inputs may be CPU or GPU tensors (or NumPy arrays) in the reference
+    layout; the step runs on the current ROCm device and the results come
+    back on the device of ``state["u"]``.
+    """
+    if differentiable:
+        raise MpasUnsupportedCapabilityError(
+            "differentiable=True: the GPU adjoint of the large step is not implemented",
+            capability="differentiable", requested=True)
+    u = state.get("u")
+    out_device = u.device if isinstance(u, torch.Tensor) else torch.device("cpu")
+    gpu = out_device if out_device.type == "cuda" else None
+    stepper = Srk3Stepper(state, mesh, config, dt, device=gpu, strict=strict)
+    stepper.step(1)
+    new = stepper.state_dict()
+    result = dict(state)
+    if out_device.type != "cuda":
+        torch.cuda.synchronize(stepper.device)
+        new = {k: v.to(out_device) for k, v in new.items()}
+    result.update(new)
+    return result

@@ -1,0 +1,181 @@
+"""HIP kernels and the HIP large step against the oracle and the golden vectors.
+
+Bars (float64 throughout):
+
+* per kernel, on the recorded reference inputs: bit for bit, except outputs
+  that pass through pow / sin / cos (``LIBM_OUTPUTS``), which must agree to
+  1e-14 relative (the device libm and glibc may differ in the last ulp);
+* whole large steps: 1e-11 relative per field (max abs error over max abs
+  value), against the reference outputs and against the oracle;
+* full-size runs: size-independent properties (dry-mass conservation,
+  run-to-run bit reproducibility, finiteness).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import kernel_calls, rel_err, srk3_case
+from kernel_cases import LIBM_OUTPUTS, run_call
+
+pytestmark = pytest.mark.gpu
+
+STEP_TOL = 1e-11
+LIBM_TOL = 1e-14
+FIELDS = ("u", "w", "theta_m", "rho_zz", "scalars", "exner", "pressure_p", "rtheta_p", "rho_p", "ru", "rw",
+          "v", "divergence", "vorticity", "ke", "pv_edge", "uReconstructZonal", "uReconstructMeridional")
+
+CALLS_MERGED, CALLS = kernel_calls()
+
+
+def _dev():
+    return torch.device("cuda", 0)
+
+
+@pytest.fixture(scope="module")
+def kdmesh(gpu_available):
+    from mpas_dycore_pytorch_amd.device import DeviceMesh
+
+    return DeviceMesh(CALLS_MERGED, _dev())
+
+
+@pytest.mark.parametrize("idx", range(len(CALLS)), ids=[c[0] for c in CALLS])
+def test_kernel_matches_reference(kdmesh, idx):
+    from mpas_dycore_pytorch_amd import kernels as K
+
+    variant, I, expected = CALLS[idx]
+    print(f"kernel case {variant}", flush=True)
+    pairs = run_call(K, kdmesh, variant, I, expected.keys())
+    torch.cuda.synchronize()
+    assert pairs
+    for name, got, key in pairs:
+        ref = np.asarray(expected[key])
+        assert np.all(np.isfinite(got)), f"{variant}:{name} not finite"
+        if name in LIBM_OUTPUTS:
+            err = rel_err(got, ref)
+            assert err <= LIBM_TOL, f"{variant}:{name} rel err {err:.3e}"
+        else:
+            same = np.array_equal(got[: ref.shape[0]], ref)
+            assert same, f"{variant}:{name} not bit-exact, rel err {rel_err(got, ref):.3e}"
+
+
+def _gpu_step(state, dt, len_disp, steps, n_scalars_cfg=None):
+    from mpas_dycore_pytorch_amd.time_integration import Srk3Stepper
+
+    cfg = {"config_time_integration_order": 3, "config_len_disp": len_disp}
+    stepper = Srk3Stepper(state, state, cfg, dt, device=_dev())
+    outs = []
+    for _ in range(steps):
+        stepper.step(1)
+        torch.cuda.synchronize()
+        outs.append({k: v.detach().cpu().numpy() for k, v in stepper.state_dict().items()})
+    return outs
+
+
+@pytest.mark.parametrize("case", ["l2_L26_dry", "l2_L20_tracer"])
+def test_srk3_matches_reference(gpu_available, case):
+    st, refs, meta = srk3_case(case)
+    print(f"srk3 case {case}", flush=True)
+    outs = _gpu_step(st, meta["dt"], float(st["nominalMinDc"]), meta["steps"])
+    for step, (got, ref) in enumerate(zip(outs, refs)):
+        for k in FIELDS:
+            if k in ref:
+                err = rel_err(got[k], ref[k])
+                assert err <= STEP_TOL, f"{case} step {step + 1} {k}: rel err {err:.3e}"
+
+
+def test_atm_srk3_dict_contract(gpu_available):
+    """The drop-in returns a new dict, leaves the caller's tensors untouched."""
+    from mpas_dycore_pytorch_amd import atm_srk3
+
+    st, refs, meta = srk3_case("l2_L20_tracer")
+    state = {k: (torch.from_numpy(np.array(v)) if isinstance(v, np.ndarray) else v) for k, v in st.items()}
+    u0 = state["u"].clone()
+    cfg = {"config_time_integration_order": 3, "config_len_disp": float(st["nominalMinDc"])}
+    out = atm_srk3(state, state, cfg, meta["dt"])
+    assert out is not state
+    assert torch.equal(state["u"], u0)
+    assert out["u"].device.type == "cpu"
+    for k in ("u", "w", "theta_m", "rho_zz", "scalars", "rt_diabatic_tend", "h_edge", "rho_edge", "exner",
+              "pressure_p", "ru", "rw", "v", "ke", "vorticity", "divergence", "pv_edge", "ur_cell", "vr_cell"):
+        assert k in out, k
+    for k in ("u", "theta_m", "scalars"):
+        assert rel_err(out[k].numpy(), refs[0][k]) <= STEP_TOL
+    # GPU-resident inputs stay on the GPU
+    gstate = {k: (v.cuda() if isinstance(v, torch.Tensor) else v) for k, v in state.items()}
+    gout = atm_srk3(gstate, gstate, cfg, meta["dt"])
+    assert gout["u"].is_cuda
+    assert torch.equal(gout["u"].cpu(), out["u"])
+
+
+def _prepared(level, n_levels, stretch, n_tracers, dt, seed):
+    """Cold-start state prepared by the oracle (the checker's own chain)."""
+    from mpas_dycore_pytorch_amd import initial_state as IS
+    from mpas_dycore_pytorch_amd import mesh as M
+    from oracle import oracle as O
+
+    m = M.build_mesh(level, n_levels, stretch=stretch)
+    raw = IS.raw_state(m, seed=seed, n_tracers=n_tracers)
+    om = O.Mesh({**m, **IS.init_coupled_diagnostics(m, raw), **raw})
+    st = IS.prepare_state(m, raw, diagnostics=lambda u, h: O.solve_diagnostics(om, dt, 0.5, None, u, h),
+                          reconstruct=lambda u: O.reconstruct(om, u))
+    return m, st
+
+
+@pytest.mark.parametrize("level,n_levels,stretch,n_tracers,dt,steps,seed", [
+    (4, 26, "uniform", 0, 1800.0, 2, 20240),      # the 2-step CPU-runnable configuration
+    (4, 55, "quadratic", 1, 900.0, 2, 77),        # 55 stretched levels with a tracer
+])
+def test_srk3_matches_oracle(gpu_available, level, n_levels, stretch, n_tracers, dt, steps, seed):
+    from oracle import oracle as O
+    from oracle import srk3 as OS
+
+    print(f"oracle case level {level} L {n_levels} tracers {n_tracers}", flush=True)
+    m, st = _prepared(level, n_levels, stretch, n_tracers, dt, seed)
+    outs = _gpu_step(st, dt, float(m["nominalMinDc"]), steps)
+    om = O.Mesh(st)
+    cur = st
+    for step in range(steps):
+        cur = OS.srk3(cur, om, dt, len_disp=float(m["nominalMinDc"]), diagnostics_current=(step > 0))
+        for k in FIELDS:
+            if k == "scalars" and n_tracers == 0:
+                continue
+            err = rel_err(outs[step][k], cur[k][: outs[step][k].shape[0]])
+            assert err <= STEP_TOL, f"step {step + 1} {k}: rel err {err:.3e}"
+
+
+def test_level6_properties(gpu_available):
+    """40962 x 55 with a tracer: dry mass conserved, tracer mass conserved,
+    bit-reproducible, finite."""
+    from mpas_dycore_pytorch_amd.time_integration import Srk3Stepper
+
+    print("level 6 properties", flush=True)
+    m, st = _prepared(6, 55, "quadratic", 1, 450.0, 11)
+    nc, L = m["nCells"], m["nVertLevels"]
+    area = np.asarray(m["areaCell"][:nc])
+    dz = 1.0 / np.asarray(m["rdzw"])
+    zz = np.asarray(m["zz"][:nc])
+
+    def mass(rho_zz):
+        return float(np.sum(area[:, None] * rho_zz[:nc] * zz * dz[None, :]))
+
+    def tracer_mass(rho_zz, q):
+        return float(np.sum(area[:, None] * rho_zz[:nc] * zz * q[:nc, :, 1] * dz[None, :]))
+
+    cfg = {"config_time_integration_order": 3, "config_len_disp": float(m["nominalMinDc"])}
+    a = Srk3Stepper(st, st, cfg, 450.0, device=_dev())
+    b = Srk3Stepper(st, st, cfg, 450.0, device=_dev())
+    m0 = mass(np.asarray(st["rho_zz"]))
+    t0 = tracer_mass(np.asarray(st["rho_zz"]), np.asarray(st["scalars"]))
+    a.step(2)
+    b.step(2)
+    torch.cuda.synchronize()
+    for k, v in a.state_dict().items():
+        assert torch.equal(v, b.state_dict()[k]), f"{k} not reproducible"
+        assert bool(torch.isfinite(v).all()), f"{k} not finite"
+    rz = a.t["rho_zz"].cpu().numpy()
+    q = a.t["scalars"].cpu().numpy()
+    assert abs(mass(rz) - m0) <= 1e-13 * m0
+    assert abs(tracer_mass(rz, q) - t0) <= 1e-12 * abs(t0)

@@ -1,0 +1,93 @@
+"""The CPU oracle against the reference's own outputs (golden vectors).
+
+Every fixture under ``tests/golden`` was written by running the reference
+(``tests/golden/make_golden.py``); these tests pin the oracle to it:
+
+* each of the 20 kernel-call variants that one reference large step makes
+  (level-1 mesh, 10 levels, one tracer) must be reproduced bit for bit;
+* whole large steps (level 2, dry and with a tracer, 2-3 steps) within
+  1e-14 relative per field;
+* a level-4, 26-level case (the 2-step CPU configuration) through field
+  norms and seeded samples, from a state prepared by this package's own
+  cold-start chain.
+"""
+
+from __future__ import annotations
+
+import json
+
+import numpy as np
+import pytest
+
+from conftest import golden_path, kernel_calls, rel_err, srk3_case
+from kernel_cases import run_call
+
+from oracle import oracle as O
+from oracle import srk3 as OS
+
+CALLS_MERGED, CALLS = kernel_calls()
+
+
+@pytest.fixture(scope="module")
+def kmesh():
+    return O.Mesh(CALLS_MERGED)
+
+
+@pytest.mark.parametrize("idx", range(len(CALLS)), ids=[c[0] for c in CALLS])
+def test_kernel_call_bit_exact(kmesh, idx):
+    variant, I, expected = CALLS[idx]
+    for name, got, key in run_call(O, kmesh, variant, I, expected.keys()):
+        ref = np.asarray(expected[key])
+        assert np.array_equal(got[: ref.shape[0]], ref), f"{variant}:{name} rel err {rel_err(got, ref):.3e}"
+
+
+def test_every_recorded_variant_is_covered():
+    kinds = {v.split(":")[0] for v, _, _ in CALLS}
+    assert len(CALLS) == 20
+    assert any("dyn_tend" in k for k in kinds) and any("monotonic" in k for k in kinds)
+
+
+FIELDS = ("u", "w", "theta_m", "rho_zz", "scalars", "exner", "pressure_p", "rtheta_p", "rho_p", "ru", "rw",
+          "v", "divergence", "vorticity", "ke", "pv_edge", "uReconstructZonal", "uReconstructMeridional")
+
+
+@pytest.mark.parametrize("case", ["l2_L26_dry", "l2_L20_tracer"])
+def test_srk3_whole_steps(case):
+    st, outs, meta = srk3_case(case)
+    mesh = O.Mesh(st)
+    cur = st
+    for step, ref in enumerate(outs):
+        cur = OS.srk3(cur, mesh, meta["dt"], len_disp=float(st["nominalMinDc"]),
+                      diagnostics_current=(step > 0))
+        for k in FIELDS:
+            if k in ref:
+                err = rel_err(cur[k], ref[k])
+                assert err <= 1e-14, f"{case} step {step + 1} {k}: rel err {err:.3e}"
+
+
+@pytest.mark.slow
+def test_level4_summary_from_own_cold_start():
+    from mpas_dycore_pytorch_amd import initial_state as IS
+    from mpas_dycore_pytorch_amd import mesh as M
+
+    with open(golden_path("summary_cfg1_l4_L26.json")) as f:
+        summ = json.load(f)
+    m = M.build_mesh(summ["level"], summ["nVertLevels"], stretch=summ["stretch"])
+    raw = IS.raw_state(m, seed=summ["seed"], n_tracers=summ["n_tracers"])
+    om = O.Mesh({**m, **IS.init_coupled_diagnostics(m, raw), **raw})
+    dt = summ["dt"]
+    st = IS.prepare_state(m, raw,
+                          diagnostics=lambda u, h: O.solve_diagnostics(om, dt, 0.5, None, u, h),
+                          reconstruct=lambda u: O.reconstruct(om, u))
+    om = O.Mesh(st)
+    cur = st
+    for step, rec in enumerate(summ["steps_out"]):
+        cur = OS.srk3(cur, om, dt, len_disp=float(m["nominalMinDc"]), diagnostics_current=(step > 0))
+        for k, r in rec.items():
+            a = np.asarray(cur[k]).reshape(cur[k].shape[0], -1)[:-1]
+            scale = max(r["linf"], 1e-300)
+            vals = a.reshape(-1)[np.asarray(r["idx"])]
+            err = float(np.max(np.abs(vals - np.asarray(r["val"])))) / scale
+            assert err <= 1e-11, f"step {step + 1} {k}: sample rel err {err:.3e}"
+            assert abs(float(np.sqrt(np.sum(a * a))) - r["l2"]) <= 1e-11 * max(r["l2"], 1e-300), k
+            assert abs(float(np.max(np.abs(a))) - r["linf"]) <= 1e-11 * scale, k
