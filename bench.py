@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Throughput of the GPU split-explicit large step (``atm_srk3``) on one MI355X.
+
+Workload (default): one large step = the reference's RK3 split-explicit
+step (3 dynamics substeps x 3 RK stages, 2 acoustic substeps, monotonic FCT
+transport of the water-vapour slot) on a synthetic icosahedral-Voronoi mesh
+of level 7 (163842 cells, 491520 edges, 327680 vertices) with 55 levels
+stretched quadratically to 30 km, flat terrain, dt = 300 s, FP64.  The
+state is the seeded balanced jet with a smoothed theta perturbation,
+prepared on the GPU.  Inputs are resident in HBM before timing starts.
+
+Prints one JSON line: throughput in cell-level-steps/s, the roofline of the
+dominant kernel (its average duration measured with HIP events on the launch
+stream during the timed steps, against its algorithmic bytes per launch),
+the whole-step algorithmic bandwidth, and a CPU baseline (the oracle, a
+single-threaded C restatement of the reference, timed on a bounded sample).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-level-steps/s at 163842x55 FP64; VJP/fwd cost ratio; % HBM roofline"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def prepare(level: int, n_levels: int, stretch: str, n_tracers: int, dt: float, seed: int, device):
+    """Mesh, raw synthetic fields and the cold-start chain (diagnostics on the GPU)."""
+    import numpy as np
+
+    from mpas_dycore_pytorch_amd import initial_state as IS
+    from mpas_dycore_pytorch_amd import kernels as K
+    from mpas_dycore_pytorch_amd import mesh as M
+    from mpas_dycore_pytorch_amd.device import DeviceMesh
+
+    t0 = time.time()
+    m = M.build_mesh(level, n_levels, stretch=stretch)
+    raw = IS.raw_state(m, seed=seed, n_tracers=n_tracers)
+    base = {**m, **raw, **IS.init_coupled_diagnostics(m, raw)}
+    dm = DeviceMesh(base, device)
+
+    def diag(u, h):
+        d = K.solve_diagnostics(dm, dt, 0.5, None, u, h)
+        return {k: v.cpu().numpy() for k, v in d.items()}
+
+    def recon(u):
+        return {k: v.cpu().numpy() for k, v in K.reconstruct(dm, u).items()}
+
+    st = IS.prepare_state(m, raw, diagnostics=diag, reconstruct=recon)
+    log(f"prepared level {level} x {n_levels} ({m['nCells']} cells) in {time.time() - t0:.1f} s")
+    return m, st, np
+
+
+def traffic_from_profile(kernel: str, workload: str):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    mm = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    return float(mm.group(1)) if mm else None
+
+
+def cpu_baseline(level: int, n_levels: int, dt: float, steps: int, seed: int) -> dict:
+    """The oracle (C restatement of the reference kernels, one thread) on a bounded sample."""
+    from mpas_dycore_pytorch_amd import initial_state as IS
+    from mpas_dycore_pytorch_amd import mesh as M
+    from oracle import oracle as O
+    from oracle import srk3 as OS
+
+    m = M.build_mesh(level, n_levels, stretch="quadratic")
+    raw = IS.raw_state(m, seed=seed, n_tracers=0)
+    om = O.Mesh({**m, **IS.init_coupled_diagnostics(m, raw), **raw})
+    st = IS.prepare_state(m, raw, diagnostics=lambda u, h: O.solve_diagnostics(om, dt, 0.5, None, u, h),
+                          reconstruct=lambda u: O.reconstruct(om, u))
+    om = O.Mesh(st)
+    cur = st
+    t0 = time.perf_counter()
+    for step in range(steps):
+        cur = OS.srk3(cur, om, dt, len_disp=float(m["nominalMinDc"]), diagnostics_current=(step > 0))
+    el = time.perf_counter() - t0
+    units = m["nCells"] * n_levels * steps
+    return {"value": units / el, "unit": "cell-level-steps/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle srk3 (single-threaded C kernels, Python orchestration), level {level} mesh "
+                       f"({m['nCells']} cells) x {n_levels} levels, dt {dt:g} s, {steps} large steps, "
+                       f"{el:.1f} s wall")}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--level", type=int, default=7)
+    ap.add_argument("--levels", type=int, default=55)
+    ap.add_argument("--dt", type=float, default=300.0)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--roofline-kernel", default="k_acoustic_cells")
+    ap.add_argument("--cpu-level", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--reorder", type=int, default=0, help="1: space-filling-curve mesh order on the device")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (one process, one MI355X); "
+              "run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from mpas_dycore_pytorch_amd import _native as N
+    from mpas_dycore_pytorch_amd import bytes_model as B
+    from mpas_dycore_pytorch_amd.time_integration import Srk3Stepper
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        return 1
+    device = torch.device("cuda", 0)
+    torch.cuda.set_device(device)
+    lib = N.load()
+
+    m, st, np = prepare(args.level, args.levels, "quadratic", 0, args.dt, args.seed, device)
+    nC, nE, nV, L = m["nCells"], m["nEdges"], m["nVertices"], args.levels
+    cfg = {"config_time_integration_order": 3, "config_len_disp": float(m["nominalMinDc"])}
+    stepper = Srk3Stepper(st, st, cfg, args.dt, device=device, reorder=bool(args.reorder))
+    S = stepper.n_scalars
+    del st
+
+    stepper.step(args.warmup)
+    torch.cuda.synchronize()
+    log(f"warmup {args.warmup} steps done")
+
+    per_step = {"k_acoustic_cells": 12, "k_acoustic_edges": 12, "k_divdamp": 12, "k_dyn_cells_b": 9,
+                "k_dyn_edges_a": 9, "k_dyn_edges_b": 9, "k_dyn_cells_a": 9, "k_recover_cells": 9,
+                "k_recover_edges": 9, "k_recover_w": 9, "k_smlstep": 9, "k_vimp": 6}.get(args.roofline_kernel, 64)
+    N.check(lib.mpas_hip_timer_arm(args.roofline_kernel.encode(), per_step * args.steps), "timer_arm")
+    stream = torch.cuda.current_stream(device)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    stepper.step(args.steps)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    total_ms = ev0.elapsed_time(ev1)
+    kt, kn = C.c_double(), C.c_int32()
+    N.check(lib.mpas_hip_timer_read(C.byref(kt), C.byref(kn)), "timer_read")
+    for k, v in stepper.state_dict().items():
+        if not bool(torch.isfinite(v).all()):
+            raise RuntimeError(f"non-finite {k} after the timed steps")
+    ms_per_step = total_ms / args.steps
+    units = nC * L
+    value = units * args.steps / (total_ms / 1e3)
+    log(f"{args.steps} steps: {ms_per_step:.3f} ms/step, {value:.4e} cell-level-steps/s")
+
+    k_avg_ms = kt.value / max(kn.value, 1)
+    k_bytes = B.kernel_bytes(args.roofline_kernel, nC, nE, nV, L, S)
+    k_gbs = k_bytes / (k_avg_ms / 1e3) / 1e9
+    workload_tag = f"l{args.level}_L{args.levels}"
+    roof = {"bound": "hbm", "kernel": args.roofline_kernel, "achieved": round(k_gbs, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(k_gbs / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profile(args.roofline_kernel, workload_tag),
+            "algorithmic_bytes_per_launch": k_bytes, "avg_launch_ms": round(k_avg_ms, 5),
+            "launches_timed": kn.value}
+    sb = B.step_bytes(nC, nE, nV, L, S)
+    step_gbs = sb / (ms_per_step / 1e3) / 1e9
+    step_roof = {"algorithmic_bytes": sb, "achieved": round(step_gbs, 1), "unit": "GB/s",
+                 "frac": round(step_gbs / HBM_PEAK_GBS, 4)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        del stepper
+        torch.cuda.empty_cache()
+        log("cpu baseline (oracle) ...")
+        cpu = cpu_baseline(args.cpu_level, args.levels, {5: 900.0, 4: 1800.0, 6: 450.0}.get(args.cpu_level, 300.0),
+                           args.cpu_steps, args.seed)
+        log(f"cpu baseline {cpu['value']:.4e} cell-level-steps/s")
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-level-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": (f"atm_srk3 large step: icosahedral level {args.level} ({nC} cells, {nE} edges) x "
+                                f"{L} quadratic levels to 30 km, dt {args.dt:g} s, RK3 split 3 x 2 acoustic "
+                                f"substeps, monotonic FCT on {S} scalar(s), flat terrain"),
+                   "cells": nC, "levels": L, "dt": args.dt, "parallelism": "single GPU",
+                   "mesh_order": "hilbert" if args.reorder else "generator"},
+        "roofline": roof, "step_roofline": step_roof, "vjp_fwd_ratio": None, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -196,11 +196,29 @@ int mpas_hip_advance_scalars_mono(const mpas_hip_mesh *m, int32_t nScalars, doub
                                   const double *rho_zz_old, const double *uhAvg,
                                   const double *wwAvg, double *scalars_new_out,
                                   double *rho_zz_int, double *scratch, void *stream);
+/* atm_init_coupled_diagnostics, init_coupled_diagnostics.py:147 (cold start
+ * from raw theta, rho, qv, u, w; uses the mesh's rho_base, zz, fzm, fzp and
+ * terrain terms; writes padded outputs with rho_zz's padding row = 1) */
+int mpas_hip_init_coupled_diagnostics(const mpas_hip_mesh *m, const double *theta, const double *rho,
+                                      const double *qv, const double *u, const double *w,
+                                      const double *theta_base, double *theta_m, double *rho_zz,
+                                      double *ru, double *rw, double *rho_p, double *rtheta_base,
+                                      double *rtheta_p, double *exner, double *exner_base,
+                                      double *pressure_p, double *pressure_base, void *stream);
 /* mpas_reconstruct_2d, vector_reconstruction.py:311 */
 int mpas_hip_reconstruct(const mpas_hip_mesh *m, const double *u, double *ux, double *uy,
                          double *uz, double *uzonal, double *umerid, void *stream);
 
 const char *mpas_hip_last_error(void);
+
+/* Measurement hook, no counterpart in the reference (its timings come from
+ * SectionTimer on the CPU): arm an event timer for the HIP kernel named
+ * kernel_name (e.g. "k_acoustic_cells"); each of its next `capacity`
+ * launches, from any entry point, is bracketed by HIP events on its stream.
+ * mpas_hip_timer_read waits for the recorded events and returns the summed
+ * device time and the launch count, then disarms. */
+int mpas_hip_timer_arm(const char *kernel_name, int32_t capacity);
+int mpas_hip_timer_read(double *total_ms, int32_t *launches);
 
 #ifdef __cplusplus
 }

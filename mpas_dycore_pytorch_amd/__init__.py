@@ -20,7 +20,9 @@ from .errors import (  # noqa: F401
     MpasUnsupportedCapabilityError,
 )
 from .time_integration import Srk3Stepper, atm_srk3  # noqa: F401
+from .atmosphere_model import run_simulation, run_timestep  # noqa: F401
 
-__all__ = ["atm_srk3", "Srk3Stepper", "MpasError", "MpasConfigError", "MpasDeviceError",
+__all__ = ["atm_srk3", "Srk3Stepper", "run_timestep", "run_simulation", "MpasError", "MpasConfigError",
+           "MpasDeviceError",
            "MpasMeshMismatchError", "MpasMissingFieldError", "MpasNumericalAgreementError",
            "MpasStrictModeError", "MpasUnsupportedCapabilityError"]

@@ -94,7 +94,10 @@ SIGNATURES = {
     "mpas_hip_advance_scalars": (C.c_int, [_PM, _I32, _F64, _F64, _F64, _F64] + [_dp] * 8 + [_vp]),
     "mpas_hip_advance_scalars_mono": (C.c_int, [_PM, _I32, _F64, _F64] + [_dp] * 8 + [_vp]),
     "mpas_hip_reconstruct": (C.c_int, [_PM] + [_dp] * 6 + [_vp]),
+    "mpas_hip_init_coupled_diagnostics": (C.c_int, [_PM] + [_dp] * 17 + [_vp]),
     "mpas_hip_last_error": (C.c_char_p, []),
+    "mpas_hip_timer_arm": (C.c_int, [C.c_char_p, _I32]),
+    "mpas_hip_timer_read": (C.c_int, [C.POINTER(C.c_double), C.POINTER(_I32)]),
 }
 
 _lib = None

@@ -1,0 +1,136 @@
+"""Public driver API over the GPU step: ``run_timestep`` and ``run_simulation``.
+
+Mirrors the reference driver (``mpas_pytorch/atmosphere_model.py``):
+
+* ``run_timestep(state, mesh, config=None, dt=None, device=..., differentiable=False)``
+  (:900) prepares the state unless it is a completed step boundary, then
+  advances one large step (:738-780);
+* ``run_simulation(initial_state, mesh, config=None, dt=None, nsteps=1, device=...,
+  on_step=None, ..., on_phase=None, on_initial=None)`` (:1474) prepares once and
+  advances ``nsteps`` steps, calling ``on_step(step, state)`` after each.
+
+The cold-start preparation (``_prepare_timestep_state`` :782-897:
+``atm_init_coupled_diagnostics``, the padding sentinels, the first
+diagnostic solve and the vector reconstruction) runs on the GPU.  Outputs are
+returned on ``device`` (the reference's default is the CPU; computation is
+always on the current ROCm GPU).  Results carry the key
+``STEP_BOUNDARY_KEY`` so that a later call does not re-derive diagnostics,
+as the reference's step-boundary receipt does (:776-779).
+"""
+
+from __future__ import annotations
+
+from typing import Any, Callable, Mapping
+
+import numpy as np
+import torch
+
+from . import kernels as K
+from .device import DeviceMesh, require_gpu, to_numpy
+from .errors import MpasConfigError, MpasMissingFieldError, MpasUnsupportedCapabilityError
+from .time_integration import Srk3Stepper
+
+STEP_BOUNDARY_KEY = "_mpas_hip_step_boundary"
+RAW_KEYS = ("theta", "rho", "qv", "u", "w", "theta_base")
+
+
+def _dt(config: Mapping[str, Any] | None, dt: float | None) -> float:
+    if dt is None:
+        dt = float((config or {}).get("config_dt", 720.0))
+    dt = float(dt)
+    if not np.isfinite(dt) or dt <= 0.0:
+        raise MpasConfigError("dt must be finite and positive")
+    return dt
+
+
+def cold_start(state: Mapping[str, Any], mesh: Mapping[str, Any], config: Mapping[str, Any] | None, dt: float,
+               device: torch.device) -> dict[str, Any]:
+    """GPU cold start of raw fields; returns the prepared mapping (device tensors)."""
+    missing = [k for k in RAW_KEYS if k not in state and k not in mesh]
+    if missing:
+        raise MpasMissingFieldError(fields=missing, source="state")
+    merged = {**mesh, **state}
+    dm = DeviceMesh(merged, device, None)
+    icd = K.init_coupled_diagnostics(dm, merged["theta"], merged["rho"], merged["qv"], merged["u"], merged["w"],
+                                     merged["theta_base"])
+    dm.bind_base("rtheta_base", icd["rtheta_base"])
+    dm.bind_base("exner_base", icd["exner_base"])
+    apvm = float((config or {}).get("config_apvm_upwinding", 0.5))
+    d = K.solve_diagnostics(dm, dt, apvm, None, merged["u"], icd["rho_zz"])
+    r = K.reconstruct(dm, merged["u"])
+    out = dict(state)
+    out.update(icd)
+    out.update({k: d[k] for k in ("h_edge", "v", "divergence", "vorticity", "ke", "pv_edge")})
+    out["rho_edge"] = d["h_edge"]
+    out.update(r)
+    out.setdefault("coef_3rd_order", float((config or {}).get("config_coef_3rd_order", 0.25)))
+    out.setdefault("c_s", float((config or {}).get("config_smagorinsky_coef", 0.125)))
+    if "scalars" in out and "num_scalars" not in out:
+        out["num_scalars"] = int(out["scalars"].shape[-1])
+    return out
+
+
+def _prepare(state, mesh, config, dt, gpu):
+    if state.get(STEP_BOUNDARY_KEY, False):
+        return dict(state)
+    return cold_start(state, mesh, config, dt, gpu)
+
+
+def _finish(stepper: Srk3Stepper, template: Mapping[str, Any], device: torch.device) -> dict[str, Any]:
+    out = dict(template)
+    for k, v in stepper.state_dict().items():
+        out[k] = v if device.type == "cuda" else v.to(device)
+    out[STEP_BOUNDARY_KEY] = True
+    return out
+
+
+def _reorder(config: Mapping[str, Any] | None) -> bool:
+    return bool((config or {}).get("config_mesh_reordering_sfc", True))
+
+
+def run_timestep(state: dict[str, Any], mesh: dict[str, Any], config: dict | None = None, dt: float | None = None,
+                 device: torch.device | str = "cuda", differentiable: bool = False) -> dict[str, Any]:
+    """Advance the state by one large step (reference ``run_timestep``)."""
+    if differentiable:
+        raise MpasUnsupportedCapabilityError(capability="differentiable", requested=True)
+    dt = _dt(config, dt)
+    gpu = require_gpu(None)
+    out_dev = torch.device(device)
+    prepared = _prepare(dict(state), dict(mesh), config, dt, gpu)
+    stepper = Srk3Stepper(prepared, mesh, config, dt, device=gpu, reorder=_reorder(config))
+    stepper.step(1)
+    return _finish(stepper, prepared, out_dev)
+
+
+def run_simulation(initial_state: dict[str, Any], mesh: dict[str, Any], config: dict | None = None,
+                   dt: float | None = None, nsteps: int = 1, device: torch.device | str = "cuda",
+                   on_step: Callable[[int, dict[str, Any]], None] | None = None, differentiable: bool = False,
+                   checkpoint_timesteps: bool | None = None, checkpoint_use_reentrant: bool = False,
+                   checkpoint_materialize_final: bool = True, on_phase: Callable[[str], None] | None = None,
+                   on_initial: Callable[[dict[str, Any]], None] | None = None) -> dict[str, Any]:
+    """Run ``nsteps`` large steps (reference ``run_simulation``)."""
+    if differentiable:
+        raise MpasUnsupportedCapabilityError(capability="differentiable", requested=True)
+    if isinstance(nsteps, bool) or not isinstance(nsteps, int) or nsteps < 0:
+        raise MpasConfigError("nsteps must be a non-negative integer")
+    dt = _dt(config, dt)
+    gpu = require_gpu(None)
+    out_dev = torch.device(device)
+    prepared = _prepare(dict(initial_state), dict(mesh), config, dt, gpu)
+    if on_initial is not None:
+        on_initial(prepared)
+    stepper = Srk3Stepper(prepared, mesh, config, dt, device=gpu, reorder=_reorder(config))
+    if on_phase is not None:
+        on_phase("forecast_integration_begin")
+    for step in range(1, nsteps + 1):
+        stepper.step(1)
+        if on_step is not None:
+            on_step(step, _finish(stepper, prepared, out_dev))
+    if on_phase is not None:
+        torch.cuda.synchronize(gpu)
+        on_phase("forecast_integration_end")
+    if nsteps == 0:
+        out = dict(prepared)
+        out[STEP_BOUNDARY_KEY] = True
+        return out
+    return _finish(stepper, prepared, out_dev)

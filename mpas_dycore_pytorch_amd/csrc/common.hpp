@@ -53,7 +53,21 @@ void fill_zero(double *p, size_t n, hipStream_t s);
 void set_error(const std::string &msg);
 int check_launch(const char *what);
 
+// Per-kernel event timer (measurement hook): when armed for a kernel name,
+// each launch of that kernel is bracketed by HIP events on its stream.
+bool timer_armed_for(const char *kernel_name);
+void timer_before(hipStream_t s);
+void timer_after(hipStream_t s);
+
 }  // namespace mpas
+
+#define MPAS_LAUNCH(kernel, grid, block, shm, stream, ...)                           \
+    do {                                                                             \
+        const bool mpas_timed_ = ::mpas::timer_armed_for(#kernel);                   \
+        if (mpas_timed_) ::mpas::timer_before(stream);                               \
+        hipLaunchKernelGGL(kernel, grid, block, shm, stream, __VA_ARGS__);           \
+        if (mpas_timed_) ::mpas::timer_after(stream);                                \
+    } while (0)
 
 // Internal launchers (one per kernel family).
 namespace mpas {
@@ -123,4 +137,12 @@ size_t mono_scratch_doubles(const mpas_hip_mesh &m);
 
 void launch_reconstruct(const mpas_hip_mesh &m, const double *u, double *ux, double *uy, double *uz,
                         double *uzonal, double *umerid, hipStream_t s);
+
+struct InitCoupledOutputs {
+    double *theta_m, *rho_zz, *ru, *rw, *rho_p, *rtheta_base, *rtheta_p, *exner, *exner_base, *pressure_p,
+        *pressure_base;
+};
+void launch_init_coupled(const mpas_hip_mesh &m, const double *theta, const double *rho, const double *qv,
+                         const double *u, const double *w, const double *theta_base, const InitCoupledOutputs &o,
+                         hipStream_t s);
 }  // namespace mpas

@@ -33,7 +33,7 @@ __global__ void k_smlstep(mpas_hip_mesh m, const double *__restrict__ u_tend, do
 }
 
 void launch_smlstep(const mpas_hip_mesh &m, const double *u_tend, double *w_tend, hipStream_t s) {
-    hipLaunchKernelGGL(k_smlstep, dim3(blocks_for((long long)m.nCells * (m.nLevels + 1))), dim3(kBlock), 0, s, m,
+    MPAS_LAUNCH(k_smlstep, dim3(blocks_for((long long)m.nCells * (m.nLevels + 1))), dim3(kBlock), 0, s, m,
                        u_tend, w_tend);
 }
 
@@ -177,9 +177,9 @@ __global__ __launch_bounds__(64 * kAcWaves) void k_acoustic_cells(mpas_hip_mesh 
 
 void launch_acoustic(const mpas_hip_mesh &m, const AcousticParams &P, const AcousticInputs &in,
                      const AcousticState &io, hipStream_t s) {
-    hipLaunchKernelGGL(k_acoustic_edges, dim3(blocks_for((long long)m.nEdges * m.nLevels)), dim3(kBlock), 0, s, m,
+    MPAS_LAUNCH(k_acoustic_edges, dim3(blocks_for((long long)m.nEdges * m.nLevels)), dim3(kBlock), 0, s, m,
                        P, io.rtheta_pp, io.rho_pp, in.exner, in.cqu, in.tend_ru, io.ru_p, io.ruAvg);
-    hipLaunchKernelGGL(k_acoustic_cells, dim3((m.nCells + kAcWaves - 1) / kAcWaves), dim3(64, kAcWaves), 0, s, m,
+    MPAS_LAUNCH(k_acoustic_cells, dim3((m.nCells + kAcWaves - 1) / kAcWaves), dim3(64, kAcWaves), 0, s, m,
                        P, in, io);
 }
 
@@ -204,7 +204,7 @@ __global__ void k_divdamp(mpas_hip_mesh m, double coef, const double *__restrict
 void launch_divdamp(const mpas_hip_mesh &m, double dts, double smdiv, double len_disp, const double *theta_m,
                     const double *rtheta_pp, const double *rtheta_pp_old, double *ru_p, hipStream_t s) {
     const double coef = 2.0 * smdiv * len_disp / dts;
-    hipLaunchKernelGGL(k_divdamp, dim3(blocks_for((long long)m.nEdges * m.nLevels)), dim3(kBlock), 0, s, m, coef,
+    MPAS_LAUNCH(k_divdamp, dim3(blocks_for((long long)m.nEdges * m.nLevels)), dim3(kBlock), 0, s, m, coef,
                        theta_m, rtheta_pp, rtheta_pp_old, ru_p);
 }
 
@@ -302,11 +302,11 @@ void launch_recover(const mpas_hip_mesh &m, int rk_step, double dt, int ns, cons
                     const RecoverOutputs &out, hipStream_t s) {
     const int L = m.nLevels, nC = m.nCells, nE = m.nEdges;
     const double invNs = 1.0 / (double)ns;
-    hipLaunchKernelGGL(k_recover_cells, dim3(blocks_for((long long)(nC + 1) * (L + 1))), dim3(kBlock), 0, s, m,
+    MPAS_LAUNCH(k_recover_cells, dim3(blocks_for((long long)(nC + 1) * (L + 1))), dim3(kBlock), 0, s, m,
                        rk_step, dt, invNs, in, out);
-    hipLaunchKernelGGL(k_recover_edges, dim3(blocks_for((long long)nE * L)), dim3(kBlock), 0, s, m, invNs,
+    MPAS_LAUNCH(k_recover_edges, dim3(blocks_for((long long)nE * L)), dim3(kBlock), 0, s, m, invNs,
                        in.ru_save, in.ru_p, out.rho_zz, out.ruAvg, out.ru, out.u);
-    hipLaunchKernelGGL(k_recover_w, dim3(blocks_for((long long)nC * (L + 1))), dim3(kBlock), 0, s, m, out.ru,
+    MPAS_LAUNCH(k_recover_w, dim3(blocks_for((long long)nC * (L + 1))), dim3(kBlock), 0, s, m, out.ru,
                        out.rw, out.rho_zz, out.w);
 }
 

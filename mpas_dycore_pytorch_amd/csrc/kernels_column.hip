@@ -55,11 +55,11 @@ void launch_moist(const mpas_hip_mesh &m, const double *scalars, int S, int ms, 
                   double *cqu, double *qtot, hipStream_t s) {
     const int nC = m.nCells, nE = m.nEdges, L = m.nLevels;
     fill_zero(qtot + (size_t)nC * L, L, s);
-    hipLaunchKernelGGL(k_moist_cells, dim3(blocks_for((long long)nC * L)), dim3(kBlock), 0, s, nC, L, S,
+    MPAS_LAUNCH(k_moist_cells, dim3(blocks_for((long long)nC * L)), dim3(kBlock), 0, s, nC, L, S,
                        ms - 1, me, scalars, qtot);
-    hipLaunchKernelGGL(k_moist_cqw, dim3(blocks_for((long long)(nC + 1) * (L + 1))), dim3(kBlock), 0, s,
+    MPAS_LAUNCH(k_moist_cqw, dim3(blocks_for((long long)(nC + 1) * (L + 1))), dim3(kBlock), 0, s,
                        nC, L, qtot, cqw);
-    hipLaunchKernelGGL(k_moist_cqu, dim3(blocks_for((long long)(nE + 1) * L)), dim3(kBlock), 0, s, nE, L,
+    MPAS_LAUNCH(k_moist_cqu, dim3(blocks_for((long long)(nE + 1) * L)), dim3(kBlock), 0, s, nE, L,
                        S, ms - 1, me, m.cellsOnEdge, scalars, cqu);
 }
 
@@ -161,7 +161,7 @@ void launch_vimp(const mpas_hip_mesh &m, double dts, double epssm, const double 
     const double dtseps = 0.5 * dts * (1.0 + epssm);
     const double rcv = MPAS_RGAS / (MPAS_CP - MPAS_RGAS);
     const double c2 = MPAS_CP * rcv;
-    hipLaunchKernelGGL(k_cofrz, dim3(1), dim3(64), 0, s, L, dtseps, m.rdzw, cofrz);
+    MPAS_LAUNCH(k_cofrz, dim3(1), dim3(64), 0, s, L, dtseps, m.rdzw, cofrz);
     // padding rows stay zero
     fill_zero(cofwr + (size_t)nC * L, L, s);
     fill_zero(cofwz + (size_t)nC * L, L, s);
@@ -170,7 +170,7 @@ void launch_vimp(const mpas_hip_mesh &m, double dts, double epssm, const double 
     fill_zero(a_tri + (size_t)nC * L, L, s);
     fill_zero(alpha_tri + (size_t)nC * L, L, s);
     fill_zero(gamma_tri + (size_t)nC * L, L, s);
-    hipLaunchKernelGGL(k_vimp, dim3((nC + kColWaves - 1) / kColWaves), dim3(64, kColWaves), 0, s, m, dtseps,
+    MPAS_LAUNCH(k_vimp, dim3((nC + kColWaves - 1) / kColWaves), dim3(64, kColWaves), 0, s, m, dtseps,
                        rcv, c2, cqw, p, t, rt, qtot, cofwr, cofwz, coftz, cofwt, a_tri, alpha_tri, gamma_tri);
 }
 
@@ -210,7 +210,7 @@ __global__ void k_reconstruct(mpas_hip_mesh m, const double *__restrict__ u, dou
 
 void launch_reconstruct(const mpas_hip_mesh &m, const double *u, double *ux, double *uy, double *uz,
                         double *uzonal, double *umerid, hipStream_t s) {
-    hipLaunchKernelGGL(k_reconstruct, dim3(blocks_for((long long)(m.nCells + 1) * m.nLevels)), dim3(kBlock), 0,
+    MPAS_LAUNCH(k_reconstruct, dim3(blocks_for((long long)(m.nCells + 1) * m.nLevels)), dim3(kBlock), 0,
                        s, m, u, ux, uy, uz, uzonal, umerid);
 }
 

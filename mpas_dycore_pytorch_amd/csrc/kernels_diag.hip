@@ -148,12 +148,12 @@ void launch_diagnostics(const mpas_hip_mesh &m, double dt, double apvm, int rk_s
                    bc = blocks_for((long long)(nC + 1) * L);
     // ke_edge / ke_vertex / pv_vertex / pv_cell have no padding row: the
     // padding-row threads of the pass that fills them only zero the outputs.
-    hipLaunchKernelGGL(k_diag_edges1, dim3(be), dim3(kBlock), 0, s, m, u, h, h_edge, ke_edge);
-    hipLaunchKernelGGL(k_diag_vertices, dim3(bv), dim3(kBlock), 0, s, m, u, ke_edge, vort, ke_vertex, pv_vertex);
-    hipLaunchKernelGGL(k_diag_cells, dim3(bc), dim3(kBlock), 0, s, m, u, ke_edge, ke_vertex, pv_vertex, div, ke,
+    MPAS_LAUNCH(k_diag_edges1, dim3(be), dim3(kBlock), 0, s, m, u, h, h_edge, ke_edge);
+    MPAS_LAUNCH(k_diag_vertices, dim3(bv), dim3(kBlock), 0, s, m, u, ke_edge, vort, ke_vertex, pv_vertex);
+    MPAS_LAUNCH(k_diag_cells, dim3(bc), dim3(kBlock), 0, s, m, u, ke_edge, ke_vertex, pv_vertex, div, ke,
                        pv_cell);
     const int reconstruct_v = (rk_step <= 0 || rk_step == 3) ? 1 : 0;
-    hipLaunchKernelGGL(k_diag_edges2, dim3(be), dim3(kBlock), 0, s, m, apvm * dt, apvm > 0.0 ? 1 : 0,
+    MPAS_LAUNCH(k_diag_edges2, dim3(be), dim3(kBlock), 0, s, m, apvm * dt, apvm > 0.0 ? 1 : 0,
                        reconstruct_v, u, v_in, pv_vertex, pv_cell, v, pv_edge);
 }
 

@@ -397,23 +397,23 @@ void launch_dyn_tend(const mpas_hip_mesh &m, const DynParams &P, const DynInputs
     DynScratch sc = carve(m, scratch);
     const bool rk1 = (P.rk_step == 1);
     if (rk1)
-        hipLaunchKernelGGL(k_dyn_kdiff, dim3(blocks_for((long long)(nC + 1) * L)), dim3(kBlock), 0, s, m, P, in.u,
+        MPAS_LAUNCH(k_dyn_kdiff, dim3(blocks_for((long long)(nC + 1) * L)), dim3(kBlock), 0, s, m, P, in.u,
                            in.v, io.kdiff);
-    hipLaunchKernelGGL(k_dyn_cells_a, dim3(blocks_for((long long)(nC + 1) * (L + 1))), dim3(kBlock), 0, s, m, P,
+    MPAS_LAUNCH(k_dyn_cells_a, dim3(blocks_for((long long)(nC + 1) * (L + 1))), dim3(kBlock), 0, s, m, P,
                        in.ru, in.rw, in.rr_save, in.qtot, in.w, in.theta_m, in.rho_edge, io.kdiff, out.h_divergence,
                        io.tend_rho, sc.dpdz, sc.delsq_w, io.tend_w_euler, sc.delsq_theta, io.tend_theta_euler);
-    hipLaunchKernelGGL(k_dyn_edges_a, dim3(blocks_for((long long)(nE + 1) * L)), dim3(kBlock), 0, s, m, P, in.u,
+    MPAS_LAUNCH(k_dyn_edges_a, dim3(blocks_for((long long)(nE + 1) * L)), dim3(kBlock), 0, s, m, P, in.u,
                        in.rw, in.pv_edge, in.ke, out.h_divergence, in.rho_edge, in.pp, sc.dpdz, in.cqu, in.divergence,
                        in.vorticity, io.kdiff, out.tend_u, io.tend_u_euler, sc.delsq_u);
     const int do_del4 = (rk1 && P.h_mom_eddy_visc4 > 0.0) ? 1 : 0;
     if (do_del4) {
         long long n = (long long)(nC > nV ? nC : nV) * L;
-        hipLaunchKernelGGL(k_dyn_del4_vc, dim3(blocks_for(n)), dim3(kBlock), 0, s, m, sc.delsq_u, sc.delsq_vort,
+        MPAS_LAUNCH(k_dyn_del4_vc, dim3(blocks_for(n)), dim3(kBlock), 0, s, m, sc.delsq_u, sc.delsq_vort,
                            sc.delsq_div);
     }
-    hipLaunchKernelGGL(k_dyn_edges_b, dim3(blocks_for((long long)nE * L)), dim3(kBlock), 0, s, m, P, do_del4,
+    MPAS_LAUNCH(k_dyn_edges_b, dim3(blocks_for((long long)nE * L)), dim3(kBlock), 0, s, m, P, do_del4,
                        in.rho_edge, sc.delsq_div, sc.delsq_vort, io.tend_u_euler, out.tend_u);
-    hipLaunchKernelGGL(k_dyn_cells_b, dim3(blocks_for((long long)(nC + 1) * (L + 1))), dim3(kBlock), 0, s, m, P,
+    MPAS_LAUNCH(k_dyn_cells_b, dim3(blocks_for((long long)(nC + 1) * (L + 1))), dim3(kBlock), 0, s, m, P,
                        in.ru, in.rw, in.ru_save, in.rw_save, in.w, in.theta_m, in.theta_m_save, in.rho_zz, in.pp,
                        in.cqw, in.rt_diabatic_tend, sc.dpdz, sc.delsq_w, sc.delsq_theta, io.tend_rho,
                        io.tend_w_euler, io.tend_theta_euler, out.tend_w, out.tend_theta, out.rthdynten);

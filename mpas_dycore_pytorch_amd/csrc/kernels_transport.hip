@@ -75,9 +75,9 @@ void launch_advance_scalars(const mpas_hip_mesh &m, int S, double dt, double coe
                             const double *rho_zz_new, const double *uhAvg, const double *wwAvg,
                             double *scalar_new_out, double *horiz_flux, hipStream_t s) {
     const int L = m.nLevels;
-    hipLaunchKernelGGL(k_scalar_hflux, dim3(blocks_for((long long)m.nEdges * L)), dim3(kBlock), 0, s, m, S, uhAvg,
+    MPAS_LAUNCH(k_scalar_hflux, dim3(blocks_for((long long)m.nEdges * L)), dim3(kBlock), 0, s, m, S, uhAvg,
                        scalar_new_in, horiz_flux);
-    hipLaunchKernelGGL(k_scalar_update, dim3(blocks_for((long long)m.nCells * L)), dim3(kBlock), 0, s, m, S, dt,
+    MPAS_LAUNCH(k_scalar_update, dim3(blocks_for((long long)m.nCells * L)), dim3(kBlock), 0, s, m, S, dt,
                        coef3, w_old, w_new, scalar_old, scalar_new_in, rho_zz_old, rho_zz_new, uhAvg, wwAvg,
                        horiz_flux, scalar_new_out);
 }
@@ -293,17 +293,17 @@ void launch_advance_scalars_mono(const mpas_hip_mesh &m, int S, double dt, doubl
     MonoScratch w = carve_mono(m, scratch);
     const unsigned bc = blocks_for((long long)nC * L), bcw = blocks_for((long long)nC * (L + 1)),
                    be = blocks_for((long long)nE * L);
-    hipLaunchKernelGGL(k_mono_density, dim3(bc), dim3(kBlock), 0, s, m, dt, uhAvg, wwAvg, rho_zz_old, rho_zz_int);
+    MPAS_LAUNCH(k_mono_density, dim3(bc), dim3(kBlock), 0, s, m, dt, uhAvg, wwAvg, rho_zz_old, rho_zz_int);
     for (int q = 0; q < S; ++q) {
-        hipLaunchKernelGGL(k_mono_old, dim3(bc), dim3(kBlock), 0, s, nC, L, S, q, dt, scalars_old, rho_zz_old, w.so);
-        hipLaunchKernelGGL(k_mono_bounds, dim3(bcw), dim3(kBlock), 0, s, m, S, q, coef3, w.so, scalars_new_in, wwAvg,
+        MPAS_LAUNCH(k_mono_old, dim3(bc), dim3(kBlock), 0, s, nC, L, S, q, dt, scalars_old, rho_zz_old, w.so);
+        MPAS_LAUNCH(k_mono_bounds, dim3(bcw), dim3(kBlock), 0, s, m, S, q, coef3, w.so, scalars_new_in, wwAvg,
                            w.smax, w.smin, w.wdtn);
-        hipLaunchKernelGGL(k_mono_hflux, dim3(be), dim3(kBlock), 0, s, m, S, q, dt, uhAvg, scalars_new_in, w.so,
+        MPAS_LAUNCH(k_mono_hflux, dim3(be), dim3(kBlock), 0, s, m, S, q, dt, uhAvg, scalars_new_in, w.so,
                            w.fup, w.ftmp);
-        hipLaunchKernelGGL(k_mono_scale, dim3(bc), dim3(kBlock), 0, s, m, dt, w.so, rho_zz_old, rho_zz_int, wwAvg,
+        MPAS_LAUNCH(k_mono_scale, dim3(bc), dim3(kBlock), 0, s, m, dt, w.so, rho_zz_old, rho_zz_int, wwAvg,
                            w.wdtn, w.smax, w.smin, w.fup, w.ftmp, w.scell, w.wmid, w.sin, w.sout);
-        hipLaunchKernelGGL(k_mono_hlimit, dim3(be), dim3(kBlock), 0, s, m, w.ftmp, w.sin, w.sout, w.flim);
-        hipLaunchKernelGGL(k_mono_final, dim3(bc), dim3(kBlock), 0, s, m, S, q, w.scell, w.wdtn, w.wmid, w.sin,
+        MPAS_LAUNCH(k_mono_hlimit, dim3(be), dim3(kBlock), 0, s, m, w.ftmp, w.sin, w.sout, w.flim);
+        MPAS_LAUNCH(k_mono_final, dim3(bc), dim3(kBlock), 0, s, m, S, q, w.scell, w.wdtn, w.wmid, w.sin,
                            w.sout, w.flim, rho_zz_int, scalars_new_out);
     }
 }

@@ -3,7 +3,7 @@
 // 3420-4894, stage body execute_rk_step :5000-9154).  Host code only
 // enqueues kernels on the caller's stream: no allocation, no host sync.
 #include <cstring>
-#include <mutex>
+#include <vector>
 
 #include "common.hpp"
 
@@ -12,6 +12,35 @@ namespace mpas {
 static thread_local std::string g_last_error;
 
 void set_error(const std::string &msg) { g_last_error = msg; }
+
+// ---------------------------------------------------------------------------
+// per-kernel event timer (host state; used only by measurement code)
+// ---------------------------------------------------------------------------
+struct KernelTimer {
+    std::string name;
+    std::vector<hipEvent_t> start, stop;
+    int used = 0;
+    bool armed = false;
+};
+static KernelTimer g_timer;
+
+bool timer_armed_for(const char *kernel_name) {
+    return g_timer.armed && g_timer.used < (int)g_timer.start.size() && g_timer.name == kernel_name;
+}
+void timer_before(hipStream_t s) { hipEventRecord(g_timer.start[g_timer.used], s); }
+void timer_after(hipStream_t s) {
+    hipEventRecord(g_timer.stop[g_timer.used], s);
+    ++g_timer.used;
+}
+
+static void timer_release() {
+    for (auto e : g_timer.start) hipEventDestroy(e);
+    for (auto e : g_timer.stop) hipEventDestroy(e);
+    g_timer.start.clear();
+    g_timer.stop.clear();
+    g_timer.used = 0;
+    g_timer.armed = false;
+}
 
 int check_launch(const char *what) {
     hipError_t e = hipGetLastError();
@@ -44,11 +73,11 @@ __global__ void k_scale(double *__restrict__ dst, const double *__restrict__ src
 
 void fill_zero(double *p, size_t n, hipStream_t s) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_fill_zero, dim3(blocks_for((long long)n)), dim3(kBlock), 0, s, p, n);
+    MPAS_LAUNCH(k_fill_zero, dim3(blocks_for((long long)n)), dim3(kBlock), 0, s, p, n);
 }
 
 static void copy_d(double *dst, const double *src, size_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_copy, dim3(blocks_for((long long)n)), dim3(kBlock), 0, s, dst, src, n);
+    MPAS_LAUNCH(k_copy, dim3(blocks_for((long long)n)), dim3(kBlock), 0, s, dst, src, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -160,6 +189,46 @@ extern "C" {
 
 const char *mpas_hip_last_error(void) {
     return g_last_error.c_str();
+}
+
+int mpas_hip_timer_arm(const char *kernel_name, int32_t capacity) {
+    timer_release();
+    if (!kernel_name || capacity < 1) {
+        set_error("timer_arm: need a kernel name and capacity >= 1");
+        return 1;
+    }
+    g_timer.name = kernel_name;
+    g_timer.start.resize(capacity);
+    g_timer.stop.resize(capacity);
+    for (int i = 0; i < capacity; ++i) {
+        if (hipEventCreate(&g_timer.start[i]) != hipSuccess || hipEventCreate(&g_timer.stop[i]) != hipSuccess) {
+            set_error("timer_arm: hipEventCreate failed");
+            return 1;
+        }
+    }
+    g_timer.used = 0;
+    g_timer.armed = true;
+    return 0;
+}
+
+int mpas_hip_timer_read(double *total_ms, int32_t *launches) {
+    double t = 0.0;
+    for (int i = 0; i < g_timer.used; ++i) {
+        if (hipEventSynchronize(g_timer.stop[i]) != hipSuccess) {
+            set_error("timer_read: hipEventSynchronize failed");
+            return 1;
+        }
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, g_timer.start[i], g_timer.stop[i]) != hipSuccess) {
+            set_error("timer_read: hipEventElapsedTime failed");
+            return 1;
+        }
+        t += ms;
+    }
+    *total_ms = t;
+    *launches = g_timer.used;
+    timer_release();
+    return 0;
 }
 
 int mpas_hip_struct_sizes(int32_t *mesh, int32_t *state, int32_t *workspace, int32_t *config) {
@@ -304,6 +373,18 @@ int mpas_hip_advance_scalars_mono(const mpas_hip_mesh *m, int32_t nScalars, doub
     return check_launch("advance_scalars_mono");
 }
 
+int mpas_hip_init_coupled_diagnostics(const mpas_hip_mesh *m, const double *theta, const double *rho,
+                                      const double *qv, const double *u, const double *w, const double *theta_base,
+                                      double *theta_m, double *rho_zz, double *ru, double *rw, double *rho_p,
+                                      double *rtheta_base, double *rtheta_p, double *exner, double *exner_base,
+                                      double *pressure_p, double *pressure_base, void *stream) {
+    if (validate_mesh(m)) return 1;
+    InitCoupledOutputs o{theta_m, rho_zz,     ru,         rw,         rho_p,       rtheta_base,
+                         rtheta_p, exner, exner_base, pressure_p, pressure_base};
+    launch_init_coupled(*m, theta, rho, qv, u, w, theta_base, o, STREAM(stream));
+    return check_launch("init_coupled_diagnostics");
+}
+
 int mpas_hip_reconstruct(const mpas_hip_mesh *m, const double *u, double *ux, double *uy, double *uz, double *uzonal,
                          double *umerid, void *stream) {
     if (validate_mesh(m)) return 1;
@@ -419,8 +500,8 @@ int mpas_hip_srk3_step(const mpas_hip_mesh *mesh, mpas_hip_state *st, const mpas
             copy_d(w.ruAvg_split, w.ruAvg, E, s);
             copy_d(w.wwAvg_split, w.wwAvg, Wd, s);
         } else {
-            hipLaunchKernelGGL(k_add, dim3(blocks_for((long long)E)), dim3(kBlock), 0, s, w.ruAvg_split, w.ruAvg, E);
-            hipLaunchKernelGGL(k_add, dim3(blocks_for((long long)Wd)), dim3(kBlock), 0, s, w.wwAvg_split, w.wwAvg, Wd);
+            MPAS_LAUNCH(k_add, dim3(blocks_for((long long)E)), dim3(kBlock), 0, s, w.ruAvg_split, w.ruAvg, E);
+            MPAS_LAUNCH(k_add, dim3(blocks_for((long long)Wd)), dim3(kBlock), 0, s, w.wwAvg_split, w.wwAvg, Wd);
         }
         if (sub < cfg->split) {
             fill_zero(st->theta_m + (size_t)nC * L, L, s);
@@ -432,9 +513,9 @@ int mpas_hip_srk3_step(const mpas_hip_mesh *mesh, mpas_hip_state *st, const mpas
         }
     }
     const double inv_split = 1.0 / (double)cfg->split;
-    hipLaunchKernelGGL(k_scale, dim3(blocks_for((long long)E)), dim3(kBlock), 0, s, st->ruAvg, w.ruAvg_split,
+    MPAS_LAUNCH(k_scale, dim3(blocks_for((long long)E)), dim3(kBlock), 0, s, st->ruAvg, w.ruAvg_split,
                        inv_split, E);
-    hipLaunchKernelGGL(k_scale, dim3(blocks_for((long long)Wd)), dim3(kBlock), 0, s, st->wwAvg, w.wwAvg_split,
+    MPAS_LAUNCH(k_scale, dim3(blocks_for((long long)Wd)), dim3(kBlock), 0, s, st->wwAvg, w.wwAvg_split,
                        inv_split, Wd);
     // split transport RK3 (time_integration.py:3976-4675); rho_zz_new is the
     // post-dynamics rho_zz already in the state

@@ -137,17 +137,35 @@ class DeviceMesh:
         self._kscratch: torch.Tensor | None = None
 
     def set_base(self, mesh: Mapping, state: Mapping | None) -> None:
-        """(Re)bind the hydrostatic base state, looked up in the mesh then the state."""
+        """(Re)bind the hydrostatic base state, looked up in the mesh then the state.
+
+        ``rtheta_base`` and ``exner_base`` are outputs of the cold start; until
+        it has run they are bound to zeros (``missing_base`` lists them)."""
+        self.missing_base = []
         for name in BASE_FIELDS:
-            src = _get(mesh, state, name)
+            try:
+                src = _get(mesh, state, name)
+            except MpasMissingFieldError:
+                if name == "rho_base":
+                    raise
+                self.missing_base.append(name)
+                if name not in self.t:
+                    self.bind_base(name, torch.zeros((self.nc + 1, self.L), dtype=torch.float64, device=self.device))
+                continue
             if self._base_src.get(name) == id(src) and name in self.t:
                 continue
             t = torch.as_tensor(np.ascontiguousarray(to_numpy(src).astype(np.float64)), device=self.device)
-            if tuple(t.shape) != (self.nc + 1, self.L):
-                raise MpasMeshMismatchError(f"{name} has shape {tuple(t.shape)}, expected {(self.nc + 1, self.L)}")
-            self.t[name] = t
+            self.bind_base(name, t)
             self._base_src[name] = id(src)
-            setattr(self.s, name, t.data_ptr())
+
+    def bind_base(self, name: str, t: torch.Tensor) -> None:
+        """Bind a device tensor as one of the base-state fields."""
+        if tuple(t.shape) != (self.nc + 1, self.L):
+            raise MpasMeshMismatchError(f"{name} has shape {tuple(t.shape)}, expected {(self.nc + 1, self.L)}")
+        t = t.to(device=self.device, dtype=torch.float64).contiguous()
+        self.t[name] = t
+        self._base_src.pop(name, None)
+        setattr(self.s, name, t.data_ptr())
 
     @classmethod
     def get(cls, mesh: Mapping, device: torch.device, state: Mapping | None = None) -> "DeviceMesh":

@@ -133,7 +133,8 @@ def dyn_tend(dm: DeviceMesh, params: N.DynParamsStruct, fields: Mapping, carried
 
 def smlstep_pert(dm: DeviceMesh, u_tend, w_tend) -> torch.Tensor:
     wt = _d(w_tend, dm).clone()
-    N.check(N.load().mpas_hip_smlstep_pert(dm.ref, _p(_d(u_tend, dm)), _p(wt), _s(dm)), "smlstep_pert")
+    ut = _d(u_tend, dm)
+    N.check(N.load().mpas_hip_smlstep_pert(dm.ref, _p(ut), _p(wt), _s(dm)), "smlstep_pert")
     return wt
 
 
@@ -147,9 +148,9 @@ def acoustic_step(dm: DeviceMesh, params: N.AcousticParamsStruct, fields: Mappin
 
 def divergence_damping(dm: DeviceMesh, dts, smdiv, len_disp, theta_m, rtheta_pp, rtheta_pp_old, ru_p):
     out = _d(ru_p, dm).clone()
+    ins = [_d(x, dm) for x in (theta_m, rtheta_pp, rtheta_pp_old)]
     N.check(N.load().mpas_hip_divergence_damping(dm.ref, float(dts), float(smdiv), float(len_disp),
-                                                 _p(_d(theta_m, dm)), _p(_d(rtheta_pp, dm)),
-                                                 _p(_d(rtheta_pp_old, dm)), _p(out), _s(dm)), "divergence_damping")
+                                                 *[_p(x) for x in ins], _p(out), _s(dm)), "divergence_damping")
     return out
 
 
@@ -181,9 +182,9 @@ def advance_scalars(dm: DeviceMesh, dt, coef3, weight_old, weight_new, scalar_ol
     S = so.shape[2]
     out = torch.empty_like(sn)
     hf = _z(dm, dm.ne + 1, dm.L, S)
+    ins = [_d(x, dm) for x in (rho_zz_old, rho_zz_new, uhAvg, wwAvg)]
     N.check(N.load().mpas_hip_advance_scalars(dm.ref, S, float(dt), float(coef3), float(weight_old),
-                                              float(weight_new), _p(so), _p(sn), _p(_d(rho_zz_old, dm)),
-                                              _p(_d(rho_zz_new, dm)), _p(_d(uhAvg, dm)), _p(_d(wwAvg, dm)),
+                                              float(weight_new), _p(so), _p(sn), *[_p(x) for x in ins],
                                               _p(out), _p(hf), _s(dm)), "advance_scalars")
     # the kernel writes the live rows; the padding row carries the input
     out[dm.nc] = sn[dm.nc]
@@ -195,16 +196,35 @@ def advance_scalars_mono(dm: DeviceMesh, dt, coef3, scalars_old, scalars_new, rh
     S = so.shape[2]
     out = sn.clone()
     rzi = _z(dm, dm.nc + 1, dm.L)
+    ins = [_d(x, dm) for x in (rho_zz_old, uhAvg, wwAvg)]
     N.check(N.load().mpas_hip_advance_scalars_mono(dm.ref, S, float(dt), float(coef3), _p(so), _p(sn),
-                                                   _p(_d(rho_zz_old, dm)), _p(_d(uhAvg, dm)), _p(_d(wwAvg, dm)),
+                                                   *[_p(x) for x in ins],
                                                    _p(out), _p(rzi), _p(dm.kernel_scratch()), _s(dm)),
             "advance_scalars_mono")
     return out, rzi
 
 
+ICD_OUT = ("theta_m", "rho_zz", "ru", "rw", "rho_p", "rtheta_base", "rtheta_p", "exner", "exner_base",
+           "pressure_p", "pressure_base")
+
+
+def init_coupled_diagnostics(dm: DeviceMesh, theta, rho, qv, u, w, theta_base) -> dict:
+    """Cold-start coupling (``atm_init_coupled_diagnostics``) of raw padded fields."""
+    ins = [_d(x, dm) for x in (theta, rho, qv, u, w, theta_base)]
+    nc, ne, L = dm.nc, dm.ne, dm.L
+    out = {k: _z(dm, nc + 1, L) for k in ICD_OUT}
+    out["ru"] = _z(dm, ne + 1, L)
+    out["rw"] = _z(dm, nc + 1, L + 1)
+    N.check(N.load().mpas_hip_init_coupled_diagnostics(dm.ref, *[_p(x) for x in ins],
+                                                       *[_p(out[k]) for k in ICD_OUT], _s(dm)),
+            "init_coupled_diagnostics")
+    return out
+
+
 def reconstruct(dm: DeviceMesh, u) -> dict:
     out = {k: _z(dm, dm.nc + 1, dm.L) for k in RECON_OUT}
-    N.check(N.load().mpas_hip_reconstruct(dm.ref, _p(_d(u, dm)), *[_p(out[k]) for k in RECON_OUT], _s(dm)),
+    ud = _d(u, dm)
+    N.check(N.load().mpas_hip_reconstruct(dm.ref, _p(ud), *[_p(out[k]) for k in RECON_OUT], _s(dm)),
             "reconstruct")
     out["ur_cell"] = out["uReconstructZonal"].clone()
     out["vr_cell"] = out["uReconstructMeridional"].clone()

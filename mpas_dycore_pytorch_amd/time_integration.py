@@ -143,17 +143,56 @@ def _device_tensor(v: Any, device: torch.device, shape: tuple[int, ...], name: s
     return t.contiguous()
 
 
+_REORDER_CACHE: dict[int, tuple[Any, Any, dict]] = {}
+
+
+def _reordered_mesh(mesh: Mapping[str, Any]):
+    """(permutation, permuted mesh) of a mesh mapping, cached per mapping."""
+    from .reorder import compute_sfc_permutation, permute_mapping
+
+    hit = _REORDER_CACHE.get(id(mesh))
+    if hit is not None and hit[0] is mesh:
+        return hit[1], hit[2]
+    perm = compute_sfc_permutation(mesh)
+    dims = tuple(int(to_numpy(mesh[k])) for k in ("nCells", "nEdges", "nVertices"))
+    pm = permute_mapping(mesh, perm, dims)
+    if len(_REORDER_CACHE) > 4:
+        _REORDER_CACHE.clear()
+    _REORDER_CACHE[id(mesh)] = (mesh, perm, pm)
+    return perm, pm
+
+
 class Srk3Stepper:
-    """Device-resident state advanced in place by successive large steps."""
+    """Device-resident state advanced in place by successive large steps.
+
+    With ``reorder=True`` the mesh and state are held on the device in
+    space-filling-curve order (``reorder.py``); ``state_dict`` maps the
+    results back to the caller's order, bit for bit the same values.
+    """
 
     def __init__(self, state: Mapping[str, Any], mesh: Mapping[str, Any], config: Mapping[str, Any] | None,
-                 dt: float, device: torch.device | str | None = None, strict: bool = True):
+                 dt: float, device: torch.device | str | None = None, strict: bool = True,
+                 reorder: bool = False):
         self.device = require_gpu(device)
         if "scalars" not in state:
             raise MpasMissingFieldError(fields=["scalars"], source="state")
         sc = state["scalars"]
         n_scalars = int(sc.shape[2]) if len(sc.shape) == 3 else 0
         self.n_scalars = n_scalars
+        self._inv_rows: dict[str, torch.Tensor] | None = None
+        if reorder:
+            from .reorder import permute_mapping
+
+            perm, mesh_p = _reordered_mesh(mesh)
+            dims = tuple(int(to_numpy(mesh[k])) for k in ("nCells", "nEdges", "nVertices"))
+            state = permute_mapping(state, perm, dims)
+            mesh = mesh_p
+            nc, ne, nv = dims
+            self._inv_rows = {
+                "cell": torch.as_tensor(np.concatenate([perm.cell_old_to_new, [nc]]), device=self.device),
+                "edge": torch.as_tensor(np.concatenate([perm.edge_old_to_new, [ne]]), device=self.device),
+                "vertex": torch.as_tensor(np.concatenate([perm.vertex_old_to_new, [nv]]), device=self.device),
+            }
         self.mesh = DeviceMesh.get(mesh, self.device, state)
         self.cfg = resolve_config(config, state, mesh, dt, n_scalars)
         dm = self.mesh
@@ -189,8 +228,14 @@ class Srk3Stepper:
                 self._diagnostics_current = True
 
     def state_dict(self) -> dict[str, torch.Tensor]:
-        out = dict(self.t)
-        out["h_edge"] = self.t["rho_edge"]
+        """Device tensors in the caller's row order (views when not reordered)."""
+        if self._inv_rows is None:
+            out = dict(self.t)
+        else:
+            dm = self.mesh
+            kind = {dm.nc + 1: "cell", dm.ne + 1: "edge", dm.nv + 1: "vertex"}
+            out = {k: v.index_select(0, self._inv_rows[kind[v.shape[0]]]) for k, v in self.t.items()}
+        out["h_edge"] = out["rho_edge"]
         return out
 
 
