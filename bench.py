@@ -114,7 +114,7 @@ def main() -> int:
     ap.add_argument("--cpu-level", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--reorder", type=int, default=0, help="1: space-filling-curve mesh order on the device")
+    ap.add_argument("--reorder", type=int, default=1, help="1: space-filling-curve mesh order on the device")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

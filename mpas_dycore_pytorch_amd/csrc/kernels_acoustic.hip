@@ -76,6 +76,7 @@ constexpr int kAcWaves = 4;
 __global__ __launch_bounds__(64 * kAcWaves) void k_acoustic_cells(mpas_hip_mesh m, AcousticParams P,
                                                                    AcousticInputs in, AcousticState io) {
     __shared__ double s_rs[kAcWaves][64], s_ts[kAcWaves][64], s_rw[kAcWaves][65];
+    __shared__ double s_a[kAcWaves][64], s_al[kAcWaves][64], s_g[kAcWaves][64];
     const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
     const int wv = threadIdx.y, k = threadIdx.x;
     const int c = blockIdx.x * kAcWaves + wv;
@@ -98,7 +99,13 @@ __global__ __launch_bounds__(64 * kAcWaves) void k_acoustic_cells(mpas_hip_mesh 
             }
             if (k < L) rw_k1 = io.rw_p[ow + k + 1];
         }
-        if (k < L) io.rtheta_pp_old[oc + k] = first ? 0.0 : rt_old;
+        if (k < L) {
+            io.rtheta_pp_old[oc + k] = first ? 0.0 : rt_old;
+            // tridiagonal factors of this column, staged for the serial sweeps
+            s_a[wv][k] = in.a_tri[oc + k];
+            s_al[wv][k] = in.alpha_tri[oc + k];
+            s_g[wv][k] = in.gamma_tri[oc + k];
+        }
     }
     // phase A: horizontal flux sums, rs and ts (:456-511, :700-716)
     if (live && k < L) {
@@ -143,9 +150,19 @@ __global__ __launch_bounds__(64 * kAcWaves) void k_acoustic_cells(mpas_hip_mesh 
     // phase C: Thomas sweeps, lane 0 of each column
     if (live && k == 0) {
         double *x = s_rw[wv];
-        const double *a = in.a_tri + oc, *al = in.alpha_tri + oc, *g = in.gamma_tri + oc;
-        for (int kk = 1; kk < L; ++kk) x[kk] = (x[kk] - a[kk] * x[kk - 1]) * al[kk];
-        for (int kk = L - 1; kk >= 0; --kk) x[kk] = x[kk] - g[kk] * x[kk + 1];
+        const double *a = s_a[wv], *al = s_al[wv], *g = s_g[wv];
+        double xp = x[0];
+#pragma unroll 8
+        for (int kk = 1; kk < L; ++kk) {
+            xp = (x[kk] - a[kk] * xp) * al[kk];
+            x[kk] = xp;
+        }
+        double xn = x[L];
+#pragma unroll 8
+        for (int kk = L - 1; kk >= 0; --kk) {
+            xn = x[kk] - g[kk] * xn;
+            x[kk] = xn;
+        }
     }
     __syncthreads();
     // phase D: implicit Rayleigh damping of w, wwAvg (:838-870)
@@ -269,9 +286,13 @@ __global__ void k_recover_w(mpas_hip_mesh m, const double *__restrict__ ru, cons
     int c = i / Lw, k = i - c * Lw;
     const int oc = c * L;
     const int n = m.nEdgesOnCell[c];
+    // Flat terrain: zb_cell = zb3_cell = 0, every terrain term is a signed
+    // zero and adding it leaves the value unchanged (up to the sign of a zero
+    // result), so the edge loops are skipped.
+    const int nt = m.flatTerrain ? 0 : n;
     if (k == 0) {
         double sacc = 0.0;
-        for (int j = 0; j < n; ++j) {
+        for (int j = 0; j < nt; ++j) {
             int e = m.edgesOnCell[c * ME + j];
             const double *re = ru + e * L;
             double fs = m.cf1 * re[0] + m.cf2 * re[1] + m.cf3 * re[2];
@@ -284,7 +305,7 @@ __global__ void k_recover_w(mpas_hip_mesh m, const double *__restrict__ ru, cons
     } else if (k < L) {
         double zi = m.fzm[k] * m.zz[oc + k] + m.fzp[k] * m.zz[oc + k - 1];
         double acc = rw[i] / zi;
-        for (int j = 0; j < n; ++j) {
+        for (int j = 0; j < nt; ++j) {
             int e = m.edgesOnCell[c * ME + j];
             const double *re = ru + e * L;
             double fi = m.fzm[k] * re[k] + m.fzp[k] * re[k - 1];

@@ -18,12 +18,13 @@ namespace mpas {
 
 size_t dyn_tend_scratch_doubles(const mpas_hip_mesh &m) {
     const size_t L = m.nLevels, C = m.nCells + 1, E = m.nEdges + 1, V = m.nVertices + 1;
-    // dpdz, delsq_u, delsq_div, delsq_vort, delsq_w, delsq_theta
-    return C * L + E * L + C * L + V * L + C * (L + 1) + C * L;
+    // dpdz, delsq_u, delsq_div, delsq_vort, delsq_w, delsq_theta, edge fluxes (w, theta, theta pert.)
+    return C * L + E * L + C * L + V * L + C * (L + 1) + C * L + 3 * E * L;
 }
 
 struct DynScratch {
     double *dpdz, *delsq_u, *delsq_div, *delsq_vort, *delsq_w, *delsq_theta;
+    double *wflux, *tflux, *tpert;
 };
 
 static DynScratch carve(const mpas_hip_mesh &m, double *base) {
@@ -35,6 +36,9 @@ static DynScratch carve(const mpas_hip_mesh &m, double *base) {
     s.delsq_vort = s.delsq_div + C * L;
     s.delsq_w = s.delsq_vort + V * L;
     s.delsq_theta = s.delsq_w + C * (L + 1);
+    s.wflux = s.delsq_theta + C * L;
+    s.tflux = s.wflux + E * L;
+    s.tpert = s.tflux + E * L;
     return s;
 }
 
@@ -251,11 +255,55 @@ __global__ void k_dyn_edges_b(mpas_hip_mesh m, DynParams P, int do_del4, const d
     tend_u[i] = tend_u[i] + eu;
 }
 
+// P6b: horizontal advective fluxes of w (at interfaces) and theta_m (at
+// levels) through each edge, once per edge: flux = ru_e * sum_a (c_a + s c3_a)
+// q_a over advCellsForEdge in slot order (dyn_tend.py:3627-3747, :4174-4380);
+// for rk_step > 1 also the theta perturbation flux
+// dv (ru_save - ru) 0.5 (theta_m_save1 + theta_m_save2) (:5901-6014).  The
+// cell pass applies the edge sign, an exact sign flip of these products.
+__global__ void k_dyn_edge_adv(mpas_hip_mesh m, DynParams P, const double *__restrict__ ru,
+                               const double *__restrict__ ru_save, const double *__restrict__ w,
+                               const double *__restrict__ theta_m, const double *__restrict__ theta_m_save,
+                               double *__restrict__ wflux, double *__restrict__ tflux, double *__restrict__ tpert) {
+    const int L = m.nLevels, nE = m.nEdges, MA = m.maxAdv, Lw = L + 1;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nE * L) return;
+    const int e = i / L, k = i - e * L;
+    const double *rue_col = ru + e * L;
+    const double rut = rue_col[k];
+    const double srt = sign_pm(rut);
+    const bool wl = (k >= 1);
+    const double ruw = wl ? m.fzm[k] * rue_col[k] + m.fzp[k] * rue_col[k - 1] : 0.0;
+    const double srw = sign_pm(ruw);
+    const int na = m.nAdvCellsForEdge[e];
+    const int *adv = m.advCellsForEdge + e * MA;
+    const double *c0 = m.adv_coefs + e * MA, *c3 = m.adv_coefs_3rd + e * MA;
+    double ft = 0.0, fw = 0.0;
+    for (int a = 0; a < na; ++a) {
+        const int ac = adv[a];
+        const double a0 = c0[a], a3 = c3[a];
+        const double ct = (a0 + srt * a3) * theta_m[ac * L + k];
+        ft = (a == 0) ? ct : ft + ct;
+        if (wl) {
+            const double cw = (a0 + srw * a3) * w[ac * Lw + k];
+            fw = (a == 0) ? cw : fw + cw;
+        }
+    }
+    tflux[i] = rut * ft;
+    wflux[i] = ruw * fw;
+    if (P.rk_step != 1) {
+        const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+        const double face = 0.5 * (theta_m_save[c2 * L + k] + theta_m_save[c1 * L + k]);
+        tpert[i] = m.dvEdge[e] * (ru_save[i] - rut) * face;
+    }
+}
+
 // P7: w and theta tendencies (advection, mixing finish, PGF/buoyancy,
 // vertical transport, diabatic forcing), one thread per (cell, w-level).
-__global__ void k_dyn_cells_b(mpas_hip_mesh m, DynParams P, const double *__restrict__ ru,
-                              const double *__restrict__ rw, const double *__restrict__ ru_save,
-                              const double *__restrict__ rw_save, const double *__restrict__ w,
+__global__ void k_dyn_cells_b(mpas_hip_mesh m, DynParams P, const double *__restrict__ wflux,
+                              const double *__restrict__ rw, const double *__restrict__ tflux,
+                              const double *__restrict__ rw_save, const double *__restrict__ tpert,
+                              const double *__restrict__ w,
                               const double *__restrict__ theta_m, const double *__restrict__ theta_m_save,
                               const double *__restrict__ rho_zz, const double *__restrict__ pp,
                               const double *__restrict__ cqw, const double *__restrict__ rt_diab,
@@ -264,7 +312,7 @@ __global__ void k_dyn_cells_b(mpas_hip_mesh m, DynParams P, const double *__rest
                               double *__restrict__ tend_w_euler, double *__restrict__ tend_theta_euler,
                               double *__restrict__ tend_w, double *__restrict__ tend_theta,
                               double *__restrict__ rthdynten) {
-    const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, MA = m.maxAdv, Lw = L + 1;
+    const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (nC + 1) * Lw) return;
     int c = i / Lw, k = i - c * Lw;
@@ -285,18 +333,7 @@ __global__ void k_dyn_cells_b(mpas_hip_mesh m, DynParams P, const double *__rest
         double hadv = 0.0;
         for (int j = 0; j < n; ++j) {
             int e = m.edgesOnCell[c * ME + j];
-            double sgn = m.edgesOnCell_sign[c * ME + j];
-            double rue = fzm[k] * ru[e * L + k] + fzp[k] * ru[e * L + k - 1];
-            double sr = sign_pm(rue);
-            const int na = m.nAdvCellsForEdge[e];
-            double fs = 0.0;
-            for (int a = 0; a < na; ++a) {
-                int ac = m.advCellsForEdge[e * MA + a];
-                double wgt = m.adv_coefs[e * MA + a] + sr * m.adv_coefs_3rd[e * MA + a];
-                double cc = wgt * w[ac * Lw + k];
-                fs = (a == 0) ? cc : fs + cc;
-            }
-            double cc = -sgn * (rue * fs);
+            double cc = -m.edgesOnCell_sign[c * ME + j] * wflux[e * L + k];
             hadv = (j == 0) ? cc : hadv + cc;
         }
         double eu = tend_w_euler[i];
@@ -333,28 +370,13 @@ __global__ void k_dyn_cells_b(mpas_hip_mesh m, DynParams P, const double *__rest
         double thadv = 0.0;
         for (int j = 0; j < n; ++j) {
             int e = m.edgesOnCell[c * ME + j];
-            double sgn = m.edgesOnCell_sign[c * ME + j];
-            double rue = ru[e * L + k];
-            double sr = sign_pm(rue);
-            const int na = m.nAdvCellsForEdge[e];
-            double fs = 0.0;
-            for (int a = 0; a < na; ++a) {
-                int ac = m.advCellsForEdge[e * MA + a];
-                double wgt = m.adv_coefs[e * MA + a] + sr * m.adv_coefs_3rd[e * MA + a];
-                double cc = wgt * theta_m[ac * L + k];
-                fs = (a == 0) ? cc : fs + cc;
-            }
-            double cc = -sgn * (rue * fs);
+            double cc = -m.edgesOnCell_sign[c * ME + j] * tflux[e * L + k];
             thadv = (j == 0) ? cc : thadv + cc;
         }
         if (!rk1) {
             for (int j = 0; j < n; ++j) {
                 int e = m.edgesOnCell[c * ME + j];
-                int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-                double sdv = m.edgesOnCell_sign[c * ME + j] * m.dvEdge[e];
-                double face = 0.5 * (theta_m_save[c2 * L + k] + theta_m_save[c1 * L + k]);
-                double cc = sdv * (ru_save[e * L + k] - ru[e * L + k]) * face;
-                thadv = thadv - cc;
+                thadv = thadv - m.edgesOnCell_sign[c * ME + j] * tpert[e * L + k];
             }
         }
         double tte = tend_theta_euler[o];
@@ -413,8 +435,10 @@ void launch_dyn_tend(const mpas_hip_mesh &m, const DynParams &P, const DynInputs
     }
     MPAS_LAUNCH(k_dyn_edges_b, dim3(blocks_for((long long)nE * L)), dim3(kBlock), 0, s, m, P, do_del4,
                        in.rho_edge, sc.delsq_div, sc.delsq_vort, io.tend_u_euler, out.tend_u);
+    MPAS_LAUNCH(k_dyn_edge_adv, dim3(blocks_for((long long)nE * L)), dim3(kBlock), 0, s, m, P, in.ru, in.ru_save,
+                in.w, in.theta_m, in.theta_m_save, sc.wflux, sc.tflux, sc.tpert);
     MPAS_LAUNCH(k_dyn_cells_b, dim3(blocks_for((long long)(nC + 1) * (L + 1))), dim3(kBlock), 0, s, m, P,
-                       in.ru, in.rw, in.ru_save, in.rw_save, in.w, in.theta_m, in.theta_m_save, in.rho_zz, in.pp,
+                       sc.wflux, in.rw, sc.tflux, in.rw_save, sc.tpert, in.w, in.theta_m, in.theta_m_save, in.rho_zz, in.pp,
                        in.cqw, in.rt_diabatic_tend, sc.dpdz, sc.delsq_w, sc.delsq_theta, io.tend_rho,
                        io.tend_w_euler, io.tend_theta_euler, out.tend_w, out.tend_theta, out.rthdynten);
 }
