@@ -41,9 +41,62 @@ __device__ __forceinline__ double flux3(double qm2, double qm1, double q0, doubl
 
 constexpr int kBlock = 256;
 
+// Compile-time caps on the slot counts, so slot loops unroll fully: every
+// load of a loop is issued before the first accumulation (the sums still run
+// in slot order).  Slots past the live count read a clamped, in-bounds slot
+// and are not accumulated.  Meshes beyond the caps are rejected on entry.
+constexpr int kMaxEdges = 8;    // edgesOnCell / cellsOnCell / verticesOnCell
+constexpr int kMaxEdges2 = 16;  // edgesOnEdge
+constexpr int kMaxAdv = 16;     // advCellsForEdge
+__device__ __forceinline__ int clamp_slot(int j, int n) { return j < n ? j : n - 1; }
+
+// Slot rows across a wavefront: lane j loads slot clamp(j, n) of a
+// connectivity or coefficient row with one vector load, and each slot is then
+// broadcast to the whole wavefront with readlane (slot index uniform).  This
+// puts every gather address of a slot loop in registers before the first
+// gather is issued.  All 64 lanes must execute the row load.
+__device__ __forceinline__ int bcast(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
+__device__ __forceinline__ double bcast(double v, int j) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
+    return __hiloint2double(hi, lo);
+}
+
 inline unsigned blocks_for(long long n, int block = kBlock) {
     return (unsigned)((n + block - 1) / block);
 }
+
+// XCD-aware block order.  The dispatcher hands consecutive workgroups to the
+// 8 XCDs round-robin; remapping block b to (b % 8) * per + b / 8 gives each
+// XCD (and its L2) one contiguous range of the space-filling-curve ordered
+// entities.  The grid is padded to a multiple of 8; blocks mapped past the
+// end exit.
+constexpr int kXcds = 8;
+inline unsigned xcd_grid(long long nblocks) { return (unsigned)(((nblocks + kXcds - 1) / kXcds) * kXcds); }
+__device__ __forceinline__ long long xcd_block(long long nblocks) {
+    const long long per = (nblocks + kXcds - 1) / kXcds;
+    const long long b = blockIdx.x;
+    return (b % kXcds) * per + b / kXcds;
+}
+
+// Wavefront-per-entity mapping: a 256-thread block holds 4 wavefronts, each
+// wavefront one entity (cell, edge or vertex) with lane = vertical level.
+// The entity index is wave-uniform, so connectivity and mesh metrics are read
+// with scalar loads and every neighbour gather is one contiguous column.
+constexpr int kWaves = 4;
+inline long long wave_blocks(long long n_entities) { return (n_entities + kWaves - 1) / kWaves; }
+__device__ __forceinline__ int wave_entity(long long nblocks) {
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    return (int)(xcd_block(nblocks) * kWaves) + w;
+}
+__device__ __forceinline__ int wave_lane() { return (int)(threadIdx.x & 63); }
+
+#define MPAS_WAVE_LAUNCH(kernel, n_entities, stream, ...)                                          \
+    do {                                                                                           \
+        const long long nb_ = ::mpas::wave_blocks(n_entities);                                     \
+        MPAS_LAUNCH(kernel, dim3(::mpas::xcd_grid(nb_)), dim3(64 * ::mpas::kWaves), 0, stream,     \
+                    nb_, __VA_ARGS__);                                                             \
+    } while (0)
 
 // Zero-fill by kernel rather than hipMemsetAsync so launch sequences stay
 // safe to capture in a HIP graph.

@@ -41,13 +41,14 @@ void launch_smlstep(const mpas_hip_mesh &m, const double *u_tend, double *w_tend
 // atm_advance_acoustic_step_work (acoustic_step.py:456-889, 1297-1951)
 // Edge pass: pressure-gradient update of ru_p and ruAvg (:576-649).
 // ---------------------------------------------------------------------------
-__global__ void k_acoustic_edges(mpas_hip_mesh m, AcousticParams P, const double *__restrict__ rtheta_pp,
+__global__ __launch_bounds__(256) void k_acoustic_edges(long long nb, mpas_hip_mesh m, AcousticParams P, const double *__restrict__ rtheta_pp,
                                  const double *__restrict__ rho_pp, const double *__restrict__ exner,
                                  const double *__restrict__ cqu, const double *__restrict__ tend_ru,
                                  double *__restrict__ ru_p, double *__restrict__ ruAvg) {
     const int L = m.nLevels, nE = m.nEdges;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nE * L) return;
+    const int e = wave_entity(nb), k = wave_lane();
+    if (e >= nE || k >= L) return;
+    const int i = e * L + k;
     const double dts = P.dts;
     if (P.small_step == 1) {
         double r = dts * tend_ru[i];
@@ -55,7 +56,6 @@ __global__ void k_acoustic_edges(mpas_hip_mesh m, AcousticParams P, const double
         ruAvg[i] = r;
         return;
     }
-    int e = i / L, k = i - e * L;
     int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
     double pg = ((rtheta_pp[c2 * L + k] - rtheta_pp[c1 * L + k]) * m.invDcEdge[e]) /
                 (0.5 * (m.zz[c2 * L + k] + m.zz[c1 * L + k]));
@@ -111,15 +111,19 @@ __global__ __launch_bounds__(64 * kAcWaves) void k_acoustic_cells(mpas_hip_mesh 
     if (live && k < L) {
         const int n = m.nEdgesOnCell[c];
         double rsum = 0.0, tsum = 0.0;
-        for (int j = 0; j < n; ++j) {
-            int e = m.edgesOnCell[c * ME + j];
-            int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-            double metric = m.edgesOnCell_sign[c * ME + j] * m.dvEdge[e];
-            double flux = dts * metric * io.ru_p[e * L + k] * m.invAreaCell[c];
-            double thedge = 0.5 * (in.theta_m[c2 * L + k] + in.theta_m[c1 * L + k]);
-            double rcon = -flux, tcon = -(flux * thedge);
-            rsum = (j == 0) ? rcon : rsum + rcon;
-            tsum = (j == 0) ? tcon : tsum + tcon;
+#pragma unroll
+        for (int j = 0; j < kMaxEdges; ++j) {
+            const int jj = clamp_slot(j, n);
+            const int e = m.edgesOnCell[c * ME + jj];
+            const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+            const double rp = io.ru_p[e * L + k];
+            const double t2 = in.theta_m[c2 * L + k], t1 = in.theta_m[c1 * L + k];
+            const double metric = m.edgesOnCell_sign[c * ME + jj] * m.dvEdge[e];
+            const double flux = dts * metric * rp * m.invAreaCell[c];
+            const double thedge = 0.5 * (t2 + t1);
+            const double rcon = -flux, tcon = -(flux * thedge);
+            rsum = (j < n) ? ((j == 0) ? rcon : rsum + rcon) : rsum;
+            tsum = (j < n) ? ((j == 0) ? tcon : tsum + tcon) : tsum;
         }
         double rs = rho_old + dts * in.tend_rho[oc + k] + rsum - in.cofrz[k] * resm * (rw_k1 - rw_k);
         double ts = rt_old + dts * in.tend_rt[oc + k] + tsum -
@@ -194,7 +198,7 @@ __global__ __launch_bounds__(64 * kAcWaves) void k_acoustic_cells(mpas_hip_mesh 
 
 void launch_acoustic(const mpas_hip_mesh &m, const AcousticParams &P, const AcousticInputs &in,
                      const AcousticState &io, hipStream_t s) {
-    MPAS_LAUNCH(k_acoustic_edges, dim3(blocks_for((long long)m.nEdges * m.nLevels)), dim3(kBlock), 0, s, m,
+    MPAS_WAVE_LAUNCH(k_acoustic_edges, m.nEdges, s, m,
                        P, io.rtheta_pp, io.rho_pp, in.exner, in.cqu, in.tend_ru, io.ru_p, io.ruAvg);
     MPAS_LAUNCH(k_acoustic_cells, dim3((m.nCells + kAcWaves - 1) / kAcWaves), dim3(64, kAcWaves), 0, s, m,
                        P, in, io);
@@ -203,13 +207,13 @@ void launch_acoustic(const mpas_hip_mesh &m, const AcousticParams &P, const Acou
 // ---------------------------------------------------------------------------
 // atm_divergence_damping_3d (divergence_damping.py:187-209, coefficient :267)
 // ---------------------------------------------------------------------------
-__global__ void k_divdamp(mpas_hip_mesh m, double coef, const double *__restrict__ theta_m,
+__global__ __launch_bounds__(256) void k_divdamp(long long nb, mpas_hip_mesh m, double coef, const double *__restrict__ theta_m,
                           const double *__restrict__ rtheta_pp, const double *__restrict__ rtheta_pp_old,
                           double *__restrict__ ru_p) {
     const int L = m.nLevels, nE = m.nEdges;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nE * L) return;
-    int e = i / L, k = i - e * L;
+    const int e = wave_entity(nb), k = wave_lane();
+    if (e >= nE || k >= L) return;
+    const int i = e * L + k;
     int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
     double d1 = -(rtheta_pp[c1 * L + k] - rtheta_pp_old[c1 * L + k]);
     double d2 = -(rtheta_pp[c2 * L + k] - rtheta_pp_old[c2 * L + k]);
@@ -221,7 +225,7 @@ __global__ void k_divdamp(mpas_hip_mesh m, double coef, const double *__restrict
 void launch_divdamp(const mpas_hip_mesh &m, double dts, double smdiv, double len_disp, const double *theta_m,
                     const double *rtheta_pp, const double *rtheta_pp_old, double *ru_p, hipStream_t s) {
     const double coef = 2.0 * smdiv * len_disp / dts;
-    MPAS_LAUNCH(k_divdamp, dim3(blocks_for((long long)m.nEdges * m.nLevels)), dim3(kBlock), 0, s, m, coef,
+    MPAS_WAVE_LAUNCH(k_divdamp, m.nEdges, s, m, coef,
                        theta_m, rtheta_pp, rtheta_pp_old, ru_p);
 }
 
@@ -264,13 +268,13 @@ __global__ void k_recover_cells(mpas_hip_mesh m, int rk_step, double dt, double 
     }
 }
 
-__global__ void k_recover_edges(mpas_hip_mesh m, double invNs, const double *__restrict__ ru_save,
+__global__ __launch_bounds__(256) void k_recover_edges(long long nb, mpas_hip_mesh m, double invNs, const double *__restrict__ ru_save,
                                 const double *__restrict__ ru_p, const double *__restrict__ rho_zz,
                                 double *__restrict__ ruAvg, double *__restrict__ ru, double *__restrict__ u) {
     const int L = m.nLevels, nE = m.nEdges;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nE * L) return;
-    int e = i / L, k = i - e * L;
+    const int e = wave_entity(nb), k = wave_lane();
+    if (e >= nE || k >= L) return;
+    const int i = e * L + k;
     int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
     ruAvg[i] = ru_save[i] + (ruAvg[i] * invNs);
     double rn = ru_save[i] + ru_p[i];
@@ -278,12 +282,12 @@ __global__ void k_recover_edges(mpas_hip_mesh m, double invNs, const double *__r
     u[i] = 2.0 * rn / (rho_zz[c1 * L + k] + rho_zz[c2 * L + k]);
 }
 
-__global__ void k_recover_w(mpas_hip_mesh m, const double *__restrict__ ru, const double *__restrict__ rw,
+__global__ __launch_bounds__(256) void k_recover_w(long long nb, mpas_hip_mesh m, const double *__restrict__ ru, const double *__restrict__ rw,
                             const double *__restrict__ rho_zz, double *__restrict__ w) {
     const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nC * Lw) return;
-    int c = i / Lw, k = i - c * Lw;
+    const int c = wave_entity(nb), k = wave_lane();
+    if (c >= nC || k >= Lw) return;
+    const int i = c * Lw + k;
     const int oc = c * L;
     const int n = m.nEdgesOnCell[c];
     // Flat terrain: zb_cell = zb3_cell = 0, every terrain term is a signed
@@ -325,9 +329,9 @@ void launch_recover(const mpas_hip_mesh &m, int rk_step, double dt, int ns, cons
     const double invNs = 1.0 / (double)ns;
     MPAS_LAUNCH(k_recover_cells, dim3(blocks_for((long long)(nC + 1) * (L + 1))), dim3(kBlock), 0, s, m,
                        rk_step, dt, invNs, in, out);
-    MPAS_LAUNCH(k_recover_edges, dim3(blocks_for((long long)nE * L)), dim3(kBlock), 0, s, m, invNs,
+    MPAS_WAVE_LAUNCH(k_recover_edges, nE, s, m, invNs,
                        in.ru_save, in.ru_p, out.rho_zz, out.ruAvg, out.ru, out.u);
-    MPAS_LAUNCH(k_recover_w, dim3(blocks_for((long long)nC * (L + 1))), dim3(kBlock), 0, s, m, out.ru,
+    MPAS_WAVE_LAUNCH(k_recover_w, nC, s, m, out.ru,
                        out.rw, out.rho_zz, out.w);
 }
 

@@ -12,12 +12,12 @@ size_t diagnostics_scratch_doubles(const mpas_hip_mesh &m) {
     return (size_t)m.nEdges * L + 2 * (size_t)m.nVertices * L + (size_t)m.nCells * L;
 }
 
-__global__ void k_diag_edges1(mpas_hip_mesh m, const double *__restrict__ u, const double *__restrict__ h,
+__global__ __launch_bounds__(256) void k_diag_edges1(long long nb, mpas_hip_mesh m, const double *__restrict__ u, const double *__restrict__ h,
                               double *__restrict__ h_edge, double *__restrict__ ke_edge) {
     const int L = m.nLevels, nE = m.nEdges;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (nE + 1) * L) return;
-    int e = i / L, k = i - e * L;
+    const int e = wave_entity(nb), k = wave_lane();
+    if (e > nE || k >= L) return;
+    const int i = e * L + k;
     if (e == nE) {
         h_edge[i] = 0.0;
         return;
@@ -29,13 +29,13 @@ __global__ void k_diag_edges1(mpas_hip_mesh m, const double *__restrict__ u, con
     ke_edge[i] = efac * (uu * uu);
 }
 
-__global__ void k_diag_vertices(mpas_hip_mesh m, const double *__restrict__ u,
+__global__ __launch_bounds__(256) void k_diag_vertices(long long nb, mpas_hip_mesh m, const double *__restrict__ u,
                                 const double *__restrict__ ke_edge, double *__restrict__ vort,
                                 double *__restrict__ ke_vertex, double *__restrict__ pv_vertex) {
     const int L = m.nLevels, nV = m.nVertices;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (nV + 1) * L) return;
-    int vx = i / L, k = i - vx * L;
+    const int vx = wave_entity(nb), k = wave_lane();
+    if (vx > nV || k >= L) return;
+    const int i = vx * L + k;
     if (vx == nV) {
         vort[i] = 0.0;
         return;
@@ -56,13 +56,13 @@ __global__ void k_diag_vertices(mpas_hip_mesh m, const double *__restrict__ u,
     pv_vertex[i] = m.fVertex[vx] + vv;
 }
 
-__global__ void k_diag_cells(mpas_hip_mesh m, const double *__restrict__ u, const double *__restrict__ ke_edge,
+__global__ __launch_bounds__(256) void k_diag_cells(long long nb, mpas_hip_mesh m, const double *__restrict__ u, const double *__restrict__ ke_edge,
                              const double *__restrict__ ke_vertex, const double *__restrict__ pv_vertex,
                              double *__restrict__ div, double *__restrict__ ke, double *__restrict__ pv_cell) {
     const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (nC + 1) * L) return;
-    int c = i / L, k = i - c * L;
+    const int c = wave_entity(nb), k = wave_lane();
+    if (c > nC || k >= L) return;
+    const int i = c * L + k;
     if (c == nC) {
         div[i] = 0.0;
         ke[i] = 0.0;
@@ -71,57 +71,79 @@ __global__ void k_diag_cells(mpas_hip_mesh m, const double *__restrict__ u, cons
     const int n = m.nEdgesOnCell[c];
     const double ia = m.invAreaCell[c];
     double d = 0.0, kc = 0.0;
-    for (int j = 0; j < n; ++j) {
-        int e = m.edgesOnCell[c * ME + j];
-        double w = m.edgesOnCell_sign[c * ME + j] * m.dvEdge[e];
-        double cd = w * u[e * L + k];
-        double ck = 0.25 * ke_edge[e * L + k];
-        d = (j == 0) ? cd : d + cd;
-        kc = (j == 0) ? ck : kc + ck;
+#pragma unroll
+    for (int j = 0; j < kMaxEdges; ++j) {
+        const int jj = clamp_slot(j, n);
+        const int e = m.edgesOnCell[c * ME + jj];
+        const double w = m.edgesOnCell_sign[c * ME + jj] * m.dvEdge[e];
+        const double ue = u[e * L + k], kee = ke_edge[e * L + k];
+        const double cd = w * ue;
+        const double ck = 0.25 * kee;
+        d = (j < n) ? ((j == 0) ? cd : d + cd) : d;
+        kc = (j < n) ? ((j == 0) ? ck : kc + ck) : kc;
     }
     div[i] = d * ia;
     const double ke_fact = 1.0 - 0.375;
     double tot = ke_fact * (kc * ia);
     double pc = 0.0;
-    for (int j = 0; j < n; ++j) {
-        int vx = m.verticesOnCell[c * ME + j];
-        double ka = m.kiteAreasOnVertex[3 * vx + m.kiteForCell[c * ME + j]];
+#pragma unroll
+    for (int j = 0; j < kMaxEdges; ++j) {
+        const int jj = clamp_slot(j, n);
+        const int vx = m.verticesOnCell[c * ME + jj];
+        const double ka = m.kiteAreasOnVertex[3 * vx + m.kiteForCell[c * ME + jj]];
+        const double kv = ke_vertex[vx * L + k], pv = pv_vertex[vx * L + k];
         double contrib = (1.0 - ke_fact) * ka;
-        contrib = contrib * ke_vertex[vx * L + k];
+        contrib = contrib * kv;
         contrib = contrib * ia;
-        tot = tot + contrib;
-        double cp = ka * pv_vertex[vx * L + k] * ia;
-        pc = (j == 0) ? cp : pc + cp;
+        const double ntot = tot + contrib;
+        tot = (j < n) ? ntot : tot;
+        const double cp = ka * pv * ia;
+        pc = (j < n) ? ((j == 0) ? cp : pc + cp) : pc;
     }
     ke[i] = tot;
     pv_cell[i] = pc;
 }
 
-__global__ void k_diag_edges2(mpas_hip_mesh m, double apvm_dt, int apvm_on, int reconstruct_v,
+__global__ __launch_bounds__(256) void k_diag_edges2(long long nb, mpas_hip_mesh m, double apvm_dt, int apvm_on, int reconstruct_v,
                               const double *__restrict__ u, const double *__restrict__ v_in,
                               const double *__restrict__ pv_vertex, const double *__restrict__ pv_cell,
                               double *__restrict__ v, double *__restrict__ pv_edge) {
     const int L = m.nLevels, nE = m.nEdges, ME2 = m.maxEdges2;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (nE + 1) * L) return;
-    int e = i / L, k = i - e * L;
+    const int e = wave_entity(nb), lane = wave_lane();
+    if (e > nE) return;
     if (e == nE) {
-        v[i] = 0.0;
-        pv_edge[i] = 0.0;
+        if (lane < L) {
+            v[e * L + lane] = 0.0;
+            pv_edge[e * L + lane] = 0.0;
+        }
         return;
     }
+    const int k = lane < L ? lane : L - 1;  // lanes past the column: loads only
+    const int i = e * L + k;
     double vv;
     if (reconstruct_v) {
         vv = 0.0;
         const int ne = m.nEdgesOnEdge[e];
-        for (int j = 0; j < ne; ++j) {
-            int ee = m.edgesOnEdge[e * ME2 + j];
-            double c = m.weightsOnEdge[e * ME2 + j] * u[ee * L + k];
-            vv = (j == 0) ? c : vv + c;
+        const int sl = lane < ne ? lane : ne - 1;
+        const int eev = m.edgesOnEdge[e * ME2 + sl];
+        const double wv = m.weightsOnEdge[e * ME2 + sl];
+#pragma unroll
+        for (int j0 = 0; j0 < kMaxEdges2; j0 += 8) {
+            if (j0 >= ne) break;
+            double uj[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) uj[t] = u[bcast(eev, j0 + t) * L + k];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int j = j0 + t;
+                const double c = bcast(wv, j) * uj[t];
+                vv = (j < ne) ? ((j == 0) ? c : vv + c) : vv;
+            }
         }
     } else {
         vv = v_in[i];
     }
+    if (lane >= L) return;
     v[i] = vv;
     int v1 = m.verticesOnEdge[2 * e], v2 = m.verticesOnEdge[2 * e + 1];
     double p1 = pv_vertex[v1 * L + k], p2 = pv_vertex[v2 * L + k];
@@ -148,12 +170,12 @@ void launch_diagnostics(const mpas_hip_mesh &m, double dt, double apvm, int rk_s
                    bc = blocks_for((long long)(nC + 1) * L);
     // ke_edge / ke_vertex / pv_vertex / pv_cell have no padding row: the
     // padding-row threads of the pass that fills them only zero the outputs.
-    MPAS_LAUNCH(k_diag_edges1, dim3(be), dim3(kBlock), 0, s, m, u, h, h_edge, ke_edge);
-    MPAS_LAUNCH(k_diag_vertices, dim3(bv), dim3(kBlock), 0, s, m, u, ke_edge, vort, ke_vertex, pv_vertex);
-    MPAS_LAUNCH(k_diag_cells, dim3(bc), dim3(kBlock), 0, s, m, u, ke_edge, ke_vertex, pv_vertex, div, ke,
+    MPAS_WAVE_LAUNCH(k_diag_edges1, nE + 1, s, m, u, h, h_edge, ke_edge);
+    MPAS_WAVE_LAUNCH(k_diag_vertices, nV + 1, s, m, u, ke_edge, vort, ke_vertex, pv_vertex);
+    MPAS_WAVE_LAUNCH(k_diag_cells, nC + 1, s, m, u, ke_edge, ke_vertex, pv_vertex, div, ke,
                        pv_cell);
     const int reconstruct_v = (rk_step <= 0 || rk_step == 3) ? 1 : 0;
-    MPAS_LAUNCH(k_diag_edges2, dim3(be), dim3(kBlock), 0, s, m, apvm * dt, apvm > 0.0 ? 1 : 0,
+    MPAS_WAVE_LAUNCH(k_diag_edges2, nE + 1, s, m, apvm * dt, apvm > 0.0 ? 1 : 0,
                        reconstruct_v, u, v_in, pv_vertex, pv_cell, v, pv_edge);
 }
 

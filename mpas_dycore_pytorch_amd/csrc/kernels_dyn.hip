@@ -43,24 +43,28 @@ static DynScratch carve(const mpas_hip_mesh &m, double *base) {
 }
 
 // P1: Smagorinsky eddy viscosity (first RK stage only)
-__global__ void k_dyn_kdiff(mpas_hip_mesh m, DynParams P, const double *__restrict__ u,
+__global__ __launch_bounds__(256) void k_dyn_kdiff(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ u,
                             const double *__restrict__ v, double *__restrict__ kdiff) {
     const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (nC + 1) * L) return;
-    int c = i / L, k = i - c * L;
+    const int c = wave_entity(nb), k = wave_lane();
+    if (c > nC || k >= L) return;
+    const int i = c * L + k;
     if (c == nC) {
         kdiff[i] = 0.0;
         return;
     }
     const int n = m.nEdgesOnCell[c];
     double dd = 0.0, dof = 0.0;
-    for (int j = 0; j < n; ++j) {
-        int e = m.edgesOnCell[c * ME + j];
-        double a = m.defc_a[c * ME + j], b = m.defc_b[c * ME + j];
-        double us = u[e * L + k], vs = v[e * L + k];
-        dd = (dd + a * us) - b * vs;
-        dof = (dof + b * us) + a * vs;
+#pragma unroll
+    for (int j = 0; j < kMaxEdges; ++j) {
+        const int jj = clamp_slot(j, n);
+        const int e = m.edgesOnCell[c * ME + jj];
+        const double a = m.defc_a[c * ME + jj], b = m.defc_b[c * ME + jj];
+        const double us = u[e * L + k], vs = v[e * L + k];
+        const double ndd = (dd + a * us) - b * vs;
+        const double ndof = (dof + b * us) + a * vs;
+        dd = (j < n) ? ndd : dd;
+        dof = (j < n) ? ndof : dof;
     }
     double def = sqrt(dd * dd + dof * dof);
     kdiff[i] = dmin(P.kdiff_coef * def, P.kdiff_cap);
@@ -68,7 +72,7 @@ __global__ void k_dyn_kdiff(mpas_hip_mesh m, DynParams P, const double *__restri
 
 // P2: mass-flux divergence, density tendency, buoyancy dpdz; first stage also
 // the del2 w / theta mixing sums and their del2 fields for the del4 pass.
-__global__ void k_dyn_cells_a(mpas_hip_mesh m, DynParams P, const double *__restrict__ ru,
+__global__ __launch_bounds__(256) void k_dyn_cells_a(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ ru,
                               const double *__restrict__ rw, const double *__restrict__ rr_save,
                               const double *__restrict__ qtot, const double *__restrict__ w,
                               const double *__restrict__ theta_m, const double *__restrict__ rho_edge,
@@ -77,9 +81,9 @@ __global__ void k_dyn_cells_a(mpas_hip_mesh m, DynParams P, const double *__rest
                               double *__restrict__ delsq_w, double *__restrict__ tend_w_euler,
                               double *__restrict__ delsq_theta, double *__restrict__ tend_theta_euler) {
     const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (nC + 1) * Lw) return;
-    int c = i / Lw, k = i - c * Lw;
+    const int c = wave_entity(nb), k = wave_lane();
+    if (c > nC || k >= Lw) return;
+    const int i = c * Lw + k;
     const bool rk1 = (P.rk_step == 1);
     if (c == nC) {
         if (k < L) h_div[c * L + k] = 0.0;
@@ -89,11 +93,14 @@ __global__ void k_dyn_cells_a(mpas_hip_mesh m, DynParams P, const double *__rest
     const double ia = m.invAreaCell[c];
     if (k < L) {
         double s = 0.0;
-        for (int j = 0; j < n; ++j) {
-            int e = m.edgesOnCell[c * ME + j];
-            double wgt = m.edgesOnCell_sign[c * ME + j] * m.dvEdge[e];
-            double cc = wgt * ru[e * L + k];
-            s = (j == 0) ? cc : s + cc;
+#pragma unroll
+        for (int j = 0; j < kMaxEdges; ++j) {
+            const int jj = clamp_slot(j, n);
+            const int e = m.edgesOnCell[c * ME + jj];
+            const double wgt = m.edgesOnCell_sign[c * ME + jj] * m.dvEdge[e];
+            const double r = ru[e * L + k];
+            const double cc = wgt * r;
+            s = (j < n) ? ((j == 0) ? cc : s + cc) : s;
         }
         double hd = s * ia;
         h_div[c * L + k] = hd;
@@ -103,15 +110,19 @@ __global__ void k_dyn_cells_a(mpas_hip_mesh m, DynParams P, const double *__rest
             dpdz[o] = -MPAS_GRAVITY * (m.rho_base[o] * qtot[o] + rr_save[o] * (1.0 + qtot[o]));
             // theta del2 (dyn_tend.py:5563-5640)
             double dsum = 0.0, tsum = 0.0;
-            for (int j = 0; j < n; ++j) {
-                int e = m.edgesOnCell[c * ME + j];
-                int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-                double es = ia * m.edgesOnCell_sign[c * ME + j] * m.dvEdge[e] * m.invDcEdge[e];
-                double prs = P.prandtl_inv * m.meshScalingDel2[e];
-                double bf = es * (theta_m[c2 * L + k] - theta_m[c1 * L + k]) * rho_edge[e * L + k];
-                double sf = bf * 0.5 * (kdiff[c1 * L + k] + kdiff[c2 * L + k]) * prs;
-                dsum = (j == 0) ? bf : dsum + bf;
-                tsum = (j == 0) ? sf : tsum + sf;
+#pragma unroll
+            for (int j = 0; j < kMaxEdges; ++j) {
+                const int jj = clamp_slot(j, n);
+                const int e = m.edgesOnCell[c * ME + jj];
+                const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+                const double t2 = theta_m[c2 * L + k], t1 = theta_m[c1 * L + k], re = rho_edge[e * L + k];
+                const double k1 = kdiff[c1 * L + k], k2 = kdiff[c2 * L + k];
+                const double es = ia * m.edgesOnCell_sign[c * ME + jj] * m.dvEdge[e] * m.invDcEdge[e];
+                const double prs = P.prandtl_inv * m.meshScalingDel2[e];
+                const double bf = es * (t2 - t1) * re;
+                const double sf = bf * 0.5 * (k1 + k2) * prs;
+                dsum = (j < n) ? ((j == 0) ? bf : dsum + bf) : dsum;
+                tsum = (j < n) ? ((j == 0) ? sf : tsum + sf) : tsum;
             }
             delsq_theta[o] = dsum;
             tend_theta_euler[o] = tsum;
@@ -121,15 +132,20 @@ __global__ void k_dyn_cells_a(mpas_hip_mesh m, DynParams P, const double *__rest
         if (k >= 1 && k < L) {
             // w del2 (dyn_tend.py:5315-5420)
             double dsum = 0.0, tsum = 0.0;
-            for (int j = 0; j < n; ++j) {
-                int e = m.edgesOnCell[c * ME + j];
-                int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-                double es = 0.5 * ia * m.edgesOnCell_sign[c * ME + j] * m.dvEdge[e] * m.invDcEdge[e];
-                double bf = es * (rho_edge[e * L + k] + rho_edge[e * L + k - 1]) * (w[c2 * Lw + k] - w[c1 * Lw + k]);
-                double sf = bf * m.meshScalingDel2[e] * 0.25 *
-                            (kdiff[c1 * L + k] + kdiff[c2 * L + k] + kdiff[c1 * L + k - 1] + kdiff[c2 * L + k - 1]);
-                dsum = (j == 0) ? bf : dsum + bf;
-                tsum = (j == 0) ? sf : tsum + sf;
+#pragma unroll
+            for (int j = 0; j < kMaxEdges; ++j) {
+                const int jj = clamp_slot(j, n);
+                const int e = m.edgesOnCell[c * ME + jj];
+                const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+                const double r0 = rho_edge[e * L + k], r1 = rho_edge[e * L + k - 1];
+                const double w2 = w[c2 * Lw + k], w1 = w[c1 * Lw + k];
+                const double k1 = kdiff[c1 * L + k], k2 = kdiff[c2 * L + k];
+                const double k1m = kdiff[c1 * L + k - 1], k2m = kdiff[c2 * L + k - 1];
+                const double es = 0.5 * ia * m.edgesOnCell_sign[c * ME + jj] * m.dvEdge[e] * m.invDcEdge[e];
+                const double bf = es * (r0 + r1) * (w2 - w1);
+                const double sf = bf * m.meshScalingDel2[e] * 0.25 * (k1 + k2 + k1m + k2m);
+                dsum = (j < n) ? ((j == 0) ? bf : dsum + bf) : dsum;
+                tsum = (j < n) ? ((j == 0) ? sf : tsum + sf) : tsum;
             }
             delsq_w[i] = dsum;
             tend_w_euler[i] = tsum;
@@ -141,7 +157,7 @@ __global__ void k_dyn_cells_a(mpas_hip_mesh m, DynParams P, const double *__rest
 
 // P4: normal-velocity tendency (vertical flux, PV flux, KE gradient,
 // divergence coupling); first stage also PGF and del2 mixing into tend_u_euler.
-__global__ void k_dyn_edges_a(mpas_hip_mesh m, DynParams P, const double *__restrict__ u,
+__global__ __launch_bounds__(256) void k_dyn_edges_a(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ u,
                               const double *__restrict__ rw, const double *__restrict__ pv_edge,
                               const double *__restrict__ ke, const double *__restrict__ h_div,
                               const double *__restrict__ rho_edge, const double *__restrict__ pp,
@@ -150,13 +166,19 @@ __global__ void k_dyn_edges_a(mpas_hip_mesh m, DynParams P, const double *__rest
                               const double *__restrict__ kdiff, double *__restrict__ tend_u,
                               double *__restrict__ tend_u_euler, double *__restrict__ delsq_u) {
     const int L = m.nLevels, nE = m.nEdges, ME2 = m.maxEdges2, Lw = L + 1;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (nE + 1) * L) return;
-    int e = i / L, k = i - e * L;
+    const int e = wave_entity(nb), lane = wave_lane();
+    if (e > nE) return;
     if (e == nE) {
-        tend_u[i] = 0.0;
+        if (lane < L) tend_u[e * L + lane] = 0.0;
         return;
     }
+    // lanes past the column repeat the top level for their loads, store nothing
+    const int k = lane < L ? lane : L - 1;
+    const int i = e * L + k;
+    const int ne = m.nEdgesOnEdge[e];
+    const int sl = lane < ne ? lane : ne - 1;
+    const int eev = m.edgesOnEdge[e * ME2 + sl];
+    const double wv = m.weightsOnEdge[e * ME2 + sl];
     const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
     const double *ue = u + e * L;
     const double *fzm = m.fzm, *fzp = m.fzp;
@@ -173,16 +195,28 @@ __global__ void k_dyn_edges_a(mpas_hip_mesh m, DynParams P, const double *__rest
     double vert = -m.rdzw[k] * delta;
     double q = 0.0;
     const double pvs = pv_edge[i];
-    const int ne = m.nEdgesOnEdge[e];
-    for (int j = 0; j < ne; ++j) {
-        int ee = m.edgesOnEdge[e * ME2 + j];
-        double workpv = 0.5 * (pvs + pv_edge[ee * L + k]);
-        double cc = m.weightsOnEdge[e * ME2 + j] * u[ee * L + k] * workpv;
-        q = (j == 0) ? cc : q + cc;
+#pragma unroll
+    for (int j0 = 0; j0 < kMaxEdges2; j0 += 8) {
+        if (j0 >= ne) break;
+        double pj[8], uj[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int ee = bcast(eev, j0 + t);
+            pj[t] = pv_edge[ee * L + k];
+            uj[t] = u[ee * L + k];
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int j = j0 + t;
+            const double workpv = 0.5 * (pvs + pj[t]);
+            const double cc = bcast(wv, j) * uj[t] * workpv;
+            q = (j < ne) ? ((j == 0) ? cc : q + cc) : q;
+        }
     }
     double ke_delta = ke[c2 * L + k] - ke[c1 * L + k];
     double hsum = h_div[c1 * L + k] + h_div[c2 * L + k];
     double base = rho_edge[i] * (q - ke_delta * m.invDcEdge[e]) - ue[k] * 0.5 * hsum;
+    if (lane >= L) return;
     tend_u[i] = vert + base;
     if (P.rk_step == 1) {
         double zzs = m.zz[c2 * L + k] + m.zz[c1 * L + k];
@@ -261,34 +295,48 @@ __global__ void k_dyn_edges_b(mpas_hip_mesh m, DynParams P, int do_del4, const d
 // for rk_step > 1 also the theta perturbation flux
 // dv (ru_save - ru) 0.5 (theta_m_save1 + theta_m_save2) (:5901-6014).  The
 // cell pass applies the edge sign, an exact sign flip of these products.
-__global__ void k_dyn_edge_adv(mpas_hip_mesh m, DynParams P, const double *__restrict__ ru,
+__global__ __launch_bounds__(256) void k_dyn_edge_adv(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ ru,
                                const double *__restrict__ ru_save, const double *__restrict__ w,
                                const double *__restrict__ theta_m, const double *__restrict__ theta_m_save,
                                double *__restrict__ wflux, double *__restrict__ tflux, double *__restrict__ tpert) {
     const int L = m.nLevels, nE = m.nEdges, MA = m.maxAdv, Lw = L + 1;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nE * L) return;
-    const int e = i / L, k = i - e * L;
+    const int e = wave_entity(nb), lane = wave_lane();
+    if (e >= nE) return;
+    const int k = lane < L ? lane : L - 1;  // lanes past the column: loads only
+    const int i = e * L + k;
+    const int na = m.nAdvCellsForEdge[e];
+    const int sl = lane < na ? lane : na - 1;
+    const int advv = m.advCellsForEdge[e * MA + sl];
+    const double c0v = m.adv_coefs[e * MA + sl], c3v = m.adv_coefs_3rd[e * MA + sl];
     const double *rue_col = ru + e * L;
     const double rut = rue_col[k];
     const double srt = sign_pm(rut);
     const bool wl = (k >= 1);
     const double ruw = wl ? m.fzm[k] * rue_col[k] + m.fzp[k] * rue_col[k - 1] : 0.0;
     const double srw = sign_pm(ruw);
-    const int na = m.nAdvCellsForEdge[e];
-    const int *adv = m.advCellsForEdge + e * MA;
-    const double *c0 = m.adv_coefs + e * MA, *c3 = m.adv_coefs_3rd + e * MA;
     double ft = 0.0, fw = 0.0;
-    for (int a = 0; a < na; ++a) {
-        const int ac = adv[a];
-        const double a0 = c0[a], a3 = c3[a];
-        const double ct = (a0 + srt * a3) * theta_m[ac * L + k];
-        ft = (a == 0) ? ct : ft + ct;
-        if (wl) {
-            const double cw = (a0 + srw * a3) * w[ac * Lw + k];
-            fw = (a == 0) ? cw : fw + cw;
+#pragma unroll
+    for (int a0 = 0; a0 < kMaxAdv; a0 += 8) {
+        if (a0 >= na) break;
+        double qt[8], qw[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int ac = bcast(advv, a0 + t);
+            qt[t] = theta_m[ac * L + k];
+            qw[t] = w[ac * Lw + k];
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int a = a0 + t;
+            const double c0 = bcast(c0v, a), c3 = bcast(c3v, a);
+            const double ct = (c0 + srt * c3) * qt[t];
+            const double cw = (c0 + srw * c3) * qw[t];
+            ft = (a < na) ? ((a == 0) ? ct : ft + ct) : ft;
+            fw = (a < na) ? ((a == 0) ? cw : fw + cw) : fw;
         }
     }
+    if (!wl) fw = 0.0;
+    if (lane >= L) return;
     tflux[i] = rut * ft;
     wflux[i] = ruw * fw;
     if (P.rk_step != 1) {
@@ -300,7 +348,7 @@ __global__ void k_dyn_edge_adv(mpas_hip_mesh m, DynParams P, const double *__res
 
 // P7: w and theta tendencies (advection, mixing finish, PGF/buoyancy,
 // vertical transport, diabatic forcing), one thread per (cell, w-level).
-__global__ void k_dyn_cells_b(mpas_hip_mesh m, DynParams P, const double *__restrict__ wflux,
+__global__ __launch_bounds__(256) void k_dyn_cells_b(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ wflux,
                               const double *__restrict__ rw, const double *__restrict__ tflux,
                               const double *__restrict__ rw_save, const double *__restrict__ tpert,
                               const double *__restrict__ w,
@@ -313,9 +361,9 @@ __global__ void k_dyn_cells_b(mpas_hip_mesh m, DynParams P, const double *__rest
                               double *__restrict__ tend_w, double *__restrict__ tend_theta,
                               double *__restrict__ rthdynten) {
     const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (nC + 1) * Lw) return;
-    int c = i / Lw, k = i - c * Lw;
+    const int c = wave_entity(nb), k = wave_lane();
+    if (c > nC || k >= Lw) return;
+    const int i = c * Lw + k;
     if (c == nC) {
         tend_w[i] = 0.0;
         if (k < L) {
@@ -328,24 +376,35 @@ __global__ void k_dyn_cells_b(mpas_hip_mesh m, DynParams P, const double *__rest
     const int n = m.nEdgesOnCell[c];
     const double ia = m.invAreaCell[c];
     const double *fzm = m.fzm, *fzp = m.fzp;
+    int eo[kMaxEdges];
+    double sg[kMaxEdges];
+#pragma unroll
+    for (int j = 0; j < kMaxEdges; ++j) {
+        const int jj = clamp_slot(j, n);
+        eo[j] = m.edgesOnCell[c * ME + jj];
+        sg[j] = m.edgesOnCell_sign[c * ME + jj];
+    }
     // ---------------- w at interface k
     if (k >= 1 && k < L) {
         double hadv = 0.0;
-        for (int j = 0; j < n; ++j) {
-            int e = m.edgesOnCell[c * ME + j];
-            double cc = -m.edgesOnCell_sign[c * ME + j] * wflux[e * L + k];
-            hadv = (j == 0) ? cc : hadv + cc;
+#pragma unroll
+        for (int j = 0; j < kMaxEdges; ++j) {
+            const double f = wflux[eo[j] * L + k];
+            const double cc = -sg[j] * f;
+            hadv = (j < n) ? ((j == 0) ? cc : hadv + cc) : hadv;
         }
         double eu = tend_w_euler[i];
         if (rk1) {
             if (P.h_mom_eddy_visc4 > 0.0) {
                 double r_area = P.h_mom_eddy_visc4 * ia;
-                for (int j = 0; j < n; ++j) {
-                    int e = m.edgesOnCell[c * ME + j];
-                    int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-                    double es = m.meshScalingDel4[e] * r_area * m.dvEdge[e] * m.edgesOnCell_sign[c * ME + j] *
-                                m.invDcEdge[e];
-                    eu = eu - es * (delsq_w[c2 * Lw + k] - delsq_w[c1 * Lw + k]);
+#pragma unroll
+                for (int j = 0; j < kMaxEdges; ++j) {
+                    const int e = eo[j];
+                    const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+                    const double d2 = delsq_w[c2 * Lw + k], d1 = delsq_w[c1 * Lw + k];
+                    const double es = m.meshScalingDel4[e] * r_area * m.dvEdge[e] * sg[j] * m.invDcEdge[e];
+                    const double nx = eu - es * (d2 - d1);
+                    eu = (j < n) ? nx : eu;
                 }
             }
             const int oc = c * L;
@@ -368,26 +427,31 @@ __global__ void k_dyn_cells_b(mpas_hip_mesh m, DynParams P, const double *__rest
     if (k < L) {
         const int o = c * L + k;
         double thadv = 0.0;
-        for (int j = 0; j < n; ++j) {
-            int e = m.edgesOnCell[c * ME + j];
-            double cc = -m.edgesOnCell_sign[c * ME + j] * tflux[e * L + k];
-            thadv = (j == 0) ? cc : thadv + cc;
+#pragma unroll
+        for (int j = 0; j < kMaxEdges; ++j) {
+            const double f = tflux[eo[j] * L + k];
+            const double cc = -sg[j] * f;
+            thadv = (j < n) ? ((j == 0) ? cc : thadv + cc) : thadv;
         }
         if (!rk1) {
-            for (int j = 0; j < n; ++j) {
-                int e = m.edgesOnCell[c * ME + j];
-                thadv = thadv - m.edgesOnCell_sign[c * ME + j] * tpert[e * L + k];
+#pragma unroll
+            for (int j = 0; j < kMaxEdges; ++j) {
+                const double f = tpert[eo[j] * L + k];
+                const double nx = thadv - sg[j] * f;
+                thadv = (j < n) ? nx : thadv;
             }
         }
         double tte = tend_theta_euler[o];
         if (rk1 && P.h_theta_eddy_visc4 > 0.0) {
             double r_area = P.h_theta_eddy_visc4 * P.prandtl_inv * ia;
-            for (int j = 0; j < n; ++j) {
-                int e = m.edgesOnCell[c * ME + j];
-                int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-                double es = m.meshScalingDel4[e] * r_area * m.dvEdge[e] * m.edgesOnCell_sign[c * ME + j] *
-                            m.invDcEdge[e];
-                tte = tte - es * (delsq_theta[c2 * L + k] - delsq_theta[c1 * L + k]);
+#pragma unroll
+            for (int j = 0; j < kMaxEdges; ++j) {
+                const int e = eo[j];
+                const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+                const double d2 = delsq_theta[c2 * L + k], d1 = delsq_theta[c1 * L + k];
+                const double es = m.meshScalingDel4[e] * r_area * m.dvEdge[e] * sg[j] * m.invDcEdge[e];
+                const double nx = tte - es * (d2 - d1);
+                tte = (j < n) ? nx : tte;
             }
             tend_theta_euler[o] = tte;
         }
@@ -419,12 +483,12 @@ void launch_dyn_tend(const mpas_hip_mesh &m, const DynParams &P, const DynInputs
     DynScratch sc = carve(m, scratch);
     const bool rk1 = (P.rk_step == 1);
     if (rk1)
-        MPAS_LAUNCH(k_dyn_kdiff, dim3(blocks_for((long long)(nC + 1) * L)), dim3(kBlock), 0, s, m, P, in.u,
+        MPAS_WAVE_LAUNCH(k_dyn_kdiff, nC + 1, s, m, P, in.u,
                            in.v, io.kdiff);
-    MPAS_LAUNCH(k_dyn_cells_a, dim3(blocks_for((long long)(nC + 1) * (L + 1))), dim3(kBlock), 0, s, m, P,
+    MPAS_WAVE_LAUNCH(k_dyn_cells_a, nC + 1, s, m, P,
                        in.ru, in.rw, in.rr_save, in.qtot, in.w, in.theta_m, in.rho_edge, io.kdiff, out.h_divergence,
                        io.tend_rho, sc.dpdz, sc.delsq_w, io.tend_w_euler, sc.delsq_theta, io.tend_theta_euler);
-    MPAS_LAUNCH(k_dyn_edges_a, dim3(blocks_for((long long)(nE + 1) * L)), dim3(kBlock), 0, s, m, P, in.u,
+    MPAS_WAVE_LAUNCH(k_dyn_edges_a, nE + 1, s, m, P, in.u,
                        in.rw, in.pv_edge, in.ke, out.h_divergence, in.rho_edge, in.pp, sc.dpdz, in.cqu, in.divergence,
                        in.vorticity, io.kdiff, out.tend_u, io.tend_u_euler, sc.delsq_u);
     const int do_del4 = (rk1 && P.h_mom_eddy_visc4 > 0.0) ? 1 : 0;
@@ -435,9 +499,9 @@ void launch_dyn_tend(const mpas_hip_mesh &m, const DynParams &P, const DynInputs
     }
     MPAS_LAUNCH(k_dyn_edges_b, dim3(blocks_for((long long)nE * L)), dim3(kBlock), 0, s, m, P, do_del4,
                        in.rho_edge, sc.delsq_div, sc.delsq_vort, io.tend_u_euler, out.tend_u);
-    MPAS_LAUNCH(k_dyn_edge_adv, dim3(blocks_for((long long)nE * L)), dim3(kBlock), 0, s, m, P, in.ru, in.ru_save,
+    MPAS_WAVE_LAUNCH(k_dyn_edge_adv, nE, s, m, P, in.ru, in.ru_save,
                 in.w, in.theta_m, in.theta_m_save, sc.wflux, sc.tflux, sc.tpert);
-    MPAS_LAUNCH(k_dyn_cells_b, dim3(blocks_for((long long)(nC + 1) * (L + 1))), dim3(kBlock), 0, s, m, P,
+    MPAS_WAVE_LAUNCH(k_dyn_cells_b, nC + 1, s, m, P,
                        sc.wflux, in.rw, sc.tflux, in.rw_save, sc.tpert, in.w, in.theta_m, in.theta_m_save, in.rho_zz, in.pp,
                        in.cqw, in.rt_diabatic_tend, sc.dpdz, sc.delsq_w, sc.delsq_theta, io.tend_rho,
                        io.tend_w_euler, io.tend_theta_euler, out.tend_w, out.tend_theta, out.rthdynten);

@@ -165,6 +165,12 @@ static int validate_mesh(const mpas_hip_mesh *m) {
         set_error("nLevels must be in [3, 63] (one wavefront per column)");
         return 1;
     }
+    if (m->maxEdges < 1 || m->maxEdges > kMaxEdges || m->maxEdges2 < 1 || m->maxEdges2 > kMaxEdges2 ||
+        m->maxAdv < 2 || m->maxAdv > kMaxAdv || m->vertexDegree != 3) {
+        set_error("mesh slot widths exceed the compiled caps (maxEdges <= 8, maxEdges2 <= 16, "
+                  "maxAdv <= 16, vertexDegree == 3)");
+        return 1;
+    }
     long long big = (long long)(m->nEdges + 1) * (m->nLevels + 1) * 16;
     if (big >= (1LL << 31)) {
         set_error("mesh too large for 32-bit flat indexing");

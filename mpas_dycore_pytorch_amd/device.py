@@ -103,8 +103,8 @@ class DeviceMesh:
                 live = a[: rows[name]]
                 if live.ndim == 2 and name in width:
                     cnt = counts[width[name]]
-                    if cnt.min() < 0 or cnt.max() > live.shape[1]:
-                        raise MpasMeshMismatchError(f"{width[name]} outside 0..{live.shape[1]}")
+                    if cnt.min() < 1 or cnt.max() > live.shape[1]:
+                        raise MpasMeshMismatchError(f"{width[name]} outside 1..{live.shape[1]}")
                     used = np.arange(live.shape[1])[None, :] < cnt[:, None]
                     bad = used & ((live < 0) | (live >= hi))
                 else:
