@@ -45,25 +45,30 @@ __global__ __launch_bounds__(256) void k_acoustic_edges(long long nb, mpas_hip_m
                                  const double *__restrict__ rho_pp, const double *__restrict__ exner,
                                  const double *__restrict__ cqu, const double *__restrict__ tend_ru,
                                  double *__restrict__ ru_p, double *__restrict__ ruAvg) {
-    const int L = m.nLevels, nE = m.nEdges;
-    const int e = wave_entity(nb), k = wave_lane();
-    if (e >= nE || k >= L) return;
-    const int i = e * L + k;
-    const double dts = P.dts;
-    if (P.small_step == 1) {
-        double r = dts * tend_ru[i];
+    auto body_ = [&](const int e) {
+        const int L = m.nLevels, nE = m.nEdges;
+        const int k = wave_lane();
+        if (e >= nE || k >= L) return;
+        const int i = e * L + k;
+        const double dts = P.dts;
+        if (P.small_step == 1) {
+            double r = dts * tend_ru[i];
+            ru_p[i] = r;
+            ruAvg[i] = r;
+            return;
+        }
+        int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+        double pg = ((rtheta_pp[c2 * L + k] - rtheta_pp[c1 * L + k]) * m.invDcEdge[e]) /
+                    (0.5 * (m.zz[c2 * L + k] + m.zz[c1 * L + k]));
+        pg = cqu[i] * 0.5 * P.c2 * (exner[c1 * L + k] + exner[c2 * L + k]) * pg;
+        pg = pg + 0.5 * m.zxu[i] * MPAS_GRAVITY * (rho_pp[c1 * L + k] + rho_pp[c2 * L + k]);
+        double r = ru_p[i] + dts * (tend_ru[i] - pg);
         ru_p[i] = r;
-        ruAvg[i] = r;
-        return;
-    }
-    int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-    double pg = ((rtheta_pp[c2 * L + k] - rtheta_pp[c1 * L + k]) * m.invDcEdge[e]) /
-                (0.5 * (m.zz[c2 * L + k] + m.zz[c1 * L + k]));
-    pg = cqu[i] * 0.5 * P.c2 * (exner[c1 * L + k] + exner[c2 * L + k]) * pg;
-    pg = pg + 0.5 * m.zxu[i] * MPAS_GRAVITY * (rho_pp[c1 * L + k] + rho_pp[c2 * L + k]);
-    double r = ru_p[i] + dts * (tend_ru[i] - pg);
-    ru_p[i] = r;
-    ruAvg[i] = ruAvg[i] + r;
+        ruAvg[i] = ruAvg[i] + r;
+    };
+    const int base_ = wave_entity_base(nb);
+#pragma unroll 1
+    for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
 // Cell pass: one wavefront per column, lane = level.  Flux sums and the
@@ -210,16 +215,21 @@ void launch_acoustic(const mpas_hip_mesh &m, const AcousticParams &P, const Acou
 __global__ __launch_bounds__(256) void k_divdamp(long long nb, mpas_hip_mesh m, double coef, const double *__restrict__ theta_m,
                           const double *__restrict__ rtheta_pp, const double *__restrict__ rtheta_pp_old,
                           double *__restrict__ ru_p) {
-    const int L = m.nLevels, nE = m.nEdges;
-    const int e = wave_entity(nb), k = wave_lane();
-    if (e >= nE || k >= L) return;
-    const int i = e * L + k;
-    int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-    double d1 = -(rtheta_pp[c1 * L + k] - rtheta_pp_old[c1 * L + k]);
-    double d2 = -(rtheta_pp[c2 * L + k] - rtheta_pp_old[c2 * L + k]);
-    double ts = theta_m[c1 * L + k] + theta_m[c2 * L + k];
-    double damp = (fabs(ts) >= 1.0e-30) ? coef * (d2 - d1) / ts : 0.0;
-    ru_p[i] = ru_p[i] + damp;
+    auto body_ = [&](const int e) {
+        const int L = m.nLevels, nE = m.nEdges;
+        const int k = wave_lane();
+        if (e >= nE || k >= L) return;
+        const int i = e * L + k;
+        int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+        double d1 = -(rtheta_pp[c1 * L + k] - rtheta_pp_old[c1 * L + k]);
+        double d2 = -(rtheta_pp[c2 * L + k] - rtheta_pp_old[c2 * L + k]);
+        double ts = theta_m[c1 * L + k] + theta_m[c2 * L + k];
+        double damp = (fabs(ts) >= 1.0e-30) ? coef * (d2 - d1) / ts : 0.0;
+        ru_p[i] = ru_p[i] + damp;
+    };
+    const int base_ = wave_entity_base(nb);
+#pragma unroll 1
+    for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
 void launch_divdamp(const mpas_hip_mesh &m, double dts, double smdiv, double len_disp, const double *theta_m,
@@ -271,56 +281,66 @@ __global__ void k_recover_cells(mpas_hip_mesh m, int rk_step, double dt, double 
 __global__ __launch_bounds__(256) void k_recover_edges(long long nb, mpas_hip_mesh m, double invNs, const double *__restrict__ ru_save,
                                 const double *__restrict__ ru_p, const double *__restrict__ rho_zz,
                                 double *__restrict__ ruAvg, double *__restrict__ ru, double *__restrict__ u) {
-    const int L = m.nLevels, nE = m.nEdges;
-    const int e = wave_entity(nb), k = wave_lane();
-    if (e >= nE || k >= L) return;
-    const int i = e * L + k;
-    int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-    ruAvg[i] = ru_save[i] + (ruAvg[i] * invNs);
-    double rn = ru_save[i] + ru_p[i];
-    ru[i] = rn;
-    u[i] = 2.0 * rn / (rho_zz[c1 * L + k] + rho_zz[c2 * L + k]);
+    auto body_ = [&](const int e) {
+        const int L = m.nLevels, nE = m.nEdges;
+        const int k = wave_lane();
+        if (e >= nE || k >= L) return;
+        const int i = e * L + k;
+        int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+        ruAvg[i] = ru_save[i] + (ruAvg[i] * invNs);
+        double rn = ru_save[i] + ru_p[i];
+        ru[i] = rn;
+        u[i] = 2.0 * rn / (rho_zz[c1 * L + k] + rho_zz[c2 * L + k]);
+    };
+    const int base_ = wave_entity_base(nb);
+#pragma unroll 1
+    for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
 __global__ __launch_bounds__(256) void k_recover_w(long long nb, mpas_hip_mesh m, const double *__restrict__ ru, const double *__restrict__ rw,
                             const double *__restrict__ rho_zz, double *__restrict__ w) {
-    const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
-    const int c = wave_entity(nb), k = wave_lane();
-    if (c >= nC || k >= Lw) return;
-    const int i = c * Lw + k;
-    const int oc = c * L;
-    const int n = m.nEdgesOnCell[c];
-    // Flat terrain: zb_cell = zb3_cell = 0, every terrain term is a signed
-    // zero and adding it leaves the value unchanged (up to the sign of a zero
-    // result), so the edge loops are skipped.
-    const int nt = m.flatTerrain ? 0 : n;
-    if (k == 0) {
-        double sacc = 0.0;
-        for (int j = 0; j < nt; ++j) {
-            int e = m.edgesOnCell[c * ME + j];
-            const double *re = ru + e * L;
-            double fs = m.cf1 * re[0] + m.cf2 * re[1] + m.cf3 * re[2];
-            size_t zo = ((size_t)c * ME + j) * Lw;
-            double coef = m.zb_cell[zo] + sign_pm(fs) * m.zb3_cell[zo];
-            double con = m.edgesOnCell_sign[c * ME + j] * coef * fs;
-            sacc = (j == 0) ? con : sacc + con;
+    auto body_ = [&](const int c) {
+        const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
+        const int k = wave_lane();
+        if (c >= nC || k >= Lw) return;
+        const int i = c * Lw + k;
+        const int oc = c * L;
+        const int n = m.nEdgesOnCell[c];
+        // Flat terrain: zb_cell = zb3_cell = 0, every terrain term is a signed
+        // zero and adding it leaves the value unchanged (up to the sign of a zero
+        // result), so the edge loops are skipped.
+        const int nt = m.flatTerrain ? 0 : n;
+        if (k == 0) {
+            double sacc = 0.0;
+            for (int j = 0; j < nt; ++j) {
+                int e = m.edgesOnCell[c * ME + j];
+                const double *re = ru + e * L;
+                double fs = m.cf1 * re[0] + m.cf2 * re[1] + m.cf3 * re[2];
+                size_t zo = ((size_t)c * ME + j) * Lw;
+                double coef = m.zb_cell[zo] + sign_pm(fs) * m.zb3_cell[zo];
+                double con = m.edgesOnCell_sign[c * ME + j] * coef * fs;
+                sacc = (j == 0) ? con : sacc + con;
+            }
+            w[i] = sacc / (m.cf1 * rho_zz[oc + 0] + m.cf2 * rho_zz[oc + 1] + m.cf3 * rho_zz[oc + 2]);
+        } else if (k < L) {
+            double zi = m.fzm[k] * m.zz[oc + k] + m.fzp[k] * m.zz[oc + k - 1];
+            double acc = rw[i] / zi;
+            for (int j = 0; j < nt; ++j) {
+                int e = m.edgesOnCell[c * ME + j];
+                const double *re = ru + e * L;
+                double fi = m.fzm[k] * re[k] + m.fzp[k] * re[k - 1];
+                size_t zo = ((size_t)c * ME + j) * Lw + k;
+                double coef = m.zb_cell[zo] + sign_pm(fi) * m.zb3_cell[zo];
+                acc = acc + m.edgesOnCell_sign[c * ME + j] * coef * fi;
+            }
+            w[i] = acc / (m.fzm[k] * rho_zz[oc + k] + m.fzp[k] * rho_zz[oc + k - 1]);
+        } else {
+            w[i] = 0.0;
         }
-        w[i] = sacc / (m.cf1 * rho_zz[oc + 0] + m.cf2 * rho_zz[oc + 1] + m.cf3 * rho_zz[oc + 2]);
-    } else if (k < L) {
-        double zi = m.fzm[k] * m.zz[oc + k] + m.fzp[k] * m.zz[oc + k - 1];
-        double acc = rw[i] / zi;
-        for (int j = 0; j < nt; ++j) {
-            int e = m.edgesOnCell[c * ME + j];
-            const double *re = ru + e * L;
-            double fi = m.fzm[k] * re[k] + m.fzp[k] * re[k - 1];
-            size_t zo = ((size_t)c * ME + j) * Lw + k;
-            double coef = m.zb_cell[zo] + sign_pm(fi) * m.zb3_cell[zo];
-            acc = acc + m.edgesOnCell_sign[c * ME + j] * coef * fi;
-        }
-        w[i] = acc / (m.fzm[k] * rho_zz[oc + k] + m.fzp[k] * rho_zz[oc + k - 1]);
-    } else {
-        w[i] = 0.0;
-    }
+    };
+    const int base_ = wave_entity_base(nb);
+#pragma unroll 1
+    for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
 void launch_recover(const mpas_hip_mesh &m, int rk_step, double dt, int ns, const RecoverInputs &in,

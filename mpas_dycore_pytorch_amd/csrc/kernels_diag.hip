@@ -14,148 +14,168 @@ size_t diagnostics_scratch_doubles(const mpas_hip_mesh &m) {
 
 __global__ __launch_bounds__(256) void k_diag_edges1(long long nb, mpas_hip_mesh m, const double *__restrict__ u, const double *__restrict__ h,
                               double *__restrict__ h_edge, double *__restrict__ ke_edge) {
-    const int L = m.nLevels, nE = m.nEdges;
-    const int e = wave_entity(nb), k = wave_lane();
-    if (e > nE || k >= L) return;
-    const int i = e * L + k;
-    if (e == nE) {
-        h_edge[i] = 0.0;
-        return;
-    }
-    int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-    h_edge[i] = 0.5 * (h[c1 * L + k] + h[c2 * L + k]);
-    double efac = m.dcEdge[e] * m.dvEdge[e];
-    double uu = u[i];
-    ke_edge[i] = efac * (uu * uu);
+    auto body_ = [&](const int e) {
+        const int L = m.nLevels, nE = m.nEdges;
+        const int k = wave_lane();
+        if (e > nE || k >= L) return;
+        const int i = e * L + k;
+        if (e == nE) {
+            h_edge[i] = 0.0;
+            return;
+        }
+        int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+        h_edge[i] = 0.5 * (h[c1 * L + k] + h[c2 * L + k]);
+        double efac = m.dcEdge[e] * m.dvEdge[e];
+        double uu = u[i];
+        ke_edge[i] = efac * (uu * uu);
+    };
+    const int base_ = wave_entity_base(nb);
+#pragma unroll 1
+    for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
 __global__ __launch_bounds__(256) void k_diag_vertices(long long nb, mpas_hip_mesh m, const double *__restrict__ u,
                                 const double *__restrict__ ke_edge, double *__restrict__ vort,
                                 double *__restrict__ ke_vertex, double *__restrict__ pv_vertex) {
-    const int L = m.nLevels, nV = m.nVertices;
-    const int vx = wave_entity(nb), k = wave_lane();
-    if (vx > nV || k >= L) return;
-    const int i = vx * L + k;
-    if (vx == nV) {
-        vort[i] = 0.0;
-        return;
-    }
-    double s = 0.0, sk = 0.0;
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        int e = m.edgesOnVertex[3 * vx + j];
-        double w = m.edgesOnVertex_sign[3 * vx + j] * m.dcEdge[e];
-        double c = w * u[e * L + k];
-        double kk = ke_edge[e * L + k];
-        s = (j == 0) ? c : s + c;
-        sk = (j == 0) ? kk : sk + kk;
-    }
-    double vv = s * m.invAreaTriangle[vx];
-    vort[i] = vv;
-    ke_vertex[i] = 0.25 * m.invAreaTriangle[vx] * sk;
-    pv_vertex[i] = m.fVertex[vx] + vv;
+    auto body_ = [&](const int vx) {
+        const int L = m.nLevels, nV = m.nVertices;
+        const int k = wave_lane();
+        if (vx > nV || k >= L) return;
+        const int i = vx * L + k;
+        if (vx == nV) {
+            vort[i] = 0.0;
+            return;
+        }
+        double s = 0.0, sk = 0.0;
+    #pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            int e = m.edgesOnVertex[3 * vx + j];
+            double w = m.edgesOnVertex_sign[3 * vx + j] * m.dcEdge[e];
+            double c = w * u[e * L + k];
+            double kk = ke_edge[e * L + k];
+            s = (j == 0) ? c : s + c;
+            sk = (j == 0) ? kk : sk + kk;
+        }
+        double vv = s * m.invAreaTriangle[vx];
+        vort[i] = vv;
+        ke_vertex[i] = 0.25 * m.invAreaTriangle[vx] * sk;
+        pv_vertex[i] = m.fVertex[vx] + vv;
+    };
+    const int base_ = wave_entity_base(nb);
+#pragma unroll 1
+    for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
 __global__ __launch_bounds__(256) void k_diag_cells(long long nb, mpas_hip_mesh m, const double *__restrict__ u, const double *__restrict__ ke_edge,
                              const double *__restrict__ ke_vertex, const double *__restrict__ pv_vertex,
                              double *__restrict__ div, double *__restrict__ ke, double *__restrict__ pv_cell) {
-    const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges;
-    const int c = wave_entity(nb), k = wave_lane();
-    if (c > nC || k >= L) return;
-    const int i = c * L + k;
-    if (c == nC) {
-        div[i] = 0.0;
-        ke[i] = 0.0;
-        return;
-    }
-    const int n = m.nEdgesOnCell[c];
-    const double ia = m.invAreaCell[c];
-    double d = 0.0, kc = 0.0;
-#pragma unroll
-    for (int j = 0; j < kMaxEdges; ++j) {
-        const int jj = clamp_slot(j, n);
-        const int e = m.edgesOnCell[c * ME + jj];
-        const double w = m.edgesOnCell_sign[c * ME + jj] * m.dvEdge[e];
-        const double ue = u[e * L + k], kee = ke_edge[e * L + k];
-        const double cd = w * ue;
-        const double ck = 0.25 * kee;
-        d = (j < n) ? ((j == 0) ? cd : d + cd) : d;
-        kc = (j < n) ? ((j == 0) ? ck : kc + ck) : kc;
-    }
-    div[i] = d * ia;
-    const double ke_fact = 1.0 - 0.375;
-    double tot = ke_fact * (kc * ia);
-    double pc = 0.0;
-#pragma unroll
-    for (int j = 0; j < kMaxEdges; ++j) {
-        const int jj = clamp_slot(j, n);
-        const int vx = m.verticesOnCell[c * ME + jj];
-        const double ka = m.kiteAreasOnVertex[3 * vx + m.kiteForCell[c * ME + jj]];
-        const double kv = ke_vertex[vx * L + k], pv = pv_vertex[vx * L + k];
-        double contrib = (1.0 - ke_fact) * ka;
-        contrib = contrib * kv;
-        contrib = contrib * ia;
-        const double ntot = tot + contrib;
-        tot = (j < n) ? ntot : tot;
-        const double cp = ka * pv * ia;
-        pc = (j < n) ? ((j == 0) ? cp : pc + cp) : pc;
-    }
-    ke[i] = tot;
-    pv_cell[i] = pc;
+    auto body_ = [&](const int c) {
+        const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges;
+        const int k = wave_lane();
+        if (c > nC || k >= L) return;
+        const int i = c * L + k;
+        if (c == nC) {
+            div[i] = 0.0;
+            ke[i] = 0.0;
+            return;
+        }
+        const int n = m.nEdgesOnCell[c];
+        const double ia = m.invAreaCell[c];
+        double d = 0.0, kc = 0.0;
+    #pragma unroll
+        for (int j = 0; j < kMaxEdges; ++j) {
+            const int jj = clamp_slot(j, n);
+            const int e = m.edgesOnCell[c * ME + jj];
+            const double w = m.edgesOnCell_sign[c * ME + jj] * m.dvEdge[e];
+            const double ue = u[e * L + k], kee = ke_edge[e * L + k];
+            const double cd = w * ue;
+            const double ck = 0.25 * kee;
+            d = (j < n) ? ((j == 0) ? cd : d + cd) : d;
+            kc = (j < n) ? ((j == 0) ? ck : kc + ck) : kc;
+        }
+        div[i] = d * ia;
+        const double ke_fact = 1.0 - 0.375;
+        double tot = ke_fact * (kc * ia);
+        double pc = 0.0;
+    #pragma unroll
+        for (int j = 0; j < kMaxEdges; ++j) {
+            const int jj = clamp_slot(j, n);
+            const int vx = m.verticesOnCell[c * ME + jj];
+            const double ka = m.kiteAreasOnVertex[3 * vx + m.kiteForCell[c * ME + jj]];
+            const double kv = ke_vertex[vx * L + k], pv = pv_vertex[vx * L + k];
+            double contrib = (1.0 - ke_fact) * ka;
+            contrib = contrib * kv;
+            contrib = contrib * ia;
+            const double ntot = tot + contrib;
+            tot = (j < n) ? ntot : tot;
+            const double cp = ka * pv * ia;
+            pc = (j < n) ? ((j == 0) ? cp : pc + cp) : pc;
+        }
+        ke[i] = tot;
+        pv_cell[i] = pc;
+    };
+    const int base_ = wave_entity_base(nb);
+#pragma unroll 1
+    for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
 __global__ __launch_bounds__(256) void k_diag_edges2(long long nb, mpas_hip_mesh m, double apvm_dt, int apvm_on, int reconstruct_v,
                               const double *__restrict__ u, const double *__restrict__ v_in,
                               const double *__restrict__ pv_vertex, const double *__restrict__ pv_cell,
                               double *__restrict__ v, double *__restrict__ pv_edge) {
-    const int L = m.nLevels, nE = m.nEdges, ME2 = m.maxEdges2;
-    const int e = wave_entity(nb), lane = wave_lane();
-    if (e > nE) return;
-    if (e == nE) {
-        if (lane < L) {
-            v[e * L + lane] = 0.0;
-            pv_edge[e * L + lane] = 0.0;
-        }
-        return;
-    }
-    const int k = lane < L ? lane : L - 1;  // lanes past the column: loads only
-    const int i = e * L + k;
-    double vv;
-    if (reconstruct_v) {
-        vv = 0.0;
-        const int ne = m.nEdgesOnEdge[e];
-        const int sl = lane < ne ? lane : ne - 1;
-        const int eev = m.edgesOnEdge[e * ME2 + sl];
-        const double wv = m.weightsOnEdge[e * ME2 + sl];
-#pragma unroll
-        for (int j0 = 0; j0 < kMaxEdges2; j0 += 8) {
-            if (j0 >= ne) break;
-            double uj[8];
-#pragma unroll
-            for (int t = 0; t < 8; ++t) uj[t] = u[bcast(eev, j0 + t) * L + k];
-#pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                const int j = j0 + t;
-                const double c = bcast(wv, j) * uj[t];
-                vv = (j < ne) ? ((j == 0) ? c : vv + c) : vv;
+    auto body_ = [&](const int e) {
+        const int L = m.nLevels, nE = m.nEdges, ME2 = m.maxEdges2;
+        const int lane = wave_lane();
+        if (e > nE) return;
+        if (e == nE) {
+            if (lane < L) {
+                v[e * L + lane] = 0.0;
+                pv_edge[e * L + lane] = 0.0;
             }
+            return;
         }
-    } else {
-        vv = v_in[i];
-    }
-    if (lane >= L) return;
-    v[i] = vv;
-    int v1 = m.verticesOnEdge[2 * e], v2 = m.verticesOnEdge[2 * e + 1];
-    double p1 = pv_vertex[v1 * L + k], p2 = pv_vertex[v2 * L + k];
-    double pe = 0.5 * (p1 + p2);
-    if (apvm_on) {
-        int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-        double grad_pvt = (p2 - p1) * m.invDvEdge[e];
-        double grad_pvn = (pv_cell[c2 * L + k] - pv_cell[c1 * L + k]) * m.invDcEdge[e];
-        double delta = apvm_dt * (vv * grad_pvt + u[i] * grad_pvn);
-        pe = pe - delta;
-    }
-    pv_edge[i] = pe;
+        const int k = lane < L ? lane : L - 1;  // lanes past the column: loads only
+        const int i = e * L + k;
+        double vv;
+        if (reconstruct_v) {
+            vv = 0.0;
+            const int ne = m.nEdgesOnEdge[e];
+            const int sl = lane < ne ? lane : ne - 1;
+            const int eev = m.edgesOnEdge[e * ME2 + sl];
+            const double wv = m.weightsOnEdge[e * ME2 + sl];
+    #pragma unroll
+            for (int j0 = 0; j0 < kMaxEdges2; j0 += 8) {
+                if (j0 >= ne) break;
+                double uj[8];
+    #pragma unroll
+                for (int t = 0; t < 8; ++t) uj[t] = u[bcast(eev, j0 + t) * L + k];
+    #pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const int j = j0 + t;
+                    const double c = bcast(wv, j) * uj[t];
+                    vv = (j < ne) ? ((j == 0) ? c : vv + c) : vv;
+                }
+            }
+        } else {
+            vv = v_in[i];
+        }
+        if (lane >= L) return;
+        v[i] = vv;
+        int v1 = m.verticesOnEdge[2 * e], v2 = m.verticesOnEdge[2 * e + 1];
+        double p1 = pv_vertex[v1 * L + k], p2 = pv_vertex[v2 * L + k];
+        double pe = 0.5 * (p1 + p2);
+        if (apvm_on) {
+            int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+            double grad_pvt = (p2 - p1) * m.invDvEdge[e];
+            double grad_pvn = (pv_cell[c2 * L + k] - pv_cell[c1 * L + k]) * m.invDcEdge[e];
+            double delta = apvm_dt * (vv * grad_pvt + u[i] * grad_pvn);
+            pe = pe - delta;
+        }
+        pv_edge[i] = pe;
+    };
+    const int base_ = wave_entity_base(nb);
+#pragma unroll 1
+    for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
 void launch_diagnostics(const mpas_hip_mesh &m, double dt, double apvm, int rk_step, const double *u,

@@ -45,29 +45,34 @@ static DynScratch carve(const mpas_hip_mesh &m, double *base) {
 // P1: Smagorinsky eddy viscosity (first RK stage only)
 __global__ __launch_bounds__(256) void k_dyn_kdiff(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ u,
                             const double *__restrict__ v, double *__restrict__ kdiff) {
-    const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges;
-    const int c = wave_entity(nb), k = wave_lane();
-    if (c > nC || k >= L) return;
-    const int i = c * L + k;
-    if (c == nC) {
-        kdiff[i] = 0.0;
-        return;
-    }
-    const int n = m.nEdgesOnCell[c];
-    double dd = 0.0, dof = 0.0;
-#pragma unroll
-    for (int j = 0; j < kMaxEdges; ++j) {
-        const int jj = clamp_slot(j, n);
-        const int e = m.edgesOnCell[c * ME + jj];
-        const double a = m.defc_a[c * ME + jj], b = m.defc_b[c * ME + jj];
-        const double us = u[e * L + k], vs = v[e * L + k];
-        const double ndd = (dd + a * us) - b * vs;
-        const double ndof = (dof + b * us) + a * vs;
-        dd = (j < n) ? ndd : dd;
-        dof = (j < n) ? ndof : dof;
-    }
-    double def = sqrt(dd * dd + dof * dof);
-    kdiff[i] = dmin(P.kdiff_coef * def, P.kdiff_cap);
+    auto body_ = [&](const int c) {
+        const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges;
+        const int k = wave_lane();
+        if (c > nC || k >= L) return;
+        const int i = c * L + k;
+        if (c == nC) {
+            kdiff[i] = 0.0;
+            return;
+        }
+        const int n = m.nEdgesOnCell[c];
+        double dd = 0.0, dof = 0.0;
+    #pragma unroll
+        for (int j = 0; j < kMaxEdges; ++j) {
+            const int jj = clamp_slot(j, n);
+            const int e = m.edgesOnCell[c * ME + jj];
+            const double a = m.defc_a[c * ME + jj], b = m.defc_b[c * ME + jj];
+            const double us = u[e * L + k], vs = v[e * L + k];
+            const double ndd = (dd + a * us) - b * vs;
+            const double ndof = (dof + b * us) + a * vs;
+            dd = (j < n) ? ndd : dd;
+            dof = (j < n) ? ndof : dof;
+        }
+        double def = sqrt(dd * dd + dof * dof);
+        kdiff[i] = dmin(P.kdiff_coef * def, P.kdiff_cap);
+    };
+    const int base_ = wave_entity_base(nb);
+#pragma unroll 1
+    for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
 // P2: mass-flux divergence, density tendency, buoyancy dpdz; first stage also
@@ -80,79 +85,84 @@ __global__ __launch_bounds__(256) void k_dyn_cells_a(long long nb, mpas_hip_mesh
                               double *__restrict__ tend_rho, double *__restrict__ dpdz,
                               double *__restrict__ delsq_w, double *__restrict__ tend_w_euler,
                               double *__restrict__ delsq_theta, double *__restrict__ tend_theta_euler) {
-    const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
-    const int c = wave_entity(nb), k = wave_lane();
-    if (c > nC || k >= Lw) return;
-    const int i = c * Lw + k;
-    const bool rk1 = (P.rk_step == 1);
-    if (c == nC) {
-        if (k < L) h_div[c * L + k] = 0.0;
-        return;
-    }
-    const int n = m.nEdgesOnCell[c];
-    const double ia = m.invAreaCell[c];
-    if (k < L) {
-        double s = 0.0;
-#pragma unroll
-        for (int j = 0; j < kMaxEdges; ++j) {
-            const int jj = clamp_slot(j, n);
-            const int e = m.edgesOnCell[c * ME + jj];
-            const double wgt = m.edgesOnCell_sign[c * ME + jj] * m.dvEdge[e];
-            const double r = ru[e * L + k];
-            const double cc = wgt * r;
-            s = (j < n) ? ((j == 0) ? cc : s + cc) : s;
+    auto body_ = [&](const int c) {
+        const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
+        const int k = wave_lane();
+        if (c > nC || k >= Lw) return;
+        const int i = c * Lw + k;
+        const bool rk1 = (P.rk_step == 1);
+        if (c == nC) {
+            if (k < L) h_div[c * L + k] = 0.0;
+            return;
         }
-        double hd = s * ia;
-        h_div[c * L + k] = hd;
+        const int n = m.nEdgesOnCell[c];
+        const double ia = m.invAreaCell[c];
+        if (k < L) {
+            double s = 0.0;
+    #pragma unroll
+            for (int j = 0; j < kMaxEdges; ++j) {
+                const int jj = clamp_slot(j, n);
+                const int e = m.edgesOnCell[c * ME + jj];
+                const double wgt = m.edgesOnCell_sign[c * ME + jj] * m.dvEdge[e];
+                const double r = ru[e * L + k];
+                const double cc = wgt * r;
+                s = (j < n) ? ((j == 0) ? cc : s + cc) : s;
+            }
+            double hd = s * ia;
+            h_div[c * L + k] = hd;
+            if (rk1) {
+                const int o = c * L + k;
+                tend_rho[o] = -hd - m.rdzw[k] * (rw[c * Lw + k + 1] - rw[c * Lw + k]);
+                dpdz[o] = -MPAS_GRAVITY * (m.rho_base[o] * qtot[o] + rr_save[o] * (1.0 + qtot[o]));
+                // theta del2 (dyn_tend.py:5563-5640)
+                double dsum = 0.0, tsum = 0.0;
+    #pragma unroll
+                for (int j = 0; j < kMaxEdges; ++j) {
+                    const int jj = clamp_slot(j, n);
+                    const int e = m.edgesOnCell[c * ME + jj];
+                    const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+                    const double t2 = theta_m[c2 * L + k], t1 = theta_m[c1 * L + k], re = rho_edge[e * L + k];
+                    const double k1 = kdiff[c1 * L + k], k2 = kdiff[c2 * L + k];
+                    const double es = ia * m.edgesOnCell_sign[c * ME + jj] * m.dvEdge[e] * m.invDcEdge[e];
+                    const double prs = P.prandtl_inv * m.meshScalingDel2[e];
+                    const double bf = es * (t2 - t1) * re;
+                    const double sf = bf * 0.5 * (k1 + k2) * prs;
+                    dsum = (j < n) ? ((j == 0) ? bf : dsum + bf) : dsum;
+                    tsum = (j < n) ? ((j == 0) ? sf : tsum + sf) : tsum;
+                }
+                delsq_theta[o] = dsum;
+                tend_theta_euler[o] = tsum;
+            }
+        }
         if (rk1) {
-            const int o = c * L + k;
-            tend_rho[o] = -hd - m.rdzw[k] * (rw[c * Lw + k + 1] - rw[c * Lw + k]);
-            dpdz[o] = -MPAS_GRAVITY * (m.rho_base[o] * qtot[o] + rr_save[o] * (1.0 + qtot[o]));
-            // theta del2 (dyn_tend.py:5563-5640)
-            double dsum = 0.0, tsum = 0.0;
-#pragma unroll
-            for (int j = 0; j < kMaxEdges; ++j) {
-                const int jj = clamp_slot(j, n);
-                const int e = m.edgesOnCell[c * ME + jj];
-                const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-                const double t2 = theta_m[c2 * L + k], t1 = theta_m[c1 * L + k], re = rho_edge[e * L + k];
-                const double k1 = kdiff[c1 * L + k], k2 = kdiff[c2 * L + k];
-                const double es = ia * m.edgesOnCell_sign[c * ME + jj] * m.dvEdge[e] * m.invDcEdge[e];
-                const double prs = P.prandtl_inv * m.meshScalingDel2[e];
-                const double bf = es * (t2 - t1) * re;
-                const double sf = bf * 0.5 * (k1 + k2) * prs;
-                dsum = (j < n) ? ((j == 0) ? bf : dsum + bf) : dsum;
-                tsum = (j < n) ? ((j == 0) ? sf : tsum + sf) : tsum;
+            if (k >= 1 && k < L) {
+                // w del2 (dyn_tend.py:5315-5420)
+                double dsum = 0.0, tsum = 0.0;
+    #pragma unroll
+                for (int j = 0; j < kMaxEdges; ++j) {
+                    const int jj = clamp_slot(j, n);
+                    const int e = m.edgesOnCell[c * ME + jj];
+                    const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+                    const double r0 = rho_edge[e * L + k], r1 = rho_edge[e * L + k - 1];
+                    const double w2 = w[c2 * Lw + k], w1 = w[c1 * Lw + k];
+                    const double k1 = kdiff[c1 * L + k], k2 = kdiff[c2 * L + k];
+                    const double k1m = kdiff[c1 * L + k - 1], k2m = kdiff[c2 * L + k - 1];
+                    const double es = 0.5 * ia * m.edgesOnCell_sign[c * ME + jj] * m.dvEdge[e] * m.invDcEdge[e];
+                    const double bf = es * (r0 + r1) * (w2 - w1);
+                    const double sf = bf * m.meshScalingDel2[e] * 0.25 * (k1 + k2 + k1m + k2m);
+                    dsum = (j < n) ? ((j == 0) ? bf : dsum + bf) : dsum;
+                    tsum = (j < n) ? ((j == 0) ? sf : tsum + sf) : tsum;
+                }
+                delsq_w[i] = dsum;
+                tend_w_euler[i] = tsum;
+            } else {
+                tend_w_euler[i] = 0.0;
             }
-            delsq_theta[o] = dsum;
-            tend_theta_euler[o] = tsum;
         }
-    }
-    if (rk1) {
-        if (k >= 1 && k < L) {
-            // w del2 (dyn_tend.py:5315-5420)
-            double dsum = 0.0, tsum = 0.0;
-#pragma unroll
-            for (int j = 0; j < kMaxEdges; ++j) {
-                const int jj = clamp_slot(j, n);
-                const int e = m.edgesOnCell[c * ME + jj];
-                const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-                const double r0 = rho_edge[e * L + k], r1 = rho_edge[e * L + k - 1];
-                const double w2 = w[c2 * Lw + k], w1 = w[c1 * Lw + k];
-                const double k1 = kdiff[c1 * L + k], k2 = kdiff[c2 * L + k];
-                const double k1m = kdiff[c1 * L + k - 1], k2m = kdiff[c2 * L + k - 1];
-                const double es = 0.5 * ia * m.edgesOnCell_sign[c * ME + jj] * m.dvEdge[e] * m.invDcEdge[e];
-                const double bf = es * (r0 + r1) * (w2 - w1);
-                const double sf = bf * m.meshScalingDel2[e] * 0.25 * (k1 + k2 + k1m + k2m);
-                dsum = (j < n) ? ((j == 0) ? bf : dsum + bf) : dsum;
-                tsum = (j < n) ? ((j == 0) ? sf : tsum + sf) : tsum;
-            }
-            delsq_w[i] = dsum;
-            tend_w_euler[i] = tsum;
-        } else {
-            tend_w_euler[i] = 0.0;
-        }
-    }
+    };
+    const int base_ = wave_entity_base(nb);
+#pragma unroll 1
+    for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
 // P4: normal-velocity tendency (vertical flux, PV flux, KE gradient,
@@ -165,74 +175,79 @@ __global__ __launch_bounds__(256) void k_dyn_edges_a(long long nb, mpas_hip_mesh
                               const double *__restrict__ divergence, const double *__restrict__ vorticity,
                               const double *__restrict__ kdiff, double *__restrict__ tend_u,
                               double *__restrict__ tend_u_euler, double *__restrict__ delsq_u) {
-    const int L = m.nLevels, nE = m.nEdges, ME2 = m.maxEdges2, Lw = L + 1;
-    const int e = wave_entity(nb), lane = wave_lane();
-    if (e > nE) return;
-    if (e == nE) {
-        if (lane < L) tend_u[e * L + lane] = 0.0;
-        return;
-    }
-    // lanes past the column repeat the top level for their loads, store nothing
-    const int k = lane < L ? lane : L - 1;
-    const int i = e * L + k;
-    const int ne = m.nEdgesOnEdge[e];
-    const int sl = lane < ne ? lane : ne - 1;
-    const int eev = m.edgesOnEdge[e * ME2 + sl];
-    const double wv = m.weightsOnEdge[e * ME2 + sl];
-    const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-    const double *ue = u + e * L;
-    const double *fzm = m.fzm, *fzp = m.fzp;
-    auto wduz = [&](int kk) -> double {
-        double erw = 0.5 * (rw[c1 * Lw + kk] + rw[c2 * Lw + kk]);
-        if (kk == 1 || kk == L - 1) return erw * (fzm[kk] * ue[kk] + fzp[kk] * ue[kk - 1]);
-        return flux3(ue[kk - 2], ue[kk - 1], ue[kk], ue[kk + 1], erw, 1.0);
+    auto body_ = [&](const int e) {
+        const int L = m.nLevels, nE = m.nEdges, ME2 = m.maxEdges2, Lw = L + 1;
+        const int lane = wave_lane();
+        if (e > nE) return;
+        if (e == nE) {
+            if (lane < L) tend_u[e * L + lane] = 0.0;
+            return;
+        }
+        // lanes past the column repeat the top level for their loads, store nothing
+        const int k = lane < L ? lane : L - 1;
+        const int i = e * L + k;
+        const int ne = m.nEdgesOnEdge[e];
+        const int sl = lane < ne ? lane : ne - 1;
+        const int eev = m.edgesOnEdge[e * ME2 + sl];
+        const double wv = m.weightsOnEdge[e * ME2 + sl];
+        const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+        const double *ue = u + e * L;
+        const double *fzm = m.fzm, *fzp = m.fzp;
+        auto wduz = [&](int kk) -> double {
+            double erw = 0.5 * (rw[c1 * Lw + kk] + rw[c2 * Lw + kk]);
+            if (kk == 1 || kk == L - 1) return erw * (fzm[kk] * ue[kk] + fzp[kk] * ue[kk - 1]);
+            return flux3(ue[kk - 2], ue[kk - 1], ue[kk], ue[kk + 1], erw, 1.0);
+        };
+        double delta;
+        if (L == 1) delta = 0.0;
+        else if (k == 0) delta = wduz(1);
+        else if (k == L - 1) delta = -wduz(L - 1);
+        else delta = wduz(k + 1) - wduz(k);
+        double vert = -m.rdzw[k] * delta;
+        double q = 0.0;
+        const double pvs = pv_edge[i];
+    #pragma unroll
+        for (int j0 = 0; j0 < kMaxEdges2; j0 += 8) {
+            if (j0 >= ne) break;
+            double pj[8], uj[8];
+    #pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int ee = bcast(eev, j0 + t);
+                pj[t] = pv_edge[ee * L + k];
+                uj[t] = u[ee * L + k];
+            }
+    #pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int j = j0 + t;
+                const double workpv = 0.5 * (pvs + pj[t]);
+                const double cc = bcast(wv, j) * uj[t] * workpv;
+                q = (j < ne) ? ((j == 0) ? cc : q + cc) : q;
+            }
+        }
+        double ke_delta = ke[c2 * L + k] - ke[c1 * L + k];
+        double hsum = h_div[c1 * L + k] + h_div[c2 * L + k];
+        double base = rho_edge[i] * (q - ke_delta * m.invDcEdge[e]) - ue[k] * 0.5 * hsum;
+        if (lane >= L) return;
+        tend_u[i] = vert + base;
+        if (P.rk_step == 1) {
+            double zzs = m.zz[c2 * L + k] + m.zz[c1 * L + k];
+            double ppd = pp[c2 * L + k] - pp[c1 * L + k];
+            double dps = dpdz[c1 * L + k] + dpdz[c2 * L + k];
+            double eu = -cqu[i] * (ppd * m.invDcEdge[e] / (0.5 * zzs) - 0.5 * m.zxu[i] * dps);
+            const int v1 = m.verticesOnEdge[2 * e], v2 = m.verticesOnEdge[2 * e + 1];
+            double r_dc = m.invDcEdge[e];
+            double r_dv = dmin(m.invDvEdge[e], 4.0 * m.invDcEdge[e]);
+            double ud = (divergence[c2 * L + k] - divergence[c1 * L + k]) * r_dc -
+                        (vorticity[v2 * L + k] - vorticity[v1 * L + k]) * r_dv;
+            delsq_u[i] = ud;
+            double kdiffu = 0.5 * (kdiff[c1 * L + k] + kdiff[c2 * L + k]);
+            eu = eu + rho_edge[i] * kdiffu * ud * m.meshScalingDel2[e];
+            tend_u_euler[i] = eu;
+        }
     };
-    double delta;
-    if (L == 1) delta = 0.0;
-    else if (k == 0) delta = wduz(1);
-    else if (k == L - 1) delta = -wduz(L - 1);
-    else delta = wduz(k + 1) - wduz(k);
-    double vert = -m.rdzw[k] * delta;
-    double q = 0.0;
-    const double pvs = pv_edge[i];
-#pragma unroll
-    for (int j0 = 0; j0 < kMaxEdges2; j0 += 8) {
-        if (j0 >= ne) break;
-        double pj[8], uj[8];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const int ee = bcast(eev, j0 + t);
-            pj[t] = pv_edge[ee * L + k];
-            uj[t] = u[ee * L + k];
-        }
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const int j = j0 + t;
-            const double workpv = 0.5 * (pvs + pj[t]);
-            const double cc = bcast(wv, j) * uj[t] * workpv;
-            q = (j < ne) ? ((j == 0) ? cc : q + cc) : q;
-        }
-    }
-    double ke_delta = ke[c2 * L + k] - ke[c1 * L + k];
-    double hsum = h_div[c1 * L + k] + h_div[c2 * L + k];
-    double base = rho_edge[i] * (q - ke_delta * m.invDcEdge[e]) - ue[k] * 0.5 * hsum;
-    if (lane >= L) return;
-    tend_u[i] = vert + base;
-    if (P.rk_step == 1) {
-        double zzs = m.zz[c2 * L + k] + m.zz[c1 * L + k];
-        double ppd = pp[c2 * L + k] - pp[c1 * L + k];
-        double dps = dpdz[c1 * L + k] + dpdz[c2 * L + k];
-        double eu = -cqu[i] * (ppd * m.invDcEdge[e] / (0.5 * zzs) - 0.5 * m.zxu[i] * dps);
-        const int v1 = m.verticesOnEdge[2 * e], v2 = m.verticesOnEdge[2 * e + 1];
-        double r_dc = m.invDcEdge[e];
-        double r_dv = dmin(m.invDvEdge[e], 4.0 * m.invDcEdge[e]);
-        double ud = (divergence[c2 * L + k] - divergence[c1 * L + k]) * r_dc -
-                    (vorticity[v2 * L + k] - vorticity[v1 * L + k]) * r_dv;
-        delsq_u[i] = ud;
-        double kdiffu = 0.5 * (kdiff[c1 * L + k] + kdiff[c2 * L + k]);
-        eu = eu + rho_edge[i] * kdiffu * ud * m.meshScalingDel2[e];
-        tend_u_euler[i] = eu;
-    }
+    const int base_ = wave_entity_base(nb);
+#pragma unroll 1
+    for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
 // P5: del2 of the u Laplacian at vertices and cells (del4 first half)
@@ -299,51 +314,56 @@ __global__ __launch_bounds__(256) void k_dyn_edge_adv(long long nb, mpas_hip_mes
                                const double *__restrict__ ru_save, const double *__restrict__ w,
                                const double *__restrict__ theta_m, const double *__restrict__ theta_m_save,
                                double *__restrict__ wflux, double *__restrict__ tflux, double *__restrict__ tpert) {
-    const int L = m.nLevels, nE = m.nEdges, MA = m.maxAdv, Lw = L + 1;
-    const int e = wave_entity(nb), lane = wave_lane();
-    if (e >= nE) return;
-    const int k = lane < L ? lane : L - 1;  // lanes past the column: loads only
-    const int i = e * L + k;
-    const int na = m.nAdvCellsForEdge[e];
-    const int sl = lane < na ? lane : na - 1;
-    const int advv = m.advCellsForEdge[e * MA + sl];
-    const double c0v = m.adv_coefs[e * MA + sl], c3v = m.adv_coefs_3rd[e * MA + sl];
-    const double *rue_col = ru + e * L;
-    const double rut = rue_col[k];
-    const double srt = sign_pm(rut);
-    const bool wl = (k >= 1);
-    const double ruw = wl ? m.fzm[k] * rue_col[k] + m.fzp[k] * rue_col[k - 1] : 0.0;
-    const double srw = sign_pm(ruw);
-    double ft = 0.0, fw = 0.0;
-#pragma unroll
-    for (int a0 = 0; a0 < kMaxAdv; a0 += 8) {
-        if (a0 >= na) break;
-        double qt[8], qw[8];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const int ac = bcast(advv, a0 + t);
-            qt[t] = theta_m[ac * L + k];
-            qw[t] = w[ac * Lw + k];
+    auto body_ = [&](const int e) {
+        const int L = m.nLevels, nE = m.nEdges, MA = m.maxAdv, Lw = L + 1;
+        const int lane = wave_lane();
+        if (e >= nE) return;
+        const int k = lane < L ? lane : L - 1;  // lanes past the column: loads only
+        const int i = e * L + k;
+        const int na = m.nAdvCellsForEdge[e];
+        const int sl = lane < na ? lane : na - 1;
+        const int advv = m.advCellsForEdge[e * MA + sl];
+        const double c0v = m.adv_coefs[e * MA + sl], c3v = m.adv_coefs_3rd[e * MA + sl];
+        const double *rue_col = ru + e * L;
+        const double rut = rue_col[k];
+        const double srt = sign_pm(rut);
+        const bool wl = (k >= 1);
+        const double ruw = wl ? m.fzm[k] * rue_col[k] + m.fzp[k] * rue_col[k - 1] : 0.0;
+        const double srw = sign_pm(ruw);
+        double ft = 0.0, fw = 0.0;
+    #pragma unroll
+        for (int a0 = 0; a0 < kMaxAdv; a0 += 8) {
+            if (a0 >= na) break;
+            double qt[8], qw[8];
+    #pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int ac = bcast(advv, a0 + t);
+                qt[t] = theta_m[ac * L + k];
+                qw[t] = w[ac * Lw + k];
+            }
+    #pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int a = a0 + t;
+                const double c0 = bcast(c0v, a), c3 = bcast(c3v, a);
+                const double ct = (c0 + srt * c3) * qt[t];
+                const double cw = (c0 + srw * c3) * qw[t];
+                ft = (a < na) ? ((a == 0) ? ct : ft + ct) : ft;
+                fw = (a < na) ? ((a == 0) ? cw : fw + cw) : fw;
+            }
         }
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const int a = a0 + t;
-            const double c0 = bcast(c0v, a), c3 = bcast(c3v, a);
-            const double ct = (c0 + srt * c3) * qt[t];
-            const double cw = (c0 + srw * c3) * qw[t];
-            ft = (a < na) ? ((a == 0) ? ct : ft + ct) : ft;
-            fw = (a < na) ? ((a == 0) ? cw : fw + cw) : fw;
+        if (!wl) fw = 0.0;
+        if (lane >= L) return;
+        tflux[i] = rut * ft;
+        wflux[i] = ruw * fw;
+        if (P.rk_step != 1) {
+            const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+            const double face = 0.5 * (theta_m_save[c2 * L + k] + theta_m_save[c1 * L + k]);
+            tpert[i] = m.dvEdge[e] * (ru_save[i] - rut) * face;
         }
-    }
-    if (!wl) fw = 0.0;
-    if (lane >= L) return;
-    tflux[i] = rut * ft;
-    wflux[i] = ruw * fw;
-    if (P.rk_step != 1) {
-        const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-        const double face = 0.5 * (theta_m_save[c2 * L + k] + theta_m_save[c1 * L + k]);
-        tpert[i] = m.dvEdge[e] * (ru_save[i] - rut) * face;
-    }
+    };
+    const int base_ = wave_entity_base(nb);
+#pragma unroll 1
+    for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
 // P7: w and theta tendencies (advection, mixing finish, PGF/buoyancy,
@@ -360,121 +380,126 @@ __global__ __launch_bounds__(256) void k_dyn_cells_b(long long nb, mpas_hip_mesh
                               double *__restrict__ tend_w_euler, double *__restrict__ tend_theta_euler,
                               double *__restrict__ tend_w, double *__restrict__ tend_theta,
                               double *__restrict__ rthdynten) {
-    const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
-    const int c = wave_entity(nb), k = wave_lane();
-    if (c > nC || k >= Lw) return;
-    const int i = c * Lw + k;
-    if (c == nC) {
-        tend_w[i] = 0.0;
-        if (k < L) {
-            tend_theta[c * L + k] = 0.0;
-            rthdynten[c * L + k] = 0.0;
+    auto body_ = [&](const int c) {
+        const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
+        const int k = wave_lane();
+        if (c > nC || k >= Lw) return;
+        const int i = c * Lw + k;
+        if (c == nC) {
+            tend_w[i] = 0.0;
+            if (k < L) {
+                tend_theta[c * L + k] = 0.0;
+                rthdynten[c * L + k] = 0.0;
+            }
+            return;
         }
-        return;
-    }
-    const bool rk1 = (P.rk_step == 1);
-    const int n = m.nEdgesOnCell[c];
-    const double ia = m.invAreaCell[c];
-    const double *fzm = m.fzm, *fzp = m.fzp;
-    int eo[kMaxEdges];
-    double sg[kMaxEdges];
-#pragma unroll
-    for (int j = 0; j < kMaxEdges; ++j) {
-        const int jj = clamp_slot(j, n);
-        eo[j] = m.edgesOnCell[c * ME + jj];
-        sg[j] = m.edgesOnCell_sign[c * ME + jj];
-    }
-    // ---------------- w at interface k
-    if (k >= 1 && k < L) {
-        double hadv = 0.0;
-#pragma unroll
+        const bool rk1 = (P.rk_step == 1);
+        const int n = m.nEdgesOnCell[c];
+        const double ia = m.invAreaCell[c];
+        const double *fzm = m.fzm, *fzp = m.fzp;
+        int eo[kMaxEdges];
+        double sg[kMaxEdges];
+    #pragma unroll
         for (int j = 0; j < kMaxEdges; ++j) {
-            const double f = wflux[eo[j] * L + k];
-            const double cc = -sg[j] * f;
-            hadv = (j < n) ? ((j == 0) ? cc : hadv + cc) : hadv;
+            const int jj = clamp_slot(j, n);
+            eo[j] = m.edgesOnCell[c * ME + jj];
+            sg[j] = m.edgesOnCell_sign[c * ME + jj];
         }
-        double eu = tend_w_euler[i];
-        if (rk1) {
-            if (P.h_mom_eddy_visc4 > 0.0) {
-                double r_area = P.h_mom_eddy_visc4 * ia;
-#pragma unroll
+        // ---------------- w at interface k
+        if (k >= 1 && k < L) {
+            double hadv = 0.0;
+    #pragma unroll
+            for (int j = 0; j < kMaxEdges; ++j) {
+                const double f = wflux[eo[j] * L + k];
+                const double cc = -sg[j] * f;
+                hadv = (j < n) ? ((j == 0) ? cc : hadv + cc) : hadv;
+            }
+            double eu = tend_w_euler[i];
+            if (rk1) {
+                if (P.h_mom_eddy_visc4 > 0.0) {
+                    double r_area = P.h_mom_eddy_visc4 * ia;
+    #pragma unroll
+                    for (int j = 0; j < kMaxEdges; ++j) {
+                        const int e = eo[j];
+                        const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+                        const double d2 = delsq_w[c2 * Lw + k], d1 = delsq_w[c1 * Lw + k];
+                        const double es = m.meshScalingDel4[e] * r_area * m.dvEdge[e] * sg[j] * m.invDcEdge[e];
+                        const double nx = eu - es * (d2 - d1);
+                        eu = (j < n) ? nx : eu;
+                    }
+                }
+                const int oc = c * L;
+                eu = eu - cqw[i] * (m.rdzu[k] * (pp[oc + k] - pp[oc + k - 1]) -
+                                    (fzm[k] * dpdz[oc + k] + fzp[k] * dpdz[oc + k - 1]));
+                tend_w_euler[i] = eu;
+            }
+            const double *wc = w + c * Lw, *rwc = rw + c * Lw;
+            auto wdwz = [&](int jj) -> double {
+                if (jj == 0 || jj == L - 2) return 0.25 * (rwc[jj + 1] + rwc[jj]) * (wc[jj + 1] + wc[jj]);
+                return flux3(wc[jj - 1], wc[jj], wc[jj + 1], wc[jj + 2], 0.5 * (rwc[jj + 1] + rwc[jj]), 1.0);
+            };
+            double delta = (k == L - 1) ? -wdwz(L - 2) : wdwz(k) - wdwz(k - 1);
+            double inner = hadv * ia - m.rdzu[k] * delta;
+            tend_w[i] = inner + eu;
+        } else {
+            tend_w[i] = 0.0;
+        }
+        // ---------------- theta at level k
+        if (k < L) {
+            const int o = c * L + k;
+            double thadv = 0.0;
+    #pragma unroll
+            for (int j = 0; j < kMaxEdges; ++j) {
+                const double f = tflux[eo[j] * L + k];
+                const double cc = -sg[j] * f;
+                thadv = (j < n) ? ((j == 0) ? cc : thadv + cc) : thadv;
+            }
+            if (!rk1) {
+    #pragma unroll
+                for (int j = 0; j < kMaxEdges; ++j) {
+                    const double f = tpert[eo[j] * L + k];
+                    const double nx = thadv - sg[j] * f;
+                    thadv = (j < n) ? nx : thadv;
+                }
+            }
+            double tte = tend_theta_euler[o];
+            if (rk1 && P.h_theta_eddy_visc4 > 0.0) {
+                double r_area = P.h_theta_eddy_visc4 * P.prandtl_inv * ia;
+    #pragma unroll
                 for (int j = 0; j < kMaxEdges; ++j) {
                     const int e = eo[j];
                     const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-                    const double d2 = delsq_w[c2 * Lw + k], d1 = delsq_w[c1 * Lw + k];
+                    const double d2 = delsq_theta[c2 * L + k], d1 = delsq_theta[c1 * L + k];
                     const double es = m.meshScalingDel4[e] * r_area * m.dvEdge[e] * sg[j] * m.invDcEdge[e];
-                    const double nx = eu - es * (d2 - d1);
-                    eu = (j < n) ? nx : eu;
+                    const double nx = tte - es * (d2 - d1);
+                    tte = (j < n) ? nx : tte;
                 }
+                tend_theta_euler[o] = tte;
             }
-            const int oc = c * L;
-            eu = eu - cqw[i] * (m.rdzu[k] * (pp[oc + k] - pp[oc + k - 1]) -
-                                (fzm[k] * dpdz[oc + k] + fzp[k] * dpdz[oc + k - 1]));
-            tend_w_euler[i] = eu;
-        }
-        const double *wc = w + c * Lw, *rwc = rw + c * Lw;
-        auto wdwz = [&](int jj) -> double {
-            if (jj == 0 || jj == L - 2) return 0.25 * (rwc[jj + 1] + rwc[jj]) * (wc[jj + 1] + wc[jj]);
-            return flux3(wc[jj - 1], wc[jj], wc[jj + 1], wc[jj + 2], 0.5 * (rwc[jj + 1] + rwc[jj]), 1.0);
-        };
-        double delta = (k == L - 1) ? -wdwz(L - 2) : wdwz(k) - wdwz(k - 1);
-        double inner = hadv * ia - m.rdzu[k] * delta;
-        tend_w[i] = inner + eu;
-    } else {
-        tend_w[i] = 0.0;
-    }
-    // ---------------- theta at level k
-    if (k < L) {
-        const int o = c * L + k;
-        double thadv = 0.0;
-#pragma unroll
-        for (int j = 0; j < kMaxEdges; ++j) {
-            const double f = tflux[eo[j] * L + k];
-            const double cc = -sg[j] * f;
-            thadv = (j < n) ? ((j == 0) ? cc : thadv + cc) : thadv;
-        }
-        if (!rk1) {
-#pragma unroll
-            for (int j = 0; j < kMaxEdges; ++j) {
-                const double f = tpert[eo[j] * L + k];
-                const double nx = thadv - sg[j] * f;
-                thadv = (j < n) ? nx : thadv;
-            }
-        }
-        double tte = tend_theta_euler[o];
-        if (rk1 && P.h_theta_eddy_visc4 > 0.0) {
-            double r_area = P.h_theta_eddy_visc4 * P.prandtl_inv * ia;
-#pragma unroll
-            for (int j = 0; j < kMaxEdges; ++j) {
-                const int e = eo[j];
-                const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-                const double d2 = delsq_theta[c2 * L + k], d1 = delsq_theta[c1 * L + k];
-                const double es = m.meshScalingDel4[e] * r_area * m.dvEdge[e] * sg[j] * m.invDcEdge[e];
-                const double nx = tte - es * (d2 - d1);
-                tte = (j < n) ? nx : tte;
-            }
-            tend_theta_euler[o] = tte;
-        }
-        const double *th = theta_m + c * L, *ths = theta_m_save + c * L;
-        const double *rwc = rw + c * Lw, *rws = rw_save + c * Lw;
-        auto wdtz = [&](int kk) -> double {
-            if (kk == 1)
-                return rwc[kk] * (fzm[kk] * th[kk] + fzp[kk] * th[kk - 1]) +
+            const double *th = theta_m + c * L, *ths = theta_m_save + c * L;
+            const double *rwc = rw + c * Lw, *rws = rw_save + c * Lw;
+            auto wdtz = [&](int kk) -> double {
+                if (kk == 1)
+                    return rwc[kk] * (fzm[kk] * th[kk] + fzp[kk] * th[kk - 1]) +
+                           (rws[kk] - rwc[kk]) * (fzm[kk] * ths[kk] + fzp[kk] * ths[kk - 1]);
+                if (kk == L - 1) return rws[kk] * (fzm[kk] * th[kk] + fzp[kk] * th[kk - 1]);
+                return flux3(th[kk - 2], th[kk - 1], th[kk], th[kk + 1], rwc[kk], P.coef_3rd_order) +
                        (rws[kk] - rwc[kk]) * (fzm[kk] * ths[kk] + fzp[kk] * ths[kk - 1]);
-            if (kk == L - 1) return rws[kk] * (fzm[kk] * th[kk] + fzp[kk] * th[kk - 1]);
-            return flux3(th[kk - 2], th[kk - 1], th[kk], th[kk + 1], rwc[kk], P.coef_3rd_order) +
-                   (rws[kk] - rwc[kk]) * (fzm[kk] * ths[kk] + fzp[kk] * ths[kk - 1]);
-        };
-        double delta;
-        if (L == 1) delta = 0.0;
-        else if (k == 0) delta = wdtz(1);
-        else if (k == L - 1) delta = -wdtz(L - 1);
-        else delta = wdtz(k + 1) - wdtz(k);
-        double base = thadv * ia - m.rdzw[k] * delta;
-        rthdynten[o] = (base - tend_rho[o] * th[k]) / rho_zz[o];
-        double tt = base + rho_zz[o] * rt_diab[o];
-        tend_theta[o] = tt + tte;
-    }
+            };
+            double delta;
+            if (L == 1) delta = 0.0;
+            else if (k == 0) delta = wdtz(1);
+            else if (k == L - 1) delta = -wdtz(L - 1);
+            else delta = wdtz(k + 1) - wdtz(k);
+            double base = thadv * ia - m.rdzw[k] * delta;
+            rthdynten[o] = (base - tend_rho[o] * th[k]) / rho_zz[o];
+            double tt = base + rho_zz[o] * rt_diab[o];
+            tend_theta[o] = tt + tte;
+        }
+    };
+    const int base_ = wave_entity_base(nb);
+#pragma unroll 1
+    for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
 void launch_dyn_tend(const mpas_hip_mesh &m, const DynParams &P, const DynInputs &in, const DynCarried &io,
