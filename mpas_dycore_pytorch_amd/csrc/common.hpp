@@ -166,8 +166,14 @@ struct AcousticInputs {
 struct AcousticState {
     double *ru_p, *ruAvg, *rho_pp, *rtheta_pp, *rtheta_pp_old, *rw_p, *wwAvg;
 };
+// Divergence damping carried into a later edge pass (on = 0: none).
+struct DampFold {
+    int on;
+    double coef;
+    const double *theta_m, *rtheta_pp, *rtheta_pp_old;
+};
 void launch_acoustic(const mpas_hip_mesh &m, const AcousticParams &P, const AcousticInputs &in,
-                     const AcousticState &io, hipStream_t s);
+                     const AcousticState &io, hipStream_t s, const DampFold &fold);
 void launch_divdamp(const mpas_hip_mesh &m, double dts, double smdiv, double len_disp,
                     const double *theta_m, const double *rtheta_pp, const double *rtheta_pp_old,
                     double *ru_p, hipStream_t s);
@@ -180,7 +186,7 @@ struct RecoverOutputs {
     double *wwAvg, *ruAvg, *rho_p, *rho_zz, *rtheta_p, *theta_m, *exner, *pressure_p, *rw, *w, *ru, *u;
 };
 void launch_recover(const mpas_hip_mesh &m, int rk_step, double dt, int ns, const RecoverInputs &in,
-                    const RecoverOutputs &out, hipStream_t s);
+                    const RecoverOutputs &out, hipStream_t s, const DampFold &fold);
 
 void launch_advance_scalars(const mpas_hip_mesh &m, int S, double dt, double coef3, double w_old,
                             double w_new, const double *scalar_old, const double *scalar_new_in,

@@ -41,10 +41,27 @@ void launch_smlstep(const mpas_hip_mesh &m, const double *u_tend, double *w_tend
 // atm_advance_acoustic_step_work (acoustic_step.py:456-889, 1297-1951)
 // Edge pass: pressure-gradient update of ru_p and ruAvg (:576-649).
 // ---------------------------------------------------------------------------
+// Divergence-damping increment of ru_p at (edge, level) (divergence_damping.py:187-209):
+// coef (d2 - d1) / (theta1 + theta2), d = -(rtheta_pp - rtheta_pp_old), zero
+// when |theta1 + theta2| < 1e-30.  Shared by the stand-alone damping pass and
+// the passes that fold it in (the next substep's edge pass, the recovery).
+__device__ __forceinline__ double divdamp_increment(double coef, const double *__restrict__ theta_m,
+                                                   const double *__restrict__ rtheta_pp,
+                                                   const double *__restrict__ rtheta_pp_old, int c1, int c2,
+                                                   int L, int k) {
+    double d1 = -(rtheta_pp[c1 * L + k] - rtheta_pp_old[c1 * L + k]);
+    double d2 = -(rtheta_pp[c2 * L + k] - rtheta_pp_old[c2 * L + k]);
+    double ts = theta_m[c1 * L + k] + theta_m[c2 * L + k];
+    return (fabs(ts) >= 1.0e-30) ? coef * (d2 - d1) / ts : 0.0;
+}
+
+// With fold.on, the damping of the previous acoustic substep is applied to
+// ru_p first (same rounding sequence as the stand-alone pass followed by
+// this one).
 __global__ __launch_bounds__(256) void k_acoustic_edges(long long nb, mpas_hip_mesh m, AcousticParams P, const double *__restrict__ rtheta_pp,
                                  const double *__restrict__ rho_pp, const double *__restrict__ exner,
                                  const double *__restrict__ cqu, const double *__restrict__ tend_ru,
-                                 double *__restrict__ ru_p, double *__restrict__ ruAvg) {
+                                 double *__restrict__ ru_p, double *__restrict__ ruAvg, DampFold fold) {
     auto body_ = [&](const int e) {
         const int L = m.nLevels, nE = m.nEdges;
         const int k = wave_lane();
@@ -58,11 +75,13 @@ __global__ __launch_bounds__(256) void k_acoustic_edges(long long nb, mpas_hip_m
             return;
         }
         int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+        double rup = ru_p[i];
+        if (fold.on) rup = rup + divdamp_increment(fold.coef, fold.theta_m, rtheta_pp, fold.rtheta_pp_old, c1, c2, L, k);
         double pg = ((rtheta_pp[c2 * L + k] - rtheta_pp[c1 * L + k]) * m.invDcEdge[e]) /
                     (0.5 * (m.zz[c2 * L + k] + m.zz[c1 * L + k]));
         pg = cqu[i] * 0.5 * P.c2 * (exner[c1 * L + k] + exner[c2 * L + k]) * pg;
         pg = pg + 0.5 * m.zxu[i] * MPAS_GRAVITY * (rho_pp[c1 * L + k] + rho_pp[c2 * L + k]);
-        double r = ru_p[i] + dts * (tend_ru[i] - pg);
+        double r = rup + dts * (tend_ru[i] - pg);
         ru_p[i] = r;
         ruAvg[i] = ruAvg[i] + r;
     };
@@ -202,9 +221,9 @@ __global__ __launch_bounds__(64 * kAcWaves) void k_acoustic_cells(mpas_hip_mesh 
 }
 
 void launch_acoustic(const mpas_hip_mesh &m, const AcousticParams &P, const AcousticInputs &in,
-                     const AcousticState &io, hipStream_t s) {
+                     const AcousticState &io, hipStream_t s, const DampFold &fold) {
     MPAS_WAVE_LAUNCH(k_acoustic_edges, m.nEdges, s, m,
-                       P, io.rtheta_pp, io.rho_pp, in.exner, in.cqu, in.tend_ru, io.ru_p, io.ruAvg);
+                       P, io.rtheta_pp, io.rho_pp, in.exner, in.cqu, in.tend_ru, io.ru_p, io.ruAvg, fold);
     MPAS_LAUNCH(k_acoustic_cells, dim3((m.nCells + kAcWaves - 1) / kAcWaves), dim3(64, kAcWaves), 0, s, m,
                        P, in, io);
 }
@@ -221,11 +240,7 @@ __global__ __launch_bounds__(256) void k_divdamp(long long nb, mpas_hip_mesh m, 
         if (e >= nE || k >= L) return;
         const int i = e * L + k;
         int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-        double d1 = -(rtheta_pp[c1 * L + k] - rtheta_pp_old[c1 * L + k]);
-        double d2 = -(rtheta_pp[c2 * L + k] - rtheta_pp_old[c2 * L + k]);
-        double ts = theta_m[c1 * L + k] + theta_m[c2 * L + k];
-        double damp = (fabs(ts) >= 1.0e-30) ? coef * (d2 - d1) / ts : 0.0;
-        ru_p[i] = ru_p[i] + damp;
+        ru_p[i] = ru_p[i] + divdamp_increment(coef, theta_m, rtheta_pp, rtheta_pp_old, c1, c2, L, k);
     };
     const int base_ = wave_entity_base(nb);
 #pragma unroll 1
@@ -280,7 +295,8 @@ __global__ void k_recover_cells(mpas_hip_mesh m, int rk_step, double dt, double 
 
 __global__ __launch_bounds__(256) void k_recover_edges(long long nb, mpas_hip_mesh m, double invNs, const double *__restrict__ ru_save,
                                 const double *__restrict__ ru_p, const double *__restrict__ rho_zz,
-                                double *__restrict__ ruAvg, double *__restrict__ ru, double *__restrict__ u) {
+                                double *__restrict__ ruAvg, double *__restrict__ ru, double *__restrict__ u,
+                                DampFold fold) {
     auto body_ = [&](const int e) {
         const int L = m.nLevels, nE = m.nEdges;
         const int k = wave_lane();
@@ -288,7 +304,10 @@ __global__ __launch_bounds__(256) void k_recover_edges(long long nb, mpas_hip_me
         const int i = e * L + k;
         int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
         ruAvg[i] = ru_save[i] + (ruAvg[i] * invNs);
-        double rn = ru_save[i] + ru_p[i];
+        double rup = ru_p[i];
+        if (fold.on)
+            rup = rup + divdamp_increment(fold.coef, fold.theta_m, fold.rtheta_pp, fold.rtheta_pp_old, c1, c2, L, k);
+        double rn = ru_save[i] + rup;
         ru[i] = rn;
         u[i] = 2.0 * rn / (rho_zz[c1 * L + k] + rho_zz[c2 * L + k]);
     };
@@ -344,13 +363,13 @@ __global__ __launch_bounds__(256) void k_recover_w(long long nb, mpas_hip_mesh m
 }
 
 void launch_recover(const mpas_hip_mesh &m, int rk_step, double dt, int ns, const RecoverInputs &in,
-                    const RecoverOutputs &out, hipStream_t s) {
+                    const RecoverOutputs &out, hipStream_t s, const DampFold &fold) {
     const int L = m.nLevels, nC = m.nCells, nE = m.nEdges;
     const double invNs = 1.0 / (double)ns;
     MPAS_LAUNCH(k_recover_cells, dim3(blocks_for((long long)(nC + 1) * (L + 1))), dim3(kBlock), 0, s, m,
                        rk_step, dt, invNs, in, out);
     MPAS_WAVE_LAUNCH(k_recover_edges, nE, s, m, invNs,
-                       in.ru_save, in.ru_p, out.rho_zz, out.ruAvg, out.ru, out.u);
+                       in.ru_save, in.ru_p, out.rho_zz, out.ruAvg, out.ru, out.u, fold);
     MPAS_WAVE_LAUNCH(k_recover_w, nC, s, m, out.ru,
                        out.rw, out.rho_zz, out.w);
 }

@@ -1,41 +1,21 @@
 // atm_compute_solve_diagnostics_work (solve_diagnostics.py:531-756):
 // h_edge, kinetic energy with the 0.375 vertex blend, vorticity, divergence,
 // APVM-upwinded pv_edge and (rk_step 3 / first call) the TRiSK tangential
-// velocity.  Four gather passes, one thread per (entity, level); every slot
-// sum runs left to right in the reference's slot order.
+// velocity.  Three gather passes (vertices, cells, edges), one wavefront per
+// entity with lane = level; every slot sum runs left to right in the
+// reference's slot order.  The edge kinetic energy dc dv u^2 is recomputed
+// where it is used instead of being stored.
 #include "common.hpp"
 
 namespace mpas {
 
 size_t diagnostics_scratch_doubles(const mpas_hip_mesh &m) {
     const size_t L = m.nLevels;
-    return (size_t)m.nEdges * L + 2 * (size_t)m.nVertices * L + (size_t)m.nCells * L;
-}
-
-__global__ __launch_bounds__(256) void k_diag_edges1(long long nb, mpas_hip_mesh m, const double *__restrict__ u, const double *__restrict__ h,
-                              double *__restrict__ h_edge, double *__restrict__ ke_edge) {
-    auto body_ = [&](const int e) {
-        const int L = m.nLevels, nE = m.nEdges;
-        const int k = wave_lane();
-        if (e > nE || k >= L) return;
-        const int i = e * L + k;
-        if (e == nE) {
-            h_edge[i] = 0.0;
-            return;
-        }
-        int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-        h_edge[i] = 0.5 * (h[c1 * L + k] + h[c2 * L + k]);
-        double efac = m.dcEdge[e] * m.dvEdge[e];
-        double uu = u[i];
-        ke_edge[i] = efac * (uu * uu);
-    };
-    const int base_ = wave_entity_base(nb);
-#pragma unroll 1
-    for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
+    return 2 * (size_t)m.nVertices * L + (size_t)m.nCells * L;
 }
 
 __global__ __launch_bounds__(256) void k_diag_vertices(long long nb, mpas_hip_mesh m, const double *__restrict__ u,
-                                const double *__restrict__ ke_edge, double *__restrict__ vort,
+                                double *__restrict__ vort,
                                 double *__restrict__ ke_vertex, double *__restrict__ pv_vertex) {
     auto body_ = [&](const int vx) {
         const int L = m.nLevels, nV = m.nVertices;
@@ -52,7 +32,8 @@ __global__ __launch_bounds__(256) void k_diag_vertices(long long nb, mpas_hip_me
             int e = m.edgesOnVertex[3 * vx + j];
             double w = m.edgesOnVertex_sign[3 * vx + j] * m.dcEdge[e];
             double c = w * u[e * L + k];
-            double kk = ke_edge[e * L + k];
+            const double ue = u[e * L + k];
+            double kk = (m.dcEdge[e] * m.dvEdge[e]) * (ue * ue);  // ke_edge of the edge
             s = (j == 0) ? c : s + c;
             sk = (j == 0) ? kk : sk + kk;
         }
@@ -66,7 +47,7 @@ __global__ __launch_bounds__(256) void k_diag_vertices(long long nb, mpas_hip_me
     for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
-__global__ __launch_bounds__(256) void k_diag_cells(long long nb, mpas_hip_mesh m, const double *__restrict__ u, const double *__restrict__ ke_edge,
+__global__ __launch_bounds__(256) void k_diag_cells(long long nb, mpas_hip_mesh m, const double *__restrict__ u,
                              const double *__restrict__ ke_vertex, const double *__restrict__ pv_vertex,
                              double *__restrict__ div, double *__restrict__ ke, double *__restrict__ pv_cell) {
     auto body_ = [&](const int c) {
@@ -87,7 +68,7 @@ __global__ __launch_bounds__(256) void k_diag_cells(long long nb, mpas_hip_mesh 
             const int jj = clamp_slot(j, n);
             const int e = m.edgesOnCell[c * ME + jj];
             const double w = m.edgesOnCell_sign[c * ME + jj] * m.dvEdge[e];
-            const double ue = u[e * L + k], kee = ke_edge[e * L + k];
+            const double ue = u[e * L + k], kee = (m.dcEdge[e] * m.dvEdge[e]) * (ue * ue);
             const double cd = w * ue;
             const double ck = 0.25 * kee;
             d = (j < n) ? ((j == 0) ? cd : d + cd) : d;
@@ -122,6 +103,7 @@ __global__ __launch_bounds__(256) void k_diag_cells(long long nb, mpas_hip_mesh 
 __global__ __launch_bounds__(256) void k_diag_edges2(long long nb, mpas_hip_mesh m, double apvm_dt, int apvm_on, int reconstruct_v,
                               const double *__restrict__ u, const double *__restrict__ v_in,
                               const double *__restrict__ pv_vertex, const double *__restrict__ pv_cell,
+                              const double *__restrict__ h, double *__restrict__ h_edge,
                               double *__restrict__ v, double *__restrict__ pv_edge) {
     auto body_ = [&](const int e) {
         const int L = m.nLevels, nE = m.nEdges, ME2 = m.maxEdges2;
@@ -129,6 +111,7 @@ __global__ __launch_bounds__(256) void k_diag_edges2(long long nb, mpas_hip_mesh
         if (e > nE) return;
         if (e == nE) {
             if (lane < L) {
+                h_edge[e * L + lane] = 0.0;
                 v[e * L + lane] = 0.0;
                 pv_edge[e * L + lane] = 0.0;
             }
@@ -164,6 +147,10 @@ __global__ __launch_bounds__(256) void k_diag_edges2(long long nb, mpas_hip_mesh
         int v1 = m.verticesOnEdge[2 * e], v2 = m.verticesOnEdge[2 * e + 1];
         double p1 = pv_vertex[v1 * L + k], p2 = pv_vertex[v2 * L + k];
         double pe = 0.5 * (p1 + p2);
+        {
+            const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+            h_edge[i] = 0.5 * (h[c1 * L + k] + h[c2 * L + k]);
+        }
         if (apvm_on) {
             int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
             double grad_pvt = (p2 - p1) * m.invDvEdge[e];
@@ -182,21 +169,17 @@ void launch_diagnostics(const mpas_hip_mesh &m, double dt, double apvm, int rk_s
                         const double *h, const double *v_in, double *h_edge, double *v, double *div,
                         double *vort, double *ke, double *pv_edge, double *scratch, hipStream_t s) {
     const int L = m.nLevels, nC = m.nCells, nE = m.nEdges, nV = m.nVertices;
-    double *ke_edge = scratch;
-    double *ke_vertex = ke_edge + (size_t)nE * L;
+    double *ke_vertex = scratch;
     double *pv_vertex = ke_vertex + (size_t)nV * L;
     double *pv_cell = pv_vertex + (size_t)nV * L;
-    const unsigned be = blocks_for((long long)(nE + 1) * L), bv = blocks_for((long long)(nV + 1) * L),
-                   bc = blocks_for((long long)(nC + 1) * L);
-    // ke_edge / ke_vertex / pv_vertex / pv_cell have no padding row: the
-    // padding-row threads of the pass that fills them only zero the outputs.
-    MPAS_WAVE_LAUNCH(k_diag_edges1, nE + 1, s, m, u, h, h_edge, ke_edge);
-    MPAS_WAVE_LAUNCH(k_diag_vertices, nV + 1, s, m, u, ke_edge, vort, ke_vertex, pv_vertex);
-    MPAS_WAVE_LAUNCH(k_diag_cells, nC + 1, s, m, u, ke_edge, ke_vertex, pv_vertex, div, ke,
+    // ke_vertex / pv_vertex / pv_cell have no padding row: the padding-row
+    // wavefronts of the pass that fills them only zero that pass's outputs.
+    MPAS_WAVE_LAUNCH(k_diag_vertices, nV + 1, s, m, u, vort, ke_vertex, pv_vertex);
+    MPAS_WAVE_LAUNCH(k_diag_cells, nC + 1, s, m, u, ke_vertex, pv_vertex, div, ke,
                        pv_cell);
     const int reconstruct_v = (rk_step <= 0 || rk_step == 3) ? 1 : 0;
     MPAS_WAVE_LAUNCH(k_diag_edges2, nE + 1, s, m, apvm * dt, apvm > 0.0 ? 1 : 0,
-                       reconstruct_v, u, v_in, pv_vertex, pv_cell, v, pv_edge);
+                       reconstruct_v, u, v_in, pv_vertex, pv_cell, h, h_edge, v, pv_edge);
 }
 
 }  // namespace mpas

@@ -174,7 +174,7 @@ __global__ __launch_bounds__(256) void k_dyn_edges_a(long long nb, mpas_hip_mesh
                               const double *__restrict__ dpdz, const double *__restrict__ cqu,
                               const double *__restrict__ divergence, const double *__restrict__ vorticity,
                               const double *__restrict__ kdiff, double *__restrict__ tend_u,
-                              double *__restrict__ tend_u_euler, double *__restrict__ delsq_u) {
+                              double *__restrict__ tend_u_euler, double *__restrict__ delsq_u, int add_euler) {
     auto body_ = [&](const int e) {
         const int L = m.nLevels, nE = m.nEdges, ME2 = m.maxEdges2, Lw = L + 1;
         const int lane = wave_lane();
@@ -228,12 +228,13 @@ __global__ __launch_bounds__(256) void k_dyn_edges_a(long long nb, mpas_hip_mesh
         double hsum = h_div[c1 * L + k] + h_div[c2 * L + k];
         double base = rho_edge[i] * (q - ke_delta * m.invDcEdge[e]) - ue[k] * 0.5 * hsum;
         if (lane >= L) return;
-        tend_u[i] = vert + base;
+        const double tu = vert + base;
+        double eu = 0.0;
         if (P.rk_step == 1) {
             double zzs = m.zz[c2 * L + k] + m.zz[c1 * L + k];
             double ppd = pp[c2 * L + k] - pp[c1 * L + k];
             double dps = dpdz[c1 * L + k] + dpdz[c2 * L + k];
-            double eu = -cqu[i] * (ppd * m.invDcEdge[e] / (0.5 * zzs) - 0.5 * m.zxu[i] * dps);
+            eu = -cqu[i] * (ppd * m.invDcEdge[e] / (0.5 * zzs) - 0.5 * m.zxu[i] * dps);
             const int v1 = m.verticesOnEdge[2 * e], v2 = m.verticesOnEdge[2 * e + 1];
             double r_dc = m.invDcEdge[e];
             double r_dv = dmin(m.invDvEdge[e], 4.0 * m.invDcEdge[e]);
@@ -243,7 +244,12 @@ __global__ __launch_bounds__(256) void k_dyn_edges_a(long long nb, mpas_hip_mesh
             double kdiffu = 0.5 * (kdiff[c1 * L + k] + kdiff[c2 * L + k]);
             eu = eu + rho_edge[i] * kdiffu * ud * m.meshScalingDel2[e];
             tend_u_euler[i] = eu;
+        } else if (add_euler) {
+            eu = tend_u_euler[i];
         }
+        // without a del4 pass the final tendency is completed here (same
+        // rounding as the separate pass: tend_u = (vert + base) + euler)
+        tend_u[i] = add_euler ? tu + eu : tu;
     };
     const int base_ = wave_entity_base(nb);
 #pragma unroll 1
@@ -513,17 +519,19 @@ void launch_dyn_tend(const mpas_hip_mesh &m, const DynParams &P, const DynInputs
     MPAS_WAVE_LAUNCH(k_dyn_cells_a, nC + 1, s, m, P,
                        in.ru, in.rw, in.rr_save, in.qtot, in.w, in.theta_m, in.rho_edge, io.kdiff, out.h_divergence,
                        io.tend_rho, sc.dpdz, sc.delsq_w, io.tend_w_euler, sc.delsq_theta, io.tend_theta_euler);
+    // Without a del4 pass (rk_step > 1, or no fourth-order mixing) the edge
+    // pass completes tend_u itself and the del4/finish pass is skipped.
+    const int do_del4 = (rk1 && P.h_mom_eddy_visc4 > 0.0) ? 1 : 0;
     MPAS_WAVE_LAUNCH(k_dyn_edges_a, nE + 1, s, m, P, in.u,
                        in.rw, in.pv_edge, in.ke, out.h_divergence, in.rho_edge, in.pp, sc.dpdz, in.cqu, in.divergence,
-                       in.vorticity, io.kdiff, out.tend_u, io.tend_u_euler, sc.delsq_u);
-    const int do_del4 = (rk1 && P.h_mom_eddy_visc4 > 0.0) ? 1 : 0;
+                       in.vorticity, io.kdiff, out.tend_u, io.tend_u_euler, sc.delsq_u, do_del4 ? 0 : 1);
     if (do_del4) {
         long long n = (long long)(nC > nV ? nC : nV) * L;
         MPAS_LAUNCH(k_dyn_del4_vc, dim3(blocks_for(n)), dim3(kBlock), 0, s, m, sc.delsq_u, sc.delsq_vort,
                            sc.delsq_div);
+        MPAS_LAUNCH(k_dyn_edges_b, dim3(blocks_for((long long)nE * L)), dim3(kBlock), 0, s, m, P, do_del4,
+                           in.rho_edge, sc.delsq_div, sc.delsq_vort, io.tend_u_euler, out.tend_u);
     }
-    MPAS_LAUNCH(k_dyn_edges_b, dim3(blocks_for((long long)nE * L)), dim3(kBlock), 0, s, m, P, do_del4,
-                       in.rho_edge, sc.delsq_div, sc.delsq_vort, io.tend_u_euler, out.tend_u);
     MPAS_WAVE_LAUNCH(k_dyn_edge_adv, nE, s, m, P, in.ru, in.ru_save,
                 in.w, in.theta_m, in.theta_m_save, sc.wflux, sc.tflux, sc.tpert);
     MPAS_WAVE_LAUNCH(k_dyn_cells_b, nC + 1, s, m, P,

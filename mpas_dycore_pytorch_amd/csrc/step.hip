@@ -329,7 +329,7 @@ int mpas_hip_acoustic_step(const mpas_hip_mesh *m, const mpas_hip_acoustic_param
     AcousticInputs in{cofrz, cofwr, cofwz, coftz, cofwt, a_tri, alpha_tri, gamma_tri, rho_zz, theta_m,
                       exner, cqu,   tend_ru, tend_rho, tend_rt, tend_rw, w, rw, rw_save};
     AcousticState io{ru_p, ruAvg, rho_pp, rtheta_pp, rtheta_pp_old, rw_p, wwAvg};
-    launch_acoustic(*m, *p, in, io, STREAM(stream));
+    launch_acoustic(*m, *p, in, io, STREAM(stream), DampFold{0, 0.0, nullptr, nullptr, nullptr});
     return check_launch("acoustic_step");
 }
 
@@ -351,7 +351,7 @@ int mpas_hip_recover_large_step(const mpas_hip_mesh *m, int32_t rk_step, double 
     if (validate_mesh(m)) return 1;
     RecoverInputs in{rho_p_save, rho_pp, rtheta_p_save, rtheta_pp, rt_diabatic_tend, rw_save, rw_p, ru_save, ru_p};
     RecoverOutputs out{wwAvg, ruAvg, rho_p, rho_zz, rtheta_p, theta_m, exner, pressure_p, rw, w, ru, u};
-    launch_recover(*m, rk_step, dt, ns, in, out, STREAM(stream));
+    launch_recover(*m, rk_step, dt, ns, in, out, STREAM(stream), DampFold{0, 0.0, nullptr, nullptr, nullptr});
     return check_launch("recover_large_step");
 }
 
@@ -482,6 +482,11 @@ int mpas_hip_srk3_step(const mpas_hip_mesh *mesh, mpas_hip_state *st, const mpas
                                w.gamma_tri, st->rho_zz,  w.theta_m_init, st->exner, w.cqu,   w.tend_u, w.tend_rho,
                                w.tend_theta, w.tend_w,   st->w,      st->rw,         w.rw_save};
             AcousticState aio{w.ru_p, w.ruAvg, w.rho_pp, w.rtheta_pp, w.rtheta_pp_old, w.rw_p, w.wwAvg};
+            // The divergence damping that closes each acoustic substep is folded
+            // into the next substep's edge pass, and the last one into the
+            // recovery's edge pass (divergence_damping.py:187-209).
+            const DampFold fold{1, 2.0 * cfg->smdiv * cfg->len_disp / rk_sub[rk - 1], w.theta_m_init, w.rtheta_pp,
+                                w.rtheta_pp_old};
             for (int small = 1; small <= nsub[rk - 1]; ++small) {
                 mpas_hip_acoustic_params AP;
                 AP.small_step = small;
@@ -489,15 +494,15 @@ int mpas_hip_srk3_step(const mpas_hip_mesh *mesh, mpas_hip_state *st, const mpas
                 AP.epssm = cfg->epssm;
                 AP.resm = resm;
                 AP.c2 = c2;
-                launch_acoustic(m, AP, ain, aio, s);
-                launch_divdamp(m, rk_sub[rk - 1], cfg->smdiv, cfg->len_disp, w.theta_m_init, w.rtheta_pp,
-                               w.rtheta_pp_old, w.ru_p, s);
+                DampFold f = fold;
+                f.on = small > 1 ? 1 : 0;
+                launch_acoustic(m, AP, ain, aio, s, f);
             }
             RecoverInputs rin{w.rho_p_save, w.rho_pp, w.rtheta_p_save, w.rtheta_pp, st->rt_diabatic_tend,
                               w.rw_save,    w.rw_p,   w.ru_save,       w.ru_p};
             RecoverOutputs rout{w.wwAvg,      w.ruAvg,       st->rho_p, st->rho_zz, st->rtheta_p, st->theta_m,
                                 st->exner,    st->pressure_p, st->rw,   st->w,      st->ru,       st->u};
-            launch_recover(m, rk, rk_timestep[rk - 1], nsub[rk - 1], rin, rout, s);
+            launch_recover(m, rk, rk_timestep[rk - 1], nsub[rk - 1], rin, rout, s, fold);
             launch_diagnostics(m, dt, cfg->apvm, rk, st->u, st->rho_zz, st->v, st->rho_edge, st->v, st->divergence,
                                st->vorticity, st->ke, st->pv_edge, w.scratch, s);
         }
