@@ -216,16 +216,44 @@ class Srk3Stepper:
         self._st = N.StateStruct(*[t[k].data_ptr() for k in N.STATE_FIELDS])
         self._ws = N.WorkspaceStruct(self.work.data_ptr(), self.work.numel())
 
+    def _launch(self) -> None:
+        lib = N.load()
+        self.cfg.diagnostics_current = 1 if self._diagnostics_current else 0
+        N.check(lib.mpas_hip_srk3_step(self.mesh.ref, C.byref(self._st), C.byref(self._ws), C.byref(self.cfg),
+                                       C.c_void_p(stream_handle(self.device))), "mpas_hip_srk3_step")
+        self._diagnostics_current = True
+
     def step(self, n: int = 1) -> None:
         """Advance ``n`` large steps in place on the current stream (no host sync)."""
-        lib = N.load()
-        stream = stream_handle(self.device)
         with torch.cuda.device(self.device):
             for _ in range(n):
-                self.cfg.diagnostics_current = 1 if self._diagnostics_current else 0
-                N.check(lib.mpas_hip_srk3_step(self.mesh.ref, C.byref(self._st), C.byref(self._ws),
-                                               C.byref(self.cfg), C.c_void_p(stream)), "mpas_hip_srk3_step")
-                self._diagnostics_current = True
+                if self._graph is not None and self._diagnostics_current:
+                    self._graph.replay()
+                else:
+                    self._launch()
+
+    _graph = None
+
+    def capture_graph(self) -> None:
+        """Capture one steady-state large step in a HIP graph; later ``step``
+        calls replay it.  The step's kernels allocate nothing, never
+        synchronise with the host and clear memory with a fill kernel, so
+        the captured sequence is the eager one.  The first step of a stepper
+        (diagnostic refresh) always runs eagerly."""
+        with torch.cuda.device(self.device):
+            if not self._diagnostics_current:
+                self._launch()
+            torch.cuda.synchronize(self.device)
+            g = torch.cuda.CUDAGraph()
+            side = torch.cuda.Stream(self.device)
+            side.wait_stream(torch.cuda.current_stream(self.device))
+            with torch.cuda.stream(side):
+                # capture only: a captured launch does not execute
+                with torch.cuda.graph(g, stream=side):
+                    self._launch()
+            torch.cuda.current_stream(self.device).wait_stream(side)
+            torch.cuda.synchronize(self.device)
+            self._graph = g
 
     def state_dict(self) -> dict[str, torch.Tensor]:
         """Device tensors in the caller's row order (views when not reordered)."""
