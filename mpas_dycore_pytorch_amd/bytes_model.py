@@ -163,6 +163,9 @@ KERNELS: dict[str, Inventory] = {
     # u tendency pass (rk > 1): u, pv_edge, rho_edge (E), rw (W), ke, h_div (C) -> tend_u (E)
     "k_dyn_edges_a": Inventory(1, ("E", "E", "E", "W", "C", "C", "edge_c", "edge_e"), ("E",)),
     "k_dyn_edges_b": Inventory(1, ("E", "E"), ("E",)),
+    # edge advective fluxes (rk > 1): ru, ru_save (E), w (W), theta_m, theta_m_save (C)
+    # -> w flux, theta flux, theta perturbation flux (E)
+    "k_dyn_edge_adv": Inventory(1, ("E", "E", "W", "C", "C", "edge_adv", "edge_c"), ("E", "E", "E")),
     "k_dyn_cells_a": Inventory(1, ("E", "W", "cell_e", "edge_c"), ("C",)),
     "k_recover_cells": Inventory(1, ("C",) * 8 + ("W",) * 3, ("C",) * 4 + ("W",) * 2),
     "k_recover_edges": Inventory(1, ("E", "E", "E", "C", "edge_c"), ("E", "E", "E")),
