@@ -209,6 +209,68 @@ int mpas_hip_init_coupled_diagnostics(const mpas_hip_mesh *m, const double *thet
 int mpas_hip_reconstruct(const mpas_hip_mesh *m, const double *u, double *ux, double *uy,
                          double *uz, double *uzonal, double *umerid, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Adjoints (vector-Jacobian products).  The reference obtains them from
+ * torch autograd through each atm_*_work function (with the hand-written
+ * backward rules of TridiagonalFactorize, tridiagonal_factor.py:183-230, and
+ * TridiagonalSolve, tridiagonal_solve.py:232-291), driven by
+ * run_simulation.vjp (atmosphere_model.py:2101-2155).  Each entry point below
+ * is the adjoint of the forward entry point of the same name:
+ *   fwd[]  device pointers: the forward call's inputs, then its outputs, in
+ *          the slot order stated per function (in/out arrays appear twice:
+ *          before the call among the inputs, after it among the outputs);
+ *   bar[]  adjoints of the same slots: input adjoints are accumulated into
+ *          (+=), output adjoints are read and may be overwritten.  A null
+ *          input slot is skipped; output slots named per function must be
+ *          non-null.
+ * Both arrays are host memory.  Cross-entity sums use FP64 atomics, so the
+ * results agree with an exact-order sum to rounding only.
+ * ---------------------------------------------------------------------- */
+/* doubles of device scratch any *_vjp entry point below needs */
+int64_t mpas_hip_vjp_scratch_doubles(int32_t nCells, int32_t nEdges, int32_t nVertices, int32_t nLevels,
+                                     int32_t nScalars);
+/* fwd: scalars | cqw, cqu, qtot */
+int mpas_hip_moist_coefficients_vjp(const mpas_hip_mesh *m, int32_t nScalars, int32_t moist_start,
+                                    int32_t moist_end, const double *const *fwd, double *const *bar,
+                                    void *stream);
+/* fwd: cqw, p, t, rt, qtot | cofrz, cofwr, cofwz, coftz, cofwt, a_tri, alpha_tri, gamma_tri
+ * (cofrz and cofwr depend on the mesh only; cofwz/coftz/cofwt adjoints required) */
+int mpas_hip_vert_imp_coefs_vjp(const mpas_hip_mesh *m, double dts, double epssm, const double *const *fwd,
+                                double *const *bar, void *stream);
+/* fwd: u, h, v_in | h_edge, v, divergence, vorticity, ke, pv_edge */
+int mpas_hip_solve_diagnostics_vjp(const mpas_hip_mesh *m, double dt, double apvm, int32_t rk_step,
+                                   const double *const *fwd, double *const *bar, double *scratch, void *stream);
+/* fwd: the 21 inputs of mpas_hip_dyn_tend, tend_rho, tend_u_euler, tend_w_euler, tend_theta_euler, kdiff
+ * (before) | the same five (after), tend_u, tend_w, tend_theta, h_divergence, rthdynten
+ * (h_divergence and kdiff adjoints required) */
+int mpas_hip_dyn_tend_vjp(const mpas_hip_mesh *m, const mpas_hip_dyn_params *p, const double *const *fwd,
+                          double *const *bar, double *scratch, void *stream);
+/* fwd: u_tend, w_tend (before) | w_tend (after) */
+int mpas_hip_smlstep_pert_vjp(const mpas_hip_mesh *m, const double *const *fwd, double *const *bar, void *stream);
+/* fwd: the 19 inputs of mpas_hip_acoustic_step, ru_p, ruAvg, rho_pp, rtheta_pp, rtheta_pp_old, rw_p,
+ * wwAvg (before) | the same seven (after); the ru_p adjoint is required */
+int mpas_hip_acoustic_step_vjp(const mpas_hip_mesh *m, const mpas_hip_acoustic_params *p,
+                               const double *const *fwd, double *const *bar, double *scratch, void *stream);
+/* fwd: theta_m, rtheta_pp, rtheta_pp_old, ru_p (before) | ru_p (after) */
+int mpas_hip_divergence_damping_vjp(const mpas_hip_mesh *m, double dts, double smdiv, double len_disp,
+                                    const double *const *fwd, double *const *bar, void *stream);
+/* fwd: rho_p_save, rho_pp, rtheta_p_save, rtheta_pp, rt_diabatic_tend, rw_save, rw_p, ru_save, ru_p,
+ * wwAvg, ruAvg, exner, pressure_p (before) | wwAvg, ruAvg, rho_p, rho_zz, rtheta_p, theta_m, exner,
+ * pressure_p, rw, w, ru, u (after); rho_zz and ru adjoints required */
+int mpas_hip_recover_large_step_vjp(const mpas_hip_mesh *m, int32_t rk_step, double dt, int32_t ns,
+                                    const double *const *fwd, double *const *bar, void *stream);
+/* fwd: scalar_old, scalar_new_in, rho_zz_old, rho_zz_new, uhAvg, wwAvg | scalar_new_out, horiz_flux */
+int mpas_hip_advance_scalars_vjp(const mpas_hip_mesh *m, int32_t nScalars, double dt, double coef3,
+                                 double weight_old, double weight_new, const double *const *fwd,
+                                 double *const *bar, double *scratch, void *stream);
+/* fwd: scalars_old, scalars_new_in, rho_zz_old, uhAvg, wwAvg | scalars_new_out, rho_zz_int
+ * (rho_zz_int adjoint required) */
+int mpas_hip_advance_scalars_mono_vjp(const mpas_hip_mesh *m, int32_t nScalars, double dt, double coef3,
+                                      const double *const *fwd, double *const *bar, double *scratch,
+                                      void *stream);
+/* fwd: u | uReconstructX, Y, Z, Zonal, Meridional */
+int mpas_hip_reconstruct_vjp(const mpas_hip_mesh *m, const double *const *fwd, double *const *bar, void *stream);
+
 const char *mpas_hip_last_error(void);
 
 /* Measurement hook, no counterpart in the reference (its timings come from

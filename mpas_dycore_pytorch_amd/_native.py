@@ -77,6 +77,7 @@ class AcousticParamsStruct(C.Structure):
 # Entry points: name -> (restype, argtypes).  "M" marks the mesh pointer.
 _I32, _F64, _I64 = C.c_int32, C.c_double, C.c_int64
 _PM = C.POINTER(MeshStruct)
+_PP = C.POINTER(C.c_void_p)  # host array of device pointers
 SIGNATURES = {
     "mpas_hip_workspace_doubles": (_I64, [_I32, _I32, _I32, _I32, _I32]),
     "mpas_hip_srk3_step": (C.c_int, [_PM, C.POINTER(StateStruct), C.POINTER(WorkspaceStruct),
@@ -95,6 +96,18 @@ SIGNATURES = {
     "mpas_hip_advance_scalars_mono": (C.c_int, [_PM, _I32, _F64, _F64] + [_dp] * 8 + [_vp]),
     "mpas_hip_reconstruct": (C.c_int, [_PM] + [_dp] * 6 + [_vp]),
     "mpas_hip_init_coupled_diagnostics": (C.c_int, [_PM] + [_dp] * 17 + [_vp]),
+    "mpas_hip_vjp_scratch_doubles": (_I64, [_I32, _I32, _I32, _I32, _I32]),
+    "mpas_hip_moist_coefficients_vjp": (C.c_int, [_PM, _I32, _I32, _I32, _PP, _PP, _vp]),
+    "mpas_hip_vert_imp_coefs_vjp": (C.c_int, [_PM, _F64, _F64, _PP, _PP, _vp]),
+    "mpas_hip_solve_diagnostics_vjp": (C.c_int, [_PM, _F64, _F64, _I32, _PP, _PP, _dp, _vp]),
+    "mpas_hip_dyn_tend_vjp": (C.c_int, [_PM, C.POINTER(DynParamsStruct), _PP, _PP, _dp, _vp]),
+    "mpas_hip_smlstep_pert_vjp": (C.c_int, [_PM, _PP, _PP, _vp]),
+    "mpas_hip_acoustic_step_vjp": (C.c_int, [_PM, C.POINTER(AcousticParamsStruct), _PP, _PP, _dp, _vp]),
+    "mpas_hip_divergence_damping_vjp": (C.c_int, [_PM, _F64, _F64, _F64, _PP, _PP, _vp]),
+    "mpas_hip_recover_large_step_vjp": (C.c_int, [_PM, _I32, _F64, _I32, _PP, _PP, _vp]),
+    "mpas_hip_advance_scalars_vjp": (C.c_int, [_PM, _I32, _F64, _F64, _F64, _F64, _PP, _PP, _dp, _vp]),
+    "mpas_hip_advance_scalars_mono_vjp": (C.c_int, [_PM, _I32, _F64, _F64, _PP, _PP, _dp, _vp]),
+    "mpas_hip_reconstruct_vjp": (C.c_int, [_PM, _PP, _PP, _vp]),
     "mpas_hip_last_error": (C.c_char_p, []),
     "mpas_hip_timer_arm": (C.c_int, [C.c_char_p, _I32]),
     "mpas_hip_timer_read": (C.c_int, [C.POINTER(C.c_double), C.POINTER(_I32)]),
