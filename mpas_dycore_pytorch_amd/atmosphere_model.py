@@ -28,7 +28,10 @@ import torch
 from . import kernels as K
 from .device import DeviceMesh, require_gpu, to_numpy
 from .errors import MpasConfigError, MpasMissingFieldError, MpasUnsupportedCapabilityError
-from .time_integration import Srk3Stepper
+from .time_integration import Srk3Stepper, atm_srk3
+
+# keys run_simulation.vjp differentiates by default (atmosphere_model.py:157-168)
+INTER_STEP_PROGNOSTIC_KEYS = ("u", "w", "theta_m", "rho_zz", "scalars", "rt_diabatic_tend")
 
 STEP_BOUNDARY_KEY = "_mpas_hip_step_boundary"
 RAW_KEYS = ("theta", "rho", "qv", "u", "w", "theta_base")
@@ -92,7 +95,7 @@ def run_timestep(state: dict[str, Any], mesh: dict[str, Any], config: dict | Non
                  device: torch.device | str = "cuda", differentiable: bool = False) -> dict[str, Any]:
     """Advance the state by one large step (reference ``run_timestep``)."""
     if differentiable:
-        raise MpasUnsupportedCapabilityError(capability="differentiable", requested=True)
+        return _run_differentiable(state, mesh, config, _dt(config, dt), 1, torch.device(device), None)
     dt = _dt(config, dt)
     gpu = require_gpu(None)
     out_dev = torch.device(device)
@@ -108,11 +111,17 @@ def run_simulation(initial_state: dict[str, Any], mesh: dict[str, Any], config: 
                    checkpoint_timesteps: bool | None = None, checkpoint_use_reentrant: bool = False,
                    checkpoint_materialize_final: bool = True, on_phase: Callable[[str], None] | None = None,
                    on_initial: Callable[[dict[str, Any]], None] | None = None) -> dict[str, Any]:
-    """Run ``nsteps`` large steps (reference ``run_simulation``)."""
-    if differentiable:
-        raise MpasUnsupportedCapabilityError(capability="differentiable", requested=True)
+    """Run ``nsteps`` large steps (reference ``run_simulation``).
+
+    With ``differentiable=True`` every large step is one autograd node that
+    keeps its input state as the checkpoint and recomputes the step, acoustic
+    substeps included, in the backward (the reference's default outer
+    timestep checkpointing, ``autodiff/checkpoint.py:106``)."""
     if isinstance(nsteps, bool) or not isinstance(nsteps, int) or nsteps < 0:
         raise MpasConfigError("nsteps must be a non-negative integer")
+    if differentiable:
+        return _run_differentiable(initial_state, mesh, config, _dt(config, dt), nsteps, torch.device(device),
+                                   on_step)
     dt = _dt(config, dt)
     gpu = require_gpu(None)
     out_dev = torch.device(device)
@@ -134,3 +143,63 @@ def run_simulation(initial_state: dict[str, Any], mesh: dict[str, Any], config: 
         out[STEP_BOUNDARY_KEY] = True
         return out
     return _finish(stepper, prepared, out_dev)
+
+
+def _run_differentiable(state: Mapping[str, Any], mesh: Mapping[str, Any], config: Mapping[str, Any] | None,
+                        dt: float, nsteps: int, device: torch.device,
+                        on_step: Callable[[int, dict[str, Any]], None] | None) -> dict[str, Any]:
+    if not state.get(STEP_BOUNDARY_KEY, False):
+        raise MpasUnsupportedCapabilityError(
+            "differentiable runs start from a prepared step-boundary state (run_simulation(..., nsteps=0) "
+            "returns one); the cold start itself is not differentiated", capability="differentiable_cold_start",
+            requested=True)
+    cur = dict(state)
+    for step in range(1, nsteps + 1):
+        cur = atm_srk3(cur, mesh, config, dt, differentiable=True)
+        cur[STEP_BOUNDARY_KEY] = True
+        if on_step is not None:
+            on_step(step, cur)
+    if device.type != "cuda":
+        cur = {k: (v.to(device) if isinstance(v, torch.Tensor) else v) for k, v in cur.items()}
+    return cur
+
+
+def _run_simulation_vjp(state: dict[str, Any], mesh: dict[str, Any], objective: Callable[[dict[str, Any]], torch.Tensor],
+                        track_keys=None, **kwargs: Any) -> dict[str, Any]:
+    """Adjoint model: d objective(run_simulation(state)) / d state (reference
+    ``run_simulation.vjp``, atmosphere_model.py:2101-2155), through the
+    hand-written adjoint kernels with one checkpoint per large step."""
+    kwargs.setdefault("differentiable", True)
+    clone_state = dict(state)
+    keys = tuple(k for k in INTER_STEP_PROGNOSTIC_KEYS if k in clone_state) if track_keys is None else tuple(track_keys)
+    tracked: dict[str, torch.Tensor] = {}
+    for key in keys:
+        if key not in clone_state:
+            raise KeyError(f"Tracked VJP key '{key}' is missing from state")
+        v = clone_state[key]
+        if not isinstance(v, torch.Tensor):
+            raise TypeError(f"Tracked VJP key '{key}' is not a tensor")
+        leaf = v.detach().clone().to(torch.float64).requires_grad_(True)
+        clone_state[key] = leaf
+        tracked[key] = leaf
+    out = run_simulation(clone_state, mesh, **kwargs)
+    loss = objective(out)
+    if not isinstance(loss, torch.Tensor):
+        raise TypeError(f"objective must return a tensor, got {type(loss).__name__}")
+    if loss.ndim != 0:
+        raise MpasConfigError("objective must return a scalar tensor")
+    grads = torch.autograd.grad(loss, list(tracked.values()), allow_unused=True)
+    return {k: g for k, g in zip(tracked, grads) if g is not None}
+
+
+def _run_simulation_jvp(state: dict[str, Any], mesh: dict[str, Any], perturbation: dict[str, Any],
+                        **kwargs: Any) -> dict[str, Any]:
+    """Tangent-linear model (reference ``run_simulation.jvp``): not provided by
+    the GPU step, which implements reverse mode only."""
+    raise MpasUnsupportedCapabilityError(
+        "run_simulation.jvp: the GPU step provides reverse mode (run_simulation.vjp) only",
+        capability="jvp", requested=True)
+
+
+run_simulation.vjp = _run_simulation_vjp  # type: ignore[attr-defined]
+run_simulation.jvp = _run_simulation_jvp  # type: ignore[attr-defined]

@@ -267,18 +267,77 @@ class Srk3Stepper:
         return out
 
 
+class _Srk3Step(torch.autograd.Function):
+    """One large step as a single autograd node.
+
+    The forward runs the fused step and keeps only the step's input state
+    (the per-large-step checkpoint).  The backward recomputes the step from
+    it on a tape of per-function kernels, acoustic substeps included, and
+    applies the hand-written adjoint kernels (``adjoint.step_vjp``)."""
+
+    @staticmethod
+    def forward(ctx, box: dict, *inputs: torch.Tensor):
+        from .adjoint import STEP_INPUTS
+
+        state = dict(box["state"])
+        state.update({k: v.detach() for k, v in zip(box["in_keys"], inputs)})
+        u = state["u"]
+        out_device = u.device if isinstance(u, torch.Tensor) else torch.device("cpu")
+        gpu = out_device if out_device.type == "cuda" else None
+        stepper = Srk3Stepper(state, box["mesh"], box["config"], box["dt"], device=gpu, strict=box["strict"])
+        ckpt = {k: stepper.t[k].clone() for k in STEP_INPUTS}
+        stepper.step(1)
+        new = stepper.state_dict()
+        new["h_edge"] = new["h_edge"].clone()
+        out_keys = sorted(new)
+        box["out_keys"] = out_keys
+        cfg = N.ConfigStruct()
+        C.pointer(cfg)[0] = stepper.cfg
+        cfg.diagnostics_current = 0
+        ctx.dm, ctx.cfg, ctx.dt, ctx.ckpt = stepper.mesh, cfg, float(box["dt"]), ckpt
+        ctx.in_keys, ctx.out_keys = list(box["in_keys"]), out_keys
+        ctx.in_devices = [x.device for x in inputs]
+        if out_device.type != "cuda":
+            torch.cuda.synchronize(stepper.device)
+        return tuple(new[k].to(out_device) for k in out_keys)
+
+    @staticmethod
+    def backward(ctx, *grads: torch.Tensor):
+        from .adjoint import step_vjp
+
+        dev = ctx.dm.device
+        cot = {k: g.to(device=dev, dtype=torch.float64) for k, g in zip(ctx.out_keys, grads) if g is not None}
+        with torch.cuda.device(dev):
+            _, gin = step_vjp(ctx.dm, ctx.ckpt, ctx.cfg, ctx.dt, cot)
+        return (None,) + tuple(gin[k].to(d) for k, d in zip(ctx.in_keys, ctx.in_devices))
+
+
+def _atm_srk3_differentiable(state: Mapping[str, Any], mesh: Mapping[str, Any], config: Mapping[str, Any],
+                             dt: float, strict: bool) -> dict[str, Any]:
+    from .adjoint import STEP_INPUTS
+
+    in_keys = [k for k in STEP_INPUTS if isinstance(state.get(k), torch.Tensor)]
+    box = {"state": {k: v for k, v in state.items() if k not in in_keys}, "mesh": mesh, "config": config,
+           "dt": dt, "strict": strict, "in_keys": in_keys}
+    outs = _Srk3Step.apply(box, *[state[k] for k in in_keys])
+    result = dict(state)
+    result.update(zip(box["out_keys"], outs))
+    return result
+
+
 def atm_srk3(state: Mapping[str, Any], mesh: Mapping[str, Any], config: Mapping[str, Any], dt: float,
              strict: bool = True, differentiable: bool = False) -> dict[str, Any]:
     """One split-explicit RK3 large step on the GPU (reference ``atm_srk3``).
 
     Inputs may be CPU or GPU tensors (or NumPy arrays) in the reference
     layout; the step runs on the current ROCm device and the results come
-    back on the device of ``state["u"]``.
+    back on the device of ``state["u"]``.  With ``differentiable=True`` the
+    returned tensors carry an autograd graph whose backward applies the
+    hand-written adjoint kernels (reverse mode; forward-mode dual numbers are
+    not supported).
     """
     if differentiable:
-        raise MpasUnsupportedCapabilityError(
-            "differentiable=True: the GPU adjoint of the large step is not implemented",
-            capability="differentiable", requested=True)
+        return _atm_srk3_differentiable(state, mesh, config, dt, strict)
     u = state.get("u")
     out_device = u.device if isinstance(u, torch.Tensor) else torch.device("cpu")
     gpu = out_device if out_device.type == "cuda" else None
