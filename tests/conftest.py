@@ -62,6 +62,28 @@ def kernel_calls():
     return merged, calls
 
 
+def perturbed_state(st: dict, seed: int) -> dict:
+    """A prepared state moved off the hard primal's switching surfaces.
+
+    Seeded noise on u, w (interior interfaces), theta_m and a small positive
+    amount on every scalar (the dry water-vapour slot is identically zero,
+    where every limiter bound ties), so that finite differences see the same
+    branch on both sides of each evaluation."""
+    st = dict(st)
+    rng = np.random.default_rng(seed)
+    nc, ne = int(st["nCells"]), int(st["nEdges"])
+    u = np.array(st["u"], dtype=np.float64)
+    u[:ne] += 0.3 * rng.standard_normal(u[:ne].shape)
+    w = np.array(st["w"], dtype=np.float64)
+    w[:nc, 1:-1] += 1e-3 * rng.standard_normal(w[:nc, 1:-1].shape)
+    sc = np.array(st["scalars"], dtype=np.float64)
+    sc[:nc] += 1e-4 * rng.random(sc[:nc].shape)
+    th = np.array(st["theta_m"], dtype=np.float64)
+    th[:nc] += 0.05 * rng.standard_normal(th[:nc].shape)
+    st.update({"u": u, "w": w, "scalars": sc, "theta_m": th})
+    return st
+
+
 def rel_err(got, ref) -> float:
     got = np.asarray(got, dtype=np.float64)
     ref = np.asarray(ref, dtype=np.float64)

@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 import torch
 
-from conftest import srk3_case
+from conftest import perturbed_state, srk3_case
 
 pytestmark = pytest.mark.gpu
 
@@ -25,22 +25,7 @@ CASE = "l2_L20_tracer"
 
 def _noisy_state(seed: int = 0):
     st, _, meta = srk3_case(CASE)
-    st = dict(st)
-    rng = np.random.default_rng(seed)
-    nc, ne = int(st["nCells"]), int(st["nEdges"])
-    u = np.array(st["u"], dtype=np.float64)
-    u[:ne] += 0.3 * rng.standard_normal(u[:ne].shape)
-    st["u"] = u
-    w = np.array(st["w"], dtype=np.float64)
-    w[:nc, 1:-1] += 1e-3 * rng.standard_normal(w[:nc, 1:-1].shape)
-    st["w"] = w
-    sc = np.array(st["scalars"], dtype=np.float64)
-    sc[:nc] += 1e-4 * rng.random(sc[:nc].shape)
-    st["scalars"] = sc
-    th = np.array(st["theta_m"], dtype=np.float64)
-    th[:nc] += 0.05 * rng.standard_normal(th[:nc].shape)
-    st["theta_m"] = th
-    return st, meta
+    return perturbed_state(st, seed), meta
 
 
 def _len_disp(st, meta):
@@ -493,3 +478,50 @@ def test_run_simulation_vjp_two_steps():
     rels = [abs(fd - ad) / max(abs(fd), abs(ad)) for fd in fds]
     print(f"[adjoint] run_simulation.vjp 2 steps: fd={fds} adjoint={ad:.12e} rel={rels}", flush=True)
     assert min(rels) < 1e-5
+
+
+@pytest.mark.parametrize("perturbed", [False, True])
+def test_vjp_matches_reference_gradients(perturbed):
+    """run_simulation.vjp (two checkpointed steps, hand-written adjoints) against
+    torch autograd of the reference (golden vectors of make_golden_vjp.py), to
+    1e-9 relative per gradient field.  The unperturbed state exercises the tie
+    conventions (the dry water-vapour slot makes every limiter bound tie)."""
+    from conftest import load_npz
+    from mpas_dycore_pytorch_amd.atmosphere_model import STEP_BOUNDARY_KEY, run_simulation
+
+    z, meta = load_npz(f"vjp_{CASE}{'_perturbed' if perturbed else ''}.npz")
+    st, _, smeta = srk3_case(CASE)
+    if perturbed:
+        st = perturbed_state(st, meta["perturb_seed"])
+    nc, ne, L = int(st["nCells"]), int(st["nEdges"]), int(st["nVertLevels"])
+    dev = torch.device("cuda")
+    state = {k: (torch.as_tensor(np.asarray(v, dtype=np.float64), device=dev)
+                 if isinstance(v, np.ndarray) and v.dtype.kind == "f" else v) for k, v in st.items()}
+    state.setdefault("rt_diabatic_tend", torch.zeros((nc + 1, L), dtype=torch.float64, device=dev))
+    state[STEP_BOUNDARY_KEY] = True
+    area = torch.as_tensor(1.0 / np.asarray(st["invAreaCell"], dtype=np.float64)[:nc], device=dev)
+    ae = torch.as_tensor((np.asarray(st["dcEdge"], dtype=np.float64) * np.asarray(st["dvEdge"]))[:ne], device=dev)
+    th0 = torch.as_tensor(np.asarray(st["theta_m"], dtype=np.float64), device=dev)
+    consts = [float(c) for c in z["loss_constants"]]
+
+    def objective(out):
+        d = out["theta_m"][:nc] - th0[:nc]
+        terms = [torch.sum(area[:, None] * d * d), torch.sum(ae[:, None] * out["u"][:ne] * out["u"][:ne]),
+                 torch.sum(area[:, None] * out["scalars"][:nc, :, 1] * out["scalars"][:nc, :, 1])]
+        return sum(t / c for t, c in zip(terms, consts))
+
+    keys = tuple(meta["tracked"])
+    config = {"config_time_integration_order": 3, "config_len_disp": float(st["nominalMinDc"])}
+    grads = run_simulation.vjp(state, st, objective, track_keys=keys, config=config, dt=smeta["dt"],
+                               nsteps=meta["steps"], device=dev)
+    torch.cuda.synchronize()
+    worst = 0.0
+    for k in keys:
+        ref = z["grad__" + k]
+        got = grads[k].cpu().numpy() if k in grads else np.zeros_like(ref)
+        scale = float(np.max(np.abs(ref)))
+        err = float(np.max(np.abs(got[: ref.shape[0] - 1] - ref[: ref.shape[0] - 1]))) / scale
+        print(f"[adjoint] reference gradient {'perturbed' if perturbed else 'as prepared'} {k}: "
+              f"max|g| {scale:.3e} rel err {err:.2e}", flush=True)
+        worst = max(worst, err)
+    assert worst <= 1e-9, worst
