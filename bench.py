@@ -101,6 +101,45 @@ def cpu_baseline(level: int, n_levels: int, dt: float, steps: int, seed: int) ->
                        f"{el:.1f} s wall")}
 
 
+def vjp_leg(m, stepper, config, dt: float, steps: int, fwd_ms_per_step: float, device) -> dict:
+    # m: the prepared mapping (mesh, base state and initial fields, host arrays)
+    """Forward + VJP of ``steps`` large steps from the stepper's current state.
+
+    Loss = sum over cells of area * sum_k (theta_m - theta_m0)^2 after the
+    steps; gradients with respect to u, w, theta_m, rho_zz through the
+    hand-written adjoint kernels, one checkpoint per large step, acoustic
+    substeps recomputed in the backward.  The ratio divides the wall time of
+    forward + VJP by ``steps`` plain forward steps."""
+    import torch
+
+    from mpas_dycore_pytorch_amd.atmosphere_model import STEP_BOUNDARY_KEY, run_simulation
+
+    nc = int(m["nCells"])
+    state = dict(m)
+    state.update({k: v.clone() for k, v in stepper.state_dict().items()})
+    state[STEP_BOUNDARY_KEY] = True
+    area = torch.as_tensor(1.0 / m["invAreaCell"][:nc], device=device)
+    th0 = state["theta_m"].detach().clone()
+
+    def objective(out):
+        d = out["theta_m"][:nc] - th0[:nc]
+        return torch.sum(area[:, None] * d * d)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    grads = run_simulation.vjp(state, m, objective, track_keys=("u", "w", "theta_m", "rho_zz"), config=config,
+                               dt=dt, nsteps=steps, device=device)
+    torch.cuda.synchronize()
+    total_s = time.perf_counter() - t0
+    for k, g in grads.items():
+        if not bool(torch.isfinite(g).all()):
+            raise RuntimeError(f"non-finite gradient of {k}")
+    peak = torch.cuda.max_memory_allocated(device) / 2**30
+    return {"steps": steps, "fwd_plus_vjp_ms_per_step": round(1e3 * total_s / steps, 2),
+            "ratio": round(1e3 * total_s / steps / fwd_ms_per_step, 3), "peak_hbm_gib": round(peak, 1),
+            "checkpoint": "per large step (acoustic substeps recomputed)"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +155,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--reorder", type=int, default=1, help="1: space-filling-curve mesh order on the device")
     ap.add_argument("--graph", type=int, default=0, help="1: replay the large step from a captured HIP graph")
+    ap.add_argument("--vjp-steps", type=int, default=2,
+                    help="large steps of the reverse-mode leg (forward + VJP of an area-weighted squared "
+                         "theta perturbation, one checkpoint per large step); 0 skips it")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -141,7 +183,6 @@ def main() -> int:
     cfg = {"config_time_integration_order": 3, "config_len_disp": float(m["nominalMinDc"])}
     stepper = Srk3Stepper(st, st, cfg, args.dt, device=device, reorder=bool(args.reorder))
     S = stepper.n_scalars
-    del st
 
     stepper.step(args.warmup)
     if args.graph:
@@ -192,6 +233,11 @@ def main() -> int:
     step_roof = {"algorithmic_bytes": sb, "achieved": round(step_gbs, 1), "unit": "GB/s",
                  "frac": round(step_gbs / HBM_PEAK_GBS, 4)}
 
+    vjp = None
+    if args.vjp_steps > 0:
+        vjp = vjp_leg(st, stepper, cfg, args.dt, args.vjp_steps, ms_per_step, device)
+        log(f"reverse mode: {vjp}")
+
     cpu = None
     if not args.no_cpu_baseline:
         del stepper
@@ -211,7 +257,7 @@ def main() -> int:
                    "cells": nC, "levels": L, "dt": args.dt, "parallelism": "single GPU",
                    "mesh_order": "hilbert" if args.reorder else "generator",
                    "launch": "hip graph" if args.graph else "stream"},
-        "roofline": roof, "step_roofline": step_roof, "vjp_fwd_ratio": None, "cpu_baseline": cpu,
+        "roofline": roof, "step_roofline": step_roof, "vjp_fwd_ratio": (vjp or {}).get("ratio"), "vjp": vjp, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -114,6 +114,315 @@ __global__ __launch_bounds__(256) void k_adj_acoustic_col(mpas_hip_mesh m, Slots
     acoustic_col_vjp(m, A.ap, ADJ_F, ADJ_B, A.col, c);
 }
 
+// ---------------------------------------------------------------------------
+// Column adjoints, one wavefront per column (lane = level or interface).  The
+// level-parallel parts use all lanes; the Thomas recurrences run on lane 0
+// over the column staged in LDS, as in the forward kernels.
+// ---------------------------------------------------------------------------
+constexpr int kColWaves = 4;
+
+// Adjoint of the acoustic column update (acoustic_col_vjp, same algebra).
+__global__ __launch_bounds__(64 * kColWaves) void k_adj_acoustic_colw(mpas_hip_mesh m, Slots S_, KArgs A) {
+    using namespace acoustic;
+    __shared__ double s_rs[kColWaves][65], s_ts[kColWaves][65], s_r[kColWaves][65], s_y[kColWaves][65];
+    __shared__ double s_x[kColWaves][65], s_gx[kColWaves][65], s_gy[kColWaves][65], s_gri[kColWaves][65];
+    __shared__ double s_gr[kColWaves][65], s_grs[kColWaves][65], s_gts[kColWaves][65];
+    const double *const *F = S_.f;
+    double *const *B = S_.b;
+    const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
+    const int wv = threadIdx.y, k = threadIdx.x;
+    const int c = blockIdx.x * kColWaves + wv;
+    const bool live = c < nC;
+    const long long oc = (long long)c * L, ow = (long long)c * Lw;
+    const mpas_hip_acoustic_params &P = A.ap;
+    const double dts = P.dts, epssm = P.epssm, resm = P.resm;
+    const bool first = (P.small_step == 1);
+    const double *zz = m.zz + oc, *rdzw = m.rdzw, *fzm = m.fzm, *fzp = m.fzp;
+    const double *cofrz_ = F[cofrz];
+    auto RPP = [&](int kk) { return first ? 0.0 : F[rho_pp_in][oc + kk]; };
+    auto RTPP = [&](int kk) { return first ? 0.0 : F[rtheta_pp_in][oc + kk]; };
+    auto RWP = [&](int kk) { return first ? 0.0 : F[rw_p_in][ow + kk]; };
+    auto COFWZ = [&](int kk) { return F[cofwz][oc + kk]; };
+    auto COFWR = [&](int kk) { return F[cofwr][oc + kk]; };
+    auto COFWT = [&](int kk) { return F[cofwt][oc + kk]; };
+    auto COFTZ = [&](int kk) { return F[coftz][ow + kk]; };
+    const int n = live ? m.nEdgesOnCell[c] : 0;
+    const double ia = live ? m.invAreaCell[c] : 0.0;
+    // ---- phase 1: rs, ts
+    if (live && k < L) {
+        double rsum = 0.0, tsum = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const int e = m.edgesOnCell[c * ME + i];
+            const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+            const double metric = m.edgesOnCell_sign[c * ME + i] * m.dvEdge[e];
+            const double flux = dts * metric * F[ru_p][(long long)e * L + k] * ia;
+            const double thedge = 0.5 * (F[theta_m][(long long)c2 * L + k] + F[theta_m][(long long)c1 * L + k]);
+            const double rcon = -flux, tcon = -(flux * thedge);
+            rsum = (i == 0) ? rcon : rsum + rcon;
+            tsum = (i == 0) ? tcon : tsum + tcon;
+        }
+        s_rs[wv][k] = RPP(k) + dts * F[tend_rho][oc + k] + rsum - cofrz_[k] * resm * (RWP(k + 1) - RWP(k));
+        s_ts[wv][k] = RTPP(k) + dts * F[tend_rt][oc + k] + tsum -
+                      resm * rdzw[k] * (COFTZ(k + 1) * RWP(k + 1) - COFTZ(k) * RWP(k));
+    }
+    __syncthreads();
+    // ---- phase 2: right-hand side r(k), 1 <= k < L
+    if (live && k >= 1 && k < L) {
+        const double tsk = s_ts[wv][k], tskm = s_ts[wv][k - 1], rsk = s_rs[wv][k], rskm = s_rs[wv][k - 1];
+        s_r[wv][k] = RWP(k) + dts * F[tend_rw][ow + k] -
+                     COFWZ(k) * ((zz[k] * tsk - zz[k - 1] * tskm) + resm * (zz[k] * RTPP(k) - zz[k - 1] * RTPP(k - 1))) -
+                     COFWR(k) * ((rsk + rskm) + resm * (RPP(k) + RPP(k - 1))) + COFWT(k) * (tsk + resm * RTPP(k)) +
+                     COFWT(k - 1) * (tskm + resm * RTPP(k - 1));
+    }
+    __syncthreads();
+    // ---- phase 3: forward sweeps (lane 0)
+    if (live && k == 0) {
+        const double *a_ = F[a_tri] + oc, *al_ = F[alpha_tri] + oc, *ga_ = F[gamma_tri] + oc;
+        double y = RWP(0);
+        s_y[wv][0] = y;
+        for (int kk = 1; kk < L; ++kk) {
+            y = (s_r[wv][kk] - a_[kk] * y) * al_[kk];
+            s_y[wv][kk] = y;
+        }
+        double x = RWP(L);
+        s_x[wv][L] = x;
+        for (int kk = L - 1; kk >= 0; --kk) {
+            x = s_y[wv][kk] - ga_[kk] * x;
+            s_x[wv][kk] = x;
+        }
+    }
+    __syncthreads();
+    // ---- phase 4: adjoint of the final update and of the damping
+    const double gr_k = (live && k < L) ? bar_of(B, rho_pp, oc + k) : 0.0;
+    const double gt_k = (live && k < L) ? bar_of(B, rtheta_pp, oc + k) : 0.0;
+    double ctz_acc = 0.0;  // adjoint of coftz(k), accumulated over the phases
+    if (live && k <= L) {
+        double gx = bar_of(B, rw_p, ow + k);
+        if (k < L) gx += cofrz_[k] * gr_k + rdzw[k] * COFTZ(k) * gt_k;
+        if (k >= 1) {
+            const double grm = bar_of(B, rho_pp, oc + k - 1), gtm = bar_of(B, rtheta_pp, oc + k - 1);
+            gx += -cofrz_[k - 1] * grm - rdzw[k - 1] * COFTZ(k) * gtm;
+            ctz_acc += -rdzw[k - 1] * F[rw_p][ow + k] * gtm;
+        }
+        if (k < L) ctz_acc += rdzw[k] * F[rw_p][ow + k] * gt_k;
+        double gri = 0.0;
+        if (k >= 1 && k < L) {
+            const double gww = bar_of(B, wwAvg, ow + k);
+            const double grr = gx + 0.5 * (1.0 + epssm) * gww;
+            const double dss = m.dss[oc + k];
+            const double D = 1.0 + dts * dss;
+            const double zi = fzm[k] * zz[k] + fzp[k] * zz[k - 1];
+            const double ri = fzm[k] * F[rho_zz][oc + k] + fzp[k] * F[rho_zz][oc + k - 1];
+            const double gq = grr / D;
+            const double grwd = gq - grr;
+            gri = -gq * dts * dss * zi * F[w][ow + k];
+            add_own(B, w, ow + k, -gq * dts * dss * zi * ri);
+            add_own(B, rw_save, ow + k, grwd);
+            add_own(B, rw, ow + k, -grwd);
+            gx = gq;
+        }
+        s_gx[wv][k] = gx;
+        s_gri[wv][k] = gri;
+    }
+    __syncthreads();
+    // ---- phase 5: rho_zz (all lanes); reverse sweeps (lane 0)
+    if (live && k < L) {
+        double g = 0.0;
+        if (k >= 1) g += fzm[k] * s_gri[wv][k];
+        if (k + 1 <= L - 1) g += fzp[k + 1] * s_gri[wv][k + 1];
+        if (g != 0.0) add_own(B, rho_zz, oc + k, g);
+    }
+    if (live && k == 0) {
+        const double *a_ = F[a_tri] + oc, *al_ = F[alpha_tri] + oc, *ga_ = F[gamma_tri] + oc;
+        for (int kk = 0; kk < L; ++kk) s_gx[wv][kk + 1] += -ga_[kk] * s_gx[wv][kk];
+        for (int kk = 0; kk < L; ++kk) s_gy[wv][kk] = s_gx[wv][kk];
+        for (int kk = L - 1; kk >= 1; --kk) s_gy[wv][kk - 1] += -a_[kk] * al_[kk] * s_gy[wv][kk];
+    }
+    __syncthreads();
+    // ---- phase 6: factor adjoints and the right-hand-side adjoint g_r
+    if (live && k < L) {
+        add_own(B, gamma_tri, oc + k, -s_x[wv][k + 1] * s_gx[wv][k]);
+        double grk = 0.0;
+        if (k >= 1) {
+            const double al = F[alpha_tri][oc + k], a = F[a_tri][oc + k];
+            const double gy = s_gy[wv][k];
+            grk = al * gy;
+            add_own(B, alpha_tri, oc + k, (s_r[wv][k] - a * s_y[wv][k - 1]) * gy);
+            add_own(B, a_tri, oc + k, -s_y[wv][k - 1] * grk);
+        }
+        s_gr[wv][k] = grk;
+    }
+    if (live && k == L) s_gr[wv][k] = 0.0;
+    __syncthreads();
+    // ---- phase 7: right-hand side -> rs, ts and the coefficients
+    double grs = 0.0, gts = 0.0;
+    if (live && k < L) {
+        const double g0 = s_gr[wv][k];                         // r(k), zero at k = 0
+        const double g1 = (k + 1 <= L - 1) ? s_gr[wv][k + 1] : 0.0;  // r(k+1)
+        grs = gr_k - COFWR(k) * g0 - ((k + 1 <= L - 1) ? COFWR(k + 1) * g1 : 0.0);
+        gts = gt_k + (-COFWZ(k) * zz[k] + COFWT(k)) * g0 +
+              ((k + 1 <= L - 1) ? (COFWZ(k + 1) * zz[k] + COFWT(k)) * g1 : 0.0);
+        add_own(B, cofwt, oc + k, (s_ts[wv][k] + resm * RTPP(k)) * (g0 + g1));
+        if (k >= 1) {
+            const double Az = zz[k] * s_ts[wv][k] - zz[k - 1] * s_ts[wv][k - 1];
+            const double Bz = zz[k] * RTPP(k) - zz[k - 1] * RTPP(k - 1);
+            const double R2 = (s_rs[wv][k] + s_rs[wv][k - 1]) + resm * (RPP(k) + RPP(k - 1));
+            add_own(B, tend_rw, ow + k, dts * g0);
+            add_own(B, cofwz, oc + k, -(Az + resm * Bz) * g0);
+            add_own(B, cofwr, oc + k, -R2 * g0);
+        }
+        if (!first) {
+            double grt = (-COFWZ(k) * resm * zz[k] + COFWT(k) * resm) * g0;
+            double grp = -COFWR(k) * resm * g0;
+            if (k + 1 <= L - 1) {
+                grt += (COFWZ(k + 1) * resm * zz[k] + COFWT(k) * resm) * g1;
+                grp += -COFWR(k + 1) * resm * g1;
+            }
+            add_own(B, rtheta_pp_in, oc + k, grt + gts + bar_of(B, rtheta_pp_old, oc + k));
+            add_own(B, rho_pp_in, oc + k, grp + grs);
+        }
+        add_own(B, tend_rho, oc + k, dts * grs);
+        add_own(B, tend_rt, oc + k, dts * gts);
+        s_grs[wv][k] = grs;
+        s_gts[wv][k] = gts;
+    }
+    __syncthreads();
+    // ---- phase 8: interface adjoints (coftz, rw_p, wwAvg) and the edge scatter
+    if (live && k <= L) {
+        const double gts_k = (k < L) ? s_gts[wv][k] : 0.0, grs_k = (k < L) ? s_grs[wv][k] : 0.0;
+        const double gts_m = (k >= 1) ? s_gts[wv][k - 1] : 0.0, grs_m = (k >= 1) ? s_grs[wv][k - 1] : 0.0;
+        ctz_acc += resm * ((k < L) ? rdzw[k] : 0.0) * RWP(k) * gts_k;
+        if (k >= 1) ctz_acc += -resm * rdzw[k - 1] * RWP(k) * gts_m;
+        add_own(B, coftz, ow + k, ctz_acc);
+        if (!first) {
+            double g = 0.0;
+            if (k < L) g += cofrz_[k] * resm * grs_k + resm * rdzw[k] * COFTZ(k) * gts_k;
+            if (k >= 1) g += -cofrz_[k - 1] * resm * grs_m - resm * rdzw[k - 1] * COFTZ(k) * gts_m;
+            const double gww = bar_of(B, wwAvg, ow + k);
+            if (k >= 1 && k < L) g += s_gr[wv][k] + 0.5 * (1.0 - epssm) * gww;
+            if (k == 0) g += s_gy[wv][0];
+            if (k == L) g += s_gx[wv][L];
+            add_own(B, rw_p_in, ow + k, g);
+            add_own(B, wwAvg_in, ow + k, gww);
+        }
+    }
+    if (live && k < L) {
+        const double grs_k = s_grs[wv][k], gts_k = s_gts[wv][k];
+        for (int i = 0; i < n; ++i) {
+            const int e = m.edgesOnCell[c * ME + i];
+            const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+            const double metric = m.edgesOnCell_sign[c * ME + i] * m.dvEdge[e];
+            const double thedge = 0.5 * (F[theta_m][(long long)c2 * L + k] + F[theta_m][(long long)c1 * L + k]);
+            const double flux = dts * metric * F[ru_p][(long long)e * L + k] * ia;
+            const double gflux = -grs_k - thedge * gts_k;
+            add(B, ru_p, (long long)e * L + k, dts * metric * ia * gflux);
+            const double gth = 0.5 * (-flux * gts_k);
+            add(B, theta_m, (long long)c2 * L + k, gth);
+            add(B, theta_m, (long long)c1 * L + k, gth);
+        }
+    }
+}
+
+// Adjoint of the vertical-implicit coefficients (vimp_col_vjp, same algebra).
+__global__ __launch_bounds__(64 * kColWaves) void k_adj_vimp_colw(mpas_hip_mesh m, Slots S_, KArgs A) {
+    using namespace vimp;
+    __shared__ double s_ga[kColWaves][65], s_gb[kColWaves][65], s_gc[kColWaves][65];
+    __shared__ double s_gpi[kColWaves][65], s_gti[kColWaves][65];
+    const double *const *F = S_.f;
+    double *const *B = S_.b;
+    const int L = m.nLevels, nC = m.nCells, Lw = L + 1;
+    const int wv = threadIdx.y, k = threadIdx.x;
+    const int c = blockIdx.x * kColWaves + wv;
+    const bool live = c < nC;
+    const double dtseps = 0.5 * A.dt * (1.0 + A.epssm);
+    const double rcv = MPAS_RGAS / (MPAS_CP - MPAS_RGAS);
+    const double c2 = MPAS_CP * rcv;
+    const long long o = (long long)c * L, ow = (long long)c * Lw;
+    const double *zz = m.zz + o, *rdzw = m.rdzw;
+    auto WZ = [&](int kk) { return F[cofwz][o + kk]; };
+    auto TZ = [&](int kk) { return F[coftz][ow + kk]; };
+    auto WT = [&](int kk) { return F[cofwt][o + kk]; };
+    auto WR = [&](int kk) { return F[cofwr][o + kk]; };
+    // ---- phase 1: reverse Thomas factor (lane 0) -> adjoints of a, b, c
+    if (live && k == 0) {
+        const double *cofrz_ = F[cofrz], *a_ = F[a_tri] + o, *al_ = F[alpha_tri] + o, *ga_ = F[gamma_tri] + o;
+        double gg_carry = 0.0;
+        s_ga[wv][0] = s_gb[wv][0] = s_gc[wv][0] = 0.0;
+        s_ga[wv][L] = s_gb[wv][L] = s_gc[wv][L] = 0.0;
+        for (int kk = L - 1; kk >= 1; --kk) {
+            const double al = al_[kk], a = a_[kk];
+            const double gprev = (kk > 1) ? ga_[kk - 1] : 0.0;
+            const double cc = -WZ(kk) * TZ(kk + 1) * rdzw[kk] * zz[kk] - WR(kk) * cofrz_[kk] +
+                              WT(kk) * TZ(kk + 1) * rdzw[kk];
+            const double g_ga = bar_of(B, gamma_tri, o + kk) + gg_carry;
+            const double g_al = bar_of(B, alpha_tri, o + kk) + cc * g_ga;
+            const double g_den = -(al * al) * g_al;
+            s_gc[wv][kk] = al * g_ga;
+            s_gb[wv][kk] = g_den;
+            s_ga[wv][kk] = bar_of(B, a_tri, o + kk) - gprev * g_den;
+            gg_carry = -a * g_den;
+        }
+    }
+    __syncthreads();
+    // ---- phase 2: totals of the coefficient adjoints; interface partials
+    double Gwt = 0.0;
+    if (live && k <= L) {
+        auto GA = [&](int kk) { return (kk >= 1 && kk <= L - 1) ? s_ga[wv][kk] : 0.0; };
+        auto GB = [&](int kk) { return (kk >= 1 && kk <= L - 1) ? s_gb[wv][kk] : 0.0; };
+        auto GC = [&](int kk) { return (kk >= 1 && kk <= L - 1) ? s_gc[wv][kk] : 0.0; };
+        double Gwz = 0.0, Gtz = 0.0;
+        if (k >= 1 && k <= L - 1) {
+            Gwz = bar_of(B, cofwz, o + k) - TZ(k - 1) * rdzw[k - 1] * zz[k - 1] * GA(k) +
+                  (TZ(k) * rdzw[k] * zz[k] + TZ(k) * rdzw[k - 1] * zz[k - 1]) * GB(k) -
+                  TZ(k + 1) * rdzw[k] * zz[k] * GC(k);
+        }
+        Gtz = bar_of(B, coftz, ow + k);
+        if (k + 1 <= L - 1) Gtz += (-WZ(k + 1) * rdzw[k] * zz[k] - WT(k) * rdzw[k]) * GA(k + 1);
+        if (k >= 1 && k <= L - 1)
+            Gtz += (WZ(k) * (rdzw[k] * zz[k] + rdzw[k - 1] * zz[k - 1]) - (WT(k) * rdzw[k] - WT(k - 1) * rdzw[k - 1])) *
+                   GB(k);
+        if (k - 1 >= 1) Gtz += (-WZ(k - 1) * rdzw[k - 1] * zz[k - 1] + WT(k - 1) * rdzw[k - 1]) * GC(k - 1);
+        if (k < L) {
+            Gwt = bar_of(B, cofwt, o + k) - TZ(k) * rdzw[k] * GA(k + 1) - TZ(k) * rdzw[k] * GB(k) +
+                  TZ(k + 1) * rdzw[k] * GB(k + 1) + TZ(k + 1) * rdzw[k] * GC(k);
+        }
+        double gpi = 0.0, gti = 0.0;
+        if (k >= 1 && k <= L - 1) {
+            const double zi = m.fzm[k] * zz[k] + m.fzp[k] * zz[k - 1];
+            const double pi = m.fzm[k] * F[p][o + k] + m.fzp[k] * F[p][o + k - 1];
+            const double base = dtseps * c2 * zi * m.rdzu[k];
+            add_own(B, cqw, ow + k, base * pi * Gwz);
+            gpi = base * F[cqw][ow + k] * Gwz;
+            gti = dtseps * Gtz;
+        }
+        s_gpi[wv][k] = gpi;
+        s_gti[wv][k] = gti;
+    }
+    __syncthreads();
+    // ---- phase 3: level adjoints of p, t, rt, qtot
+    if (live && k < L) {
+        double gp = 0.0, gt = 0.0;
+        if (k >= 1) {
+            gp += m.fzm[k] * s_gpi[wv][k];
+            gt += m.fzm[k] * s_gti[wv][k];
+        }
+        if (k + 1 <= L - 1) {
+            gp += m.fzp[k + 1] * s_gpi[wv][k + 1];
+            gt += m.fzp[k + 1] * s_gti[wv][k + 1];
+        }
+        if (Gwt != 0.0) {
+            const double A0 = 0.5 * dtseps * rcv * zz[k] * MPAS_GRAVITY * m.rho_base[o + k];
+            const double q1 = 1.0 + F[qtot][o + k], rr = m.rtheta_base[o + k] + F[rt][o + k], pb = m.exner_base[o + k];
+            const double cw = WT(k);
+            gp += A0 / q1 / (rr * pb) * Gwt;
+            add_own(B, qtot, o + k, -(cw / q1) * Gwt);
+            add_own(B, rt, o + k, -(cw / rr) * Gwt);
+        }
+        add_own(B, p, o + k, gp);
+        add_own(B, t, o + k, gt);
+    }
+}
+
 #define ADJ_LAUNCH(kernel, n_ent, n_lev, stream)                                                      \
     do {                                                                                              \
         const long long nb_ = ::mpas::wave_blocks(n_ent);                                             \
@@ -302,7 +611,8 @@ int mpas_hip_vert_imp_coefs_vjp(const mpas_hip_mesh *mesh, double dts, double ep
     KArgs A{};
     A.dt = dts;
     A.epssm = epssm;
-    MPAS_LAUNCH(k_adj_vimp_col, dim3(blocks_for(m.nCells)), dim3(kBlock), 0, STREAM(stream), m, S_, A);
+    MPAS_LAUNCH(k_adj_vimp_colw, dim3((m.nCells + kColWaves - 1) / kColWaves), dim3(64, kColWaves), 0,
+                STREAM(stream), m, S_, A);
     return check_launch("vert_imp_coefs_vjp");
 }
 
@@ -316,7 +626,8 @@ int mpas_hip_acoustic_step_vjp(const mpas_hip_mesh *mesh, const mpas_hip_acousti
     KArgs A{};
     A.ap = *p;
     A.col = scratch;
-    MPAS_LAUNCH(k_adj_acoustic_col, dim3(blocks_for(m.nCells)), dim3(kBlock), 0, s, m, S_, A);
+    MPAS_LAUNCH(k_adj_acoustic_colw, dim3((m.nCells + kColWaves - 1) / kColWaves), dim3(64, kColWaves), 0, s, m,
+                S_, A);
     ADJ_LAUNCH(k_adj_acoustic_edge, m.nEdges, m.nLevels, s);
     return check_launch("acoustic_step_vjp");
 }

