@@ -125,10 +125,13 @@ def vjp_leg(m, stepper, config, dt: float, steps: int, fwd_ms_per_step: float, d
         d = out["theta_m"][:nc] - th0[:nc]
         return torch.sum(area[:, None] * d * d)
 
+    keys = ("u", "w", "theta_m", "rho_zz")
+    # untimed warm-up: device mesh, inverse tables and allocator pools
+    run_simulation.vjp(state, m, objective, track_keys=keys, config=config, dt=dt, nsteps=1, device=device)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    grads = run_simulation.vjp(state, m, objective, track_keys=("u", "w", "theta_m", "rho_zz"), config=config,
-                               dt=dt, nsteps=steps, device=device)
+    grads = run_simulation.vjp(state, m, objective, track_keys=keys, config=config, dt=dt, nsteps=steps,
+                               device=device)
     torch.cuda.synchronize()
     total_s = time.perf_counter() - t0
     for k, g in grads.items():

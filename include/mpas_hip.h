@@ -226,6 +226,18 @@ int mpas_hip_reconstruct(const mpas_hip_mesh *m, const double *u, double *ux, do
  * Both arrays are host memory.  Cross-entity sums use FP64 atomics, so the
  * results agree with an exact-order sum to rounding only.
  * ---------------------------------------------------------------------- */
+/* Inverse connectivity for atomic-free adjoint gathers: the counterpart of the
+ * reference's ordered output transposes (build_ordered_output_transpose,
+ * static_payloads.py:44-118).  CSR tables ordered by (edge, slot):
+ *   advStart[c] .. advStart[c+1]-1  index the (advEdge, advSlot) pairs with
+ *                                   advCellsForEdge[advEdge][advSlot] == c;
+ *   eoeStart[e] .. eoeStart[e+1]-1  index the (eoeEdge, eoeSlot) pairs with
+ *                                   edgesOnEdge[eoeEdge][eoeSlot] == e. */
+typedef struct mpas_hip_inverse_tables {
+    const int32_t *advStart, *advEdge, *advSlot; /* nCells + 1, then one entry per (edge, slot) */
+    const int32_t *eoeStart, *eoeEdge, *eoeSlot; /* nEdges + 1, then one entry per (edge, slot) */
+} mpas_hip_inverse_tables;
+
 /* doubles of device scratch any *_vjp entry point below needs */
 int64_t mpas_hip_vjp_scratch_doubles(int32_t nCells, int32_t nEdges, int32_t nVertices, int32_t nLevels,
                                      int32_t nScalars);
@@ -242,9 +254,12 @@ int mpas_hip_solve_diagnostics_vjp(const mpas_hip_mesh *m, double dt, double apv
                                    const double *const *fwd, double *const *bar, double *scratch, void *stream);
 /* fwd: the 21 inputs of mpas_hip_dyn_tend, tend_rho, tend_u_euler, tend_w_euler, tend_theta_euler, kdiff
  * (before) | the same five (after), tend_u, tend_w, tend_theta, h_divergence, rthdynten
- * (h_divergence and kdiff adjoints required) */
+ * (h_divergence and kdiff adjoints required).  With inverse tables (inv non-null)
+ * the transposes of the advection and PV-flux stencils are gathers; without,
+ * atomic scatters. */
 int mpas_hip_dyn_tend_vjp(const mpas_hip_mesh *m, const mpas_hip_dyn_params *p, const double *const *fwd,
-                          double *const *bar, double *scratch, void *stream);
+                          double *const *bar, double *scratch, const mpas_hip_inverse_tables *inv,
+                          void *stream);
 /* fwd: u_tend, w_tend (before) | w_tend (after) */
 int mpas_hip_smlstep_pert_vjp(const mpas_hip_mesh *m, const double *const *fwd, double *const *bar, void *stream);
 /* fwd: the 19 inputs of mpas_hip_acoustic_step, ru_p, ruAvg, rho_pp, rtheta_pp, rtheta_pp_old, rw_p,

@@ -74,6 +74,10 @@ class AcousticParamsStruct(C.Structure):
                 ("c2", C.c_double)]
 
 
+class InverseTablesStruct(C.Structure):
+    _fields_ = [(n, _dp) for n in ("advStart", "advEdge", "advSlot", "eoeStart", "eoeEdge", "eoeSlot")]
+
+
 # Entry points: name -> (restype, argtypes).  "M" marks the mesh pointer.
 _I32, _F64, _I64 = C.c_int32, C.c_double, C.c_int64
 _PM = C.POINTER(MeshStruct)
@@ -100,7 +104,8 @@ SIGNATURES = {
     "mpas_hip_moist_coefficients_vjp": (C.c_int, [_PM, _I32, _I32, _I32, _PP, _PP, _vp]),
     "mpas_hip_vert_imp_coefs_vjp": (C.c_int, [_PM, _F64, _F64, _PP, _PP, _vp]),
     "mpas_hip_solve_diagnostics_vjp": (C.c_int, [_PM, _F64, _F64, _I32, _PP, _PP, _dp, _vp]),
-    "mpas_hip_dyn_tend_vjp": (C.c_int, [_PM, C.POINTER(DynParamsStruct), _PP, _PP, _dp, _vp]),
+    "mpas_hip_dyn_tend_vjp": (C.c_int, [_PM, C.POINTER(DynParamsStruct), _PP, _PP, _dp,
+                                        C.POINTER(InverseTablesStruct), _vp]),
     "mpas_hip_smlstep_pert_vjp": (C.c_int, [_PM, _PP, _PP, _vp]),
     "mpas_hip_acoustic_step_vjp": (C.c_int, [_PM, C.POINTER(AcousticParamsStruct), _PP, _PP, _dp, _vp]),
     "mpas_hip_divergence_damping_vjp": (C.c_int, [_PM, _F64, _F64, _F64, _PP, _PP, _vp]),

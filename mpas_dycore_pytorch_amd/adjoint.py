@@ -193,7 +193,7 @@ def _call_vjp(lib, op: Op, dm: DeviceMesh, fwd, bar, scratch: torch.Tensor, stre
     elif k == "solve_diagnostics":
         rc = lib.mpas_hip_solve_diagnostics_vjp(dm.ref, *a, fwd, bar, sp, stream)
     elif k == "dyn_tend":
-        rc = lib.mpas_hip_dyn_tend_vjp(dm.ref, C.byref(a[0]), fwd, bar, sp, stream)
+        rc = lib.mpas_hip_dyn_tend_vjp(dm.ref, C.byref(a[0]), fwd, bar, sp, C.byref(dm.inverse_tables()), stream)
     elif k == "smlstep_pert":
         rc = lib.mpas_hip_smlstep_pert_vjp(dm.ref, fwd, bar, stream)
     elif k == "acoustic_step":

@@ -33,6 +33,9 @@ enum {
 struct DynAdjScratch {
     double *dpdz, *delsq_u, *delsq_div, *delsq_vort;                         // forward (rk 1)
     double *g_thadv, *g_hadv, *g_dpdz, *g_delsq_w, *g_delsq_theta, *g_delsq_div, *g_delsq_vort, *g_delsq_u;
+    // per-edge adjoints of the theta flux, w flux and PV-flux sum, consumed by
+    // the gather passes (used when inverse tables are given)
+    double *g_tf, *g_wf, *g_q;
 };
 
 // sign of edge e in the edge list of cell c
@@ -304,7 +307,7 @@ __device__ __forceinline__ void dyn_a2(const mpas_hip_mesh &m, const mpas_hip_dy
 
 // ---------------------------------------------------------------- A3
 __device__ __forceinline__ void dyn_a3(const mpas_hip_mesh &m, const mpas_hip_dyn_params &P, Fwd F, Bar B,
-                                       const DynAdjScratch &S, int e, int k) {
+                                       const DynAdjScratch &S, bool gather, int e, int k) {
     using namespace dyn;
     const int L = m.nLevels, Lw = L + 1, MA = m.maxAdv, ME2 = m.maxEdges2;
     if (e >= m.nEdges || k >= L) return;
@@ -324,8 +327,9 @@ __device__ __forceinline__ void dyn_a3(const mpas_hip_mesh &m, const mpas_hip_dy
             const double wgt = m.adv_coefs[e * MA + a] + srt * m.adv_coefs_3rd[e * MA + a];
             const double cc = wgt * F[theta_m][(long long)ac * L + k];
             ft = (a == 0) ? cc : ft + cc;
-            add(B, theta_m, (long long)ac * L + k, wgt * rut * gf);
+            if (!gather) add(B, theta_m, (long long)ac * L + k, wgt * rut * gf);
         }
+        if (gather) S.g_tf[i] = gf;
         add(B, ru, i, ft * gf);
         if (!rk1) {
             // thadv(c) -= sgn(c, e) dv (ru_save - ru) face
@@ -350,8 +354,9 @@ __device__ __forceinline__ void dyn_a3(const mpas_hip_mesh &m, const mpas_hip_dy
             const double wgt = m.adv_coefs[e * MA + a] + srw * m.adv_coefs_3rd[e * MA + a];
             const double cc = wgt * F[w][(long long)ac * Lw + k];
             fw = (a == 0) ? cc : fw + cc;
-            add(B, w, (long long)ac * Lw + k, wgt * ruw * gf);
+            if (!gather) add(B, w, (long long)ac * Lw + k, wgt * ruw * gf);
         }
+        if (gather) S.g_wf[i] = gf;
         add(B, ru, i, m.fzm[k] * fw * gf);
         add(B, ru, i - 1, m.fzp[k] * fw * gf);
     }
@@ -381,15 +386,20 @@ __device__ __forceinline__ void dyn_a3(const mpas_hip_mesh &m, const mpas_hip_dy
         add(B, u, i, -0.5 * hsum * gtu);
         acc(B[h_divergence] + (long long)c1 * L + k, -ue[k] * 0.5 * gtu);
         acc(B[h_divergence] + (long long)c2 * L + k, -ue[k] * 0.5 * gtu);
+        double gpe = 0.0;
         for (int j = 0; j < ne; ++j) {
             const int ee = m.edgesOnEdge[e * ME2 + j];
             const double wv = m.weightsOnEdge[e * ME2 + j];
             const double uj = F[u][(long long)ee * L + k], pj = F[pv_edge][(long long)ee * L + k];
-            add(B, u, (long long)ee * L + k, wv * 0.5 * (pvs + pj) * gq);
             const double gp = wv * uj * 0.5 * gq;
-            add(B, pv_edge, i, gp);
-            add(B, pv_edge, (long long)ee * L + k, gp);
+            gpe += gp;
+            if (!gather) {
+                add(B, u, (long long)ee * L + k, wv * 0.5 * (pvs + pj) * gq);
+                add(B, pv_edge, (long long)ee * L + k, gp);
+            }
         }
+        add(B, pv_edge, i, gpe);
+        if (gather) S.g_q[i] = gq;
     }
     {
         // vertical flux of u at interface k (1..L-1): g_wduz(k) = g_delta(k-1) - g_delta(k)
@@ -461,6 +471,54 @@ __device__ __forceinline__ void dyn_a3(const mpas_hip_mesh &m, const mpas_hip_dy
         acc(S.g_delsq_vort + (long long)v2 * L + k, re * r_dv * gtue);
         acc(S.g_delsq_vort + (long long)v1 * L + k, -re * r_dv * gtue);
     }
+}
+
+// ---------------------------------------------------------------- A3 gathers
+// thread (cell c, level / interface k): adjoints of theta_m (k < L) and w
+// (1 <= k < L) at the cells of the advection stencils, gathered over the
+// (edge, slot) pairs naming cell c, in (edge, slot) order
+__device__ __forceinline__ void dyn_a3_cells(const mpas_hip_mesh &m, Fwd F, Bar B, const DynAdjScratch &S,
+                                             const mpas_hip_inverse_tables &inv, int c, int k) {
+    using namespace dyn;
+    const int L = m.nLevels, Lw = L + 1, MA = m.maxAdv;
+    if (c >= m.nCells || k >= L) return;
+    const int j0 = inv.advStart[c], j1 = inv.advStart[c + 1];
+    double gt = 0.0, gw = 0.0;
+    for (int j = j0; j < j1; ++j) {
+        const int e = inv.advEdge[j], a = inv.advSlot[j];
+        const long long i = (long long)e * L + k;
+        const double c0 = m.adv_coefs[e * MA + a], c3 = m.adv_coefs_3rd[e * MA + a];
+        const double rut = F[ru][i];
+        gt += (c0 + sign_pm(rut) * c3) * rut * S.g_tf[i];
+        if (k >= 1) {
+            const double ruw = m.fzm[k] * rut + m.fzp[k] * F[ru][i - 1];
+            gw += (c0 + sign_pm(ruw) * c3) * ruw * S.g_wf[i];
+        }
+    }
+    add_own(B, theta_m, (long long)c * L + k, gt);
+    if (k >= 1) add_own(B, w, (long long)c * Lw + k, gw);
+}
+
+// thread (edge e, level k): adjoints of u and pv_edge at the edges of the
+// PV-flux stencils, gathered over the (edge, slot) pairs naming edge e
+__device__ __forceinline__ void dyn_a3_edges(const mpas_hip_mesh &m, Fwd F, Bar B, const DynAdjScratch &S,
+                                             const mpas_hip_inverse_tables &inv, int e, int k) {
+    using namespace dyn;
+    const int L = m.nLevels, ME2 = m.maxEdges2;
+    if (e >= m.nEdges || k >= L) return;
+    const long long i = (long long)e * L + k;
+    const int j0 = inv.eoeStart[e], j1 = inv.eoeStart[e + 1];
+    const double ue = F[u][i], pve = F[pv_edge][i];
+    double gu = 0.0, gp = 0.0;
+    for (int j = j0; j < j1; ++j) {
+        const int eo = inv.eoeEdge[j], sl = inv.eoeSlot[j];
+        const long long io = (long long)eo * L + k;
+        const double wq = m.weightsOnEdge[eo * ME2 + sl] * S.g_q[io];
+        gu += wq * 0.5 * (F[pv_edge][io] + pve);
+        gp += wq * ue * 0.5;
+    }
+    add_own(B, u, i, gu);
+    add_own(B, pv_edge, i, gp);
 }
 
 // ---------------------------------------------------------------- A4 (rk 1, del4)

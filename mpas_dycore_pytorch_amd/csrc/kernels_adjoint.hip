@@ -62,6 +62,8 @@ struct KArgs {
     ScalAdjScratch sc;
     MonoScratch mo;
     double *col;
+    int gather;
+    mpas_hip_inverse_tables inv;
 };
 
 ADJ_WAVE_KERNEL(k_adj_smlstep, smlstep_vjp(m, ADJ_F, ADJ_B, ent, k))
@@ -82,7 +84,9 @@ ADJ_WAVE_KERNEL(k_adj_dyn_fwd_edges, dyn_fwd_edges(m, ADJ_F, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_fwd_del4, dyn_fwd_del4(m, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_a1, dyn_a1(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_a2, dyn_a2(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
-ADJ_WAVE_KERNEL(k_adj_dyn_a3, dyn_a3(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
+ADJ_WAVE_KERNEL(k_adj_dyn_a3, dyn_a3(m, A.dp, ADJ_F, ADJ_B, A.dy, A.gather != 0, ent, k))
+ADJ_WAVE_KERNEL(k_adj_dyn_a3_cells, dyn_a3_cells(m, ADJ_F, ADJ_B, A.dy, A.inv, ent, k))
+ADJ_WAVE_KERNEL(k_adj_dyn_a3_edges, dyn_a3_edges(m, ADJ_F, ADJ_B, A.dy, A.inv, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_a4, dyn_a4(m, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_a5, dyn_a5(m, ADJ_B, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_a6, dyn_a6(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
@@ -434,7 +438,7 @@ __global__ __launch_bounds__(64 * kColWaves) void k_adj_vimp_colw(mpas_hip_mesh 
 static size_t dyn_adj_doubles(const mpas_hip_mesh &m) {
     const size_t L = m.nLevels, C = (m.nCells + 1) * L, E = (m.nEdges + 1) * L, V = (m.nVertices + 1) * L,
                  W = (m.nCells + 1) * (L + 1);
-    return C + E + C + V + C + W + C + W + C + C + V + E;
+    return C + E + C + V + C + W + C + W + C + C + V + E + 3 * E;
 }
 static DynAdjScratch carve_dyn(const mpas_hip_mesh &m, double *p) {
     const size_t L = m.nLevels, C = (m.nCells + 1) * L, E = (m.nEdges + 1) * L, V = (m.nVertices + 1) * L,
@@ -452,6 +456,9 @@ static DynAdjScratch carve_dyn(const mpas_hip_mesh &m, double *p) {
     S.g_delsq_div = p; p += C;
     S.g_delsq_vort = p; p += V;
     S.g_delsq_u = p; p += E;
+    S.g_tf = p; p += E;
+    S.g_wf = p; p += E;
+    S.g_q = p; p += E;
     return S;
 }
 static size_t mono_adj_doubles(const mpas_hip_mesh &m) {
@@ -633,7 +640,7 @@ int mpas_hip_acoustic_step_vjp(const mpas_hip_mesh *mesh, const mpas_hip_acousti
 }
 
 int mpas_hip_dyn_tend_vjp(const mpas_hip_mesh *mesh, const mpas_hip_dyn_params *p, const double *const *fwd,
-                          double *const *bar, double *scratch, void *stream) {
+                          double *const *bar, double *scratch, const mpas_hip_inverse_tables *inv, void *stream) {
     if (vjp_check(mesh, "dyn_tend_vjp")) return 1;
     if (need(bar, dyn::h_divergence, "dyn_tend_vjp") || need(bar, dyn::kdiff, "dyn_tend_vjp")) return 1;
     const mpas_hip_mesh &m = *mesh;
@@ -642,6 +649,8 @@ int mpas_hip_dyn_tend_vjp(const mpas_hip_mesh *mesh, const mpas_hip_dyn_params *
     KArgs A{};
     A.dp = *p;
     A.dy = carve_dyn(m, scratch);
+    A.gather = inv ? 1 : 0;
+    if (inv) A.inv = *inv;
     const int L = m.nLevels, nC = m.nCells, nE = m.nEdges, nV = m.nVertices;
     const bool rk1 = (p->rk_step == 1);
     fill_zero(scratch, dyn_adj_doubles(m), s);
@@ -654,6 +663,10 @@ int mpas_hip_dyn_tend_vjp(const mpas_hip_mesh *mesh, const mpas_hip_dyn_params *
     ADJ_LAUNCH(k_adj_dyn_a1, nC, L + 1, s);
     if (rk1) ADJ_LAUNCH(k_adj_dyn_a2, nC, L + 1, s);
     ADJ_LAUNCH(k_adj_dyn_a3, nE, L, s);
+    if (inv) {
+        ADJ_LAUNCH(k_adj_dyn_a3_cells, nC, L, s);
+        ADJ_LAUNCH(k_adj_dyn_a3_edges, nE, L, s);
+    }
     if (rk1 && p->h_mom_eddy_visc4 > 0.0) ADJ_LAUNCH(k_adj_dyn_a4, ncv, L, s);
     if (rk1) ADJ_LAUNCH(k_adj_dyn_a5, nE, L, s);
     ADJ_LAUNCH(k_adj_dyn_a6, nC, L, s);

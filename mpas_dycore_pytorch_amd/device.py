@@ -209,11 +209,47 @@ class DeviceMesh:
             self._work[n_scalars] = w
         return w
 
+    def inverse_tables(self) -> "N.InverseTablesStruct":
+        """CSR inverse connectivity of advCellsForEdge and edgesOnEdge (built once).
+
+        For each cell, the (edge, slot) pairs of the advection stencils that
+        name it; for each edge, the (edge, slot) pairs of the PV-flux stencils
+        that name it; both ordered by (edge, slot).  The adjoint kernels gather
+        over them instead of scattering with atomics."""
+        if self._inv is None:
+            t = {}
+            for prefix, table, count, n_targets in (("adv", "advCellsForEdge", "nAdvCellsForEdge", self.nc),
+                                                    ("eoe", "edgesOnEdge", "nEdgesOnEdge", self.ne)):
+                tab = self.t[table][: self.ne].cpu().numpy()
+                cnt = self.t[count][: self.ne].cpu().numpy().astype(np.int64)
+                start, src, slot = build_inverse(tab, cnt, n_targets)
+                t[prefix + "Start"], t[prefix + "Edge"], t[prefix + "Slot"] = start, src, slot
+            dev = {k: torch.as_tensor(v, device=self.device) for k, v in t.items()}
+            s = N.InverseTablesStruct(*[dev[k].data_ptr() for k in ("advStart", "advEdge", "advSlot",
+                                                                     "eoeStart", "eoeEdge", "eoeSlot")])
+            self._inv = (dev, s)
+        return self._inv[1]
+
+    _inv = None
+
     def kernel_scratch(self) -> torch.Tensor:
         if self._kscratch is None:
             n = N.load().mpas_hip_kernel_scratch_doubles(self.nc, self.ne, self.nv, self.L)
             self._kscratch = torch.zeros(int(n), dtype=torch.float64, device=self.device)
         return self._kscratch
+
+
+def build_inverse(table: np.ndarray, counts: np.ndarray, n_targets: int):
+    """CSR transpose of a slot table: (start, source row, slot) ordered by (source row, slot).
+
+    ``table[r, j]`` (0-based, the first ``counts[r]`` slots live) names a
+    target; ``start[t]:start[t+1]`` lists every (r, j) with ``table[r, j] == t``."""
+    rows, slots = np.nonzero(np.arange(table.shape[1])[None, :] < counts[:, None])
+    targets = table[rows, slots].astype(np.int64)
+    order = np.argsort(targets, kind="stable")  # (row, slot) order kept within a target
+    start = np.zeros(n_targets + 1, dtype=np.int64)
+    np.cumsum(np.bincount(targets, minlength=n_targets), out=start[1:])
+    return (start.astype(np.int32), rows[order].astype(np.int32), slots[order].astype(np.int32))
 
 
 def stream_handle(device: torch.device) -> int:

@@ -121,3 +121,25 @@ def test_product_path_has_no_oracle_import():
     for path in glob.glob(os.path.join(pkg, "*.py")):
         text = open(path).read()
         assert "import oracle" not in text and "from oracle" not in text, path
+
+
+def test_inverse_tables_transpose_the_slot_tables():
+    """CSR inverses of advCellsForEdge / edgesOnEdge list every (edge, slot) naming a target, in order."""
+    from mpas_dycore_pytorch_amd.device import build_inverse
+
+    st, _ = _case()
+    dm = DeviceMesh(st, torch.device("cpu"))
+    for table, count, n_targets in (("advCellsForEdge", "nAdvCellsForEdge", dm.nc),
+                                    ("edgesOnEdge", "nEdgesOnEdge", dm.ne)):
+        tab = dm.t[table][: dm.ne].numpy()
+        cnt = dm.t[count][: dm.ne].numpy().astype(np.int64)
+        start, src, slot = build_inverse(tab, cnt, n_targets)
+        assert start[0] == 0 and start[-1] == int(cnt.sum()) and np.all(np.diff(start) >= 0)
+        seen = set()
+        for t in range(n_targets):
+            pairs = list(zip(src[start[t]:start[t + 1]].tolist(), slot[start[t]:start[t + 1]].tolist()))
+            assert pairs == sorted(pairs)
+            for e, j in pairs:
+                assert j < cnt[e] and tab[e, j] == t
+                seen.add((e, j))
+        assert len(seen) == int(cnt.sum())
