@@ -39,8 +39,8 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def prepare(level: int, n_levels: int, stretch: str, n_tracers: int, dt: float, seed: int, device):
-    """Mesh, raw synthetic fields and the cold-start chain (diagnostics on the GPU)."""
+def prepare(level: int, n_levels: int, stretch: str, n_tracers: int, dt: float, seed: int, device, m=None):
+    """Mesh (built unless given), raw synthetic fields and the cold-start chain (diagnostics on the GPU)."""
     import numpy as np
 
     from mpas_dycore_pytorch_amd import initial_state as IS
@@ -49,7 +49,8 @@ def prepare(level: int, n_levels: int, stretch: str, n_tracers: int, dt: float, 
     from mpas_dycore_pytorch_amd.device import DeviceMesh
 
     t0 = time.time()
-    m = M.build_mesh(level, n_levels, stretch=stretch)
+    if m is None:
+        m = M.build_mesh(level, n_levels, stretch=stretch)
     raw = IS.raw_state(m, seed=seed, n_tracers=n_tracers)
     base = {**m, **raw, **IS.init_coupled_diagnostics(m, raw)}
     dm = DeviceMesh(base, device)
@@ -62,6 +63,7 @@ def prepare(level: int, n_levels: int, stretch: str, n_tracers: int, dt: float, 
         return {k: v.cpu().numpy() for k, v in K.reconstruct(dm, u).items()}
 
     st = IS.prepare_state(m, raw, diagnostics=diag, reconstruct=recon)
+    del dm
     log(f"prepared level {level} x {n_levels} ({m['nCells']} cells) in {time.time() - t0:.1f} s")
     return m, st, np
 
@@ -143,6 +145,75 @@ def vjp_leg(m, stepper, config, dt: float, steps: int, fwd_ms_per_step: float, d
             "checkpoint": "per large step (acoustic substeps recomputed)"}
 
 
+ENSEMBLE_DT = {4: 1800.0, 5: 900.0, 6: 450.0, 7: 300.0, 8: 150.0}
+
+
+def ensemble_bench(args, device) -> int:
+    """Uneven ensemble: members on several mesh levels, each with its own seeded
+    theta perturbation and the level's dt.  Batched (one grid per kernel per
+    mesh, HIP graphs) against the members stepped one by one."""
+    import torch
+
+    from mpas_dycore_pytorch_amd.ensemble import EnsembleStepper
+    from mpas_dycore_pytorch_amd.time_integration import Srk3Stepper
+
+    spec = [tuple(int(x) for x in part.split("x")) for part in args.ensemble.split(",")]
+    members = []
+    for level, count in spec:
+        dt = ENSEMBLE_DT[level]
+        m, st0, _ = prepare(level, args.levels, "quadratic", 1, dt, args.seed, device)
+        mesh = {**m, **{k: st0[k] for k in ("rho_base", "rtheta_base", "exner_base") if k in st0}}
+        cfg = {"config_time_integration_order": 3, "config_len_disp": float(m["nominalMinDc"])}
+        members.append((st0, mesh, dt, cfg))
+        for i in range(1, count):
+            _, st, _ = prepare(level, args.levels, "quadratic", 1, dt, args.seed + 1000 * level + i, device, m=m)
+            members.append((st, mesh, dt, cfg))
+        log(f"level {level}: {count} members prepared")
+    cell_levels = sum(int(mem[1]["nCells"]) * args.levels for mem in members)
+    steps = args.steps
+
+    ens = EnsembleStepper(members, reorder=bool(args.reorder))
+    ens.step(1)
+    ens.capture_graphs()
+    ens.step(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.step(steps)
+    torch.cuda.synchronize()
+    batched_s = time.perf_counter() - t0
+    batched = cell_levels * steps / batched_s
+    log(f"batched: {1e3 * batched_s / steps:.2f} ms per ensemble step, {batched:.4e} cell-level-steps/s")
+    del ens
+    torch.cuda.empty_cache()
+
+    singles = []
+    for st, mesh, dt, cfg in members:
+        sp = Srk3Stepper(st, mesh, cfg, dt, device=device, reorder=bool(args.reorder))
+        sp.step(1)
+        sp.capture_graph()
+        singles.append(sp)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for sp in singles:
+        sp.step(steps)
+    torch.cuda.synchronize()
+    single_s = time.perf_counter() - t0
+    one_by_one = cell_levels * steps / single_s
+    log(f"one by one: {1e3 * single_s / steps:.2f} ms per ensemble step, {one_by_one:.4e} cell-level-steps/s")
+    out = {
+        "metric": "cell-level-steps/s of an uneven ensemble (batched vs one by one)", "value": batched,
+        "unit": "cell-level-steps/s", "n_gpus": 1, "steps": steps, "warmup": 2,
+        "ms_per_step": 1e3 * batched_s / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "uneven ensemble " + ", ".join(f"{c} x level {lv} (dt {ENSEMBLE_DT[lv]:g} s)"
+                                                                for lv, c in spec) + f", {args.levels} levels",
+                   "members": len(members), "cell_levels_per_step": cell_levels},
+        "one_by_one": one_by_one, "speedup_vs_one_by_one": round(batched / one_by_one, 3),
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +229,10 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--reorder", type=int, default=1, help="1: space-filling-curve mesh order on the device")
     ap.add_argument("--graph", type=int, default=0, help="1: replay the large step from a captured HIP graph")
+    ap.add_argument("--ensemble", type=str, default="",
+                    help="measure an uneven ensemble instead, e.g. '5x32,6x16,7x4' (level x members): the batched "
+                         "form (one grid per kernel per mesh, HIP graphs, groups on their own streams) against the "
+                         "members stepped one by one")
     ap.add_argument("--vjp-steps", type=int, default=2,
                     help="large steps of the reverse-mode leg (forward + VJP of an area-weighted squared "
                          "theta perturbation, one checkpoint per large step); 0 skips it")
@@ -180,6 +255,9 @@ def main() -> int:
     device = torch.device("cuda", 0)
     torch.cuda.set_device(device)
     lib = N.load()
+
+    if args.ensemble:
+        return ensemble_bench(args, device)
 
     m, st, np = prepare(args.level, args.levels, "quadratic", 0, args.dt, args.seed, device)
     nC, nE, nV, L = m["nCells"], m["nEdges"], m["nVertices"], args.levels
