@@ -229,6 +229,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--reorder", type=int, default=1, help="1: space-filling-curve mesh order on the device")
     ap.add_argument("--graph", type=int, default=0, help="1: replay the large step from a captured HIP graph")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="1: tune the launch shape of each wavefront-per-entity kernel during warm-up")
     ap.add_argument("--ensemble", type=str, default="",
                     help="measure an uneven ensemble instead, e.g. '5x32,6x16,7x4' (level x members): the batched "
                          "form (one grid per kernel per mesh, HIP graphs, groups on their own streams) against the "
@@ -266,6 +268,14 @@ def main() -> int:
     S = stepper.n_scalars
 
     stepper.step(args.warmup)
+    tuning = None
+    if args.autotune:
+        from mpas_dycore_pytorch_amd import autotune as AT
+
+        t_tune = time.time()
+        tuning = AT.autotune(stepper)
+        log(f"autotuned {len(tuning)} kernels in {time.time() - t_tune:.1f} s: "
+            + ", ".join(f"{k}={v['waves']}" for k, v in tuning.items()))
     if args.graph:
         stepper.capture_graph()
     torch.cuda.synchronize()
@@ -337,7 +347,8 @@ def main() -> int:
                                 f"substeps, monotonic FCT on {S} scalar(s), flat terrain"),
                    "cells": nC, "levels": L, "dt": args.dt, "parallelism": "single GPU",
                    "mesh_order": "hilbert" if args.reorder else "generator",
-                   "launch": "hip graph" if args.graph else "stream"},
+                   "launch": "hip graph" if args.graph else "stream",
+                   "waves_per_block": ({k: v["waves"] for k, v in tuning.items()} if tuning else "default 4")},
         "roofline": roof, "step_roofline": step_roof, "vjp_fwd_ratio": (vjp or {}).get("ratio"), "vjp": vjp, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -286,6 +286,13 @@ int mpas_hip_advance_scalars_mono_vjp(const mpas_hip_mesh *m, int32_t nScalars, 
 /* fwd: u | uReconstructX, Y, Z, Zonal, Meridional */
 int mpas_hip_reconstruct_vjp(const mpas_hip_mesh *m, const double *const *fwd, double *const *bar, void *stream);
 
+/* Launch-shape table, no counterpart in the reference (its kernels are torch
+ * operators): wavefronts per block (1, 2 or 4; 0 restores the default 4) of a
+ * wavefront-per-entity kernel, by HIP kernel name (e.g. "k_dyn_edges_a").
+ * Written by the autotuner (autotune.py) and read at every launch. */
+int mpas_hip_tuning_set(const char *kernel_name, int32_t waves_per_block);
+int32_t mpas_hip_tuning_get(const char *kernel_name);
+
 const char *mpas_hip_last_error(void);
 
 /* Measurement hook, no counterpart in the reference (its timings come from

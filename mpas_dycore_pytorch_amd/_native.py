@@ -113,6 +113,8 @@ SIGNATURES = {
     "mpas_hip_advance_scalars_vjp": (C.c_int, [_PM, _I32, _F64, _F64, _F64, _F64, _PP, _PP, _dp, _vp]),
     "mpas_hip_advance_scalars_mono_vjp": (C.c_int, [_PM, _I32, _F64, _F64, _PP, _PP, _dp, _vp]),
     "mpas_hip_reconstruct_vjp": (C.c_int, [_PM, _PP, _PP, _vp]),
+    "mpas_hip_tuning_set": (C.c_int, [C.c_char_p, _I32]),
+    "mpas_hip_tuning_get": (_I32, [C.c_char_p]),
     "mpas_hip_last_error": (C.c_char_p, []),
     "mpas_hip_timer_arm": (C.c_int, [C.c_char_p, _I32]),
     "mpas_hip_timer_read": (C.c_int, [C.POINTER(C.c_double), C.POINTER(_I32)]),

@@ -68,22 +68,28 @@ __device__ __forceinline__ long long xcd_block(long long nblocks) {
 // Each wavefront walks kEntPerWave consecutive entities (1 measured fastest
 // at level 7 x 55 on MI355X: walking 4 did not raise L1 reuse enough to
 // offset the lost parallelism).
+// The number of wavefronts per block (1, 2 or 4) is a launch-time choice per
+// kernel: kWaves unless the autotuner set another (tuned_waves); the kernels
+// read it back from blockDim.
 constexpr int kWaves = 4;
 constexpr int kEntPerWave = 1;
-inline long long wave_blocks(long long n_entities) {
-    return (n_entities + kWaves * kEntPerWave - 1) / (kWaves * kEntPerWave);
+int tuned_waves(const char *kernel_name);
+inline long long wave_blocks(long long n_entities, int waves = kWaves) {
+    return (n_entities + waves * kEntPerWave - 1) / (waves * kEntPerWave);
 }
 __device__ __forceinline__ int wave_entity_base(long long nblocks) {
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    return ((int)(xcd_block(nblocks) * kWaves) + w) * kEntPerWave;
+    const int waves = (int)(blockDim.x >> 6);
+    return ((int)(xcd_block(nblocks) * waves) + w) * kEntPerWave;
 }
 __device__ __forceinline__ int wave_lane() { return (int)(threadIdx.x & 63); }
 
 #define MPAS_WAVE_LAUNCH(kernel, n_entities, stream, ...)                                          \
     do {                                                                                           \
-        const long long nb_ = ::mpas::wave_blocks(n_entities);                                     \
-        MPAS_LAUNCH(kernel, dim3(::mpas::xcd_grid(nb_)), dim3(64 * ::mpas::kWaves), 0, stream,     \
-                    nb_, __VA_ARGS__);                                                             \
+        const int wpb_ = ::mpas::tuned_waves(#kernel);                                             \
+        const long long nb_ = ::mpas::wave_blocks(n_entities, wpb_);                               \
+        MPAS_LAUNCH(kernel, dim3(::mpas::xcd_grid(nb_)), dim3(64 * wpb_), 0, stream, nb_,          \
+                    __VA_ARGS__);                                                                  \
     } while (0)
 
 // Zero-fill by kernel rather than hipMemsetAsync so launch sequences stay

@@ -429,8 +429,9 @@ __global__ __launch_bounds__(64 * kColWaves) void k_adj_vimp_colw(mpas_hip_mesh 
 
 #define ADJ_LAUNCH(kernel, n_ent, n_lev, stream)                                                      \
     do {                                                                                              \
-        const long long nb_ = ::mpas::wave_blocks(n_ent);                                             \
-        MPAS_LAUNCH(kernel, dim3(::mpas::xcd_grid(nb_)), dim3(64 * ::mpas::kWaves), 0, stream, nb_,   \
+        const int wpb_ = ::mpas::tuned_waves(#kernel);                                                \
+        const long long nb_ = ::mpas::wave_blocks(n_ent, wpb_);                                       \
+        MPAS_LAUNCH(kernel, dim3(::mpas::xcd_grid(nb_)), dim3(64 * wpb_), 0, stream, nb_,             \
                     (long long)(n_ent), (int)(n_lev), m, S_, A);                                      \
     } while (0)
 

@@ -42,6 +42,25 @@ static void timer_release() {
     g_timer.armed = false;
 }
 
+// ---------------------------------------------------------------------------
+// launch-shape table written by the autotuner: wavefronts per block of the
+// wavefront-per-entity kernels (fixed capacity, so lookups never see a
+// reallocation)
+// ---------------------------------------------------------------------------
+struct TuneEntry {
+    char name[64];
+    int waves;
+};
+constexpr int kTuneCapacity = 128;
+static TuneEntry g_tune[kTuneCapacity];
+static int g_tune_n = 0;
+
+int tuned_waves(const char *kernel_name) {
+    for (int i = 0; i < g_tune_n; ++i)
+        if (g_tune[i].waves > 0 && std::strcmp(g_tune[i].name, kernel_name) == 0) return g_tune[i].waves;
+    return kWaves;
+}
+
 int check_launch(const char *what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -192,6 +211,33 @@ static int validate_moist(int S, int ms, int me) {
 using namespace mpas;
 
 extern "C" {
+
+int mpas_hip_tuning_set(const char *kernel_name, int32_t waves_per_block) {
+    if (!kernel_name || std::strlen(kernel_name) >= sizeof(g_tune[0].name)) {
+        set_error("tuning_set: kernel name missing or too long");
+        return 1;
+    }
+    if (waves_per_block != 0 && waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4) {
+        set_error("tuning_set: waves per block must be 1, 2 or 4 (0 restores the default)");
+        return 1;
+    }
+    for (int i = 0; i < g_tune_n; ++i)
+        if (std::strcmp(g_tune[i].name, kernel_name) == 0) {
+            g_tune[i].waves = waves_per_block;
+            return 0;
+        }
+    if (waves_per_block == 0) return 0;
+    if (g_tune_n == kTuneCapacity) {
+        set_error("tuning_set: table full");
+        return 1;
+    }
+    std::strcpy(g_tune[g_tune_n].name, kernel_name);
+    g_tune[g_tune_n].waves = waves_per_block;
+    ++g_tune_n;
+    return 0;
+}
+
+int32_t mpas_hip_tuning_get(const char *kernel_name) { return kernel_name ? tuned_waves(kernel_name) : kWaves; }
 
 const char *mpas_hip_last_error(void) {
     return g_last_error.c_str();

@@ -143,3 +143,13 @@ def test_inverse_tables_transpose_the_slot_tables():
                 assert j < cnt[e] and tab[e, j] == t
                 seen.add((e, j))
         assert len(seen) == int(cnt.sum())
+
+
+def test_launch_shape_table():
+    lib = N.load()
+    assert lib.mpas_hip_tuning_get(b"k_dyn_edges_a") == 4
+    assert lib.mpas_hip_tuning_set(b"k_dyn_edges_a", 2) == 0
+    assert lib.mpas_hip_tuning_get(b"k_dyn_edges_a") == 2
+    assert lib.mpas_hip_tuning_set(b"k_dyn_edges_a", 3) != 0  # not 1, 2 or 4
+    assert lib.mpas_hip_tuning_set(b"k_dyn_edges_a", 0) == 0
+    assert lib.mpas_hip_tuning_get(b"k_dyn_edges_a") == 4
