@@ -286,6 +286,46 @@ int mpas_hip_advance_scalars_mono_vjp(const mpas_hip_mesh *m, int32_t nScalars, 
 /* fwd: u | uReconstructX, Y, Z, Zonal, Meridional */
 int mpas_hip_reconstruct_vjp(const mpas_hip_mesh *m, const double *const *fwd, double *const *bar, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Tangents (Jacobian-vector products).  The reference obtains them with torch
+ * forward-mode AD (dual numbers) through each atm_*_work function and the
+ * jvp rules of TridiagonalFactorize / TridiagonalSolve
+ * (tridiagonal_factor.py:233, tridiagonal_solve.py:294), driven by
+ * run_simulation.jvp (atmosphere_model.py:2041-2098).  Each entry point below
+ * is the tangent of the forward entry point of the same name, with the slot
+ * order of its *_vjp counterpart above:
+ *   fwd[]  device pointers: the forward call's inputs, then its outputs;
+ *   dot[]  tangents of the same slots: input tangents are read (a null slot
+ *          is a zero tangent), output tangents are written (all must be
+ *          non-null).
+ * Derivatives follow the adjoints' conventions (branches frozen, torch's
+ * one-sided values at switching points), so each is the exact transpose of
+ * its *_vjp.  Every thread gathers; results are deterministic.  Entry points
+ * taking scratch use the same buffer size as the *_vjp ones. */
+int mpas_hip_moist_coefficients_jvp(const mpas_hip_mesh *m, int32_t nScalars, int32_t moist_start,
+                                    int32_t moist_end, const double *const *fwd, double *const *dot,
+                                    void *stream);
+int mpas_hip_vert_imp_coefs_jvp(const mpas_hip_mesh *m, double dts, double epssm, const double *const *fwd,
+                                double *const *dot, void *stream);
+int mpas_hip_solve_diagnostics_jvp(const mpas_hip_mesh *m, double dt, double apvm, int32_t rk_step,
+                                   const double *const *fwd, double *const *dot, double *scratch, void *stream);
+int mpas_hip_dyn_tend_jvp(const mpas_hip_mesh *m, const mpas_hip_dyn_params *p, const double *const *fwd,
+                          double *const *dot, double *scratch, void *stream);
+int mpas_hip_smlstep_pert_jvp(const mpas_hip_mesh *m, const double *const *fwd, double *const *dot, void *stream);
+int mpas_hip_acoustic_step_jvp(const mpas_hip_mesh *m, const mpas_hip_acoustic_params *p,
+                               const double *const *fwd, double *const *dot, double *scratch, void *stream);
+int mpas_hip_divergence_damping_jvp(const mpas_hip_mesh *m, double dts, double smdiv, double len_disp,
+                                    const double *const *fwd, double *const *dot, void *stream);
+int mpas_hip_recover_large_step_jvp(const mpas_hip_mesh *m, int32_t rk_step, double dt, int32_t ns,
+                                    const double *const *fwd, double *const *dot, void *stream);
+int mpas_hip_advance_scalars_jvp(const mpas_hip_mesh *m, int32_t nScalars, double dt, double coef3,
+                                 double weight_old, double weight_new, const double *const *fwd,
+                                 double *const *dot, void *stream);
+int mpas_hip_advance_scalars_mono_jvp(const mpas_hip_mesh *m, int32_t nScalars, double dt, double coef3,
+                                      const double *const *fwd, double *const *dot, double *scratch,
+                                      void *stream);
+int mpas_hip_reconstruct_jvp(const mpas_hip_mesh *m, const double *const *fwd, double *const *dot, void *stream);
+
 /* Launch-shape table, no counterpart in the reference (its kernels are torch
  * operators): wavefronts per block (1, 2 or 4; 0 restores the default 4) of a
  * wavefront-per-entity kernel, by HIP kernel name (e.g. "k_dyn_edges_a").

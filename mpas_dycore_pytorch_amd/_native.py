@@ -113,6 +113,17 @@ SIGNATURES = {
     "mpas_hip_advance_scalars_vjp": (C.c_int, [_PM, _I32, _F64, _F64, _F64, _F64, _PP, _PP, _dp, _vp]),
     "mpas_hip_advance_scalars_mono_vjp": (C.c_int, [_PM, _I32, _F64, _F64, _PP, _PP, _dp, _vp]),
     "mpas_hip_reconstruct_vjp": (C.c_int, [_PM, _PP, _PP, _vp]),
+    "mpas_hip_moist_coefficients_jvp": (C.c_int, [_PM, _I32, _I32, _I32, _PP, _PP, _vp]),
+    "mpas_hip_vert_imp_coefs_jvp": (C.c_int, [_PM, _F64, _F64, _PP, _PP, _vp]),
+    "mpas_hip_solve_diagnostics_jvp": (C.c_int, [_PM, _F64, _F64, _I32, _PP, _PP, _dp, _vp]),
+    "mpas_hip_dyn_tend_jvp": (C.c_int, [_PM, C.POINTER(DynParamsStruct), _PP, _PP, _dp, _vp]),
+    "mpas_hip_smlstep_pert_jvp": (C.c_int, [_PM, _PP, _PP, _vp]),
+    "mpas_hip_acoustic_step_jvp": (C.c_int, [_PM, C.POINTER(AcousticParamsStruct), _PP, _PP, _dp, _vp]),
+    "mpas_hip_divergence_damping_jvp": (C.c_int, [_PM, _F64, _F64, _F64, _PP, _PP, _vp]),
+    "mpas_hip_recover_large_step_jvp": (C.c_int, [_PM, _I32, _F64, _I32, _PP, _PP, _vp]),
+    "mpas_hip_advance_scalars_jvp": (C.c_int, [_PM, _I32, _F64, _F64, _F64, _F64, _PP, _PP, _vp]),
+    "mpas_hip_advance_scalars_mono_jvp": (C.c_int, [_PM, _I32, _F64, _F64, _PP, _PP, _dp, _vp]),
+    "mpas_hip_reconstruct_jvp": (C.c_int, [_PM, _PP, _PP, _vp]),
     "mpas_hip_tuning_set": (C.c_int, [C.c_char_p, _I32]),
     "mpas_hip_tuning_get": (_I32, [C.c_char_p]),
     "mpas_hip_last_error": (C.c_char_p, []),

@@ -15,7 +15,11 @@ checkpointing each large step (``autodiff/checkpoint.py:106``).  Here:
 * :func:`step_vjp` is the VJP of one large step; the multi-step driver in
   ``atmosphere_model`` stores the state at every large step and recomputes
   one step's tape at a time in the backward sweep (acoustic substeps are
-  recomputed, never stored across steps).
+  recomputed, never stored across steps);
+* :meth:`Tape.forward_tangents` walks the same tape forward through the
+  ``mpas_hip_*_jvp`` entry points (the tangent-linear model), and
+  :func:`step_jvp` is the JVP of one large step (the reference's forward-mode
+  ``run_simulation.jvp``, ``atmosphere_model.py:2041-2098``).
 
 Derivatives are those of the hard primal (branch selections frozen) with
 torch's conventions at switching points, as the reference's primitives define
@@ -180,6 +184,87 @@ class Tape:
             for t in op.outs:
                 bars.pop(id(t), None)
         return bars
+
+
+    # ---- forward (tangent) sweep
+    def forward_tangents(self, seeds: Mapping[int, torch.Tensor], scratch: torch.Tensor,
+                         keep: set[int] | None = None) -> dict[int, torch.Tensor]:
+        """Tangents of every tensor reached from ``seeds``, keyed by ``id(tensor)``.
+
+        ``seeds`` maps ``id(input tensor)`` to its tangent.  A tangent is
+        dropped after the last call that reads it unless its id is in
+        ``keep``; tensors no seed reaches get no tangent."""
+        dots: dict[int, torch.Tensor] = dict(seeds)
+        lib = N.load()
+        dm = self.dm
+        stream = C.c_void_p(stream_handle(dm.device))
+        keep = set(keep or ())
+        last: dict[int, int] = {}
+        for i, op in enumerate(self.ops):
+            for t in op.ins:
+                last[id(t)] = i
+        for i, op in enumerate(self.ops):
+            tin = [dots.get(id(t)) for t in op.ins]
+            if op.kind in ("add", "scale", "copy", "mask_pad"):
+                live = [d for d in tin if d is not None]
+                if live:
+                    if op.kind == "add":
+                        out = live[0] if len(live) == 1 else live[0] + live[1]
+                    elif op.kind == "scale":
+                        out = live[0] * op.args[0]
+                    elif op.kind == "copy":
+                        out = live[0]
+                    else:
+                        out = live[0].clone()
+                        out[op.args[0]] = 0.0
+                    dots[id(op.outs[0])] = out
+            else:
+                si, so = SLOTS[op.kind]
+                const = CONSTANT_IN.get(op.kind, ())
+                tin = [None if name in const else d for name, d in zip(si, tin)]
+                if any(d is not None for d in tin):
+                    tout = [torch.empty_like(t) for t in op.outs]
+                    fwd = (C.c_void_p * (len(op.ins) + len(op.outs)))(*[x.data_ptr() for x in op.ins + op.outs])
+                    dot = (C.c_void_p * (len(op.ins) + len(op.outs)))(
+                        *[(d.data_ptr() if d is not None else None) for d in tin + tout])
+                    _call_jvp(lib, op, dm, fwd, dot, scratch, stream)
+                    for t, d in zip(op.outs, tout):
+                        dots[id(t)] = d
+            for t in op.ins:
+                if last.get(id(t)) == i and id(t) not in keep:
+                    dots.pop(id(t), None)
+        return dots
+
+
+def _call_jvp(lib, op: Op, dm: DeviceMesh, fwd, dot, scratch: torch.Tensor, stream) -> None:
+    a = op.args
+    sp = C.c_void_p(scratch.data_ptr())
+    k = op.kind
+    if k == "moist_coefficients":
+        rc = lib.mpas_hip_moist_coefficients_jvp(dm.ref, *a, fwd, dot, stream)
+    elif k == "vert_imp_coefs":
+        rc = lib.mpas_hip_vert_imp_coefs_jvp(dm.ref, *a, fwd, dot, stream)
+    elif k == "solve_diagnostics":
+        rc = lib.mpas_hip_solve_diagnostics_jvp(dm.ref, *a, fwd, dot, sp, stream)
+    elif k == "dyn_tend":
+        rc = lib.mpas_hip_dyn_tend_jvp(dm.ref, C.byref(a[0]), fwd, dot, sp, stream)
+    elif k == "smlstep_pert":
+        rc = lib.mpas_hip_smlstep_pert_jvp(dm.ref, fwd, dot, stream)
+    elif k == "acoustic_step":
+        rc = lib.mpas_hip_acoustic_step_jvp(dm.ref, C.byref(a[0]), fwd, dot, sp, stream)
+    elif k == "divergence_damping":
+        rc = lib.mpas_hip_divergence_damping_jvp(dm.ref, *a, fwd, dot, stream)
+    elif k == "recover_large_step":
+        rc = lib.mpas_hip_recover_large_step_jvp(dm.ref, *a, fwd, dot, stream)
+    elif k == "advance_scalars":
+        rc = lib.mpas_hip_advance_scalars_jvp(dm.ref, *a, fwd, dot, stream)
+    elif k == "advance_scalars_mono":
+        rc = lib.mpas_hip_advance_scalars_mono_jvp(dm.ref, *a, fwd, dot, sp, stream)
+    elif k == "reconstruct":
+        rc = lib.mpas_hip_reconstruct_jvp(dm.ref, fwd, dot, stream)
+    else:  # pragma: no cover - every recorded kind is handled above
+        raise KeyError(k)
+    N.check(rc, k + "_jvp")
 
 
 def _call_vjp(lib, op: Op, dm: DeviceMesh, fwd, bar, scratch: torch.Tensor, stream) -> None:
@@ -413,3 +498,30 @@ def step_vjp(dm: DeviceMesh, state: Mapping[str, torch.Tensor], cfg: Any, dt: fl
         b = bars.get(id(state[k]))
         grads[k] = b if b is not None else torch.zeros_like(state[k])
     return out, grads
+
+
+def step_jvp(dm: DeviceMesh, state: Mapping[str, torch.Tensor], cfg: Any, dt: float,
+             tangents: Mapping[str, torch.Tensor], scratch: torch.Tensor | None = None
+             ) -> tuple[dict[str, torch.Tensor], dict[str, torch.Tensor]]:
+    """(new state, tangents of the new state) for input tangents ``tangents``.
+
+    ``tangents`` maps keys of :data:`STEP_INPUTS` to their tangents; the step
+    is run on a tape from ``state`` and the tape is swept forward.  Output
+    keys that no input tangent reaches get no entry."""
+    tape = Tape(dm)
+    out = taped_step(dm, state, cfg, dt, tape)
+    seeds = {}
+    for k, t in tangents.items():
+        if t is None or k not in STEP_INPUTS:
+            continue
+        x = state[k]
+        seeds[id(x)] = t.to(device=x.device, dtype=torch.float64).contiguous()
+    if scratch is None:
+        scratch = vjp_scratch(dm, int(state["scalars"].shape[2]))
+    keep = {id(v) for v in out.values() if isinstance(v, torch.Tensor)}
+    dots = tape.forward_tangents(seeds, scratch, keep)
+    res = {}
+    for k, v in out.items():
+        if isinstance(v, torch.Tensor) and id(v) in dots:
+            res[k] = dots[id(v)]
+    return out, res

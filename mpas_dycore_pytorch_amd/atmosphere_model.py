@@ -194,11 +194,49 @@ def _run_simulation_vjp(state: dict[str, Any], mesh: dict[str, Any], objective: 
 
 def _run_simulation_jvp(state: dict[str, Any], mesh: dict[str, Any], perturbation: dict[str, Any],
                         **kwargs: Any) -> dict[str, Any]:
-    """Tangent-linear model (reference ``run_simulation.jvp``): not provided by
-    the GPU step, which implements reverse mode only."""
-    raise MpasUnsupportedCapabilityError(
-        "run_simulation.jvp: the GPU step provides reverse mode (run_simulation.vjp) only",
-        capability="jvp", requested=True)
+    """Tangent-linear model (reference ``run_simulation.jvp``,
+    atmosphere_model.py:2041-2098): forward-mode dual numbers through
+    ``run_simulation(differentiable=True)``, whose every large step takes its
+    tangent from the hand-written tangent-linear kernels.
+
+    ``perturbation`` maps state keys to tangents of the same shape, dtype and
+    device; other entries are constants.  Returns the tangents of the output
+    state's tensors."""
+    from torch.autograd import forward_ad
+
+    from .adjoint import STEP_INPUTS
+
+    kwargs.setdefault("differentiable", True)
+    with forward_ad.dual_level():
+        dual_state = dict(state)
+        for key, tangent in perturbation.items():
+            if not isinstance(tangent, torch.Tensor):
+                raise TypeError(f"Perturbation '{key}' must be a tensor, got {type(tangent).__name__}")
+            if key not in dual_state:
+                raise KeyError(f"Perturbation key '{key}' is missing from state")
+            value = dual_state[key]
+            if not isinstance(value, torch.Tensor):
+                raise TypeError(f"State entry '{key}' is not a tensor and cannot receive a tangent")
+            if tangent.shape != value.shape:
+                raise MpasConfigError(f"Perturbation '{key}' has shape {tuple(tangent.shape)}; "
+                                      f"expected {tuple(value.shape)}")
+            if tangent.dtype != value.dtype:
+                raise MpasConfigError(f"Perturbation '{key}' has dtype {tangent.dtype}; expected {value.dtype}")
+            if tangent.device != value.device:
+                raise MpasConfigError(f"Perturbation '{key}' is on {tangent.device}; expected {value.device}")
+            if key not in STEP_INPUTS:
+                raise MpasUnsupportedCapabilityError(
+                    f"run_simulation.jvp: '{key}' is not an input of the large step; tangents are taken "
+                    f"with respect to {', '.join(STEP_INPUTS)}", capability="jvp_input", requested=key)
+            dual_state[key] = forward_ad.make_dual(value, tangent)
+        out = run_simulation(dual_state, mesh, **kwargs)
+        result: dict[str, Any] = {}
+        for key, val in out.items():
+            if isinstance(val, torch.Tensor):
+                _, t = forward_ad.unpack_dual(val)
+                if t is not None:
+                    result[key] = t
+        return result
 
 
 run_simulation.vjp = _run_simulation_vjp  # type: ignore[attr-defined]

@@ -268,15 +268,17 @@ class Srk3Stepper:
 
 
 class _Srk3Step(torch.autograd.Function):
-    """One large step as a single autograd node.
+    """One large step as a single autograd node, in both modes.
 
     The forward runs the fused step and keeps only the step's input state
     (the per-large-step checkpoint).  The backward recomputes the step from
     it on a tape of per-function kernels, acoustic substeps included, and
-    applies the hand-written adjoint kernels (``adjoint.step_vjp``)."""
+    applies the hand-written adjoint kernels (``adjoint.step_vjp``); under
+    ``torch.autograd.forward_ad`` the jvp runs the same tape forward through
+    the tangent-linear kernels (``adjoint.step_jvp``)."""
 
     @staticmethod
-    def forward(ctx, box: dict, *inputs: torch.Tensor):
+    def forward(box: dict, *inputs: torch.Tensor):
         from .adjoint import STEP_INPUTS
 
         state = dict(box["state"])
@@ -294,12 +296,17 @@ class _Srk3Step(torch.autograd.Function):
         cfg = N.ConfigStruct()
         C.pointer(cfg)[0] = stepper.cfg
         cfg.diagnostics_current = 0
-        ctx.dm, ctx.cfg, ctx.dt, ctx.ckpt = stepper.mesh, cfg, float(box["dt"]), ckpt
-        ctx.in_keys, ctx.out_keys = list(box["in_keys"]), out_keys
-        ctx.in_devices = [x.device for x in inputs]
+        box["_ctx"] = (stepper.mesh, cfg, float(box["dt"]), ckpt, list(box["in_keys"]), out_keys,
+                       [x.device for x in inputs])
         if out_device.type != "cuda":
             torch.cuda.synchronize(stepper.device)
         return tuple(new[k].to(out_device) for k in out_keys)
+
+    @staticmethod
+    def setup_context(ctx, inputs, output):
+        box = inputs[0]
+        ctx.dm, ctx.cfg, ctx.dt, ctx.ckpt, ctx.in_keys, ctx.out_keys, ctx.in_devices = box.pop("_ctx")
+        ctx.out_like = [(o.shape, o.dtype, o.device) for o in output]
 
     @staticmethod
     def backward(ctx, *grads: torch.Tensor):
@@ -310,6 +317,21 @@ class _Srk3Step(torch.autograd.Function):
         with torch.cuda.device(dev):
             _, gin = step_vjp(ctx.dm, ctx.ckpt, ctx.cfg, ctx.dt, cot)
         return (None,) + tuple(gin[k].to(d) for k, d in zip(ctx.in_keys, ctx.in_devices))
+
+    @staticmethod
+    def jvp(ctx, _box_tangent, *tangents: torch.Tensor):
+        from .adjoint import step_jvp
+
+        dev = ctx.dm.device
+        tan = {k: t.to(device=dev, dtype=torch.float64) for k, t in zip(ctx.in_keys, tangents) if t is not None}
+        with torch.cuda.device(dev):
+            _, tout = step_jvp(ctx.dm, ctx.ckpt, ctx.cfg, ctx.dt, tan)
+        res = []
+        for k, (shape, dtype, device) in zip(ctx.out_keys, ctx.out_like):
+            t = tout.get(k)
+            res.append(t.to(device=device, dtype=dtype) if t is not None
+                       else torch.zeros(shape, dtype=dtype, device=device))
+        return tuple(res)
 
 
 def _atm_srk3_differentiable(state: Mapping[str, Any], mesh: Mapping[str, Any], config: Mapping[str, Any],
@@ -333,8 +355,9 @@ def atm_srk3(state: Mapping[str, Any], mesh: Mapping[str, Any], config: Mapping[
     layout; the step runs on the current ROCm device and the results come
     back on the device of ``state["u"]``.  With ``differentiable=True`` the
     returned tensors carry an autograd graph whose backward applies the
-    hand-written adjoint kernels (reverse mode; forward-mode dual numbers are
-    not supported).
+    hand-written adjoint kernels, and forward-mode dual numbers
+    (``torch.autograd.forward_ad``) get their tangents from the
+    tangent-linear kernels.
     """
     if differentiable:
         return _atm_srk3_differentiable(state, mesh, config, dt, strict)

@@ -153,3 +153,24 @@ def test_launch_shape_table():
     assert lib.mpas_hip_tuning_set(b"k_dyn_edges_a", 3) != 0  # not 1, 2 or 4
     assert lib.mpas_hip_tuning_set(b"k_dyn_edges_a", 0) == 0
     assert lib.mpas_hip_tuning_get(b"k_dyn_edges_a") == 4
+
+
+def test_run_simulation_jvp_validates_the_perturbation():
+    """run_simulation.jvp rejects perturbations as the reference does (atmosphere_model.py:2063-2087),
+    before anything runs."""
+    from mpas_dycore_pytorch_amd.atmosphere_model import run_simulation
+
+    state = {"u": torch.zeros(4, 3, dtype=torch.float64), "w": torch.zeros(2, 4, dtype=torch.float64),
+             "nCells": 1, "raw": torch.zeros(2, dtype=torch.float64)}
+    with pytest.raises(TypeError):
+        run_simulation.jvp(state, {}, {"u": np.zeros((4, 3))})
+    with pytest.raises(KeyError):
+        run_simulation.jvp(state, {}, {"theta_m": torch.zeros(2, 3, dtype=torch.float64)})
+    with pytest.raises(TypeError):
+        run_simulation.jvp(state, {}, {"nCells": torch.zeros(1, dtype=torch.float64)})
+    with pytest.raises(MpasConfigError):
+        run_simulation.jvp(state, {}, {"u": torch.zeros(4, 2, dtype=torch.float64)})
+    with pytest.raises(MpasConfigError):
+        run_simulation.jvp(state, {}, {"u": torch.zeros(4, 3, dtype=torch.float32)})
+    with pytest.raises(MpasUnsupportedCapabilityError):
+        run_simulation.jvp(state, {}, {"raw": torch.zeros(2, dtype=torch.float64)})

@@ -1,0 +1,306 @@
+"""GPU tangent-linear (JVP) tests.
+
+The reference obtains tangents with torch forward-mode AD
+(``run_simulation.jvp``, atmosphere_model.py:2041-2098).  Here every
+``mpas_hip_*_jvp`` kernel is checked to be the exact transpose of its
+``mpas_hip_*_vjp`` counterpart (whose gradients are pinned to the
+reference's autograd and to oracle finite differences) through the identity
+<ybar, J t> = <J^T ybar, t>, and the whole-step and multi-step tangents are
+checked against float64 central finite differences of the CPU oracle.
+Inputs carry noise so that no value sits on a switching surface.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import perturbed_state, srk3_case
+
+pytestmark = pytest.mark.gpu
+
+CASE = "l2_L20_tracer"
+
+
+def _noisy_state(seed: int):
+    st, _, meta = srk3_case(CASE)
+    return perturbed_state(st, seed), meta
+
+
+def _len_disp(st, meta):
+    return float(st["nominalMinDc"]) if "nominalMinDc" in st else float(meta["len_disp"])
+
+
+def _cfg(st, meta):
+    from mpas_dycore_pytorch_amd.time_integration import resolve_config
+
+    c = {"config_time_integration_order": 3, "config_len_disp": _len_disp(st, meta)}
+    return resolve_config(c, st, st, meta["dt"], int(np.asarray(st["scalars"]).shape[2]))
+
+
+def _rand_like(gen: torch.Generator, x: torch.Tensor, pad_rows: set[int]) -> torch.Tensor:
+    """Random tensor of x's shape scaled by x's spread, padding row zeroed."""
+    s = float(x.double().std()) if x.numel() > 1 else 1.0
+    r = torch.randn(x.shape, generator=gen, dtype=torch.float64).to(x.device) * (s or 1.0)
+    if r.ndim >= 1 and r.shape[0] in pad_rows:
+        r[-1] = 0.0
+    return r
+
+
+def _step_inputs(st, dm):
+    from mpas_dycore_pytorch_amd.adjoint import STEP_INPUTS
+
+    nc1, L = int(st["nCells"]) + 1, int(st["nVertLevels"])
+    out = {}
+    for k in STEP_INPUTS:
+        v = st.get(k)
+        if v is None and k == "rho_edge":
+            v = st.get("h_edge")
+        if v is None:
+            v = np.zeros((nc1, L))
+        out[k] = torch.as_tensor(np.asarray(v, dtype=np.float64), device=dm.device).contiguous()
+    return out
+
+
+@pytest.fixture(scope="module")
+def taped():
+    from mpas_dycore_pytorch_amd.adjoint import Tape, taped_step, vjp_scratch
+    from mpas_dycore_pytorch_amd.device import DeviceMesh
+
+    st, meta = _noisy_state(11)
+    dm = DeviceMesh(st, torch.device("cuda"))
+    state = _step_inputs(st, dm)
+    tape = Tape(dm)
+    taped_step(dm, state, _cfg(st, meta), meta["dt"], tape)
+    torch.cuda.synchronize()
+    scratch = vjp_scratch(dm, int(state["scalars"].shape[2]))
+    return dm, tape, scratch
+
+
+def _variants(tape):
+    """First call of every kernel kind per distinguishing argument (RK stage, substep)."""
+    seen, ops = set(), []
+    for op in tape.ops:
+        if op.kind in ("add", "scale", "copy", "mask_pad"):
+            continue
+        a = op.args[0] if op.args else None
+        key = (op.kind, getattr(a, "rk_step", None), getattr(a, "small_step", None),
+               op.args[0] if op.kind == "recover_large_step" else None,
+               op.args[2] if op.kind == "solve_diagnostics" else None)
+        if key not in seen:
+            seen.add(key)
+            ops.append(op)
+    return ops
+
+
+def test_every_tangent_kernel_is_the_transpose_of_its_adjoint(taped):
+    from mpas_dycore_pytorch_amd.adjoint import CONSTANT_IN, SLOTS, Tape
+
+    dm, tape, scratch = taped
+    pad = {dm.nc + 1, dm.ne + 1, dm.nv + 1}
+    gen = torch.Generator().manual_seed(123)
+    ops = _variants(tape)
+    kinds = {op.kind for op in ops}
+    assert kinds == set(SLOTS), sorted(set(SLOTS) - kinds)
+    worst = 0.0
+    for op in ops:
+        si, so = SLOTS[op.kind]
+        const = CONSTANT_IN.get(op.kind, ())
+        t = {}
+        for name, x in zip(si, op.ins):
+            if name not in const and id(x) not in t:
+                t[id(x)] = _rand_like(gen, x, pad)
+        yb = {id(o): _rand_like(gen, o, pad) / (float(o.double().std()) or 1.0) ** 2 for o in op.outs}
+        one = Tape(dm)
+        one.ops = [op]
+        dots = one.forward_tangents(t, scratch, keep={id(o) for o in op.outs})
+        bars = one.backward(yb, scratch)
+        torch.cuda.synchronize()
+        jv = sum(float(torch.sum(yb[id(o)] * dots[id(o)])) for o in op.outs if id(o) in dots)
+        vj = sum(float(torch.sum(bars[i] * t[i])) for i in t if i in bars)
+        rel = abs(jv - vj) / max(abs(jv), abs(vj), 1e-300)
+        tag = f"{op.kind} {getattr(op.args[0], 'rk_step', '') if op.args else ''}"
+        print(f"[tangent] {tag:28s} <yb,Jt>={jv: .15e} <J'yb,t>={vj: .15e} rel={rel:.1e}", flush=True)
+        worst = max(worst, rel)
+    assert worst <= 1e-11, worst
+
+
+def test_tangent_kernels_are_deterministic(taped):
+    """Gathers only: two tangent sweeps of the whole tape agree bit for bit."""
+    from mpas_dycore_pytorch_amd.adjoint import STEP_INPUTS  # noqa: F401
+
+    dm, tape, scratch = taped
+    pad = {dm.nc + 1, dm.ne + 1, dm.nv + 1}
+    gen = torch.Generator().manual_seed(5)
+    first = tape.ops[0]
+    seeds = {id(x): _rand_like(gen, x, pad) for x in first.ins}
+    outs = tape.ops[-1].outs
+    keep = {id(o) for o in outs}
+    a = tape.forward_tangents(seeds, scratch, keep)
+    b = tape.forward_tangents(seeds, scratch, keep)
+    torch.cuda.synchronize()
+    for o in outs:
+        if id(o) in a:
+            assert torch.equal(a[id(o)], b[id(o)])
+
+
+def test_step_jvp_is_transpose_of_step_vjp():
+    from mpas_dycore_pytorch_amd.adjoint import step_jvp, step_vjp
+    from mpas_dycore_pytorch_amd.device import DeviceMesh
+
+    st, meta = _noisy_state(12)
+    dm = DeviceMesh(st, torch.device("cuda"))
+    cfg = _cfg(st, meta)
+    state = _step_inputs(st, dm)
+    pad = {dm.nc + 1, dm.ne + 1, dm.nv + 1}
+    gen = torch.Generator().manual_seed(9)
+    wrt = ("u", "w", "theta_m", "rho_zz", "scalars", "exner", "rtheta_p", "rho_p", "ru", "rw", "pressure_p")
+    outk = ("u", "w", "theta_m", "rho_zz", "scalars", "exner", "pressure_p", "rtheta_p", "ru", "rw")
+    t = {k: _rand_like(gen, state[k], pad) for k in wrt}
+    for k in ("w", "rw"):
+        t[k][:, 0] = 0.0
+        t[k][:, -1] = 0.0
+    out, jt = step_jvp(dm, state, cfg, meta["dt"], t)
+    yb = {k: _rand_like(gen, out[k], pad) / (float(out[k].std()) or 1.0) ** 2 for k in outk}
+    _, g = step_vjp(dm, state, cfg, meta["dt"], yb)
+    torch.cuda.synchronize()
+    jv = sum(float(torch.sum(yb[k] * jt[k])) for k in outk)
+    vj = sum(float(torch.sum(g[k] * t[k])) for k in wrt)
+    rel = abs(jv - vj) / max(abs(jv), abs(vj))
+    print(f"[tangent] whole step <yb,Jt>={jv:.15e} <J'yb,t>={vj:.15e} rel={rel:.1e}", flush=True)
+    assert rel <= 1e-11
+
+
+def test_step_jvp_against_oracle_finite_differences():
+    """Whole large step: J t from the tangent kernels vs oracle central differences, per output field."""
+    from oracle import oracle as O, srk3 as OS
+    from mpas_dycore_pytorch_amd.adjoint import step_jvp
+    from mpas_dycore_pytorch_amd.device import DeviceMesh
+
+    st, meta = _noisy_state(13)
+    dm = DeviceMesh(st, torch.device("cuda"))
+    cfg = _cfg(st, meta)
+    om = O.Mesh(st)
+    ld = _len_disp(st, meta)
+    state = _step_inputs(st, dm)
+    pad = {dm.nc + 1, dm.ne + 1, dm.nv + 1}
+    gen = torch.Generator().manual_seed(21)
+    wrt = ("u", "w", "theta_m", "rho_zz", "scalars")
+    t = {k: _rand_like(gen, state[k], pad) for k in wrt}
+    t["w"][:, 0] = 0.0
+    t["w"][:, -1] = 0.0
+    _, jt = step_jvp(dm, state, cfg, meta["dt"], t)
+    torch.cuda.synchronize()
+    X = {k: np.array(st[k], dtype=np.float64) for k in wrt}
+    tn = {k: t[k].cpu().numpy() for k in wrt}
+
+    def run(sign, h):
+        s2 = dict(st)
+        s2.update({k: X[k] + sign * h * tn[k] for k in wrt})
+        return OS.srk3(s2, om, meta["dt"], len_disp=ld)
+
+    fd = {}
+    for h in (1e-7, 1e-8):
+        yp, ym = run(1, h), run(-1, h)
+        fd[h] = {k: (np.asarray(yp[k]) - np.asarray(ym[k])) / (2 * h) for k in ("u", "w", "theta_m", "rho_zz",
+                                                                              "scalars", "exner", "pressure_p")}
+    for k in fd[1e-7]:
+        got = jt[k].cpu().numpy()
+        errs = []
+        for h in fd:
+            ref = fd[h][k]
+            scale = float(np.max(np.abs(ref))) or 1.0
+            errs.append(float(np.linalg.norm((got - ref).ravel())) / (float(np.linalg.norm(ref.ravel())) or 1.0))
+        print(f"[tangent] whole step J t {k}: rel l2 err vs fd {errs}", flush=True)
+        assert min(errs) < 1e-5, k
+
+
+def test_run_simulation_jvp_matches_vjp_and_finite_differences():
+    """run_simulation.jvp over two steps: <seed, J d> equals <J' seed, d> from run_simulation.vjp
+    (the reference's own JVP/VJP consistency check, examples/validate_model.py:320-345) and the
+    oracle's central differences."""
+    from oracle import oracle as O, srk3 as OS
+    from mpas_dycore_pytorch_amd.atmosphere_model import STEP_BOUNDARY_KEY, run_simulation
+
+    st, meta = _noisy_state(14)
+    om = O.Mesh(st)
+    ld = _len_disp(st, meta)
+    config = {"config_time_integration_order": 3, "config_len_disp": ld}
+    dev = torch.device("cuda")
+    state = {k: (torch.as_tensor(np.asarray(v, dtype=np.float64), device=dev)
+                 if isinstance(v, np.ndarray) and v.dtype.kind == "f" else v) for k, v in st.items()}
+    state[STEP_BOUNDARY_KEY] = True
+    keys = ("u", "w", "theta_m")
+    gen = torch.Generator().manual_seed(20261001)
+    pad = {om.nc + 1, om.ne + 1, om.nv + 1}
+    direction = {k: _rand_like(gen, state[k], pad) for k in keys}
+    direction["w"][:, 0] = 0.0
+    direction["w"][:, -1] = 0.0
+    seed = {k: _rand_like(gen, state[k], pad) / float(state[k].std()) ** 2 for k in keys}
+    kw = {"config": config, "dt": meta["dt"], "nsteps": 2, "device": dev}
+    tangent = run_simulation.jvp(dict(state), st, direction, **kw)
+
+    def objective(out):
+        return sum((out[k] * seed[k]).sum() for k in keys)
+
+    grads = run_simulation.vjp(dict(state), st, objective, track_keys=keys, **kw)
+    torch.cuda.synchronize()
+    for k in keys:
+        assert tangent[k].shape == state[k].shape and tangent[k].dtype == torch.float64
+        assert bool(torch.isfinite(tangent[k]).all())
+    jv = sum(float(torch.sum(seed[k] * tangent[k])) for k in keys)
+    vj = sum(float(torch.sum(grads[k] * direction[k])) for k in keys)
+    rel = abs(jv - vj) / max(abs(jv), abs(vj))
+    X = {k: np.array(st[k], dtype=np.float64) for k in keys}
+    dn = {k: direction[k].cpu().numpy() for k in keys}
+    sn = {k: seed[k].cpu().numpy() for k in keys}
+
+    def loss(sign, h):
+        s2 = dict(st)
+        s2.update({k: X[k] + sign * h * dn[k] for k in keys})
+        s2 = OS.srk3(s2, om, meta["dt"], len_disp=ld)
+        s2 = OS.srk3(s2, om, meta["dt"], len_disp=ld)
+        return sum(float(np.sum(sn[k] * np.asarray(s2[k]))) for k in keys)
+
+    fds = [(loss(1, h) - loss(-1, h)) / (2 * h) for h in (1e-7, 1e-8)]
+    rf = [abs(fd - jv) / max(abs(fd), abs(jv)) for fd in fds]
+    print(f"[tangent] run_simulation.jvp 2 steps: jvp={jv:.15e} vjp={vj:.15e} rel={rel:.1e} fd={fds} rel={rf}",
+          flush=True)
+    assert rel <= 1e-11
+    assert min(rf) < 1e-5
+
+
+def test_forward_ad_dual_numbers_through_atm_srk3():
+    """atm_srk3(differentiable=True) carries torch forward-mode duals: the tangent equals step_jvp's."""
+    from torch.autograd import forward_ad
+
+    from mpas_dycore_pytorch_amd.adjoint import step_jvp
+    from mpas_dycore_pytorch_amd.device import DeviceMesh
+    from mpas_dycore_pytorch_amd.time_integration import Srk3Stepper, atm_srk3
+
+    st, meta = _noisy_state(15)
+    ld = _len_disp(st, meta)
+    config = {"config_time_integration_order": 3, "config_len_disp": ld}
+    dev = torch.device("cuda")
+    state = {k: (torch.as_tensor(np.asarray(v, dtype=np.float64), device=dev)
+                 if isinstance(v, np.ndarray) and v.dtype.kind == "f" else v) for k, v in st.items()}
+    gen = torch.Generator().manual_seed(2)
+    tu = _rand_like(gen, state["u"], {int(st["nEdges"]) + 1})
+    with forward_ad.dual_level():
+        s2 = dict(state)
+        s2["u"] = forward_ad.make_dual(state["u"], tu)
+        out = atm_srk3(s2, st, config, meta["dt"], differentiable=True)
+        _, tth = forward_ad.unpack_dual(out["theta_m"])
+    stepper = Srk3Stepper(state, st, config, meta["dt"])
+    from mpas_dycore_pytorch_amd.adjoint import STEP_INPUTS
+
+    ckpt = {k: stepper.t[k].clone() for k in STEP_INPUTS}
+    dm = stepper.mesh
+    _, jt = step_jvp(dm, ckpt, _cfg(st, meta), meta["dt"], {"u": tu})
+    torch.cuda.synchronize()
+    assert tth is not None
+    ref = jt["theta_m"].to(tth.device)
+    err = float((tth - ref).abs().max()) / max(float(ref.abs().max()), 1e-300)
+    print(f"[tangent] forward-mode dual through atm_srk3: theta_m rel err {err:.1e}", flush=True)
+    assert err <= 1e-12
