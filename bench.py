@@ -148,6 +148,45 @@ def vjp_leg(m, stepper, config, dt: float, steps: int, fwd_ms_per_step: float, d
 ENSEMBLE_DT = {4: 1800.0, 5: 900.0, 6: 450.0, 7: 300.0, 8: 150.0}
 
 
+def mixed_leg(st, config, dt: float, steps: int, err_steps: int, fp64_ms_per_step: float, reorder: bool,
+              device) -> dict:
+    """The mixed-precision step (edge/vertex fields in float32) on the same workload: time per step,
+    speed-up over the FP64 step, and per-field relative L2/Linf error against the FP64 GPU step after
+    ``err_steps`` large steps from the same state."""
+    import torch
+
+    from mpas_dycore_pytorch_amd.time_integration import Srk3Stepper
+
+    mix = Srk3Stepper(st, st, config, dt, device=device, reorder=reorder, precision="mixed")
+    ref = Srk3Stepper(st, st, config, dt, device=device, reorder=reorder)
+    mix.step(err_steps)
+    ref.step(err_steps)
+    torch.cuda.synchronize()
+    a, b = ref.state_dict(), mix.state_dict()
+    errors = {}
+    for k in ("u", "w", "theta_m", "rho_zz", "exner", "pressure_p", "rtheta_p", "pv_edge", "ke", "scalars"):
+        x, y = a[k][:-1], b[k][:-1]
+        if x.numel() == 0:
+            continue
+        d = x - y
+        errors[k] = {"rel_l2": float(torch.linalg.vector_norm(d) / max(float(torch.linalg.vector_norm(x)), 1e-300)),
+                     "rel_linf": float(d.abs().max() / max(float(x.abs().max()), 1e-300))}
+    del ref, a, b
+    stream = torch.cuda.current_stream(device)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    mix.step(steps)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / steps
+    units = mix.mesh.nc * mix.mesh.L
+    return {"ms_per_step": ms, "value": units / (ms / 1e3), "speedup_vs_fp64": fp64_ms_per_step / ms,
+            "error_steps": err_steps, "errors_vs_fp64_gpu": errors,
+            "storage": "edge/vertex fields float32; cell/interface fields, column solve, exner recovery and "
+                       "base state float64; float64 arithmetic"}
+
+
 def ensemble_bench(args, device) -> int:
     """Uneven ensemble: members on several mesh levels, each with its own seeded
     theta perturbation and the level's dt.  Batched (one grid per kernel per
@@ -235,6 +274,9 @@ def main() -> int:
                     help="measure an uneven ensemble instead, e.g. '5x32,6x16,7x4' (level x members): the batched "
                          "form (one grid per kernel per mesh, HIP graphs, groups on their own streams) against the "
                          "members stepped one by one")
+    ap.add_argument("--mixed-steps", type=int, default=6,
+                    help="large steps of the mixed-precision leg's error check against the FP64 step (its "
+                         "timing uses --steps); 0 skips the leg")
     ap.add_argument("--vjp-steps", type=int, default=2,
                     help="large steps of the reverse-mode leg (forward + VJP of an area-weighted squared "
                          "theta perturbation, one checkpoint per large step); 0 skips it")
@@ -324,6 +366,12 @@ def main() -> int:
     step_roof = {"algorithmic_bytes": sb, "achieved": round(step_gbs, 1), "unit": "GB/s",
                  "frac": round(step_gbs / HBM_PEAK_GBS, 4)}
 
+    mixed = None
+    if args.mixed_steps > 0:
+        mixed = mixed_leg(st, cfg, args.dt, args.steps, args.mixed_steps, ms_per_step, bool(args.reorder), device)
+        log(f"mixed precision: {mixed['ms_per_step']:.3f} ms/step, x{mixed['speedup_vs_fp64']:.2f}; errors "
+            + ", ".join(f"{k} {v['rel_l2']:.1e}" for k, v in mixed["errors_vs_fp64_gpu"].items()))
+
     vjp = None
     if args.vjp_steps > 0:
         vjp = vjp_leg(st, stepper, cfg, args.dt, args.vjp_steps, ms_per_step, device)
@@ -349,7 +397,7 @@ def main() -> int:
                    "mesh_order": "hilbert" if args.reorder else "generator",
                    "launch": "hip graph" if args.graph else "stream",
                    "waves_per_block": ({k: v["waves"] for k, v in tuning.items()} if tuning else "default 4")},
-        "roofline": roof, "step_roofline": step_roof, "vjp_fwd_ratio": (vjp or {}).get("ratio"), "vjp": vjp, "cpu_baseline": cpu,
+        "roofline": roof, "step_roofline": step_roof, "vjp_fwd_ratio": (vjp or {}).get("ratio"), "vjp": vjp, "mixed_precision": mixed, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -159,7 +159,51 @@ def load() -> C.CDLL:
     return lib
 
 
-def check(rc: int, what: str) -> None:
+def check(rc: int, what: str, lib: C.CDLL | None = None) -> None:
     if rc != 0:
-        msg = load().mpas_hip_last_error()
+        msg = (lib or load()).mpas_hip_last_error()
         raise MpasError(f"{what} failed: {msg.decode() if msg else 'unknown error'}")
+
+
+# ---------------------------------------------------------------------------
+# mixed-precision library (include/mpas_hip_mixed.h)
+# ---------------------------------------------------------------------------
+MIXED_LIB_PATH = os.path.join(HERE, "libmpas_dycore_hip_mixed.so")
+# state fields stored in binary32 by the mixed-precision step
+MIXED_F32_FIELDS = ("u", "ru", "v", "vorticity", "pv_edge", "rho_edge", "ruAvg")
+MixedStateStruct = StateStruct  # same field order; every field is a device address
+
+MIXED_SIGNATURES = {
+    "mpas_hip_mixed_state_size": (_I32, []),
+    "mpas_hip_mixed_workspace_doubles": (_I64, [_I32, _I32, _I32, _I32, _I32]),
+    "mpas_hip_mixed_srk3_step": (C.c_int, [_PM, C.POINTER(StateStruct), C.POINTER(WorkspaceStruct),
+                                           C.POINTER(ConfigStruct), _vp]),
+    "mpas_hip_tuning_set": (C.c_int, [C.c_char_p, _I32]),
+    "mpas_hip_tuning_get": (_I32, [C.c_char_p]),
+    "mpas_hip_last_error": (C.c_char_p, []),
+    "mpas_hip_timer_arm": (C.c_int, [C.c_char_p, _I32]),
+    "mpas_hip_timer_read": (C.c_int, [C.POINTER(C.c_double), C.POINTER(_I32)]),
+}
+
+_mixed = None
+
+
+def load_mixed() -> C.CDLL:
+    """Load the mixed-precision step library once (fails loudly when missing)."""
+    global _mixed
+    if _mixed is not None:
+        return _mixed
+    if not os.path.exists(MIXED_LIB_PATH):
+        raise MpasDeviceError(
+            f"HIP library not built: {MIXED_LIB_PATH} is missing "
+            "(run `python -m mpas_dycore_pytorch_amd.build`)"
+        )
+    lib = C.CDLL(MIXED_LIB_PATH)
+    for name, (res, args) in MIXED_SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.mpas_hip_mixed_state_size() != C.sizeof(MixedStateStruct):
+        raise MpasError("ABI struct size mismatch for mpas_hip_mixed_state")
+    _mixed = lib
+    return lib

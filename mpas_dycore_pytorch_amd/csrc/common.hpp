@@ -17,6 +17,21 @@
 
 namespace mpas {
 
+// Storage type of the edge- and vertex-located fields (u, ru, v, pv_edge,
+// rho_edge, tendencies of u, the acoustic edge fluxes, vorticity, the
+// transport fluxes, and the edge/vertex scratch).  binary64 in the plain
+// build; binary32 in the mixed-precision build (-DMPAS_MIXED=1), which halves
+// the bytes of the arrays that dominate the step's traffic.  Cell and
+// interface fields (the vertical-implicit coefficients and column solve, rho,
+// theta, exner, pressure, w) stay binary64 in both, and all arithmetic is
+// binary64 in registers: values are widened on load and rounded once on
+// store.
+#if defined(MPAS_MIXED) && MPAS_MIXED
+using hreal = float;
+#else
+using hreal = double;
+#endif
+
 // Host-evaluated kernel scalars (layouts fixed by the C ABI header).
 using DynParams = mpas_hip_dyn_params;
 using AcousticParams = mpas_hip_acoustic_params;
@@ -95,6 +110,25 @@ __device__ __forceinline__ int wave_lane() { return (int)(threadIdx.x & 63); }
 // Zero-fill by kernel rather than hipMemsetAsync so launch sequences stay
 // safe to capture in a HIP graph.
 void fill_zero(double *p, size_t n, hipStream_t s);
+#if defined(MPAS_MIXED) && MPAS_MIXED
+void fill_zero(float *p, size_t n, hipStream_t s);
+#endif
+
+// Bump allocator over a caller-owned scratch block: slices of any element
+// type, each starting on a 256-byte boundary (size with slice_doubles).
+struct Carver {
+    char *p;
+    template <typename T>
+    T *take(size_t n) {
+        T *r = reinterpret_cast<T *>(p);
+        p += (n * sizeof(T) + 255) & ~(size_t)255;
+        return r;
+    }
+};
+template <typename T>
+inline size_t slice_doubles(size_t n) {
+    return ((n * sizeof(T) + 255) & ~(size_t)255) / sizeof(double);
+}
 
 // Error plumbing: every launcher records the first HIP error.
 void set_error(const std::string &msg);
@@ -119,40 +153,56 @@ void timer_after(hipStream_t s);
 // Internal launchers (one per kernel family).
 namespace mpas {
 void launch_moist(const mpas_hip_mesh &m, const double *scalars, int S, int ms, int me, double *cqw,
-                  double *cqu, double *qtot, hipStream_t s);
+                  hreal *cqu, double *qtot, hipStream_t s);
 void launch_vimp(const mpas_hip_mesh &m, double dts, double epssm, const double *cqw, const double *p,
                  const double *t, const double *rt, const double *qtot, double *cofrz, double *cofwr,
                  double *cofwz, double *coftz, double *cofwt, double *a_tri, double *alpha_tri,
                  double *gamma_tri, hipStream_t s);
-void launch_diagnostics(const mpas_hip_mesh &m, double dt, double apvm, int rk_step, const double *u,
-                        const double *h, const double *v_in, double *h_edge, double *v, double *div,
-                        double *vort, double *ke, double *pv_edge, double *scratch, hipStream_t s);
+void launch_diagnostics(const mpas_hip_mesh &m, double dt, double apvm, int rk_step, const hreal *u,
+                        const double *h, const hreal *v_in, hreal *h_edge, hreal *v, double *div,
+                        hreal *vort, double *ke, hreal *pv_edge, double *scratch, hipStream_t s);
 size_t diagnostics_scratch_doubles(const mpas_hip_mesh &m);
 
 struct DynInputs {
-    const double *u, *v, *w, *ru, *rw, *ru_save, *rw_save, *theta_m, *theta_m_save, *rho_zz;
-    const double *rho_edge, *divergence, *vorticity, *ke, *pv_edge, *pp, *rr_save, *cqw, *cqu;
+    const hreal *u, *v;
+    const double *w;
+    const hreal *ru;
+    const double *rw;
+    const hreal *ru_save;
+    const double *rw_save, *theta_m, *theta_m_save, *rho_zz;
+    const hreal *rho_edge;
+    const double *divergence;
+    const hreal *vorticity;
+    const double *ke;
+    const hreal *pv_edge;
+    const double *pp, *rr_save, *cqw;
+    const hreal *cqu;
     const double *rt_diabatic_tend, *qtot;
 };
 struct DynCarried {
-    double *tend_rho, *tend_u_euler, *tend_w_euler, *tend_theta_euler, *kdiff;
+    double *tend_rho;
+    hreal *tend_u_euler;
+    double *tend_w_euler, *tend_theta_euler, *kdiff;
 };
 struct DynOutputs {
-    double *tend_u, *tend_w, *tend_theta, *h_divergence, *rthdynten;
+    hreal *tend_u;
+    double *tend_w, *tend_theta, *h_divergence, *rthdynten;
 };
 void launch_dyn_tend(const mpas_hip_mesh &m, const DynParams &P, const DynInputs &in,
                      const DynCarried &io, const DynOutputs &out, double *scratch, hipStream_t s);
 size_t dyn_tend_scratch_doubles(const mpas_hip_mesh &m);
 
-void launch_smlstep(const mpas_hip_mesh &m, const double *u_tend, double *w_tend, hipStream_t s);
+void launch_smlstep(const mpas_hip_mesh &m, const hreal *u_tend, double *w_tend, hipStream_t s);
 
 struct AcousticInputs {
     const double *cofrz, *cofwr, *cofwz, *coftz, *cofwt, *a_tri, *alpha_tri, *gamma_tri;
-    const double *rho_zz, *theta_m, *exner, *cqu, *tend_ru, *tend_rho, *tend_rt, *tend_rw, *w, *rw,
-        *rw_save;
+    const double *rho_zz, *theta_m, *exner;
+    const hreal *cqu, *tend_ru;
+    const double *tend_rho, *tend_rt, *tend_rw, *w, *rw, *rw_save;
 };
 struct AcousticState {
-    double *ru_p, *ruAvg, *rho_pp, *rtheta_pp, *rtheta_pp_old, *rw_p, *wwAvg;
+    hreal *ru_p, *ruAvg;
+    double *rho_pp, *rtheta_pp, *rtheta_pp_old, *rw_p, *wwAvg;
 };
 // Divergence damping carried into a later edge pass (on = 0: none).
 struct DampFold {
@@ -164,38 +214,42 @@ void launch_acoustic(const mpas_hip_mesh &m, const AcousticParams &P, const Acou
                      const AcousticState &io, hipStream_t s, const DampFold &fold);
 void launch_divdamp(const mpas_hip_mesh &m, double dts, double smdiv, double len_disp,
                     const double *theta_m, const double *rtheta_pp, const double *rtheta_pp_old,
-                    double *ru_p, hipStream_t s);
+                    hreal *ru_p, hipStream_t s);
 
 struct RecoverInputs {
-    const double *rho_p_save, *rho_pp, *rtheta_p_save, *rtheta_pp, *rt_diabatic_tend, *rw_save, *rw_p,
-        *ru_save, *ru_p;
+    const double *rho_p_save, *rho_pp, *rtheta_p_save, *rtheta_pp, *rt_diabatic_tend, *rw_save, *rw_p;
+    const hreal *ru_save, *ru_p;
 };
 struct RecoverOutputs {
-    double *wwAvg, *ruAvg, *rho_p, *rho_zz, *rtheta_p, *theta_m, *exner, *pressure_p, *rw, *w, *ru, *u;
+    double *wwAvg;
+    hreal *ruAvg;
+    double *rho_p, *rho_zz, *rtheta_p, *theta_m, *exner, *pressure_p, *rw, *w;
+    hreal *ru, *u;
 };
 void launch_recover(const mpas_hip_mesh &m, int rk_step, double dt, int ns, const RecoverInputs &in,
                     const RecoverOutputs &out, hipStream_t s, const DampFold &fold);
 
 void launch_advance_scalars(const mpas_hip_mesh &m, int S, double dt, double coef3, double w_old,
                             double w_new, const double *scalar_old, const double *scalar_new_in,
-                            const double *rho_zz_old, const double *rho_zz_new, const double *uhAvg,
-                            const double *wwAvg, double *scalar_new_out, double *horiz_flux,
+                            const double *rho_zz_old, const double *rho_zz_new, const hreal *uhAvg,
+                            const double *wwAvg, double *scalar_new_out, hreal *horiz_flux,
                             hipStream_t s);
 void launch_advance_scalars_mono(const mpas_hip_mesh &m, int S, double dt, double coef3,
                                  const double *scalars_old, const double *scalars_new_in,
-                                 const double *rho_zz_old, const double *uhAvg, const double *wwAvg,
+                                 const double *rho_zz_old, const hreal *uhAvg, const double *wwAvg,
                                  double *scalars_new_out, double *rho_zz_int, double *scratch,
                                  hipStream_t s);
 size_t mono_scratch_doubles(const mpas_hip_mesh &m);
 
-void launch_reconstruct(const mpas_hip_mesh &m, const double *u, double *ux, double *uy, double *uz,
+void launch_reconstruct(const mpas_hip_mesh &m, const hreal *u, double *ux, double *uy, double *uz,
                         double *uzonal, double *umerid, hipStream_t s);
 
 struct InitCoupledOutputs {
-    double *theta_m, *rho_zz, *ru, *rw, *rho_p, *rtheta_base, *rtheta_p, *exner, *exner_base, *pressure_p,
-        *pressure_base;
+    double *theta_m, *rho_zz;
+    hreal *ru;
+    double *rw, *rho_p, *rtheta_base, *rtheta_p, *exner, *exner_base, *pressure_p, *pressure_base;
 };
 void launch_init_coupled(const mpas_hip_mesh &m, const double *theta, const double *rho, const double *qv,
-                         const double *u, const double *w, const double *theta_base, const InitCoupledOutputs &o,
+                         const hreal *u, const double *w, const double *theta_base, const InitCoupledOutputs &o,
                          hipStream_t s);
 }  // namespace mpas

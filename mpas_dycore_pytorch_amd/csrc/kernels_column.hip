@@ -33,7 +33,7 @@ __global__ void k_moist_cqw(int nC, int L, const double *__restrict__ qtot, doub
 }
 
 __global__ void k_moist_cqu(int nE, int L, int S, int q0, int q1, const int *__restrict__ coe,
-                            const double *__restrict__ sc, double *__restrict__ cqu) {
+                            const double *__restrict__ sc, hreal *__restrict__ cqu) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (nE + 1) * L) return;
     int e = i / L, k = i - e * L;
@@ -52,7 +52,7 @@ __global__ void k_moist_cqu(int nE, int L, int S, int q0, int q1, const int *__r
 }
 
 void launch_moist(const mpas_hip_mesh &m, const double *scalars, int S, int ms, int me, double *cqw,
-                  double *cqu, double *qtot, hipStream_t s) {
+                  hreal *cqu, double *qtot, hipStream_t s) {
     const int nC = m.nCells, nE = m.nEdges, L = m.nLevels;
     fill_zero(qtot + (size_t)nC * L, L, s);
     MPAS_LAUNCH(k_moist_cells, dim3(blocks_for((long long)nC * L)), dim3(kBlock), 0, s, nC, L, S,
@@ -177,7 +177,7 @@ void launch_vimp(const mpas_hip_mesh &m, double dts, double epssm, const double 
 // ---------------------------------------------------------------------------
 // mpas_reconstruct_2d (vector_reconstruction.py:311-441), on a sphere
 // ---------------------------------------------------------------------------
-__global__ void k_reconstruct(mpas_hip_mesh m, const double *__restrict__ u, double *__restrict__ ux,
+__global__ void k_reconstruct(mpas_hip_mesh m, const hreal *__restrict__ u, double *__restrict__ ux,
                               double *__restrict__ uy, double *__restrict__ uz, double *__restrict__ uzonal,
                               double *__restrict__ umerid) {
     const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges;
@@ -208,7 +208,7 @@ __global__ void k_reconstruct(mpas_hip_mesh m, const double *__restrict__ u, dou
     umerid[i] = -(x * clon + y * slon) * slat + z * clat;
 }
 
-void launch_reconstruct(const mpas_hip_mesh &m, const double *u, double *ux, double *uy, double *uz,
+void launch_reconstruct(const mpas_hip_mesh &m, const hreal *u, double *ux, double *uy, double *uz,
                         double *uzonal, double *umerid, hipStream_t s) {
     MPAS_LAUNCH(k_reconstruct, dim3(blocks_for((long long)(m.nCells + 1) * m.nLevels)), dim3(kBlock), 0,
                        s, m, u, ux, uy, uz, uzonal, umerid);

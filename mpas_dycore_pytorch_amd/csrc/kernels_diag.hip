@@ -11,12 +11,12 @@ namespace mpas {
 
 size_t diagnostics_scratch_doubles(const mpas_hip_mesh &m) {
     const size_t L = m.nLevels;
-    return 2 * (size_t)m.nVertices * L + (size_t)m.nCells * L;
+    return 2 * slice_doubles<hreal>((size_t)m.nVertices * L) + slice_doubles<double>((size_t)m.nCells * L);
 }
 
-__global__ __launch_bounds__(256) void k_diag_vertices(long long nb, mpas_hip_mesh m, const double *__restrict__ u,
-                                double *__restrict__ vort,
-                                double *__restrict__ ke_vertex, double *__restrict__ pv_vertex) {
+__global__ __launch_bounds__(256) void k_diag_vertices(long long nb, mpas_hip_mesh m, const hreal *__restrict__ u,
+                                hreal *__restrict__ vort,
+                                hreal *__restrict__ ke_vertex, hreal *__restrict__ pv_vertex) {
     auto body_ = [&](const int vx) {
         const int L = m.nLevels, nV = m.nVertices;
         const int k = wave_lane();
@@ -47,8 +47,8 @@ __global__ __launch_bounds__(256) void k_diag_vertices(long long nb, mpas_hip_me
     for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
-__global__ __launch_bounds__(256) void k_diag_cells(long long nb, mpas_hip_mesh m, const double *__restrict__ u,
-                             const double *__restrict__ ke_vertex, const double *__restrict__ pv_vertex,
+__global__ __launch_bounds__(256) void k_diag_cells(long long nb, mpas_hip_mesh m, const hreal *__restrict__ u,
+                             const hreal *__restrict__ ke_vertex, const hreal *__restrict__ pv_vertex,
                              double *__restrict__ div, double *__restrict__ ke, double *__restrict__ pv_cell) {
     auto body_ = [&](const int c) {
         const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges;
@@ -101,10 +101,10 @@ __global__ __launch_bounds__(256) void k_diag_cells(long long nb, mpas_hip_mesh 
 }
 
 __global__ __launch_bounds__(256) void k_diag_edges2(long long nb, mpas_hip_mesh m, double apvm_dt, int apvm_on, int reconstruct_v,
-                              const double *__restrict__ u, const double *__restrict__ v_in,
-                              const double *__restrict__ pv_vertex, const double *__restrict__ pv_cell,
-                              const double *__restrict__ h, double *__restrict__ h_edge,
-                              double *__restrict__ v, double *__restrict__ pv_edge) {
+                              const hreal *__restrict__ u, const hreal *__restrict__ v_in,
+                              const hreal *__restrict__ pv_vertex, const double *__restrict__ pv_cell,
+                              const double *__restrict__ h, hreal *__restrict__ h_edge,
+                              hreal *__restrict__ v, hreal *__restrict__ pv_edge) {
     auto body_ = [&](const int e) {
         const int L = m.nLevels, nE = m.nEdges, ME2 = m.maxEdges2;
         const int lane = wave_lane();
@@ -165,13 +165,14 @@ __global__ __launch_bounds__(256) void k_diag_edges2(long long nb, mpas_hip_mesh
     for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
-void launch_diagnostics(const mpas_hip_mesh &m, double dt, double apvm, int rk_step, const double *u,
-                        const double *h, const double *v_in, double *h_edge, double *v, double *div,
-                        double *vort, double *ke, double *pv_edge, double *scratch, hipStream_t s) {
+void launch_diagnostics(const mpas_hip_mesh &m, double dt, double apvm, int rk_step, const hreal *u,
+                        const double *h, const hreal *v_in, hreal *h_edge, hreal *v, double *div,
+                        hreal *vort, double *ke, hreal *pv_edge, double *scratch, hipStream_t s) {
     const int L = m.nLevels, nC = m.nCells, nE = m.nEdges, nV = m.nVertices;
-    double *ke_vertex = scratch;
-    double *pv_vertex = ke_vertex + (size_t)nV * L;
-    double *pv_cell = pv_vertex + (size_t)nV * L;
+    Carver k{reinterpret_cast<char *>(scratch)};
+    hreal *ke_vertex = k.take<hreal>((size_t)nV * L);
+    hreal *pv_vertex = k.take<hreal>((size_t)nV * L);
+    double *pv_cell = k.take<double>((size_t)nC * L);
     // ke_vertex / pv_vertex / pv_cell have no padding row: the padding-row
     // wavefronts of the pass that fills them only zero that pass's outputs.
     MPAS_WAVE_LAUNCH(k_diag_vertices, nV + 1, s, m, u, vort, ke_vertex, pv_vertex);

@@ -19,32 +19,39 @@ namespace mpas {
 size_t dyn_tend_scratch_doubles(const mpas_hip_mesh &m) {
     const size_t L = m.nLevels, C = m.nCells + 1, E = m.nEdges + 1, V = m.nVertices + 1;
     // dpdz, delsq_u, delsq_div, delsq_vort, delsq_w, delsq_theta, edge fluxes (w, theta, theta pert.)
-    return C * L + E * L + C * L + V * L + C * (L + 1) + C * L + 3 * E * L;
+    return slice_doubles<double>(C * L) + slice_doubles<hreal>(E * L) + slice_doubles<double>(C * L) +
+           slice_doubles<hreal>(V * L) + slice_doubles<double>(C * (L + 1)) + slice_doubles<double>(C * L) +
+           3 * slice_doubles<hreal>(E * L);
 }
 
 struct DynScratch {
-    double *dpdz, *delsq_u, *delsq_div, *delsq_vort, *delsq_w, *delsq_theta;
-    double *wflux, *tflux, *tpert;
+    double *dpdz;
+    hreal *delsq_u;
+    double *delsq_div;
+    hreal *delsq_vort;
+    double *delsq_w, *delsq_theta;
+    hreal *wflux, *tflux, *tpert;
 };
 
 static DynScratch carve(const mpas_hip_mesh &m, double *base) {
     const size_t L = m.nLevels, C = m.nCells + 1, E = m.nEdges + 1, V = m.nVertices + 1;
+    Carver k{reinterpret_cast<char *>(base)};
     DynScratch s;
-    s.dpdz = base;
-    s.delsq_u = s.dpdz + C * L;
-    s.delsq_div = s.delsq_u + E * L;
-    s.delsq_vort = s.delsq_div + C * L;
-    s.delsq_w = s.delsq_vort + V * L;
-    s.delsq_theta = s.delsq_w + C * (L + 1);
-    s.wflux = s.delsq_theta + C * L;
-    s.tflux = s.wflux + E * L;
-    s.tpert = s.tflux + E * L;
+    s.dpdz = k.take<double>(C * L);
+    s.delsq_u = k.take<hreal>(E * L);
+    s.delsq_div = k.take<double>(C * L);
+    s.delsq_vort = k.take<hreal>(V * L);
+    s.delsq_w = k.take<double>(C * (L + 1));
+    s.delsq_theta = k.take<double>(C * L);
+    s.wflux = k.take<hreal>(E * L);
+    s.tflux = k.take<hreal>(E * L);
+    s.tpert = k.take<hreal>(E * L);
     return s;
 }
 
 // P1: Smagorinsky eddy viscosity (first RK stage only)
-__global__ __launch_bounds__(256) void k_dyn_kdiff(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ u,
-                            const double *__restrict__ v, double *__restrict__ kdiff) {
+__global__ __launch_bounds__(256) void k_dyn_kdiff(long long nb, mpas_hip_mesh m, DynParams P, const hreal *__restrict__ u,
+                            const hreal *__restrict__ v, double *__restrict__ kdiff) {
     auto body_ = [&](const int c) {
         const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges;
         const int k = wave_lane();
@@ -77,10 +84,10 @@ __global__ __launch_bounds__(256) void k_dyn_kdiff(long long nb, mpas_hip_mesh m
 
 // P2: mass-flux divergence, density tendency, buoyancy dpdz; first stage also
 // the del2 w / theta mixing sums and their del2 fields for the del4 pass.
-__global__ __launch_bounds__(256) void k_dyn_cells_a(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ ru,
+__global__ __launch_bounds__(256) void k_dyn_cells_a(long long nb, mpas_hip_mesh m, DynParams P, const hreal *__restrict__ ru,
                               const double *__restrict__ rw, const double *__restrict__ rr_save,
                               const double *__restrict__ qtot, const double *__restrict__ w,
-                              const double *__restrict__ theta_m, const double *__restrict__ rho_edge,
+                              const double *__restrict__ theta_m, const hreal *__restrict__ rho_edge,
                               const double *__restrict__ kdiff, double *__restrict__ h_div,
                               double *__restrict__ tend_rho, double *__restrict__ dpdz,
                               double *__restrict__ delsq_w, double *__restrict__ tend_w_euler,
@@ -167,14 +174,14 @@ __global__ __launch_bounds__(256) void k_dyn_cells_a(long long nb, mpas_hip_mesh
 
 // P4: normal-velocity tendency (vertical flux, PV flux, KE gradient,
 // divergence coupling); first stage also PGF and del2 mixing into tend_u_euler.
-__global__ __launch_bounds__(256) void k_dyn_edges_a(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ u,
-                              const double *__restrict__ rw, const double *__restrict__ pv_edge,
+__global__ __launch_bounds__(256) void k_dyn_edges_a(long long nb, mpas_hip_mesh m, DynParams P, const hreal *__restrict__ u,
+                              const double *__restrict__ rw, const hreal *__restrict__ pv_edge,
                               const double *__restrict__ ke, const double *__restrict__ h_div,
-                              const double *__restrict__ rho_edge, const double *__restrict__ pp,
-                              const double *__restrict__ dpdz, const double *__restrict__ cqu,
-                              const double *__restrict__ divergence, const double *__restrict__ vorticity,
-                              const double *__restrict__ kdiff, double *__restrict__ tend_u,
-                              double *__restrict__ tend_u_euler, double *__restrict__ delsq_u, int add_euler) {
+                              const hreal *__restrict__ rho_edge, const double *__restrict__ pp,
+                              const double *__restrict__ dpdz, const hreal *__restrict__ cqu,
+                              const double *__restrict__ divergence, const hreal *__restrict__ vorticity,
+                              const double *__restrict__ kdiff, hreal *__restrict__ tend_u,
+                              hreal *__restrict__ tend_u_euler, hreal *__restrict__ delsq_u, int add_euler) {
     auto body_ = [&](const int e) {
         const int L = m.nLevels, nE = m.nEdges, ME2 = m.maxEdges2, Lw = L + 1;
         const int lane = wave_lane();
@@ -191,7 +198,7 @@ __global__ __launch_bounds__(256) void k_dyn_edges_a(long long nb, mpas_hip_mesh
         const int eev = m.edgesOnEdge[e * ME2 + sl];
         const double wv = m.weightsOnEdge[e * ME2 + sl];
         const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-        const double *ue = u + e * L;
+        const hreal *ue = u + e * L;
         const double *fzm = m.fzm, *fzp = m.fzp;
         auto wduz = [&](int kk) -> double {
             double erw = 0.5 * (rw[c1 * Lw + kk] + rw[c2 * Lw + kk]);
@@ -257,8 +264,8 @@ __global__ __launch_bounds__(256) void k_dyn_edges_a(long long nb, mpas_hip_mesh
 }
 
 // P5: del2 of the u Laplacian at vertices and cells (del4 first half)
-__global__ void k_dyn_del4_vc(mpas_hip_mesh m, const double *__restrict__ delsq_u,
-                              double *__restrict__ delsq_vort, double *__restrict__ delsq_div) {
+__global__ void k_dyn_del4_vc(mpas_hip_mesh m, const hreal *__restrict__ delsq_u,
+                              hreal *__restrict__ delsq_vort, double *__restrict__ delsq_div) {
     const int L = m.nLevels, nC = m.nCells, nV = m.nVertices, ME = m.maxEdges;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nV * L) {
@@ -288,9 +295,9 @@ __global__ void k_dyn_del4_vc(mpas_hip_mesh m, const double *__restrict__ delsq_
 }
 
 // P6: del4 u mixing (first stage) and the final u tendency
-__global__ void k_dyn_edges_b(mpas_hip_mesh m, DynParams P, int do_del4, const double *__restrict__ rho_edge,
-                              const double *__restrict__ delsq_div, const double *__restrict__ delsq_vort,
-                              double *__restrict__ tend_u_euler, double *__restrict__ tend_u) {
+__global__ void k_dyn_edges_b(mpas_hip_mesh m, DynParams P, int do_del4, const hreal *__restrict__ rho_edge,
+                              const double *__restrict__ delsq_div, const hreal *__restrict__ delsq_vort,
+                              hreal *__restrict__ tend_u_euler, hreal *__restrict__ tend_u) {
     const int L = m.nLevels, nE = m.nEdges;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nE * L) return;
@@ -316,10 +323,10 @@ __global__ void k_dyn_edges_b(mpas_hip_mesh m, DynParams P, int do_del4, const d
 // for rk_step > 1 also the theta perturbation flux
 // dv (ru_save - ru) 0.5 (theta_m_save1 + theta_m_save2) (:5901-6014).  The
 // cell pass applies the edge sign, an exact sign flip of these products.
-__global__ __launch_bounds__(256) void k_dyn_edge_adv(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ ru,
-                               const double *__restrict__ ru_save, const double *__restrict__ w,
+__global__ __launch_bounds__(256) void k_dyn_edge_adv(long long nb, mpas_hip_mesh m, DynParams P, const hreal *__restrict__ ru,
+                               const hreal *__restrict__ ru_save, const double *__restrict__ w,
                                const double *__restrict__ theta_m, const double *__restrict__ theta_m_save,
-                               double *__restrict__ wflux, double *__restrict__ tflux, double *__restrict__ tpert) {
+                               hreal *__restrict__ wflux, hreal *__restrict__ tflux, hreal *__restrict__ tpert) {
     auto body_ = [&](const int e) {
         const int L = m.nLevels, nE = m.nEdges, MA = m.maxAdv, Lw = L + 1;
         const int lane = wave_lane();
@@ -330,7 +337,7 @@ __global__ __launch_bounds__(256) void k_dyn_edge_adv(long long nb, mpas_hip_mes
         const int sl = lane < na ? lane : na - 1;
         const int advv = m.advCellsForEdge[e * MA + sl];
         const double c0v = m.adv_coefs[e * MA + sl], c3v = m.adv_coefs_3rd[e * MA + sl];
-        const double *rue_col = ru + e * L;
+        const hreal *rue_col = ru + e * L;
         const double rut = rue_col[k];
         const double srt = sign_pm(rut);
         const bool wl = (k >= 1);
@@ -374,9 +381,9 @@ __global__ __launch_bounds__(256) void k_dyn_edge_adv(long long nb, mpas_hip_mes
 
 // P7: w and theta tendencies (advection, mixing finish, PGF/buoyancy,
 // vertical transport, diabatic forcing), one thread per (cell, w-level).
-__global__ __launch_bounds__(256) void k_dyn_cells_b(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ wflux,
-                              const double *__restrict__ rw, const double *__restrict__ tflux,
-                              const double *__restrict__ rw_save, const double *__restrict__ tpert,
+__global__ __launch_bounds__(256) void k_dyn_cells_b(long long nb, mpas_hip_mesh m, DynParams P, const hreal *__restrict__ wflux,
+                              const double *__restrict__ rw, const hreal *__restrict__ tflux,
+                              const double *__restrict__ rw_save, const hreal *__restrict__ tpert,
                               const double *__restrict__ w,
                               const double *__restrict__ theta_m, const double *__restrict__ theta_m_save,
                               const double *__restrict__ rho_zz, const double *__restrict__ pp,

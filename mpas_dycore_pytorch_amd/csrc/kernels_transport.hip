@@ -24,8 +24,8 @@ __device__ __forceinline__ double patternc_value(const mpas_hip_mesh &m, int e, 
 // ---------------------------------------------------------------------------
 // explicit stages
 // ---------------------------------------------------------------------------
-__global__ void k_scalar_hflux(mpas_hip_mesh m, int S, const double *__restrict__ uhAvg,
-                               const double *__restrict__ scalar_new, double *__restrict__ hflux) {
+__global__ void k_scalar_hflux(mpas_hip_mesh m, int S, const hreal *__restrict__ uhAvg,
+                               const double *__restrict__ scalar_new, hreal *__restrict__ hflux) {
     const int L = m.nLevels, nE = m.nEdges;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nE * L) return;
@@ -37,8 +37,8 @@ __global__ void k_scalar_hflux(mpas_hip_mesh m, int S, const double *__restrict_
 __global__ void k_scalar_update(mpas_hip_mesh m, int S, double dt, double coef3, double w_old, double w_new,
                                 const double *__restrict__ scalar_old, const double *__restrict__ sn,
                                 const double *__restrict__ rho_old, const double *__restrict__ rho_new,
-                                const double *__restrict__ uhAvg, const double *__restrict__ wwAvg,
-                                const double *__restrict__ hflux, double *__restrict__ out) {
+                                const hreal *__restrict__ uhAvg, const double *__restrict__ wwAvg,
+                                const hreal *__restrict__ hflux, double *__restrict__ out) {
     const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nC * L) return;
@@ -72,8 +72,8 @@ __global__ void k_scalar_update(mpas_hip_mesh m, int S, double dt, double coef3,
 
 void launch_advance_scalars(const mpas_hip_mesh &m, int S, double dt, double coef3, double w_old, double w_new,
                             const double *scalar_old, const double *scalar_new_in, const double *rho_zz_old,
-                            const double *rho_zz_new, const double *uhAvg, const double *wwAvg,
-                            double *scalar_new_out, double *horiz_flux, hipStream_t s) {
+                            const double *rho_zz_new, const hreal *uhAvg, const double *wwAvg,
+                            double *scalar_new_out, hreal *horiz_flux, hipStream_t s) {
     const int L = m.nLevels;
     MPAS_LAUNCH(k_scalar_hflux, dim3(blocks_for((long long)m.nEdges * L)), dim3(kBlock), 0, s, m, S, uhAvg,
                        scalar_new_in, horiz_flux);
@@ -88,32 +88,34 @@ void launch_advance_scalars(const mpas_hip_mesh &m, int S, double dt, double coe
 size_t mono_scratch_doubles(const mpas_hip_mesh &m) {
     const size_t L = m.nLevels, C = m.nCells + 1, E = m.nEdges + 1;
     // so, smax, smin, scell, sin, sout (C*L); wdtn, wmid (C*(L+1)); fup, ftmp, flim (E*L)
-    return 6 * C * L + 2 * C * (L + 1) + 3 * E * L;
+    return 6 * slice_doubles<double>(C * L) + 2 * slice_doubles<double>(C * (L + 1)) + 3 * slice_doubles<hreal>(E * L);
 }
 
 struct MonoScratch {
-    double *so, *smax, *smin, *scell, *sin, *sout, *wdtn, *wmid, *fup, *ftmp, *flim;
+    double *so, *smax, *smin, *scell, *sin, *sout, *wdtn, *wmid;
+    hreal *fup, *ftmp, *flim;
 };
 
 static MonoScratch carve_mono(const mpas_hip_mesh &m, double *base) {
     const size_t L = m.nLevels, C = m.nCells + 1, E = m.nEdges + 1;
+    Carver k{reinterpret_cast<char *>(base)};
     MonoScratch s;
-    s.so = base;
-    s.smax = s.so + C * L;
-    s.smin = s.smax + C * L;
-    s.scell = s.smin + C * L;
-    s.sin = s.scell + C * L;
-    s.sout = s.sin + C * L;
-    s.wdtn = s.sout + C * L;
-    s.wmid = s.wdtn + C * (L + 1);
-    s.fup = s.wmid + C * (L + 1);
-    s.ftmp = s.fup + E * L;
-    s.flim = s.ftmp + E * L;
+    s.so = k.take<double>(C * L);
+    s.smax = k.take<double>(C * L);
+    s.smin = k.take<double>(C * L);
+    s.scell = k.take<double>(C * L);
+    s.sin = k.take<double>(C * L);
+    s.sout = k.take<double>(C * L);
+    s.wdtn = k.take<double>(C * (L + 1));
+    s.wmid = k.take<double>(C * (L + 1));
+    s.fup = k.take<hreal>(E * L);
+    s.ftmp = k.take<hreal>(E * L);
+    s.flim = k.take<hreal>(E * L);
     return s;
 }
 
 // density with the averaged fluxes: rho_zz_int = rho_old + dt*(hdiv - vdiv)
-__global__ void k_mono_density(mpas_hip_mesh m, double dt, const double *__restrict__ uhAvg,
+__global__ void k_mono_density(mpas_hip_mesh m, double dt, const hreal *__restrict__ uhAvg,
                                const double *__restrict__ wwAvg, const double *__restrict__ rho_old,
                                double *__restrict__ rho_int) {
     const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
@@ -180,9 +182,9 @@ __global__ void k_mono_bounds(mpas_hip_mesh m, int S, int s, double coef3, const
 }
 
 // horizontal upwind and antidiffusive fluxes
-__global__ void k_mono_hflux(mpas_hip_mesh m, int S, int s, double dt, const double *__restrict__ uhAvg,
+__global__ void k_mono_hflux(mpas_hip_mesh m, int S, int s, double dt, const hreal *__restrict__ uhAvg,
                              const double *__restrict__ sn, const double *__restrict__ so,
-                             double *__restrict__ fup, double *__restrict__ ftmp) {
+                             hreal *__restrict__ fup, hreal *__restrict__ ftmp) {
     const int L = m.nLevels, nE = m.nEdges;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nE * L) return;
@@ -202,7 +204,7 @@ __global__ void k_mono_scale(mpas_hip_mesh m, double dt, const double *__restric
                              const double *__restrict__ rho_old, const double *__restrict__ rho_int,
                              const double *__restrict__ wwAvg, const double *__restrict__ wdtn,
                              const double *__restrict__ smax, const double *__restrict__ smin,
-                             const double *__restrict__ fup, const double *__restrict__ ftmp,
+                             const hreal *__restrict__ fup, const hreal *__restrict__ ftmp,
                              double *__restrict__ scell, double *__restrict__ wmid, double *__restrict__ sin_,
                              double *__restrict__ sout) {
     const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
@@ -243,8 +245,8 @@ __global__ void k_mono_scale(mpas_hip_mesh m, double dt, const double *__restric
 }
 
 // limited horizontal flux
-__global__ void k_mono_hlimit(mpas_hip_mesh m, const double *__restrict__ ftmp, const double *__restrict__ sin_,
-                              const double *__restrict__ sout, double *__restrict__ flim) {
+__global__ void k_mono_hlimit(mpas_hip_mesh m, const hreal *__restrict__ ftmp, const double *__restrict__ sin_,
+                              const double *__restrict__ sout, hreal *__restrict__ flim) {
     const int L = m.nLevels, nE = m.nEdges;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nE * L) return;
@@ -259,7 +261,7 @@ __global__ void k_mono_hlimit(mpas_hip_mesh m, const double *__restrict__ ftmp, 
 __global__ void k_mono_final(mpas_hip_mesh m, int S, int s, const double *__restrict__ scell,
                              const double *__restrict__ wdtn, const double *__restrict__ wmid,
                              const double *__restrict__ sin_, const double *__restrict__ sout,
-                             const double *__restrict__ flim, const double *__restrict__ rho_int,
+                             const hreal *__restrict__ flim, const double *__restrict__ rho_int,
                              double *__restrict__ out) {
     const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -286,7 +288,7 @@ __global__ void k_mono_final(mpas_hip_mesh m, int S, int s, const double *__rest
 }
 
 void launch_advance_scalars_mono(const mpas_hip_mesh &m, int S, double dt, double coef3, const double *scalars_old,
-                                 const double *scalars_new_in, const double *rho_zz_old, const double *uhAvg,
+                                 const double *scalars_new_in, const double *rho_zz_old, const hreal *uhAvg,
                                  const double *wwAvg, double *scalars_new_out, double *rho_zz_int, double *scratch,
                                  hipStream_t s) {
     const int L = m.nLevels, nC = m.nCells, nE = m.nEdges;

@@ -6,6 +6,9 @@
 #include <vector>
 
 #include "common.hpp"
+#if defined(MPAS_MIXED) && MPAS_MIXED
+#include "../../include/mpas_hip_mixed.h"
+#endif
 
 namespace mpas {
 
@@ -98,20 +101,70 @@ void fill_zero(double *p, size_t n, hipStream_t s) {
 static void copy_d(double *dst, const double *src, size_t n, hipStream_t s) {
     MPAS_LAUNCH(k_copy, dim3(blocks_for((long long)n)), dim3(kBlock), 0, s, dst, src, n);
 }
+static void add_d(double *dst, const double *src, size_t n, hipStream_t s) {
+    MPAS_LAUNCH(k_add, dim3(blocks_for((long long)n)), dim3(kBlock), 0, s, dst, src, n);
+}
+static void scale_d(double *dst, const double *src, double f, size_t n, hipStream_t s) {
+    MPAS_LAUNCH(k_scale, dim3(blocks_for((long long)n)), dim3(kBlock), 0, s, dst, src, f, n);
+}
+
+#if defined(MPAS_MIXED) && MPAS_MIXED
+// binary32 forms for the edge/vertex fields of the mixed-precision step
+__global__ void k_fill_zero32(float *p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+__global__ void k_copy32(float *__restrict__ dst, const float *__restrict__ src, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+__global__ void k_add32(float *__restrict__ dst, const float *__restrict__ src, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (float)((double)dst[i] + (double)src[i]);
+}
+__global__ void k_scale32(float *__restrict__ dst, const float *__restrict__ src, double f, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (float)((double)src[i] * f);
+}
+void fill_zero(float *p, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    MPAS_LAUNCH(k_fill_zero32, dim3(blocks_for((long long)n)), dim3(kBlock), 0, s, p, n);
+}
+static void copy_d(float *dst, const float *src, size_t n, hipStream_t s) {
+    MPAS_LAUNCH(k_copy32, dim3(blocks_for((long long)n)), dim3(kBlock), 0, s, dst, src, n);
+}
+static void add_d(float *dst, const float *src, size_t n, hipStream_t s) {
+    MPAS_LAUNCH(k_add32, dim3(blocks_for((long long)n)), dim3(kBlock), 0, s, dst, src, n);
+}
+static void scale_d(float *dst, const float *src, double f, size_t n, hipStream_t s) {
+    MPAS_LAUNCH(k_scale32, dim3(blocks_for((long long)n)), dim3(kBlock), 0, s, dst, src, f, n);
+}
+#endif
 
 // ---------------------------------------------------------------------------
 // workspace layout
 // ---------------------------------------------------------------------------
 struct Work {
-    double *ru_save, *rw_save, *rtheta_p_save, *rho_p_save, *theta_m_init, *rho_zz_old_split;
-    double *scalars_old, *scalars_tmp, *ruAvg_split, *wwAvg_split;
-    double *cqw, *cqu, *qtot;
+    hreal *ru_save;
+    double *rw_save, *rtheta_p_save, *rho_p_save, *theta_m_init, *rho_zz_old_split;
+    double *scalars_old, *scalars_tmp;
+    hreal *ruAvg_split;
+    double *wwAvg_split;
+    double *cqw;
+    hreal *cqu;
+    double *qtot;
     double *cofrz, *cofwr, *cofwz, *coftz, *cofwt, *a_tri, *alpha_tri, *gamma_tri;
-    double *tend_rho, *tend_u_euler, *tend_w_euler, *tend_theta_euler, *kdiff;
-    double *tend_u, *tend_w, *tend_theta, *h_divergence, *rthdynten;
-    double *ru_p, *ruAvg, *rho_pp, *rtheta_pp, *rtheta_pp_old, *rw_p, *wwAvg;
-    double *rho_zz_int, *horiz_flux, *scratch;
-    size_t total;
+    double *tend_rho;
+    hreal *tend_u_euler;
+    double *tend_w_euler, *tend_theta_euler, *kdiff;
+    hreal *tend_u;
+    double *tend_w, *tend_theta, *h_divergence, *rthdynten;
+    hreal *ru_p, *ruAvg;
+    double *rho_pp, *rtheta_pp, *rtheta_pp_old, *rw_p, *wwAvg;
+    double *rho_zz_int;
+    hreal *horiz_flux;
+    double *scratch;
+    size_t total;  // doubles
 };
 
 static size_t kernel_scratch(const mpas_hip_mesh &m) {
@@ -124,55 +177,76 @@ static Work carve_work(const mpas_hip_mesh &m, int S, double *base) {
     const size_t L = m.nLevels, C = (size_t)(m.nCells + 1) * L, E = (size_t)(m.nEdges + 1) * L,
                  W = (size_t)(m.nCells + 1) * (L + 1);
     Work w;
-    double *p = base;
-    auto take = [&](size_t n) {
-        double *r = p;
-        p += (n + 31) & ~(size_t)31;  // 256-byte aligned slices
-        return r;
-    };
-    w.ru_save = take(E);
-    w.rw_save = take(W);
-    w.rtheta_p_save = take(C);
-    w.rho_p_save = take(C);
-    w.theta_m_init = take(C);
-    w.rho_zz_old_split = take(C);
-    w.scalars_old = take(C * S);
-    w.scalars_tmp = take(C * S);
-    w.ruAvg_split = take(E);
-    w.wwAvg_split = take(W);
-    w.cqw = take(W);
-    w.cqu = take(E);
-    w.qtot = take(C);
-    w.cofrz = take(L);
-    w.cofwr = take(C);
-    w.cofwz = take(C);
-    w.coftz = take(W);
-    w.cofwt = take(C);
-    w.a_tri = take(C);
-    w.alpha_tri = take(C);
-    w.gamma_tri = take(C);
-    w.tend_rho = take(C);
-    w.tend_u_euler = take(E);
-    w.tend_w_euler = take(W);
-    w.tend_theta_euler = take(C);
-    w.kdiff = take(C);
-    w.tend_u = take(E);
-    w.tend_w = take(W);
-    w.tend_theta = take(C);
-    w.h_divergence = take(C);
-    w.rthdynten = take(C);
-    w.ru_p = take(E);
-    w.ruAvg = take(E);
-    w.rho_pp = take(C);
-    w.rtheta_pp = take(C);
-    w.rtheta_pp_old = take(C);
-    w.rw_p = take(W);
-    w.wwAvg = take(W);
-    w.rho_zz_int = take(C);
-    w.horiz_flux = take(E * (S > 0 ? S : 1));
-    w.scratch = take(kernel_scratch(m));
-    w.total = (size_t)(p - base);
+    Carver k{reinterpret_cast<char *>(base)};  // 256-byte aligned slices
+    w.ru_save = k.take<hreal>(E);
+    w.rw_save = k.take<double>(W);
+    w.rtheta_p_save = k.take<double>(C);
+    w.rho_p_save = k.take<double>(C);
+    w.theta_m_init = k.take<double>(C);
+    w.rho_zz_old_split = k.take<double>(C);
+    w.scalars_old = k.take<double>(C * S);
+    w.scalars_tmp = k.take<double>(C * S);
+    w.ruAvg_split = k.take<hreal>(E);
+    w.wwAvg_split = k.take<double>(W);
+    w.cqw = k.take<double>(W);
+    w.cqu = k.take<hreal>(E);
+    w.qtot = k.take<double>(C);
+    w.cofrz = k.take<double>(L);
+    w.cofwr = k.take<double>(C);
+    w.cofwz = k.take<double>(C);
+    w.coftz = k.take<double>(W);
+    w.cofwt = k.take<double>(C);
+    w.a_tri = k.take<double>(C);
+    w.alpha_tri = k.take<double>(C);
+    w.gamma_tri = k.take<double>(C);
+    w.tend_rho = k.take<double>(C);
+    w.tend_u_euler = k.take<hreal>(E);
+    w.tend_w_euler = k.take<double>(W);
+    w.tend_theta_euler = k.take<double>(C);
+    w.kdiff = k.take<double>(C);
+    w.tend_u = k.take<hreal>(E);
+    w.tend_w = k.take<double>(W);
+    w.tend_theta = k.take<double>(C);
+    w.h_divergence = k.take<double>(C);
+    w.rthdynten = k.take<double>(C);
+    w.ru_p = k.take<hreal>(E);
+    w.ruAvg = k.take<hreal>(E);
+    w.rho_pp = k.take<double>(C);
+    w.rtheta_pp = k.take<double>(C);
+    w.rtheta_pp_old = k.take<double>(C);
+    w.rw_p = k.take<double>(W);
+    w.wwAvg = k.take<double>(W);
+    w.rho_zz_int = k.take<double>(C);
+    w.horiz_flux = k.take<hreal>(E * (S > 0 ? S : 1));
+    w.scratch = k.take<double>(kernel_scratch(m));
+    w.total = (size_t)(k.p - reinterpret_cast<char *>(base)) / sizeof(double);
     return w;
+}
+
+// The state as the step sees it: the edge/vertex fields in hreal.
+struct StepState {
+    hreal *u;
+    double *w, *theta_m, *rho_zz, *scalars, *rt_diabatic_tend;
+    double *rho_p, *rtheta_p, *exner, *pressure_p;
+    hreal *ru;
+    double *rw;
+    hreal *v;
+    double *divergence;
+    hreal *vorticity;
+    double *ke;
+    hreal *pv_edge, *rho_edge;
+    double *ur_cell, *vr_cell;
+    double *uReconstructX, *uReconstructY, *uReconstructZ;
+    double *uReconstructZonal, *uReconstructMeridional;
+    hreal *ruAvg;
+    double *wwAvg;
+};
+template <typename St>
+static StepState step_state(St *p) {
+    return StepState{p->u, p->w, p->theta_m, p->rho_zz, p->scalars, p->rt_diabatic_tend, p->rho_p, p->rtheta_p,
+                     p->exner, p->pressure_p, p->ru, p->rw, p->v, p->divergence, p->vorticity, p->ke, p->pv_edge,
+                     p->rho_edge, p->ur_cell, p->vr_cell, p->uReconstructX, p->uReconstructY, p->uReconstructZ,
+                     p->uReconstructZonal, p->uReconstructMeridional, p->ruAvg, p->wwAvg};
 }
 
 static int validate_mesh(const mpas_hip_mesh *m) {
@@ -209,6 +283,8 @@ static int validate_moist(int S, int ms, int me) {
 }  // namespace mpas
 
 using namespace mpas;
+
+#define STREAM(s) ((hipStream_t)(s))
 
 extern "C" {
 
@@ -283,6 +359,21 @@ int mpas_hip_timer_read(double *total_ms, int32_t *launches) {
     return 0;
 }
 
+#if defined(MPAS_MIXED) && MPAS_MIXED
+int32_t mpas_hip_mixed_state_size(void) { return (int32_t)sizeof(mpas_hip_mixed_state); }
+
+int64_t mpas_hip_mixed_workspace_doubles(int32_t nCells, int32_t nEdges, int32_t nVertices, int32_t nLevels,
+                                         int32_t nScalars) {
+    mpas_hip_mesh m;
+    std::memset(&m, 0, sizeof(m));
+    m.nCells = nCells;
+    m.nEdges = nEdges;
+    m.nVertices = nVertices;
+    m.nLevels = nLevels;
+    Work w = carve_work(m, nScalars, nullptr);
+    return (int64_t)w.total;
+}
+#else
 int mpas_hip_struct_sizes(int32_t *mesh, int32_t *state, int32_t *workspace, int32_t *config) {
     *mesh = (int32_t)sizeof(mpas_hip_mesh);
     *state = (int32_t)sizeof(mpas_hip_state);
@@ -313,8 +404,7 @@ int64_t mpas_hip_workspace_doubles(int32_t nCells, int32_t nEdges, int32_t nVert
     return (int64_t)w.total;
 }
 
-#define STREAM(s) ((hipStream_t)(s))
-
+#if !(defined(MPAS_MIXED) && MPAS_MIXED)
 int mpas_hip_moist_coefficients(const mpas_hip_mesh *m, const double *scalars, int32_t nScalars, int32_t moist_start,
                                 int32_t moist_end, double *cqw, double *cqu, double *qtot, void *stream) {
     if (validate_mesh(m) || validate_moist(nScalars, moist_start, moist_end)) return 1;
@@ -444,14 +534,17 @@ int mpas_hip_reconstruct(const mpas_hip_mesh *m, const double *u, double *ux, do
     return check_launch("reconstruct");
 }
 
+#endif  // plain build exports
+
+#endif  // per-kernel entry points (plain build)
+
 // ---------------------------------------------------------------------------
 // one large step
 // ---------------------------------------------------------------------------
-int mpas_hip_srk3_step(const mpas_hip_mesh *mesh, mpas_hip_state *st, const mpas_hip_workspace *work,
-                       const mpas_hip_config *cfg, void *stream) {
+static int srk3_step_impl(const mpas_hip_mesh *mesh, StepState *st, const mpas_hip_workspace *work,
+                          const mpas_hip_config *cfg, hipStream_t s) {
     if (validate_mesh(mesh)) return 1;
     const mpas_hip_mesh &m = *mesh;
-    hipStream_t s = STREAM(stream);
     const int S = cfg->nScalars;
     if (validate_moist(S, cfg->moist_start, cfg->moist_end)) return 1;
     Work w = carve_work(m, S, work->base);
@@ -557,8 +650,8 @@ int mpas_hip_srk3_step(const mpas_hip_mesh *mesh, mpas_hip_state *st, const mpas
             copy_d(w.ruAvg_split, w.ruAvg, E, s);
             copy_d(w.wwAvg_split, w.wwAvg, Wd, s);
         } else {
-            MPAS_LAUNCH(k_add, dim3(blocks_for((long long)E)), dim3(kBlock), 0, s, w.ruAvg_split, w.ruAvg, E);
-            MPAS_LAUNCH(k_add, dim3(blocks_for((long long)Wd)), dim3(kBlock), 0, s, w.wwAvg_split, w.wwAvg, Wd);
+            add_d(w.ruAvg_split, w.ruAvg, E, s);
+            add_d(w.wwAvg_split, w.wwAvg, Wd, s);
         }
         if (sub < cfg->split) {
             fill_zero(st->theta_m + (size_t)nC * L, L, s);
@@ -570,10 +663,8 @@ int mpas_hip_srk3_step(const mpas_hip_mesh *mesh, mpas_hip_state *st, const mpas
         }
     }
     const double inv_split = 1.0 / (double)cfg->split;
-    MPAS_LAUNCH(k_scale, dim3(blocks_for((long long)E)), dim3(kBlock), 0, s, st->ruAvg, w.ruAvg_split,
-                       inv_split, E);
-    MPAS_LAUNCH(k_scale, dim3(blocks_for((long long)Wd)), dim3(kBlock), 0, s, st->wwAvg, w.wwAvg_split,
-                       inv_split, Wd);
+    scale_d(st->ruAvg, w.ruAvg_split, inv_split, E, s);
+    scale_d(st->wwAvg, w.wwAvg_split, inv_split, Wd, s);
     // split transport RK3 (time_integration.py:3976-4675); rho_zz_new is the
     // post-dynamics rho_zz already in the state
     if (S > 0) {
@@ -599,5 +690,19 @@ int mpas_hip_srk3_step(const mpas_hip_mesh *mesh, mpas_hip_state *st, const mpas
     copy_d(st->vr_cell, st->uReconstructMeridional, C, s);
     return check_launch("srk3_step");
 }
+
+#if defined(MPAS_MIXED) && MPAS_MIXED
+int mpas_hip_mixed_srk3_step(const mpas_hip_mesh *mesh, mpas_hip_mixed_state *state, const mpas_hip_workspace *work,
+                             const mpas_hip_config *cfg, void *stream) {
+    StepState st = step_state(state);
+    return srk3_step_impl(mesh, &st, work, cfg, STREAM(stream));
+}
+#else
+int mpas_hip_srk3_step(const mpas_hip_mesh *mesh, mpas_hip_state *state, const mpas_hip_workspace *work,
+                       const mpas_hip_config *cfg, void *stream) {
+    StepState st = step_state(state);
+    return srk3_step_impl(mesh, &st, work, cfg, STREAM(stream));
+}
+#endif
 
 }  // extern "C"

@@ -133,7 +133,7 @@ class DeviceMesh:
         self.s = s
         self._base_src: dict[str, int] = {}
         self.set_base(mesh, state)
-        self._work: dict[int, torch.Tensor] = {}
+        self._work: dict[tuple[int, bool], torch.Tensor] = {}
         self._kscratch: torch.Tensor | None = None
 
     def set_base(self, mesh: Mapping, state: Mapping | None) -> None:
@@ -200,13 +200,17 @@ class DeviceMesh:
             return (nc + 1, L, n_scalars)
         return (nc + 1, L)
 
-    def workspace(self, n_scalars: int) -> torch.Tensor:
-        """Zero-initialised per-mesh workspace of one large step (kept for reuse)."""
-        w = self._work.get(n_scalars)
+    def workspace(self, n_scalars: int, mixed: bool = False) -> torch.Tensor:
+        """Zero-initialised per-mesh workspace of one large step (kept for reuse);
+        ``mixed`` sizes it for the mixed-precision step."""
+        w = self._work.get((n_scalars, mixed))
         if w is None:
-            n = N.load().mpas_hip_workspace_doubles(self.nc, self.ne, self.nv, self.L, n_scalars)
+            if mixed:
+                n = N.load_mixed().mpas_hip_mixed_workspace_doubles(self.nc, self.ne, self.nv, self.L, n_scalars)
+            else:
+                n = N.load().mpas_hip_workspace_doubles(self.nc, self.ne, self.nv, self.L, n_scalars)
             w = torch.zeros(int(n), dtype=torch.float64, device=self.device)
-            self._work[n_scalars] = w
+            self._work[(n_scalars, mixed)] = w
         return w
 
     def inverse_tables(self) -> "N.InverseTablesStruct":

@@ -162,17 +162,30 @@ def _reordered_mesh(mesh: Mapping[str, Any]):
     return perm, pm
 
 
+PRECISIONS = ("fp64", "mixed")
+
+
 class Srk3Stepper:
     """Device-resident state advanced in place by successive large steps.
 
     With ``reorder=True`` the mesh and state are held on the device in
     space-filling-curve order (``reorder.py``); ``state_dict`` maps the
     results back to the caller's order, bit for bit the same values.
+
+    ``precision="mixed"`` runs the mixed-precision step
+    (``libmpas_dycore_hip_mixed.so``): the edge- and vertex-located state
+    fields (``_native.MIXED_F32_FIELDS``) are held in float32 on the device,
+    every cell and interface field in float64; ``state_dict`` widens the
+    float32 fields back to float64.
     """
 
     def __init__(self, state: Mapping[str, Any], mesh: Mapping[str, Any], config: Mapping[str, Any] | None,
                  dt: float, device: torch.device | str | None = None, strict: bool = True,
-                 reorder: bool = False):
+                 reorder: bool = False, precision: str = "fp64"):
+        if precision not in PRECISIONS:
+            raise MpasConfigError(f"precision must be one of {PRECISIONS}, got {precision!r}")
+        self.precision = precision
+        self.mixed = precision == "mixed"
         self.device = require_gpu(device)
         if "scalars" not in state:
             raise MpasMissingFieldError(fields=["scalars"], source="state")
@@ -210,17 +223,30 @@ class Srk3Stepper:
                     t[k] = torch.zeros(shape, dtype=torch.float64, device=self.device)
                 else:
                     t[k] = _device_tensor(src, self.device, shape, k)
-            self.work = dm.workspace(n_scalars)
+                if self.mixed and k in N.MIXED_F32_FIELDS:
+                    t[k] = t[k].to(torch.float32).contiguous()
+            self.work = dm.workspace(n_scalars, self.mixed)
         self.t = t
         self._diagnostics_current = False
         self._st = N.StateStruct(*[t[k].data_ptr() for k in N.STATE_FIELDS])
         self._ws = N.WorkspaceStruct(self.work.data_ptr(), self.work.numel())
 
+    @property
+    def lib(self):
+        """The library whose step this stepper runs (its tuning table and timer)."""
+        return N.load_mixed() if self.mixed else N.load()
+
     def _launch(self) -> None:
-        lib = N.load()
         self.cfg.diagnostics_current = 1 if self._diagnostics_current else 0
-        N.check(lib.mpas_hip_srk3_step(self.mesh.ref, C.byref(self._st), C.byref(self._ws), C.byref(self.cfg),
-                                       C.c_void_p(stream_handle(self.device))), "mpas_hip_srk3_step")
+        stream = C.c_void_p(stream_handle(self.device))
+        if self.mixed:
+            lib = N.load_mixed()
+            N.check(lib.mpas_hip_mixed_srk3_step(self.mesh.ref, C.byref(self._st), C.byref(self._ws),
+                                                 C.byref(self.cfg), stream), "mpas_hip_mixed_srk3_step", lib)
+        else:
+            lib = N.load()
+            N.check(lib.mpas_hip_srk3_step(self.mesh.ref, C.byref(self._st), C.byref(self._ws), C.byref(self.cfg),
+                                           stream), "mpas_hip_srk3_step")
         self._diagnostics_current = True
 
     def step(self, n: int = 1) -> None:
@@ -263,6 +289,8 @@ class Srk3Stepper:
             dm = self.mesh
             kind = {dm.nc + 1: "cell", dm.ne + 1: "edge", dm.nv + 1: "vertex"}
             out = {k: v.index_select(0, self._inv_rows[kind[v.shape[0]]]) for k, v in self.t.items()}
+        if self.mixed:
+            out = {k: (v.to(torch.float64) if v.dtype == torch.float32 else v) for k, v in out.items()}
         out["h_edge"] = out["rho_edge"]
         return out
 

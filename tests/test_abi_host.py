@@ -30,6 +30,8 @@ from mpas_dycore_pytorch_amd.time_integration import resolve_config
 def _declared_functions() -> set[str]:
     names = set()
     for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        if h.endswith("mpas_hip_mixed.h"):
+            continue  # the mixed-precision library's header (its own test below)
         text = open(h).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"\b(mpas_hip_\w+)\s*\(", text):
@@ -174,3 +176,19 @@ def test_run_simulation_jvp_validates_the_perturbation():
         run_simulation.jvp(state, {}, {"u": torch.zeros(4, 3, dtype=torch.float32)})
     with pytest.raises(MpasUnsupportedCapabilityError):
         run_simulation.jvp(state, {}, {"raw": torch.zeros(2, dtype=torch.float64)})
+
+
+def test_mixed_library_exports_its_header():
+    """The mixed-precision library loads, exports what include/mpas_hip_mixed.h declares, and its
+    state struct matches the binding."""
+    text = open(os.path.join(ROOT, "include", "mpas_hip_mixed.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    declared = set(re.findall(r"\b(mpas_hip_mixed_\w+)\s*\(", text))
+    lib = N.load_mixed()
+    assert declared and all(hasattr(lib, n) for n in declared), declared
+    assert declared <= set(N.MIXED_SIGNATURES)
+    assert lib.mpas_hip_mixed_state_size() == C.sizeof(N.MixedStateStruct)
+    # the edge/vertex fields halve: the mixed workspace is smaller than the plain one
+    plain = N.load().mpas_hip_workspace_doubles(2562, 7680, 5120, 26, 1)
+    mixed = lib.mpas_hip_mixed_workspace_doubles(2562, 7680, 5120, 26, 1)
+    assert 0 < mixed < plain
