@@ -326,6 +326,16 @@ int mpas_hip_advance_scalars_mono_jvp(const mpas_hip_mesh *m, int32_t nScalars, 
                                       void *stream);
 int mpas_hip_reconstruct_jvp(const mpas_hip_mesh *m, const double *const *fwd, double *const *dot, void *stream);
 
+/* History diagnostics from the device state (output_writer.py:264-282): on
+ * the nCells x nLevels active rows of the padded inputs, theta = theta_m /
+ * (1 + rvord qv) with qv = scalars[..., 0] (0 when scalars is null), rho =
+ * rho_zz zz and pressure = pressure_p + pressure_base.  A null output is
+ * skipped. */
+int mpas_hip_history_fields(int32_t nCells, int32_t nLevels, int32_t nScalars, const double *theta_m,
+                            const double *scalars, const double *rho_zz, const double *zz,
+                            const double *pressure_p, const double *pressure_base, double *theta,
+                            double *rho, double *pressure, void *stream);
+
 /* Launch-shape table, no counterpart in the reference (its kernels are torch
  * operators): wavefronts per block (1, 2 or 4; 0 restores the default 4) of a
  * wavefront-per-entity kernel, by HIP kernel name (e.g. "k_dyn_edges_a").

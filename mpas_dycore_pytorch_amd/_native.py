@@ -124,6 +124,7 @@ SIGNATURES = {
     "mpas_hip_advance_scalars_jvp": (C.c_int, [_PM, _I32, _F64, _F64, _F64, _F64, _PP, _PP, _vp]),
     "mpas_hip_advance_scalars_mono_jvp": (C.c_int, [_PM, _I32, _F64, _F64, _PP, _PP, _dp, _vp]),
     "mpas_hip_reconstruct_jvp": (C.c_int, [_PM, _PP, _PP, _vp]),
+    "mpas_hip_history_fields": (C.c_int, [_I32, _I32, _I32, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _vp]),
     "mpas_hip_tuning_set": (C.c_int, [C.c_char_p, _I32]),
     "mpas_hip_tuning_get": (_I32, [C.c_char_p]),
     "mpas_hip_last_error": (C.c_char_p, []),

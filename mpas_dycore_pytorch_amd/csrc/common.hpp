@@ -252,4 +252,7 @@ struct InitCoupledOutputs {
 void launch_init_coupled(const mpas_hip_mesh &m, const double *theta, const double *rho, const double *qv,
                          const hreal *u, const double *w, const double *theta_base, const InitCoupledOutputs &o,
                          hipStream_t s);
+void launch_history_fields(int nC, int L, int S, const double *theta_m, const double *scalars, const double *rho_zz,
+                           const double *zz, const double *pressure_p, const double *pressure_base, double *theta,
+                           double *rho, double *pressure, hipStream_t s);
 }  // namespace mpas

@@ -99,4 +99,31 @@ void launch_init_coupled(const mpas_hip_mesh &m, const double *theta, const doub
                 o.rw);
 }
 
+// History diagnostics (output_writer.py:264-282) on the active rows:
+// theta = theta_m / (1 + rvord qv), rho = rho_zz zz, pressure = p' + p_base.
+__global__ __launch_bounds__(256) void k_history_fields(int nC, int L, int S, const double *__restrict__ theta_m,
+                                                        const double *__restrict__ scalars,
+                                                        const double *__restrict__ rho_zz,
+                                                        const double *__restrict__ zz,
+                                                        const double *__restrict__ pressure_p,
+                                                        const double *__restrict__ pressure_base,
+                                                        double *__restrict__ theta, double *__restrict__ rho,
+                                                        double *__restrict__ pressure) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)nC * L) return;
+    const double qv = (scalars && S > 0) ? scalars[i * S] : 0.0;
+    if (theta) theta[i] = theta_m[i] / (1.0 + kRvord * qv);
+    if (rho) rho[i] = rho_zz[i] * zz[i];
+    if (pressure) pressure[i] = pressure_p[i] + pressure_base[i];
+}
+
+void launch_history_fields(int nC, int L, int S, const double *theta_m, const double *scalars, const double *rho_zz,
+                           const double *zz, const double *pressure_p, const double *pressure_base, double *theta,
+                           double *rho, double *pressure, hipStream_t s) {
+    const long long n = (long long)nC * L;
+    if (n == 0) return;
+    MPAS_LAUNCH(k_history_fields, dim3(blocks_for(n)), dim3(kBlock), 0, s, nC, L, S, theta_m, scalars, rho_zz, zz,
+                pressure_p, pressure_base, theta, rho, pressure);
+}
+
 }  // namespace mpas

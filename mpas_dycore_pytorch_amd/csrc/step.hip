@@ -536,6 +536,22 @@ int mpas_hip_reconstruct(const mpas_hip_mesh *m, const double *u, double *ux, do
 
 #endif  // plain build exports
 
+int mpas_hip_history_fields(int32_t nCells, int32_t nLevels, int32_t nScalars, const double *theta_m,
+                            const double *scalars, const double *rho_zz, const double *zz, const double *pressure_p,
+                            const double *pressure_base, double *theta, double *rho, double *pressure, void *stream) {
+    if (nCells < 0 || nLevels < 1 || nScalars < 0) {
+        set_error("history_fields: bad dimensions");
+        return 1;
+    }
+    if ((theta && !theta_m) || (rho && (!rho_zz || !zz)) || (pressure && (!pressure_p || !pressure_base))) {
+        set_error("history_fields: an output was requested without its inputs");
+        return 1;
+    }
+    launch_history_fields(nCells, nLevels, nScalars, theta_m, scalars, rho_zz, zz, pressure_p, pressure_base, theta,
+                          rho, pressure, STREAM(stream));
+    return check_launch("history_fields");
+}
+
 #endif  // per-kernel entry points (plain build)
 
 // ---------------------------------------------------------------------------
