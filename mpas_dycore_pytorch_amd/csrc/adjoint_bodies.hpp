@@ -108,13 +108,46 @@ enum { theta_m, rtheta_pp, rtheta_pp_old, ru_p_in, ru_p_out, N };
 }
 
 // thread (edge e, level k), e in [0, nEdges]
-__device__ __forceinline__ void divdamp_vjp(const mpas_hip_mesh &m, double coef, Fwd F, Bar B, int e, int k) {
+// Gather form of the cell adjoints below, thread (cell c, level k): the sum
+// over the cell's edges of what divdamp_vjp scatters to that cell (used with
+// divdamp_vjp(..., cells = false), which then keeps the edge part only).
+__device__ __forceinline__ void divdamp_cells_vjp(const mpas_hip_mesh &m, double coef, Fwd F, Bar B, int c, int k) {
+    using namespace divdamp;
+    const int L = m.nLevels, ME = m.maxEdges;
+    if (c >= m.nCells || k >= L) return;
+    const int n = m.nEdgesOnCell[c];
+    double grt = 0.0, gold = 0.0, gth = 0.0;
+    for (int j = 0; j < n; ++j) {
+        const int e = m.edgesOnCell[c * ME + j];
+        const double g = bar_of(B, ru_p_out, (long long)e * L + k);
+        if (g == 0.0) continue;
+        const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+        const long long o1 = (long long)c1 * L + k, o2 = (long long)c2 * L + k;
+        const double d1 = -(F[rtheta_pp][o1] - F[rtheta_pp_old][o1]);
+        const double d2 = -(F[rtheta_pp][o2] - F[rtheta_pp_old][o2]);
+        const double ts = F[theta_m][o1] + F[theta_m][o2];
+        if (!(fabs(ts) >= 1.0e-30)) continue;
+        const double gd = coef * g / ts;
+        const double gts = -(coef * (d2 - d1) / ts) / ts * g;
+        const double sg = (c == c2) ? -1.0 : 1.0;
+        grt += sg * gd;
+        gold -= sg * gd;
+        gth += gts;
+    }
+    const long long o = (long long)c * L + k;
+    add_own(B, rtheta_pp, o, grt);
+    add_own(B, rtheta_pp_old, o, gold);
+    add_own(B, theta_m, o, gth);
+}
+
+__device__ __forceinline__ void divdamp_vjp(const mpas_hip_mesh &m, double coef, Fwd F, Bar B, int e, int k,
+                                            bool cells = true) {
     using namespace divdamp;
     const int L = m.nLevels;
     const long long i = (long long)e * L + k;
     const double g = bar_of(B, ru_p_out, i);
     add_own(B, ru_p_in, i, g);
-    if (e >= m.nEdges || g == 0.0) return;
+    if (!cells || e >= m.nEdges || g == 0.0) return;
     const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
     const long long o1 = (long long)c1 * L + k, o2 = (long long)c2 * L + k;
     const double d1 = -(F[rtheta_pp][o1] - F[rtheta_pp_old][o1]);

@@ -15,7 +15,8 @@ namespace adj {
 
 
 ADJ_WAVE_KERNEL(k_adj_smlstep, smlstep_vjp(m, ADJ_F, ADJ_B, ent, k))
-ADJ_WAVE_KERNEL(k_adj_divdamp, divdamp_vjp(m, A.coef, ADJ_F, ADJ_B, ent, k))
+ADJ_WAVE_KERNEL(k_adj_divdamp, divdamp_vjp(m, A.coef, ADJ_F, ADJ_B, ent, k, false))
+ADJ_WAVE_KERNEL(k_adj_divdamp_cells, divdamp_cells_vjp(m, A.coef, ADJ_F, ADJ_B, ent, k))
 ADJ_WAVE_KERNEL(k_adj_moist_cell, moist_cell_vjp(m, A.S, A.q0, A.q1, ADJ_F, ADJ_B, ent, k))
 ADJ_WAVE_KERNEL(k_adj_moist_edge, moist_edge_vjp(m, A.S, A.q0, A.q1, ADJ_F, ADJ_B, ent, k))
 ADJ_WAVE_KERNEL(k_adj_reconstruct, reconstruct_vjp(m, ADJ_F, ADJ_B, ent, k))
@@ -54,6 +55,213 @@ ADJ_WAVE_KERNEL(k_adj_mono_r6, mono_r6(m, A.S, A.s, A.dt, A.coef3, ADJ_F, ADJ_B,
 ADJ_WAVE_KERNEL(k_adj_mono_r7, mono_r7(m, A.S, A.s, A.dt, ADJ_F, ADJ_B, A.mo, ent, k))
 ADJ_WAVE_KERNEL(k_adj_mono_r8, mono_r8(m, A.dt, ADJ_F, ADJ_B, ent, k))
 
+// ---------------------------------------------------------------------------
+// Dynamics-tendency edge adjoint, one wavefront per edge (lane = level), for
+// the gather form (inverse tables given).  The algebra is dyn_a3's; what
+// changes is where the sums are formed:
+//  * contributions to the edge's own column (u, ru, rho_edge, pv_edge, ...)
+//    are summed in registers, the ones a lane owes the levels next to it
+//    (the flux3 stencil of the u vertical flux, the averaged ru of the w
+//    flux) cross lanes with shuffles, and each lane writes its level once;
+//  * contributions to the two cells (and vertices) of the edge stay FP64
+//    atomics (storing them per edge for a cell-gather pass moved more bytes
+//    than the atomics cost).
+// All 64 lanes run through the shuffles; lanes past the column compute on a
+// clamped level and contribute nothing.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_ent, int n_lev, mpas_hip_mesh m,
+                                                     Slots S_, KArgs A) {
+    using namespace dyn;
+    const int e = wave_entity_base(nb);
+    if (e >= m.nEdges) return;  // wave-uniform
+    const double *const *F = S_.f;
+    double *const *B = S_.b;
+    const DynAdjScratch &S = A.dy;
+    const mpas_hip_dyn_params &P = A.dp;
+    const int L = m.nLevels, Lw = L + 1, MA = m.maxAdv, ME2 = m.maxEdges2;
+    const int lane = wave_lane();
+    const bool act = lane < L;
+    const double on = act ? 1.0 : 0.0;
+    const int k = act ? lane : L - 1;
+    const bool rk1 = (P.rk_step == 1);
+    const long long i = (long long)e * L + k;
+    const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+    const double s1 = edge_sign_in_cell(m, c1, e), s2 = edge_sign_in_cell(m, c2, e);
+    const double *ru_ = F[ru] + (long long)e * L;
+    const int na = m.nAdvCellsForEdge[e];
+    double g_ru_own = 0.0, g_ru_dn = 0.0;  // to ru(k) and ru(k-1)
+    double x_ths = 0.0;
+    // ---- theta edge flux
+    {
+        const double gf = -(s1 * S.g_thadv[(long long)c1 * L + k] + s2 * S.g_thadv[(long long)c2 * L + k]);
+        const double rut = ru_[k], srt = sign_pm(rut);
+        double ft = 0.0;
+        for (int a = 0; a < na; ++a) {
+            const int ac = m.advCellsForEdge[e * MA + a];
+            const double wgt = m.adv_coefs[e * MA + a] + srt * m.adv_coefs_3rd[e * MA + a];
+            const double cc = wgt * F[theta_m][(long long)ac * L + k];
+            ft = (a == 0) ? cc : ft + cc;
+        }
+        if (act) S.g_tf[i] = gf;
+        g_ru_own += ft * gf;
+        if (!rk1) {
+            const double face = 0.5 * (F[theta_m_save][(long long)c2 * L + k] + F[theta_m_save][(long long)c1 * L + k]);
+            const double dv = m.dvEdge[e];
+            if (act) add_own(B, ru_save, i, dv * face * gf);
+            g_ru_own += -dv * face * gf;
+            x_ths = 0.5 * (dv * (F[ru_save][i] - ru_[k]) * gf);
+        }
+    }
+    // ---- w edge flux at interface k (1..L-1)
+    if (k >= 1) {
+        const double gf = -(s1 * S.g_hadv[(long long)c1 * Lw + k] + s2 * S.g_hadv[(long long)c2 * Lw + k]);
+        const double ruw = m.fzm[k] * ru_[k] + m.fzp[k] * ru_[k - 1];
+        const double srw = sign_pm(ruw);
+        double fw = 0.0;
+        for (int a = 0; a < na; ++a) {
+            const int ac = m.advCellsForEdge[e * MA + a];
+            const double wgt = m.adv_coefs[e * MA + a] + srw * m.adv_coefs_3rd[e * MA + a];
+            const double cc = wgt * F[w][(long long)ac * Lw + k];
+            fw = (a == 0) ? cc : fw + cc;
+        }
+        if (act) S.g_wf[i] = gf;
+        g_ru_own += m.fzm[k] * fw * gf;
+        g_ru_dn = m.fzp[k] * fw * gf;
+    }
+    // ---- u tendency
+    const double gtu = bar_of(B, tend_u, i);
+    const double *ue = F[u] + (long long)e * L;
+    const double re = F[rho_edge][i];
+    double g_re = 0.0, g_u_own = 0.0, g_pv_own = 0.0;
+    double x_ke, x_hd;
+    {
+        const int ne = m.nEdgesOnEdge[e];
+        const double pvs = F[pv_edge][i];
+        double q = 0.0;
+        for (int j = 0; j < ne; ++j) {
+            const int ee = m.edgesOnEdge[e * ME2 + j];
+            const double wv = m.weightsOnEdge[e * ME2 + j];
+            const double uj = F[u][(long long)ee * L + k], pj = F[pv_edge][(long long)ee * L + k];
+            const double cc = wv * uj * (0.5 * (pvs + pj));
+            q = (j == 0) ? cc : q + cc;
+        }
+        const double ked = F[ke][(long long)c2 * L + k] - F[ke][(long long)c1 * L + k];
+        const double hsum = F[h_divergence][(long long)c1 * L + k] + F[h_divergence][(long long)c2 * L + k];
+        g_re += (q - ked * m.invDcEdge[e]) * gtu;
+        const double gq = re * gtu;
+        x_ke = re * m.invDcEdge[e] * gtu;
+        g_u_own += -0.5 * hsum * gtu;
+        x_hd = -ue[k] * 0.5 * gtu;
+        double gpe = 0.0;
+        for (int j = 0; j < ne; ++j) {
+            const int ee = m.edgesOnEdge[e * ME2 + j];
+            gpe += m.weightsOnEdge[e * ME2 + j] * F[u][(long long)ee * L + k] * 0.5 * gq;
+        }
+        g_pv_own += gpe;
+        if (act) S.g_q[i] = gq;
+    }
+    // ---- vertical flux of u at interface k: contributions to u(k-2..k+1)
+    double c_m2 = 0.0, c_m1 = 0.0, c_p1 = 0.0, x_rw = 0.0;
+    if (k >= 1) {
+        const double g = -m.rdzw[k - 1] * bar_of(B, tend_u, i - 1) + m.rdzw[k] * gtu;
+        const double erw = 0.5 * (F[rw][(long long)c1 * Lw + k] + F[rw][(long long)c2 * Lw + k]);
+        double gerw;
+        if (k == 1 || k == L - 1) {
+            const double ui = m.fzm[k] * ue[k] + m.fzp[k] * ue[k - 1];
+            gerw = ui * g;
+            g_u_own += erw * m.fzm[k] * g;
+            c_m1 = erw * m.fzp[k] * g;
+        } else {
+            double d[5];
+            flux3_partials(ue[k - 2], ue[k - 1], ue[k], ue[k + 1], erw, 1.0, d);
+            c_m2 = d[0] * g;
+            c_m1 = d[1] * g;
+            g_u_own += d[2] * g;
+            c_p1 = d[3] * g;
+            gerw = d[4] * g;
+        }
+        x_rw = 0.5 * gerw;
+    }
+    // ---- tend_u_euler
+    const double gtue = bar_of(B, tend_u_euler, i) + gtu;
+    double x_pp = 0.0, x_dp = 0.0, x_kd = 0.0, x_dd = 0.0, x_dv = 0.0, g_dsu = 0.0;
+    if (!rk1) {
+        if (act) add_own(B, tend_u_euler_in, i, gtue);
+    } else {
+        const double zzs = m.zz[(long long)c2 * L + k] + m.zz[(long long)c1 * L + k];
+        const double ppd = F[pp][(long long)c2 * L + k] - F[pp][(long long)c1 * L + k];
+        const double dps = S.dpdz[(long long)c1 * L + k] + S.dpdz[(long long)c2 * L + k];
+        const double cq = F[cqu][i];
+        if (act) add_own(B, cqu, i, -(ppd * m.invDcEdge[e] / (0.5 * zzs) - 0.5 * m.zxu[i] * dps) * gtue);
+        x_pp = cq * m.invDcEdge[e] / (0.5 * zzs) * gtue;  // pp(c1) += x, pp(c2) -= x
+        x_dp = cq * 0.5 * m.zxu[i] * gtue;
+        const double ud = S.delsq_u[i];
+        const double kdu = 0.5 * (F[kdiff][(long long)c1 * L + k] + F[kdiff][(long long)c2 * L + k]);
+        const double ms2 = m.meshScalingDel2[e];
+        g_re += kdu * ud * ms2 * gtue;
+        x_kd = 0.5 * re * ud * ms2 * gtue;
+        g_dsu = re * kdu * ms2 * gtue;
+        if (P.h_mom_eddy_visc4 > 0.0) {
+            const int v1 = m.verticesOnEdge[2 * e], v2 = m.verticesOnEdge[2 * e + 1];
+            const double ums = m.meshScalingDel4[e] * P.h_mom_eddy_visc4;
+            const double r_dc = ums * P.del4u_div_factor * m.invDcEdge[e];
+            const double r_dv = ums * dmin(m.invDvEdge[e], 4.0 * m.invDcEdge[e]);
+            const double dd = S.delsq_div[(long long)c2 * L + k] - S.delsq_div[(long long)c1 * L + k];
+            const double dvv = S.delsq_vort[(long long)v2 * L + k] - S.delsq_vort[(long long)v1 * L + k];
+            g_re += -(dd * r_dc - dvv * r_dv) * gtue;
+            x_dd = re * r_dc * gtue;
+            x_dv = re * r_dv * gtue;
+        }
+    }
+    // ---- lanes exchange what they owe the neighbouring levels
+    c_m2 *= on;
+    c_m1 *= on;
+    c_p1 *= on;
+    g_ru_dn *= on;
+    const double from_p1 = __shfl_down(c_m1, 1);   // lane k+1's share for level k
+    const double from_p2 = __shfl_down(c_m2, 2);   // lane k+2's share for level k
+    const double from_m1 = __shfl_up(c_p1, 1);     // lane k-1's share for level k
+    const double ru_from_p1 = __shfl_down(g_ru_dn, 1);
+    if (!act) return;
+    double gu = g_u_own;
+    if (k + 1 <= L - 1) gu += from_p1;
+    if (k + 2 <= L - 1) gu += from_p2;
+    if (k >= 2) gu += from_m1;
+    add_own(B, u, i, gu);
+    add_own(B, ru, i, g_ru_own + ((k + 1 <= L - 1) ? ru_from_p1 : 0.0));
+    add_own(B, rho_edge, i, g_re);
+    add_own(B, pv_edge, i, g_pv_own);
+    // the two cells (and vertices) of the edge: hardware FP64 atomics
+    const long long o1 = (long long)c1 * L + k, o2 = (long long)c2 * L + k;
+    add(B, ke, o1, x_ke);
+    add(B, ke, o2, -x_ke);
+    acc(B[h_divergence] + o1, x_hd);
+    acc(B[h_divergence] + o2, x_hd);
+    if (k >= 1) {
+        add(B, rw, (long long)c1 * Lw + k, x_rw);
+        add(B, rw, (long long)c2 * Lw + k, x_rw);
+    }
+    if (rk1) {
+        add(B, pp, o1, x_pp);
+        add(B, pp, o2, -x_pp);
+        acc(S.g_dpdz + o1, x_dp);
+        acc(S.g_dpdz + o2, x_dp);
+        acc(B[kdiff] + o1, x_kd);
+        acc(B[kdiff] + o2, x_kd);
+        S.g_delsq_u[i] += g_dsu;
+        if (P.h_mom_eddy_visc4 > 0.0) {
+            const int v1 = m.verticesOnEdge[2 * e], v2 = m.verticesOnEdge[2 * e + 1];
+            acc(S.g_delsq_div + o1, x_dd);
+            acc(S.g_delsq_div + o2, -x_dd);
+            acc(S.g_delsq_vort + (long long)v1 * L + k, -x_dv);
+            acc(S.g_delsq_vort + (long long)v2 * L + k, x_dv);
+        }
+    } else {
+        add(B, theta_m_save, o1, x_ths);
+        add(B, theta_m_save, o2, x_ths);
+    }
+}
+
 // column kernels: one thread per column
 __global__ __launch_bounds__(256) void k_adj_vimp_col(mpas_hip_mesh m, Slots S_, KArgs A) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -79,6 +287,7 @@ __global__ __launch_bounds__(64 * kColWaves) void k_adj_acoustic_colw(mpas_hip_m
     __shared__ double s_rs[kColWaves][65], s_ts[kColWaves][65], s_r[kColWaves][65], s_y[kColWaves][65];
     __shared__ double s_x[kColWaves][65], s_gx[kColWaves][65], s_gy[kColWaves][65], s_gri[kColWaves][65];
     __shared__ double s_gr[kColWaves][65], s_grs[kColWaves][65], s_gts[kColWaves][65];
+    __shared__ double s_fa[kColWaves][65], s_fal[kColWaves][65], s_fga[kColWaves][65];  // staged factor
     const double *const *F = S_.f;
     double *const *B = S_.b;
     const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
@@ -100,8 +309,11 @@ __global__ __launch_bounds__(64 * kColWaves) void k_adj_acoustic_colw(mpas_hip_m
     auto COFTZ = [&](int kk) { return F[coftz][ow + kk]; };
     const int n = live ? m.nEdgesOnCell[c] : 0;
     const double ia = live ? m.invAreaCell[c] : 0.0;
-    // ---- phase 1: rs, ts
+    // ---- phase 1: rs, ts; stage the factor for the serial sweeps
     if (live && k < L) {
+        s_fa[wv][k] = F[a_tri][oc + k];
+        s_fal[wv][k] = F[alpha_tri][oc + k];
+        s_fga[wv][k] = F[gamma_tri][oc + k];
         double rsum = 0.0, tsum = 0.0;
         for (int i = 0; i < n; ++i) {
             const int e = m.edgesOnCell[c * ME + i];
@@ -129,17 +341,16 @@ __global__ __launch_bounds__(64 * kColWaves) void k_adj_acoustic_colw(mpas_hip_m
     __syncthreads();
     // ---- phase 3: forward sweeps (lane 0)
     if (live && k == 0) {
-        const double *a_ = F[a_tri] + oc, *al_ = F[alpha_tri] + oc, *ga_ = F[gamma_tri] + oc;
         double y = RWP(0);
         s_y[wv][0] = y;
         for (int kk = 1; kk < L; ++kk) {
-            y = (s_r[wv][kk] - a_[kk] * y) * al_[kk];
+            y = (s_r[wv][kk] - s_fa[wv][kk] * y) * s_fal[wv][kk];
             s_y[wv][kk] = y;
         }
         double x = RWP(L);
         s_x[wv][L] = x;
         for (int kk = L - 1; kk >= 0; --kk) {
-            x = s_y[wv][kk] - ga_[kk] * x;
+            x = s_y[wv][kk] - s_fga[wv][kk] * x;
             s_x[wv][kk] = x;
         }
     }
@@ -185,10 +396,9 @@ __global__ __launch_bounds__(64 * kColWaves) void k_adj_acoustic_colw(mpas_hip_m
         if (g != 0.0) add_own(B, rho_zz, oc + k, g);
     }
     if (live && k == 0) {
-        const double *a_ = F[a_tri] + oc, *al_ = F[alpha_tri] + oc, *ga_ = F[gamma_tri] + oc;
-        for (int kk = 0; kk < L; ++kk) s_gx[wv][kk + 1] += -ga_[kk] * s_gx[wv][kk];
+        for (int kk = 0; kk < L; ++kk) s_gx[wv][kk + 1] += -s_fga[wv][kk] * s_gx[wv][kk];
         for (int kk = 0; kk < L; ++kk) s_gy[wv][kk] = s_gx[wv][kk];
-        for (int kk = L - 1; kk >= 1; --kk) s_gy[wv][kk - 1] += -a_[kk] * al_[kk] * s_gy[wv][kk];
+        for (int kk = L - 1; kk >= 1; --kk) s_gy[wv][kk - 1] += -s_fa[wv][kk] * s_fal[wv][kk] * s_gy[wv][kk];
     }
     __syncthreads();
     // ---- phase 6: factor adjoints and the right-hand-side adjoint g_r
@@ -280,6 +490,9 @@ __global__ __launch_bounds__(64 * kColWaves) void k_adj_vimp_colw(mpas_hip_mesh 
     using namespace vimp;
     __shared__ double s_ga[kColWaves][65], s_gb[kColWaves][65], s_gc[kColWaves][65];
     __shared__ double s_gpi[kColWaves][65], s_gti[kColWaves][65];
+    // the column the serial sweep walks, staged by all lanes
+    __shared__ double s_al[kColWaves][65], s_a[kColWaves][65], s_gam[kColWaves][65], s_cc[kColWaves][65];
+    __shared__ double s_bg[kColWaves][65], s_bl[kColWaves][65], s_ba[kColWaves][65];
     const double *const *F = S_.f;
     double *const *B = S_.b;
     const int L = m.nLevels, nC = m.nCells, Lw = L + 1;
@@ -295,23 +508,34 @@ __global__ __launch_bounds__(64 * kColWaves) void k_adj_vimp_colw(mpas_hip_mesh 
     auto TZ = [&](int kk) { return F[coftz][ow + kk]; };
     auto WT = [&](int kk) { return F[cofwt][o + kk]; };
     auto WR = [&](int kk) { return F[cofwr][o + kk]; };
-    // ---- phase 1: reverse Thomas factor (lane 0) -> adjoints of a, b, c
+    // ---- phase 0: stage the factor, its adjoints and the upper diagonal c (all lanes)
+    if (live && k < L) {
+        s_al[wv][k] = F[alpha_tri][o + k];
+        s_a[wv][k] = F[a_tri][o + k];
+        s_gam[wv][k] = F[gamma_tri][o + k];
+        s_cc[wv][k] = (k >= 1) ? -WZ(k) * TZ(k + 1) * rdzw[k] * zz[k] - WR(k) * F[cofrz][k] +
+                                     WT(k) * TZ(k + 1) * rdzw[k]
+                               : 0.0;
+        s_bg[wv][k] = bar_of(B, gamma_tri, o + k);
+        s_bl[wv][k] = bar_of(B, alpha_tri, o + k);
+        s_ba[wv][k] = bar_of(B, a_tri, o + k);
+    }
+    __syncthreads();
+    // ---- phase 1: reverse Thomas factor (lane 0, over LDS) -> adjoints of a, b, c
     if (live && k == 0) {
-        const double *cofrz_ = F[cofrz], *a_ = F[a_tri] + o, *al_ = F[alpha_tri] + o, *ga_ = F[gamma_tri] + o;
         double gg_carry = 0.0;
         s_ga[wv][0] = s_gb[wv][0] = s_gc[wv][0] = 0.0;
         s_ga[wv][L] = s_gb[wv][L] = s_gc[wv][L] = 0.0;
         for (int kk = L - 1; kk >= 1; --kk) {
-            const double al = al_[kk], a = a_[kk];
-            const double gprev = (kk > 1) ? ga_[kk - 1] : 0.0;
-            const double cc = -WZ(kk) * TZ(kk + 1) * rdzw[kk] * zz[kk] - WR(kk) * cofrz_[kk] +
-                              WT(kk) * TZ(kk + 1) * rdzw[kk];
-            const double g_ga = bar_of(B, gamma_tri, o + kk) + gg_carry;
-            const double g_al = bar_of(B, alpha_tri, o + kk) + cc * g_ga;
+            const double al = s_al[wv][kk], a = s_a[wv][kk];
+            const double gprev = (kk > 1) ? s_gam[wv][kk - 1] : 0.0;
+            const double cc = s_cc[wv][kk];
+            const double g_ga = s_bg[wv][kk] + gg_carry;
+            const double g_al = s_bl[wv][kk] + cc * g_ga;
             const double g_den = -(al * al) * g_al;
             s_gc[wv][kk] = al * g_ga;
             s_gb[wv][kk] = g_den;
-            s_ga[wv][kk] = bar_of(B, a_tri, o + kk) - gprev * g_den;
+            s_ga[wv][kk] = s_ba[wv][kk] - gprev * g_den;
             gg_carry = -a * g_den;
         }
     }
@@ -442,6 +666,7 @@ int mpas_hip_divergence_damping_vjp(const mpas_hip_mesh *mesh, double dts, doubl
     KArgs A{};
     A.coef = 2.0 * smdiv * len_disp / dts;
     ADJ_LAUNCH(k_adj_divdamp, m.nEdges + 1, m.nLevels, STREAM(stream));
+    ADJ_LAUNCH(k_adj_divdamp_cells, m.nCells, m.nLevels, STREAM(stream));
     return check_launch("divergence_damping_vjp");
 }
 
@@ -560,10 +785,12 @@ int mpas_hip_dyn_tend_vjp(const mpas_hip_mesh *mesh, const mpas_hip_dyn_params *
     }
     ADJ_LAUNCH(k_adj_dyn_a1, nC, L + 1, s);
     if (rk1) ADJ_LAUNCH(k_adj_dyn_a2, nC, L + 1, s);
-    ADJ_LAUNCH(k_adj_dyn_a3, nE, L, s);
     if (inv) {
+        ADJ_LAUNCH(k_adj_dyn_a3w, nE, L, s);
         ADJ_LAUNCH(k_adj_dyn_a3_cells, nC, L, s);
         ADJ_LAUNCH(k_adj_dyn_a3_edges, nE, L, s);
+    } else {
+        ADJ_LAUNCH(k_adj_dyn_a3, nE, L, s);
     }
     if (rk1 && p->h_mom_eddy_visc4 > 0.0) ADJ_LAUNCH(k_adj_dyn_a4, ncv, L, s);
     if (rk1) ADJ_LAUNCH(k_adj_dyn_a5, nE, L, s);
