@@ -109,11 +109,23 @@ class Tape:
         self.ops.append(Op("mask_pad", (row,), [a], [out]))
         return out
 
+    def depending_on(self, sources) -> set[int]:
+        """Ids of the recorded tensors that depend on any of ``sources`` (a forward scan)."""
+        dep = {id(t) for t in sources}
+        for op in self.ops:
+            if any(id(t) in dep for t in op.ins):
+                dep.update(id(t) for t in op.outs)
+        return dep
+
     # ---- reverse sweep
-    def backward(self, seeds: Mapping[int, torch.Tensor], scratch: torch.Tensor) -> dict[int, torch.Tensor]:
+    def backward(self, seeds: Mapping[int, torch.Tensor], scratch: torch.Tensor,
+                 wrt: set[int] | None = None) -> dict[int, torch.Tensor]:
         """Adjoints of every tensor reached, keyed by ``id(tensor)``.
 
-        ``seeds`` maps ``id(output tensor)`` to its adjoint (cotangent)."""
+        ``seeds`` maps ``id(output tensor)`` to its adjoint (cotangent).  With
+        ``wrt`` (ids of the tensors that depend on the inputs of interest, see
+        :meth:`depending_on`) no adjoint is formed for any other tensor: the
+        kernels get a null slot and skip that work."""
         bars: dict[int, torch.Tensor] = {k: v.clone() for k, v in seeds.items()}
         lib = N.load()
         dm = self.dm
@@ -154,6 +166,10 @@ class Tape:
             out_bars = [bars.get(id(t)) for t in op.outs]
             if all(b is None for b in out_bars):
                 continue  # nothing downstream depends on this call
+            if wrt is not None and not any(id(t) in wrt for t in op.ins):
+                for t in op.outs:
+                    bars.pop(id(t), None)
+                continue  # no input of interest upstream of this call
             req = REQUIRED_OUT.get(op.kind, ())
             for j, name in enumerate(so):
                 if out_bars[j] is None and name in req:
@@ -163,7 +179,7 @@ class Tape:
             pending: list[tuple[torch.Tensor, torch.Tensor]] = []
             seen: set[int] = set()
             for name, t in zip(si, op.ins):
-                if name in const:
+                if name in const or (wrt is not None and id(t) not in wrt):
                     in_bars.append(None)
                     continue
                 if id(t) in seen:
@@ -492,7 +508,7 @@ def step_vjp(dm: DeviceMesh, state: Mapping[str, torch.Tensor], cfg: Any, dt: fl
             seeds[id(t)] = g.to(device=t.device, dtype=torch.float64).contiguous()
     if scratch is None:
         scratch = vjp_scratch(dm, int(state["scalars"].shape[2]))
-    bars = tape.backward(seeds, scratch)
+    bars = tape.backward(seeds, scratch, tape.depending_on(state[k] for k in STEP_INPUTS))
     grads = {}
     for k in STEP_INPUTS:
         b = bars.get(id(state[k]))

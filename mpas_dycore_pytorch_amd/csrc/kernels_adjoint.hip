@@ -469,20 +469,73 @@ __global__ __launch_bounds__(64 * kColWaves) void k_adj_acoustic_colw(mpas_hip_m
         }
     }
     if (live && k < L) {
+        // The horizontal flux terms reach the cell's edges and neighbours:
+        // keep this cell's rs/ts adjoints for the gather passes
+        // (k_adj_acoustic_edge_flux, k_adj_acoustic_theta) and add the part
+        // of theta_m(c) that uses them here.
         const double grs_k = s_grs[wv][k], gts_k = s_gts[wv][k];
+        A.col[oc + k] = grs_k;
+        A.col[(long long)(nC + 1) * L + oc + k] = gts_k;
+        double gth = 0.0;
         for (int i = 0; i < n; ++i) {
             const int e = m.edgesOnCell[c * ME + i];
-            const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
             const double metric = m.edgesOnCell_sign[c * ME + i] * m.dvEdge[e];
-            const double thedge = 0.5 * (F[theta_m][(long long)c2 * L + k] + F[theta_m][(long long)c1 * L + k]);
             const double flux = dts * metric * F[ru_p][(long long)e * L + k] * ia;
-            const double gflux = -grs_k - thedge * gts_k;
-            add(B, ru_p, (long long)e * L + k, dts * metric * ia * gflux);
-            const double gth = 0.5 * (-flux * gts_k);
-            add(B, theta_m, (long long)c2 * L + k, gth);
-            add(B, theta_m, (long long)c1 * L + k, gth);
+            gth += 0.5 * (-flux * gts_k);
         }
+        add_own(B, theta_m, oc + k, gth);
     }
+}
+
+// thread (edge e, level k) after k_adj_acoustic_colw: the ru_p adjoint of the
+// two cells' flux terms, gathered from their rs/ts adjoints, then the edge
+// update's own adjoint.
+__device__ __forceinline__ void acoustic_flux_gather(const mpas_hip_mesh &m, const mpas_hip_acoustic_params &P,
+                                                     Fwd F, Bar B, const double *col, int e, int k) {
+    using namespace acoustic;
+    const int L = m.nLevels;
+    if (e >= m.nEdges || k >= L) return;
+    const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+    const long long o1 = (long long)c1 * L + k, o2 = (long long)c2 * L + k;
+    const long long gts0 = (long long)(m.nCells + 1) * L;
+    const double thedge = 0.5 * (F[theta_m][o2] + F[theta_m][o1]);
+    const double dv = m.dvEdge[e];
+    const double s1 = edge_sign_in_cell(m, c1, e), s2 = edge_sign_in_cell(m, c2, e);
+    const double g1 = -col[o1] - thedge * col[gts0 + o1];
+    const double g2 = -col[o2] - thedge * col[gts0 + o2];
+    add_own(B, ru_p, (long long)e * L + k,
+            P.dts * (s1 * dv) * m.invAreaCell[c1] * g1 + P.dts * (s2 * dv) * m.invAreaCell[c2] * g2);
+}
+__global__ __launch_bounds__(256) void k_adj_acoustic_edge_flux(long long nb, long long n_ent, int n_lev,
+                                                                mpas_hip_mesh m, Slots S_, KArgs A) {
+    const int ent = wave_entity_base(nb);
+    const int k = wave_lane();
+    if (ent >= n_ent || k >= n_lev) return;
+    acoustic_flux_gather(m, A.ap, S_.f, S_.b, A.col, ent, k);
+    acoustic_edge_vjp(m, A.ap, S_.f, S_.b, ent, k);
+}
+// thread (cell c, level k): theta_m(c) through the flux terms of its
+// neighbours (the cell's own were added in k_adj_acoustic_colw)
+__global__ __launch_bounds__(256) void k_adj_acoustic_theta(long long nb, long long n_ent, int n_lev,
+                                                            mpas_hip_mesh m, Slots S_, KArgs A) {
+    using namespace acoustic;
+    const int c = wave_entity_base(nb);
+    const int k = wave_lane();
+    if (c >= n_ent || k >= n_lev) return;
+    const double *const *F = S_.f;
+    const int L = m.nLevels, ME = m.maxEdges;
+    const long long gts0 = (long long)(m.nCells + 1) * L;
+    const int n = m.nEdgesOnCell[c];
+    double gth = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const int e = m.edgesOnCell[c * ME + i];
+        const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+        const int cn = (c1 == c) ? c2 : c1;
+        const double metric_n = -m.edgesOnCell_sign[c * ME + i] * m.dvEdge[e];  // sign of e in cn
+        const double flux = A.ap.dts * metric_n * F[ru_p][(long long)e * L + k] * m.invAreaCell[cn];
+        gth += 0.5 * (-flux * A.col[gts0 + (long long)cn * L + k]);
+    }
+    add_own(S_.b, theta_m, (long long)c * L + k, gth);
 }
 
 // Adjoint of the vertical-implicit coefficients (vimp_col_vjp, same algebra).
@@ -758,7 +811,8 @@ int mpas_hip_acoustic_step_vjp(const mpas_hip_mesh *mesh, const mpas_hip_acousti
     A.col = scratch;
     MPAS_LAUNCH(k_adj_acoustic_colw, dim3((m.nCells + kColWaves - 1) / kColWaves), dim3(64, kColWaves), 0, s, m,
                 S_, A);
-    ADJ_LAUNCH(k_adj_acoustic_edge, m.nEdges, m.nLevels, s);
+    ADJ_LAUNCH(k_adj_acoustic_theta, m.nCells, m.nLevels, s);
+    ADJ_LAUNCH(k_adj_acoustic_edge_flux, m.nEdges, m.nLevels, s);
     return check_launch("acoustic_step_vjp");
 }
 
