@@ -337,7 +337,7 @@ int mpas_hip_history_fields(int32_t nCells, int32_t nLevels, int32_t nScalars, c
                             double *rho, double *pressure, void *stream);
 
 /* Launch-shape table, no counterpart in the reference (its kernels are torch
- * operators): wavefronts per block (1, 2 or 4; 0 restores the default 4) of a
+ * operators): wavefronts per block (1, 2, 4, 8 or 16; 0 restores the default 4) of a
  * wavefront-per-entity kernel, by HIP kernel name (e.g. "k_dyn_edges_a").
  * Written by the autotuner (autotune.py) and read at every launch. */
 int mpas_hip_tuning_set(const char *kernel_name, int32_t waves_per_block);

@@ -1,7 +1,7 @@
 """Launch-shape autotuner for the wavefront-per-entity kernels.
 
 Each wavefront-per-entity kernel (one entity per wavefront, lane = level)
-can be launched with 1, 2 or 4 wavefronts per block; the best choice depends
+can be launched with 1, 2, 4, 8 or 16 wavefronts per block; the best choice depends
 on the kernel's register and LDS footprint, on the mesh size and on how the
 launch spreads over the 8 XCDs.  :func:`autotune` times every candidate of
 every kernel on the real workload with HIP events around that kernel's
@@ -27,7 +27,7 @@ WAVE_KERNELS: dict[str, int] = {
     "k_acoustic_edges": 12, "k_recover_edges": 9, "k_recover_w": 9, "k_diag_edges2": 9, "k_diag_cells": 9,
     "k_diag_vertices": 9,
 }
-CANDIDATES = (1, 2, 4)
+CANDIDATES = (1, 2, 4, 8, 16)
 
 
 def current() -> dict[str, int]:

@@ -113,7 +113,7 @@ inline MonoScratch carve_mono(const mpas_hip_mesh &m, double *p) {
 
 #define ADJ_LAUNCH(kernel, n_ent, n_lev, stream)                                                      \
     do {                                                                                              \
-        const int wpb_ = ::mpas::tuned_waves(#kernel);                                                \
+        const int wpb_ = ::mpas::tuned_waves(#kernel) > 4 ? 4 : ::mpas::tuned_waves(#kernel);         \
         const long long nb_ = ::mpas::wave_blocks(n_ent, wpb_);                                       \
         MPAS_LAUNCH(kernel, dim3(::mpas::xcd_grid(nb_)), dim3(64 * wpb_), 0, stream, nb_,             \
                     (long long)(n_ent), (int)(n_lev), m, S_, A);                                      \

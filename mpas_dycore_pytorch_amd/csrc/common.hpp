@@ -83,10 +83,13 @@ __device__ __forceinline__ long long xcd_block(long long nblocks) {
 // Each wavefront walks kEntPerWave consecutive entities (1 measured fastest
 // at level 7 x 55 on MI355X: walking 4 did not raise L1 reuse enough to
 // offset the lost parallelism).
-// The number of wavefronts per block (1, 2 or 4) is a launch-time choice per
-// kernel: kWaves unless the autotuner set another (tuned_waves); the kernels
-// read it back from blockDim.
+// The number of wavefronts per block (1, 2, 4, 8 or 16; the derivative
+// kernels at most 4) is a launch-time choice per kernel: kWaves unless the
+// autotuner set another (tuned_waves); the kernels read it back from
+// blockDim.  Larger blocks keep more consecutive (curve-ordered) entities on
+// one CU, so neighbour columns they share are found in its L1.
 constexpr int kWaves = 4;
+constexpr int kMaxWaveBlock = 1024;  // the forward wavefront kernels take up to 16 wavefronts per block
 constexpr int kEntPerWave = 1;
 int tuned_waves(const char *kernel_name);
 inline long long wave_blocks(long long n_entities, int waves = kWaves) {

@@ -58,7 +58,7 @@ __device__ __forceinline__ double divdamp_increment(double coef, const double *_
 // With fold.on, the damping of the previous acoustic substep is applied to
 // ru_p first (same rounding sequence as the stand-alone pass followed by
 // this one).
-__global__ __launch_bounds__(256) void k_acoustic_edges(long long nb, mpas_hip_mesh m, AcousticParams P, const double *__restrict__ rtheta_pp,
+__global__ __launch_bounds__(kMaxWaveBlock) void k_acoustic_edges(long long nb, mpas_hip_mesh m, AcousticParams P, const double *__restrict__ rtheta_pp,
                                  const double *__restrict__ rho_pp, const double *__restrict__ exner,
                                  const hreal *__restrict__ cqu, const hreal *__restrict__ tend_ru,
                                  hreal *__restrict__ ru_p, hreal *__restrict__ ruAvg, DampFold fold) {
@@ -231,7 +231,7 @@ void launch_acoustic(const mpas_hip_mesh &m, const AcousticParams &P, const Acou
 // ---------------------------------------------------------------------------
 // atm_divergence_damping_3d (divergence_damping.py:187-209, coefficient :267)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_divdamp(long long nb, mpas_hip_mesh m, double coef, const double *__restrict__ theta_m,
+__global__ __launch_bounds__(kMaxWaveBlock) void k_divdamp(long long nb, mpas_hip_mesh m, double coef, const double *__restrict__ theta_m,
                           const double *__restrict__ rtheta_pp, const double *__restrict__ rtheta_pp_old,
                           hreal *__restrict__ ru_p) {
     auto body_ = [&](const int e) {
@@ -293,7 +293,7 @@ __global__ void k_recover_cells(mpas_hip_mesh m, int rk_step, double dt, double 
     }
 }
 
-__global__ __launch_bounds__(256) void k_recover_edges(long long nb, mpas_hip_mesh m, double invNs, const hreal *__restrict__ ru_save,
+__global__ __launch_bounds__(kMaxWaveBlock) void k_recover_edges(long long nb, mpas_hip_mesh m, double invNs, const hreal *__restrict__ ru_save,
                                 const hreal *__restrict__ ru_p, const double *__restrict__ rho_zz,
                                 hreal *__restrict__ ruAvg, hreal *__restrict__ ru, hreal *__restrict__ u,
                                 DampFold fold) {
@@ -316,7 +316,7 @@ __global__ __launch_bounds__(256) void k_recover_edges(long long nb, mpas_hip_me
     for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
-__global__ __launch_bounds__(256) void k_recover_w(long long nb, mpas_hip_mesh m, const hreal *__restrict__ ru, const double *__restrict__ rw,
+__global__ __launch_bounds__(kMaxWaveBlock) void k_recover_w(long long nb, mpas_hip_mesh m, const hreal *__restrict__ ru, const double *__restrict__ rw,
                             const double *__restrict__ rho_zz, double *__restrict__ w) {
     auto body_ = [&](const int c) {
         const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;

@@ -14,7 +14,7 @@ size_t diagnostics_scratch_doubles(const mpas_hip_mesh &m) {
     return 2 * slice_doubles<hreal>((size_t)m.nVertices * L) + slice_doubles<double>((size_t)m.nCells * L);
 }
 
-__global__ __launch_bounds__(256) void k_diag_vertices(long long nb, mpas_hip_mesh m, const hreal *__restrict__ u,
+__global__ __launch_bounds__(kMaxWaveBlock) void k_diag_vertices(long long nb, mpas_hip_mesh m, const hreal *__restrict__ u,
                                 hreal *__restrict__ vort,
                                 hreal *__restrict__ ke_vertex, hreal *__restrict__ pv_vertex) {
     auto body_ = [&](const int vx) {
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(256) void k_diag_vertices(long long nb, mpas_hip_me
     for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
-__global__ __launch_bounds__(256) void k_diag_cells(long long nb, mpas_hip_mesh m, const hreal *__restrict__ u,
+__global__ __launch_bounds__(kMaxWaveBlock) void k_diag_cells(long long nb, mpas_hip_mesh m, const hreal *__restrict__ u,
                              const hreal *__restrict__ ke_vertex, const hreal *__restrict__ pv_vertex,
                              double *__restrict__ div, double *__restrict__ ke, double *__restrict__ pv_cell) {
     auto body_ = [&](const int c) {
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void k_diag_cells(long long nb, mpas_hip_mesh 
     for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
-__global__ __launch_bounds__(256) void k_diag_edges2(long long nb, mpas_hip_mesh m, double apvm_dt, int apvm_on, int reconstruct_v,
+__global__ __launch_bounds__(kMaxWaveBlock) void k_diag_edges2(long long nb, mpas_hip_mesh m, double apvm_dt, int apvm_on, int reconstruct_v,
                               const hreal *__restrict__ u, const hreal *__restrict__ v_in,
                               const hreal *__restrict__ pv_vertex, const double *__restrict__ pv_cell,
                               const double *__restrict__ h, hreal *__restrict__ h_edge,

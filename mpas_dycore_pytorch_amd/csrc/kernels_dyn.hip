@@ -50,7 +50,7 @@ static DynScratch carve(const mpas_hip_mesh &m, double *base) {
 }
 
 // P1: Smagorinsky eddy viscosity (first RK stage only)
-__global__ __launch_bounds__(256) void k_dyn_kdiff(long long nb, mpas_hip_mesh m, DynParams P, const hreal *__restrict__ u,
+__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_kdiff(long long nb, mpas_hip_mesh m, DynParams P, const hreal *__restrict__ u,
                             const hreal *__restrict__ v, double *__restrict__ kdiff) {
     auto body_ = [&](const int c) {
         const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges;
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(256) void k_dyn_kdiff(long long nb, mpas_hip_mesh m
 
 // P2: mass-flux divergence, density tendency, buoyancy dpdz; first stage also
 // the del2 w / theta mixing sums and their del2 fields for the del4 pass.
-__global__ __launch_bounds__(256) void k_dyn_cells_a(long long nb, mpas_hip_mesh m, DynParams P, const hreal *__restrict__ ru,
+__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_a(long long nb, mpas_hip_mesh m, DynParams P, const hreal *__restrict__ ru,
                               const double *__restrict__ rw, const double *__restrict__ rr_save,
                               const double *__restrict__ qtot, const double *__restrict__ w,
                               const double *__restrict__ theta_m, const hreal *__restrict__ rho_edge,
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(256) void k_dyn_cells_a(long long nb, mpas_hip_mesh
 
 // P4: normal-velocity tendency (vertical flux, PV flux, KE gradient,
 // divergence coupling); first stage also PGF and del2 mixing into tend_u_euler.
-__global__ __launch_bounds__(256) void k_dyn_edges_a(long long nb, mpas_hip_mesh m, DynParams P, const hreal *__restrict__ u,
+__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_a(long long nb, mpas_hip_mesh m, DynParams P, const hreal *__restrict__ u,
                               const double *__restrict__ rw, const hreal *__restrict__ pv_edge,
                               const double *__restrict__ ke, const double *__restrict__ h_div,
                               const hreal *__restrict__ rho_edge, const double *__restrict__ pp,
@@ -323,7 +323,7 @@ __global__ void k_dyn_edges_b(mpas_hip_mesh m, DynParams P, int do_del4, const h
 // for rk_step > 1 also the theta perturbation flux
 // dv (ru_save - ru) 0.5 (theta_m_save1 + theta_m_save2) (:5901-6014).  The
 // cell pass applies the edge sign, an exact sign flip of these products.
-__global__ __launch_bounds__(256) void k_dyn_edge_adv(long long nb, mpas_hip_mesh m, DynParams P, const hreal *__restrict__ ru,
+__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edge_adv(long long nb, mpas_hip_mesh m, DynParams P, const hreal *__restrict__ ru,
                                const hreal *__restrict__ ru_save, const double *__restrict__ w,
                                const double *__restrict__ theta_m, const double *__restrict__ theta_m_save,
                                hreal *__restrict__ wflux, hreal *__restrict__ tflux, hreal *__restrict__ tpert) {
@@ -381,7 +381,7 @@ __global__ __launch_bounds__(256) void k_dyn_edge_adv(long long nb, mpas_hip_mes
 
 // P7: w and theta tendencies (advection, mixing finish, PGF/buoyancy,
 // vertical transport, diabatic forcing), one thread per (cell, w-level).
-__global__ __launch_bounds__(256) void k_dyn_cells_b(long long nb, mpas_hip_mesh m, DynParams P, const hreal *__restrict__ wflux,
+__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_b(long long nb, mpas_hip_mesh m, DynParams P, const hreal *__restrict__ wflux,
                               const double *__restrict__ rw, const hreal *__restrict__ tflux,
                               const double *__restrict__ rw_save, const hreal *__restrict__ tpert,
                               const double *__restrict__ w,

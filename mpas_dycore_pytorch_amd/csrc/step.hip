@@ -293,8 +293,9 @@ int mpas_hip_tuning_set(const char *kernel_name, int32_t waves_per_block) {
         set_error("tuning_set: kernel name missing or too long");
         return 1;
     }
-    if (waves_per_block != 0 && waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4) {
-        set_error("tuning_set: waves per block must be 1, 2 or 4 (0 restores the default)");
+    if (waves_per_block != 0 && waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4 &&
+        waves_per_block != 8 && waves_per_block != 16) {
+        set_error("tuning_set: waves per block must be 1, 2, 4, 8 or 16 (0 restores the default)");
         return 1;
     }
     for (int i = 0; i < g_tune_n; ++i)

@@ -152,7 +152,7 @@ def test_launch_shape_table():
     assert lib.mpas_hip_tuning_get(b"k_dyn_edges_a") == 4
     assert lib.mpas_hip_tuning_set(b"k_dyn_edges_a", 2) == 0
     assert lib.mpas_hip_tuning_get(b"k_dyn_edges_a") == 2
-    assert lib.mpas_hip_tuning_set(b"k_dyn_edges_a", 3) != 0  # not 1, 2 or 4
+    assert lib.mpas_hip_tuning_set(b"k_dyn_edges_a", 3) != 0  # not 1, 2, 4, 8 or 16
     assert lib.mpas_hip_tuning_set(b"k_dyn_edges_a", 0) == 0
     assert lib.mpas_hip_tuning_get(b"k_dyn_edges_a") == 4
 
