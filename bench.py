@@ -264,7 +264,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--roofline-kernel", default="k_acoustic_cells")
     ap.add_argument("--cpu-level", type=int, default=5)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--reorder", type=int, default=1, help="1: space-filling-curve mesh order on the device")
     ap.add_argument("--graph", type=int, default=0, help="1: replay the large step from a captured HIP graph")
