@@ -140,9 +140,28 @@ def vjp_leg(m, stepper, config, dt: float, steps: int, fwd_ms_per_step: float, d
         if not bool(torch.isfinite(g).all()):
             raise RuntimeError(f"non-finite gradient of {k}")
     peak = torch.cuda.max_memory_allocated(device) / 2**30
+    # forward mode on the same state: seeded tangents N(0, 0.1) on u and N(0, 0.01) on w (interior
+    # interfaces), padding rows zero, through run_simulation.jvp
+    gen = torch.Generator(device="cpu").manual_seed(20261001)
+    tu = (0.1 * torch.randn(tuple(state["u"].shape), generator=gen, dtype=torch.float64)).to(device)
+    tw = (0.01 * torch.randn(tuple(state["w"].shape), generator=gen, dtype=torch.float64)).to(device)
+    tu[-1] = 0.0
+    tw[-1] = 0.0
+    tw[:, 0] = 0.0
+    tw[:, -1] = 0.0
+    run_simulation.jvp(state, m, {"u": tu, "w": tw}, config=config, dt=dt, nsteps=1, device=device)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tan = run_simulation.jvp(state, m, {"u": tu, "w": tw}, config=config, dt=dt, nsteps=steps, device=device)
+    torch.cuda.synchronize()
+    jvp_s = time.perf_counter() - t0
+    if not bool(torch.isfinite(tan["theta_m"]).all()):
+        raise RuntimeError("non-finite theta_m tangent")
     return {"steps": steps, "fwd_plus_vjp_ms_per_step": round(1e3 * total_s / steps, 2),
             "ratio": round(1e3 * total_s / steps / fwd_ms_per_step, 3), "peak_hbm_gib": round(peak, 1),
-            "checkpoint": "per large step (acoustic substeps recomputed)"}
+            "checkpoint": "per large step (acoustic substeps recomputed)",
+            "fwd_plus_jvp_ms_per_step": round(1e3 * jvp_s / steps, 2),
+            "jvp_ratio": round(1e3 * jvp_s / steps / fwd_ms_per_step, 3)}
 
 
 ENSEMBLE_DT = {4: 1800.0, 5: 900.0, 6: 450.0, 7: 300.0, 8: 150.0}

@@ -262,18 +262,6 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
     }
 }
 
-// column kernels: one thread per column
-__global__ __launch_bounds__(256) void k_adj_vimp_col(mpas_hip_mesh m, Slots S_, KArgs A) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= m.nCells) return;
-    vimp_col_vjp(m, A.dt, A.epssm, ADJ_F, ADJ_B, c);
-}
-__global__ __launch_bounds__(256) void k_adj_acoustic_col(mpas_hip_mesh m, Slots S_, KArgs A) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= m.nCells) return;
-    acoustic_col_vjp(m, A.ap, ADJ_F, ADJ_B, A.col, c);
-}
-
 // ---------------------------------------------------------------------------
 // Column adjoints, one wavefront per column (lane = level or interface).  The
 // level-parallel parts use all lanes; the Thomas recurrences run on lane 0

@@ -160,6 +160,7 @@ def _prepared(level, n_levels, stretch, n_tracers, dt, seed):
 @pytest.mark.parametrize("level,n_levels,stretch,n_tracers,dt,steps,seed", [
     (4, 26, "uniform", 0, 1800.0, 2, 20240),      # the 2-step CPU-runnable configuration
     (4, 55, "quadratic", 1, 900.0, 2, 77),        # 55 stretched levels with a tracer
+    (6, 55, "quadratic", 1, 450.0, 2, 2026),      # the level-6 drop-in configuration (40962 cells), 2 of its 12 steps
 ])
 def test_srk3_matches_oracle(gpu_available, level, n_levels, stretch, n_tracers, dt, steps, seed):
     from oracle import oracle as O
