@@ -110,7 +110,8 @@ def vjp_leg(m, stepper, config, dt: float, steps: int, fwd_ms_per_step: float, d
     Loss = sum over cells of area * sum_k (theta_m - theta_m0)^2 after the
     steps; gradients with respect to u, w, theta_m, rho_zz through the
     hand-written adjoint kernels, one checkpoint per large step, acoustic
-    substeps recomputed in the backward.  The ratio divides the wall time of
+    substeps recomputed in the backward (the last step keeps the tape of its
+    forward).  The ratio divides the wall time of
     forward + VJP by ``steps`` plain forward steps."""
     import torch
 
@@ -159,7 +160,7 @@ def vjp_leg(m, stepper, config, dt: float, steps: int, fwd_ms_per_step: float, d
         raise RuntimeError("non-finite theta_m tangent")
     return {"steps": steps, "fwd_plus_vjp_ms_per_step": round(1e3 * total_s / steps, 2),
             "ratio": round(1e3 * total_s / steps / fwd_ms_per_step, 3), "peak_hbm_gib": round(peak, 1),
-            "checkpoint": "per large step (acoustic substeps recomputed)",
+            "checkpoint": "per large step (acoustic substeps recomputed); the last step keeps its forward tape",
             "fwd_plus_jvp_ms_per_step": round(1e3 * jvp_s / steps, 2),
             "jvp_ratio": round(1e3 * jvp_s / steps / fwd_ms_per_step, 3)}
 

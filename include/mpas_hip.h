@@ -249,9 +249,11 @@ int mpas_hip_moist_coefficients_vjp(const mpas_hip_mesh *m, int32_t nScalars, in
  * (cofrz and cofwr depend on the mesh only; cofwz/coftz/cofwt adjoints required) */
 int mpas_hip_vert_imp_coefs_vjp(const mpas_hip_mesh *m, double dts, double epssm, const double *const *fwd,
                                 double *const *bar, void *stream);
-/* fwd: u, h, v_in | h_edge, v, divergence, vorticity, ke, pv_edge */
+/* fwd: u, h, v_in | h_edge, v, divergence, vorticity, ke, pv_edge.  With inverse tables
+ * (inv non-null) every transpose is a gather; without, atomic scatters. */
 int mpas_hip_solve_diagnostics_vjp(const mpas_hip_mesh *m, double dt, double apvm, int32_t rk_step,
-                                   const double *const *fwd, double *const *bar, double *scratch, void *stream);
+                                   const double *const *fwd, double *const *bar, double *scratch,
+                                   const mpas_hip_inverse_tables *inv, void *stream);
 /* fwd: the 21 inputs of mpas_hip_dyn_tend, tend_rho, tend_u_euler, tend_w_euler, tend_theta_euler, kdiff
  * (before) | the same five (after), tend_u, tend_w, tend_theta, h_divergence, rthdynten
  * (h_divergence and kdiff adjoints required).  With inverse tables (inv non-null)

@@ -103,7 +103,8 @@ SIGNATURES = {
     "mpas_hip_vjp_scratch_doubles": (_I64, [_I32, _I32, _I32, _I32, _I32]),
     "mpas_hip_moist_coefficients_vjp": (C.c_int, [_PM, _I32, _I32, _I32, _PP, _PP, _vp]),
     "mpas_hip_vert_imp_coefs_vjp": (C.c_int, [_PM, _F64, _F64, _PP, _PP, _vp]),
-    "mpas_hip_solve_diagnostics_vjp": (C.c_int, [_PM, _F64, _F64, _I32, _PP, _PP, _dp, _vp]),
+    "mpas_hip_solve_diagnostics_vjp": (C.c_int, [_PM, _F64, _F64, _I32, _PP, _PP, _dp,
+                                                 C.POINTER(InverseTablesStruct), _vp]),
     "mpas_hip_dyn_tend_vjp": (C.c_int, [_PM, C.POINTER(DynParamsStruct), _PP, _PP, _dp,
                                         C.POINTER(InverseTablesStruct), _vp]),
     "mpas_hip_smlstep_pert_vjp": (C.c_int, [_PM, _PP, _PP, _vp]),

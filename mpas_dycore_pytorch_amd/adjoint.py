@@ -292,7 +292,7 @@ def _call_vjp(lib, op: Op, dm: DeviceMesh, fwd, bar, scratch: torch.Tensor, stre
     elif k == "vert_imp_coefs":
         rc = lib.mpas_hip_vert_imp_coefs_vjp(dm.ref, *a, fwd, bar, stream)
     elif k == "solve_diagnostics":
-        rc = lib.mpas_hip_solve_diagnostics_vjp(dm.ref, *a, fwd, bar, sp, stream)
+        rc = lib.mpas_hip_solve_diagnostics_vjp(dm.ref, *a, fwd, bar, sp, C.byref(dm.inverse_tables()), stream)
     elif k == "dyn_tend":
         rc = lib.mpas_hip_dyn_tend_vjp(dm.ref, C.byref(a[0]), fwd, bar, sp, C.byref(dm.inverse_tables()), stream)
     elif k == "smlstep_pert":
@@ -497,6 +497,15 @@ def step_vjp(dm: DeviceMesh, state: Mapping[str, torch.Tensor], cfg: Any, dt: fl
     the step does not depend on the input)."""
     tape = Tape(dm)
     out = taped_step(dm, state, cfg, dt, tape)
+    return out, tape_vjp(tape, state, out, cotangents, scratch)
+
+
+def tape_vjp(tape: Tape, state: Mapping[str, torch.Tensor], out: Mapping[str, torch.Tensor],
+             cotangents: Mapping[str, torch.Tensor], scratch: torch.Tensor | None = None
+             ) -> dict[str, torch.Tensor]:
+    """Adjoints of the step inputs ``state`` from a large step already
+    recorded on ``tape`` (``out`` is what :func:`taped_step` returned)."""
+    dm = tape.dm
     seeds = {}
     for k, g in cotangents.items():
         if g is None or k not in out:
@@ -513,7 +522,7 @@ def step_vjp(dm: DeviceMesh, state: Mapping[str, torch.Tensor], cfg: Any, dt: fl
     for k in STEP_INPUTS:
         b = bars.get(id(state[k]))
         grads[k] = b if b is not None else torch.zeros_like(state[k])
-    return out, grads
+    return grads
 
 
 def step_jvp(dm: DeviceMesh, state: Mapping[str, torch.Tensor], cfg: Any, dt: float,
@@ -526,6 +535,15 @@ def step_jvp(dm: DeviceMesh, state: Mapping[str, torch.Tensor], cfg: Any, dt: fl
     keys that no input tangent reaches get no entry."""
     tape = Tape(dm)
     out = taped_step(dm, state, cfg, dt, tape)
+    return out, tape_jvp(tape, state, out, tangents, scratch)
+
+
+def tape_jvp(tape: Tape, state: Mapping[str, torch.Tensor], out: Mapping[str, torch.Tensor],
+             tangents: Mapping[str, torch.Tensor], scratch: torch.Tensor | None = None
+             ) -> dict[str, torch.Tensor]:
+    """Tangents of the new state ``out`` of a large step already recorded on
+    ``tape`` from ``state``, for input tangents ``tangents``."""
+    dm = tape.dm
     seeds = {}
     for k, t in tangents.items():
         if t is None or k not in STEP_INPUTS:
@@ -540,4 +558,4 @@ def step_jvp(dm: DeviceMesh, state: Mapping[str, torch.Tensor], cfg: Any, dt: fl
     for k, v in out.items():
         if isinstance(v, torch.Tensor) and id(v) in dots:
             res[k] = dots[id(v)]
-    return out, res
+    return res

@@ -28,7 +28,8 @@ import torch
 from . import kernels as K
 from .device import DeviceMesh, require_gpu, to_numpy
 from .errors import MpasConfigError, MpasMissingFieldError, MpasUnsupportedCapabilityError
-from .time_integration import Srk3Stepper, atm_srk3
+from .adjoint import STEP_INPUTS
+from .time_integration import Srk3Stepper, _atm_srk3_differentiable, atm_srk3
 
 # keys run_simulation.vjp differentiates by default (atmosphere_model.py:157-168)
 INTER_STEP_PROGNOSTIC_KEYS = ("u", "w", "theta_m", "rho_zz", "scalars", "rt_diabatic_tend")
@@ -116,12 +117,15 @@ def run_simulation(initial_state: dict[str, Any], mesh: dict[str, Any], config: 
     With ``differentiable=True`` every large step is one autograd node that
     keeps its input state as the checkpoint and recomputes the step, acoustic
     substeps included, in the backward (the reference's default outer
-    timestep checkpointing, ``autodiff/checkpoint.py:106``)."""
+    timestep checkpointing, ``autodiff/checkpoint.py:106``).  The last step
+    keeps the kernel tape of its forward instead, since its backward comes
+    first; ``checkpoint_timesteps=False`` keeps every step's tape (no
+    recompute, memory grows with ``nsteps``; ``atmosphere_model.py:1504-1509``)."""
     if isinstance(nsteps, bool) or not isinstance(nsteps, int) or nsteps < 0:
         raise MpasConfigError("nsteps must be a non-negative integer")
     if differentiable:
         return _run_differentiable(initial_state, mesh, config, _dt(config, dt), nsteps, torch.device(device),
-                                   on_step)
+                                   on_step, keep_all=checkpoint_timesteps is False)
     dt = _dt(config, dt)
     gpu = require_gpu(None)
     out_dev = torch.device(device)
@@ -147,7 +151,8 @@ def run_simulation(initial_state: dict[str, Any], mesh: dict[str, Any], config: 
 
 def _run_differentiable(state: Mapping[str, Any], mesh: Mapping[str, Any], config: Mapping[str, Any] | None,
                         dt: float, nsteps: int, device: torch.device,
-                        on_step: Callable[[int, dict[str, Any]], None] | None) -> dict[str, Any]:
+                        on_step: Callable[[int, dict[str, Any]], None] | None,
+                        keep_all: bool = False) -> dict[str, Any]:
     if not state.get(STEP_BOUNDARY_KEY, False):
         raise MpasUnsupportedCapabilityError(
             "differentiable runs start from a prepared step-boundary state (run_simulation(..., nsteps=0) "
@@ -155,7 +160,11 @@ def _run_differentiable(state: Mapping[str, Any], mesh: Mapping[str, Any], confi
             requested=True)
     cur = dict(state)
     for step in range(1, nsteps + 1):
-        cur = atm_srk3(cur, mesh, config, dt, differentiable=True)
+        # a tape is worth keeping only if a backward can reach this step
+        graph = torch.is_grad_enabled() and any(
+            isinstance(v, torch.Tensor) and v.requires_grad for k, v in cur.items() if k in STEP_INPUTS)
+        keep = graph and (keep_all or step == nsteps)
+        cur = _atm_srk3_differentiable(cur, mesh, config, dt, strict=True, keep_tape=keep)
         cur[STEP_BOUNDARY_KEY] = True
         if on_step is not None:
             on_step(step, cur)
@@ -203,8 +212,6 @@ def _run_simulation_jvp(state: dict[str, Any], mesh: dict[str, Any], perturbatio
     device; other entries are constants.  Returns the tangents of the output
     state's tensors."""
     from torch.autograd import forward_ad
-
-    from .adjoint import STEP_INPUTS
 
     kwargs.setdefault("differentiable", True)
     with forward_ad.dual_level():

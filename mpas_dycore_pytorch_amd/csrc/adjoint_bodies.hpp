@@ -39,6 +39,14 @@ __device__ __forceinline__ void add_own(Bar B, int slot, long long i, double v) 
 }
 __device__ __forceinline__ double bar_of(Bar B, int slot, long long i) { return B[slot] ? B[slot][i] : 0.0; }
 
+// sign of edge e in the edge list of cell c
+__device__ __forceinline__ double edge_sign_in_cell(const mpas_hip_mesh &m, int c, int e) {
+    const int n = m.nEdgesOnCell[c];
+    for (int j = 0; j < n; ++j)
+        if (m.edgesOnCell[c * m.maxEdges + j] == e) return m.edgesOnCell_sign[c * m.maxEdges + j];
+    return 0.0;
+}
+
 // d|x|/dx as torch.abs differentiates it
 __device__ __forceinline__ double sgn0(double x) { return (x > 0.0) ? 1.0 : ((x < 0.0) ? -1.0 : 0.0); }
 
@@ -485,6 +493,148 @@ __device__ __forceinline__ void diag_edge1_vjp(const mpas_hip_mesh &m, Fwd F, Ba
     add(B, h, (long long)c1 * L + k, gh);
     add(B, h, (long long)c2 * L + k, gh);
     add(B, u, i, (m.dcEdge[e] * m.dvEdge[e]) * 2.0 * F[u][i] * S.g_kee[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Gather form of the diagnostics adjoint (no atomics), given the inverse
+// edgesOnEdge table.  Each target sums its sources; the per-edge apvm terms
+// are recomputed where they are consumed, from the output adjoints, u, v and
+// vorticity.
+//   G1 edges:    adjoint of v (own + apvm tangential-gradient term), kept in
+//                g_kee, or added to v_in when v was not reconstructed
+//   G2 cells:    the pv_cell adjoint (kept in g_pvc) and h
+//   G3 vertices: vorticity / pv_vertex adjoint (kept in g_pvv) and the
+//                vertex-KE adjoint (kept in g_kev)
+//   G4 edges:    u from its own apvm term, the v transpose over the inverse
+//                table, the cells' divergence / KE and the vertices' terms
+// ---------------------------------------------------------------------------
+// t = apvm dt (-g_pv_edge) of the edge row i (0 without apvm)
+__device__ __forceinline__ double diag_t(double apvm_dt, bool apvm_on, Bar B, long long i) {
+    using namespace diag;
+    const double gpe = bar_of(B, pv_edge, i);
+    return (apvm_on && gpe != 0.0) ? apvm_dt * (-gpe) : 0.0;
+}
+__device__ __forceinline__ void diag_g1(const mpas_hip_mesh &m, double dt, double apvm, int rk_step, Fwd F, Bar B,
+                                        const DiagScratch &S, int e, int k) {
+    using namespace diag;
+    const int L = m.nLevels;
+    if (e >= m.nEdges || k >= L) return;
+    const long long i = (long long)e * L + k;
+    double gv = bar_of(B, v, i);
+    const double t = diag_t(apvm * dt, apvm > 0.0, B, i);
+    if (t != 0.0) {
+        const int v1 = m.verticesOnEdge[2 * e], v2 = m.verticesOnEdge[2 * e + 1];
+        const double p1 = m.fVertex[v1] + F[vorticity][(long long)v1 * L + k];
+        const double p2 = m.fVertex[v2] + F[vorticity][(long long)v2 * L + k];
+        gv += t * ((p2 - p1) * m.invDvEdge[e]);
+    }
+    if (rk_step <= 0 || rk_step == 3) S.g_kee[i] = gv;
+    else add_own(B, v_in, i, gv);
+}
+__device__ __forceinline__ void diag_g2(const mpas_hip_mesh &m, double dt, double apvm, Fwd F, Bar B,
+                                        const DiagScratch &S, int c, int k) {
+    using namespace diag;
+    const int L = m.nLevels, ME = m.maxEdges;
+    if (c >= m.nCells || k >= L) return;
+    const double apvm_dt = apvm * dt;
+    const bool on = apvm > 0.0;
+    const int n = m.nEdgesOnCell[c];
+    double gpc = 0.0, gh = 0.0;
+    for (int j = 0; j < n; ++j) {
+        const int e = m.edgesOnCell[c * ME + j];
+        const long long ie = (long long)e * L + k;
+        const double t = diag_t(apvm_dt, on, B, ie);
+        if (t != 0.0) {
+            const double ggn = t * F[u][ie] * m.invDcEdge[e];
+            gpc += (m.cellsOnEdge[2 * e + 1] == c) ? ggn : -ggn;
+        }
+        gh += 0.5 * bar_of(B, h_edge, ie);
+    }
+    const long long o = (long long)c * L + k;
+    S.g_pvc[o] = gpc;
+    add_own(B, h, o, gh);
+}
+__device__ __forceinline__ void diag_g3(const mpas_hip_mesh &m, double dt, double apvm, Fwd F, Bar B,
+                                        const DiagScratch &S, int vx, int k) {
+    using namespace diag;
+    const int L = m.nLevels, ME = m.maxEdges;
+    if (vx >= m.nVertices || k >= L) return;
+    const double apvm_dt = apvm * dt;
+    const bool on = apvm > 0.0;
+    const long long o = (long long)vx * L + k;
+    const double ke_fact = 1.0 - 0.375;
+    double gz = bar_of(B, vorticity, o), gkv = 0.0;
+    int seen0 = -1, seen1 = -1, seen2 = -1;
+    for (int j = 0; j < 3; ++j) {
+        const int e = m.edgesOnVertex[3 * vx + j];
+        const long long ie = (long long)e * L + k;
+        gz += 0.5 * bar_of(B, pv_edge, ie);
+        const double t = diag_t(apvm_dt, on, B, ie);
+        if (t != 0.0) {
+            const double ggt = t * F[v][ie] * m.invDvEdge[e];
+            gz += (m.verticesOnEdge[2 * e + 1] == vx) ? ggt : -ggt;
+        }
+        // the kite terms of the cells around the vertex, each cell once (the
+        // three edges name each of the three cells twice)
+        for (int q = 0; q < 2; ++q) {
+            const int c = m.cellsOnEdge[2 * e + q];
+            if (c == seen0 || c == seen1 || c == seen2) continue;
+            if (seen0 < 0) seen0 = c;
+            else if (seen1 < 0) seen1 = c;
+            else seen2 = c;
+            const int n = m.nEdgesOnCell[c];
+            for (int i = 0; i < n; ++i) {
+                if (m.verticesOnCell[c * ME + i] != vx) continue;
+                const double ka = m.kiteAreasOnVertex[3 * vx + m.kiteForCell[c * ME + i]];
+                const long long oc = (long long)c * L + k;
+                const double ia = m.invAreaCell[c];
+                gz += ka * ia * S.g_pvc[oc];
+                gkv += (1.0 - ke_fact) * ka * ia * bar_of(B, ke, oc);
+                break;
+            }
+        }
+    }
+    S.g_pvv[o] = gz;
+    S.g_kev[o] = gkv;
+}
+__device__ __forceinline__ void diag_g4(const mpas_hip_mesh &m, double dt, double apvm, int rk_step, Fwd F, Bar B,
+                                        const DiagScratch &S, const mpas_hip_inverse_tables &inv, int e, int k) {
+    using namespace diag;
+    const int L = m.nLevels, ME2 = m.maxEdges2;
+    if (e >= m.nEdges || k >= L) return;
+    const long long i = (long long)e * L + k;
+    const double ke_fact = 1.0 - 0.375;
+    double gu = 0.0, gkee = 0.0;
+    const double t = diag_t(apvm * dt, apvm > 0.0, B, i);
+    if (t != 0.0) {
+        const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+        const double pc1 = diag_pv_cell(m, F, c1, k), pc2 = diag_pv_cell(m, F, c2, k);
+        gu += t * ((pc2 - pc1) * m.invDcEdge[e]);
+    }
+    if (rk_step <= 0 || rk_step == 3) {
+        const int j1 = inv.eoeStart[e + 1];
+        for (int j = inv.eoeStart[e]; j < j1; ++j) {
+            const int eo = inv.eoeEdge[j];
+            gu += m.weightsOnEdge[eo * ME2 + inv.eoeSlot[j]] * S.g_kee[(long long)eo * L + k];
+        }
+    }
+    for (int q = 0; q < 2; ++q) {
+        const int c = m.cellsOnEdge[2 * e + q];
+        const long long oc = (long long)c * L + k;
+        const double ia = m.invAreaCell[c];
+        gu += ia * edge_sign_in_cell(m, c, e) * m.dvEdge[e] * bar_of(B, divergence, oc);
+        gkee += 0.25 * ia * ke_fact * bar_of(B, ke, oc);
+        const int vx = m.verticesOnEdge[2 * e + q];
+        const long long ov = (long long)vx * L + k;
+        const double iat = m.invAreaTriangle[vx];
+        double sv = 0.0;
+        for (int j = 0; j < 3; ++j)
+            if (m.edgesOnVertex[3 * vx + j] == e) sv = m.edgesOnVertex_sign[3 * vx + j];
+        gu += iat * sv * m.dcEdge[e] * S.g_pvv[ov];
+        gkee += 0.25 * iat * S.g_kev[ov];
+    }
+    gu += (m.dcEdge[e] * m.dvEdge[e]) * 2.0 * F[u][i] * gkee;
+    add_own(B, u, i, gu);
 }
 
 // ===========================================================================

@@ -38,14 +38,6 @@ struct DynAdjScratch {
     double *g_tf, *g_wf, *g_q;
 };
 
-// sign of edge e in the edge list of cell c
-__device__ __forceinline__ double edge_sign_in_cell(const mpas_hip_mesh &m, int c, int e) {
-    const int n = m.nEdgesOnCell[c];
-    for (int j = 0; j < n; ++j)
-        if (m.edgesOnCell[c * m.maxEdges + j] == e) return m.edgesOnCell_sign[c * m.maxEdges + j];
-    return 0.0;
-}
-
 // ---------------------------------------------------------------- F0
 __device__ __forceinline__ void dyn_fwd_cells(const mpas_hip_mesh &m, Fwd F, const DynAdjScratch &S, int c, int k) {
     using namespace dyn;
@@ -303,6 +295,101 @@ __device__ __forceinline__ void dyn_a2(const mpas_hip_mesh &m, const mpas_hip_dy
             add(B, w, (long long)c1 * Lw + k, -es * (r0 + r1) * gbf);
         }
     }
+}
+
+// Gather form of A2 (no atomics).  The per-edge sums over the edge's two
+// cells s (sign sg_s of the edge in s, area factor ia_s):
+//   theta at level k:   A  = sum_s es_s gbf_s,   K  = sum_s bf_s 0.5 prs gtte_s
+//   w at interface k:   Aw = sum_s esw_s gbfw_s, Kw = sum_s bfw_s ms2 0.25 gtwe_s
+// are recomputed by the edge pass (rho_edge) and by the cell pass (kdiff,
+// theta_m, w) from the forward values and the output adjoints.
+struct A2Edge {
+    double A, K;    // theta at level k
+    double Aw, Kw;  // w at interface k (0 outside 1..L-1)
+};
+__device__ __forceinline__ A2Edge dyn_a2_edge(const mpas_hip_mesh &m, const mpas_hip_dyn_params &P, Fwd F, Bar B,
+                                              const DynAdjScratch &S, int e, double sg1, double sg2, int k,
+                                              bool theta) {
+    using namespace dyn;
+    const int L = m.nLevels, Lw = L + 1;
+    const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+    const double *kd = F[kdiff];
+    const double ia1 = m.invAreaCell[c1], ia2 = m.invAreaCell[c2];
+    A2Edge r{0.0, 0.0, 0.0, 0.0};
+    if (theta) {
+        const long long o1 = (long long)c1 * L + k, o2 = (long long)c2 * L + k;
+        const double prs = P.prandtl_inv * m.meshScalingDel2[e];
+        const double dt21 = F[theta_m][o2] - F[theta_m][o1];
+        const double re = F[rho_edge][(long long)e * L + k];
+        const double kk = 0.5 * (kd[o1] + kd[o2]);
+        const double es1 = ia1 * sg1 * m.dvEdge[e] * m.invDcEdge[e];
+        const double es2 = ia2 * sg2 * m.dvEdge[e] * m.invDcEdge[e];
+        const double g1 = bar_of(B, tend_theta_euler, o1) + bar_of(B, tend_theta, o1);
+        const double g2 = bar_of(B, tend_theta_euler, o2) + bar_of(B, tend_theta, o2);
+        r.A = es1 * (kk * prs * g1 + S.g_delsq_theta[o1]) + es2 * (kk * prs * g2 + S.g_delsq_theta[o2]);
+        r.K = (es1 * dt21 * re) * 0.5 * prs * g1 + (es2 * dt21 * re) * 0.5 * prs * g2;
+    }
+    if (k >= 1 && k < L) {
+        const long long w1i = (long long)c1 * Lw + k, w2i = (long long)c2 * Lw + k;
+        const double rr = F[rho_edge][(long long)e * L + k] + F[rho_edge][(long long)e * L + k - 1];
+        const double dw = F[w][w2i] - F[w][w1i];
+        const double ksum = kd[(long long)c1 * L + k] + kd[(long long)c2 * L + k] + kd[(long long)c1 * L + k - 1] +
+                            kd[(long long)c2 * L + k - 1];
+        const double ms2 = m.meshScalingDel2[e];
+        const double es1 = 0.5 * ia1 * sg1 * m.dvEdge[e] * m.invDcEdge[e];
+        const double es2 = 0.5 * ia2 * sg2 * m.dvEdge[e] * m.invDcEdge[e];
+        const double g1 = bar_of(B, tend_w_euler, w1i) + bar_of(B, tend_w, w1i);
+        const double g2 = bar_of(B, tend_w_euler, w2i) + bar_of(B, tend_w, w2i);
+        r.Aw = es1 * (ms2 * 0.25 * ksum * g1 + S.g_delsq_w[w1i]) + es2 * (ms2 * 0.25 * ksum * g2 + S.g_delsq_w[w2i]);
+        r.Kw = (es1 * rr * dw) * ms2 * 0.25 * g1 + (es2 * rr * dw) * ms2 * 0.25 * g2;
+    }
+    return r;
+}
+// A2 gather, thread (edge e, level k): rho_edge
+__device__ __forceinline__ void dyn_a2_edges(const mpas_hip_mesh &m, const mpas_hip_dyn_params &P, Fwd F, Bar B,
+                                             const DynAdjScratch &S, int e, int k) {
+    using namespace dyn;
+    const int L = m.nLevels, Lw = L + 1;
+    if (e >= m.nEdges || k >= L) return;
+    const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+    const double sg1 = edge_sign_in_cell(m, c1, e), sg2 = edge_sign_in_cell(m, c2, e);
+    const A2Edge a = dyn_a2_edge(m, P, F, B, S, e, sg1, sg2, k, true);
+    double g = (F[theta_m][(long long)c2 * L + k] - F[theta_m][(long long)c1 * L + k]) * a.A;
+    if (k >= 1) g += (F[w][(long long)c2 * Lw + k] - F[w][(long long)c1 * Lw + k]) * a.Aw;
+    if (k + 1 < L) {
+        const A2Edge b = dyn_a2_edge(m, P, F, B, S, e, sg1, sg2, k + 1, false);
+        g += (F[w][(long long)c2 * Lw + k + 1] - F[w][(long long)c1 * Lw + k + 1]) * b.Aw;
+    }
+    add_own(B, rho_edge, (long long)e * L + k, g);
+}
+// A2 gather, thread (cell c, level k): kdiff, theta_m and w (interface k)
+__device__ __forceinline__ void dyn_a2_cells(const mpas_hip_mesh &m, const mpas_hip_dyn_params &P, Fwd F, Bar B,
+                                             const DynAdjScratch &S, int c, int k) {
+    using namespace dyn;
+    const int L = m.nLevels, Lw = L + 1, ME = m.maxEdges;
+    if (c >= m.nCells || k >= L) return;
+    const int n = m.nEdgesOnCell[c];
+    double gk = 0.0, gt = 0.0, gw = 0.0;
+    for (int j = 0; j < n; ++j) {
+        const int e = m.edgesOnCell[c * ME + j];
+        const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+        const double sc = m.edgesOnCell_sign[c * ME + j];
+        const double sg1 = (c1 == c) ? sc : edge_sign_in_cell(m, c1, e);
+        const double sg2 = (c2 == c) ? sc : edge_sign_in_cell(m, c2, e);
+        const double side = (c2 == c) ? 1.0 : -1.0;
+        const A2Edge a = dyn_a2_edge(m, P, F, B, S, e, sg1, sg2, k, true);
+        gk += a.K;
+        gt += side * F[rho_edge][(long long)e * L + k] * a.A;
+        if (k >= 1) {
+            gk += a.Kw;
+            gw += side * (F[rho_edge][(long long)e * L + k] + F[rho_edge][(long long)e * L + k - 1]) * a.Aw;
+        }
+        if (k + 1 < L) gk += dyn_a2_edge(m, P, F, B, S, e, sg1, sg2, k + 1, false).Kw;
+    }
+    const long long o = (long long)c * L + k;
+    add_own(B, kdiff, o, gk);
+    add_own(B, theta_m, o, gt);
+    if (k >= 1) add_own(B, w, (long long)c * Lw + k, gw);
 }
 
 // ---------------------------------------------------------------- A3

@@ -27,12 +27,18 @@ ADJ_WAVE_KERNEL(k_adj_diag_edge2, diag_edge2_vjp(m, A.dt, A.apvm, A.rk, ADJ_F, A
 ADJ_WAVE_KERNEL(k_adj_diag_cell, diag_cell_vjp(m, ADJ_F, ADJ_B, A.ds, ent, k))
 ADJ_WAVE_KERNEL(k_adj_diag_vertex, diag_vertex_vjp(m, ADJ_F, ADJ_B, A.ds, ent, k))
 ADJ_WAVE_KERNEL(k_adj_diag_edge1, diag_edge1_vjp(m, ADJ_F, ADJ_B, A.ds, ent, k))
+ADJ_WAVE_KERNEL(k_adj_diag_g1, diag_g1(m, A.dt, A.apvm, A.rk, ADJ_F, ADJ_B, A.ds, ent, k))
+ADJ_WAVE_KERNEL(k_adj_diag_g2, diag_g2(m, A.dt, A.apvm, ADJ_F, ADJ_B, A.ds, ent, k))
+ADJ_WAVE_KERNEL(k_adj_diag_g3, diag_g3(m, A.dt, A.apvm, ADJ_F, ADJ_B, A.ds, ent, k))
+ADJ_WAVE_KERNEL(k_adj_diag_g4, diag_g4(m, A.dt, A.apvm, A.rk, ADJ_F, ADJ_B, A.ds, A.inv, ent, k))
 ADJ_WAVE_KERNEL(k_adj_acoustic_edge, acoustic_edge_vjp(m, A.ap, ADJ_F, ADJ_B, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_fwd_cells, dyn_fwd_cells(m, ADJ_F, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_fwd_edges, dyn_fwd_edges(m, ADJ_F, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_fwd_del4, dyn_fwd_del4(m, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_a1, dyn_a1(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_a2, dyn_a2(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
+ADJ_WAVE_KERNEL(k_adj_dyn_a2_edges, dyn_a2_edges(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
+ADJ_WAVE_KERNEL(k_adj_dyn_a2_cells, dyn_a2_cells(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_a3, dyn_a3(m, A.dp, ADJ_F, ADJ_B, A.dy, A.gather != 0, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_a3_cells, dyn_a3_cells(m, ADJ_F, ADJ_B, A.dy, A.inv, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_a3_edges, dyn_a3_edges(m, ADJ_F, ADJ_B, A.dy, A.inv, ent, k))
@@ -752,7 +758,8 @@ int mpas_hip_recover_large_step_vjp(const mpas_hip_mesh *mesh, int32_t rk_step, 
 }
 
 int mpas_hip_solve_diagnostics_vjp(const mpas_hip_mesh *mesh, double dt, double apvm, int32_t rk_step,
-                                   const double *const *fwd, double *const *bar, double *scratch, void *stream) {
+                                   const double *const *fwd, double *const *bar, double *scratch,
+                                   const mpas_hip_inverse_tables *inv, void *stream) {
     if (vjp_check(mesh, "solve_diagnostics_vjp")) return 1;
     const mpas_hip_mesh &m = *mesh;
     hipStream_t s = STREAM(stream);
@@ -763,6 +770,15 @@ int mpas_hip_solve_diagnostics_vjp(const mpas_hip_mesh *mesh, double dt, double 
     A.apvm = apvm;
     A.rk = rk_step;
     A.ds = DiagScratch{scratch, scratch + V, scratch + 2 * V, scratch + 2 * V + C};
+    if (inv) {
+        // gather form: every live row of the scratch is written before it is read
+        A.inv = *inv;
+        ADJ_LAUNCH(k_adj_diag_g1, m.nEdges, m.nLevels, s);
+        ADJ_LAUNCH(k_adj_diag_g2, m.nCells, m.nLevels, s);
+        ADJ_LAUNCH(k_adj_diag_g3, m.nVertices, m.nLevels, s);
+        ADJ_LAUNCH(k_adj_diag_g4, m.nEdges, m.nLevels, s);
+        return check_launch("solve_diagnostics_vjp");
+    }
     fill_zero(scratch, 2 * V + C + E, s);
     ADJ_LAUNCH(k_adj_diag_edge2, m.nEdges, m.nLevels, s);
     ADJ_LAUNCH(k_adj_diag_cell, m.nCells, m.nLevels, s);
@@ -826,7 +842,12 @@ int mpas_hip_dyn_tend_vjp(const mpas_hip_mesh *mesh, const mpas_hip_dyn_params *
         ADJ_LAUNCH(k_adj_dyn_fwd_del4, ncv, L, s);
     }
     ADJ_LAUNCH(k_adj_dyn_a1, nC, L + 1, s);
-    if (rk1) ADJ_LAUNCH(k_adj_dyn_a2, nC, L + 1, s);
+    if (rk1 && inv) {
+        ADJ_LAUNCH(k_adj_dyn_a2_edges, nE, L, s);
+        ADJ_LAUNCH(k_adj_dyn_a2_cells, nC, L, s);
+    } else if (rk1) {
+        ADJ_LAUNCH(k_adj_dyn_a2, nC, L + 1, s);
+    }
     if (inv) {
         ADJ_LAUNCH(k_adj_dyn_a3w, nE, L, s);
         ADJ_LAUNCH(k_adj_dyn_a3_cells, nC, L, s);

@@ -298,16 +298,22 @@ class Srk3Stepper:
 class _Srk3Step(torch.autograd.Function):
     """One large step as a single autograd node, in both modes.
 
-    The forward runs the fused step and keeps only the step's input state
-    (the per-large-step checkpoint).  The backward recomputes the step from
-    it on a tape of per-function kernels, acoustic substeps included, and
-    applies the hand-written adjoint kernels (``adjoint.step_vjp``); under
-    ``torch.autograd.forward_ad`` the jvp runs the same tape forward through
-    the tangent-linear kernels (``adjoint.step_jvp``)."""
+    By default the forward runs the fused step and keeps only the step's
+    input state (the per-large-step checkpoint); the backward recomputes the
+    step from it on a tape of per-function kernels, acoustic substeps
+    included, and applies the hand-written adjoint kernels
+    (``adjoint.step_vjp``); under ``torch.autograd.forward_ad`` the jvp runs
+    the same tape forward through the tangent-linear kernels
+    (``adjoint.step_jvp``).
+
+    With ``box["keep_tape"]`` the forward runs the step on the tape instead
+    (bit for bit the fused step's results) and keeps the tape until the
+    first backward or jvp uses it, which then skips the recompute; a later
+    use (``retain_graph``) recomputes."""
 
     @staticmethod
     def forward(box: dict, *inputs: torch.Tensor):
-        from .adjoint import STEP_INPUTS
+        from .adjoint import STEP_INPUTS, Tape, taped_step
 
         state = dict(box["state"])
         state.update({k: v.detach() for k, v in zip(box["in_keys"], inputs)})
@@ -316,16 +322,26 @@ class _Srk3Step(torch.autograd.Function):
         gpu = out_device if out_device.type == "cuda" else None
         stepper = Srk3Stepper(state, box["mesh"], box["config"], box["dt"], device=gpu, strict=box["strict"])
         ckpt = {k: stepper.t[k].clone() for k in STEP_INPUTS}
-        stepper.step(1)
-        new = stepper.state_dict()
-        new["h_edge"] = new["h_edge"].clone()
-        out_keys = sorted(new)
-        box["out_keys"] = out_keys
         cfg = N.ConfigStruct()
         C.pointer(cfg)[0] = stepper.cfg
         cfg.diagnostics_current = 0
+        kept = None
+        if box.get("keep_tape", False):
+            tape = Tape(stepper.mesh)
+            with torch.cuda.device(stepper.device):
+                taped = taped_step(stepper.mesh, ckpt, cfg, float(box["dt"]), tape)
+            # the caller gets copies: the tape's tensors stay private to the tape
+            new = {k: taped[k].clone() for k in N.STATE_FIELDS}
+            new["h_edge"] = taped["rho_edge"].clone()
+            kept = (tape, taped)
+        else:
+            stepper.step(1)
+            new = stepper.state_dict()
+            new["h_edge"] = new["h_edge"].clone()
+        out_keys = sorted(new)
+        box["out_keys"] = out_keys
         box["_ctx"] = (stepper.mesh, cfg, float(box["dt"]), ckpt, list(box["in_keys"]), out_keys,
-                       [x.device for x in inputs])
+                       [x.device for x in inputs], kept)
         if out_device.type != "cuda":
             torch.cuda.synchronize(stepper.device)
         return tuple(new[k].to(out_device) for k in out_keys)
@@ -333,27 +349,41 @@ class _Srk3Step(torch.autograd.Function):
     @staticmethod
     def setup_context(ctx, inputs, output):
         box = inputs[0]
-        ctx.dm, ctx.cfg, ctx.dt, ctx.ckpt, ctx.in_keys, ctx.out_keys, ctx.in_devices = box.pop("_ctx")
+        ctx.dm, ctx.cfg, ctx.dt, ctx.ckpt, ctx.in_keys, ctx.out_keys, ctx.in_devices, ctx.kept = box.pop("_ctx")
         ctx.out_like = [(o.shape, o.dtype, o.device) for o in output]
 
     @staticmethod
+    def _take_tape(ctx):
+        """(tape, taped outputs) recorded by the forward, handed out once."""
+        kept, ctx.kept = ctx.kept, None
+        return kept
+
+    @staticmethod
     def backward(ctx, *grads: torch.Tensor):
-        from .adjoint import step_vjp
+        from .adjoint import step_vjp, tape_vjp
 
         dev = ctx.dm.device
         cot = {k: g.to(device=dev, dtype=torch.float64) for k, g in zip(ctx.out_keys, grads) if g is not None}
+        kept = _Srk3Step._take_tape(ctx)
         with torch.cuda.device(dev):
-            _, gin = step_vjp(ctx.dm, ctx.ckpt, ctx.cfg, ctx.dt, cot)
+            if kept is not None:
+                gin = tape_vjp(kept[0], ctx.ckpt, kept[1], cot)
+            else:
+                _, gin = step_vjp(ctx.dm, ctx.ckpt, ctx.cfg, ctx.dt, cot)
         return (None,) + tuple(gin[k].to(d) for k, d in zip(ctx.in_keys, ctx.in_devices))
 
     @staticmethod
     def jvp(ctx, _box_tangent, *tangents: torch.Tensor):
-        from .adjoint import step_jvp
+        from .adjoint import step_jvp, tape_jvp
 
         dev = ctx.dm.device
         tan = {k: t.to(device=dev, dtype=torch.float64) for k, t in zip(ctx.in_keys, tangents) if t is not None}
+        kept = _Srk3Step._take_tape(ctx)
         with torch.cuda.device(dev):
-            _, tout = step_jvp(ctx.dm, ctx.ckpt, ctx.cfg, ctx.dt, tan)
+            if kept is not None:
+                tout = tape_jvp(kept[0], ctx.ckpt, kept[1], tan)
+            else:
+                _, tout = step_jvp(ctx.dm, ctx.ckpt, ctx.cfg, ctx.dt, tan)
         res = []
         for k, (shape, dtype, device) in zip(ctx.out_keys, ctx.out_like):
             t = tout.get(k)
@@ -363,12 +393,19 @@ class _Srk3Step(torch.autograd.Function):
 
 
 def _atm_srk3_differentiable(state: Mapping[str, Any], mesh: Mapping[str, Any], config: Mapping[str, Any],
-                             dt: float, strict: bool) -> dict[str, Any]:
+                             dt: float, strict: bool, keep_tape: bool = False) -> dict[str, Any]:
+    """The differentiable step.  The forward records the step's tape when
+    ``keep_tape`` is set (the caller knows a backward follows) or when an
+    input carries a forward-mode tangent (the jvp runs right after the
+    forward); otherwise the backward recomputes it from the checkpoint."""
+    from torch.autograd import forward_ad
+
     from .adjoint import STEP_INPUTS
 
     in_keys = [k for k in STEP_INPUTS if isinstance(state.get(k), torch.Tensor)]
+    dual = any(forward_ad.unpack_dual(state[k]).tangent is not None for k in in_keys)
     box = {"state": {k: v for k, v in state.items() if k not in in_keys}, "mesh": mesh, "config": config,
-           "dt": dt, "strict": strict, "in_keys": in_keys}
+           "dt": dt, "strict": strict, "in_keys": in_keys, "keep_tape": bool(keep_tape or dual)}
     outs = _Srk3Step.apply(box, *[state[k] for k in in_keys])
     result = dict(state)
     result.update(zip(box["out_keys"], outs))

@@ -178,6 +178,75 @@ def test_dyn_tend_adjoint(case, idx):
     _check_identity(f"dyn_tend rk{rk}", fn, gpu_op, X, K.DYN_CARRIED + K.DYN_OUTPUTS, rng, rows)
 
 
+def _scatter_and_gather(dm, op, rng):
+    """Input adjoints of one recorded op from the atomic scatter form (no
+    inverse tables) and from the gather form (inverse tables given)."""
+    import ctypes as C
+
+    from mpas_dycore_pytorch_amd import _native as N
+    from mpas_dycore_pytorch_amd.adjoint import vjp_scratch
+
+    lib = N.load()
+    seeds = []
+    for o in op.outs:
+        sd = torch.as_tensor(rng.standard_normal(tuple(o.shape)), device=dm.device)
+        sd[-1] = 0.0
+        seeds.append(sd)
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    res = []
+    for inv in (None, C.byref(dm.inverse_tables())):
+        bars = [torch.zeros_like(x) for x in op.ins] + [sd.clone() for sd in seeds]
+        n = len(bars)
+        fwd = (C.c_void_p * n)(*[x.data_ptr() for x in op.ins + op.outs])
+        bar = (C.c_void_p * n)(*[b.data_ptr() for b in bars])
+        scratch = vjp_scratch(dm, 4)
+        sp = C.c_void_p(scratch.data_ptr())
+        if op.kind == "solve_diagnostics":
+            rc = lib.mpas_hip_solve_diagnostics_vjp(dm.ref, *op.args, fwd, bar, sp, inv, stream)
+        else:
+            rc = lib.mpas_hip_dyn_tend_vjp(dm.ref, C.byref(op.args[0]), fwd, bar, sp, inv, stream)
+        N.check(rc, op.kind + "_vjp")
+        torch.cuda.synchronize()
+        res.append([b.cpu().numpy() for b in bars[: len(op.ins)]])
+    return res
+
+
+@pytest.mark.parametrize("kind,idx", [("solve_diagnostics", 1), ("solve_diagnostics", 2),
+                                      ("solve_diagnostics", 3), ("dyn_tend", 0), ("dyn_tend", 1)])
+def test_gather_adjoint_matches_scatter(case, kind, idx):
+    """The atomic-free gather forms of the diagnostics and dynamics-tendency
+    adjoints (inverse tables given) against their atomic scatter forms: the
+    same sums in another order, so equal to rounding."""
+    from mpas_dycore_pytorch_amd import kernels as K
+    from mpas_dycore_pytorch_amd.adjoint import Tape
+
+    st, meta, om, calls, dm, rows = case
+    _, a, _, r = _first(calls, kind, idx)
+    tape = Tape(dm)
+    if kind == "solve_diagnostics":
+        dt, apvm, rk = float(a[1]), float(a[2]), int(a[3])
+        ins = {"u": _gpu(a[4], dm), "h": _gpu(a[5], dm), "v_in": _gpu(a[6], dm)}
+        outs = K.solve_diagnostics(dm, dt, apvm, rk, ins["u"], ins["h"], ins["v_in"])
+        tape.kernel(kind, (dt, apvm, rk), ins, outs)
+    else:
+        params, f, carried = a[1], a[2], (a[3] if len(a) > 3 else None)
+        rk = params.rk_step
+        ins = {n: _gpu(f[n], dm) for n in K.DYN_INPUTS}
+        for n in K.DYN_CARRIED:
+            ins[n + "_in"] = _gpu(carried[n] if carried is not None else np.zeros_like(np.asarray(r[n])), dm)
+        p = K.dyn_params(rk, params.dt, 0.0)
+        for fld, _ in type(params)._fields_:
+            setattr(p, fld, getattr(params, fld))
+        cin = {n: ins[n + "_in"] for n in K.DYN_CARRIED}
+        outs = K.dyn_tend(dm, p, {n: ins[n] for n in K.DYN_INPUTS}, cin if rk != 1 else None)
+        tape.kernel(kind, (p,), ins, outs)
+    print(f"[gather] {kind} call {idx} rk={rk}", flush=True)
+    scat, gath = _scatter_and_gather(dm, tape.ops[0], np.random.default_rng(100 + idx))
+    for x, y in zip(scat, gath):
+        scale = max(float(np.max(np.abs(x))), 1e-300)
+        assert float(np.max(np.abs(x - y))) <= 1e-12 * scale, (kind, idx, float(np.max(np.abs(x - y))), scale)
+
+
 @pytest.mark.parametrize("idx", [0, 3])
 def test_acoustic_adjoint(case, idx):
     from oracle import oracle as O
@@ -481,6 +550,46 @@ def test_run_simulation_vjp_two_steps():
     rels = [abs(fd - ad) / max(abs(fd), abs(ad)) for fd in fds]
     print(f"[adjoint] run_simulation.vjp 2 steps: fd={fds} adjoint={ad:.12e} rel={rels}", flush=True)
     assert min(rels) < 1e-5
+
+
+def test_kept_tape_matches_recompute():
+    """The step that records its tape in the forward (the last step, or every
+    step with checkpoint_timesteps=False) returns the fused step's state bit
+    for bit, and its gradients equal the recomputing backward's."""
+    from mpas_dycore_pytorch_amd.atmosphere_model import STEP_BOUNDARY_KEY, run_simulation
+
+    st, meta = _noisy_state(6)
+    config = {"config_time_integration_order": 3, "config_len_disp": _len_disp(st, meta)}
+    dev = torch.device("cuda")
+    state = {k: (torch.as_tensor(np.asarray(v, dtype=np.float64), device=dev)
+                 if isinstance(v, np.ndarray) and v.dtype.kind == "f" else v) for k, v in st.items()}
+    state[STEP_BOUNDARY_KEY] = True
+    with torch.no_grad():  # no backward can follow: every step runs fused
+        plain = run_simulation(dict(state), st, config=config, dt=meta["dt"], nsteps=2, device=dev,
+                               differentiable=True)
+    diff_in = dict(state)
+    diff_in["u"] = state["u"].detach().clone().requires_grad_(True)
+    diff = run_simulation(diff_in, st, config=config, dt=meta["dt"], nsteps=2, device=dev, differentiable=True)
+    for k in ("u", "w", "theta_m", "rho_zz", "scalars", "exner", "pressure_p", "ru", "rw", "rho_p", "rtheta_p",
+              "v", "pv_edge", "ke", "divergence", "vorticity", "h_edge", "uReconstructZonal", "ruAvg", "wwAvg"):
+        assert torch.equal(plain[k], diff[k].detach()), k
+    nc = int(st["nCells"])
+    w8 = torch.as_tensor(np.random.default_rng(8).standard_normal((nc, int(st["nVertLevels"]))), device=dev)
+
+    def objective(out):
+        return torch.sum(w8 * out["theta_m"][:nc]) + torch.sum(out["u"][:-1] ** 2)
+
+    keys = ("u", "theta_m", "w", "rho_zz")
+    g_ck = run_simulation.vjp(dict(state), st, objective, track_keys=keys, config=config, dt=meta["dt"],
+                              nsteps=2, device=dev)
+    g_all = run_simulation.vjp(dict(state), st, objective, track_keys=keys, config=config, dt=meta["dt"],
+                               nsteps=2, device=dev, checkpoint_timesteps=False)
+    for k in keys:
+        a, b = g_ck[k].cpu().numpy(), g_all[k].cpu().numpy()
+        scale = max(float(np.max(np.abs(a))), 1e-300)
+        err = float(np.max(np.abs(a - b))) / scale
+        print(f"[adjoint] kept tape vs recompute d/d{k}: {err:.2e}", flush=True)
+        assert err <= 1e-12, k
 
 
 @pytest.mark.parametrize("perturbed", [False, True])
