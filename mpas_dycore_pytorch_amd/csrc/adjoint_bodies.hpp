@@ -107,6 +107,48 @@ __device__ __forceinline__ void smlstep_vjp(const mpas_hip_mesh &m, Fwd F, Bar B
     }
 }
 
+// Gather form (no atomics): the cell pass keeps the w_tend part, the edge pass
+// sums what the edge's two cells send its level k from interfaces k and k+1.
+__device__ __forceinline__ void smlstep_cells_vjp(const mpas_hip_mesh &m, Bar B, int c, int k) {
+    using namespace smlstep;
+    const int L = m.nLevels, Lw = L + 1;
+    const long long i = (long long)c * Lw + k;
+    const double g = bar_of(B, w_tend_out, i);
+    if (c >= m.nCells || k < 1 || k >= L) {
+        add_own(B, w_tend_in, i, g);
+        return;
+    }
+    add_own(B, w_tend_in, i, (m.fzm[k] * m.zz[c * L + k] + m.fzp[k] * m.zz[c * L + k - 1]) * g);
+}
+// gf of cell c, slot j (edge e) at interface kk in 1..L-1
+__device__ __forceinline__ double smlstep_gf(const mpas_hip_mesh &m, Fwd F, Bar B, int c, int j, int e, int kk) {
+    using namespace smlstep;
+    const int L = m.nLevels, ME = m.maxEdges, Lw = L + 1;
+    const double zi = m.fzm[kk] * m.zz[c * L + kk] + m.fzp[kk] * m.zz[c * L + kk - 1];
+    const double gw = zi * bar_of(B, w_tend_out, (long long)c * Lw + kk);
+    const double uk = F[u_tend][(long long)e * L + kk];
+    const long long zo = ((long long)c * ME + j) * Lw + kk;
+    const double coef = m.zb_cell[zo] + sign_pm(uk) * m.zb3_cell[zo];
+    return -coef * m.edgesOnCell_sign[c * ME + j] * gw;
+}
+__device__ __forceinline__ void smlstep_edges_vjp(const mpas_hip_mesh &m, Fwd F, Bar B, int e, int k) {
+    using namespace smlstep;
+    const int L = m.nLevels, ME = m.maxEdges;
+    if (e >= m.nEdges || k >= L) return;
+    double gu = 0.0;
+    for (int q = 0; q < 2; ++q) {
+        const int c = m.cellsOnEdge[2 * e + q];
+        if (c >= m.nCells) continue;
+        const int n = m.nEdgesOnCell[c];
+        int j = 0;
+        while (j < n && m.edgesOnCell[c * ME + j] != e) ++j;
+        if (j == n) continue;
+        if (k >= 1) gu += m.fzm[k] * smlstep_gf(m, F, B, c, j, e, k);
+        if (k + 1 < L) gu += m.fzp[k + 1] * smlstep_gf(m, F, B, c, j, e, k + 1);
+    }
+    add_own(B, u_tend, (long long)e * L + k, gu);
+}
+
 // ===========================================================================
 // atm_divergence_damping_3d (divergence_damping.py:187-209, coefficient :267)
 // fwd: theta_m, rtheta_pp, rtheta_pp_old, ru_p (before), ru_p (after)

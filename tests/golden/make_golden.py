@@ -54,12 +54,68 @@ CASES = {
     "l2_L26_dry": (2, 26, "uniform", 0, 1800.0, 2, 1234),
     "l2_L20_tracer": (2, 20, "quadratic", 1, 900.0, 3, 4321),
 }
+# whole-step cases over synthetic terrain: the same tuple plus the mountain
+# height in metres (the cases above are flat, so every zb/zb3/zxu term is 0)
+TERRAIN_CASES = {
+    "l2_L20_terrain": (2, 20, "quadratic", 1, 900.0, 2, 6060, 1500.0),
+}
 SUMMARY_CASES = {
     "cfg1_l4_L26": (4, 26, "uniform", 0, 1800.0, 2, 20240),
 }
 SUMMARY_FIELDS = ("u", "w", "theta_m", "rho_zz", "scalars", "exner", "pressure_p",
                   "rtheta_p", "rho_p", "ru", "rw", "v", "divergence", "vorticity",
                   "ke", "pv_edge", "uReconstructZonal", "uReconstructMeridional")
+
+
+def add_synthetic_terrain(m: dict, h0: float, coef_3rd_order: float = 0.25) -> dict:
+    """The mesh over two smooth mountains (peak ``h0`` m), for fixtures only.
+
+    Terrain-following heights zgrid = zw + hx (1 - zw / z_top); zz is the
+    ratio of the flat to the terrain-following layer depths, zxu the slope of
+    the layer mid-height across each edge, zb the cell-side slopes
+    (z_edge - z_cell) dvEdge / areaCell and zb3 a tenth of zb.  zb_cell and
+    zb3_cell are picked from zb / zb3 by the cell's side of each edge, as the
+    reference's loader forms them (ic_loader.py:220-276)."""
+    m = dict(m)
+    nc, ne, L = int(m["nCells"]), int(m["nEdges"]), int(m["nVertLevels"])
+    zw = np.array(m["zgrid"][0], dtype=np.float64)
+    ztop = zw[-1]
+    lat, lon = np.asarray(m["latCell"][:nc]), np.asarray(m["lonCell"][:nc])
+    radius = float(m["sphere_radius"])
+
+    def dist(lat0, lon0):
+        c = np.sin(lat) * np.sin(lat0) + np.cos(lat) * np.cos(lat0) * np.cos(lon - lon0)
+        return radius * np.arccos(np.clip(c, -1.0, 1.0))
+
+    hx = h0 * np.exp(-(dist(0.5, 1.0) / 1.5e6) ** 2) + 0.5 * h0 * np.exp(-(dist(-0.3, 3.5) / 1.0e6) ** 2)
+    zgrid = np.array(m["zgrid"], dtype=np.float64)
+    zgrid[:nc] = zw[None, :] + hx[:, None] * (1.0 - zw[None, :] / ztop)
+    zz = np.array(m["zz"], dtype=np.float64)
+    zz[:nc] = (zw[1:] - zw[:-1])[None, :] / (zgrid[:nc, 1:] - zgrid[:nc, :-1])
+    coe = np.asarray(m["cellsOnEdge"][:ne], dtype=np.int64) - 1
+    c1, c2 = coe[:, 0], coe[:, 1]
+    dz = zgrid[c2] - zgrid[c1]
+    zxu = np.zeros_like(np.asarray(m["zxu"], dtype=np.float64))
+    zxu[:ne] = 0.5 * (dz[:, :-1] + dz[:, 1:]) / np.asarray(m["dcEdge"][:ne])[:, None]
+    area = np.asarray(m["areaCell"], dtype=np.float64)
+    dv = np.asarray(m["dvEdge"][:ne], dtype=np.float64)[:, None]
+    z_edge = 0.5 * (zgrid[c1] + zgrid[c2])
+    zb = np.zeros_like(np.asarray(m["zb"], dtype=np.float64))
+    zb[:ne, 0] = (z_edge - zgrid[c1]) * dv / area[c1][:, None]
+    zb[:ne, 1] = (z_edge - zgrid[c2]) * dv / area[c2][:, None]
+    zb3 = 0.1 * zb
+    me = int(m["maxEdges"])
+    eoc = np.asarray(m["edgesOnCell"][:nc, :me], dtype=np.int64)
+    valid = (eoc > 0) & (np.arange(me)[None, :] < np.asarray(m["nEdgesOnCell"][:nc])[:, None])
+    erow = np.where(valid, eoc - 1, 0)
+    side = (coe[erow, 1] == np.arange(nc)[:, None]).astype(np.int64)
+    zb_cell = np.zeros_like(np.asarray(m["zb_cell"], dtype=np.float64))
+    zb3_cell = np.zeros_like(zb_cell)
+    zb_cell[:nc] = zb[erow, side] * valid[..., None]
+    zb3_cell[:nc] = zb3[erow, side] * valid[..., None] * coef_3rd_order
+    m.update({"zgrid": zgrid, "zz": zz, "zxu": zxu, "zb": zb, "zb3": zb3, "zb_cell": zb_cell,
+              "zb3_cell": zb3_cell})
+    return m
 
 
 def reference_config(mesh: dict, dt: float) -> dict:
@@ -242,8 +298,10 @@ def _state_arrays(st: dict, keys=None) -> dict:
     return out
 
 
-def run_case(level, n_levels, stretch, n_tracers, dt, steps, seed):
+def run_case(level, n_levels, stretch, n_tracers, dt, steps, seed, terrain=0.0):
     m = M.build_mesh(level, n_levels, stretch=stretch)
+    if terrain:
+        m = add_synthetic_terrain(m, terrain)
     raw = IS.raw_state(m, seed=seed, n_tracers=n_tracers)
     st, mesh = prepared_reference_state(m, raw, dt)
     cfg = reference_config(m, dt)
@@ -260,8 +318,8 @@ def run_case(level, n_levels, stretch, n_tracers, dt, steps, seed):
 
 
 def write_srk3_case(name: str, out_dir: str) -> None:
-    level, L, stretch, ntr, dt, steps, seed = CASES[name]
-    m, raw, prepared, outs, times = run_case(level, L, stretch, ntr, dt, steps, seed)
+    level, L, stretch, ntr, dt, steps, seed, *terrain = CASES.get(name) or TERRAIN_CASES[name]
+    m, raw, prepared, outs, times = run_case(level, L, stretch, ntr, dt, steps, seed, *terrain)
     payload = {}
     for k, v in _state_arrays(prepared).items():
         payload["in__" + k] = v
@@ -270,6 +328,7 @@ def write_srk3_case(name: str, out_dir: str) -> None:
             payload[f"out{i + 1}__{k}"] = v
     meta = {"case": name, "level": level, "nVertLevels": L, "stretch": stretch,
             "n_tracers": ntr, "dt": dt, "steps": steps, "seed": seed,
+            "terrain_height": terrain[0] if terrain else 0.0,
             "reference_seconds_per_step": times}
     path = os.path.join(out_dir, f"srk3_{name}.npz")
     np.savez_compressed(path, __meta__=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8),
@@ -313,13 +372,15 @@ def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--which", default="all")
     ap.add_argument("--out", default=HERE)
+    ap.add_argument("--case", default=None, help="write only this whole-step case")
     args = ap.parse_args()
     torch.set_num_threads(8)
     if args.which in ("kernels", "all"):
         record_kernels(1, 10, "quadratic", 1, 900.0, 99, os.path.join(args.out, "kernels_l1_L10.npz"))
     if args.which in ("srk3", "all"):
-        for name in CASES:
-            write_srk3_case(name, args.out)
+        for name in list(CASES) + list(TERRAIN_CASES):
+            if args.case is None or args.case == name:
+                write_srk3_case(name, args.out)
     if args.which in ("summary", "all"):
         for name in SUMMARY_CASES:
             write_summary_case(name, args.out)

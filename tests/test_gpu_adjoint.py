@@ -25,8 +25,8 @@ pytestmark = pytest.mark.gpu
 CASE = "l2_L20_tracer"
 
 
-def _noisy_state(seed: int = 0):
-    st, _, meta = srk3_case(CASE)
+def _noisy_state(seed: int = 0, case: str = CASE):
+    st, _, meta = srk3_case(case)
     return perturbed_state(st, seed), meta
 
 
@@ -456,13 +456,14 @@ def test_taped_step_matches_fused_step():
         assert err <= 1e-13, k
 
 
-def test_step_vjp_against_oracle_finite_differences():
+@pytest.mark.parametrize("case", [CASE, "l2_L20_terrain"])
+def test_step_vjp_against_oracle_finite_differences(case):
     """Whole large step: <ybar, J t> by oracle central differences vs the GPU VJP."""
     from oracle import oracle as O, srk3 as OS
     from mpas_dycore_pytorch_amd.adjoint import STEP_INPUTS, step_vjp
     from mpas_dycore_pytorch_amd.device import DeviceMesh
 
-    st, meta = _noisy_state(2)
+    st, meta = _noisy_state(2, case)
     cfg = _cfg(st, meta)
     dm = DeviceMesh(st, torch.device("cuda"))
     om = O.Mesh(st)

@@ -53,6 +53,28 @@ def test_init_coupled_on_gpu(gpu_available):
     assert got["rho_zz"][nc].tolist() == [1.0] * m["nVertLevels"]
 
 
+def test_init_coupled_on_gpu_over_terrain(gpu_available):
+    """Cold start over synthetic terrain (the zb/zb3 terms of rw) against the
+    reference's own prepared fields: bit for bit except exner (pow, 1 ulp)."""
+    from mpas_dycore_pytorch_amd import kernels as K
+    from mpas_dycore_pytorch_amd.device import DeviceMesh
+
+    print("init_coupled terrain l2_L20_terrain", flush=True)
+    st, _, _ = srk3_case("l2_L20_terrain")
+    dm = DeviceMesh(st, torch.device("cuda", 0))
+    assert dm.s.flatTerrain == 0
+    got = K.init_coupled_diagnostics(dm, st["theta"], st["rho"], st["qv"], st["u"], st["w"], st["theta_base"])
+    got = {k: v.cpu().numpy() for k, v in got.items()}
+    nc, ne = int(st["nCells"]), int(st["nEdges"])
+    rows = {"ru": ne}
+    for k in ("theta_m", "rho_zz", "ru", "rw", "rho_p", "rtheta_p"):
+        n = rows.get(k, nc)
+        ref = np.asarray(st[k])[:n]
+        assert np.array_equal(got[k][:n], ref), (k, rel_err(got[k][:n], ref))
+    ulps = np.abs(got["exner"][:nc].view(np.int64) - np.asarray(st["exner"])[:nc].view(np.int64))
+    assert ulps.max() <= 1
+
+
 def test_run_timestep_from_raw_matches_reference(gpu_available):
     from mpas_dycore_pytorch_amd import run_timestep
 

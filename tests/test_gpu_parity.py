@@ -91,7 +91,7 @@ def _gpu_step(state, dt, len_disp, steps, n_scalars_cfg=None):
     return outs
 
 
-@pytest.mark.parametrize("case", ["l2_L26_dry", "l2_L20_tracer"])
+@pytest.mark.parametrize("case", ["l2_L26_dry", "l2_L20_tracer", "l2_L20_terrain"])
 def test_srk3_matches_reference(gpu_available, case):
     st, refs, meta = srk3_case(case)
     print(f"srk3 case {case}", flush=True)

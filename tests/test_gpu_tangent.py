@@ -23,8 +23,8 @@ pytestmark = pytest.mark.gpu
 CASE = "l2_L20_tracer"
 
 
-def _noisy_state(seed: int):
-    st, _, meta = srk3_case(CASE)
+def _noisy_state(seed: int, case: str = CASE):
+    st, _, meta = srk3_case(case)
     return perturbed_state(st, seed), meta
 
 
@@ -172,13 +172,14 @@ def test_step_jvp_is_transpose_of_step_vjp():
     assert rel <= 1e-11
 
 
-def test_step_jvp_against_oracle_finite_differences():
+@pytest.mark.parametrize("case", [CASE, "l2_L20_terrain"])
+def test_step_jvp_against_oracle_finite_differences(case):
     """Whole large step: J t from the tangent kernels vs oracle central differences, per output field."""
     from oracle import oracle as O, srk3 as OS
     from mpas_dycore_pytorch_amd.adjoint import step_jvp
     from mpas_dycore_pytorch_amd.device import DeviceMesh
 
-    st, meta = _noisy_state(13)
+    st, meta = _noisy_state(13, case)
     dm = DeviceMesh(st, torch.device("cuda"))
     cfg = _cfg(st, meta)
     om = O.Mesh(st)

@@ -51,7 +51,7 @@ FIELDS = ("u", "w", "theta_m", "rho_zz", "scalars", "exner", "pressure_p", "rthe
           "v", "divergence", "vorticity", "ke", "pv_edge", "uReconstructZonal", "uReconstructMeridional")
 
 
-@pytest.mark.parametrize("case", ["l2_L26_dry", "l2_L20_tracer"])
+@pytest.mark.parametrize("case", ["l2_L26_dry", "l2_L20_tracer", "l2_L20_terrain"])
 def test_srk3_whole_steps(case):
     st, outs, meta = srk3_case(case)
     mesh = O.Mesh(st)
