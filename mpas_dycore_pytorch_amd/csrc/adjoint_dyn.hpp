@@ -426,10 +426,8 @@ __device__ __forceinline__ void dyn_a3(const mpas_hip_mesh &m, const mpas_hip_dy
             add(B, ru_save, i, dv * face * gp);
             add(B, ru, i, -dv * face * gp);
             const double gface = dv * (F[ru_save][i] - ru_[k]) * gp;
-            if (!gather) {
-                add(B, theta_m_save, (long long)c1 * L + k, 0.5 * gface);
-                add(B, theta_m_save, (long long)c2 * L + k, 0.5 * gface);
-            }
+            add(B, theta_m_save, (long long)c1 * L + k, 0.5 * gface);
+            add(B, theta_m_save, (long long)c2 * L + k, 0.5 * gface);
         }
     }
     // ---- w edge flux at interface k (1..L-1)
@@ -470,13 +468,11 @@ __device__ __forceinline__ void dyn_a3(const mpas_hip_mesh &m, const mpas_hip_dy
         const double hsum = F[h_divergence][(long long)c1 * L + k] + F[h_divergence][(long long)c2 * L + k];
         add(B, rho_edge, i, (q - ked * m.invDcEdge[e]) * gtu);
         const double gq = re * gtu;
+        add(B, ke, (long long)c2 * L + k, -re * m.invDcEdge[e] * gtu);
+        add(B, ke, (long long)c1 * L + k, re * m.invDcEdge[e] * gtu);
         add(B, u, i, -0.5 * hsum * gtu);
-        if (!gather) {  // the cell terms: gathered by dyn_a3_cells in the gather form
-            add(B, ke, (long long)c2 * L + k, -re * m.invDcEdge[e] * gtu);
-            add(B, ke, (long long)c1 * L + k, re * m.invDcEdge[e] * gtu);
-            acc(B[h_divergence] + (long long)c1 * L + k, -ue[k] * 0.5 * gtu);
-            acc(B[h_divergence] + (long long)c2 * L + k, -ue[k] * 0.5 * gtu);
-        }
+        acc(B[h_divergence] + (long long)c1 * L + k, -ue[k] * 0.5 * gtu);
+        acc(B[h_divergence] + (long long)c2 * L + k, -ue[k] * 0.5 * gtu);
         double gpe = 0.0;
         for (int j = 0; j < ne; ++j) {
             const int ee = m.edgesOnEdge[e * ME2 + j];
@@ -512,10 +508,8 @@ __device__ __forceinline__ void dyn_a3(const mpas_hip_mesh &m, const mpas_hip_dy
                 add(B, u, i + 1, d[3] * g);
                 gerw = d[4] * g;
             }
-            if (!gather) {
-                add(B, rw, (long long)c1 * Lw + k, 0.5 * gerw);
-                add(B, rw, (long long)c2 * Lw + k, 0.5 * gerw);
-            }
+            add(B, rw, (long long)c1 * Lw + k, 0.5 * gerw);
+            add(B, rw, (long long)c2 * Lw + k, 0.5 * gerw);
         }
     }
     // ---- tend_u_euler
@@ -531,14 +525,12 @@ __device__ __forceinline__ void dyn_a3(const mpas_hip_mesh &m, const mpas_hip_dy
         const double dps = S.dpdz[(long long)c1 * L + k] + S.dpdz[(long long)c2 * L + k];
         const double cq = F[cqu][i];
         add_own(B, cqu, i, -(ppd * m.invDcEdge[e] / (0.5 * zzs) - 0.5 * m.zxu[i] * dps) * gtue);
-        if (!gather) {
-            const double gppd = -cq * m.invDcEdge[e] / (0.5 * zzs) * gtue;
-            add(B, pp, (long long)c2 * L + k, gppd);
-            add(B, pp, (long long)c1 * L + k, -gppd);
-            const double gdps = cq * 0.5 * m.zxu[i] * gtue;
-            acc(S.g_dpdz + (long long)c1 * L + k, gdps);
-            acc(S.g_dpdz + (long long)c2 * L + k, gdps);
-        }
+        const double gppd = -cq * m.invDcEdge[e] / (0.5 * zzs) * gtue;
+        add(B, pp, (long long)c2 * L + k, gppd);
+        add(B, pp, (long long)c1 * L + k, -gppd);
+        const double gdps = cq * 0.5 * m.zxu[i] * gtue;
+        acc(S.g_dpdz + (long long)c1 * L + k, gdps);
+        acc(S.g_dpdz + (long long)c2 * L + k, gdps);
     }
     {
         // del2: + re kdiffu ud ms2 (ud = delsq_u, adjoint finished in A5)
@@ -547,11 +539,9 @@ __device__ __forceinline__ void dyn_a3(const mpas_hip_mesh &m, const mpas_hip_dy
         const double kdu = 0.5 * (kd1 + kd2);
         const double ms2 = m.meshScalingDel2[e];
         add(B, rho_edge, i, kdu * ud * ms2 * gtue);
-        if (!gather) {
-            const double gk = 0.5 * re * ud * ms2 * gtue;
-            acc(B[kdiff] + (long long)c1 * L + k, gk);
-            acc(B[kdiff] + (long long)c2 * L + k, gk);
-        }
+        const double gk = 0.5 * re * ud * ms2 * gtue;
+        acc(B[kdiff] + (long long)c1 * L + k, gk);
+        acc(B[kdiff] + (long long)c2 * L + k, gk);
         acc(S.g_delsq_u + i, re * kdu * ms2 * gtue);
     }
     if (P.h_mom_eddy_visc4 > 0.0) {
@@ -563,10 +553,8 @@ __device__ __forceinline__ void dyn_a3(const mpas_hip_mesh &m, const mpas_hip_dy
         const double dd = S.delsq_div[(long long)c2 * L + k] - S.delsq_div[(long long)c1 * L + k];
         const double dvv = S.delsq_vort[(long long)v2 * L + k] - S.delsq_vort[(long long)v1 * L + k];
         add(B, rho_edge, i, -(dd * r_dc - dvv * r_dv) * gtue);
-        if (!gather) {
-            acc(S.g_delsq_div + (long long)c2 * L + k, -re * r_dc * gtue);
-            acc(S.g_delsq_div + (long long)c1 * L + k, re * r_dc * gtue);
-        }
+        acc(S.g_delsq_div + (long long)c2 * L + k, -re * r_dc * gtue);
+        acc(S.g_delsq_div + (long long)c1 * L + k, re * r_dc * gtue);
         acc(S.g_delsq_vort + (long long)v2 * L + k, re * r_dv * gtue);
         acc(S.g_delsq_vort + (long long)v1 * L + k, -re * r_dv * gtue);
     }
@@ -576,81 +564,11 @@ __device__ __forceinline__ void dyn_a3(const mpas_hip_mesh &m, const mpas_hip_dy
 // thread (cell c, level / interface k): adjoints of theta_m (k < L) and w
 // (1 <= k < L) at the cells of the advection stencils, gathered over the
 // (edge, slot) pairs naming cell c, in (edge, slot) order
-// The terms of A3 that edge e sends to one of its two cells c (gather form):
-// ke, h_divergence and rw; on the first stage pp, dpdz, kdiff and the del4
-// divergence, on later stages theta_m_save.  Recomputed from the output
-// adjoints of the edge, as A3 forms them.
-struct A3CellTerms {
-    double ke = 0.0, hd = 0.0, rw = 0.0, pp = 0.0, dp = 0.0, kd = 0.0, dd = 0.0, ths = 0.0;
-};
-__device__ __forceinline__ void dyn_a3_cell_terms(const mpas_hip_mesh &m, const mpas_hip_dyn_params &P, Fwd F,
-                                                  Bar B, const DynAdjScratch &S, int c, double sc, int e, int k,
-                                                  A3CellTerms &t) {
+__device__ __forceinline__ void dyn_a3_cells(const mpas_hip_mesh &m, Fwd F, Bar B, const DynAdjScratch &S,
+                                             const mpas_hip_inverse_tables &inv, int c, int k) {
     using namespace dyn;
-    const int L = m.nLevels, Lw = L + 1;
-    const long long i = (long long)e * L + k;
-    const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-    const double side = (c == c1) ? 1.0 : -1.0;  // +x to c1, -x to c2 for the differenced terms
-    const double gtu = bar_of(B, tend_u, i);
-    const double re = F[rho_edge][i];
-    const double *ue = F[u] + (long long)e * L;
-    t.ke += side * (re * m.invDcEdge[e] * gtu);
-    t.hd += -ue[k] * 0.5 * gtu;
-    if (k >= 1) {
-        const double g = -m.rdzw[k - 1] * bar_of(B, tend_u, i - 1) + m.rdzw[k] * gtu;
-        double gerw;
-        if (k == 1 || k == L - 1) {
-            gerw = (m.fzm[k] * ue[k] + m.fzp[k] * ue[k - 1]) * g;
-        } else {
-            const double erw = 0.5 * (F[rw][(long long)c1 * Lw + k] + F[rw][(long long)c2 * Lw + k]);
-            double d[5];
-            flux3_partials(ue[k - 2], ue[k - 1], ue[k], ue[k + 1], erw, 1.0, d);
-            gerw = d[4] * g;
-        }
-        t.rw += 0.5 * gerw;
-    }
-    if (P.rk_step == 1) {
-        const double gtue = bar_of(B, tend_u_euler, i) + gtu;
-        const double zzs = m.zz[(long long)c2 * L + k] + m.zz[(long long)c1 * L + k];
-        const double cq = F[cqu][i];
-        t.pp += side * (cq * m.invDcEdge[e] / (0.5 * zzs) * gtue);
-        t.dp += cq * 0.5 * m.zxu[i] * gtue;
-        t.kd += 0.5 * re * S.delsq_u[i] * m.meshScalingDel2[e] * gtue;
-        if (P.h_mom_eddy_visc4 > 0.0) {
-            const double ums = m.meshScalingDel4[e] * P.h_mom_eddy_visc4;
-            t.dd += side * (re * (ums * P.del4u_div_factor * m.invDcEdge[e]) * gtue);
-        }
-    } else {
-        const double s1 = (c == c1) ? sc : edge_sign_in_cell(m, c1, e);
-        const double s2 = (c == c2) ? sc : edge_sign_in_cell(m, c2, e);
-        const double gf = -(s1 * S.g_thadv[(long long)c1 * L + k] + s2 * S.g_thadv[(long long)c2 * L + k]);
-        t.ths += 0.5 * (m.dvEdge[e] * (F[ru_save][i] - F[ru][i]) * gf);
-    }
-}
-__device__ __forceinline__ void dyn_a3_cells(const mpas_hip_mesh &m, const mpas_hip_dyn_params &P, Fwd F, Bar B,
-                                             const DynAdjScratch &S, const mpas_hip_inverse_tables &inv, int c,
-                                             int k) {
-    using namespace dyn;
-    const int L = m.nLevels, Lw = L + 1, MA = m.maxAdv, ME = m.maxEdges;
+    const int L = m.nLevels, Lw = L + 1, MA = m.maxAdv;
     if (c >= m.nCells || k >= L) return;
-    {
-        A3CellTerms t;
-        const int n = m.nEdgesOnCell[c];
-        for (int j = 0; j < n; ++j)
-            dyn_a3_cell_terms(m, P, F, B, S, c, m.edgesOnCell_sign[c * ME + j], m.edgesOnCell[c * ME + j], k, t);
-        const long long o = (long long)c * L + k;
-        add_own(B, ke, o, t.ke);
-        B[h_divergence][o] += t.hd;
-        if (k >= 1) add_own(B, rw, (long long)c * Lw + k, t.rw);
-        if (P.rk_step == 1) {
-            add_own(B, pp, o, t.pp);
-            S.g_dpdz[o] += t.dp;
-            B[kdiff][o] += t.kd;
-            if (P.h_mom_eddy_visc4 > 0.0) S.g_delsq_div[o] += t.dd;
-        } else {
-            add_own(B, theta_m_save, o, t.ths);
-        }
-    }
     const int j0 = inv.advStart[c], j1 = inv.advStart[c + 1];
     double gt = 0.0, gw = 0.0;
     for (int j = j0; j < j1; ++j) {

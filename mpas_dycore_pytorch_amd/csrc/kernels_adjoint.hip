@@ -41,7 +41,7 @@ ADJ_WAVE_KERNEL(k_adj_dyn_a2, dyn_a2(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_a2_edges, dyn_a2_edges(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_a2_cells, dyn_a2_cells(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_a3, dyn_a3(m, A.dp, ADJ_F, ADJ_B, A.dy, A.gather != 0, ent, k))
-ADJ_WAVE_KERNEL(k_adj_dyn_a3_cells, dyn_a3_cells(m, A.dp, ADJ_F, ADJ_B, A.dy, A.inv, ent, k))
+ADJ_WAVE_KERNEL(k_adj_dyn_a3_cells, dyn_a3_cells(m, ADJ_F, ADJ_B, A.dy, A.inv, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_a3_edges, dyn_a3_edges(m, ADJ_F, ADJ_B, A.dy, A.inv, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_a4, dyn_a4(m, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_a5, dyn_a5(m, ADJ_B, A.dy, ent, k))
@@ -97,6 +97,7 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
     const double *ru_ = F[ru] + (long long)e * L;
     const int na = m.nAdvCellsForEdge[e];
     double g_ru_own = 0.0, g_ru_dn = 0.0;  // to ru(k) and ru(k-1)
+    double x_ths = 0.0;
     // ---- theta edge flux
     {
         const double gf = -(s1 * S.g_thadv[(long long)c1 * L + k] + s2 * S.g_thadv[(long long)c2 * L + k]);
@@ -115,6 +116,7 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
             const double dv = m.dvEdge[e];
             if (act) add_own(B, ru_save, i, dv * face * gf);
             g_ru_own += -dv * face * gf;
+            x_ths = 0.5 * (dv * (F[ru_save][i] - ru_[k]) * gf);
         }
     }
     // ---- w edge flux at interface k (1..L-1)
@@ -138,6 +140,7 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
     const double *ue = F[u] + (long long)e * L;
     const double re = F[rho_edge][i];
     double g_re = 0.0, g_u_own = 0.0, g_pv_own = 0.0;
+    double x_ke, x_hd;
     {
         const int ne = m.nEdgesOnEdge[e];
         const double pvs = F[pv_edge][i];
@@ -153,7 +156,9 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
         const double hsum = F[h_divergence][(long long)c1 * L + k] + F[h_divergence][(long long)c2 * L + k];
         g_re += (q - ked * m.invDcEdge[e]) * gtu;
         const double gq = re * gtu;
+        x_ke = re * m.invDcEdge[e] * gtu;
         g_u_own += -0.5 * hsum * gtu;
+        x_hd = -ue[k] * 0.5 * gtu;
         double gpe = 0.0;
         for (int j = 0; j < ne; ++j) {
             const int ee = m.edgesOnEdge[e * ME2 + j];
@@ -163,11 +168,14 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
         if (act) S.g_q[i] = gq;
     }
     // ---- vertical flux of u at interface k: contributions to u(k-2..k+1)
-    double c_m2 = 0.0, c_m1 = 0.0, c_p1 = 0.0;
+    double c_m2 = 0.0, c_m1 = 0.0, c_p1 = 0.0, x_rw = 0.0;
     if (k >= 1) {
         const double g = -m.rdzw[k - 1] * bar_of(B, tend_u, i - 1) + m.rdzw[k] * gtu;
         const double erw = 0.5 * (F[rw][(long long)c1 * Lw + k] + F[rw][(long long)c2 * Lw + k]);
+        double gerw;
         if (k == 1 || k == L - 1) {
+            const double ui = m.fzm[k] * ue[k] + m.fzp[k] * ue[k - 1];
+            gerw = ui * g;
             g_u_own += erw * m.fzm[k] * g;
             c_m1 = erw * m.fzp[k] * g;
         } else {
@@ -177,22 +185,28 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
             c_m1 = d[1] * g;
             g_u_own += d[2] * g;
             c_p1 = d[3] * g;
+            gerw = d[4] * g;
         }
+        x_rw = 0.5 * gerw;
     }
     // ---- tend_u_euler
     const double gtue = bar_of(B, tend_u_euler, i) + gtu;
-    double x_dv = 0.0, g_dsu = 0.0;
+    double x_pp = 0.0, x_dp = 0.0, x_kd = 0.0, x_dd = 0.0, x_dv = 0.0, g_dsu = 0.0;
     if (!rk1) {
         if (act) add_own(B, tend_u_euler_in, i, gtue);
     } else {
         const double zzs = m.zz[(long long)c2 * L + k] + m.zz[(long long)c1 * L + k];
         const double ppd = F[pp][(long long)c2 * L + k] - F[pp][(long long)c1 * L + k];
         const double dps = S.dpdz[(long long)c1 * L + k] + S.dpdz[(long long)c2 * L + k];
+        const double cq = F[cqu][i];
         if (act) add_own(B, cqu, i, -(ppd * m.invDcEdge[e] / (0.5 * zzs) - 0.5 * m.zxu[i] * dps) * gtue);
+        x_pp = cq * m.invDcEdge[e] / (0.5 * zzs) * gtue;  // pp(c1) += x, pp(c2) -= x
+        x_dp = cq * 0.5 * m.zxu[i] * gtue;
         const double ud = S.delsq_u[i];
         const double kdu = 0.5 * (F[kdiff][(long long)c1 * L + k] + F[kdiff][(long long)c2 * L + k]);
         const double ms2 = m.meshScalingDel2[e];
         g_re += kdu * ud * ms2 * gtue;
+        x_kd = 0.5 * re * ud * ms2 * gtue;
         g_dsu = re * kdu * ms2 * gtue;
         if (P.h_mom_eddy_visc4 > 0.0) {
             const int v1 = m.verticesOnEdge[2 * e], v2 = m.verticesOnEdge[2 * e + 1];
@@ -202,6 +216,7 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
             const double dd = S.delsq_div[(long long)c2 * L + k] - S.delsq_div[(long long)c1 * L + k];
             const double dvv = S.delsq_vort[(long long)v2 * L + k] - S.delsq_vort[(long long)v1 * L + k];
             g_re += -(dd * r_dc - dvv * r_dv) * gtue;
+            x_dd = re * r_dc * gtue;
             x_dv = re * r_dv * gtue;
         }
     }
@@ -223,15 +238,34 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
     add_own(B, ru, i, g_ru_own + ((k + 1 <= L - 1) ? ru_from_p1 : 0.0));
     add_own(B, rho_edge, i, g_re);
     add_own(B, pv_edge, i, g_pv_own);
-    // the cell terms are gathered by k_adj_dyn_a3_cells; the vertices of the
-    // edge (del4 vorticity) take hardware FP64 atomics
+    // the two cells (and vertices) of the edge: hardware FP64 atomics
+    const long long o1 = (long long)c1 * L + k, o2 = (long long)c2 * L + k;
+    add(B, ke, o1, x_ke);
+    add(B, ke, o2, -x_ke);
+    acc(B[h_divergence] + o1, x_hd);
+    acc(B[h_divergence] + o2, x_hd);
+    if (k >= 1) {
+        add(B, rw, (long long)c1 * Lw + k, x_rw);
+        add(B, rw, (long long)c2 * Lw + k, x_rw);
+    }
     if (rk1) {
+        add(B, pp, o1, x_pp);
+        add(B, pp, o2, -x_pp);
+        acc(S.g_dpdz + o1, x_dp);
+        acc(S.g_dpdz + o2, x_dp);
+        acc(B[kdiff] + o1, x_kd);
+        acc(B[kdiff] + o2, x_kd);
         S.g_delsq_u[i] += g_dsu;
         if (P.h_mom_eddy_visc4 > 0.0) {
             const int v1 = m.verticesOnEdge[2 * e], v2 = m.verticesOnEdge[2 * e + 1];
+            acc(S.g_delsq_div + o1, x_dd);
+            acc(S.g_delsq_div + o2, -x_dd);
             acc(S.g_delsq_vort + (long long)v1 * L + k, -x_dv);
             acc(S.g_delsq_vort + (long long)v2 * L + k, x_dv);
         }
+    } else {
+        add(B, theta_m_save, o1, x_ths);
+        add(B, theta_m_save, o2, x_ths);
     }
 }
 

@@ -62,13 +62,25 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_diag_cells(long long nb, mpas
         }
         const int n = m.nEdgesOnCell[c];
         const double ia = m.invAreaCell[c];
+        int eo[kMaxEdges], vo[kMaxEdges];
+    #pragma unroll
+        for (int j = 0; j < kMaxEdges; ++j) {  // indices first: one wait for all scalar loads
+            eo[j] = m.edgesOnCell[c * ME + clamp_slot(j, n)];
+            vo[j] = m.verticesOnCell[c * ME + clamp_slot(j, n)];
+        }
+        double uo[kMaxEdges];
+    #pragma unroll
+        for (int j = 0; j < kMaxEdges; ++j) {
+            uo[j] = 0.0;
+            if (j < n) uo[j] = u[eo[j] * L + k];  // wave-uniform guard
+        }
         double d = 0.0, kc = 0.0;
     #pragma unroll
         for (int j = 0; j < kMaxEdges; ++j) {
             const int jj = clamp_slot(j, n);
             const int e = m.edgesOnCell[c * ME + jj];
             const double w = m.edgesOnCell_sign[c * ME + jj] * m.dvEdge[e];
-            const double ue = u[e * L + k], kee = (m.dcEdge[e] * m.dvEdge[e]) * (ue * ue);
+            const double ue = uo[j], kee = (m.dcEdge[e] * m.dvEdge[e]) * (ue * ue);
             const double cd = w * ue;
             const double ck = 0.25 * kee;
             d = (j < n) ? ((j == 0) ? cd : d + cd) : d;
@@ -78,12 +90,21 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_diag_cells(long long nb, mpas
         const double ke_fact = 1.0 - 0.375;
         double tot = ke_fact * (kc * ia);
         double pc = 0.0;
+        double kvv[kMaxEdges], pvv[kMaxEdges];
+    #pragma unroll
+        for (int j = 0; j < kMaxEdges; ++j) {
+            kvv[j] = pvv[j] = 0.0;
+            if (j < n) {
+                kvv[j] = ke_vertex[vo[j] * L + k];
+                pvv[j] = pv_vertex[vo[j] * L + k];
+            }
+        }
     #pragma unroll
         for (int j = 0; j < kMaxEdges; ++j) {
             const int jj = clamp_slot(j, n);
             const int vx = m.verticesOnCell[c * ME + jj];
             const double ka = m.kiteAreasOnVertex[3 * vx + m.kiteForCell[c * ME + jj]];
-            const double kv = ke_vertex[vx * L + k], pv = pv_vertex[vx * L + k];
+            const double kv = kvv[j], pv = pvv[j];
             double contrib = (1.0 - ke_fact) * ka;
             contrib = contrib * kv;
             contrib = contrib * ia;
@@ -131,7 +152,10 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_diag_edges2(long long nb, mpa
                 if (j0 >= ne) break;
                 double uj[8];
     #pragma unroll
-                for (int t = 0; t < 8; ++t) uj[t] = u[bcast(eev, j0 + t) * L + k];
+                for (int t = 0; t < 8; ++t) {
+                    uj[t] = 0.0;
+                    if (j0 + t < ne) uj[t] = u[bcast(eev, j0 + t) * L + k];  // wave-uniform guard
+                }
     #pragma unroll
                 for (int t = 0; t < 8; ++t) {
                     const int j = j0 + t;

@@ -62,13 +62,24 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_kdiff(long long nb, mpas_
             return;
         }
         const int n = m.nEdgesOnCell[c];
+        int eo[kMaxEdges];
+    #pragma unroll
+        for (int j = 0; j < kMaxEdges; ++j) eo[j] = m.edgesOnCell[c * ME + clamp_slot(j, n)];
+        double uv[kMaxEdges], vv[kMaxEdges];
+    #pragma unroll
+        for (int j = 0; j < kMaxEdges; ++j) {
+            uv[j] = vv[j] = 0.0;
+            if (j < n) {  // wave-uniform: slots past the live count skip the loads
+                uv[j] = u[eo[j] * L + k];
+                vv[j] = v[eo[j] * L + k];
+            }
+        }
         double dd = 0.0, dof = 0.0;
     #pragma unroll
         for (int j = 0; j < kMaxEdges; ++j) {
             const int jj = clamp_slot(j, n);
-            const int e = m.edgesOnCell[c * ME + jj];
             const double a = m.defc_a[c * ME + jj], b = m.defc_b[c * ME + jj];
-            const double us = u[e * L + k], vs = v[e * L + k];
+            const double us = uv[j], vs = vv[j];
             const double ndd = (dd + a * us) - b * vs;
             const double ndof = (dof + b * us) + a * vs;
             dd = (j < n) ? ndd : dd;
@@ -95,7 +106,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_a(long long nb, mpa
     auto body_ = [&](const int c) {
         const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
         const int k = wave_lane();
-        if (c > nC || k >= Lw) return;
+        if (c > nC) return;  // wave-uniform
         const int i = c * Lw + k;
         const bool rk1 = (P.rk_step == 1);
         if (c == nC) {
@@ -104,67 +115,109 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_a(long long nb, mpa
         }
         const int n = m.nEdgesOnCell[c];
         const double ia = m.invAreaCell[c];
+        const int oc = c * L, ow = c * Lw;
+        const int kl = k < L ? k : L - 1, kw = k < Lw ? k : L;  // clamped into the column
+        int eo[kMaxEdges], nb_[kMaxEdges];
+        double sg[kMaxEdges];
+        bool own2[kMaxEdges];  // this cell is the edge's second cell
+    #pragma unroll
+        for (int j = 0; j < kMaxEdges; ++j) {
+            const int jj = clamp_slot(j, n);
+            eo[j] = m.edgesOnCell[c * ME + jj];
+            sg[j] = m.edgesOnCell_sign[c * ME + jj];
+            const int c1 = m.cellsOnEdge[2 * eo[j]], c2 = m.cellsOnEdge[2 * eo[j] + 1];
+            own2[j] = (c2 == c);
+            nb_[j] = own2[j] ? c1 : c2;
+        }
+        // slots past the live count skip their loads (wave-uniform guards)
+        double rv[kMaxEdges];
+    #pragma unroll
+        for (int j = 0; j < kMaxEdges; ++j) {
+            rv[j] = 0.0;
+            if (j < n) rv[j] = ru[eo[j] * L + kl];
+        }
         if (k < L) {
             double s = 0.0;
     #pragma unroll
             for (int j = 0; j < kMaxEdges; ++j) {
-                const int jj = clamp_slot(j, n);
-                const int e = m.edgesOnCell[c * ME + jj];
-                const double wgt = m.edgesOnCell_sign[c * ME + jj] * m.dvEdge[e];
-                const double r = ru[e * L + k];
-                const double cc = wgt * r;
+                const double wgt = sg[j] * m.dvEdge[eo[j]];
+                const double cc = wgt * rv[j];
                 s = (j < n) ? ((j == 0) ? cc : s + cc) : s;
             }
             double hd = s * ia;
-            h_div[c * L + k] = hd;
+            h_div[oc + k] = hd;
             if (rk1) {
-                const int o = c * L + k;
-                tend_rho[o] = -hd - m.rdzw[k] * (rw[c * Lw + k + 1] - rw[c * Lw + k]);
+                const int o = oc + k;
+                tend_rho[o] = -hd - m.rdzw[k] * (rw[ow + k + 1] - rw[ow + k]);
                 dpdz[o] = -MPAS_GRAVITY * (m.rho_base[o] * qtot[o] + rr_save[o] * (1.0 + qtot[o]));
-                // theta del2 (dyn_tend.py:5563-5640)
-                double dsum = 0.0, tsum = 0.0;
-    #pragma unroll
-                for (int j = 0; j < kMaxEdges; ++j) {
-                    const int jj = clamp_slot(j, n);
-                    const int e = m.edgesOnCell[c * ME + jj];
-                    const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-                    const double t2 = theta_m[c2 * L + k], t1 = theta_m[c1 * L + k], re = rho_edge[e * L + k];
-                    const double k1 = kdiff[c1 * L + k], k2 = kdiff[c2 * L + k];
-                    const double es = ia * m.edgesOnCell_sign[c * ME + jj] * m.dvEdge[e] * m.invDcEdge[e];
-                    const double prs = P.prandtl_inv * m.meshScalingDel2[e];
-                    const double bf = es * (t2 - t1) * re;
-                    const double sf = bf * 0.5 * (k1 + k2) * prs;
-                    dsum = (j < n) ? ((j == 0) ? bf : dsum + bf) : dsum;
-                    tsum = (j < n) ? ((j == 0) ? sf : tsum + sf) : tsum;
-                }
-                delsq_theta[o] = dsum;
-                tend_theta_euler[o] = tsum;
             }
         }
-        if (rk1) {
-            if (k >= 1 && k < L) {
-                // w del2 (dyn_tend.py:5315-5420)
-                double dsum = 0.0, tsum = 0.0;
+        if (!rk1) return;  // wave-uniform
+        // First stage: del2 of theta (levels, dyn_tend.py:5563-5640) and of w
+        // (interfaces, :5315-5420).  Each slot loads the other cell's theta,
+        // kdiff and w and the edge's rho_edge at this lane's level; the own
+        // cell's values are loaded once, and the level k-1 values the
+        // interface stencil needs come from lane k-1.  All lanes run the loads
+        // and shuffles.
+        const double th0 = theta_m[oc + kl], kd0 = kdiff[oc + kl], w0 = w[ow + kw];
+        double re[kMaxEdges], thn[kMaxEdges], kn[kMaxEdges], wn[kMaxEdges];
     #pragma unroll
-                for (int j = 0; j < kMaxEdges; ++j) {
-                    const int jj = clamp_slot(j, n);
-                    const int e = m.edgesOnCell[c * ME + jj];
-                    const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-                    const double r0 = rho_edge[e * L + k], r1 = rho_edge[e * L + k - 1];
-                    const double w2 = w[c2 * Lw + k], w1 = w[c1 * Lw + k];
-                    const double k1 = kdiff[c1 * L + k], k2 = kdiff[c2 * L + k];
-                    const double k1m = kdiff[c1 * L + k - 1], k2m = kdiff[c2 * L + k - 1];
-                    const double es = 0.5 * ia * m.edgesOnCell_sign[c * ME + jj] * m.dvEdge[e] * m.invDcEdge[e];
-                    const double bf = es * (r0 + r1) * (w2 - w1);
-                    const double sf = bf * m.meshScalingDel2[e] * 0.25 * (k1 + k2 + k1m + k2m);
-                    dsum = (j < n) ? ((j == 0) ? bf : dsum + bf) : dsum;
-                    tsum = (j < n) ? ((j == 0) ? sf : tsum + sf) : tsum;
-                }
-                delsq_w[i] = dsum;
-                tend_w_euler[i] = tsum;
-            } else {
-                tend_w_euler[i] = 0.0;
+        for (int j = 0; j < kMaxEdges; ++j) {
+            re[j] = thn[j] = kn[j] = wn[j] = 0.0;
+            if (j < n) {
+                re[j] = rho_edge[eo[j] * L + kl];
+                thn[j] = theta_m[nb_[j] * L + kl];
+                kn[j] = kdiff[nb_[j] * L + kl];
+                wn[j] = w[nb_[j] * Lw + kw];
             }
+        }
+        const double kd0_m1 = __shfl_up(kd0, 1);
+        double re_m1[kMaxEdges], kn_m1[kMaxEdges];
+    #pragma unroll
+        for (int j = 0; j < kMaxEdges; ++j) {
+            re_m1[j] = kn_m1[j] = 0.0;
+            if (j < n) {
+                re_m1[j] = __shfl_up(re[j], 1);
+                kn_m1[j] = __shfl_up(kn[j], 1);
+            }
+        }
+        if (k < L) {
+            const int o = oc + k;
+            double dsum = 0.0, tsum = 0.0;
+    #pragma unroll
+            for (int j = 0; j < kMaxEdges; ++j) {
+                const int e = eo[j];
+                const double t2 = own2[j] ? th0 : thn[j], t1 = own2[j] ? thn[j] : th0;
+                const double k1 = own2[j] ? kn[j] : kd0, k2 = own2[j] ? kd0 : kn[j];
+                const double es = ia * sg[j] * m.dvEdge[e] * m.invDcEdge[e];
+                const double prs = P.prandtl_inv * m.meshScalingDel2[e];
+                const double bf = es * (t2 - t1) * re[j];
+                const double sf = bf * 0.5 * (k1 + k2) * prs;
+                dsum = (j < n) ? ((j == 0) ? bf : dsum + bf) : dsum;
+                tsum = (j < n) ? ((j == 0) ? sf : tsum + sf) : tsum;
+            }
+            delsq_theta[o] = dsum;
+            tend_theta_euler[o] = tsum;
+        }
+        if (k >= 1 && k < L) {
+            double dsum = 0.0, tsum = 0.0;
+    #pragma unroll
+            for (int j = 0; j < kMaxEdges; ++j) {
+                const int e = eo[j];
+                const double r0 = re[j], r1 = re_m1[j];
+                const double w2 = own2[j] ? w0 : wn[j], w1 = own2[j] ? wn[j] : w0;
+                const double k1 = own2[j] ? kn[j] : kd0, k2 = own2[j] ? kd0 : kn[j];
+                const double k1m = own2[j] ? kn_m1[j] : kd0_m1, k2m = own2[j] ? kd0_m1 : kn_m1[j];
+                const double es = 0.5 * ia * sg[j] * m.dvEdge[e] * m.invDcEdge[e];
+                const double bf = es * (r0 + r1) * (w2 - w1);
+                const double sf = bf * m.meshScalingDel2[e] * 0.25 * (k1 + k2 + k1m + k2m);
+                dsum = (j < n) ? ((j == 0) ? bf : dsum + bf) : dsum;
+                tsum = (j < n) ? ((j == 0) ? sf : tsum + sf) : tsum;
+            }
+            delsq_w[i] = dsum;
+            tend_w_euler[i] = tsum;
+        } else if (k < Lw) {
+            tend_w_euler[i] = 0.0;
         }
     };
     const int base_ = wave_entity_base(nb);
@@ -200,16 +253,21 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_a(long long nb, mpa
         const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
         const hreal *ue = u + e * L;
         const double *fzm = m.fzm, *fzp = m.fzp;
-        auto wduz = [&](int kk) -> double {
-            double erw = 0.5 * (rw[c1 * Lw + kk] + rw[c2 * Lw + kk]);
-            if (kk == 1 || kk == L - 1) return erw * (fzm[kk] * ue[kk] + fzp[kk] * ue[kk - 1]);
-            return flux3(ue[kk - 2], ue[kk - 1], ue[kk], ue[kk + 1], erw, 1.0);
-        };
+        // vertical flux of u at this lane's interface k (1..L-1); the stencil
+        // values u(k-2), u(k-1), u(k+1) are the neighbouring lanes' own u,
+        // exchanged by shuffles instead of re-read through the vector cache
+        const double u0 = ue[k];
+        const double um1 = __shfl_up(u0, 1), um2 = __shfl_up(u0, 2), up1 = __shfl_down(u0, 1);
+        double wz = 0.0;
+        if (k >= 1) {
+            const double erw = 0.5 * (rw[c1 * Lw + k] + rw[c2 * Lw + k]);
+            wz = (k == 1 || k == L - 1) ? erw * (fzm[k] * u0 + fzp[k] * um1) : flux3(um2, um1, u0, up1, erw, 1.0);
+        }
+        const double wz_up = __shfl_down(wz, 1);  // the flux at interface k + 1
         double delta;
-        if (L == 1) delta = 0.0;
-        else if (k == 0) delta = wduz(1);
-        else if (k == L - 1) delta = -wduz(L - 1);
-        else delta = wduz(k + 1) - wduz(k);
+        if (k == 0) delta = wz_up;
+        else if (k == L - 1) delta = -wz;
+        else delta = wz_up - wz;
         double vert = -m.rdzw[k] * delta;
         double q = 0.0;
         const double pvs = pv_edge[i];
@@ -219,9 +277,14 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_a(long long nb, mpa
             double pj[8], uj[8];
     #pragma unroll
             for (int t = 0; t < 8; ++t) {
-                const int ee = bcast(eev, j0 + t);
-                pj[t] = pv_edge[ee * L + k];
-                uj[t] = u[ee * L + k];
+                // slots past the live count skip the load (wave-uniform branch)
+                pj[t] = 0.0;
+                uj[t] = 0.0;
+                if (j0 + t < ne) {
+                    const int ee = bcast(eev, j0 + t);
+                    pj[t] = pv_edge[ee * L + k];
+                    uj[t] = u[ee * L + k];
+                }
             }
     #pragma unroll
             for (int t = 0; t < 8; ++t) {
@@ -350,9 +413,13 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edge_adv(long long nb, mp
             double qt[8], qw[8];
     #pragma unroll
             for (int t = 0; t < 8; ++t) {
-                const int ac = bcast(advv, a0 + t);
-                qt[t] = theta_m[ac * L + k];
-                qw[t] = w[ac * Lw + k];
+                qt[t] = 0.0;
+                qw[t] = 0.0;
+                if (a0 + t < na) {  // wave-uniform: slots past the live count skip the load
+                    const int ac = bcast(advv, a0 + t);
+                    qt[t] = theta_m[ac * L + k];
+                    qw[t] = w[ac * Lw + k];
+                }
             }
     #pragma unroll
             for (int t = 0; t < 8; ++t) {
@@ -396,10 +463,10 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_b(long long nb, mpa
     auto body_ = [&](const int c) {
         const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
         const int k = wave_lane();
-        if (c > nC || k >= Lw) return;
+        if (c > nC) return;  // wave-uniform
         const int i = c * Lw + k;
         if (c == nC) {
-            tend_w[i] = 0.0;
+            if (k < Lw) tend_w[i] = 0.0;
             if (k < L) {
                 tend_theta[c * L + k] = 0.0;
                 rthdynten[c * L + k] = 0.0;
@@ -410,48 +477,95 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_b(long long nb, mpa
         const int n = m.nEdgesOnCell[c];
         const double ia = m.invAreaCell[c];
         const double *fzm = m.fzm, *fzp = m.fzp;
+        const int oc = c * L, ow = c * Lw;
+        // Each lane loads its own level / interface of the column (clamped into
+        // the column for the spare lanes); the neighbouring levels the vertical
+        // stencils need come from the neighbouring lanes by shuffles, and each
+        // lane forms the vertical flux at its own interface only.  All lanes
+        // run this part.
+        const int kl = k < L ? k : L - 1, kw = k < Lw ? k : L;
+        const double w0 = w[ow + kw], rw0 = rw[ow + kw], rws0 = rw_save[ow + kw];
+        const double th0 = theta_m[oc + kl], ths0 = theta_m_save[oc + kl];
+        const double w_m1 = __shfl_up(w0, 1), w_p1 = __shfl_down(w0, 1), w_p2 = __shfl_down(w0, 2);
+        const double rw_p1 = __shfl_down(rw0, 1);
+        const double th_m2 = __shfl_up(th0, 2), th_m1 = __shfl_up(th0, 1), th_p1 = __shfl_down(th0, 1);
+        const double ths_m1 = __shfl_up(ths0, 1);
+        // vertical flux of w at mass level k (0..L-2)
+        double wz = 0.0;
+        if (k <= L - 2) {
+            wz = (k == 0 || k == L - 2) ? 0.25 * (rw_p1 + rw0) * (w_p1 + w0)
+                                        : flux3(w_m1, w0, w_p1, w_p2, 0.5 * (rw_p1 + rw0), 1.0);
+        }
+        const double wz_m1 = __shfl_up(wz, 1);
+        // vertical flux of theta at interface k (1..L-1)
+        double tz = 0.0;
+        if (k >= 1 && k <= L - 1) {
+            if (k == 1)
+                tz = rw0 * (fzm[k] * th0 + fzp[k] * th_m1) + (rws0 - rw0) * (fzm[k] * ths0 + fzp[k] * ths_m1);
+            else if (k == L - 1)
+                tz = rws0 * (fzm[k] * th0 + fzp[k] * th_m1);
+            else
+                tz = flux3(th_m2, th_m1, th0, th_p1, rw0, P.coef_3rd_order) +
+                     (rws0 - rw0) * (fzm[k] * ths0 + fzp[k] * ths_m1);
+        }
+        const double tz_p1 = __shfl_down(tz, 1);
+        double pp0 = 0.0, dp0 = 0.0;
+        if (rk1) {
+            pp0 = pp[oc + kl];
+            dp0 = dpdz[oc + kl];
+        }
+        const double pp_m1 = __shfl_up(pp0, 1), dp_m1 = __shfl_up(dp0, 1);
+        if (k >= Lw) return;
         int eo[kMaxEdges];
         double sg[kMaxEdges];
+        int nb_[kMaxEdges];  // the other cell of each edge
     #pragma unroll
         for (int j = 0; j < kMaxEdges; ++j) {
             const int jj = clamp_slot(j, n);
             eo[j] = m.edgesOnCell[c * ME + jj];
             sg[j] = m.edgesOnCell_sign[c * ME + jj];
+            const int c1 = m.cellsOnEdge[2 * eo[j]], c2 = m.cellsOnEdge[2 * eo[j] + 1];
+            nb_[j] = (c1 == c) ? c2 : c1;
         }
         // ---------------- w at interface k
         if (k >= 1 && k < L) {
+            double fv[kMaxEdges];
+    #pragma unroll
+            for (int j = 0; j < kMaxEdges; ++j) {
+                fv[j] = 0.0;
+                if (j < n) fv[j] = wflux[eo[j] * L + k];  // wave-uniform guard
+            }
             double hadv = 0.0;
     #pragma unroll
             for (int j = 0; j < kMaxEdges; ++j) {
-                const double f = wflux[eo[j] * L + k];
-                const double cc = -sg[j] * f;
+                const double cc = -sg[j] * fv[j];
                 hadv = (j < n) ? ((j == 0) ? cc : hadv + cc) : hadv;
             }
             double eu = tend_w_euler[i];
             if (rk1) {
                 if (P.h_mom_eddy_visc4 > 0.0) {
                     double r_area = P.h_mom_eddy_visc4 * ia;
+                    const double dsw = delsq_w[ow + k];
+                    double dn[kMaxEdges];
+    #pragma unroll
+                    for (int j = 0; j < kMaxEdges; ++j) {
+                        dn[j] = 0.0;
+                        if (j < n) dn[j] = delsq_w[nb_[j] * Lw + k];
+                    }
     #pragma unroll
                     for (int j = 0; j < kMaxEdges; ++j) {
                         const int e = eo[j];
-                        const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-                        const double d2 = delsq_w[c2 * Lw + k], d1 = delsq_w[c1 * Lw + k];
+                        const bool own2 = (m.cellsOnEdge[2 * e + 1] == c);
+                        const double d2 = own2 ? dsw : dn[j], d1 = own2 ? dn[j] : dsw;
                         const double es = m.meshScalingDel4[e] * r_area * m.dvEdge[e] * sg[j] * m.invDcEdge[e];
                         const double nx = eu - es * (d2 - d1);
                         eu = (j < n) ? nx : eu;
                     }
                 }
-                const int oc = c * L;
-                eu = eu - cqw[i] * (m.rdzu[k] * (pp[oc + k] - pp[oc + k - 1]) -
-                                    (fzm[k] * dpdz[oc + k] + fzp[k] * dpdz[oc + k - 1]));
+                eu = eu - cqw[i] * (m.rdzu[k] * (pp0 - pp_m1) - (fzm[k] * dp0 + fzp[k] * dp_m1));
                 tend_w_euler[i] = eu;
             }
-            const double *wc = w + c * Lw, *rwc = rw + c * Lw;
-            auto wdwz = [&](int jj) -> double {
-                if (jj == 0 || jj == L - 2) return 0.25 * (rwc[jj + 1] + rwc[jj]) * (wc[jj + 1] + wc[jj]);
-                return flux3(wc[jj - 1], wc[jj], wc[jj + 1], wc[jj + 2], 0.5 * (rwc[jj + 1] + rwc[jj]), 1.0);
-            };
-            double delta = (k == L - 1) ? -wdwz(L - 2) : wdwz(k) - wdwz(k - 1);
+            const double delta = (k == L - 1) ? -wz_m1 : wz - wz_m1;
             double inner = hadv * ia - m.rdzu[k] * delta;
             tend_w[i] = inner + eu;
         } else {
@@ -460,52 +574,57 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_b(long long nb, mpa
         // ---------------- theta at level k
         if (k < L) {
             const int o = c * L + k;
+            double fv[kMaxEdges];
+    #pragma unroll
+            for (int j = 0; j < kMaxEdges; ++j) {
+                fv[j] = 0.0;
+                if (j < n) fv[j] = tflux[eo[j] * L + k];
+            }
             double thadv = 0.0;
     #pragma unroll
             for (int j = 0; j < kMaxEdges; ++j) {
-                const double f = tflux[eo[j] * L + k];
-                const double cc = -sg[j] * f;
+                const double cc = -sg[j] * fv[j];
                 thadv = (j < n) ? ((j == 0) ? cc : thadv + cc) : thadv;
             }
             if (!rk1) {
     #pragma unroll
                 for (int j = 0; j < kMaxEdges; ++j) {
-                    const double f = tpert[eo[j] * L + k];
-                    const double nx = thadv - sg[j] * f;
+                    fv[j] = 0.0;
+                    if (j < n) fv[j] = tpert[eo[j] * L + k];
+                }
+    #pragma unroll
+                for (int j = 0; j < kMaxEdges; ++j) {
+                    const double nx = thadv - sg[j] * fv[j];
                     thadv = (j < n) ? nx : thadv;
                 }
             }
             double tte = tend_theta_euler[o];
             if (rk1 && P.h_theta_eddy_visc4 > 0.0) {
                 double r_area = P.h_theta_eddy_visc4 * P.prandtl_inv * ia;
+                const double dst = delsq_theta[o];
+                double dn[kMaxEdges];
+    #pragma unroll
+                for (int j = 0; j < kMaxEdges; ++j) {
+                    dn[j] = 0.0;
+                    if (j < n) dn[j] = delsq_theta[nb_[j] * L + k];
+                }
     #pragma unroll
                 for (int j = 0; j < kMaxEdges; ++j) {
                     const int e = eo[j];
-                    const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-                    const double d2 = delsq_theta[c2 * L + k], d1 = delsq_theta[c1 * L + k];
+                    const bool own2 = (m.cellsOnEdge[2 * e + 1] == c);
+                    const double d2 = own2 ? dst : dn[j], d1 = own2 ? dn[j] : dst;
                     const double es = m.meshScalingDel4[e] * r_area * m.dvEdge[e] * sg[j] * m.invDcEdge[e];
                     const double nx = tte - es * (d2 - d1);
                     tte = (j < n) ? nx : tte;
                 }
                 tend_theta_euler[o] = tte;
             }
-            const double *th = theta_m + c * L, *ths = theta_m_save + c * L;
-            const double *rwc = rw + c * Lw, *rws = rw_save + c * Lw;
-            auto wdtz = [&](int kk) -> double {
-                if (kk == 1)
-                    return rwc[kk] * (fzm[kk] * th[kk] + fzp[kk] * th[kk - 1]) +
-                           (rws[kk] - rwc[kk]) * (fzm[kk] * ths[kk] + fzp[kk] * ths[kk - 1]);
-                if (kk == L - 1) return rws[kk] * (fzm[kk] * th[kk] + fzp[kk] * th[kk - 1]);
-                return flux3(th[kk - 2], th[kk - 1], th[kk], th[kk + 1], rwc[kk], P.coef_3rd_order) +
-                       (rws[kk] - rwc[kk]) * (fzm[kk] * ths[kk] + fzp[kk] * ths[kk - 1]);
-            };
             double delta;
-            if (L == 1) delta = 0.0;
-            else if (k == 0) delta = wdtz(1);
-            else if (k == L - 1) delta = -wdtz(L - 1);
-            else delta = wdtz(k + 1) - wdtz(k);
+            if (k == 0) delta = tz_p1;
+            else if (k == L - 1) delta = -tz;
+            else delta = tz_p1 - tz;
             double base = thadv * ia - m.rdzw[k] * delta;
-            rthdynten[o] = (base - tend_rho[o] * th[k]) / rho_zz[o];
+            rthdynten[o] = (base - tend_rho[o] * th0) / rho_zz[o];
             double tt = base + rho_zz[o] * rt_diab[o];
             tend_theta[o] = tt + tte;
         }
