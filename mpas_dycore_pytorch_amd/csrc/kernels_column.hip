@@ -134,17 +134,23 @@ __global__ __launch_bounds__(64 * kColWaves) void k_vimp(
         a_tri[o + k] = a;
     }
     __syncthreads();
-    // stage 3: Thomas factorisation, lane 0 of each column
+    // stage 3: Thomas factorisation, lane 0 of each column.  Each factor is
+    // written over the LDS row just read (b -> alpha, c -> gamma) and the
+    // wavefront stores the column once, coalesced, instead of lane 0
+    // storing one element at a time.
     if (live && k == 0) {
         double g = 0.0;
-        alpha_tri[o] = 0.0;
-        gamma_tri[o] = 0.0;
         for (int kk = 1; kk < L; ++kk) {
             double al = 1.0 / (s_b[w][kk] - s_a[w][kk] * g);
             g = s_c[w][kk] * al;
-            alpha_tri[o + kk] = al;
-            gamma_tri[o + kk] = g;
+            s_b[w][kk] = al;
+            s_c[w][kk] = g;
         }
+    }
+    __syncthreads();
+    if (live && k < L) {
+        alpha_tri[o + k] = (k == 0) ? 0.0 : s_b[w][k];
+        gamma_tri[o + k] = (k == 0) ? 0.0 : s_c[w][k];
     }
 }
 
