@@ -98,15 +98,28 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
     const int na = m.nAdvCellsForEdge[e];
     double g_ru_own = 0.0, g_ru_dn = 0.0;  // to ru(k) and ru(k-1)
     double x_ths = 0.0;
+    // the stencil gathers of each slot loop are all issued before the first
+    // is used (a loop that loads and accumulates slot by slot waits one full
+    // memory round trip per slot)
+    int acv[kMaxAdv];
+#pragma unroll
+    for (int a = 0; a < kMaxAdv; ++a) acv[a] = m.advCellsForEdge[e * MA + (a < na ? a : na - 1)];
     // ---- theta edge flux
     {
         const double gf = -(s1 * S.g_thadv[(long long)c1 * L + k] + s2 * S.g_thadv[(long long)c2 * L + k]);
         const double rut = ru_[k], srt = sign_pm(rut);
+        double qv[kMaxAdv];
+#pragma unroll
+        for (int a = 0; a < kMaxAdv; ++a) {
+            qv[a] = 0.0;
+            if (a < na) qv[a] = F[theta_m][(long long)acv[a] * L + k];
+        }
         double ft = 0.0;
-        for (int a = 0; a < na; ++a) {
-            const int ac = m.advCellsForEdge[e * MA + a];
+#pragma unroll
+        for (int a = 0; a < kMaxAdv; ++a) {
+            if (a >= na) break;
             const double wgt = m.adv_coefs[e * MA + a] + srt * m.adv_coefs_3rd[e * MA + a];
-            const double cc = wgt * F[theta_m][(long long)ac * L + k];
+            const double cc = wgt * qv[a];
             ft = (a == 0) ? cc : ft + cc;
         }
         if (act) S.g_tf[i] = gf;
@@ -124,11 +137,18 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
         const double gf = -(s1 * S.g_hadv[(long long)c1 * Lw + k] + s2 * S.g_hadv[(long long)c2 * Lw + k]);
         const double ruw = m.fzm[k] * ru_[k] + m.fzp[k] * ru_[k - 1];
         const double srw = sign_pm(ruw);
+        double qv[kMaxAdv];
+#pragma unroll
+        for (int a = 0; a < kMaxAdv; ++a) {
+            qv[a] = 0.0;
+            if (a < na) qv[a] = F[w][(long long)acv[a] * Lw + k];
+        }
         double fw = 0.0;
-        for (int a = 0; a < na; ++a) {
-            const int ac = m.advCellsForEdge[e * MA + a];
+#pragma unroll
+        for (int a = 0; a < kMaxAdv; ++a) {
+            if (a >= na) break;
             const double wgt = m.adv_coefs[e * MA + a] + srw * m.adv_coefs_3rd[e * MA + a];
-            const double cc = wgt * F[w][(long long)ac * Lw + k];
+            const double cc = wgt * qv[a];
             fw = (a == 0) ? cc : fw + cc;
         }
         if (act) S.g_wf[i] = gf;
@@ -144,12 +164,22 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
     {
         const int ne = m.nEdgesOnEdge[e];
         const double pvs = F[pv_edge][i];
+        double uv[kMaxEdges2], pv[kMaxEdges2];
+#pragma unroll
+        for (int j = 0; j < kMaxEdges2; ++j) {
+            uv[j] = pv[j] = 0.0;
+            if (j < ne) {
+                const long long io = (long long)m.edgesOnEdge[e * ME2 + j] * L + k;
+                uv[j] = F[u][io];
+                pv[j] = F[pv_edge][io];
+            }
+        }
         double q = 0.0;
-        for (int j = 0; j < ne; ++j) {
-            const int ee = m.edgesOnEdge[e * ME2 + j];
+#pragma unroll
+        for (int j = 0; j < kMaxEdges2; ++j) {
+            if (j >= ne) break;
             const double wv = m.weightsOnEdge[e * ME2 + j];
-            const double uj = F[u][(long long)ee * L + k], pj = F[pv_edge][(long long)ee * L + k];
-            const double cc = wv * uj * (0.5 * (pvs + pj));
+            const double cc = wv * uv[j] * (0.5 * (pvs + pv[j]));
             q = (j == 0) ? cc : q + cc;
         }
         const double ked = F[ke][(long long)c2 * L + k] - F[ke][(long long)c1 * L + k];
@@ -160,9 +190,10 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
         g_u_own += -0.5 * hsum * gtu;
         x_hd = -ue[k] * 0.5 * gtu;
         double gpe = 0.0;
-        for (int j = 0; j < ne; ++j) {
-            const int ee = m.edgesOnEdge[e * ME2 + j];
-            gpe += m.weightsOnEdge[e * ME2 + j] * F[u][(long long)ee * L + k] * 0.5 * gq;
+#pragma unroll
+        for (int j = 0; j < kMaxEdges2; ++j) {
+            if (j >= ne) break;
+            gpe += m.weightsOnEdge[e * ME2 + j] * uv[j] * 0.5 * gq;
         }
         g_pv_own += gpe;
         if (act) S.g_q[i] = gq;
