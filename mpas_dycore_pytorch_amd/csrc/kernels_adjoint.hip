@@ -340,13 +340,31 @@ __global__ __launch_bounds__(64 * kColWaves) void k_adj_acoustic_colw(mpas_hip_m
         s_fa[wv][k] = F[a_tri][oc + k];
         s_fal[wv][k] = F[alpha_tri][oc + k];
         s_fga[wv][k] = F[gamma_tri][oc + k];
+        // indices first, then every slot's loads, then the sums in slot order
+        int eo[kMaxEdges], c1v[kMaxEdges], c2v[kMaxEdges];
+#pragma unroll
+        for (int i = 0; i < kMaxEdges; ++i) {
+            eo[i] = m.edgesOnCell[c * ME + clamp_slot(i, n)];
+            c1v[i] = m.cellsOnEdge[2 * eo[i]];
+            c2v[i] = m.cellsOnEdge[2 * eo[i] + 1];
+        }
+        double rp[kMaxEdges], t1v[kMaxEdges], t2v[kMaxEdges];
+#pragma unroll
+        for (int i = 0; i < kMaxEdges; ++i) {
+            rp[i] = t1v[i] = t2v[i] = 0.0;
+            if (i < n) {
+                rp[i] = F[ru_p][(long long)eo[i] * L + k];
+                t2v[i] = F[theta_m][(long long)c2v[i] * L + k];
+                t1v[i] = F[theta_m][(long long)c1v[i] * L + k];
+            }
+        }
         double rsum = 0.0, tsum = 0.0;
-        for (int i = 0; i < n; ++i) {
-            const int e = m.edgesOnCell[c * ME + i];
-            const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-            const double metric = m.edgesOnCell_sign[c * ME + i] * m.dvEdge[e];
-            const double flux = dts * metric * F[ru_p][(long long)e * L + k] * ia;
-            const double thedge = 0.5 * (F[theta_m][(long long)c2 * L + k] + F[theta_m][(long long)c1 * L + k]);
+#pragma unroll
+        for (int i = 0; i < kMaxEdges; ++i) {
+            if (i >= n) break;
+            const double metric = m.edgesOnCell_sign[c * ME + i] * m.dvEdge[eo[i]];
+            const double flux = dts * metric * rp[i] * ia;
+            const double thedge = 0.5 * (t2v[i] + t1v[i]);
             const double rcon = -flux, tcon = -(flux * thedge);
             rsum = (i == 0) ? rcon : rsum + rcon;
             tsum = (i == 0) ? tcon : tsum + tcon;
@@ -502,11 +520,21 @@ __global__ __launch_bounds__(64 * kColWaves) void k_adj_acoustic_colw(mpas_hip_m
         const double grs_k = s_grs[wv][k], gts_k = s_gts[wv][k];
         A.col[oc + k] = grs_k;
         A.col[(long long)(nC + 1) * L + oc + k] = gts_k;
+        int eo[kMaxEdges];
+#pragma unroll
+        for (int i = 0; i < kMaxEdges; ++i) eo[i] = m.edgesOnCell[c * ME + clamp_slot(i, n)];
+        double rp[kMaxEdges];
+#pragma unroll
+        for (int i = 0; i < kMaxEdges; ++i) {
+            rp[i] = 0.0;
+            if (i < n) rp[i] = F[ru_p][(long long)eo[i] * L + k];
+        }
         double gth = 0.0;
-        for (int i = 0; i < n; ++i) {
-            const int e = m.edgesOnCell[c * ME + i];
-            const double metric = m.edgesOnCell_sign[c * ME + i] * m.dvEdge[e];
-            const double flux = dts * metric * F[ru_p][(long long)e * L + k] * ia;
+#pragma unroll
+        for (int i = 0; i < kMaxEdges; ++i) {
+            if (i >= n) break;
+            const double metric = m.edgesOnCell_sign[c * ME + i] * m.dvEdge[eo[i]];
+            const double flux = dts * metric * rp[i] * ia;
             gth += 0.5 * (-flux * gts_k);
         }
         add_own(B, theta_m, oc + k, gth);
@@ -552,14 +580,29 @@ __global__ __launch_bounds__(256) void k_adj_acoustic_theta(long long nb, long l
     const int L = m.nLevels, ME = m.maxEdges;
     const long long gts0 = (long long)(m.nCells + 1) * L;
     const int n = m.nEdgesOnCell[c];
+    int eo[kMaxEdges], cnv[kMaxEdges];
+#pragma unroll
+    for (int i = 0; i < kMaxEdges; ++i) {
+        eo[i] = m.edgesOnCell[c * ME + clamp_slot(i, n)];
+        const int c1 = m.cellsOnEdge[2 * eo[i]], c2 = m.cellsOnEdge[2 * eo[i] + 1];
+        cnv[i] = (c1 == c) ? c2 : c1;
+    }
+    double rp[kMaxEdges], gt[kMaxEdges];
+#pragma unroll
+    for (int i = 0; i < kMaxEdges; ++i) {
+        rp[i] = gt[i] = 0.0;
+        if (i < n) {
+            rp[i] = F[ru_p][(long long)eo[i] * L + k];
+            gt[i] = A.col[gts0 + (long long)cnv[i] * L + k];
+        }
+    }
     double gth = 0.0;
-    for (int i = 0; i < n; ++i) {
-        const int e = m.edgesOnCell[c * ME + i];
-        const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-        const int cn = (c1 == c) ? c2 : c1;
-        const double metric_n = -m.edgesOnCell_sign[c * ME + i] * m.dvEdge[e];  // sign of e in cn
-        const double flux = A.ap.dts * metric_n * F[ru_p][(long long)e * L + k] * m.invAreaCell[cn];
-        gth += 0.5 * (-flux * A.col[gts0 + (long long)cn * L + k]);
+#pragma unroll
+    for (int i = 0; i < kMaxEdges; ++i) {
+        if (i >= n) break;
+        const double metric_n = -m.edgesOnCell_sign[c * ME + i] * m.dvEdge[eo[i]];  // sign of e in cn
+        const double flux = A.ap.dts * metric_n * rp[i] * m.invAreaCell[cnv[i]];
+        gth += 0.5 * (-flux * gt[i]);
     }
     add_own(S_.b, theta_m, (long long)c * L + k, gth);
 }
