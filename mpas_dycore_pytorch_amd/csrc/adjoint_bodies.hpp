@@ -82,33 +82,9 @@ namespace smlstep {
 enum { u_tend, w_tend_in, w_tend_out, N };
 }
 
-// thread (cell c, interface k), c in [0, nCells], k in [0, L]
-__device__ __forceinline__ void smlstep_vjp(const mpas_hip_mesh &m, Fwd F, Bar B, int c, int k) {
-    using namespace smlstep;
-    const int L = m.nLevels, ME = m.maxEdges, Lw = L + 1;
-    const long long i = (long long)c * Lw + k;
-    const double g = bar_of(B, w_tend_out, i);
-    if (c >= m.nCells || k < 1 || k >= L) {  // untouched: passes through
-        add_own(B, w_tend_in, i, g);
-        return;
-    }
-    const double zi = m.fzm[k] * m.zz[c * L + k] + m.fzp[k] * m.zz[c * L + k - 1];
-    const double gw = zi * g;
-    add_own(B, w_tend_in, i, gw);
-    const int n = m.nEdgesOnCell[c];
-    for (int j = 0; j < n; ++j) {
-        const int e = m.edgesOnCell[c * ME + j];
-        const double uk = F[u_tend][(long long)e * L + k];
-        const long long zo = ((long long)c * ME + j) * Lw + k;
-        const double coef = m.zb_cell[zo] + sign_pm(uk) * m.zb3_cell[zo];
-        const double gf = -coef * m.edgesOnCell_sign[c * ME + j] * gw;
-        add(B, u_tend, (long long)e * L + k, m.fzm[k] * gf);
-        add(B, u_tend, (long long)e * L + k - 1, m.fzp[k] * gf);
-    }
-}
-
-// Gather form (no atomics): the cell pass keeps the w_tend part, the edge pass
-// sums what the edge's two cells send its level k from interfaces k and k+1.
+// Gather form (no atomics): the cell pass keeps the w_tend part (thread (cell
+// c, interface k), c in [0, nCells], k in [0, L]); the edge pass sums what the
+// edge's two cells send its level k from interfaces k and k+1.
 __device__ __forceinline__ void smlstep_cells_vjp(const mpas_hip_mesh &m, Bar B, int c, int k) {
     using namespace smlstep;
     const int L = m.nLevels, Lw = L + 1;

@@ -326,13 +326,18 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_a(long long nb, mpa
     for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
-// P5: del2 of the u Laplacian at vertices and cells (del4 first half)
-__global__ void k_dyn_del4_vc(mpas_hip_mesh m, const hreal *__restrict__ delsq_u,
-                              hreal *__restrict__ delsq_vort, double *__restrict__ delsq_div) {
+// P5: del2 of the u Laplacian at vertices and cells (del4 first half); one
+// wavefront per index (vertex and cell of the same number), lane = level
+__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_del4_vc(long long nb, mpas_hip_mesh m,
+                                                               const hreal *__restrict__ delsq_u,
+                                                               hreal *__restrict__ delsq_vort,
+                                                               double *__restrict__ delsq_div) {
     const int L = m.nLevels, nC = m.nCells, nV = m.nVertices, ME = m.maxEdges;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nV * L) {
-        int vx = i / L, k = i - vx * L;
+    const int id = wave_entity_base(nb);
+    const int k = wave_lane();
+    if (k >= L) return;
+    if (id < nV) {
+        const int vx = id;
         double s = 0.0;
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
@@ -341,30 +346,44 @@ __global__ void k_dyn_del4_vc(mpas_hip_mesh m, const hreal *__restrict__ delsq_u
             double cc = wgt * delsq_u[e * L + k];
             s = (j == 0) ? cc : s + cc;
         }
-        delsq_vort[i] = s;
+        delsq_vort[vx * L + k] = s;
     }
-    if (i < nC * L) {
-        int c = i / L, k = i - c * L;
+    if (id < nC) {
+        const int c = id;
         const int n = m.nEdgesOnCell[c];
-        double s = 0.0;
-        for (int j = 0; j < n; ++j) {
-            int e = m.edgesOnCell[c * ME + j];
-            double wgt = m.invAreaCell[c] * m.dvEdge[e] * m.edgesOnCell_sign[c * ME + j];
-            double cc = wgt * delsq_u[e * L + k];
-            s = (j == 0) ? cc : s + cc;
+        int eo[kMaxEdges];
+#pragma unroll
+        for (int j = 0; j < kMaxEdges; ++j) eo[j] = m.edgesOnCell[c * ME + clamp_slot(j, n)];
+        double du[kMaxEdges];
+#pragma unroll
+        for (int j = 0; j < kMaxEdges; ++j) {
+            du[j] = 0.0;
+            if (j < n) du[j] = delsq_u[eo[j] * L + k];  // wave-uniform guard
         }
-        delsq_div[i] = s;
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < kMaxEdges; ++j) {
+            double wgt = m.invAreaCell[c] * m.dvEdge[eo[j]] * m.edgesOnCell_sign[c * ME + clamp_slot(j, n)];
+            double cc = wgt * du[j];
+            s = (j < n) ? ((j == 0) ? cc : s + cc) : s;
+        }
+        delsq_div[c * L + k] = s;
     }
 }
 
-// P6: del4 u mixing (first stage) and the final u tendency
-__global__ void k_dyn_edges_b(mpas_hip_mesh m, DynParams P, int do_del4, const hreal *__restrict__ rho_edge,
-                              const double *__restrict__ delsq_div, const hreal *__restrict__ delsq_vort,
-                              hreal *__restrict__ tend_u_euler, hreal *__restrict__ tend_u) {
+// P6: del4 u mixing (first stage) and the final u tendency; one wavefront per
+// edge, lane = level
+__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_b(long long nb, mpas_hip_mesh m, DynParams P,
+                                                               int do_del4, const hreal *__restrict__ rho_edge,
+                                                               const double *__restrict__ delsq_div,
+                                                               const hreal *__restrict__ delsq_vort,
+                                                               hreal *__restrict__ tend_u_euler,
+                                                               hreal *__restrict__ tend_u) {
     const int L = m.nLevels, nE = m.nEdges;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nE * L) return;
-    int e = i / L, k = i - e * L;
+    const int e = wave_entity_base(nb);
+    const int k = wave_lane();
+    if (e >= nE || k >= L) return;
+    const int i = e * L + k;
     double eu = tend_u_euler[i];
     if (do_del4) {
         const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
@@ -652,11 +671,9 @@ void launch_dyn_tend(const mpas_hip_mesh &m, const DynParams &P, const DynInputs
                        in.rw, in.pv_edge, in.ke, out.h_divergence, in.rho_edge, in.pp, sc.dpdz, in.cqu, in.divergence,
                        in.vorticity, io.kdiff, out.tend_u, io.tend_u_euler, sc.delsq_u, do_del4 ? 0 : 1);
     if (do_del4) {
-        long long n = (long long)(nC > nV ? nC : nV) * L;
-        MPAS_LAUNCH(k_dyn_del4_vc, dim3(blocks_for(n)), dim3(kBlock), 0, s, m, sc.delsq_u, sc.delsq_vort,
-                           sc.delsq_div);
-        MPAS_LAUNCH(k_dyn_edges_b, dim3(blocks_for((long long)nE * L)), dim3(kBlock), 0, s, m, P, do_del4,
-                           in.rho_edge, sc.delsq_div, sc.delsq_vort, io.tend_u_euler, out.tend_u);
+        MPAS_WAVE_LAUNCH(k_dyn_del4_vc, nC > nV ? nC : nV, s, m, sc.delsq_u, sc.delsq_vort, sc.delsq_div);
+        MPAS_WAVE_LAUNCH(k_dyn_edges_b, nE, s, m, P, do_del4, in.rho_edge, sc.delsq_div, sc.delsq_vort,
+                         io.tend_u_euler, out.tend_u);
     }
     MPAS_WAVE_LAUNCH(k_dyn_edge_adv, nE, s, m, P, in.ru, in.ru_save,
                 in.w, in.theta_m, in.theta_m_save, sc.wflux, sc.tflux, sc.tpert);
