@@ -280,7 +280,8 @@ __device__ __forceinline__ void recover_w_vjp(const mpas_hip_mesh &m, double inv
     if (c >= m.nCells || k > L) return;
     const long long oc = (long long)c * L;
     const double *rz = F[rho_zz] + oc;
-    const int n = m.nEdgesOnCell[c];
+    // the edge loops carry the terrain terms only (nothing on a flat mesh)
+    const int n = m.flatTerrain ? 0 : m.nEdgesOnCell[c];
     const double gw = bar_of(B, w, ow + k);
     if (k == 0) {
         const double den = m.cf1 * rz[0] + m.cf2 * rz[1] + m.cf3 * rz[2];

@@ -778,7 +778,8 @@ int mpas_hip_smlstep_pert_vjp(const mpas_hip_mesh *mesh, const double *const *fw
     KArgs A{};
     // gather form: w_tend per (cell, interface), u_tend per (edge, level)
     ADJ_LAUNCH(k_adj_smlstep_cells, m.nCells + 1, m.nLevels + 1, STREAM(stream));
-    ADJ_LAUNCH(k_adj_smlstep_edges, m.nEdges, m.nLevels, STREAM(stream));
+    // u_tend enters through the terrain coefficients only: nothing on a flat mesh
+    if (!m.flatTerrain) ADJ_LAUNCH(k_adj_smlstep_edges, m.nEdges, m.nLevels, STREAM(stream));
     return check_launch("smlstep_pert_vjp");
 }
 

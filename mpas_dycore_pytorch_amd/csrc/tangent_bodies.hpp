@@ -50,7 +50,8 @@ __device__ __forceinline__ void smlstep_jvp(const mpas_hip_mesh &m, Fwd F, Dot T
         dot_put(T, w_tend_out, i, dw);
         return;
     }
-    const int n = m.nEdgesOnCell[c];
+    // terrain terms only: a flat mesh has zb_cell = zb3_cell = 0 and adds nothing
+    const int n = m.flatTerrain ? 0 : m.nEdgesOnCell[c];
     for (int j = 0; j < n; ++j) {
         const int e = m.edgesOnCell[c * ME + j];
         const double uk = F[u_tend][(long long)e * L + k];
@@ -235,7 +236,8 @@ __device__ __forceinline__ void recover_w_jvp(const mpas_hip_mesh &m, double inv
         dot_put(T, w, ow + k, 0.0);
         return;
     }
-    const int n = m.nEdgesOnCell[c];
+    // the edge loops carry the terrain terms only (nothing on a flat mesh)
+    const int n = m.flatTerrain ? 0 : m.nEdgesOnCell[c];
     const double *rz = F[rho_zz] + oc;
     auto drz = [&](int kk) { return dot_of(T, rho_zz, oc + kk); };
     if (k == 0) {
