@@ -401,6 +401,7 @@ enum { u, h, v_in, h_edge, v, divergence, vorticity, ke, pv_edge, N };
 }
 struct DiagScratch {
     double *g_pvv, *g_kev, *g_pvc, *g_kee;
+    double *pvc;  // gather form: pv_cell of every cell (C), formed once by the cell pass
 };
 
 __device__ __forceinline__ double diag_pv_cell(const mpas_hip_mesh &m, Fwd F, int c, int k) {
@@ -558,6 +559,8 @@ __device__ __forceinline__ void diag_g2(const mpas_hip_mesh &m, double dt, doubl
     const double apvm_dt = apvm * dt;
     const bool on = apvm > 0.0;
     const int n = m.nEdgesOnCell[c];
+    // pv_cell of this cell, read by the edge pass for both cells of each edge
+    if (on) S.pvc[(long long)c * L + k] = diag_pv_cell(m, F, c, k);
     double gpc = 0.0, gh = 0.0;
     for (int j = 0; j < n; ++j) {
         const int e = m.edgesOnCell[c * ME + j];
@@ -627,7 +630,7 @@ __device__ __forceinline__ void diag_g4(const mpas_hip_mesh &m, double dt, doubl
     const double t = diag_t(apvm * dt, apvm > 0.0, B, i);
     if (t != 0.0) {
         const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-        const double pc1 = diag_pv_cell(m, F, c1, k), pc2 = diag_pv_cell(m, F, c2, k);
+        const double pc1 = S.pvc[(long long)c1 * L + k], pc2 = S.pvc[(long long)c2 * L + k];
         gu += t * ((pc2 - pc1) * m.invDcEdge[e]);
     }
     if (rk_step <= 0 || rk_step == 3) {

@@ -725,7 +725,7 @@ __global__ __launch_bounds__(64 * kColWaves) void k_adj_vimp_colw(mpas_hip_mesh 
 size_t vjp_scratch_doubles(const mpas_hip_mesh &m, int S) {
     const size_t L = m.nLevels, C = (m.nCells + 1) * L, E = (m.nEdges + 1) * L, V = (m.nVertices + 1) * L;
     size_t n = dyn_adj_doubles(m);
-    n = std::max(n, 2 * V + C + E);                                          // diagnostics
+    n = std::max(n, 2 * V + 2 * C + E);                                      // diagnostics
     n = std::max(n, (size_t)acoustic::kScratchArrays * (L + 1) * m.nCells);  // acoustic columns
     n = std::max(n, C * (size_t)(S > 0 ? S : 1));                            // explicit transport
     n = std::max(n, mono_adj_doubles(m));                                    // monotonic transport
@@ -847,7 +847,7 @@ int mpas_hip_solve_diagnostics_vjp(const mpas_hip_mesh *mesh, double dt, double 
     A.dt = dt;
     A.apvm = apvm;
     A.rk = rk_step;
-    A.ds = DiagScratch{scratch, scratch + V, scratch + 2 * V, scratch + 2 * V + C};
+    A.ds = DiagScratch{scratch, scratch + V, scratch + 2 * V, scratch + 2 * V + C, scratch + 2 * V + C + E};
     if (inv) {
         // gather form: every live row of the scratch is written before it is read
         A.inv = *inv;

@@ -407,13 +407,13 @@ int mpas_hip_solve_diagnostics_jvp(const mpas_hip_mesh *mesh, double dt, double 
         return 1;
     const mpas_hip_mesh &m = *mesh;
     hipStream_t s = STREAM(stream);
-    const size_t L = m.nLevels, V = (m.nVertices + 1) * L;
+    const size_t L = m.nLevels, V = (m.nVertices + 1) * L, C = (m.nCells + 1) * L;
     Slots S_ = make_slots(diag::N, fwd, dot);
     KArgs A{};
     A.dt = dt;
     A.apvm = apvm;
     A.rk = rk_step;
-    A.dd = DiagDot{scratch, scratch + V, scratch + 2 * V};
+    A.dd = DiagDot{scratch, scratch + V, scratch + 2 * V, apvm > 0.0 ? scratch + 2 * V + C : nullptr};
     ADJ_LAUNCH(k_tan_diag_vertex, m.nVertices + 1, m.nLevels, s);
     ADJ_LAUNCH(k_tan_diag_cell, m.nCells + 1, m.nLevels, s);
     ADJ_LAUNCH(k_tan_diag_edge, m.nEdges + 1, m.nLevels, s);

@@ -284,6 +284,7 @@ __device__ __forceinline__ void recover_w_jvp(const mpas_hip_mesh &m, double inv
 // ===========================================================================
 struct DiagDot {
     double *d_pvv, *d_kev, *d_pvc;
+    double *pvc;  // pv_cell values, formed once by the cell pass when apvm is on (else null)
 };
 // thread (edge e, level k): d ke_edge = dc dv 2 u du (inline helper)
 __device__ __forceinline__ double diag_dke_edge(const mpas_hip_mesh &m, Fwd F, Dot T, int e, int k) {
@@ -341,6 +342,7 @@ __device__ __forceinline__ void diag_cell_jvp(const mpas_hip_mesh &m, Fwd F, Dot
     dot_put(T, divergence, o, d * ia);
     dot_put(T, ke, o, tot);
     S.d_pvc[o] = pc;
+    if (S.pvc) S.pvc[o] = diag_pv_cell(m, F, c, k);
 }
 __device__ __forceinline__ void diag_edge_jvp(const mpas_hip_mesh &m, double dt, double apvm, int rk_step, Fwd F,
                                               Dot T, const DiagDot &S, int e, int k) {
@@ -374,7 +376,7 @@ __device__ __forceinline__ void diag_edge_jvp(const mpas_hip_mesh &m, double dt,
         const double p2 = m.fVertex[v2] + F[vorticity][(long long)v2 * L + k];
         const double gpt = (p2 - p1) * m.invDvEdge[e];
         const double dgpt = (dp2 - dp1) * m.invDvEdge[e];
-        const double pc1 = diag_pv_cell(m, F, c1, k), pc2 = diag_pv_cell(m, F, c2, k);
+        const double pc1 = S.pvc[(long long)c1 * L + k], pc2 = S.pvc[(long long)c2 * L + k];
         const double gpn = (pc2 - pc1) * m.invDcEdge[e];
         const double dgpn = (S.d_pvc[(long long)c2 * L + k] - S.d_pvc[(long long)c1 * L + k]) * m.invDcEdge[e];
         dpe -= (apvm * dt) * (dvv * gpt + F[v][i] * dgpt + dot_of(T, u, i) * gpn + F[u][i] * dgpn);
