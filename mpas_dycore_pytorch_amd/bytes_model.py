@@ -172,10 +172,9 @@ KERNELS: dict[str, Inventory] = {
     "k_recover_w": Inventory(1, ("E", "W", "C", "C", "cell_e"), ("W",)),
     "k_smlstep": Inventory(1, ("W", "C"), ("W",)),
     "k_vimp": Inventory(1, ("C",) * 8 + ("W",), ("C",) * 6 + ("W",)),
-    "k_diag_edges1": Inventory(1, ("E", "C", "edge_c"), ("E", "E")),
     "k_diag_vertices": Inventory(1, ("E", "E", "vert_e"), ("V", "V", "V")),
     "k_diag_cells": Inventory(1, ("E", "E", "V", "V", "cell_e", "cell_v", "vert_kite"), ("C", "C", "C")),
-    "k_diag_edges2": Inventory(1, ("E", "E", "V", "C", "edge_c", "edge_v"), ("E", "E")),
+    "k_diag_edges2": Inventory(1, ("E", "V", "C", "C", "edge_c", "edge_v", "edge_e"), ("E", "E", "E")),
 }
 
 
