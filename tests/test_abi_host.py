@@ -192,3 +192,28 @@ def test_mixed_library_exports_its_header():
     plain = N.load().mpas_hip_workspace_doubles(2562, 7680, 5120, 26, 1)
     mixed = lib.mpas_hip_mixed_workspace_doubles(2562, 7680, 5120, 26, 1)
     assert 0 < mixed < plain
+
+
+def test_missing_library_fails_loudly():
+    # A fresh interpreter whose binding points at a library that does not exist:
+    # loading must raise the device error, not fall back to anything else.
+    import subprocess
+    import sys
+    code = (
+        "import mpas_dycore_pytorch_amd._native as N\n"
+        "from mpas_dycore_pytorch_amd.errors import MpasDeviceError\n"
+        "N.LIB_PATH = N.LIB_PATH + '.absent'\n"
+        "N.MIXED_LIB_PATH = N.MIXED_LIB_PATH + '.absent'\n"
+        "for f in (N.load, N.load_mixed):\n"
+        "    try:\n"
+        "        f()\n"
+        "    except MpasDeviceError as e:\n"
+        "        assert 'not built' in str(e)\n"
+        "    else:\n"
+        "        raise SystemExit(3)\n"
+        "print('ok')\n"
+    )
+    root = __import__("pathlib").Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120, cwd=root)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip() == "ok"
