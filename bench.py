@@ -78,8 +78,8 @@ def traffic_from_profile(kernel: str, workload: str):
     return float(mm.group(1)) if mm else None
 
 
-def cpu_baseline(level: int, n_levels: int, dt: float, steps: int, seed: int) -> dict:
-    """The oracle (C restatement of the reference kernels, one thread) on a bounded sample."""
+def cpu_worker(level: int, n_levels: int, dt: float, steps: int, seed: int) -> None:
+    """One single-threaded oracle run of the CPU sample; prints its work and time as JSON."""
     from mpas_dycore_pytorch_amd import initial_state as IS
     from mpas_dycore_pytorch_amd import mesh as M
     from oracle import oracle as O
@@ -96,11 +96,55 @@ def cpu_baseline(level: int, n_levels: int, dt: float, steps: int, seed: int) ->
     for step in range(steps):
         cur = OS.srk3(cur, om, dt, len_disp=float(m["nominalMinDc"]), diagnostics_current=(step > 0))
     el = time.perf_counter() - t0
-    units = m["nCells"] * n_levels * steps
-    return {"value": units / el, "unit": "cell-level-steps/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle srk3 (single-threaded C kernels, Python orchestration), level {level} mesh "
-                       f"({m['nCells']} cells) x {n_levels} levels, dt {dt:g} s, {steps} large steps, "
-                       f"{el:.1f} s wall")}
+    print(json.dumps({"units": int(m["nCells"]) * n_levels * steps, "seconds": el, "cells": int(m["nCells"])}),
+          flush=True)
+
+
+def cpu_baseline(level: int, n_levels: int, dt: float, steps: int, seed: int) -> dict:
+    """The oracle (C restatement of the reference kernels) on every host core this process may use:
+    one single-threaded run of the sample per core, started together as child processes (the GPU
+    is not touched by them); throughput = all the runs' work over the slowest run's time."""
+    import subprocess
+
+    cores = len(os.sched_getaffinity(0))
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-worker", str(level), str(n_levels), str(dt),
+           str(steps), str(seed)]
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    procs = [subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, env=env)
+             for _ in range(cores)]
+    runs = []
+    for p in procs:
+        out, err = p.communicate(timeout=600)
+        if p.returncode != 0:
+            raise RuntimeError(f"cpu baseline worker failed: {err[-2000:]}")
+        runs.append(json.loads(out.strip().splitlines()[-1]))
+    units = sum(r["units"] for r in runs)
+    slowest = max(r["seconds"] for r in runs)
+    one = runs[0]
+    ref = reference_cpu_timing()
+    return {"value": units / slowest, "unit": "cell-level-steps/s", "cores": cores, "kind": "port",
+            "sample": (f"oracle srk3 (single-threaded C kernels, Python orchestration), {cores} concurrent runs, "
+                       f"one per host core, each level {level} ({one['cells']} cells) x {n_levels} levels, "
+                       f"dt {dt:g} s, {steps} large steps; slowest run {slowest:.1f} s, single-run throughput "
+                       f"{one['units'] / one['seconds']:.3e}; the unit is per cell-level, so it carries to the "
+                       f"level-7 workload by the cell count (not measured at level 7)"),
+            "reference_measured": ref}
+
+
+def reference_cpu_timing() -> dict | None:
+    """The reference's own CPU float64 step time, measured in the build container when its golden
+    summary was written (tests/golden/summary_cfg1_l4_L26.json)."""
+    path = os.path.join(ROOT, "tests", "golden", "summary_cfg1_l4_L26.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        summ = json.load(f)
+    secs = summ["reference_seconds_per_step"]
+    n = 2562 if summ["level"] == 4 else None
+    best = min(secs)
+    return {"seconds_per_step": secs, "threads": summ.get("torch_threads"), "level": summ["level"],
+            "levels": summ["nVertLevels"], "cell_level_steps_per_s": (n * summ["nVertLevels"] / best) if n else None,
+            "where": "reference atm_srk3 on the build container's CPU (Intel Xeon 2.7 GHz), torch float64"}
 
 
 def vjp_leg(m, stepper, config, dt: float, steps: int, fwd_ms_per_step: float, device) -> dict:
@@ -115,12 +159,13 @@ def vjp_leg(m, stepper, config, dt: float, steps: int, fwd_ms_per_step: float, d
     forward + VJP by ``steps`` plain forward steps."""
     import torch
 
-    from mpas_dycore_pytorch_amd.atmosphere_model import STEP_BOUNDARY_KEY, run_simulation
+    from mpas_dycore_pytorch_amd.atmosphere_model import run_simulation
+    from mpas_dycore_pytorch_amd.boundary import mark_complete_step_boundary
 
     nc = int(m["nCells"])
-    state = dict(m)
-    state.update({k: v.clone() for k, v in stepper.state_dict().items()})
-    state[STEP_BOUNDARY_KEY] = True
+    # the stepper's state (device tensors) as a bound step boundary of this mesh, config and dt
+    state = {k: v.clone() for k, v in stepper.state_dict().items()}
+    mark_complete_step_boundary(state, mesh=m, config=config, dt=dt)
     area = torch.as_tensor(1.0 / m["invAreaCell"][:nc], device=device)
     th0 = state["theta_m"].detach().clone()
 
@@ -274,6 +319,10 @@ def ensemble_bench(args, device) -> int:
 
 
 def main() -> int:
+    if len(sys.argv) > 1 and sys.argv[1] == "--cpu-worker":
+        level, n_levels, dt, steps, seed = sys.argv[2:7]
+        cpu_worker(int(level), int(n_levels), float(dt), int(steps), int(seed))
+        return 0
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
@@ -297,9 +346,10 @@ def main() -> int:
     ap.add_argument("--mixed-steps", type=int, default=6,
                     help="large steps of the mixed-precision leg's error check against the FP64 step (its "
                          "timing uses --steps); 0 skips the leg")
-    ap.add_argument("--vjp-steps", type=int, default=2,
+    ap.add_argument("--vjp-steps", type=int, default=24,
                     help="large steps of the reverse-mode leg (forward + VJP of an area-weighted squared "
-                         "theta perturbation, one checkpoint per large step); 0 skips it")
+                         "theta perturbation, one checkpoint per large step; 24 is the differentiation "
+                         "workload's run length); 0 skips it")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

@@ -77,15 +77,16 @@ typedef struct mpas_hip_config {
     int32_t nScalars;
     int32_t moist_start, moist_end; /* 1-based inclusive species range */
     int32_t diagnostics_current; /* 1: skip the first-stage diagnostic refresh */
-    int32_t reserved;
+    int32_t order;       /* [config_time_integration_order] 2 or 3 (0 is read as 3) */
 } mpas_hip_config;
 
 /* Number of doubles of workspace needed for one large step. */
 int64_t mpas_hip_workspace_doubles(int32_t nCells, int32_t nEdges, int32_t nVertices,
                                    int32_t nLevels, int32_t nScalars);
 
-/* One split-explicit RK3 large step in place.  Replaces the reference's
- * atm_srk3 (mpas_pytorch/dynamics/time_integration.py:4897-4956). */
+/* One split-explicit large step in place (three RK stages with the RK3 or
+ * RK2 weights of compute_rk_weights, time_integration.py:2787-2856).
+ * Replaces the reference's atm_srk3 (time_integration.py:4897-4956). */
 int mpas_hip_srk3_step(const mpas_hip_mesh *mesh, mpas_hip_state *state,
                        const mpas_hip_workspace *work, const mpas_hip_config *cfg,
                        void *stream);
@@ -198,7 +199,9 @@ int mpas_hip_advance_scalars_mono(const mpas_hip_mesh *m, int32_t nScalars, doub
                                   double *rho_zz_int, double *scratch, void *stream);
 /* atm_init_coupled_diagnostics, init_coupled_diagnostics.py:147 (cold start
  * from raw theta, rho, qv, u, w; uses the mesh's rho_base, zz, fzm, fzp and
- * terrain terms; writes padded outputs with rho_zz's padding row = 1) */
+ * terrain terms; writes padded outputs with rho_zz's padding row = 1).  With
+ * theta, rho and qv all null it takes the derived path of :256-262: theta_m
+ * and rho_zz are read from the output arrays, which the caller fills. */
 int mpas_hip_init_coupled_diagnostics(const mpas_hip_mesh *m, const double *theta, const double *rho,
                                       const double *qv, const double *u, const double *w,
                                       const double *theta_base, double *theta_m, double *rho_zz,
@@ -337,6 +340,23 @@ int mpas_hip_history_fields(int32_t nCells, int32_t nLevels, int32_t nScalars, c
                             const double *scalars, const double *rho_zz, const double *zz,
                             const double *pressure_p, const double *pressure_base, double *theta,
                             double *rho, double *pressure, void *stream);
+
+/* Derivatives of the cold start (atm_init_coupled_diagnostics,
+ * init_coupled_diagnostics.py:147-380) for differentiable runs from raw
+ * fields (run_simulation -> _prepare_timestep_state, atmosphere_model.py:782).
+ * Slots: theta, rho, qv, theta_m_in, rho_zz_in, u, w | theta_m, rho_zz, ru,
+ * rw, rho_p, rtheta_p, exner, pressure_p.  derived = 0 reads theta, rho, qv;
+ * derived = 1 reads theta_m_in, rho_zz_in (the unused slots may be null).  The
+ * base-state outputs carry no derivative.  The VJP accumulates input
+ * adjoints (gathers only, deterministic) and needs
+ * mpas_hip_init_coupled_vjp_scratch_doubles() doubles of scratch; the JVP
+ * writes every output tangent (all output slots non-null). */
+int64_t mpas_hip_init_coupled_vjp_scratch_doubles(int32_t nEdges, int32_t nLevels);
+int mpas_hip_init_coupled_diagnostics_vjp(const mpas_hip_mesh *m, int32_t derived, const double *theta_base,
+                                          const double *const *fwd, double *const *bar, double *scratch,
+                                          void *stream);
+int mpas_hip_init_coupled_diagnostics_jvp(const mpas_hip_mesh *m, int32_t derived, const double *theta_base,
+                                          const double *const *fwd, double *const *dot, void *stream);
 
 /* Launch-shape table, no counterpart in the reference (its kernels are torch
  * operators): wavefronts per block (1, 2, 4, 8 or 16; 0 restores the default 4) of a

@@ -59,7 +59,7 @@ class ConfigStruct(C.Structure):
                 ("apvm", C.c_double), ("coef3", C.c_double), ("c_s", C.c_double), ("visc4_2dsmag", C.c_double),
                 ("del4u_div_factor", C.c_double), ("ns", C.c_int32), ("split", C.c_int32),
                 ("monotonic", C.c_int32), ("nScalars", C.c_int32), ("moist_start", C.c_int32),
-                ("moist_end", C.c_int32), ("diagnostics_current", C.c_int32), ("reserved", C.c_int32)]
+                ("moist_end", C.c_int32), ("diagnostics_current", C.c_int32), ("order", C.c_int32)]
 
 
 class DynParamsStruct(C.Structure):
@@ -125,6 +125,9 @@ SIGNATURES = {
     "mpas_hip_advance_scalars_jvp": (C.c_int, [_PM, _I32, _F64, _F64, _F64, _F64, _PP, _PP, _vp]),
     "mpas_hip_advance_scalars_mono_jvp": (C.c_int, [_PM, _I32, _F64, _F64, _PP, _PP, _dp, _vp]),
     "mpas_hip_reconstruct_jvp": (C.c_int, [_PM, _PP, _PP, _vp]),
+    "mpas_hip_init_coupled_vjp_scratch_doubles": (_I64, [_I32, _I32]),
+    "mpas_hip_init_coupled_diagnostics_vjp": (C.c_int, [_PM, _I32, _dp, _PP, _PP, _dp, _vp]),
+    "mpas_hip_init_coupled_diagnostics_jvp": (C.c_int, [_PM, _I32, _dp, _PP, _PP, _vp]),
     "mpas_hip_history_fields": (C.c_int, [_I32, _I32, _I32, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _vp]),
     "mpas_hip_tuning_set": (C.c_int, [C.c_char_p, _I32]),
     "mpas_hip_tuning_get": (_I32, [C.c_char_p]),

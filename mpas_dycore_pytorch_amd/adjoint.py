@@ -38,6 +38,9 @@ from . import _native as N
 from . import kernels as K
 from .device import DeviceMesh, stream_handle
 
+# outputs of the cold start that carry a derivative (the base-state ones do not)
+ICD_DIFF_OUT = ("theta_m", "rho_zz", "ru", "rw", "rho_p", "rtheta_p", "exner", "pressure_p")
+
 # slot order of each adjoint entry point: inputs, then outputs (include/mpas_hip.h)
 SLOTS: dict[str, tuple[tuple[str, ...], tuple[str, ...]]] = {
     "moist_coefficients": (("scalars",), ("cqw", "cqu", "qtot")),
@@ -55,6 +58,8 @@ SLOTS: dict[str, tuple[tuple[str, ...], tuple[str, ...]]] = {
     "advance_scalars_mono": (("scalars_old", "scalars_new_in", "rho_zz_old", "uhAvg", "wwAvg"),
                              ("scalars_new_out", "rho_zz_int")),
     "reconstruct": (("u",), K.RECON_OUT),
+    # the cold start (raw slots 0-2 or derived slots 3-4 are None in the other mode)
+    "init_coupled": (("theta", "rho", "qv", "theta_m_in", "rho_zz_in", "u", "w"), ICD_DIFF_OUT),
 }
 # output adjoints the entry points use as accumulators (must be non-null)
 REQUIRED_OUT = {
@@ -84,9 +89,11 @@ class Tape:
         self.ops: list[Op] = []
 
     # ---- recording helpers (pure tensor ops of the orchestration)
-    def kernel(self, name: str, args: tuple, ins: Mapping[str, torch.Tensor], outs: Mapping[str, torch.Tensor]):
+    def kernel(self, name: str, args: tuple, ins: Mapping[str, torch.Tensor | None],
+               outs: Mapping[str, torch.Tensor]):
+        """Record one kernel call; an input slot the call does not read is None."""
         si, so = SLOTS[name]
-        self.ops.append(Op(name, args, [ins[k] for k in si], [outs[k] for k in so]))
+        self.ops.append(Op(name, args, [ins.get(k) for k in si], [outs[k] for k in so]))
 
     def add(self, a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
         out = a + b
@@ -113,7 +120,7 @@ class Tape:
         """Ids of the recorded tensors that depend on any of ``sources`` (a forward scan)."""
         dep = {id(t) for t in sources}
         for op in self.ops:
-            if any(id(t) in dep for t in op.ins):
+            if any(t is not None and id(t) in dep for t in op.ins):
                 dep.update(id(t) for t in op.outs)
         return dep
 
@@ -166,7 +173,7 @@ class Tape:
             out_bars = [bars.get(id(t)) for t in op.outs]
             if all(b is None for b in out_bars):
                 continue  # nothing downstream depends on this call
-            if wrt is not None and not any(id(t) in wrt for t in op.ins):
+            if wrt is not None and not any(t is not None and id(t) in wrt for t in op.ins):
                 for t in op.outs:
                     bars.pop(id(t), None)
                 continue  # no input of interest upstream of this call
@@ -179,7 +186,7 @@ class Tape:
             pending: list[tuple[torch.Tensor, torch.Tensor]] = []
             seen: set[int] = set()
             for name, t in zip(si, op.ins):
-                if name in const or (wrt is not None and id(t) not in wrt):
+                if t is None or name in const or (wrt is not None and id(t) not in wrt):
                     in_bars.append(None)
                     continue
                 if id(t) in seen:
@@ -190,7 +197,8 @@ class Tape:
                     continue
                 seen.add(id(t))
                 in_bars.append(get(t, True))
-            fwd = (C.c_void_p * (len(op.ins) + len(op.outs)))(*[x.data_ptr() for x in op.ins + op.outs])
+            fwd = (C.c_void_p * (len(op.ins) + len(op.outs)))(
+                *[(x.data_ptr() if x is not None else None) for x in op.ins + op.outs])
             bar = (C.c_void_p * (len(op.ins) + len(op.outs)))(
                 *[(b.data_ptr() if b is not None else None) for b in in_bars + out_bars])
             _call_vjp(lib, op, dm, fwd, bar, scratch, stream)
@@ -218,9 +226,10 @@ class Tape:
         last: dict[int, int] = {}
         for i, op in enumerate(self.ops):
             for t in op.ins:
-                last[id(t)] = i
+                if t is not None:
+                    last[id(t)] = i
         for i, op in enumerate(self.ops):
-            tin = [dots.get(id(t)) for t in op.ins]
+            tin = [(dots.get(id(t)) if t is not None else None) for t in op.ins]
             if op.kind in ("add", "scale", "copy", "mask_pad"):
                 live = [d for d in tin if d is not None]
                 if live:
@@ -240,14 +249,15 @@ class Tape:
                 tin = [None if name in const else d for name, d in zip(si, tin)]
                 if any(d is not None for d in tin):
                     tout = [torch.empty_like(t) for t in op.outs]
-                    fwd = (C.c_void_p * (len(op.ins) + len(op.outs)))(*[x.data_ptr() for x in op.ins + op.outs])
+                    fwd = (C.c_void_p * (len(op.ins) + len(op.outs)))(
+                        *[(x.data_ptr() if x is not None else None) for x in op.ins + op.outs])
                     dot = (C.c_void_p * (len(op.ins) + len(op.outs)))(
                         *[(d.data_ptr() if d is not None else None) for d in tin + tout])
                     _call_jvp(lib, op, dm, fwd, dot, scratch, stream)
                     for t, d in zip(op.outs, tout):
                         dots[id(t)] = d
             for t in op.ins:
-                if last.get(id(t)) == i and id(t) not in keep:
+                if t is not None and last.get(id(t)) == i and id(t) not in keep:
                     dots.pop(id(t), None)
         return dots
 
@@ -278,6 +288,8 @@ def _call_jvp(lib, op: Op, dm: DeviceMesh, fwd, dot, scratch: torch.Tensor, stre
         rc = lib.mpas_hip_advance_scalars_mono_jvp(dm.ref, *a, fwd, dot, sp, stream)
     elif k == "reconstruct":
         rc = lib.mpas_hip_reconstruct_jvp(dm.ref, fwd, dot, stream)
+    elif k == "init_coupled":
+        rc = lib.mpas_hip_init_coupled_diagnostics_jvp(dm.ref, int(a[0]), a[1].data_ptr(), fwd, dot, stream)
     else:  # pragma: no cover - every recorded kind is handled above
         raise KeyError(k)
     N.check(rc, k + "_jvp")
@@ -309,6 +321,11 @@ def _call_vjp(lib, op: Op, dm: DeviceMesh, fwd, bar, scratch: torch.Tensor, stre
         rc = lib.mpas_hip_advance_scalars_mono_vjp(dm.ref, *a, fwd, bar, sp, stream)
     elif k == "reconstruct":
         rc = lib.mpas_hip_reconstruct_vjp(dm.ref, fwd, bar, stream)
+    elif k == "init_coupled":
+        need = lib.mpas_hip_init_coupled_vjp_scratch_doubles(dm.ne, dm.L)
+        if scratch.numel() < need:
+            raise ValueError(f"init_coupled_vjp needs {need} doubles of scratch, got {scratch.numel()}")
+        rc = lib.mpas_hip_init_coupled_diagnostics_vjp(dm.ref, int(a[0]), a[1].data_ptr(), fwd, bar, sp, stream)
     else:  # pragma: no cover - every recorded kind is handled above
         raise KeyError(k)
     N.check(rc, k + "_vjp")
@@ -327,9 +344,19 @@ STEP_INPUTS = ("u", "v", "w", "ru", "rw", "theta_m", "rho_zz", "rho_edge", "dive
                "pv_edge", "pressure_p", "exner", "rtheta_p", "rho_p", "scalars", "rt_diabatic_tend")
 
 
-def _explicit_weights(rk_step: int) -> tuple[float, float]:
-    wn = {1: 1.0 / 3.0, 2: 1.0 / 2.0}.get(rk_step, 1.0)
+def _explicit_weights(rk_step: int, order: int = 3) -> tuple[float, float]:
+    """Old/new density weights of an explicit transport stage (advance_scalars.py:271-286)."""
+    wn = {1: 1.0 / 2.0 if order == 2 else 1.0 / 3.0, 2: 1.0 / 2.0}.get(rk_step, 1.0)
     return 1.0 - wn, wn
+
+
+def rk_weights(dt_dyn: float, ns: int, order: int) -> tuple[list, list, list]:
+    """(stage lengths, acoustic substep lengths, substep counts) of the three
+    stages, as ``compute_rk_weights`` forms them (time_integration.py:2787-2856)."""
+    half = max(1, ns // 2)
+    if order == 2:
+        return [dt_dyn / 2.0, dt_dyn / 2.0, dt_dyn], [dt_dyn / float(ns)] * 3, [half, half, ns]
+    return [dt_dyn / 3.0, dt_dyn / 2.0, dt_dyn], [dt_dyn / 3.0, dt_dyn / float(ns), dt_dyn / float(ns)], [1, half, ns]
 
 
 def taped_step(dm: DeviceMesh, state: Mapping[str, torch.Tensor], cfg: Any, dt: float,
@@ -358,10 +385,8 @@ def taped_step(dm: DeviceMesh, state: Mapping[str, torch.Tensor], cfg: Any, dt: 
     acoustic = {"ru_p": z(dm.ne + 1, L), "ruAvg": z(dm.ne + 1, L), "rho_pp": z(nc + 1, L),
                 "rtheta_pp": z(nc + 1, L), "rtheta_pp_old": z(nc + 1, L), "rw_p": z(nc + 1, L + 1),
                 "wwAvg": z(nc + 1, L + 1)}
-    dt_dyn = dt / split
-    rk_timestep = [dt_dyn / 3.0, dt_dyn / 2.0, dt_dyn]
-    rk_sub = [dt_dyn / 3.0, dt_dyn / float(ns), dt_dyn / float(ns)]
-    nsub = [1, max(1, ns // 2), ns]
+    order = 2 if int(cfg.order) == 2 else 3
+    rk_timestep, rk_sub, nsub = rk_weights(dt / split, ns, order)
     moist = vcoef = carried = None
     ruAvg_split = wwAvg_split = None
     ms, me = int(cfg.moist_start), int(cfg.moist_end)
@@ -377,7 +402,7 @@ def taped_step(dm: DeviceMesh, state: Mapping[str, torch.Tensor], cfg: Any, dt: 
                                 {"u": st["u"], "h": st["rho_zz"], "v_in": v0}, d)
                     st.update({k: d[k] for k in DIAG_KEYS})
                     st["rho_edge"] = d["h_edge"]
-            if rk in (1, 2):
+            if rk == 1 or (rk == 2 and order == 3):
                 vin = {"cqw": moist["cqw"], "p": st["exner"], "t": st["theta_m"], "rt": st["rtheta_p"],
                        "qtot": moist["qtot"]}
                 vcoef = K.vert_imp_coefs(dm, rk_sub[rk - 1], epssm, *vin.values())
@@ -459,7 +484,7 @@ def taped_step(dm: DeviceMesh, state: Mapping[str, torch.Tensor], cfg: Any, dt: 
     rho_zz_new = st["rho_zz"]
     if S > 0:
         for rk_t in (1, 2, 3):
-            dt_t = [dt / 3.0, dt / 2.0, dt][rk_t - 1]
+            dt_t = [dt / 2.0 if order == 2 else dt / 3.0, dt / 2.0, dt][rk_t - 1]
             if rk_t == 3 and cfg.monotonic:
                 out, rzi = K.advance_scalars_mono(dm, dt_t, coef3, scalars_old, scalars_new, rho_zz_old_split,
                                                   ruAvg, wwAvg)
@@ -468,7 +493,7 @@ def taped_step(dm: DeviceMesh, state: Mapping[str, torch.Tensor], cfg: Any, dt: 
                              "rho_zz_old": rho_zz_old_split, "uhAvg": ruAvg, "wwAvg": wwAvg},
                             {"scalars_new_out": out, "rho_zz_int": rzi})
             else:
-                wo, wn = _explicit_weights(rk_t) if not (rk_t == 3) else (0.0, 1.0)
+                wo, wn = _explicit_weights(rk_t, order) if not (rk_t == 3) else (0.0, 1.0)
                 out, hf = K.advance_scalars(dm, dt_t, coef3, wo, wn, scalars_old, scalars_new, rho_zz_old_split,
                                             rho_zz_new, ruAvg, wwAvg)
                 tape.kernel("advance_scalars", (S, float(dt_t), float(coef3), float(wo), float(wn)),
@@ -484,6 +509,9 @@ def taped_step(dm: DeviceMesh, state: Mapping[str, torch.Tensor], cfg: Any, dt: 
         st[k] = rc[k]
     st["ur_cell"] = tape.copy(rc["uReconstructZonal"])
     st["vr_cell"] = tape.copy(rc["uReconstructMeridional"])
+    # h_edge is rho_edge under its other name (atmosphere_model.py:176-200): the
+    # same tensor, so a cotangent on either reaches the diagnostic solve
+    st["h_edge"] = st["rho_edge"]
     return st
 
 
@@ -559,3 +587,45 @@ def tape_jvp(tape: Tape, state: Mapping[str, torch.Tensor], out: Mapping[str, to
         if isinstance(v, torch.Tensor) and id(v) in dots:
             res[k] = dots[id(v)]
     return res
+
+
+# ---------------------------------------------------------------------------
+# the taped cold start
+# ---------------------------------------------------------------------------
+COLD_START_INPUTS = ("theta", "rho", "qv", "theta_m", "rho_zz", "u", "w")
+
+
+def taped_cold_start(dm: DeviceMesh, fields: Mapping[str, torch.Tensor], theta_base: torch.Tensor, dt: float,
+                     apvm: float, tape: Tape) -> dict[str, torch.Tensor]:
+    """The cold start of a differentiable run, recorded on ``tape``.
+
+    The reference prepares the state inside its differentiable
+    ``run_simulation`` (``atmosphere_model.py:1589-1597`` ->
+    ``_prepare_timestep_state`` ``:782``): ``atm_init_coupled_diagnostics``,
+    the padding sentinels, the first diagnostic solve and the vector
+    reconstruction.  ``fields`` holds ``u``, ``w`` and either ``theta_m`` and
+    ``rho_zz`` (used as given, the derived path) or raw ``theta``, ``rho``,
+    ``qv``.  Returns the prepared fields; the base-state outputs
+    (``rtheta_base``, ``exner_base``, ``pressure_base``) are constants."""
+    derived = "theta_m" in fields and "rho_zz" in fields
+    u, w = fields["u"], fields["w"]
+    if derived:
+        icd = K.init_coupled_diagnostics(dm, None, None, None, u, w, theta_base, fields["theta_m"], fields["rho_zz"])
+        ins = {"theta_m_in": fields["theta_m"], "rho_zz_in": fields["rho_zz"], "u": u, "w": w}
+    else:
+        icd = K.init_coupled_diagnostics(dm, fields["theta"], fields["rho"], fields["qv"], u, w, theta_base)
+        ins = {"theta": fields["theta"], "rho": fields["rho"], "qv": fields["qv"], "u": u, "w": w}
+    tape.kernel("init_coupled", (1 if derived else 0, theta_base), ins, icd)
+    out = dict(icd)
+    v0 = torch.zeros_like(u)
+    d = K.solve_diagnostics(dm, dt, apvm, None, u, icd["rho_zz"], v0)
+    tape.kernel("solve_diagnostics", (float(dt), float(apvm), -1), {"u": u, "h": icd["rho_zz"], "v_in": v0}, d)
+    out.update({k: d[k] for k in DIAG_KEYS})
+    out["rho_edge"] = d["h_edge"]
+    rc = K.reconstruct(dm, u)
+    tape.kernel("reconstruct", (), {"u": u}, rc)
+    for k in K.RECON_OUT:
+        out[k] = rc[k]
+    out["ur_cell"] = tape.copy(rc["uReconstructZonal"])
+    out["vr_cell"] = tape.copy(rc["uReconstructMeridional"])
+    return out

@@ -1,6 +1,7 @@
 // Cold-start coupling of the raw initial fields: atm_init_coupled_diagnostics
 // (mpas_pytorch/dynamics/init_coupled_diagnostics.py:147-380).
 //   theta_m = theta (1 + rvord qv), rho_zz = rho / zz           (:240-262)
+//   (or, with theta null, the theta_m and rho_zz given in the outputs)
 //   ru = 0.5 u (rho_zz1 + rho_zz2)                             (:283)
 //   rw = w rho_zz_i zz_i - sum_slots sgn (zb + sign(f) zb3) f zz_i (:285-343)
 //   rho_p, rtheta_base, rtheta_p, exner, exner_base, pressure_p,
@@ -12,8 +13,7 @@ namespace mpas {
 
 constexpr double kRvord = 461.6 / 287.0;
 
-__global__ void k_icd_cells(mpas_hip_mesh m, const double *__restrict__ theta, const double *__restrict__ rho,
-                            const double *__restrict__ qv, const double *__restrict__ theta_base,
+__global__ void k_icd_cells(mpas_hip_mesh m, const double *theta, const double *rho, const double *qv, const double *__restrict__ theta_base,
                             InitCoupledOutputs o) {
     const int L = m.nLevels, nC = m.nCells;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -28,8 +28,10 @@ __global__ void k_icd_cells(mpas_hip_mesh m, const double *__restrict__ theta, c
     const double rgas_over_p0 = MPAS_RGAS / MPAS_P0;
     const double rcv = MPAS_RGAS / (MPAS_CP - MPAS_RGAS);
     const double zz = m.zz[i], rb = m.rho_base[i], tb = theta_base[i];
-    const double tm = theta[i] * (1.0 + kRvord * qv[i]);
-    const double rz = rho[i] / zz;
+    // raw fields, or (theta null) the theta_m / rho_zz already held in the
+    // outputs: the reference's derived path (init_coupled_diagnostics.py:256-262)
+    const double tm = theta ? theta[i] * (1.0 + kRvord * qv[i]) : o.theta_m[i];
+    const double rz = theta ? rho[i] / zz : o.rho_zz[i];
     const double rp = rz - rb;
     const double rtb = tb * rb;
     const double rtp = tm * rp + rb * (tm - tb);

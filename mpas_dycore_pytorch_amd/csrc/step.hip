@@ -573,6 +573,10 @@ static int srk3_step_impl(const mpas_hip_mesh *mesh, StepState *st, const mpas_h
         set_error("split and ns must be >= 1");
         return 1;
     }
+    if (cfg->order != 0 && cfg->order != 2 && cfg->order != 3) {
+        set_error("time integration order must be 2 or 3");
+        return 1;
+    }
     const int L = m.nLevels, nC = m.nCells;
     const size_t C = (size_t)(nC + 1) * L, E = (size_t)(m.nEdges + 1) * L, Wd = (size_t)(nC + 1) * (L + 1);
     const double dt = cfg->dt;
@@ -587,11 +591,15 @@ static int srk3_step_impl(const mpas_hip_mesh *mesh, StepState *st, const mpas_h
     copy_d(w.rtheta_p_save, st->rtheta_p, C, s);
     copy_d(w.rho_p_save, st->rho_p, C, s);
 
-    // compute_rk_weights, order 3 (time_integration.py:2787-2856)
+    // compute_rk_weights (time_integration.py:2787-2856): three stages for
+    // both orders; RK2 changes the stage lengths and substep counts
+    const bool rk2 = cfg->order == 2;
     const double dt_dyn = dt / (double)cfg->split;
-    const double rk_timestep[3] = {dt_dyn / 3.0, dt_dyn / 2.0, dt_dyn};
-    const double rk_sub[3] = {dt_dyn / 3.0, dt_dyn / (double)cfg->ns, dt_dyn / (double)cfg->ns};
-    const int nsub[3] = {1, cfg->ns / 2 > 1 ? cfg->ns / 2 : 1, cfg->ns};
+    const int half_ns = cfg->ns / 2 > 1 ? cfg->ns / 2 : 1;
+    const double dts_ns = dt_dyn / (double)cfg->ns;
+    const double rk_timestep[3] = {rk2 ? dt_dyn / 2.0 : dt_dyn / 3.0, dt_dyn / 2.0, dt_dyn};
+    const double rk_sub[3] = {rk2 ? dts_ns : dt_dyn / 3.0, dts_ns, dts_ns};
+    const int nsub[3] = {rk2 ? half_ns : 1, half_ns, cfg->ns};
     const double resm = (1.0 - cfg->epssm) / (1.0 + cfg->epssm);
     const double rcv = MPAS_RGAS / (MPAS_CP - MPAS_RGAS);
     const double c2 = MPAS_CP * rcv;
@@ -621,7 +629,9 @@ static int srk3_step_impl(const mpas_hip_mesh *mesh, StepState *st, const mpas_h
                                        st->divergence, st->vorticity, st->ke, st->pv_edge, w.scratch, s);
                 }
             }
-            if (rk == 1 || rk == 2)
+            // the coefficients depend on dts: RK3 recomputes them for its
+            // second stage, RK2 keeps the first stage's (time_integration.py:5558-5569)
+            if (rk == 1 || (rk == 2 && !rk2))
                 launch_vimp(m, rk_sub[rk - 1], cfg->epssm, w.cqw, st->exner, st->theta_m, st->rtheta_p, w.qtot,
                             w.cofrz, w.cofwr, w.cofwz, w.coftz, w.cofwt, w.a_tri, w.alpha_tri, w.gamma_tri, s);
             P.rk_step = rk;
@@ -685,9 +695,12 @@ static int srk3_step_impl(const mpas_hip_mesh *mesh, StepState *st, const mpas_h
     // split transport RK3 (time_integration.py:3976-4675); rho_zz_new is the
     // post-dynamics rho_zz already in the state
     if (S > 0) {
-        const double dts[3] = {dt / 3.0, dt / 2.0, dt};
+        // transport stage lengths and density weights (time_integration.py:3367-3371,
+        // advance_scalars.py:271-286)
+        const double dts[3] = {rk2 ? dt / 2.0 : dt / 3.0, dt / 2.0, dt};
+        const double wn1 = rk2 ? 1.0 / 2.0 : 1.0 / 3.0;
         // stage 1: state.scalars -> tmp; stage 2: tmp -> state.scalars
-        launch_advance_scalars(m, S, dts[0], cfg->coef3, 1.0 - 1.0 / 3.0, 1.0 / 3.0, w.scalars_old, st->scalars,
+        launch_advance_scalars(m, S, dts[0], cfg->coef3, 1.0 - wn1, wn1, w.scalars_old, st->scalars,
                                w.rho_zz_old_split, st->rho_zz, st->ruAvg, st->wwAvg, w.scalars_tmp, w.horiz_flux, s);
         launch_advance_scalars(m, S, dts[1], cfg->coef3, 1.0 - 1.0 / 2.0, 1.0 / 2.0, w.scalars_old, w.scalars_tmp,
                                w.rho_zz_old_split, st->rho_zz, st->ruAvg, st->wwAvg, st->scalars, w.horiz_flux, s);

@@ -131,7 +131,7 @@ class DeviceMesh:
         for name in ("cf1", "cf2", "cf3"):
             setattr(s, name, float(to_numpy(_get(mesh, state, name))))
         self.s = s
-        self._base_src: dict[str, int] = {}
+        self._base_src: dict[str, Any] = {}
         self.set_base(mesh, state)
         self._work: dict[tuple[int, bool], torch.Tensor] = {}
         self._kscratch: torch.Tensor | None = None
@@ -152,11 +152,28 @@ class DeviceMesh:
                 if name not in self.t:
                     self.bind_base(name, torch.zeros((self.nc + 1, self.L), dtype=torch.float64, device=self.device))
                 continue
-            if self._base_src.get(name) == id(src) and name in self.t:
+            if name in self.t and self._same_source(name, src):
                 continue
-            t = torch.as_tensor(np.ascontiguousarray(to_numpy(src).astype(np.float64)), device=self.device)
+            if isinstance(src, torch.Tensor):
+                t = src.detach().to(device=self.device, dtype=torch.float64).contiguous().clone()
+            else:
+                t = torch.as_tensor(np.ascontiguousarray(to_numpy(src).astype(np.float64)), device=self.device)
             self.bind_base(name, t)
-            self._base_src[name] = id(src)
+            self._base_src[name] = self._source_key(src)
+
+    @staticmethod
+    def _source_key(src: Any):
+        """What identifies a bound base-state source: the object itself (held,
+        so its id cannot be reused) and, for a tensor, its version counter,
+        which every in-place change advances.  Other sources (NumPy arrays)
+        carry no version and are uploaded again at every binding."""
+        if isinstance(src, torch.Tensor):
+            return (src, src._version)
+        return None
+
+    def _same_source(self, name: str, src: Any) -> bool:
+        key = self._base_src.get(name)
+        return key is not None and key[0] is src and key[1] == src._version
 
     def bind_base(self, name: str, t: torch.Tensor) -> None:
         """Bind a device tensor as one of the base-state fields."""

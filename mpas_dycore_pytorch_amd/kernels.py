@@ -208,14 +208,27 @@ ICD_OUT = ("theta_m", "rho_zz", "ru", "rw", "rho_p", "rtheta_base", "rtheta_p", 
            "pressure_p", "pressure_base")
 
 
-def init_coupled_diagnostics(dm: DeviceMesh, theta, rho, qv, u, w, theta_base) -> dict:
-    """Cold-start coupling (``atm_init_coupled_diagnostics``) of raw padded fields."""
-    ins = [_d(x, dm) for x in (theta, rho, qv, u, w, theta_base)]
+def init_coupled_diagnostics(dm: DeviceMesh, theta, rho, qv, u, w, theta_base, theta_m=None,
+                             rho_zz=None) -> dict:
+    """Cold-start coupling (``atm_init_coupled_diagnostics``) of padded fields.
+
+    From raw ``theta``, ``rho``, ``qv``; or, when ``theta_m`` and ``rho_zz``
+    are given, from those (the reference's derived path,
+    ``init_coupled_diagnostics.py:256-262``), ``theta``/``rho``/``qv`` unused."""
+    derived = theta_m is not None and rho_zz is not None
     nc, ne, L = dm.nc, dm.ne, dm.L
     out = {k: _z(dm, nc + 1, L) for k in ICD_OUT}
     out["ru"] = _z(dm, ne + 1, L)
     out["rw"] = _z(dm, nc + 1, L + 1)
-    N.check(N.load().mpas_hip_init_coupled_diagnostics(dm.ref, *[_p(x) for x in ins],
+    uw = [_d(x, dm) for x in (u, w, theta_base)]
+    if derived:
+        out["theta_m"].copy_(_d(theta_m, dm))
+        out["rho_zz"].copy_(_d(rho_zz, dm))
+        raw = [None, None, None]
+    else:
+        keep = [_d(x, dm) for x in (theta, rho, qv)]  # alive across the launch
+        raw = [_p(x) for x in keep]
+    N.check(N.load().mpas_hip_init_coupled_diagnostics(dm.ref, *raw, *[_p(x) for x in uw],
                                                        *[_p(out[k]) for k in ICD_OUT], _s(dm)),
             "init_coupled_diagnostics")
     return out

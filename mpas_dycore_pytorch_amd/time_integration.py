@@ -17,6 +17,7 @@ callers that integrate many steps (``run_simulation`` style).
 from __future__ import annotations
 
 import ctypes as C
+import warnings
 from typing import Any, Mapping
 
 import numpy as np
@@ -51,7 +52,6 @@ ALIASES = {"smdiv": "config_smdiv", "len_disp": "config_len_disp", "monotonic": 
 
 # Options whose non-default values select code the GPU step does not have.
 REQUIRED_VALUES = {
-    "config_time_integration_order": 3,
     "config_split_dynamics_transport": True,
     "config_horiz_mixing": "2d_smagorinsky",
     "config_h_mom_eddy_visc2": 0.0,
@@ -95,22 +95,63 @@ def _same(a: Any, b: Any) -> bool:
         return False
 
 
+def resolve_len_disp(config_len_disp: float, mesh: Mapping[str, Any], state: Mapping[str, Any] | None = None,
+                     strict: bool = True) -> float:
+    """The horizontal length scale of the damping and mixing coefficients.
+
+    A positive ``config_len_disp`` is used as given; otherwise the mesh's
+    ``nominalMinDc``; failing that the smallest positive ``dcEdge`` over rows
+    1..nEdges, with a ``RuntimeWarning`` (the reference's fallbacks,
+    ``initialization.py:365-380`` and ``time_integration.py:1882-1940``)."""
+    if config_len_disp > 0.0:
+        return float(config_len_disp)
+    src = dict(state or {})
+    src.update(mesh)
+    nominal = src.get("nominalMinDc")
+    if nominal is not None:
+        v = float(np.asarray(to_numpy(nominal)).reshape(-1)[0])
+        if v > 0.0:
+            return v
+    dc = src.get("dcEdge")
+    if dc is not None:
+        a = np.asarray(to_numpy(dc), dtype=np.float64).reshape(-1)
+        ne = src.get("nEdges")
+        if ne is not None and a.shape[0] >= int(to_numpy(ne)) + 1:
+            a = a[1: int(to_numpy(ne)) + 1]
+        pos = a[a > 0.0]
+        if pos.size:
+            fallback = float(pos.min())
+            warnings.warn(f"config_len_disp <= 0 and no nominalMinDc: using the smallest dcEdge {fallback:.6f}",
+                          RuntimeWarning, stacklevel=3)
+            return fallback
+    msg = ("config_len_disp <= 0 and the mesh has neither nominalMinDc nor a positive dcEdge to fall back on; "
+           "set config_len_disp > 0")
+    if strict:
+        raise MpasConfigError(msg)
+    return 0.0
+
+
 def resolve_config(config: Mapping[str, Any] | None, state: Mapping[str, Any], mesh: Mapping[str, Any],
-                   dt: float, n_scalars: int) -> N.ConfigStruct:
+                   dt: float, n_scalars: int, strict: bool = True) -> N.ConfigStruct:
     """Check the configuration against what the GPU step implements and pack it."""
     c = _cfg(config)
     for key, want in REQUIRED_VALUES.items():
         if key in c and not _same(c[key], want):
             raise MpasUnsupportedCapabilityError(capability=key, requested=c[key])
+    order = c["config_time_integration_order"]
+    try:
+        order_ok = not isinstance(order, bool) and float(order) == int(order) and int(order) in (2, 3)
+    except (TypeError, ValueError):
+        order_ok = False
+    if not order_ok:
+        raise MpasConfigError(f"config_time_integration_order must be 2 or 3, got {order!r}")
     dt = float(dt)
     if not dt > 0.0:
         raise MpasConfigError(f"dt must be > 0, got {dt}")
     ns, split = int(c["config_number_of_sub_steps"]), int(c["config_dynamics_split_steps"])
     if ns < 1 or split < 1:
         raise MpasConfigError("config_number_of_sub_steps and config_dynamics_split_steps must be >= 1")
-    len_disp = float(c["config_len_disp"])
-    if not len_disp > 0.0:
-        raise MpasConfigError("config_len_disp must be set to a positive length (e.g. the nominal dc)")
+    len_disp = resolve_len_disp(float(c["config_len_disp"]), mesh, state, strict=strict)
     coef3 = float(c["config_coef_3rd_order"])
     if "coef_3rd_order" in state and float(to_numpy(state["coef_3rd_order"])) != coef3:
         raise MpasUnsupportedCapabilityError(
@@ -129,6 +170,7 @@ def resolve_config(config: Mapping[str, Any] | None, state: Mapping[str, Any], m
     if s.moist_start < 1 or s.moist_end > n_scalars:
         raise MpasConfigError(f"moist species range {s.moist_start}..{s.moist_end} outside 1..{n_scalars}")
     s.diagnostics_current = 0
+    s.order = int(order)
     return s
 
 
@@ -207,7 +249,7 @@ class Srk3Stepper:
                 "vertex": torch.as_tensor(np.concatenate([perm.vertex_old_to_new, [nv]]), device=self.device),
             }
         self.mesh = DeviceMesh.get(mesh, self.device, state)
-        self.cfg = resolve_config(config, state, mesh, dt, n_scalars)
+        self.cfg = resolve_config(config, state, mesh, dt, n_scalars, strict=strict)
         dm = self.mesh
         missing = [k for k in PROGNOSTIC_KEYS[:5] + CARRIED_KEYS if k not in state]
         if missing:

@@ -12,11 +12,12 @@ kernels of ``mpas_oracle.c``:
   (stages 1 and 2), dynamics tendency, small-step perturbation, acoustic
   substeps with divergence damping, large-step recovery, diagnostic solve;
 * split-flux averaging and substep state reset (:3820-3942);
-* split scalar transport RK3 (:3976-4675): explicit stages 1-2, monotonic
-  stage 3, stage dt = [dt/3, dt/2, dt];
+* split scalar transport (:3976-4675): explicit stages 1-2, monotonic
+  stage 3, stage dt = [dt/3, dt/2, dt] (RK2: [dt/2, dt/2, dt]);
 * cell-vector reconstruction (:4693-4754).
 
-Stage weights follow ``compute_rk_weights`` (:2787-2856).
+Stage weights follow ``compute_rk_weights`` (:2787-2856) for
+``config_time_integration_order`` 3 or 2 (the reference's default).
 """
 
 from __future__ import annotations
@@ -26,17 +27,22 @@ import numpy as np
 from . import oracle as O  # noqa: F401  (package-relative import when used as oracle.srk3)
 
 
-def _weights(dt_dyn: float, ns: int):
-    rk_timestep = [dt_dyn / 3.0, dt_dyn / 2.0, dt_dyn]
-    rk_sub = [dt_dyn / 3.0, dt_dyn / float(ns), dt_dyn / float(ns)]
-    nsub = [1, max(1, ns // 2), ns]
-    return rk_timestep, rk_sub, nsub
+def _weights(dt_dyn: float, ns: int, order: int = 3):
+    """``compute_rk_weights`` (time_integration.py:2787-2856): stage lengths,
+    acoustic substep lengths and substep counts of the three stages."""
+    half = max(1, ns // 2)
+    if order == 2:
+        return [dt_dyn / 2.0, dt_dyn / 2.0, dt_dyn], [dt_dyn / float(ns)] * 3, [half, half, ns]
+    if order != 3:
+        raise ValueError(f"time integration order {order} (2 or 3)")
+    return [dt_dyn / 3.0, dt_dyn / 2.0, dt_dyn], [dt_dyn / 3.0, dt_dyn / float(ns), dt_dyn / float(ns)], [1, half, ns]
 
 
-def _explicit_weights(rk_step: int) -> tuple[float, float]:
-    """``_explicit_scalar_transport_density_time_weights`` (advance_scalars.py)."""
+def _explicit_weights(rk_step: int, order: int = 3) -> tuple[float, float]:
+    """``_explicit_scalar_transport_density_time_weights`` (advance_scalars.py:271-286)
+    with advance_density = 1 (split transport)."""
     if rk_step == 1:
-        wn = 1.0 / 3.0
+        wn = 1.0 / 2.0 if order == 2 else 1.0 / 3.0
     elif rk_step == 2:
         wn = 1.0 / 2.0
     else:
@@ -49,7 +55,7 @@ DIAG_KEYS = ("h_edge", "v", "divergence", "vorticity", "ke", "pv_edge")
 
 def srk3(state: dict, mesh: "O.Mesh", dt: float, *, len_disp: float, ns: int = 2, split: int = 3,
          epssm: float = 0.1, smdiv: float = 0.1, apvm: float = 0.5, coef3: float = 0.25,
-         monotonic: bool = True, diagnostics_current: bool = False) -> dict:
+         monotonic: bool = True, diagnostics_current: bool = False, order: int = 3) -> dict:
     """One large step on NumPy arrays in the reference layout; returns a new dict."""
     st = {k: (np.array(v, dtype=np.float64, copy=True) if isinstance(v, np.ndarray) and v.dtype.kind == "f" else v)
           for k, v in state.items()}
@@ -75,7 +81,7 @@ def srk3(state: dict, mesh: "O.Mesh", dt: float, *, len_disp: float, ns: int = 2
                 "rw_p": np.zeros((nc + 1, L + 1)), "wwAvg": np.zeros((nc + 1, L + 1))}
 
     dt_dyn = dt / split
-    rk_timestep, rk_sub, nsub = _weights(dt_dyn, ns)
+    rk_timestep, rk_sub, nsub = _weights(dt_dyn, ns, order)
     moist = None
     vcoef = None
     carried = None
@@ -88,7 +94,9 @@ def srk3(state: dict, mesh: "O.Mesh", dt: float, *, len_disp: float, ns: int = 2
                     d = O.solve_diagnostics(mesh, dt, apvm, None, st["u"], st["rho_zz"])
                     st.update({k: d[k] for k in DIAG_KEYS})
                     st["rho_edge"] = d["h_edge"]
-            if rk in (1, 2):
+            # RK3 recomputes the coefficients for stage 2; RK2 keeps stage 1's
+            # (time_integration.py:5558-5569)
+            if rk == 1 or (rk == 2 and order == 3):
                 vcoef = O.vert_imp_coefs(mesh, rk_sub[rk - 1], epssm, moist["cqw"], st["exner"],
                                          st["theta_m"], st["rtheta_p"], moist["qtot"])
             fields = {
@@ -148,12 +156,13 @@ def srk3(state: dict, mesh: "O.Mesh", dt: float, *, len_disp: float, ns: int = 2
     rho_zz_new = st["rho_zz"].copy()
     # split transport (time_integration.py:3976-4675)
     for rk_t in (1, 2, 3):
-        dt_t = [dt / 3.0, dt / 2.0, dt][rk_t - 1]
+        # stage lengths (time_integration.py:3367-3371)
+        dt_t = [dt / 2.0 if order == 2 else dt / 3.0, dt / 2.0, dt][rk_t - 1]
         if rk_t == 3 and monotonic:
             scalars_new, _ = O.advance_scalars_mono(mesh, dt_t, coef3, scalars_old, scalars_new,
                                                     rho_zz_old_split, ruAvg, wwAvg)
         else:
-            wo, wn = _explicit_weights(rk_t)
+            wo, wn = _explicit_weights(rk_t, order)
             scalars_new, _ = O.advance_scalars(mesh, dt_t, coef3, wo, wn, scalars_old, scalars_new,
                                                rho_zz_old_split, rho_zz_new, ruAvg, wwAvg)
     st["scalars"] = scalars_new

@@ -59,6 +59,11 @@ CASES = {
 TERRAIN_CASES = {
     "l2_L20_terrain": (2, 20, "quadratic", 1, 900.0, 2, 6060, 1500.0),
 }
+# whole-step cases under the reference's default configuration
+# (config_defaults.py: RK2, config_len_disp 0 resolved from nominalMinDc)
+DEFAULT_CONFIG_CASES = {
+    "l2_L20_defaults": (2, 20, "quadratic", 1, 900.0, 2, 8080),
+}
 SUMMARY_CASES = {
     "cfg1_l4_L26": (4, 26, "uniform", 0, 1800.0, 2, 20240),
 }
@@ -298,13 +303,13 @@ def _state_arrays(st: dict, keys=None) -> dict:
     return out
 
 
-def run_case(level, n_levels, stretch, n_tracers, dt, steps, seed, terrain=0.0):
+def run_case(level, n_levels, stretch, n_tracers, dt, steps, seed, terrain=0.0, defaults=False):
     m = M.build_mesh(level, n_levels, stretch=stretch)
     if terrain:
         m = add_synthetic_terrain(m, terrain)
     raw = IS.raw_state(m, seed=seed, n_tracers=n_tracers)
     st, mesh = prepared_reference_state(m, raw, dt)
-    cfg = reference_config(m, dt)
+    cfg = R.cfg.validate_config({"config_dt": float(dt)}) if defaults else reference_config(m, dt)
     prepared = dict(st)
     outs = []
     cur = st
@@ -318,8 +323,11 @@ def run_case(level, n_levels, stretch, n_tracers, dt, steps, seed, terrain=0.0):
 
 
 def write_srk3_case(name: str, out_dir: str) -> None:
-    level, L, stretch, ntr, dt, steps, seed, *terrain = CASES.get(name) or TERRAIN_CASES[name]
-    m, raw, prepared, outs, times = run_case(level, L, stretch, ntr, dt, steps, seed, *terrain)
+    defaults = name in DEFAULT_CONFIG_CASES
+    level, L, stretch, ntr, dt, steps, seed, *terrain = (CASES.get(name) or TERRAIN_CASES.get(name)
+                                                         or DEFAULT_CONFIG_CASES[name])
+    m, raw, prepared, outs, times = run_case(level, L, stretch, ntr, dt, steps, seed, *(terrain or [0.0]),
+                                             defaults=defaults)
     payload = {}
     for k, v in _state_arrays(prepared).items():
         payload["in__" + k] = v
@@ -329,7 +337,8 @@ def write_srk3_case(name: str, out_dir: str) -> None:
     meta = {"case": name, "level": level, "nVertLevels": L, "stretch": stretch,
             "n_tracers": ntr, "dt": dt, "steps": steps, "seed": seed,
             "terrain_height": terrain[0] if terrain else 0.0,
-            "reference_seconds_per_step": times}
+            "reference_seconds_per_step": times,
+            "config": "reference defaults (config_defaults.py)" if defaults else "RK3, len_disp = nominalMinDc"}
     path = os.path.join(out_dir, f"srk3_{name}.npz")
     np.savez_compressed(path, __meta__=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8),
                         **payload)
@@ -378,7 +387,7 @@ def main() -> None:
     if args.which in ("kernels", "all"):
         record_kernels(1, 10, "quadratic", 1, 900.0, 99, os.path.join(args.out, "kernels_l1_L10.npz"))
     if args.which in ("srk3", "all"):
-        for name in list(CASES) + list(TERRAIN_CASES):
+        for name in list(CASES) + list(TERRAIN_CASES) + list(DEFAULT_CONFIG_CASES):
             if args.case is None or args.case == name:
                 write_srk3_case(name, args.out)
     if args.which in ("summary", "all"):

@@ -78,16 +78,30 @@ def test_config_gate():
     cfg = resolve_config(ok, st, st, meta["dt"], 2)
     assert cfg.ns == 2 and cfg.split == 3 and cfg.monotonic == 1 and cfg.nScalars == 2
     assert cfg.moist_start == 1 and cfg.moist_end == 1
-    with pytest.raises(MpasUnsupportedCapabilityError):
-        resolve_config({**ok, "config_time_integration_order": 2}, st, st, meta["dt"], 2)
+    assert cfg.order == 3
+    # RK2 is the reference's default order (config_defaults.py:34-35)
+    assert resolve_config({**ok, "config_time_integration_order": 2}, st, st, meta["dt"], 2).order == 2
+    assert resolve_config({"config_len_disp": 1.0}, st, st, meta["dt"], 2).order == 2
+    for bad in (1, 4, 2.5, "three", True):
+        with pytest.raises(MpasConfigError):
+            resolve_config({**ok, "config_time_integration_order": bad}, st, st, meta["dt"], 2)
     with pytest.raises(MpasUnsupportedCapabilityError):
         resolve_config({**ok, "config_horiz_mixing": "2d_fixed"}, st, st, meta["dt"], 2)
     with pytest.raises(MpasUnsupportedCapabilityError):
         resolve_config({**ok, "config_rayleigh_damp_u": True}, st, st, meta["dt"], 2)
     with pytest.raises(MpasConfigError):
-        resolve_config({"config_time_integration_order": 3}, st, st, meta["dt"], 2)  # no len_disp
-    with pytest.raises(MpasConfigError):
         resolve_config(ok, st, st, -1.0, 2)
+    # config_len_disp <= 0 (the reference default) resolves from nominalMinDc, then from
+    # the smallest dcEdge of rows 1..nEdges (initialization.py:365-380, time_integration.py:1882-1940)
+    assert resolve_config({}, st, st, meta["dt"], 2).len_disp == float(st["nominalMinDc"])
+    no_nominal = {k: v for k, v in st.items() if k != "nominalMinDc"}
+    ne = int(st["nEdges"])
+    want = float(np.min(st["dcEdge"][1: ne + 1][st["dcEdge"][1: ne + 1] > 0]))
+    with pytest.warns(RuntimeWarning):
+        assert resolve_config({}, no_nominal, no_nominal, meta["dt"], 2).len_disp == want
+    bare = {k: v for k, v in no_nominal.items() if k != "dcEdge"}
+    with pytest.raises(MpasConfigError):
+        resolve_config({}, bare, bare, meta["dt"], 2)
     # short aliases are honoured as in the reference
     cfg = resolve_config({**ok, "smdiv": 0.2, "monotonic": False}, st, st, meta["dt"], 2)
     assert cfg.smdiv == 0.2 and cfg.monotonic == 0

@@ -65,6 +65,24 @@ def test_srk3_whole_steps(case):
                 assert err <= 1e-14, f"{case} step {step + 1} {k}: rel err {err:.3e}"
 
 
+def test_srk3_whole_steps_rk2_reference_defaults():
+    """The reference's default configuration (RK2, config_len_disp 0 resolved from nominalMinDc):
+    the oracle's RK2 orchestration matches the reference's two steps."""
+    st, outs, meta = srk3_case("l2_L20_defaults")
+    mesh = O.Mesh(st)
+    cur = st
+    for step, ref in enumerate(outs):
+        cur = OS.srk3(cur, mesh, meta["dt"], len_disp=float(st["nominalMinDc"]), order=2,
+                      diagnostics_current=(step > 0))
+        for k in FIELDS:
+            if k in ref:
+                err = rel_err(cur[k], ref[k])
+                assert err <= 1e-14, f"rk2 step {step + 1} {k}: rel err {err:.3e}"
+    # a plausible slip (RK3 weights) is caught by the same bar
+    wrong = OS.srk3(st, mesh, meta["dt"], len_disp=float(st["nominalMinDc"]), order=3)
+    assert rel_err(wrong["theta_m"], outs[0]["theta_m"]) > 1e-12
+
+
 @pytest.mark.slow
 def test_level4_summary_from_own_cold_start():
     from mpas_dycore_pytorch_amd import initial_state as IS
