@@ -91,6 +91,19 @@ int mpas_hip_srk3_step(const mpas_hip_mesh *mesh, mpas_hip_state *state,
                        const mpas_hip_workspace *work, const mpas_hip_config *cfg,
                        void *stream);
 
+/* The large step of mpas_hip_srk3_step without a workspace argument and
+ * without any allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The workspace is a device array inside
+ * the library (mpas_hip_srk3_step_noalloc_capacity() doubles, 128 MiB, held
+ * by each GPU the library runs on until the process ends); a mesh whose
+ * workspace (mpas_hip_workspace_doubles) exceeds it is rejected with an
+ * error.  The call clears the part it uses with a fill kernel first.  All
+ * calls on one GPU share the array: calls that overlap on one GPU race. */
+int mpas_hip_srk3_step_noalloc(const mpas_hip_mesh *mesh, mpas_hip_state *state,
+                               const mpas_hip_config *cfg, void *stream);
+int64_t mpas_hip_srk3_step_noalloc_capacity(void);
+
 /* Sizes of the ABI structs, for binding checks. */
 int mpas_hip_struct_sizes(int32_t *mesh, int32_t *state, int32_t *workspace, int32_t *config);
 

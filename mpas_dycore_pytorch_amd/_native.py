@@ -87,6 +87,8 @@ SIGNATURES = {
     "mpas_hip_srk3_step": (C.c_int, [_PM, C.POINTER(StateStruct), C.POINTER(WorkspaceStruct),
                                      C.POINTER(ConfigStruct), _vp]),
     "mpas_hip_struct_sizes": (C.c_int, [C.POINTER(_I32)] * 4),
+    "mpas_hip_srk3_step_noalloc": (C.c_int, [_PM, C.POINTER(StateStruct), C.POINTER(ConfigStruct), _vp]),
+    "mpas_hip_srk3_step_noalloc_capacity": (_I64, []),
     "mpas_hip_kernel_scratch_doubles": (_I64, [_I32, _I32, _I32, _I32]),
     "mpas_hip_moist_coefficients": (C.c_int, [_PM, _dp, _I32, _I32, _I32, _dp, _dp, _dp, _vp]),
     "mpas_hip_vert_imp_coefs": (C.c_int, [_PM, _F64, _F64] + [_dp] * 13 + [_vp]),

@@ -3,6 +3,8 @@
 // 3420-4894, stage body execute_rk_step :5000-9154).  Host code only
 // enqueues kernels on the caller's stream: no allocation, no host sync.
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -271,6 +273,42 @@ static int validate_mesh(const mpas_hip_mesh *m) {
     }
     return 0;
 }
+
+#if !(defined(MPAS_MIXED) && MPAS_MIXED)
+// ---------------------------------------------------------------------------
+// Workspace of mpas_hip_srk3_step_noalloc: a file-scope device array, so the
+// call needs no allocation.  Every GPU the library runs on holds its own copy
+// (128 MiB) from the module's load until the process ends; its device address
+// is looked up once per device (hipGetSymbolAddress) and kept.  All calls on
+// one GPU share the array: calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr size_t kStaticWorkDoubles = (size_t)16 << 20;  // 128 MiB of doubles
+static __device__ __align__(256) double step_static_work[kStaticWorkDoubles];
+
+constexpr int kStaticWorkDevices = 64;
+static double *g_static_work_addr[kStaticWorkDevices];
+static std::mutex g_static_work_mu;
+
+static double *static_work_address() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kStaticWorkDevices) {
+        set_error("srk3_step_noalloc: no current HIP device (or device index beyond 63)");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_static_work_mu);
+    if (!g_static_work_addr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(step_static_work));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("srk3_step_noalloc: hipGetSymbolAddress of the static workspace failed: ") +
+                      hipGetErrorString(e));
+            return nullptr;
+        }
+        g_static_work_addr[dev] = static_cast<double *>(p);
+    }
+    return g_static_work_addr[dev];
+}
+#endif
 
 static int validate_moist(int S, int ms, int me) {
     if (S < 0 || ms < 1 || me > S) {
@@ -732,6 +770,33 @@ int mpas_hip_srk3_step(const mpas_hip_mesh *mesh, mpas_hip_state *state, const m
                        const mpas_hip_config *cfg, void *stream) {
     StepState st = step_state(state);
     return srk3_step_impl(mesh, &st, work, cfg, STREAM(stream));
+}
+
+int64_t mpas_hip_srk3_step_noalloc_capacity(void) { return (int64_t)kStaticWorkDoubles; }
+
+int mpas_hip_srk3_step_noalloc(const mpas_hip_mesh *mesh, mpas_hip_state *state, const mpas_hip_config *cfg,
+                               void *stream) {
+    if (validate_mesh(mesh)) return 1;
+    if (!state || !cfg) {
+        set_error("srk3_step_noalloc: null state or config");
+        return 1;
+    }
+    const Work need = carve_work(*mesh, cfg->nScalars, nullptr);
+    if (need.total > kStaticWorkDoubles) {
+        set_error("srk3_step_noalloc: this mesh needs " + std::to_string(need.total * 8 / 1024) +
+                  " KiB of workspace, beyond the " + std::to_string(kStaticWorkDoubles * 8 / 1024) +
+                  " KiB the library holds per GPU; call mpas_hip_srk3_step with a workspace instead");
+        return 1;
+    }
+    double *base = static_work_address();
+    if (!base) return 1;
+    hipStream_t s = STREAM(stream);
+    // the array keeps the previous call's contents: clear the carved part, as a
+    // freshly allocated workspace is (a fill kernel, so the call stays capturable)
+    fill_zero(base, need.total, s);
+    const mpas_hip_workspace ws{base, (int64_t)kStaticWorkDoubles};
+    StepState st = step_state(state);
+    return srk3_step_impl(mesh, &st, &ws, cfg, s);
 }
 #endif
 
