@@ -393,9 +393,8 @@ def main() -> int:
     torch.cuda.synchronize()
     log(f"warmup {args.warmup} steps done")
 
-    per_step = {"k_acoustic_cells": 12, "k_acoustic_edges": 12, "k_divdamp": 12, "k_dyn_cells_b": 9,
-                "k_dyn_edges_a": 9, "k_dyn_edges_b": 9, "k_dyn_cells_a": 9, "k_recover_cells": 9,
-                "k_recover_edges": 9, "k_recover_w": 9, "k_smlstep": 9, "k_vimp": 6}.get(args.roofline_kernel, 64)
+    schedule = B.launch_schedule(args.roofline_kernel)  # variant of each launch in one large step
+    per_step = len(schedule)
     if not args.graph:
         N.check(lib.mpas_hip_timer_arm(args.roofline_kernel.encode(), per_step * args.steps), "timer_arm")
     stream = torch.cuda.current_stream(device)
@@ -406,14 +405,19 @@ def main() -> int:
     ev1.record(stream)
     torch.cuda.synchronize()
     total_ms = ev0.elapsed_time(ev1)
+    timed_steps = args.steps
     if args.graph:
         # a replayed graph carries no timer events: time the kernel in one eager step
         N.check(lib.mpas_hip_timer_arm(args.roofline_kernel.encode(), per_step), "timer_arm")
         with torch.cuda.device(device):
             stepper._launch()
         torch.cuda.synchronize()
-    kt, kn = C.c_double(), C.c_int32()
-    N.check(lib.mpas_hip_timer_read(C.byref(kt), C.byref(kn)), "timer_read")
+        timed_steps = 1
+    each = (C.c_double * (per_step * timed_steps))()
+    kn = C.c_int32()
+    N.check(lib.mpas_hip_timer_read_each(each, per_step * timed_steps, C.byref(kn)), "timer_read_each")
+    if kn.value != per_step * timed_steps:
+        raise RuntimeError(f"{args.roofline_kernel}: {kn.value} launches timed, {per_step * timed_steps} expected")
     for k, v in stepper.state_dict().items():
         if not bool(torch.isfinite(v).all()):
             raise RuntimeError(f"non-finite {k} after the timed steps")
@@ -422,15 +426,27 @@ def main() -> int:
     value = units * args.steps / (total_ms / 1e3)
     log(f"{args.steps} steps: {ms_per_step:.3f} ms/step, {value:.4e} cell-level-steps/s")
 
-    k_avg_ms = kt.value / max(kn.value, 1)
-    k_bytes = B.kernel_bytes(args.roofline_kernel, nC, nE, nV, L, S)
+    # each launch is charged the algorithmic bytes of its own variant (first or later acoustic
+    # substep, first or later RK stage), in launch order
+    variants = schedule * timed_steps
+    ms = [each[i] for i in range(kn.value)]
+    k_total_ms = sum(ms)
+    k_avg_ms = k_total_ms / kn.value
+    k_bytes = B.schedule_bytes(args.roofline_kernel, variants, nC, nE, nV, L, S) / kn.value
     k_gbs = k_bytes / (k_avg_ms / 1e3) / 1e9
+    by_variant = {}
+    for v in sorted(set(variants)):
+        vm = [t for t, vv in zip(ms, variants) if vv == v]
+        vb = B.kernel_bytes(args.roofline_kernel, nC, nE, nV, L, S, v)
+        vavg = sum(vm) / len(vm)
+        by_variant[v] = {"launches": len(vm), "avg_ms": round(vavg, 5), "algorithmic_bytes": vb,
+                         "achieved_gbs": round(vb / (vavg / 1e3) / 1e9, 1)}
     workload_tag = f"l{args.level}_L{args.levels}"
     roof = {"bound": "hbm", "kernel": args.roofline_kernel, "achieved": round(k_gbs, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(k_gbs / HBM_PEAK_GBS, 4),
             "traffic": traffic_from_profile(args.roofline_kernel, workload_tag),
-            "algorithmic_bytes_per_launch": k_bytes, "avg_launch_ms": round(k_avg_ms, 5),
-            "launches_timed": kn.value}
+            "algorithmic_bytes_per_launch": round(k_bytes), "avg_launch_ms": round(k_avg_ms, 5),
+            "launches_timed": kn.value, "variants": by_variant}
     sb = B.step_bytes(nC, nE, nV, L, S)
     step_gbs = sb / (ms_per_step / 1e3) / 1e9
     step_roof = {"algorithmic_bytes": sb, "achieved": round(step_gbs, 1), "unit": "GB/s",

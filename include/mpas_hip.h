@@ -388,6 +388,8 @@ const char *mpas_hip_last_error(void);
  * device time and the launch count, then disarms. */
 int mpas_hip_timer_arm(const char *kernel_name, int32_t capacity);
 int mpas_hip_timer_read(double *total_ms, int32_t *launches);
+/* The same, per launch in launch order (ms_each holds capacity entries). */
+int mpas_hip_timer_read_each(double *ms_each, int32_t capacity, int32_t *launches);
 
 #ifdef __cplusplus
 }

@@ -136,6 +136,7 @@ SIGNATURES = {
     "mpas_hip_last_error": (C.c_char_p, []),
     "mpas_hip_timer_arm": (C.c_int, [C.c_char_p, _I32]),
     "mpas_hip_timer_read": (C.c_int, [C.POINTER(C.c_double), C.POINTER(_I32)]),
+    "mpas_hip_timer_read_each": (C.c_int, [C.POINTER(C.c_double), _I32, C.POINTER(_I32)]),
 }
 
 _lib = None

@@ -71,7 +71,8 @@ class Inventory:
 
 # ---------------------------------------------------------------------------
 # reference functions, per call (RK3, split 3, 2 acoustic substeps: the
-# per-stage counts are 1, 1, 2 substeps -> 4 per split step, 12 per step)
+# per-stage counts are 1, 1, 2 substeps -> 4 per split step, 12 per step, of
+# which 9 are the first substep of their stage)
 # ---------------------------------------------------------------------------
 FUNCTIONS: dict[str, Inventory] = {
     # moist_coefficients.py:124: scalars -> cqw, cqu, qtot
@@ -96,11 +97,19 @@ FUNCTIONS: dict[str, Inventory] = {
         ("E", "W", "C", "C", "C")),
     # smlstep_pert.py:373: u_tend, w_tend, zz (+ flat-terrain zb/zb3 skipped) -> w_tend
     "atm_set_smlstep_pert_variables": Inventory(9, ("E", "W", "C", "cell_e"), ("W",)),
-    # acoustic_step.py:1297: ru_p, ruAvg, tend_ru, cqu, zxu (E); rho_pp, rtheta_pp, tend_rho, tend_rt,
-    # cofwr, cofwz, cofwt, a, alpha, gamma, zz, rho_zz, theta_m, exner, dss (C); rw_p, wwAvg, tend_rw,
-    # coftz, rw_save, rw, w (W) -> ru_p, ruAvg (E), rho_pp, rtheta_pp, rtheta_pp_old (C), rw_p, wwAvg (W)
-    "atm_advance_acoustic_step": Inventory(
-        12, ("E",) * 5 + ("C",) * 15 + ("W",) * 7 + ("cell_e", "edge_c", "L1"),
+    # acoustic_step.py:1297, a later substep (small_step > 1; 3 per step): ru_p, ruAvg, tend_ru, cqu,
+    # zxu (E); rho_pp, rtheta_pp, tend_rho, tend_rt, cofwr, cofwz, cofwt, a, alpha, gamma, zz, rho_zz,
+    # theta_m, exner, dss (C); rw_p, wwAvg, tend_rw, coftz, rw_save, rw, w (W) -> ru_p, ruAvg (E),
+    # rho_pp, rtheta_pp, rtheta_pp_old (C), rw_p, wwAvg (W)
+    "atm_advance_acoustic_step[later substeps]": Inventory(
+        3, ("E",) * 5 + ("C",) * 15 + ("W",) * 7 + ("cell_e", "edge_c", "L1"),
+        ("E", "E", "C", "C", "C", "W", "W")),
+    # the first substep of a stage (small_step == 1; 9 per step, acoustic_step.py:1702-1761): the
+    # perturbations start from zero, so no ru_p, ruAvg, cqu, zxu, rho_pp, rtheta_pp, exner, rw_p or
+    # wwAvg is read: tend_ru (E); tend_rho, tend_rt, cofwr, cofwz, cofwt, a, alpha, gamma, zz,
+    # rho_zz, theta_m, dss (C); tend_rw, coftz, rw_save, rw, w (W) -> the same writes
+    "atm_advance_acoustic_step[first substep]": Inventory(
+        9, ("E",) + ("C",) * 12 + ("W",) * 5 + ("cell_e", "edge_c", "L1"),
         ("E", "E", "C", "C", "C", "W", "W")),
     # divergence_damping.py:299: ru_p, theta_m, rtheta_pp, rtheta_pp_old -> ru_p
     "atm_divergence_damping_3d": Inventory(12, ("E", "C", "C", "C", "edge_c"), ("E",)),
@@ -144,39 +153,113 @@ def step_bytes_breakdown(nC: int, nE: int, nV: int, L: int, S: int) -> dict[str,
 
 
 # ---------------------------------------------------------------------------
-# HIP kernels of this library, per launch (general-stage variant)
+# HIP kernels of this library, per launch, per variant.  A kernel whose inputs
+# depend on the RK stage or the acoustic substep has one inventory per variant;
+# ``launch_schedule`` gives the variant of every launch of one large step, in
+# launch order, so a timed set of launches is charged launch by launch.
 # ---------------------------------------------------------------------------
-KERNELS: dict[str, Inventory] = {
-    # column Thomas acoustic cell pass (small_step > 1): ru_p (E); theta_m, tend_rho, tend_rt,
+KERNEL_VARIANTS: dict[str, dict[str, Inventory]] = {
+    # column Thomas acoustic cell pass.  later substeps: ru_p (E); theta_m, tend_rho, tend_rt,
     # rho_pp, rtheta_pp, cofwz, cofwr, cofwt, zz, a, alpha, gamma, rho_zz, dss (C); rw_p, wwAvg,
     # tend_rw, coftz, rw_save, rw, w (W); cofrz, rdzw, fzm, fzp (L) -> rtheta_pp_old, rho_pp,
-    # rtheta_pp (C), rw_p, wwAvg (W)
-    "k_acoustic_cells": Inventory(1, ("E",) + ("C",) * 14 + ("W",) * 7 + ("L1",) * 4
-                                  + ("cell_e", "edge_c"), ("C", "C", "C", "W", "W")),
-    "k_acoustic_edges": Inventory(1, ("C", "C", "C", "C", "E", "E", "E", "E", "edge_c"), ("E", "E")),
-    "k_divdamp": Inventory(1, ("C", "C", "C", "E", "edge_c"), ("E",)),
-    # w/theta tendency pass: ru, ru_save (E); theta_m, theta_m_save, rho_zz, pp, rt_diab, tend_rho,
-    # tend_theta_euler (C); rw, rw_save, w, tend_w_euler, cqw (W) -> tend_w (W), tend_theta,
-    # rthdynten (C)
-    "k_dyn_cells_b": Inventory(1, ("E", "E") + ("C",) * 7 + ("W",) * 5 + ("cell_e", "edge_c", "edge_adv"),
-                               ("W", "C", "C")),
-    # u tendency pass (rk > 1): u, pv_edge, rho_edge (E), rw (W), ke, h_div (C) -> tend_u (E)
-    "k_dyn_edges_a": Inventory(1, ("E", "E", "E", "W", "C", "C", "edge_c", "edge_e"), ("E",)),
-    "k_dyn_edges_b": Inventory(1, ("E", "E"), ("E",)),
-    # edge advective fluxes (rk > 1): ru, ru_save (E), w (W), theta_m, theta_m_save (C)
-    # -> w flux, theta flux, theta perturbation flux (E)
-    "k_dyn_edge_adv": Inventory(1, ("E", "E", "W", "C", "C", "edge_adv", "edge_c"), ("E", "E", "E")),
-    "k_dyn_cells_a": Inventory(1, ("E", "W", "cell_e", "edge_c"), ("C",)),
-    "k_recover_cells": Inventory(1, ("C",) * 8 + ("W",) * 3, ("C",) * 4 + ("W",) * 2),
-    "k_recover_edges": Inventory(1, ("E", "E", "E", "C", "edge_c"), ("E", "E", "E")),
-    "k_recover_w": Inventory(1, ("E", "W", "C", "C", "cell_e"), ("W",)),
-    "k_smlstep": Inventory(1, ("W", "C"), ("W",)),
-    "k_vimp": Inventory(1, ("C",) * 8 + ("W",), ("C",) * 6 + ("W",)),
-    "k_diag_vertices": Inventory(1, ("E", "E", "vert_e"), ("V", "V", "V")),
-    "k_diag_cells": Inventory(1, ("E", "E", "V", "V", "cell_e", "cell_v", "vert_kite"), ("C", "C", "C")),
-    "k_diag_edges2": Inventory(1, ("E", "V", "C", "C", "edge_c", "edge_v", "edge_e"), ("E", "E", "E")),
+    # rtheta_pp (C), rw_p, wwAvg (W).  first substep: no rho_pp, rtheta_pp, rw_p, wwAvg read
+    "k_acoustic_cells": {
+        "later": Inventory(1, ("E",) + ("C",) * 14 + ("W",) * 7 + ("L1",) * 4 + ("cell_e", "edge_c"),
+                           ("C", "C", "C", "W", "W")),
+        "first": Inventory(1, ("E",) + ("C",) * 12 + ("W",) * 5 + ("L1",) * 4 + ("cell_e", "edge_c"),
+                           ("C", "C", "C", "W", "W")),
+    },
+    # edge pass.  later substeps (with the previous substep's divergence damping folded in):
+    # rtheta_pp, rho_pp, exner, zz, theta_m, rtheta_pp_old (C); cqu, tend_ru, ru_p, ruAvg, zxu (E)
+    # -> ru_p, ruAvg (E).  first substep: tend_ru (E) -> ru_p, ruAvg (E)
+    "k_acoustic_edges": {
+        "later": Inventory(1, ("C",) * 6 + ("E",) * 5 + ("edge_c",), ("E", "E")),
+        "first": Inventory(1, ("E",), ("E", "E")),
+    },
+    # stand-alone divergence damping (per-function entry point only; the step folds it)
+    "k_divdamp": {"any": Inventory(1, ("C", "C", "C", "E", "edge_c"), ("E",))},
+    # Smagorinsky kdiff (first stage only): u, v (E) -> kdiff (C)
+    "k_dyn_kdiff": {"rk1": Inventory(1, ("E", "E", "cell_e", "cell_def"), ("C",))},
+    # mass-flux divergence.  rk1: ru, rho_edge (E); rw, w (W); rr_save, qtot, rho_base, theta_m,
+    # kdiff (C) -> h_divergence, tend_rho, dpdz, delsq_theta, tend_theta_euler (C), delsq_w,
+    # tend_w_euler (W).  rk2/3: ru (E) -> h_divergence (C)
+    "k_dyn_cells_a": {
+        "rk1": Inventory(1, ("E", "E", "W", "W") + ("C",) * 5 + ("cell_e", "edge_c", "edge_1", "L1"),
+                         ("C",) * 5 + ("W", "W")),
+        "rk23": Inventory(1, ("E", "cell_e", "edge_c"), ("C",)),
+    },
+    # u tendency.  rk1 (del4 follows): u, pv_edge, rho_edge, cqu, zxu (E); rw (W); ke, h_div, pp,
+    # dpdz, zz, divergence, kdiff (C); vorticity (V) -> tend_u, tend_u_euler, delsq_u (E).
+    # rk2/3: u, pv_edge, rho_edge, tend_u_euler (E); rw (W); ke, h_div (C) -> tend_u (E)
+    "k_dyn_edges_a": {
+        "rk1": Inventory(1, ("E",) * 5 + ("W",) + ("C",) * 7 + ("V",)
+                         + ("edge_c", "edge_e", "edge_v", "edge_1", "L1"), ("E", "E", "E")),
+        "rk23": Inventory(1, ("E",) * 4 + ("W",) + ("C",) * 2 + ("edge_c", "edge_e", "L1"), ("E",)),
+    },
+    # del4 first half (first stage): delsq_u (E) -> delsq_vort (V), delsq_div (C)
+    "k_dyn_del4_vc": {"rk1": Inventory(1, ("E", "vert_e", "cell_e"), ("V", "C"))},
+    # del4 second half (first stage): rho_edge, tend_u_euler, tend_u (E); delsq_div (C);
+    # delsq_vort (V) -> tend_u_euler, tend_u (E)
+    "k_dyn_edges_b": {"rk1": Inventory(1, ("E", "E", "E", "C", "V", "edge_c", "edge_v", "edge_1"), ("E", "E"))},
+    # advective fluxes through each edge.  rk1: ru (E), w (W), theta_m (C) -> w and theta fluxes (E).
+    # rk2/3: ru, ru_save (E), w (W), theta_m, theta_m_save (C) -> w, theta and theta-perturbation fluxes
+    "k_dyn_edge_adv": {
+        "rk1": Inventory(1, ("E", "W", "C", "edge_adv", "L1"), ("E", "E")),
+        "rk23": Inventory(1, ("E", "E", "W", "C", "C", "edge_adv", "edge_c", "L1"), ("E", "E", "E")),
+    },
+    # w / theta tendencies.  rk1: wflux, tflux (E); w, rw, rw_save, tend_w_euler, cqw, delsq_w (W);
+    # theta_m, theta_m_save, rho_zz, pp, rt_diab, dpdz, delsq_theta, tend_rho, tend_theta_euler (C)
+    # -> tend_w_euler, tend_w (W), tend_theta_euler, tend_theta, rthdynten (C).  rk2/3: wflux, tflux,
+    # tpert (E); w, rw, rw_save, tend_w_euler (W); theta_m, theta_m_save, rho_zz, rt_diab, tend_rho,
+    # tend_theta_euler (C) -> tend_w (W), tend_theta, rthdynten (C)
+    "k_dyn_cells_b": {
+        "rk1": Inventory(1, ("E",) * 2 + ("W",) * 6 + ("C",) * 9 + ("cell_e", "edge_c", "edge_1", "L1"),
+                         ("W", "W", "C", "C", "C")),
+        "rk23": Inventory(1, ("E",) * 3 + ("W",) * 4 + ("C",) * 6 + ("cell_e", "L1"), ("W", "C", "C")),
+    },
+    "k_recover_cells": {"any": Inventory(1, ("C",) * 8 + ("W",) * 3, ("C",) * 4 + ("W",) * 2)},
+    "k_recover_edges": {"any": Inventory(1, ("E", "E", "E", "C", "edge_c"), ("E", "E", "E"))},
+    "k_recover_w": {"any": Inventory(1, ("E", "W", "C", "C", "cell_e"), ("W",))},
+    "k_smlstep": {"any": Inventory(1, ("W", "C"), ("W",))},
+    "k_vimp": {"any": Inventory(1, ("C",) * 8 + ("W",), ("C",) * 6 + ("W",))},
+    "k_diag_vertices": {"any": Inventory(1, ("E", "E", "vert_e"), ("V", "V", "V"))},
+    "k_diag_cells": {"any": Inventory(1, ("E", "E", "V", "V", "cell_e", "cell_v", "vert_kite"), ("C", "C", "C"))},
+    "k_diag_edges2": {"any": Inventory(1, ("E", "V", "C", "C", "edge_c", "edge_v", "edge_e"), ("E", "E", "E"))},
 }
+# one inventory per kernel, for callers that do not split by variant: the general (later / rk2,3) one
+KERNELS: dict[str, Inventory] = {
+    name: v.get("later") or v.get("rk23") or v.get("any") or v.get("rk1") for name, v in KERNEL_VARIANTS.items()}
 
 
-def kernel_bytes(name: str, nC: int, nE: int, nV: int, L: int, S: int) -> int:
-    return KERNELS[name].bytes(nC, nE, nV, L, S)
+def substep_counts(order: int = 3, ns: int = 2) -> list[int]:
+    """Acoustic substeps of the three RK stages (compute_rk_weights, time_integration.py:2787-2856)."""
+    half = max(1, ns // 2)
+    return [half, half, ns] if order == 2 else [1, half, ns]
+
+
+def launch_schedule(name: str, order: int = 3, ns: int = 2, split: int = 3) -> list[str]:
+    """The variant of every launch of kernel ``name`` in one large step, in launch order."""
+    if name in ("k_acoustic_cells", "k_acoustic_edges"):
+        per_split = [("first" if s == 1 else "later") for n in substep_counts(order, ns) for s in range(1, n + 1)]
+        return per_split * split
+    if name in ("k_dyn_cells_a", "k_dyn_edges_a", "k_dyn_edge_adv", "k_dyn_cells_b"):
+        return ["rk1", "rk23", "rk23"] * split
+    if name in ("k_dyn_kdiff", "k_dyn_del4_vc", "k_dyn_edges_b"):
+        return ["rk1"] * split
+    if name in ("k_recover_cells", "k_recover_edges", "k_recover_w", "k_smlstep"):
+        return ["any"] * (3 * split)
+    if name == "k_vimp":
+        return ["any"] * ((2 if order == 3 else 1) * split)
+    return ["any"]
+
+
+def kernel_bytes(name: str, nC: int, nE: int, nV: int, L: int, S: int, variant: str | None = None) -> int:
+    """Algorithmic bytes of one launch of ``name`` (the given variant, or the general one)."""
+    if variant is None:
+        return KERNELS[name].bytes(nC, nE, nV, L, S)
+    return KERNEL_VARIANTS[name][variant].bytes(nC, nE, nV, L, S)
+
+
+def schedule_bytes(name: str, variants: list[str], nC: int, nE: int, nV: int, L: int, S: int) -> int:
+    """Algorithmic bytes of a sequence of launches of ``name`` (variant by variant)."""
+    return sum(kernel_bytes(name, nC, nE, nV, L, S, v) for v in variants)

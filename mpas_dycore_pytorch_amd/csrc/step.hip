@@ -378,6 +378,25 @@ int mpas_hip_timer_arm(const char *kernel_name, int32_t capacity) {
     return 0;
 }
 
+int mpas_hip_timer_read_each(double *ms_each, int32_t capacity, int32_t *launches) {
+    if (!ms_each || !launches || capacity < g_timer.used) {
+        set_error("timer_read_each: need an output array of at least the recorded launch count");
+        return 1;
+    }
+    for (int i = 0; i < g_timer.used; ++i) {
+        float ms = 0.0f;
+        if (hipEventSynchronize(g_timer.stop[i]) != hipSuccess ||
+            hipEventElapsedTime(&ms, g_timer.start[i], g_timer.stop[i]) != hipSuccess) {
+            set_error("timer_read_each: event query failed");
+            return 1;
+        }
+        ms_each[i] = ms;
+    }
+    *launches = g_timer.used;
+    timer_release();
+    return 0;
+}
+
 int mpas_hip_timer_read(double *total_ms, int32_t *launches) {
     double t = 0.0;
     for (int i = 0; i < g_timer.used; ++i) {

@@ -1,0 +1,177 @@
+"""Per-variant summary of a rocprofv3 run of the large step, against the byte model.
+
+    python -m mpas_dycore_pytorch_amd.profile_summary DIR --kernel k_acoustic_cells \\
+        [--level 7 --levels 55 --scalars 1]
+
+``DIR`` is a rocprofv3 output directory (searched recursively) of a run that
+launched ``kernel`` only inside whole large steps (``bench.py`` with its
+derivative and mixed-precision legs off).  Each dispatch of the kernel is
+labelled with its variant by its position in the step's launch order
+(``bytes_model.launch_schedule``: the first or a later acoustic substep, the
+first or a later RK stage), then:
+
+* from a ``--kernel-trace`` CSV: launches, mean duration and algorithmic GB/s
+  per variant, and the launch-weighted whole;
+* from ``--pmc FETCH_SIZE`` / ``WRITE_SIZE`` runs: the counters per variant
+  (KiB per dispatch, as rocprofv3 reports them), and bytes after dividing by
+  what the same counter reads per byte of the plain copy kernel ``k_copy`` of
+  the same run.
+
+rocprofv3's rocpd databases (``*_results.db``) and its CSV output are both read.
+
+The figures are printed as a Markdown table.
+"""
+
+from __future__ import annotations
+
+import argparse
+import csv
+import glob
+import os
+import re
+import sys
+
+from . import bytes_model as B
+
+ICOSA = {lv: (10 * 4 ** lv + 2, 30 * 4 ** lv, 20 * 4 ** lv) for lv in range(0, 10)}
+
+
+def _rows(directory: str, suffix: str) -> list[dict]:
+    out = []
+    for path in sorted(glob.glob(os.path.join(directory, "**", f"*{suffix}"), recursive=True)):
+        with open(path, newline="") as f:
+            out.extend(csv.DictReader(f))
+    return out
+
+
+def _db_rows(directory: str, query: str) -> list[tuple]:
+    """Rows of a query over every rocpd database (``*_results.db``, rocprofv3's default output)."""
+    import sqlite3
+
+    out = []
+    for path in sorted(glob.glob(os.path.join(directory, "**", "*_results.db"), recursive=True)):
+        con = sqlite3.connect(f"file:{path}?mode=ro", uri=True)
+        try:
+            out.extend(con.execute(query).fetchall())
+        finally:
+            con.close()
+    return out
+
+
+def _matches(kernel: str):
+    pat = re.compile(r"(^|[\s:*&])" + re.escape(kernel) + r"(\(|<|$|\s)")
+    return lambda name: bool(pat.search(name or ""))
+
+
+def dispatches(directory: str, kernel: str) -> list[dict]:
+    """Kernel-trace rows of ``kernel`` in dispatch order (rocpd database or CSV)."""
+    hit = _matches(kernel)
+    rows = [{"Kernel_Name": n, "Dispatch_Id": d, "Start_Timestamp": a, "End_Timestamp": b}
+            for n, d, a, b in _db_rows(directory, "select name, dispatch_id, start, end from kernels")]
+    rows += _rows(directory, "kernel_trace.csv")
+    rows = [r for r in rows if hit(r.get("Kernel_Name", ""))]
+    rows.sort(key=lambda r: (int(r.get("Dispatch_Id") or 0), int(r.get("Start_Timestamp") or 0)))
+    return rows
+
+
+def counters(directory: str, kernel: str, counter: str) -> list[float]:
+    """Per-dispatch values of one counter for ``kernel`` in dispatch order (rocpd database or CSV)."""
+    hit = _matches(kernel)
+    rows = [{"Kernel_Name": n, "Dispatch_Id": d, "Counter_Name": c, "Counter_Value": v}
+            for n, d, c, v in _db_rows(directory, "select name, dispatch_id, counter_name, counter_value "
+                                                  "from pmc_events")]
+    rows += _rows(directory, "counter_collection.csv")
+    rows = [r for r in rows if hit(r.get("Kernel_Name", "")) and r.get("Counter_Name") == counter]
+    rows.sort(key=lambda r: int(r.get("Dispatch_Id") or r.get("Correlation_Id") or 0))
+    return [float(r["Counter_Value"]) for r in rows]
+
+
+def copy_calibration(directory: str, counter: str) -> float | None:
+    """The counter's reading per byte moved by the plain copy kernel ``k_copy`` of the same run
+    (each thread reads and writes one double; copies of at least 1 Mi doubles), or None."""
+    grid = {d: g for n, d, g in _db_rows(directory, "select name, dispatch_id, grid_x from kernels")
+            if _matches("k_copy")(n)}
+    vals = {d: v for n, d, c, v in _db_rows(directory, "select name, dispatch_id, counter_name, counter_value "
+                                                        "from pmc_events")
+            if _matches("k_copy")(n) and c == counter}
+    ratios = sorted(vals[d] * 1024.0 / (grid[d] * 8.0) for d in vals if d in grid and grid[d] >= 1 << 20)
+    return ratios[len(ratios) // 2] if ratios else None
+
+
+def label(n: int, kernel: str) -> list[str]:
+    sched = B.launch_schedule(kernel)
+    if n % len(sched):
+        raise ValueError(f"{n} dispatches of {kernel} are not whole large steps of {len(sched)} launches")
+    return sched * (n // len(sched))
+
+
+def summarize(directory: str, kernel: str, level: int, levels: int, scalars: int) -> str:
+    nC, nE, nV = ICOSA[level]
+    dims = (nC, nE, nV, levels, scalars)
+    lines = [f"| variant | launches | mean us | algorithmic bytes | GB/s | fraction of 8 TB/s |", "|---|---|---|---|---|---|"]
+    rows = dispatches(directory, kernel)
+    total_b = total_t = 0.0
+    if rows:
+        labels = label(len(rows), kernel)
+        dur = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9 for r in rows]
+        for v in sorted(set(labels)):
+            ts = [d for d, lv in zip(dur, labels) if lv == v]
+            b = B.kernel_bytes(kernel, *dims, variant=v)
+            mean = sum(ts) / len(ts)
+            total_b += b * len(ts)
+            total_t += sum(ts)
+            lines.append(f"| {v} | {len(ts)} | {mean * 1e6:.1f} | {b:.4e} | {b / mean / 1e9:.1f} | "
+                         f"{b / mean / 8e12:.4f} |")
+        lines.append(f"| all (launch-weighted) | {len(rows)} | {total_t / len(rows) * 1e6:.1f} | "
+                     f"{total_b / len(rows):.4e} | {total_b / total_t / 1e9:.1f} | {total_b / total_t / 8e12:.4f} |")
+    out = ["\n".join(lines)]
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        vals = counters(directory, kernel, counter)
+        if not vals:
+            continue
+        labels = label(len(vals), kernel)
+        cal = copy_calibration(directory, counter)
+        cl = [f"| variant | dispatches | {counter} mean KiB | calibrated bytes (k_copy factor "
+              f"{cal if cal is None else round(cal, 4)}) |", "|---|---|---|---|"]
+        for v in sorted(set(labels)):
+            vs = [x for x, lv in zip(vals, labels) if lv == v]
+            mean = sum(vs) / len(vs)
+            cb = f"{mean * 1024 / cal:.4e}" if cal else "-"
+            cl.append(f"| {v} | {len(vs)} | {mean:.0f} | {cb} |")
+        mean = sum(vals) / len(vals)
+        cl.append(f"| all | {len(vals)} | {mean:.0f} | {(f'{mean * 1024 / cal:.4e}' if cal else '-')} |")
+        out.append("\n".join(cl))
+    return "\n\n".join(out)
+
+
+def top_kernels(directory: str, n: int) -> str:
+    """rocprofv3's kernel statistics (the ``top_kernels`` view), the ``n`` largest by total time."""
+    rows = _db_rows(directory, "select name, total_calls, total_duration, average, percentage from top_kernels")
+    rows.sort(key=lambda r: -float(r[2]))
+    lines = ["| kernel | calls | total ms | mean us | % |", "|---|---|---|---|---|"]
+    for name, calls, total, avg, pct in rows[:n]:
+        short = re.sub(r"\(.*", "", name).replace("mpas::", "").replace("adj::", "")
+        # the view reports microseconds
+        lines.append(f"| {short} | {calls} | {float(total) / 1e3:.2f} | {float(avg):.1f} | {float(pct):.1f} |")
+    return "\n".join(lines)
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("directory")
+    ap.add_argument("--kernel", default="k_acoustic_cells")
+    ap.add_argument("--level", type=int, default=7)
+    ap.add_argument("--levels", type=int, default=55)
+    ap.add_argument("--scalars", type=int, default=1)
+    ap.add_argument("--top", type=int, default=0, help="also print the N largest kernels by total time")
+    a = ap.parse_args(argv)
+    if a.kernel:
+        print(summarize(a.directory, a.kernel, a.level, a.levels, a.scalars))
+    if a.top:
+        print()
+        print(top_kernels(a.directory, a.top))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
