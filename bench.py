@@ -215,7 +215,9 @@ ENSEMBLE_DT = {4: 1800.0, 5: 900.0, 6: 450.0, 7: 300.0, 8: 150.0}
 
 def mixed_leg(st, config, dt: float, steps: int, err_steps: int, fp64_ms_per_step: float, reorder: bool,
               device) -> dict:
-    """The mixed-precision step (edge/vertex fields in float32) on the same workload: time per step,
+    """The mixed-precision step (binary32 edge/vertex diagnostics, tendencies and fluxes; binary32
+    arithmetic with compensated sums in the PV flux, diagnostics and tracer transport) on the same
+    workload: time per step,
     speed-up over the FP64 step, and per-field relative L2/Linf error against the FP64 GPU step after
     ``err_steps`` large steps from the same state."""
     import torch
@@ -248,8 +250,12 @@ def mixed_leg(st, config, dt: float, steps: int, err_steps: int, fp64_ms_per_ste
     units = mix.mesh.nc * mix.mesh.L
     return {"ms_per_step": ms, "value": units / (ms / 1e3), "speedup_vs_fp64": fp64_ms_per_step / ms,
             "error_steps": err_steps, "errors_vs_fp64_gpu": errors,
-            "storage": "edge/vertex fields float32; cell/interface fields, column solve, exner recovery and "
-                       "base state float64; float64 arithmetic"}
+            "storage": "binary32: v, pv_edge, rho_edge, vorticity, edge tendencies, cqu, tracer fluxes, edge/"
+                       "vertex scratch; binary64: u and the mass fluxes, every cell/interface field, the column "
+                       "solve, exner recovery and base state",
+            "arithmetic": "binary32 with Neumaier-compensated slot sums: PV flux and KE gradient of the u "
+                          "tendency, KE / tangential velocity / APVM of the diagnostic solve, tracer transport; "
+                          "binary64: mass, theta and w flux divergences, divergence, vorticity, column work"}
 
 
 def ensemble_bench(args, device) -> int:

@@ -1,16 +1,25 @@
 /* Mixed-precision variant of the split-explicit large step.
  *
  * libmpas_dycore_hip_mixed.so is built from the same kernel sources as
- * libmpas_dycore_hip.so with the edge- and vertex-located fields stored in
- * binary32: u, ru, v, pv_edge, rho_edge (h_edge), vorticity and the split
- * averaged ruAvg in the state, and the edge tendencies (tend_u,
- * tend_u_euler), acoustic edge fluxes (ru_p, ruAvg), cqu, the transport
- * fluxes and the edge/vertex scratch in the workspace.  Every cell and
- * interface field stays binary64, including the vertical-implicit
- * coefficients and the column tridiagonal solve, rho, theta_m, rho_theta',
- * the exner/pressure recovery from rho_theta, w, rw and the scalars, and the
- * hydrostatic base state in the mesh.  Arithmetic is binary64 in registers:
- * edge values are widened on load and rounded once on store.
+ * libmpas_dycore_hip.so with two changes:
+ *
+ * - storage: the edge- and vertex-located diagnostics, tendencies and
+ *   fluxes are binary32 (v, pv_edge, rho_edge (h_edge) and vorticity in the
+ *   state; the edge tendencies, cqu, the theta and tracer transport fluxes
+ *   and the edge/vertex scratch in the workspace).  The normal velocity u
+ *   and the mass fluxes ru, ru_p, ruAvg stay binary64: their divergence and
+ *   curl nearly cancel, and binary32 rounding of the terms would carry into
+ *   w, the divergence and the vorticity above 1e-5;
+ * - arithmetic: the PV flux and kinetic-energy gradient of the u tendency,
+ *   the theta advection fluxes and their divergence, the kinetic energy,
+ *   tangential velocity and APVM potential vorticity of the diagnostic
+ *   solve, and the tracer transport fluxes and their divergence compute in
+ *   binary32; their slot sums carry a Neumaier compensation term (TwoSum).
+ *
+ * Binary64 throughout: the vertical-implicit coefficients and the column
+ * tridiagonal solve, rho, theta_m, rho_theta', the exner/pressure recovery
+ * from rho_theta, w, rw, the scalars, the vertical fluxes and the hydrostatic
+ * base state in the mesh.
  *
  * The reference has no such variant (it runs float64 on the CPU only); this
  * is the reduced-precision form of mpas_hip_srk3_step
@@ -26,12 +35,13 @@
 extern "C" {
 #endif
 
-/* mpas_hip_state with the edge/vertex fields in binary32 (same field order). */
+/* mpas_hip_state with v, vorticity, pv_edge and rho_edge in binary32
+ * (same field order). */
 typedef struct mpas_hip_mixed_state {
-    float *u;
+    double *u;
     double *w, *theta_m, *rho_zz, *scalars, *rt_diabatic_tend;
     double *rho_p, *rtheta_p, *exner, *pressure_p;
-    float *ru;
+    double *ru;
     double *rw;
     float *v;
     double *divergence;
@@ -41,7 +51,7 @@ typedef struct mpas_hip_mixed_state {
     double *ur_cell, *vr_cell;
     double *uReconstructX, *uReconstructY, *uReconstructZ;
     double *uReconstructZonal, *uReconstructMeridional;
-    float *ruAvg;
+    double *ruAvg;
     double *wwAvg;
 } mpas_hip_mixed_state;
 

@@ -178,7 +178,7 @@ def check(rc: int, what: str, lib: C.CDLL | None = None) -> None:
 # ---------------------------------------------------------------------------
 MIXED_LIB_PATH = os.path.join(HERE, "libmpas_dycore_hip_mixed.so")
 # state fields stored in binary32 by the mixed-precision step
-MIXED_F32_FIELDS = ("u", "ru", "v", "vorticity", "pv_edge", "rho_edge", "ruAvg")
+MIXED_F32_FIELDS = ("v", "vorticity", "pv_edge", "rho_edge")
 MixedStateStruct = StateStruct  # same field order; every field is a device address
 
 MIXED_SIGNATURES = {

@@ -32,6 +32,58 @@ using hreal = float;
 using hreal = double;
 #endif
 
+// Arithmetic type of the horizontal tendency, diagnostic and transport
+// gathers: binary64 in the plain build, binary32 in the mixed build.  The
+// column work (vertical-implicit coefficients, the tridiagonal solve, exner
+// and pressure recovery, the base state) is binary64 in both.
+#if defined(MPAS_MIXED) && MPAS_MIXED
+using freal = float;
+#else
+using freal = double;
+#endif
+
+// A slot-ordered sum.  The plain build forms the reference's left-to-right
+// sum (the first slot assigned, each later one added).  The mixed build keeps
+// a Neumaier compensation term beside the binary32 running sum (TwoSum of the
+// sum and the new term, the rounding error accumulated separately) and adds
+// it in once at the end, so a flux divergence that nearly cancels keeps the
+// accuracy of its binary32 terms instead of losing it to the running sum.
+struct SlotSum {
+    freal s;
+#if defined(MPAS_MIXED) && MPAS_MIXED
+    freal c;
+#endif
+    __device__ __forceinline__ void first(freal x) {
+        s = x;
+#if defined(MPAS_MIXED) && MPAS_MIXED
+        c = 0.0f;
+#endif
+    }
+    __device__ __forceinline__ void add(freal x) {
+#if defined(MPAS_MIXED) && MPAS_MIXED
+        const freal t = s + x;
+        c = c + ((fabsf(s) >= fabsf(x)) ? ((s - t) + x) : ((x - t) + s));
+        s = t;
+#else
+        s = s + x;
+#endif
+    }
+    // slot j of a row with n live slots: assigned at j == 0, added below n
+    __device__ __forceinline__ void slot(int j, int n, freal x) {
+        if (j < n) {
+            if (j == 0) first(x);
+            else add(x);
+        }
+    }
+    __device__ __forceinline__ freal value() const {
+#if defined(MPAS_MIXED) && MPAS_MIXED
+        return s + c;
+#else
+        return s;
+#endif
+    }
+};
+
 // Host-evaluated kernel scalars (layouts fixed by the C ABI header).
 using DynParams = mpas_hip_dyn_params;
 using AcousticParams = mpas_hip_acoustic_params;
@@ -161,17 +213,18 @@ void launch_vimp(const mpas_hip_mesh &m, double dts, double epssm, const double 
                  const double *t, const double *rt, const double *qtot, double *cofrz, double *cofwr,
                  double *cofwz, double *coftz, double *cofwt, double *a_tri, double *alpha_tri,
                  double *gamma_tri, hipStream_t s);
-void launch_diagnostics(const mpas_hip_mesh &m, double dt, double apvm, int rk_step, const hreal *u,
+void launch_diagnostics(const mpas_hip_mesh &m, double dt, double apvm, int rk_step, const double *u,
                         const double *h, const hreal *v_in, hreal *h_edge, hreal *v, double *div,
                         hreal *vort, double *ke, hreal *pv_edge, double *scratch, hipStream_t s);
 size_t diagnostics_scratch_doubles(const mpas_hip_mesh &m);
 
 struct DynInputs {
-    const hreal *u, *v;
+    const double *u;
+    const hreal *v;
     const double *w;
-    const hreal *ru;
+    const double *ru;
     const double *rw;
-    const hreal *ru_save;
+    const double *ru_save;
     const double *rw_save, *theta_m, *theta_m_save, *rho_zz;
     const hreal *rho_edge;
     const double *divergence;
@@ -204,7 +257,7 @@ struct AcousticInputs {
     const double *tend_rho, *tend_rt, *tend_rw, *w, *rw, *rw_save;
 };
 struct AcousticState {
-    hreal *ru_p, *ruAvg;
+    double *ru_p, *ruAvg;
     double *rho_pp, *rtheta_pp, *rtheta_pp_old, *rw_p, *wwAvg;
 };
 // Divergence damping carried into a later edge pass (on = 0: none).
@@ -217,43 +270,43 @@ void launch_acoustic(const mpas_hip_mesh &m, const AcousticParams &P, const Acou
                      const AcousticState &io, hipStream_t s, const DampFold &fold);
 void launch_divdamp(const mpas_hip_mesh &m, double dts, double smdiv, double len_disp,
                     const double *theta_m, const double *rtheta_pp, const double *rtheta_pp_old,
-                    hreal *ru_p, hipStream_t s);
+                    double *ru_p, hipStream_t s);
 
 struct RecoverInputs {
     const double *rho_p_save, *rho_pp, *rtheta_p_save, *rtheta_pp, *rt_diabatic_tend, *rw_save, *rw_p;
-    const hreal *ru_save, *ru_p;
+    const double *ru_save, *ru_p;
 };
 struct RecoverOutputs {
     double *wwAvg;
-    hreal *ruAvg;
+    double *ruAvg;
     double *rho_p, *rho_zz, *rtheta_p, *theta_m, *exner, *pressure_p, *rw, *w;
-    hreal *ru, *u;
+    double *ru, *u;
 };
 void launch_recover(const mpas_hip_mesh &m, int rk_step, double dt, int ns, const RecoverInputs &in,
                     const RecoverOutputs &out, hipStream_t s, const DampFold &fold);
 
 void launch_advance_scalars(const mpas_hip_mesh &m, int S, double dt, double coef3, double w_old,
                             double w_new, const double *scalar_old, const double *scalar_new_in,
-                            const double *rho_zz_old, const double *rho_zz_new, const hreal *uhAvg,
+                            const double *rho_zz_old, const double *rho_zz_new, const double *uhAvg,
                             const double *wwAvg, double *scalar_new_out, hreal *horiz_flux,
                             hipStream_t s);
 void launch_advance_scalars_mono(const mpas_hip_mesh &m, int S, double dt, double coef3,
                                  const double *scalars_old, const double *scalars_new_in,
-                                 const double *rho_zz_old, const hreal *uhAvg, const double *wwAvg,
+                                 const double *rho_zz_old, const double *uhAvg, const double *wwAvg,
                                  double *scalars_new_out, double *rho_zz_int, double *scratch,
                                  hipStream_t s);
 size_t mono_scratch_doubles(const mpas_hip_mesh &m);
 
-void launch_reconstruct(const mpas_hip_mesh &m, const hreal *u, double *ux, double *uy, double *uz,
+void launch_reconstruct(const mpas_hip_mesh &m, const double *u, double *ux, double *uy, double *uz,
                         double *uzonal, double *umerid, hipStream_t s);
 
 struct InitCoupledOutputs {
     double *theta_m, *rho_zz;
-    hreal *ru;
+    double *ru;
     double *rw, *rho_p, *rtheta_base, *rtheta_p, *exner, *exner_base, *pressure_p, *pressure_base;
 };
 void launch_init_coupled(const mpas_hip_mesh &m, const double *theta, const double *rho, const double *qv,
-                         const hreal *u, const double *w, const double *theta_base, const InitCoupledOutputs &o,
+                         const double *u, const double *w, const double *theta_base, const InitCoupledOutputs &o,
                          hipStream_t s);
 void launch_history_fields(int nC, int L, int S, const double *theta_m, const double *scalars, const double *rho_zz,
                            const double *zz, const double *pressure_p, const double *pressure_base, double *theta,

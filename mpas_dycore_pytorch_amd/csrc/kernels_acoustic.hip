@@ -61,7 +61,7 @@ __device__ __forceinline__ double divdamp_increment(double coef, const double *_
 __global__ __launch_bounds__(kMaxWaveBlock) void k_acoustic_edges(long long nb, mpas_hip_mesh m, AcousticParams P, const double *__restrict__ rtheta_pp,
                                  const double *__restrict__ rho_pp, const double *__restrict__ exner,
                                  const hreal *__restrict__ cqu, const hreal *__restrict__ tend_ru,
-                                 hreal *__restrict__ ru_p, hreal *__restrict__ ruAvg, DampFold fold) {
+                                 double *__restrict__ ru_p, double *__restrict__ ruAvg, DampFold fold) {
     auto body_ = [&](const int e) {
         const int L = m.nLevels, nE = m.nEdges;
         const int k = wave_lane();
@@ -233,7 +233,7 @@ void launch_acoustic(const mpas_hip_mesh &m, const AcousticParams &P, const Acou
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kMaxWaveBlock) void k_divdamp(long long nb, mpas_hip_mesh m, double coef, const double *__restrict__ theta_m,
                           const double *__restrict__ rtheta_pp, const double *__restrict__ rtheta_pp_old,
-                          hreal *__restrict__ ru_p) {
+                          double *__restrict__ ru_p) {
     auto body_ = [&](const int e) {
         const int L = m.nLevels, nE = m.nEdges;
         const int k = wave_lane();
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_divdamp(long long nb, mpas_hi
 }
 
 void launch_divdamp(const mpas_hip_mesh &m, double dts, double smdiv, double len_disp, const double *theta_m,
-                    const double *rtheta_pp, const double *rtheta_pp_old, hreal *ru_p, hipStream_t s) {
+                    const double *rtheta_pp, const double *rtheta_pp_old, double *ru_p, hipStream_t s) {
     const double coef = 2.0 * smdiv * len_disp / dts;
     MPAS_WAVE_LAUNCH(k_divdamp, m.nEdges, s, m, coef,
                        theta_m, rtheta_pp, rtheta_pp_old, ru_p);
@@ -293,9 +293,9 @@ __global__ void k_recover_cells(mpas_hip_mesh m, int rk_step, double dt, double 
     }
 }
 
-__global__ __launch_bounds__(kMaxWaveBlock) void k_recover_edges(long long nb, mpas_hip_mesh m, double invNs, const hreal *__restrict__ ru_save,
-                                const hreal *__restrict__ ru_p, const double *__restrict__ rho_zz,
-                                hreal *__restrict__ ruAvg, hreal *__restrict__ ru, hreal *__restrict__ u,
+__global__ __launch_bounds__(kMaxWaveBlock) void k_recover_edges(long long nb, mpas_hip_mesh m, double invNs, const double *__restrict__ ru_save,
+                                const double *__restrict__ ru_p, const double *__restrict__ rho_zz,
+                                double *__restrict__ ruAvg, double *__restrict__ ru, double *__restrict__ u,
                                 DampFold fold) {
     auto body_ = [&](const int e) {
         const int L = m.nLevels, nE = m.nEdges;
@@ -316,7 +316,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_recover_edges(long long nb, m
     for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
-__global__ __launch_bounds__(kMaxWaveBlock) void k_recover_w(long long nb, mpas_hip_mesh m, const hreal *__restrict__ ru, const double *__restrict__ rw,
+__global__ __launch_bounds__(kMaxWaveBlock) void k_recover_w(long long nb, mpas_hip_mesh m, const double *__restrict__ ru, const double *__restrict__ rw,
                             const double *__restrict__ rho_zz, double *__restrict__ w) {
     auto body_ = [&](const int c) {
         const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
@@ -333,7 +333,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_recover_w(long long nb, mpas_
             double sacc = 0.0;
             for (int j = 0; j < nt; ++j) {
                 int e = m.edgesOnCell[c * ME + j];
-                const hreal *re = ru + e * L;
+                const double *re = ru + e * L;
                 double fs = m.cf1 * re[0] + m.cf2 * re[1] + m.cf3 * re[2];
                 size_t zo = ((size_t)c * ME + j) * Lw;
                 double coef = m.zb_cell[zo] + sign_pm(fs) * m.zb3_cell[zo];
@@ -346,7 +346,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_recover_w(long long nb, mpas_
             double acc = rw[i] / zi;
             for (int j = 0; j < nt; ++j) {
                 int e = m.edgesOnCell[c * ME + j];
-                const hreal *re = ru + e * L;
+                const double *re = ru + e * L;
                 double fi = m.fzm[k] * re[k] + m.fzp[k] * re[k - 1];
                 size_t zo = ((size_t)c * ME + j) * Lw + k;
                 double coef = m.zb_cell[zo] + sign_pm(fi) * m.zb3_cell[zo];

@@ -183,7 +183,7 @@ void launch_vimp(const mpas_hip_mesh &m, double dts, double epssm, const double 
 // ---------------------------------------------------------------------------
 // mpas_reconstruct_2d (vector_reconstruction.py:311-441), on a sphere
 // ---------------------------------------------------------------------------
-__global__ void k_reconstruct(mpas_hip_mesh m, const hreal *__restrict__ u, double *__restrict__ ux,
+__global__ void k_reconstruct(mpas_hip_mesh m, const double *__restrict__ u, double *__restrict__ ux,
                               double *__restrict__ uy, double *__restrict__ uz, double *__restrict__ uzonal,
                               double *__restrict__ umerid) {
     const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges;
@@ -214,7 +214,7 @@ __global__ void k_reconstruct(mpas_hip_mesh m, const hreal *__restrict__ u, doub
     umerid[i] = -(x * clon + y * slon) * slat + z * clat;
 }
 
-void launch_reconstruct(const mpas_hip_mesh &m, const hreal *u, double *ux, double *uy, double *uz,
+void launch_reconstruct(const mpas_hip_mesh &m, const double *u, double *ux, double *uy, double *uz,
                         double *uzonal, double *umerid, hipStream_t s) {
     MPAS_LAUNCH(k_reconstruct, dim3(blocks_for((long long)(m.nCells + 1) * m.nLevels)), dim3(kBlock), 0,
                        s, m, u, ux, uy, uz, uzonal, umerid);

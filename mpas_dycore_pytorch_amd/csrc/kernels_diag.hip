@@ -14,7 +14,7 @@ size_t diagnostics_scratch_doubles(const mpas_hip_mesh &m) {
     return 2 * slice_doubles<hreal>((size_t)m.nVertices * L) + slice_doubles<double>((size_t)m.nCells * L);
 }
 
-__global__ __launch_bounds__(kMaxWaveBlock) void k_diag_vertices(long long nb, mpas_hip_mesh m, const hreal *__restrict__ u,
+__global__ __launch_bounds__(kMaxWaveBlock) void k_diag_vertices(long long nb, mpas_hip_mesh m, const double *__restrict__ u,
                                 hreal *__restrict__ vort,
                                 hreal *__restrict__ ke_vertex, hreal *__restrict__ pv_vertex) {
     auto body_ = [&](const int vx) {
@@ -26,20 +26,23 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_diag_vertices(long long nb, m
             vort[i] = 0.0;
             return;
         }
-        double s = 0.0, sk = 0.0;
+        // the circulation (terms that nearly cancel) in binary64; the kinetic
+        // energy (positive terms) in binary32 in the mixed build
+        double s = 0.0;
+        SlotSum sk;
     #pragma unroll
         for (int j = 0; j < 3; ++j) {
             int e = m.edgesOnVertex[3 * vx + j];
             double w = m.edgesOnVertex_sign[3 * vx + j] * m.dcEdge[e];
             double c = w * u[e * L + k];
-            const double ue = u[e * L + k];
-            double kk = (m.dcEdge[e] * m.dvEdge[e]) * (ue * ue);  // ke_edge of the edge
+            const freal ue = (freal)u[e * L + k];
+            const freal kk = (freal)(m.dcEdge[e] * m.dvEdge[e]) * (ue * ue);  // ke_edge of the edge
             s = (j == 0) ? c : s + c;
-            sk = (j == 0) ? kk : sk + kk;
+            sk.slot(j, 3, kk);
         }
         double vv = s * m.invAreaTriangle[vx];
         vort[i] = vv;
-        ke_vertex[i] = 0.25 * m.invAreaTriangle[vx] * sk;
+        ke_vertex[i] = (freal)(0.25 * m.invAreaTriangle[vx]) * sk.value();
         pv_vertex[i] = m.fVertex[vx] + vv;
     };
     const int base_ = wave_entity_base(nb);
@@ -47,7 +50,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_diag_vertices(long long nb, m
     for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
-__global__ __launch_bounds__(kMaxWaveBlock) void k_diag_cells(long long nb, mpas_hip_mesh m, const hreal *__restrict__ u,
+__global__ __launch_bounds__(kMaxWaveBlock) void k_diag_cells(long long nb, mpas_hip_mesh m, const double *__restrict__ u,
                              const hreal *__restrict__ ke_vertex, const hreal *__restrict__ pv_vertex,
                              double *__restrict__ div, double *__restrict__ ke, double *__restrict__ pv_cell) {
     auto body_ = [&](const int c) {
@@ -74,29 +77,33 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_diag_cells(long long nb, mpas
             uo[j] = 0.0;
             if (j < n) uo[j] = u[eo[j] * L + k];  // wave-uniform guard
         }
-        double d = 0.0, kc = 0.0;
+        // the divergence (terms that nearly cancel) in binary64; the kinetic
+        // energy in binary32 in the mixed build
+        double d = 0.0;
+        SlotSum kc;
     #pragma unroll
         for (int j = 0; j < kMaxEdges; ++j) {
             const int jj = clamp_slot(j, n);
             const int e = m.edgesOnCell[c * ME + jj];
             const double w = m.edgesOnCell_sign[c * ME + jj] * m.dvEdge[e];
-            const double ue = uo[j], kee = (m.dcEdge[e] * m.dvEdge[e]) * (ue * ue);
-            const double cd = w * ue;
-            const double ck = 0.25 * kee;
+            const freal ue = (freal)uo[j], kee = (freal)(m.dcEdge[e] * m.dvEdge[e]) * (ue * ue);
+            const double cd = w * uo[j];
+            const freal ck = (freal)0.25 * kee;
             d = (j < n) ? ((j == 0) ? cd : d + cd) : d;
-            kc = (j < n) ? ((j == 0) ? ck : kc + ck) : kc;
+            kc.slot(j, n, ck);
         }
+        const freal fia = (freal)ia;
         div[i] = d * ia;
         const double ke_fact = 1.0 - 0.375;
-        double tot = ke_fact * (kc * ia);
-        double pc = 0.0;
-        double kvv[kMaxEdges], pvv[kMaxEdges];
+        SlotSum tot, pc;
+        tot.first((freal)ke_fact * (kc.value() * fia));
+        freal kvv[kMaxEdges], pvv[kMaxEdges];
     #pragma unroll
         for (int j = 0; j < kMaxEdges; ++j) {
             kvv[j] = pvv[j] = 0.0;
             if (j < n) {
-                kvv[j] = ke_vertex[vo[j] * L + k];
-                pvv[j] = pv_vertex[vo[j] * L + k];
+                kvv[j] = (freal)ke_vertex[vo[j] * L + k];
+                pvv[j] = (freal)pv_vertex[vo[j] * L + k];
             }
         }
     #pragma unroll
@@ -104,17 +111,16 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_diag_cells(long long nb, mpas
             const int jj = clamp_slot(j, n);
             const int vx = m.verticesOnCell[c * ME + jj];
             const double ka = m.kiteAreasOnVertex[3 * vx + m.kiteForCell[c * ME + jj]];
-            const double kv = kvv[j], pv = pvv[j];
-            double contrib = (1.0 - ke_fact) * ka;
+            const freal kv = kvv[j], pv = pvv[j];
+            freal contrib = (freal)((1.0 - ke_fact) * ka);
             contrib = contrib * kv;
-            contrib = contrib * ia;
-            const double ntot = tot + contrib;
-            tot = (j < n) ? ntot : tot;
-            const double cp = ka * pv * ia;
-            pc = (j < n) ? ((j == 0) ? cp : pc + cp) : pc;
+            contrib = contrib * fia;
+            if (j < n) tot.add(contrib);
+            const freal cp = (freal)ka * pv * fia;
+            pc.slot(j, n, cp);
         }
-        ke[i] = tot;
-        pv_cell[i] = pc;
+        ke[i] = tot.value();
+        pv_cell[i] = pc.value();
     };
     const int base_ = wave_entity_base(nb);
 #pragma unroll 1
@@ -122,7 +128,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_diag_cells(long long nb, mpas
 }
 
 __global__ __launch_bounds__(kMaxWaveBlock) void k_diag_edges2(long long nb, mpas_hip_mesh m, double apvm_dt, int apvm_on, int reconstruct_v,
-                              const hreal *__restrict__ u, const hreal *__restrict__ v_in,
+                              const double *__restrict__ u, const hreal *__restrict__ v_in,
                               const hreal *__restrict__ pv_vertex, const double *__restrict__ pv_cell,
                               const double *__restrict__ h, hreal *__restrict__ h_edge,
                               hreal *__restrict__ v, hreal *__restrict__ pv_edge) {
@@ -140,9 +146,10 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_diag_edges2(long long nb, mpa
         }
         const int k = lane < L ? lane : L - 1;  // lanes past the column: loads only
         const int i = e * L + k;
-        double vv;
+        freal vv;
         if (reconstruct_v) {
-            vv = 0.0;
+            SlotSum vs;
+            vs.first(0.0);
             const int ne = m.nEdgesOnEdge[e];
             const int sl = lane < ne ? lane : ne - 1;
             const int eev = m.edgesOnEdge[e * ME2 + sl];
@@ -150,36 +157,37 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_diag_edges2(long long nb, mpa
     #pragma unroll
             for (int j0 = 0; j0 < kMaxEdges2; j0 += 8) {
                 if (j0 >= ne) break;
-                double uj[8];
+                freal uj[8];
     #pragma unroll
                 for (int t = 0; t < 8; ++t) {
                     uj[t] = 0.0;
-                    if (j0 + t < ne) uj[t] = u[bcast(eev, j0 + t) * L + k];  // wave-uniform guard
+                    if (j0 + t < ne) uj[t] = (freal)u[bcast(eev, j0 + t) * L + k];  // wave-uniform guard
                 }
     #pragma unroll
                 for (int t = 0; t < 8; ++t) {
                     const int j = j0 + t;
-                    const double c = bcast(wv, j) * uj[t];
-                    vv = (j < ne) ? ((j == 0) ? c : vv + c) : vv;
+                    const freal c = (freal)bcast(wv, j) * uj[t];
+                    vs.slot(j, ne, c);
                 }
             }
+            vv = vs.value();
         } else {
-            vv = v_in[i];
+            vv = (freal)v_in[i];
         }
         if (lane >= L) return;
         v[i] = vv;
         int v1 = m.verticesOnEdge[2 * e], v2 = m.verticesOnEdge[2 * e + 1];
-        double p1 = pv_vertex[v1 * L + k], p2 = pv_vertex[v2 * L + k];
-        double pe = 0.5 * (p1 + p2);
+        const freal p1 = (freal)pv_vertex[v1 * L + k], p2 = (freal)pv_vertex[v2 * L + k];
+        freal pe = (freal)0.5 * (p1 + p2);
         {
             const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
             h_edge[i] = 0.5 * (h[c1 * L + k] + h[c2 * L + k]);
         }
         if (apvm_on) {
             int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-            double grad_pvt = (p2 - p1) * m.invDvEdge[e];
-            double grad_pvn = (pv_cell[c2 * L + k] - pv_cell[c1 * L + k]) * m.invDcEdge[e];
-            double delta = apvm_dt * (vv * grad_pvt + u[i] * grad_pvn);
+            const freal grad_pvt = (p2 - p1) * (freal)m.invDvEdge[e];
+            const freal grad_pvn = ((freal)pv_cell[c2 * L + k] - (freal)pv_cell[c1 * L + k]) * (freal)m.invDcEdge[e];
+            const freal delta = (freal)apvm_dt * (vv * grad_pvt + (freal)u[i] * grad_pvn);
             pe = pe - delta;
         }
         pv_edge[i] = pe;
@@ -189,7 +197,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_diag_edges2(long long nb, mpa
     for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
-void launch_diagnostics(const mpas_hip_mesh &m, double dt, double apvm, int rk_step, const hreal *u,
+void launch_diagnostics(const mpas_hip_mesh &m, double dt, double apvm, int rk_step, const double *u,
                         const double *h, const hreal *v_in, hreal *h_edge, hreal *v, double *div,
                         hreal *vort, double *ke, hreal *pv_edge, double *scratch, hipStream_t s) {
     const int L = m.nLevels, nC = m.nCells, nE = m.nEdges, nV = m.nVertices;

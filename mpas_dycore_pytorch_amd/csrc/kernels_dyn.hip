@@ -21,7 +21,7 @@ size_t dyn_tend_scratch_doubles(const mpas_hip_mesh &m) {
     // dpdz, delsq_u, delsq_div, delsq_vort, delsq_w, delsq_theta, edge fluxes (w, theta, theta pert.)
     return slice_doubles<double>(C * L) + slice_doubles<hreal>(E * L) + slice_doubles<double>(C * L) +
            slice_doubles<hreal>(V * L) + slice_doubles<double>(C * (L + 1)) + slice_doubles<double>(C * L) +
-           3 * slice_doubles<hreal>(E * L);
+           3 * slice_doubles<double>(E * L);
 }
 
 struct DynScratch {
@@ -30,7 +30,7 @@ struct DynScratch {
     double *delsq_div;
     hreal *delsq_vort;
     double *delsq_w, *delsq_theta;
-    hreal *wflux, *tflux, *tpert;
+    double *wflux, *tflux, *tpert;
 };
 
 static DynScratch carve(const mpas_hip_mesh &m, double *base) {
@@ -43,14 +43,14 @@ static DynScratch carve(const mpas_hip_mesh &m, double *base) {
     s.delsq_vort = k.take<hreal>(V * L);
     s.delsq_w = k.take<double>(C * (L + 1));
     s.delsq_theta = k.take<double>(C * L);
-    s.wflux = k.take<hreal>(E * L);
-    s.tflux = k.take<hreal>(E * L);
-    s.tpert = k.take<hreal>(E * L);
+    s.wflux = k.take<double>(E * L);
+    s.tflux = k.take<double>(E * L);
+    s.tpert = k.take<double>(E * L);
     return s;
 }
 
 // P1: Smagorinsky eddy viscosity (first RK stage only)
-__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_kdiff(long long nb, mpas_hip_mesh m, DynParams P, const hreal *__restrict__ u,
+__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_kdiff(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ u,
                             const hreal *__restrict__ v, double *__restrict__ kdiff) {
     auto body_ = [&](const int c) {
         const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges;
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_kdiff(long long nb, mpas_
 
 // P2: mass-flux divergence, density tendency, buoyancy dpdz; first stage also
 // the del2 w / theta mixing sums and their del2 fields for the del4 pass.
-__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_a(long long nb, mpas_hip_mesh m, DynParams P, const hreal *__restrict__ ru,
+__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_a(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ ru,
                               const double *__restrict__ rw, const double *__restrict__ rr_save,
                               const double *__restrict__ qtot, const double *__restrict__ w,
                               const double *__restrict__ theta_m, const hreal *__restrict__ rho_edge,
@@ -137,6 +137,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_a(long long nb, mpa
             if (j < n) rv[j] = ru[eo[j] * L + kl];
         }
         if (k < L) {
+            // the mass-flux divergence stays binary64 in both builds: its terms nearly cancel
             double s = 0.0;
     #pragma unroll
             for (int j = 0; j < kMaxEdges; ++j) {
@@ -227,7 +228,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_a(long long nb, mpa
 
 // P4: normal-velocity tendency (vertical flux, PV flux, KE gradient,
 // divergence coupling); first stage also PGF and del2 mixing into tend_u_euler.
-__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_a(long long nb, mpas_hip_mesh m, DynParams P, const hreal *__restrict__ u,
+__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_a(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ u,
                               const double *__restrict__ rw, const hreal *__restrict__ pv_edge,
                               const double *__restrict__ ke, const double *__restrict__ h_div,
                               const hreal *__restrict__ rho_edge, const double *__restrict__ pp,
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_a(long long nb, mpa
         const int eev = m.edgesOnEdge[e * ME2 + sl];
         const double wv = m.weightsOnEdge[e * ME2 + sl];
         const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-        const hreal *ue = u + e * L;
+        const double *ue = u + e * L;
         const double *fzm = m.fzm, *fzp = m.fzp;
         // vertical flux of u at this lane's interface k (1..L-1); the stencil
         // values u(k-2), u(k-1), u(k+1) are the neighbouring lanes' own u,
@@ -269,12 +270,15 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_a(long long nb, mpa
         else if (k == L - 1) delta = -wz;
         else delta = wz_up - wz;
         double vert = -m.rdzw[k] * delta;
-        double q = 0.0;
-        const double pvs = pv_edge[i];
+        // horizontal terms (binary32 in the mixed build): the PV flux over
+        // edgesOnEdge in slot order, the KE gradient and the divergence coupling
+        SlotSum q;
+        q.first(0.0);
+        const freal pvs = (freal)pv_edge[i];
     #pragma unroll
         for (int j0 = 0; j0 < kMaxEdges2; j0 += 8) {
             if (j0 >= ne) break;
-            double pj[8], uj[8];
+            freal pj[8], uj[8];
     #pragma unroll
             for (int t = 0; t < 8; ++t) {
                 // slots past the live count skip the load (wave-uniform branch)
@@ -282,21 +286,22 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_a(long long nb, mpa
                 uj[t] = 0.0;
                 if (j0 + t < ne) {
                     const int ee = bcast(eev, j0 + t);
-                    pj[t] = pv_edge[ee * L + k];
-                    uj[t] = u[ee * L + k];
+                    pj[t] = (freal)pv_edge[ee * L + k];
+                    uj[t] = (freal)u[ee * L + k];
                 }
             }
     #pragma unroll
             for (int t = 0; t < 8; ++t) {
                 const int j = j0 + t;
-                const double workpv = 0.5 * (pvs + pj[t]);
-                const double cc = bcast(wv, j) * uj[t] * workpv;
-                q = (j < ne) ? ((j == 0) ? cc : q + cc) : q;
+                const freal workpv = (freal)0.5 * (pvs + pj[t]);
+                const freal cc = (freal)bcast(wv, j) * uj[t] * workpv;
+                q.slot(j, ne, cc);
             }
         }
-        double ke_delta = ke[c2 * L + k] - ke[c1 * L + k];
-        double hsum = h_div[c1 * L + k] + h_div[c2 * L + k];
-        double base = rho_edge[i] * (q - ke_delta * m.invDcEdge[e]) - ue[k] * 0.5 * hsum;
+        const freal ke_delta = (freal)ke[c2 * L + k] - (freal)ke[c1 * L + k];
+        const freal hsum = (freal)h_div[c1 * L + k] + (freal)h_div[c2 * L + k];
+        const double base = (double)((freal)rho_edge[i] * (q.value() - ke_delta * (freal)m.invDcEdge[e]) -
+                                     (freal)ue[k] * (freal)0.5 * hsum);
         if (lane >= L) return;
         const double tu = vert + base;
         double eu = 0.0;
@@ -405,10 +410,10 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_b(long long nb, mpa
 // for rk_step > 1 also the theta perturbation flux
 // dv (ru_save - ru) 0.5 (theta_m_save1 + theta_m_save2) (:5901-6014).  The
 // cell pass applies the edge sign, an exact sign flip of these products.
-__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edge_adv(long long nb, mpas_hip_mesh m, DynParams P, const hreal *__restrict__ ru,
-                               const hreal *__restrict__ ru_save, const double *__restrict__ w,
+__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edge_adv(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ ru,
+                               const double *__restrict__ ru_save, const double *__restrict__ w,
                                const double *__restrict__ theta_m, const double *__restrict__ theta_m_save,
-                               hreal *__restrict__ wflux, hreal *__restrict__ tflux, hreal *__restrict__ tpert) {
+                               double *__restrict__ wflux, double *__restrict__ tflux, double *__restrict__ tpert) {
     auto body_ = [&](const int e) {
         const int L = m.nLevels, nE = m.nEdges, MA = m.maxAdv, Lw = L + 1;
         const int lane = wave_lane();
@@ -419,12 +424,14 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edge_adv(long long nb, mp
         const int sl = lane < na ? lane : na - 1;
         const int advv = m.advCellsForEdge[e * MA + sl];
         const double c0v = m.adv_coefs[e * MA + sl], c3v = m.adv_coefs_3rd[e * MA + sl];
-        const hreal *rue_col = ru + e * L;
+        const double *rue_col = ru + e * L;
         const double rut = rue_col[k];
         const double srt = sign_pm(rut);
         const bool wl = (k >= 1);
         const double ruw = wl ? m.fzm[k] * rue_col[k] + m.fzp[k] * rue_col[k - 1] : 0.0;
         const double srw = sign_pm(ruw);
+        // binary64 in both builds: the theta and w fluxes feed flux divergences whose
+        // terms nearly cancel (theta ~ 300 K against its gradients, w against its mean)
         double ft = 0.0, fw = 0.0;
     #pragma unroll
         for (int a0 = 0; a0 < kMaxAdv; a0 += 8) {
@@ -467,9 +474,9 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edge_adv(long long nb, mp
 
 // P7: w and theta tendencies (advection, mixing finish, PGF/buoyancy,
 // vertical transport, diabatic forcing), one thread per (cell, w-level).
-__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_b(long long nb, mpas_hip_mesh m, DynParams P, const hreal *__restrict__ wflux,
-                              const double *__restrict__ rw, const hreal *__restrict__ tflux,
-                              const double *__restrict__ rw_save, const hreal *__restrict__ tpert,
+__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_b(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ wflux,
+                              const double *__restrict__ rw, const double *__restrict__ tflux,
+                              const double *__restrict__ rw_save, const double *__restrict__ tpert,
                               const double *__restrict__ w,
                               const double *__restrict__ theta_m, const double *__restrict__ theta_m_save,
                               const double *__restrict__ rho_zz, const double *__restrict__ pp,

@@ -48,8 +48,8 @@ __global__ void k_icd_cells(mpas_hip_mesh m, const double *theta, const double *
     o.pressure_base[i] = zz * MPAS_RGAS * exb * rtb;
 }
 
-__global__ void k_icd_edges(mpas_hip_mesh m, const hreal *__restrict__ u, const double *__restrict__ rho_zz,
-                            hreal *__restrict__ ru) {
+__global__ void k_icd_edges(mpas_hip_mesh m, const double *__restrict__ u, const double *__restrict__ rho_zz,
+                            double *__restrict__ ru) {
     const int L = m.nLevels, nE = m.nEdges;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (nE + 1) * L) return;
@@ -63,7 +63,7 @@ __global__ void k_icd_edges(mpas_hip_mesh m, const hreal *__restrict__ u, const 
 }
 
 __global__ void k_icd_rw(mpas_hip_mesh m, const double *__restrict__ w, const double *__restrict__ rho_zz,
-                         const hreal *__restrict__ ru, double *__restrict__ rw) {
+                         const double *__restrict__ ru, double *__restrict__ rw) {
     const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (nC + 1) * Lw) return;
@@ -91,7 +91,7 @@ __global__ void k_icd_rw(mpas_hip_mesh m, const double *__restrict__ w, const do
 }
 
 void launch_init_coupled(const mpas_hip_mesh &m, const double *theta, const double *rho, const double *qv,
-                         const hreal *u, const double *w, const double *theta_base, const InitCoupledOutputs &o,
+                         const double *u, const double *w, const double *theta_base, const InitCoupledOutputs &o,
                          hipStream_t s) {
     const int L = m.nLevels, nC = m.nCells, nE = m.nEdges;
     MPAS_LAUNCH(k_icd_cells, dim3(blocks_for((long long)(nC + 1) * L)), dim3(kBlock), 0, s, m, theta, rho, qv,

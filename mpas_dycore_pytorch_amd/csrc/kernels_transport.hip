@@ -15,27 +15,29 @@ namespace mpas {
 // scalar s at an edge, one wavefront per edge (lane = level): advv / c0v / c3v
 // hold the edge's stencil row (lane j = slot j); all slot gathers are issued
 // before the first is used and the sum runs in slot order.
-__device__ __forceinline__ double patternc_wave(int na, int advv, double c0v, double c3v, double sgn,
-                                                const double *__restrict__ q, int L, int S, int k, int s) {
-    double f = 0.0;
+// The sum is binary32 with a compensation term in the mixed build.
+__device__ __forceinline__ freal patternc_wave(int na, int advv, double c0v, double c3v, double sgn,
+                                               const double *__restrict__ q, int L, int S, int k, int s) {
+    SlotSum f;
+    f.first(0.0);
 #pragma unroll
     for (int a0 = 0; a0 < kMaxAdv; a0 += 8) {
         if (a0 >= na) break;
-        double qv[8];
+        freal qv[8];
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
             qv[t] = 0.0;
-            if (a0 + t < na) qv[t] = q[((size_t)bcast(advv, a0 + t) * L + k) * S + s];
+            if (a0 + t < na) qv[t] = (freal)q[((size_t)bcast(advv, a0 + t) * L + k) * S + s];
         }
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
             const int a = a0 + t;
-            const double wgt = bcast(c0v, a) + sgn * bcast(c3v, a);
-            const double cc = wgt * qv[t];
-            f = (a < na) ? ((a == 0) ? cc : f + cc) : f;
+            const freal wgt = (freal)bcast(c0v, a) + (freal)sgn * (freal)bcast(c3v, a);
+            const freal cc = wgt * qv[t];
+            f.slot(a, na, cc);
         }
     }
-    return f;
+    return f.value();
 }
 
 // One wavefront per edge (hflux) or cell (update), lane = level.  The
@@ -44,7 +46,7 @@ __device__ __forceinline__ double patternc_wave(int na, int advv, double c0v, do
 // neighbouring levels of the wavefront's own column from lane shuffles.  The
 // sums run in slot order.
 __global__ __launch_bounds__(kMaxWaveBlock) void k_scalar_hflux(long long nb, mpas_hip_mesh m, int S,
-                                                                const hreal *__restrict__ uhAvg,
+                                                                const double *__restrict__ uhAvg,
                                                                 const double *__restrict__ scalar_new,
                                                                 hreal *__restrict__ hflux) {
     const int L = m.nLevels, nE = m.nEdges, MA = m.maxAdv;
@@ -59,7 +61,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_scalar_hflux(long long nb, mp
     const double c0v = m.adv_coefs[e * MA + sl], c3v = m.adv_coefs_3rd[e * MA + sl];
     const double sgn = sign_pm(uhAvg[i]);
     for (int s = 0; s < S; ++s) {
-        const double f = patternc_wave(na, advv, c0v, c3v, sgn, scalar_new, L, S, k, s);
+        const freal f = patternc_wave(na, advv, c0v, c3v, sgn, scalar_new, L, S, k, s);
         if (lane < L) hflux[(size_t)i * S + s] = f;
     }
 }
@@ -67,7 +69,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_scalar_hflux(long long nb, mp
 __global__ __launch_bounds__(kMaxWaveBlock) void k_scalar_update(
     long long nb, mpas_hip_mesh m, int S, double dt, double coef3, double w_old, double w_new,
     const double *__restrict__ scalar_old, const double *__restrict__ sn, const double *__restrict__ rho_old,
-    const double *__restrict__ rho_new, const hreal *__restrict__ uhAvg, const double *__restrict__ wwAvg,
+    const double *__restrict__ rho_new, const double *__restrict__ uhAvg, const double *__restrict__ wwAvg,
     const hreal *__restrict__ hflux, double *__restrict__ out) {
     const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
     const int c = wave_entity_base(nb);
@@ -102,19 +104,21 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_scalar_update(
         else if (k == L - 1) wd = ww[L - 1] * (m.fnm[L - 1] * sn0 + m.fnp[L - 1] * sn_m1);
         else if (k >= 2 && k <= L - 2) wd = flux3(sn_m2, sn_m1, sn0, sn_p1, ww[k], coef3);
         const double wd_p1 = __shfl_down(wd, 1);  // the flux at interface k + 1
-        double hf[kMaxEdges];
+        freal hf[kMaxEdges];
 #pragma unroll
         for (int j = 0; j < kMaxEdges; ++j) {
             hf[j] = 0.0;
-            if (j < n) hf[j] = hflux[((size_t)eo[j] * L + k) * S + s];
+            if (j < n) hf[j] = (freal)hflux[((size_t)eo[j] * L + k) * S + s];
         }
-        double hd = 0.0;
+        // the flux divergence: a compensated binary32 sum in the mixed build
+        SlotSum hd;
+        hd.first(0.0);
 #pragma unroll
         for (int j = 0; j < kMaxEdges; ++j) {
-            const double cc = -sg[j] * uh[j] * hf[j];
-            hd = (j < n) ? ((j == 0) ? cc : hd + cc) : hd;
+            const freal cc = -(freal)sg[j] * (freal)uh[j] * hf[j];
+            hd.slot(j, n, cc);
         }
-        const double tend_col = hd * m.invAreaCell[c] + 0.0;
+        const double tend_col = (double)(hd.value() * (freal)m.invAreaCell[c]) + 0.0;
         double vdiv;
         if (k == 0) vdiv = m.rdnw[0] * wd_p1;
         else if (k == L - 1) vdiv = m.rdnw[L - 1] * (-wd);
@@ -125,7 +129,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_scalar_update(
 
 void launch_advance_scalars(const mpas_hip_mesh &m, int S, double dt, double coef3, double w_old, double w_new,
                             const double *scalar_old, const double *scalar_new_in, const double *rho_zz_old,
-                            const double *rho_zz_new, const hreal *uhAvg, const double *wwAvg,
+                            const double *rho_zz_new, const double *uhAvg, const double *wwAvg,
                             double *scalar_new_out, hreal *horiz_flux, hipStream_t s) {
     const int L = m.nLevels;
     (void)L;
@@ -171,7 +175,7 @@ static MonoScratch carve_mono(const mpas_hip_mesh &m, double *base) {
 // the explicit passes above: scalar connectivity loads, slot gathers issued
 // before use, and neighbouring levels of the own column by lane shuffles.
 __global__ __launch_bounds__(kMaxWaveBlock) void k_mono_density(long long nb, mpas_hip_mesh m, double dt,
-                                                                const hreal *__restrict__ uhAvg,
+                                                                const double *__restrict__ uhAvg,
                                                                 const double *__restrict__ wwAvg,
                                                                 const double *__restrict__ rho_old,
                                                                 double *__restrict__ rho_int) {
@@ -269,7 +273,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_mono_bounds(long long nb, mpa
 
 // horizontal upwind and antidiffusive fluxes
 __global__ __launch_bounds__(kMaxWaveBlock) void k_mono_hflux(long long nb, mpas_hip_mesh m, int S, int s, double dt,
-                                                              const hreal *__restrict__ uhAvg,
+                                                              const double *__restrict__ uhAvg,
                                                               const double *__restrict__ sn,
                                                               const double *__restrict__ so, hreal *__restrict__ fup,
                                                               hreal *__restrict__ ftmp) {
@@ -429,7 +433,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_mono_final(long long nb, mpas
 }
 
 void launch_advance_scalars_mono(const mpas_hip_mesh &m, int S, double dt, double coef3, const double *scalars_old,
-                                 const double *scalars_new_in, const double *rho_zz_old, const hreal *uhAvg,
+                                 const double *scalars_new_in, const double *rho_zz_old, const double *uhAvg,
                                  const double *wwAvg, double *scalars_new_out, double *rho_zz_int, double *scratch,
                                  hipStream_t s) {
     const int L = m.nLevels, nC = m.nCells, nE = m.nEdges;

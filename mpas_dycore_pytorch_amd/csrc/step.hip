@@ -147,10 +147,10 @@ static void scale_d(float *dst, const float *src, double f, size_t n, hipStream_
 // workspace layout
 // ---------------------------------------------------------------------------
 struct Work {
-    hreal *ru_save;
+    double *ru_save;
     double *rw_save, *rtheta_p_save, *rho_p_save, *theta_m_init, *rho_zz_old_split;
     double *scalars_old, *scalars_tmp;
-    hreal *ruAvg_split;
+    double *ruAvg_split;
     double *wwAvg_split;
     double *cqw;
     hreal *cqu;
@@ -161,7 +161,7 @@ struct Work {
     double *tend_w_euler, *tend_theta_euler, *kdiff;
     hreal *tend_u;
     double *tend_w, *tend_theta, *h_divergence, *rthdynten;
-    hreal *ru_p, *ruAvg;
+    double *ru_p, *ruAvg;
     double *rho_pp, *rtheta_pp, *rtheta_pp_old, *rw_p, *wwAvg;
     double *rho_zz_int;
     hreal *horiz_flux;
@@ -180,7 +180,7 @@ static Work carve_work(const mpas_hip_mesh &m, int S, double *base) {
                  W = (size_t)(m.nCells + 1) * (L + 1);
     Work w;
     Carver k{reinterpret_cast<char *>(base)};  // 256-byte aligned slices
-    w.ru_save = k.take<hreal>(E);
+    w.ru_save = k.take<double>(E);
     w.rw_save = k.take<double>(W);
     w.rtheta_p_save = k.take<double>(C);
     w.rho_p_save = k.take<double>(C);
@@ -188,7 +188,7 @@ static Work carve_work(const mpas_hip_mesh &m, int S, double *base) {
     w.rho_zz_old_split = k.take<double>(C);
     w.scalars_old = k.take<double>(C * S);
     w.scalars_tmp = k.take<double>(C * S);
-    w.ruAvg_split = k.take<hreal>(E);
+    w.ruAvg_split = k.take<double>(E);
     w.wwAvg_split = k.take<double>(W);
     w.cqw = k.take<double>(W);
     w.cqu = k.take<hreal>(E);
@@ -211,8 +211,8 @@ static Work carve_work(const mpas_hip_mesh &m, int S, double *base) {
     w.tend_theta = k.take<double>(C);
     w.h_divergence = k.take<double>(C);
     w.rthdynten = k.take<double>(C);
-    w.ru_p = k.take<hreal>(E);
-    w.ruAvg = k.take<hreal>(E);
+    w.ru_p = k.take<double>(E);
+    w.ruAvg = k.take<double>(E);
     w.rho_pp = k.take<double>(C);
     w.rtheta_pp = k.take<double>(C);
     w.rtheta_pp_old = k.take<double>(C);
@@ -227,10 +227,10 @@ static Work carve_work(const mpas_hip_mesh &m, int S, double *base) {
 
 // The state as the step sees it: the edge/vertex fields in hreal.
 struct StepState {
-    hreal *u;
+    double *u;
     double *w, *theta_m, *rho_zz, *scalars, *rt_diabatic_tend;
     double *rho_p, *rtheta_p, *exner, *pressure_p;
-    hreal *ru;
+    double *ru;
     double *rw;
     hreal *v;
     double *divergence;
@@ -240,7 +240,7 @@ struct StepState {
     double *ur_cell, *vr_cell;
     double *uReconstructX, *uReconstructY, *uReconstructZ;
     double *uReconstructZonal, *uReconstructMeridional;
-    hreal *ruAvg;
+    double *ruAvg;
     double *wwAvg;
 };
 template <typename St>

@@ -223,7 +223,10 @@ class Srk3Stepper:
 
     def __init__(self, state: Mapping[str, Any], mesh: Mapping[str, Any], config: Mapping[str, Any] | None,
                  dt: float, device: torch.device | str | None = None, strict: bool = True,
-                 reorder: bool = False, precision: str = "fp64"):
+                 reorder: bool = False, precision: str | None = None):
+        if precision is None:
+            # the build's own configuration key: "fp64" (default) or "mixed"
+            precision = str(_cfg(config).get("config_precision", "fp64"))
         if precision not in PRECISIONS:
             raise MpasConfigError(f"precision must be one of {PRECISIONS}, got {precision!r}")
         self.precision = precision
@@ -363,12 +366,14 @@ class _Srk3Step(torch.autograd.Function):
         out_device = u.device if isinstance(u, torch.Tensor) else torch.device("cpu")
         gpu = out_device if out_device.type == "cuda" else None
         stepper = Srk3Stepper(state, box["mesh"], box["config"], box["dt"], device=gpu, strict=box["strict"])
-        ckpt = {k: stepper.t[k].clone() for k in STEP_INPUTS}
+        # the checkpoint is binary64 (widened from the mixed step's binary32 fields): the
+        # backward recomputes the step in binary64 around the trajectory the forward took
+        ckpt = {k: stepper.t[k].to(torch.float64).clone() for k in STEP_INPUTS}
         cfg = N.ConfigStruct()
         C.pointer(cfg)[0] = stepper.cfg
         cfg.diagnostics_current = 0
         kept = None
-        if box.get("keep_tape", False):
+        if box.get("keep_tape", False) and not stepper.mixed:
             tape = Tape(stepper.mesh)
             with torch.cuda.device(stepper.device):
                 taped = taped_step(stepper.mesh, ckpt, cfg, float(box["dt"]), tape)
