@@ -239,19 +239,24 @@ int mpas_hip_reconstruct(const mpas_hip_mesh *m, const double *u, double *ux, do
  *          (+=), output adjoints are read and may be overwritten.  A null
  *          input slot is skipped; output slots named per function must be
  *          non-null.
- * Both arrays are host memory.  Cross-entity sums use FP64 atomics, so the
- * results agree with an exact-order sum to rounding only.
+ * Both arrays are host memory.  No kernel uses atomics: each element is
+ * written by one thread per launch and every cross-entity sum is gathered
+ * in slot order, so the adjoints are the same bit for bit on every run.
  * ---------------------------------------------------------------------- */
-/* Inverse connectivity for atomic-free adjoint gathers: the counterpart of the
+/* Inverse connectivity for the adjoint gathers: the counterpart of the
  * reference's ordered output transposes (build_ordered_output_transpose,
- * static_payloads.py:44-118).  CSR tables ordered by (edge, slot):
+ * static_payloads.py:44-118).  CSR tables ordered by (source row, slot):
  *   advStart[c] .. advStart[c+1]-1  index the (advEdge, advSlot) pairs with
  *                                   advCellsForEdge[advEdge][advSlot] == c;
  *   eoeStart[e] .. eoeStart[e+1]-1  index the (eoeEdge, eoeSlot) pairs with
- *                                   edgesOnEdge[eoeEdge][eoeSlot] == e. */
+ *                                   edgesOnEdge[eoeEdge][eoeSlot] == e;
+ *   cocStart[c] .. cocStart[c+1]-1  index the (cocCell, cocSlot) pairs with
+ *                                   cellsOnCell[cocCell][cocSlot] == c.
+ * Required by every *_vjp entry point that takes it (a null pointer is an error). */
 typedef struct mpas_hip_inverse_tables {
     const int32_t *advStart, *advEdge, *advSlot; /* nCells + 1, then one entry per (edge, slot) */
     const int32_t *eoeStart, *eoeEdge, *eoeSlot; /* nEdges + 1, then one entry per (edge, slot) */
+    const int32_t *cocStart, *cocCell, *cocSlot; /* nCells + 1, then one entry per (cell, slot) */
 } mpas_hip_inverse_tables;
 
 /* doubles of device scratch any *_vjp entry point below needs */
@@ -265,16 +270,13 @@ int mpas_hip_moist_coefficients_vjp(const mpas_hip_mesh *m, int32_t nScalars, in
  * (cofrz and cofwr depend on the mesh only; cofwz/coftz/cofwt adjoints required) */
 int mpas_hip_vert_imp_coefs_vjp(const mpas_hip_mesh *m, double dts, double epssm, const double *const *fwd,
                                 double *const *bar, void *stream);
-/* fwd: u, h, v_in | h_edge, v, divergence, vorticity, ke, pv_edge.  With inverse tables
- * (inv non-null) every transpose is a gather; without, atomic scatters. */
+/* fwd: u, h, v_in | h_edge, v, divergence, vorticity, ke, pv_edge */
 int mpas_hip_solve_diagnostics_vjp(const mpas_hip_mesh *m, double dt, double apvm, int32_t rk_step,
                                    const double *const *fwd, double *const *bar, double *scratch,
                                    const mpas_hip_inverse_tables *inv, void *stream);
 /* fwd: the 21 inputs of mpas_hip_dyn_tend, tend_rho, tend_u_euler, tend_w_euler, tend_theta_euler, kdiff
  * (before) | the same five (after), tend_u, tend_w, tend_theta, h_divergence, rthdynten
- * (h_divergence and kdiff adjoints required).  With inverse tables (inv non-null)
- * the transposes of the advection and PV-flux stencils are gathers; without,
- * atomic scatters. */
+ * (h_divergence and kdiff adjoints required) */
 int mpas_hip_dyn_tend_vjp(const mpas_hip_mesh *m, const mpas_hip_dyn_params *p, const double *const *fwd,
                           double *const *bar, double *scratch, const mpas_hip_inverse_tables *inv,
                           void *stream);
@@ -295,12 +297,13 @@ int mpas_hip_recover_large_step_vjp(const mpas_hip_mesh *m, int32_t rk_step, dou
 /* fwd: scalar_old, scalar_new_in, rho_zz_old, rho_zz_new, uhAvg, wwAvg | scalar_new_out, horiz_flux */
 int mpas_hip_advance_scalars_vjp(const mpas_hip_mesh *m, int32_t nScalars, double dt, double coef3,
                                  double weight_old, double weight_new, const double *const *fwd,
-                                 double *const *bar, double *scratch, void *stream);
+                                 double *const *bar, double *scratch, const mpas_hip_inverse_tables *inv,
+                                 void *stream);
 /* fwd: scalars_old, scalars_new_in, rho_zz_old, uhAvg, wwAvg | scalars_new_out, rho_zz_int
  * (rho_zz_int adjoint required) */
 int mpas_hip_advance_scalars_mono_vjp(const mpas_hip_mesh *m, int32_t nScalars, double dt, double coef3,
                                       const double *const *fwd, double *const *bar, double *scratch,
-                                      void *stream);
+                                      const mpas_hip_inverse_tables *inv, void *stream);
 /* fwd: u | uReconstructX, Y, Z, Zonal, Meridional */
 int mpas_hip_reconstruct_vjp(const mpas_hip_mesh *m, const double *const *fwd, double *const *bar, void *stream);
 

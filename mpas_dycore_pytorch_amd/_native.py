@@ -75,7 +75,8 @@ class AcousticParamsStruct(C.Structure):
 
 
 class InverseTablesStruct(C.Structure):
-    _fields_ = [(n, _dp) for n in ("advStart", "advEdge", "advSlot", "eoeStart", "eoeEdge", "eoeSlot")]
+    _fields_ = [(n, _dp) for n in ("advStart", "advEdge", "advSlot", "eoeStart", "eoeEdge", "eoeSlot",
+                                   "cocStart", "cocCell", "cocSlot")]
 
 
 # Entry points: name -> (restype, argtypes).  "M" marks the mesh pointer.
@@ -113,8 +114,10 @@ SIGNATURES = {
     "mpas_hip_acoustic_step_vjp": (C.c_int, [_PM, C.POINTER(AcousticParamsStruct), _PP, _PP, _dp, _vp]),
     "mpas_hip_divergence_damping_vjp": (C.c_int, [_PM, _F64, _F64, _F64, _PP, _PP, _vp]),
     "mpas_hip_recover_large_step_vjp": (C.c_int, [_PM, _I32, _F64, _I32, _PP, _PP, _vp]),
-    "mpas_hip_advance_scalars_vjp": (C.c_int, [_PM, _I32, _F64, _F64, _F64, _F64, _PP, _PP, _dp, _vp]),
-    "mpas_hip_advance_scalars_mono_vjp": (C.c_int, [_PM, _I32, _F64, _F64, _PP, _PP, _dp, _vp]),
+    "mpas_hip_advance_scalars_vjp": (C.c_int, [_PM, _I32, _F64, _F64, _F64, _F64, _PP, _PP, _dp,
+                                               C.POINTER(InverseTablesStruct), _vp]),
+    "mpas_hip_advance_scalars_mono_vjp": (C.c_int, [_PM, _I32, _F64, _F64, _PP, _PP, _dp,
+                                                    C.POINTER(InverseTablesStruct), _vp]),
     "mpas_hip_reconstruct_vjp": (C.c_int, [_PM, _PP, _PP, _vp]),
     "mpas_hip_moist_coefficients_jvp": (C.c_int, [_PM, _I32, _I32, _I32, _PP, _PP, _vp]),
     "mpas_hip_vert_imp_coefs_jvp": (C.c_int, [_PM, _F64, _F64, _PP, _PP, _vp]),

@@ -316,9 +316,10 @@ def _call_vjp(lib, op: Op, dm: DeviceMesh, fwd, bar, scratch: torch.Tensor, stre
     elif k == "recover_large_step":
         rc = lib.mpas_hip_recover_large_step_vjp(dm.ref, *a, fwd, bar, stream)
     elif k == "advance_scalars":
-        rc = lib.mpas_hip_advance_scalars_vjp(dm.ref, *a, fwd, bar, sp, stream)
+        rc = lib.mpas_hip_advance_scalars_vjp(dm.ref, *a, fwd, bar, sp, C.byref(dm.inverse_tables()), stream)
     elif k == "advance_scalars_mono":
-        rc = lib.mpas_hip_advance_scalars_mono_vjp(dm.ref, *a, fwd, bar, sp, stream)
+        rc = lib.mpas_hip_advance_scalars_mono_vjp(dm.ref, *a, fwd, bar, sp, C.byref(dm.inverse_tables()),
+                                                   stream)
     elif k == "reconstruct":
         rc = lib.mpas_hip_reconstruct_vjp(dm.ref, fwd, bar, stream)
     elif k == "init_coupled":

@@ -5,12 +5,6 @@
 #pragma once
 #include "common.hpp"
 
-namespace mpas {
-namespace adj {
-__device__ __forceinline__ void acc(double *p, double v) { unsafeAtomicAdd(p, v); }
-}  // namespace adj
-}  // namespace mpas
-
 #include "adjoint_bodies.hpp"
 #include "adjoint_dyn.hpp"
 #include "adjoint_transport.hpp"
@@ -49,6 +43,19 @@ inline Slots make_slots(int n, const double *const *fwd, double *const *bar) {
         call;                                                                                       \
     }
 
+// (entity, level) kernels whose lanes exchange values with shuffles: one
+// wavefront per entity, every lane present (the only early return is
+// wave-uniform); the call guards the lanes past the column itself
+#define ADJ_COLUMN_KERNEL(name, call)                                                               \
+    __global__ __launch_bounds__(256) void name(long long nb, long long n_ent, int n_lev, mpas_hip_mesh m, \
+                                                Slots S_, KArgs A) {                                \
+        const int ent = wave_entity_base(nb);                                                       \
+        if (ent >= n_ent) return;                                                                   \
+        const int k = wave_lane();                                                                  \
+        (void)n_lev;                                                                                \
+        call;                                                                                       \
+    }
+
 // Scalar arguments shared by the adjoint and tangent kernels (unused fields ignored).
 struct KArgs {
     int rk, S, q0, q1, s;
@@ -61,7 +68,6 @@ struct KArgs {
     ScalAdjScratch sc;
     MonoScratch mo;
     double *col;
-    int gather;
     mpas_hip_inverse_tables inv;
 };
 
@@ -69,7 +75,7 @@ struct KArgs {
 inline size_t dyn_adj_doubles(const mpas_hip_mesh &m) {
     const size_t L = m.nLevels, C = (m.nCells + 1) * L, E = (m.nEdges + 1) * L, V = (m.nVertices + 1) * L,
                  W = (m.nCells + 1) * (L + 1);
-    return C + E + C + V + C + W + C + W + C + C + V + E + 3 * E;
+    return C + E + C + V + C + W + C + W + C + C + V + E + 3 * E + 8 * E + 2 * C;
 }
 inline DynAdjScratch carve_dyn(const mpas_hip_mesh &m, double *p) {
     const size_t L = m.nLevels, C = (m.nCells + 1) * L, E = (m.nEdges + 1) * L, V = (m.nVertices + 1) * L,
@@ -90,11 +96,17 @@ inline DynAdjScratch carve_dyn(const mpas_hip_mesh &m, double *p) {
     S.g_tf = p; p += E;
     S.g_wf = p; p += E;
     S.g_q = p; p += E;
+    for (int j = 0; j < 8; ++j) {
+        S.x[j] = p;
+        p += E;
+    }
+    S.g_dd = p; p += C;
+    S.g_dof = p; p += C;
     return S;
 }
 inline size_t mono_adj_doubles(const mpas_hip_mesh &m) {
     const size_t L = m.nLevels, C = (m.nCells + 1) * L, E = (m.nEdges + 1) * L, W = (m.nCells + 1) * (L + 1);
-    return 16 * C + 6 * W + 6 * E;
+    return 16 * C + 6 * W + 6 * E + (4 + kMaxEdgesAdj) * C + 2 * E;
 }
 inline MonoScratch carve_mono(const mpas_hip_mesh &m, double *p) {
     const size_t L = m.nLevels, C = (m.nCells + 1) * L, E = (m.nEdges + 1) * L, W = (m.nCells + 1) * (L + 1);
@@ -105,6 +117,12 @@ inline MonoScratch carve_mono(const mpas_hip_mesh &m, double *p) {
     X.g_srf = p; p += C; X.g_sinl = p; p += C; X.g_soutl = p; p += C; X.g_sinr = p; p += C; X.g_soutr = p; p += C;
     X.g_wmid = p; p += W; X.g_wup = p; p += W; X.g_fup = p; p += E; X.g_ftmp = p; p += E;
     X.g_smax = p; p += C; X.g_smin = p; p += C; X.g_wdtn = p; p += W;
+    X.g_sr = p; p += C;
+    for (int j = 0; j < 3 + kMaxEdgesAdj; ++j) {
+        X.r5[j] = p;
+        p += C;
+    }
+    X.g_hv = p; p += E; X.g_up = p; p += E;
     return X;
 }
 

@@ -5,9 +5,10 @@
 // Every entry point takes fwd[] (the forward inputs, then the forward
 // outputs, of one call) and bar[] (their adjoints, same slots; input
 // adjoints are accumulated into, output adjoints are read and may be used as
-// accumulators).  Both are host arrays of device pointers.  Cross-entity
-// contributions are added with hardware FP64 atomics (global_atomic_add_f64),
-// so results are reproducible to rounding, not bit for bit.
+// accumulators).  Both are host arrays of device pointers.  No kernel uses
+// atomics: every element is written by one thread of a launch, contributions
+// between entities are gathered in slot order, so the adjoints are the same
+// bit for bit on every run.
 #include "adjoint_launch.hpp"
 
 namespace mpas {
@@ -16,18 +17,13 @@ namespace adj {
 
 ADJ_WAVE_KERNEL(k_adj_smlstep_cells, smlstep_cells_vjp(m, ADJ_B, ent, k))
 ADJ_WAVE_KERNEL(k_adj_smlstep_edges, smlstep_edges_vjp(m, ADJ_F, ADJ_B, ent, k))
-ADJ_WAVE_KERNEL(k_adj_divdamp, divdamp_vjp(m, A.coef, ADJ_F, ADJ_B, ent, k, false))
+ADJ_WAVE_KERNEL(k_adj_divdamp, divdamp_edges_vjp(m, ADJ_B, ent, k))
 ADJ_WAVE_KERNEL(k_adj_divdamp_cells, divdamp_cells_vjp(m, A.coef, ADJ_F, ADJ_B, ent, k))
 ADJ_WAVE_KERNEL(k_adj_moist_cell, moist_cell_vjp(m, A.S, A.q0, A.q1, ADJ_F, ADJ_B, ent, k))
-ADJ_WAVE_KERNEL(k_adj_moist_edge, moist_edge_vjp(m, A.S, A.q0, A.q1, ADJ_F, ADJ_B, ent, k))
 ADJ_WAVE_KERNEL(k_adj_reconstruct, reconstruct_vjp(m, ADJ_F, ADJ_B, ent, k))
 ADJ_WAVE_KERNEL(k_adj_recover_w, recover_w_vjp(m, A.coef, ADJ_F, ADJ_B, ent, k))
 ADJ_WAVE_KERNEL(k_adj_recover_edge, recover_edge_vjp(m, A.coef, ADJ_F, ADJ_B, ent, k))
 ADJ_WAVE_KERNEL(k_adj_recover_cell, recover_cell_vjp(m, A.rk, A.dt, ADJ_F, ADJ_B, ent, k))
-ADJ_WAVE_KERNEL(k_adj_diag_edge2, diag_edge2_vjp(m, A.dt, A.apvm, A.rk, ADJ_F, ADJ_B, A.ds, ent, k))
-ADJ_WAVE_KERNEL(k_adj_diag_cell, diag_cell_vjp(m, ADJ_F, ADJ_B, A.ds, ent, k))
-ADJ_WAVE_KERNEL(k_adj_diag_vertex, diag_vertex_vjp(m, ADJ_F, ADJ_B, A.ds, ent, k))
-ADJ_WAVE_KERNEL(k_adj_diag_edge1, diag_edge1_vjp(m, ADJ_F, ADJ_B, A.ds, ent, k))
 ADJ_WAVE_KERNEL(k_adj_diag_g1, diag_g1(m, A.dt, A.apvm, A.rk, ADJ_F, ADJ_B, A.ds, ent, k))
 ADJ_WAVE_KERNEL(k_adj_diag_g2, diag_g2(m, A.dt, A.apvm, ADJ_F, ADJ_B, A.ds, ent, k))
 ADJ_WAVE_KERNEL(k_adj_diag_g3, diag_g3(m, A.dt, A.apvm, ADJ_F, ADJ_B, A.ds, ent, k))
@@ -36,43 +32,42 @@ ADJ_WAVE_KERNEL(k_adj_acoustic_edge, acoustic_edge_vjp(m, A.ap, ADJ_F, ADJ_B, en
 ADJ_WAVE_KERNEL(k_adj_dyn_fwd_cells, dyn_fwd_cells(m, ADJ_F, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_fwd_edges, dyn_fwd_edges(m, ADJ_F, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_fwd_del4, dyn_fwd_del4(m, A.dy, ent, k))
-ADJ_WAVE_KERNEL(k_adj_dyn_a1, dyn_a1(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
-ADJ_WAVE_KERNEL(k_adj_dyn_a2, dyn_a2(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
+ADJ_WAVE_KERNEL(k_adj_dyn_a1_del4, dyn_a1_del4(m, A.dp, ADJ_B, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_a2_edges, dyn_a2_edges(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_a2_cells, dyn_a2_cells(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
-ADJ_WAVE_KERNEL(k_adj_dyn_a3, dyn_a3(m, A.dp, ADJ_F, ADJ_B, A.dy, A.gather != 0, ent, k))
-ADJ_WAVE_KERNEL(k_adj_dyn_a3_cells, dyn_a3_cells(m, ADJ_F, ADJ_B, A.dy, A.inv, ent, k))
-ADJ_WAVE_KERNEL(k_adj_dyn_a3_edges, dyn_a3_edges(m, ADJ_F, ADJ_B, A.dy, A.inv, ent, k))
-ADJ_WAVE_KERNEL(k_adj_dyn_a4, dyn_a4(m, A.dy, ent, k))
-ADJ_WAVE_KERNEL(k_adj_dyn_a5, dyn_a5(m, ADJ_B, A.dy, ent, k))
+ADJ_WAVE_KERNEL(k_adj_dyn_a3_cells, dyn_a3_cells(m, A.dp, ADJ_F, ADJ_B, A.dy, A.inv, ent, k))
+ADJ_WAVE_KERNEL(k_adj_dyn_a3_edges, dyn_a3_edges(m, A.dp, ADJ_F, ADJ_B, A.dy, A.inv, ent, k))
 ADJ_WAVE_KERNEL(k_adj_dyn_a6, dyn_a6(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
+ADJ_WAVE_KERNEL(k_adj_dyn_a7, dyn_a7(m, A.dp, ADJ_B, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_adj_scal_cell, scal_cell_vjp(m, A.S, A.dt, A.coef3, A.w_old, A.w_new, ADJ_F, ADJ_B, A.sc, ent, k))
 ADJ_WAVE_KERNEL(k_adj_scal_edge, scal_edge_vjp(m, A.S, ADJ_F, ADJ_B, A.sc, ent, k))
-ADJ_WAVE_KERNEL(k_adj_scal_vert, scal_vert_vjp(m, A.S, A.dt, A.coef3, A.w_old, A.w_new, ADJ_F, ADJ_B, ent, k))
+ADJ_COLUMN_KERNEL(k_adj_scal_col,
+                  scal_col_vjp(m, A.inv, A.S, A.dt, A.coef3, A.w_old, A.w_new, ADJ_F, ADJ_B, A.sc, ent, k))
 ADJ_WAVE_KERNEL(k_adj_mono_f1, mono_f1(m, A.S, A.s, A.coef3, ADJ_F, A.mo, ent, k))
 ADJ_WAVE_KERNEL(k_adj_mono_f2, mono_f2(m, A.S, A.s, A.dt, ADJ_F, A.mo, ent, k))
 ADJ_WAVE_KERNEL(k_adj_mono_f3, mono_f3(m, A.S, A.s, A.dt, ADJ_F, A.mo, ent, k))
 ADJ_WAVE_KERNEL(k_adj_mono_f4, mono_f4(m, A.S, A.s, ADJ_F, A.mo, ent, k))
 ADJ_WAVE_KERNEL(k_adj_mono_r1, mono_r1(m, A.S, A.s, ADJ_F, ADJ_B, A.mo, ent, k))
 ADJ_WAVE_KERNEL(k_adj_mono_r2, mono_r2(m, A.mo, ent, k))
-ADJ_WAVE_KERNEL(k_adj_mono_r3, mono_r3(m, A.mo, ent, k))
 ADJ_WAVE_KERNEL(k_adj_mono_r4, mono_r4(m, A.S, A.s, ADJ_F, ADJ_B, A.mo, ent, k))
 ADJ_WAVE_KERNEL(k_adj_mono_r5, mono_r5(m, A.S, A.s, ADJ_F, ADJ_B, A.mo, ent, k))
-ADJ_WAVE_KERNEL(k_adj_mono_r6, mono_r6(m, A.S, A.s, A.dt, A.coef3, ADJ_F, ADJ_B, A.mo, ent, k))
+ADJ_COLUMN_KERNEL(k_adj_mono_r6, mono_r6(m, A.S, A.s, A.dt, A.coef3, ADJ_F, ADJ_B, A.mo, ent, k))
 ADJ_WAVE_KERNEL(k_adj_mono_r7, mono_r7(m, A.S, A.s, A.dt, ADJ_F, ADJ_B, A.mo, ent, k))
+ADJ_WAVE_KERNEL(k_adj_mono_r7g, mono_r7g(m, A.inv, A.S, A.s, ADJ_F, ADJ_B, A.mo, ent, k))
 ADJ_WAVE_KERNEL(k_adj_mono_r8, mono_r8(m, A.dt, ADJ_F, ADJ_B, ent, k))
+ADJ_WAVE_KERNEL(k_adj_mono_r8e, mono_r8e(m, A.dt, ADJ_B, ent, k))
+
+ADJ_COLUMN_KERNEL(k_adj_dyn_a1, dyn_a1(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
 
 // ---------------------------------------------------------------------------
-// Dynamics-tendency edge adjoint, one wavefront per edge (lane = level), for
-// the gather form (inverse tables given).  The algebra is dyn_a3's; what
-// changes is where the sums are formed:
+// Dynamics-tendency edge adjoint (A3), one wavefront per edge (lane = level):
 //  * contributions to the edge's own column (u, ru, rho_edge, pv_edge, ...)
 //    are summed in registers, the ones a lane owes the levels next to it
 //    (the flux3 stencil of the u vertical flux, the averaged ru of the w
 //    flux) cross lanes with shuffles, and each lane writes its level once;
-//  * contributions to the two cells (and vertices) of the edge stay FP64
-//    atomics (storing them per edge for a cell-gather pass moved more bytes
-//    than the atomics cost).
+//  * what the edge owes its two cells and vertices is stored per edge (the
+//    x[] arrays) and summed by the cell / vertex gather pass, as are the
+//    advection-stencil and PV-flux transposes (g_tf, g_wf, g_q).
 // All 64 lanes run through the shuffles; lanes past the column compute on a
 // clamped level and contribute nothing.
 // ---------------------------------------------------------------------------
@@ -269,34 +264,22 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
     add_own(B, ru, i, g_ru_own + ((k + 1 <= L - 1) ? ru_from_p1 : 0.0));
     add_own(B, rho_edge, i, g_re);
     add_own(B, pv_edge, i, g_pv_own);
-    // the two cells (and vertices) of the edge: hardware FP64 atomics
-    const long long o1 = (long long)c1 * L + k, o2 = (long long)c2 * L + k;
-    add(B, ke, o1, x_ke);
-    add(B, ke, o2, -x_ke);
-    acc(B[h_divergence] + o1, x_hd);
-    acc(B[h_divergence] + o2, x_hd);
-    if (k >= 1) {
-        add(B, rw, (long long)c1 * Lw + k, x_rw);
-        add(B, rw, (long long)c2 * Lw + k, x_rw);
-    }
+    // what the two cells (and vertices) of the edge receive, summed there by
+    // k_adj_dyn_a3_cells
+    S.x[0][i] = x_ke;
+    S.x[1][i] = x_hd;
+    S.x[2][i] = (k >= 1) ? x_rw : 0.0;
     if (rk1) {
-        add(B, pp, o1, x_pp);
-        add(B, pp, o2, -x_pp);
-        acc(S.g_dpdz + o1, x_dp);
-        acc(S.g_dpdz + o2, x_dp);
-        acc(B[kdiff] + o1, x_kd);
-        acc(B[kdiff] + o2, x_kd);
-        S.g_delsq_u[i] += g_dsu;
+        S.x[3][i] = x_pp;
+        S.x[4][i] = x_dp;
+        S.x[5][i] = x_kd;
+        S.g_delsq_u[i] = g_dsu;
         if (P.h_mom_eddy_visc4 > 0.0) {
-            const int v1 = m.verticesOnEdge[2 * e], v2 = m.verticesOnEdge[2 * e + 1];
-            acc(S.g_delsq_div + o1, x_dd);
-            acc(S.g_delsq_div + o2, -x_dd);
-            acc(S.g_delsq_vort + (long long)v1 * L + k, -x_dv);
-            acc(S.g_delsq_vort + (long long)v2 * L + k, x_dv);
+            S.x[6][i] = x_dd;
+            S.x[7][i] = x_dv;
         }
     } else {
-        add(B, theta_m_save, o1, x_ths);
-        add(B, theta_m_save, o2, x_ths);
+        S.x[3][i] = x_ths;
     }
 }
 
@@ -568,8 +551,9 @@ __global__ __launch_bounds__(256) void k_adj_acoustic_edge_flux(long long nb, lo
     acoustic_flux_gather(m, A.ap, S_.f, S_.b, A.col, ent, k);
     acoustic_edge_vjp(m, A.ap, S_.f, S_.b, ent, k);
 }
-// thread (cell c, level k): theta_m(c) through the flux terms of its
-// neighbours (the cell's own were added in k_adj_acoustic_colw)
+// thread (cell c, level k), after k_adj_acoustic_edge_flux: theta_m(c)
+// through the flux terms of its neighbours (the cell's own were added in
+// k_adj_acoustic_colw), then the pressure-gradient terms of its edges
 __global__ __launch_bounds__(256) void k_adj_acoustic_theta(long long nb, long long n_ent, int n_lev,
                                                             mpas_hip_mesh m, Slots S_, KArgs A) {
     using namespace acoustic;
@@ -605,6 +589,7 @@ __global__ __launch_bounds__(256) void k_adj_acoustic_theta(long long nb, long l
         gth += 0.5 * (-flux * gt[i]);
     }
     add_own(S_.b, theta_m, (long long)c * L + k, gth);
+    acoustic_pg_cell(m, A.ap, F, S_.b, c, k);
 }
 
 // Adjoint of the vertical-implicit coefficients (vimp_col_vjp, same algebra).
@@ -727,7 +712,7 @@ size_t vjp_scratch_doubles(const mpas_hip_mesh &m, int S) {
     size_t n = dyn_adj_doubles(m);
     n = std::max(n, 2 * V + 2 * C + E);                                      // diagnostics
     n = std::max(n, (size_t)acoustic::kScratchArrays * (L + 1) * m.nCells);  // acoustic columns
-    n = std::max(n, C * (size_t)(S > 0 ? S : 1));                            // explicit transport
+    n = std::max(n, (C + E) * (size_t)(S > 0 ? S : 1));                      // explicit transport
     n = std::max(n, mono_adj_doubles(m));                                    // monotonic transport
     return n;
 }
@@ -805,7 +790,6 @@ int mpas_hip_moist_coefficients_vjp(const mpas_hip_mesh *mesh, int32_t nScalars,
     A.q0 = moist_start - 1;
     A.q1 = moist_end;
     ADJ_LAUNCH(k_adj_moist_cell, m.nCells, m.nLevels, STREAM(stream));
-    ADJ_LAUNCH(k_adj_moist_edge, m.nEdges, m.nLevels, STREAM(stream));
     return check_launch("moist_coefficients_vjp");
 }
 
@@ -814,7 +798,7 @@ int mpas_hip_reconstruct_vjp(const mpas_hip_mesh *mesh, const double *const *fwd
     const mpas_hip_mesh &m = *mesh;
     Slots S_ = make_slots(recon::N, fwd, bar);
     KArgs A{};
-    ADJ_LAUNCH(k_adj_reconstruct, m.nCells, m.nLevels, STREAM(stream));
+    ADJ_LAUNCH(k_adj_reconstruct, m.nEdges, m.nLevels, STREAM(stream));
     return check_launch("reconstruct_vjp");
 }
 
@@ -848,20 +832,16 @@ int mpas_hip_solve_diagnostics_vjp(const mpas_hip_mesh *mesh, double dt, double 
     A.apvm = apvm;
     A.rk = rk_step;
     A.ds = DiagScratch{scratch, scratch + V, scratch + 2 * V, scratch + 2 * V + C, scratch + 2 * V + C + E};
-    if (inv) {
-        // gather form: every live row of the scratch is written before it is read
-        A.inv = *inv;
-        ADJ_LAUNCH(k_adj_diag_g1, m.nEdges, m.nLevels, s);
-        ADJ_LAUNCH(k_adj_diag_g2, m.nCells, m.nLevels, s);
-        ADJ_LAUNCH(k_adj_diag_g3, m.nVertices, m.nLevels, s);
-        ADJ_LAUNCH(k_adj_diag_g4, m.nEdges, m.nLevels, s);
-        return check_launch("solve_diagnostics_vjp");
+    if (!inv) {
+        set_error("solve_diagnostics_vjp: the inverse connectivity tables are required");
+        return 1;
     }
-    fill_zero(scratch, 2 * V + C + E, s);
-    ADJ_LAUNCH(k_adj_diag_edge2, m.nEdges, m.nLevels, s);
-    ADJ_LAUNCH(k_adj_diag_cell, m.nCells, m.nLevels, s);
-    ADJ_LAUNCH(k_adj_diag_vertex, m.nVertices, m.nLevels, s);
-    ADJ_LAUNCH(k_adj_diag_edge1, m.nEdges, m.nLevels, s);
+    // every live row of the scratch is written before it is read
+    A.inv = *inv;
+    ADJ_LAUNCH(k_adj_diag_g1, m.nEdges, m.nLevels, s);
+    ADJ_LAUNCH(k_adj_diag_g2, m.nCells, m.nLevels, s);
+    ADJ_LAUNCH(k_adj_diag_g3, m.nVertices, m.nLevels, s);
+    ADJ_LAUNCH(k_adj_diag_g4, m.nEdges, m.nLevels, s);
     return check_launch("solve_diagnostics_vjp");
 }
 
@@ -893,8 +873,8 @@ int mpas_hip_acoustic_step_vjp(const mpas_hip_mesh *mesh, const mpas_hip_acousti
     A.col = scratch;
     MPAS_LAUNCH(k_adj_acoustic_colw, dim3((m.nCells + kColWaves - 1) / kColWaves), dim3(64, kColWaves), 0, s, m,
                 S_, A);
-    ADJ_LAUNCH(k_adj_acoustic_theta, m.nCells, m.nLevels, s);
     ADJ_LAUNCH(k_adj_acoustic_edge_flux, m.nEdges, m.nLevels, s);
+    ADJ_LAUNCH(k_adj_acoustic_theta, m.nCells, m.nLevels, s);
     return check_launch("acoustic_step_vjp");
 }
 
@@ -908,41 +888,43 @@ int mpas_hip_dyn_tend_vjp(const mpas_hip_mesh *mesh, const mpas_hip_dyn_params *
     KArgs A{};
     A.dp = *p;
     A.dy = carve_dyn(m, scratch);
-    A.gather = inv ? 1 : 0;
-    if (inv) A.inv = *inv;
+    if (!inv) {
+        set_error("dyn_tend_vjp: the inverse connectivity tables are required");
+        return 1;
+    }
+    A.inv = *inv;
     const int L = m.nLevels, nC = m.nCells, nE = m.nEdges, nV = m.nVertices;
     const bool rk1 = (p->rk_step == 1);
-    fill_zero(scratch, dyn_adj_doubles(m), s);
+    const bool del4 = rk1 && p->h_mom_eddy_visc4 > 0.0;
     const int ncv = nC > nV ? nC : nV;
+    // every scratch element a pass reads was written by an earlier pass of this call
     if (rk1) {
         ADJ_LAUNCH(k_adj_dyn_fwd_cells, nC, L, s);
         ADJ_LAUNCH(k_adj_dyn_fwd_edges, nE, L, s);
         ADJ_LAUNCH(k_adj_dyn_fwd_del4, ncv, L, s);
     }
     ADJ_LAUNCH(k_adj_dyn_a1, nC, L + 1, s);
-    if (rk1 && inv) {
+    if (rk1) {
+        ADJ_LAUNCH(k_adj_dyn_a1_del4, nC, L + 1, s);
         ADJ_LAUNCH(k_adj_dyn_a2_edges, nE, L, s);
         ADJ_LAUNCH(k_adj_dyn_a2_cells, nC, L, s);
-    } else if (rk1) {
-        ADJ_LAUNCH(k_adj_dyn_a2, nC, L + 1, s);
     }
-    if (inv) {
-        ADJ_LAUNCH(k_adj_dyn_a3w, nE, L, s);
-        ADJ_LAUNCH(k_adj_dyn_a3_cells, nC, L, s);
-        ADJ_LAUNCH(k_adj_dyn_a3_edges, nE, L, s);
-    } else {
-        ADJ_LAUNCH(k_adj_dyn_a3, nE, L, s);
-    }
-    if (rk1 && p->h_mom_eddy_visc4 > 0.0) ADJ_LAUNCH(k_adj_dyn_a4, ncv, L, s);
-    if (rk1) ADJ_LAUNCH(k_adj_dyn_a5, nE, L, s);
-    ADJ_LAUNCH(k_adj_dyn_a6, nC, L, s);
+    ADJ_LAUNCH(k_adj_dyn_a3w, nE, L, s);
+    ADJ_LAUNCH(k_adj_dyn_a3_cells, del4 ? ncv : nC, L, s);
+    ADJ_LAUNCH(k_adj_dyn_a3_edges, nE, L, s);
+    ADJ_LAUNCH(k_adj_dyn_a6, rk1 ? ncv : nC, L, s);
+    ADJ_LAUNCH(k_adj_dyn_a7, nE, L, s);
     return check_launch("dyn_tend_vjp");
 }
 
 int mpas_hip_advance_scalars_vjp(const mpas_hip_mesh *mesh, int32_t nScalars, double dt, double coef3,
                                  double weight_old, double weight_new, const double *const *fwd, double *const *bar,
-                                 double *scratch, void *stream) {
+                                 double *scratch, const mpas_hip_inverse_tables *inv, void *stream) {
     if (vjp_check(mesh, "advance_scalars_vjp")) return 1;
+    if (!inv) {
+        set_error("advance_scalars_vjp: the inverse connectivity tables are required");
+        return 1;
+    }
     const mpas_hip_mesh &m = *mesh;
     hipStream_t s = STREAM(stream);
     Slots S_ = make_slots(scal::N, fwd, bar);
@@ -952,17 +934,24 @@ int mpas_hip_advance_scalars_vjp(const mpas_hip_mesh *mesh, int32_t nScalars, do
     A.coef3 = coef3;
     A.w_old = weight_old;
     A.w_new = weight_new;
-    A.sc = ScalAdjScratch{scratch};
+    A.inv = *inv;
+    const size_t C = (size_t)(m.nCells + 1) * m.nLevels;
+    A.sc = ScalAdjScratch{scratch, scratch + C * (size_t)(nScalars > 0 ? nScalars : 1)};
     ADJ_LAUNCH(k_adj_scal_cell, m.nCells, m.nLevels, s);
     ADJ_LAUNCH(k_adj_scal_edge, m.nEdges, m.nLevels, s);
-    ADJ_LAUNCH(k_adj_scal_vert, m.nCells, m.nLevels + 1, s);
+    ADJ_LAUNCH(k_adj_scal_col, m.nCells, m.nLevels + 1, s);
     return check_launch("advance_scalars_vjp");
 }
 
 int mpas_hip_advance_scalars_mono_vjp(const mpas_hip_mesh *mesh, int32_t nScalars, double dt, double coef3,
-                                      const double *const *fwd, double *const *bar, double *scratch, void *stream) {
+                                      const double *const *fwd, double *const *bar, double *scratch,
+                                      const mpas_hip_inverse_tables *inv, void *stream) {
     if (vjp_check(mesh, "advance_scalars_mono_vjp")) return 1;
     if (need(bar, mono::rho_zz_int, "advance_scalars_mono_vjp")) return 1;
+    if (!inv) {
+        set_error("advance_scalars_mono_vjp: the inverse connectivity tables are required");
+        return 1;
+    }
     const mpas_hip_mesh &m = *mesh;
     hipStream_t s = STREAM(stream);
     Slots S_ = make_slots(mono::N, fwd, bar);
@@ -970,24 +959,26 @@ int mpas_hip_advance_scalars_mono_vjp(const mpas_hip_mesh *mesh, int32_t nScalar
     A.S = nScalars;
     A.dt = dt;
     A.coef3 = coef3;
+    A.inv = *inv;
     A.mo = carve_mono(m, scratch);
     const int L = m.nLevels, nC = m.nCells, nE = m.nEdges;
+    // every scratch element a pass reads was written by an earlier pass for the same scalar
     for (int sc = 0; sc < nScalars; ++sc) {
         A.s = sc;
-        fill_zero(scratch, mono_adj_doubles(m), s);
         ADJ_LAUNCH(k_adj_mono_f1, nC, L + 1, s);
         ADJ_LAUNCH(k_adj_mono_f2, nE, L, s);
         ADJ_LAUNCH(k_adj_mono_f3, nC, L + 1, s);
         ADJ_LAUNCH(k_adj_mono_f4, nC, L, s);
         ADJ_LAUNCH(k_adj_mono_r1, nC, L, s);
         ADJ_LAUNCH(k_adj_mono_r2, nC, L + 1, s);
-        ADJ_LAUNCH(k_adj_mono_r3, nE, L, s);
         ADJ_LAUNCH(k_adj_mono_r4, nC, L, s);
         ADJ_LAUNCH(k_adj_mono_r5, nC, L, s);
         ADJ_LAUNCH(k_adj_mono_r6, nC, L + 1, s);
         ADJ_LAUNCH(k_adj_mono_r7, nE, L, s);
+        ADJ_LAUNCH(k_adj_mono_r7g, nC, L, s);
     }
-    ADJ_LAUNCH(k_adj_mono_r8, nC, L, s);
+    ADJ_LAUNCH(k_adj_mono_r8, nC, L + 1, s);
+    ADJ_LAUNCH(k_adj_mono_r8e, nE, L, s);
     return check_launch("advance_scalars_mono_vjp");
 }
 

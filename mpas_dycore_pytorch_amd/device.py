@@ -231,23 +231,25 @@ class DeviceMesh:
         return w
 
     def inverse_tables(self) -> "N.InverseTablesStruct":
-        """CSR inverse connectivity of advCellsForEdge and edgesOnEdge (built once).
+        """CSR inverse connectivity of advCellsForEdge, edgesOnEdge and cellsOnCell (built once).
 
         For each cell, the (edge, slot) pairs of the advection stencils that
-        name it; for each edge, the (edge, slot) pairs of the PV-flux stencils
-        that name it; both ordered by (edge, slot).  The adjoint kernels gather
-        over them instead of scattering with atomics."""
+        name it and the (cell, slot) pairs of the neighbour lists that name it;
+        for each edge, the (edge, slot) pairs of the PV-flux stencils that name
+        it; all ordered by (source row, slot).  The adjoint kernels gather over
+        them (each target sums its sources in that order) instead of
+        scattering."""
         if self._inv is None:
             t = {}
-            for prefix, table, count, n_targets in (("adv", "advCellsForEdge", "nAdvCellsForEdge", self.nc),
-                                                    ("eoe", "edgesOnEdge", "nEdgesOnEdge", self.ne)):
-                tab = self.t[table][: self.ne].cpu().numpy()
-                cnt = self.t[count][: self.ne].cpu().numpy().astype(np.int64)
-                start, src, slot = build_inverse(tab, cnt, n_targets)
-                t[prefix + "Start"], t[prefix + "Edge"], t[prefix + "Slot"] = start, src, slot
-            dev = {k: torch.as_tensor(v, device=self.device) for k, v in t.items()}
-            s = N.InverseTablesStruct(*[dev[k].data_ptr() for k in ("advStart", "advEdge", "advSlot",
-                                                                     "eoeStart", "eoeEdge", "eoeSlot")])
+            for prefix, table, count, n_rows, n_targets in (
+                    ("adv", "advCellsForEdge", "nAdvCellsForEdge", self.ne, self.nc),
+                    ("eoe", "edgesOnEdge", "nEdgesOnEdge", self.ne, self.ne),
+                    ("coc", "cellsOnCell", "nEdgesOnCell", self.nc, self.nc)):
+                tab = self.t[table][:n_rows].cpu().numpy()
+                cnt = self.t[count][:n_rows].cpu().numpy().astype(np.int64)
+                t[prefix] = build_inverse(tab, cnt, n_targets)
+            dev = {k: [torch.as_tensor(a, device=self.device) for a in v] for k, v in t.items()}
+            s = N.InverseTablesStruct(*[a.data_ptr() for k in ("adv", "eoe", "coc") for a in dev[k]])
             self._inv = (dev, s)
         return self._inv[1]
 
@@ -264,9 +266,13 @@ def build_inverse(table: np.ndarray, counts: np.ndarray, n_targets: int):
     """CSR transpose of a slot table: (start, source row, slot) ordered by (source row, slot).
 
     ``table[r, j]`` (0-based, the first ``counts[r]`` slots live) names a
-    target; ``start[t]:start[t+1]`` lists every (r, j) with ``table[r, j] == t``."""
+    target; ``start[t]:start[t+1]`` lists every (r, j) with ``table[r, j] == t``;
+    entries naming no target in ``0..n_targets-1`` are left out."""
     rows, slots = np.nonzero(np.arange(table.shape[1])[None, :] < counts[:, None])
     targets = table[rows, slots].astype(np.int64)
+    # entries outside 0..n_targets-1 (a padding row on a bounded mesh) name no target
+    keep = (targets >= 0) & (targets < n_targets)
+    rows, slots, targets = rows[keep], slots[keep], targets[keep]
     order = np.argsort(targets, kind="stable")  # (row, slot) order kept within a target
     start = np.zeros(n_targets + 1, dtype=np.int64)
     np.cumsum(np.bincount(targets, minlength=n_targets), out=start[1:])

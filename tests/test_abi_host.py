@@ -140,15 +140,17 @@ def test_product_path_has_no_oracle_import():
 
 
 def test_inverse_tables_transpose_the_slot_tables():
-    """CSR inverses of advCellsForEdge / edgesOnEdge list every (edge, slot) naming a target, in order."""
+    """CSR inverses of advCellsForEdge / edgesOnEdge / cellsOnCell list every (row, slot) naming a
+    target, in order."""
     from mpas_dycore_pytorch_amd.device import build_inverse
 
     st, _ = _case()
     dm = DeviceMesh(st, torch.device("cpu"))
-    for table, count, n_targets in (("advCellsForEdge", "nAdvCellsForEdge", dm.nc),
-                                    ("edgesOnEdge", "nEdgesOnEdge", dm.ne)):
-        tab = dm.t[table][: dm.ne].numpy()
-        cnt = dm.t[count][: dm.ne].numpy().astype(np.int64)
+    for table, count, n_rows, n_targets in (("advCellsForEdge", "nAdvCellsForEdge", dm.ne, dm.nc),
+                                            ("edgesOnEdge", "nEdgesOnEdge", dm.ne, dm.ne),
+                                            ("cellsOnCell", "nEdgesOnCell", dm.nc, dm.nc)):
+        tab = dm.t[table][:n_rows].numpy()
+        cnt = dm.t[count][:n_rows].numpy().astype(np.int64)
         start, src, slot = build_inverse(tab, cnt, n_targets)
         assert start[0] == 0 and start[-1] == int(cnt.sum()) and np.all(np.diff(start) >= 0)
         seen = set()
@@ -159,6 +161,17 @@ def test_inverse_tables_transpose_the_slot_tables():
                 assert j < cnt[e] and tab[e, j] == t
                 seen.add((e, j))
         assert len(seen) == int(cnt.sum())
+
+
+def test_inverse_table_skips_entries_naming_no_target():
+    """A slot naming a padding row (index n_targets) is left out of the transpose."""
+    from mpas_dycore_pytorch_amd.device import build_inverse
+
+    tab = np.array([[1, 3, 0], [2, 0, 3], [1, 2, 0]])
+    cnt = np.array([3, 2, 2])
+    start, src, slot = build_inverse(tab, cnt, 3)
+    assert start.tolist() == [0, 2, 4, 6]
+    assert list(zip(src.tolist(), slot.tolist())) == [(0, 2), (1, 1), (0, 0), (2, 0), (1, 0), (2, 1)]
 
 
 def test_launch_shape_table():
