@@ -147,7 +147,7 @@ def reference_cpu_timing() -> dict | None:
             "where": "reference atm_srk3 on the build container's CPU (Intel Xeon 2.7 GHz), torch float64"}
 
 
-def vjp_leg(m, stepper, config, dt: float, steps: int, fwd_ms_per_step: float, device) -> dict:
+def vjp_leg(m, stepper, config, dt: float, steps: int, fwd_ms_per_step: float, device, jvp: bool = True) -> dict:
     # m: the prepared mapping (mesh, base state and initial fields, host arrays)
     """Forward + VJP of ``steps`` large steps from the stepper's current state.
 
@@ -186,6 +186,11 @@ def vjp_leg(m, stepper, config, dt: float, steps: int, fwd_ms_per_step: float, d
         if not bool(torch.isfinite(g).all()):
             raise RuntimeError(f"non-finite gradient of {k}")
     peak = torch.cuda.max_memory_allocated(device) / 2**30
+    out = {"steps": steps, "fwd_plus_vjp_ms_per_step": round(1e3 * total_s / steps, 2),
+           "ratio": round(1e3 * total_s / steps / fwd_ms_per_step, 3), "peak_hbm_gib": round(peak, 1),
+           "checkpoint": "per large step (acoustic substeps recomputed); the last step keeps its forward tape"}
+    if not jvp:
+        return out
     # forward mode on the same state: seeded tangents N(0, 0.1) on u and N(0, 0.01) on w (interior
     # interfaces), padding rows zero, through run_simulation.jvp
     gen = torch.Generator(device="cpu").manual_seed(20261001)
@@ -203,11 +208,9 @@ def vjp_leg(m, stepper, config, dt: float, steps: int, fwd_ms_per_step: float, d
     jvp_s = time.perf_counter() - t0
     if not bool(torch.isfinite(tan["theta_m"]).all()):
         raise RuntimeError("non-finite theta_m tangent")
-    return {"steps": steps, "fwd_plus_vjp_ms_per_step": round(1e3 * total_s / steps, 2),
-            "ratio": round(1e3 * total_s / steps / fwd_ms_per_step, 3), "peak_hbm_gib": round(peak, 1),
-            "checkpoint": "per large step (acoustic substeps recomputed); the last step keeps its forward tape",
-            "fwd_plus_jvp_ms_per_step": round(1e3 * jvp_s / steps, 2),
-            "jvp_ratio": round(1e3 * jvp_s / steps / fwd_ms_per_step, 3)}
+    out.update({"fwd_plus_jvp_ms_per_step": round(1e3 * jvp_s / steps, 2),
+                "jvp_ratio": round(1e3 * jvp_s / steps / fwd_ms_per_step, 3)})
+    return out
 
 
 ENSEMBLE_DT = {4: 1800.0, 5: 900.0, 6: 450.0, 7: 300.0, 8: 150.0}
@@ -352,6 +355,7 @@ def main() -> int:
     ap.add_argument("--mixed-steps", type=int, default=6,
                     help="large steps of the mixed-precision leg's error check against the FP64 step (its "
                          "timing uses --steps); 0 skips the leg")
+    ap.add_argument("--no-jvp", action="store_true", help="skip the forward-mode part of the derivative leg")
     ap.add_argument("--vjp-steps", type=int, default=24,
                     help="large steps of the reverse-mode leg (forward + VJP of an area-weighted squared "
                          "theta perturbation, one checkpoint per large step; 24 is the differentiation "
@@ -466,7 +470,7 @@ def main() -> int:
 
     vjp = None
     if args.vjp_steps > 0:
-        vjp = vjp_leg(st, stepper, cfg, args.dt, args.vjp_steps, ms_per_step, device)
+        vjp = vjp_leg(st, stepper, cfg, args.dt, args.vjp_steps, ms_per_step, device, jvp=not args.no_jvp)
         log(f"reverse mode: {vjp}")
 
     cpu = None

@@ -133,10 +133,12 @@ def compute_sfc_permutation(mesh: Mapping[str, Any]) -> Permutation:
     return Permutation(cell_n2o, edge_n2o, vertex_n2o)
 
 
-def _row_entity(name: str, a: np.ndarray, nc: int, ne: int, nv: int) -> str | None:
-    if a.ndim == 0 or (a.ndim == 1 and name in VERTICAL_1D):
+def _row_entity(name: str, a: Any, nc: int, ne: int, nv: int) -> str | None:
+    """The entity kind (cell / edge / vertex) of an array's rows, from its shape."""
+    shape = tuple(a.shape)
+    if len(shape) == 0 or (len(shape) == 1 and name in VERTICAL_1D):
         return None
-    n = a.shape[0]
+    n = shape[0]
     if n == nc + 1:
         return "cell"
     if n == ne + 1:
@@ -159,9 +161,23 @@ def permute_mapping(d: Mapping[str, Any], perm: Permutation, dims: tuple[int, in
         n2o, o2n = o2n, n2o
     counts = {"cell": nc, "edge": ne, "vertex": nv}
     out: dict[str, Any] = {}
+    rows_on: dict[tuple[str, Any], Any] = {}
     for name, v in d.items():
         if isinstance(v, (int, float, str, bool)) or v is None:
             out[name] = v
+            continue
+        if name not in VALUE_ENTITY and type(v).__module__.startswith("torch") and hasattr(v, "index_select"):
+            # a tensor of values: its rows are gathered where it lives (no host round trip)
+            ent = _row_entity(name, v, nc, ne, nv)
+            if ent is None:
+                out[name] = v
+                continue
+            key = (ent, v.device)
+            if key not in rows_on:
+                import torch
+
+                rows_on[key] = torch.as_tensor(np.concatenate([n2o[ent], [counts[ent]]]), device=v.device)
+            out[name] = v.detach().index_select(0, rows_on[key])
             continue
         a = _np(v)
         ent = _row_entity(name, a, nc, ne, nv)
