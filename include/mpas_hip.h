@@ -374,6 +374,43 @@ int mpas_hip_init_coupled_diagnostics_vjp(const mpas_hip_mesh *m, int32_t derive
 int mpas_hip_init_coupled_diagnostics_jvp(const mpas_hip_mesh *m, int32_t derived, const double *theta_base,
                                           const double *const *fwd, double *const *dot, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Stand-alone batched tridiagonal operators (the large step fuses its own
+ * copies of these recurrences).  a, b, c, alpha, gamma: (nColumns, nLevels)
+ * row-major; rhs and the solution: (nColumns, nLevels + 1), the last level is
+ * the boundary value.  Device pointers; every output is written in full
+ * (level-0 entries the recurrence leaves undefined are written as zero); a
+ * null tangent / cotangent input reads as zero.
+ * ---------------------------------------------------------------------- */
+/* TridiagonalFactorize.forward (autodiff/tridiagonal_factor.py:164-181):
+ * alpha(k) = 1 / (b(k) - a(k) gamma(k-1)), gamma(k) = c(k) alpha(k), k = 1..nLevels-1 */
+int mpas_hip_tridiagonal_factorize(int64_t nColumns, int32_t nLevels, const double *a, const double *b,
+                                   const double *c, double *alpha, double *gamma, void *stream);
+/* TridiagonalFactorize.backward (tridiagonal_factor.py:183-230) */
+int mpas_hip_tridiagonal_factorize_vjp(int64_t nColumns, int32_t nLevels, const double *a, const double *c,
+                                       const double *alpha, const double *gamma, const double *grad_alpha,
+                                       const double *grad_gamma, double *grad_a, double *grad_b, double *grad_c,
+                                       void *stream);
+/* TridiagonalFactorize.jvp (tridiagonal_factor.py:232-277) */
+int mpas_hip_tridiagonal_factorize_jvp(int64_t nColumns, int32_t nLevels, const double *a, const double *c,
+                                       const double *alpha, const double *gamma, const double *da, const double *db,
+                                       const double *dc, double *d_alpha, double *d_gamma, void *stream);
+/* TridiagonalSolve.forward (autodiff/tridiagonal_solve.py:194-231):
+ * u(k) = (rhs(k) - a(k) u(k-1)) alpha(k) for k = 1..nLevels-1, then
+ * out(k) = u(k) - gamma(k) out(k+1) for k = nLevels-1..0, out(nLevels) = rhs(nLevels) */
+int mpas_hip_tridiagonal_solve(int64_t nColumns, int32_t nLevels, const double *a, const double *alpha,
+                               const double *gamma, const double *rhs, double *out, void *stream);
+/* TridiagonalSolve.backward (tridiagonal_solve.py:233-291); out is the forward solution */
+int mpas_hip_tridiagonal_solve_vjp(int64_t nColumns, int32_t nLevels, const double *a, const double *alpha,
+                                   const double *gamma, const double *rhs, const double *out, const double *grad_out,
+                                   double *grad_a, double *grad_alpha, double *grad_gamma, double *grad_rhs,
+                                   void *stream);
+/* TridiagonalSolve.jvp (tridiagonal_solve.py:293-352) */
+int mpas_hip_tridiagonal_solve_jvp(int64_t nColumns, int32_t nLevels, const double *a, const double *alpha,
+                                   const double *gamma, const double *rhs, const double *out, const double *da,
+                                   const double *d_alpha, const double *d_gamma, const double *d_rhs, double *d_out,
+                                   void *stream);
+
 /* Launch-shape table, no counterpart in the reference (its kernels are torch
  * operators): wavefronts per block (1, 2, 4, 8 or 16; 0 restores the default 4) of a
  * wavefront-per-entity kernel, by HIP kernel name (e.g. "k_dyn_edges_a").

@@ -119,6 +119,12 @@ SIGNATURES = {
     "mpas_hip_advance_scalars_mono_vjp": (C.c_int, [_PM, _I32, _F64, _F64, _PP, _PP, _dp,
                                                     C.POINTER(InverseTablesStruct), _vp]),
     "mpas_hip_reconstruct_vjp": (C.c_int, [_PM, _PP, _PP, _vp]),
+    "mpas_hip_tridiagonal_factorize": (C.c_int, [_I64, _I32] + [_vp] * 5 + [_vp]),
+    "mpas_hip_tridiagonal_factorize_vjp": (C.c_int, [_I64, _I32] + [_vp] * 9 + [_vp]),
+    "mpas_hip_tridiagonal_factorize_jvp": (C.c_int, [_I64, _I32] + [_vp] * 9 + [_vp]),
+    "mpas_hip_tridiagonal_solve": (C.c_int, [_I64, _I32] + [_vp] * 5 + [_vp]),
+    "mpas_hip_tridiagonal_solve_vjp": (C.c_int, [_I64, _I32] + [_vp] * 10 + [_vp]),
+    "mpas_hip_tridiagonal_solve_jvp": (C.c_int, [_I64, _I32] + [_vp] * 10 + [_vp]),
     "mpas_hip_moist_coefficients_jvp": (C.c_int, [_PM, _I32, _I32, _I32, _PP, _PP, _vp]),
     "mpas_hip_vert_imp_coefs_jvp": (C.c_int, [_PM, _F64, _F64, _PP, _PP, _vp]),
     "mpas_hip_solve_diagnostics_jvp": (C.c_int, [_PM, _F64, _F64, _I32, _PP, _PP, _dp, _vp]),
