@@ -327,6 +327,28 @@ def ensemble_bench(args, device) -> int:
     return 0
 
 
+def layout_probe(args, lib, N) -> int:
+    """Column streams over as many columns as the level has edges, one wavefront
+    per column, at row pitch L (the device layout), L + 1 (whole 64-byte
+    sectors, the spare lane writing the padding) and 64 (whole 128-byte lines):
+    a three-read two-write stream and a gather of the own and six neighbour
+    columns with one write.  Algorithmic bytes count the L live levels only."""
+    n = 30 * 4 ** args.level
+    L = args.levels
+    rows = []
+    for variant, shape, arrays in ((0, "stream: 3 column reads + 2 column writes", 5),
+                                   (1, "gather: own + 6 neighbour columns read, 1 written", 2)):
+        for pitch in (L, L + 1, 64):
+            ms = C.c_double()
+            N.check(lib.mpas_hip_layout_probe(n, L, pitch, variant, 20, C.byref(ms)), "mpas_hip_layout_probe")
+            alg = n * L * 8 * arrays
+            rows.append({"shape": shape, "pitch": pitch, "ms": round(ms.value, 4),
+                         "algorithmic_bytes": alg, "achieved_gbs": round(alg / ms.value / 1e6, 1)})
+            log(f"{shape}, pitch {pitch}: {ms.value:.4f} ms, {alg / ms.value / 1e6:.1f} GB/s")
+    print(json.dumps({"metric": "column-stream time by level pitch", "columns": n, "levels": L, "rows": rows}))
+    return 0
+
+
 def main() -> int:
     if len(sys.argv) > 1 and sys.argv[1] == "--cpu-worker":
         level, n_levels, dt, steps, seed = sys.argv[2:7]
@@ -356,6 +378,8 @@ def main() -> int:
                     help="large steps of the mixed-precision leg's error check against the FP64 step (its "
                          "timing uses --steps); 0 skips the leg")
     ap.add_argument("--no-jvp", action="store_true", help="skip the forward-mode part of the derivative leg")
+    ap.add_argument("--layout-probe", action="store_true",
+                    help="measure column streams at level pitch 55, 56 and 64 over the level's edge count instead")
     ap.add_argument("--vjp-steps", type=int, default=24,
                     help="large steps of the reverse-mode leg (forward + VJP of an area-weighted squared "
                          "theta perturbation, one checkpoint per large step; 24 is the differentiation "
@@ -382,6 +406,8 @@ def main() -> int:
 
     if args.ensemble:
         return ensemble_bench(args, device)
+    if args.layout_probe:
+        return layout_probe(args, lib, N)
 
     m, st, np = prepare(args.level, args.levels, "quadratic", 0, args.dt, args.seed, device)
     nC, nE, nV, L = m["nCells"], m["nEdges"], m["nVertices"], args.levels

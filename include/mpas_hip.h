@@ -431,6 +431,18 @@ int mpas_hip_timer_read(double *total_ms, int32_t *launches);
 /* The same, per launch in launch order (ms_each holds capacity entries). */
 int mpas_hip_timer_read_each(double *ms_each, int32_t capacity, int32_t *launches);
 
+/* Measurement hook, no counterpart in the reference: the column-stream probe
+ * behind the level pitch of the device state.  One wavefront per column (lane =
+ * level) over nColumns columns of nLevels live levels stored at row pitch
+ * `pitch` doubles (nLevels <= pitch <= 64; at pitch nLevels + 1 the spare lane
+ * writes the padding element).  variant 0 reads three arrays and writes two;
+ * variant 1 reads one array at the column and at six neighbour columns (+-97,
+ * +-194, +-291 in storage order) and writes one.  Allocates its own five
+ * buffers of nColumns x pitch doubles, runs 3 untimed and `reps` timed launches
+ * on the null stream and returns the mean launch time in *ms. */
+int mpas_hip_layout_probe(int64_t nColumns, int32_t nLevels, int32_t pitch, int32_t variant, int32_t reps,
+                          double *ms);
+
 #ifdef __cplusplus
 }
 #endif

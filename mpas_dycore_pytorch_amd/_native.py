@@ -119,6 +119,7 @@ SIGNATURES = {
     "mpas_hip_advance_scalars_mono_vjp": (C.c_int, [_PM, _I32, _F64, _F64, _PP, _PP, _dp,
                                                     C.POINTER(InverseTablesStruct), _vp]),
     "mpas_hip_reconstruct_vjp": (C.c_int, [_PM, _PP, _PP, _vp]),
+    "mpas_hip_layout_probe": (C.c_int, [_I64, _I32, _I32, _I32, _I32, C.POINTER(C.c_double)]),
     "mpas_hip_tridiagonal_factorize": (C.c_int, [_I64, _I32] + [_vp] * 5 + [_vp]),
     "mpas_hip_tridiagonal_factorize_vjp": (C.c_int, [_I64, _I32] + [_vp] * 9 + [_vp]),
     "mpas_hip_tridiagonal_factorize_jvp": (C.c_int, [_I64, _I32] + [_vp] * 9 + [_vp]),
