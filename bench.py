@@ -378,6 +378,9 @@ def main() -> int:
                     help="large steps of the mixed-precision leg's error check against the FP64 step (its "
                          "timing uses --steps); 0 skips the leg")
     ap.add_argument("--no-jvp", action="store_true", help="skip the forward-mode part of the derivative leg")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
+                    help="step of the main leg: fp64 (the workload's precision) or the mixed-precision step "
+                         "(the roofline bytes stay the FP64 model's)")
     ap.add_argument("--layout-probe", action="store_true",
                     help="measure column streams at level pitch 55, 56 and 64 over the level's edge count instead")
     ap.add_argument("--vjp-steps", type=int, default=24,
@@ -412,8 +415,9 @@ def main() -> int:
     m, st, np = prepare(args.level, args.levels, "quadratic", 0, args.dt, args.seed, device)
     nC, nE, nV, L = m["nCells"], m["nEdges"], m["nVertices"], args.levels
     cfg = {"config_time_integration_order": 3, "config_len_disp": float(m["nominalMinDc"])}
-    stepper = Srk3Stepper(st, st, cfg, args.dt, device=device, reorder=bool(args.reorder))
+    stepper = Srk3Stepper(st, st, cfg, args.dt, device=device, reorder=bool(args.reorder), precision=args.precision)
     S = stepper.n_scalars
+    tlib = stepper.lib  # the library whose kernels the stepper runs (and whose timer sees them)
 
     stepper.step(args.warmup)
     tuning = None
@@ -432,7 +436,7 @@ def main() -> int:
     schedule = B.launch_schedule(args.roofline_kernel)  # variant of each launch in one large step
     per_step = len(schedule)
     if not args.graph:
-        N.check(lib.mpas_hip_timer_arm(args.roofline_kernel.encode(), per_step * args.steps), "timer_arm")
+        N.check(tlib.mpas_hip_timer_arm(args.roofline_kernel.encode(), per_step * args.steps), "timer_arm")
     stream = torch.cuda.current_stream(device)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -444,14 +448,14 @@ def main() -> int:
     timed_steps = args.steps
     if args.graph:
         # a replayed graph carries no timer events: time the kernel in one eager step
-        N.check(lib.mpas_hip_timer_arm(args.roofline_kernel.encode(), per_step), "timer_arm")
+        N.check(tlib.mpas_hip_timer_arm(args.roofline_kernel.encode(), per_step), "timer_arm")
         with torch.cuda.device(device):
             stepper._launch()
         torch.cuda.synchronize()
         timed_steps = 1
     each = (C.c_double * (per_step * timed_steps))()
     kn = C.c_int32()
-    N.check(lib.mpas_hip_timer_read_each(each, per_step * timed_steps, C.byref(kn)), "timer_read_each")
+    N.check(tlib.mpas_hip_timer_read_each(each, per_step * timed_steps, C.byref(kn)), "timer_read_each")
     if kn.value != per_step * timed_steps:
         raise RuntimeError(f"{args.roofline_kernel}: {kn.value} launches timed, {per_step * timed_steps} expected")
     for k, v in stepper.state_dict().items():
@@ -489,7 +493,7 @@ def main() -> int:
                  "frac": round(step_gbs / HBM_PEAK_GBS, 4)}
 
     mixed = None
-    if args.mixed_steps > 0:
+    if args.mixed_steps > 0 and args.precision == "fp64":
         mixed = mixed_leg(st, cfg, args.dt, args.steps, args.mixed_steps, ms_per_step, bool(args.reorder), device)
         log(f"mixed precision: {mixed['ms_per_step']:.3f} ms/step, x{mixed['speedup_vs_fp64']:.2f}; errors "
             + ", ".join(f"{k} {v['rel_l2']:.1e}" for k, v in mixed["errors_vs_fp64_gpu"].items()))
@@ -511,7 +515,7 @@ def main() -> int:
     out = {
         "metric": METRIC, "value": value, "unit": "cell-level-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32/f64 mixed", "data": "synthetic",
         "config": {"workload": (f"atm_srk3 large step: icosahedral level {args.level} ({nC} cells, {nE} edges) x "
                                 f"{L} quadratic levels to 30 km, dt {args.dt:g} s, RK3 split 3 x 2 acoustic "
                                 f"substeps, monotonic FCT on {S} scalar(s), flat terrain"),
