@@ -78,6 +78,8 @@ typedef struct mpas_hip_config {
     int32_t moist_start, moist_end; /* 1-based inclusive species range */
     int32_t diagnostics_current; /* 1: skip the first-stage diagnostic refresh */
     int32_t order;       /* [config_time_integration_order] 2 or 3 (0 is read as 3) */
+    int32_t rayleigh_levels;      /* [config_number_rayleigh_damp_u_levels] when [config_rayleigh_damp_u], else 0 */
+    double rayleigh_coef_inverse; /* 1 / (rayleigh_levels * [config_rayleigh_damp_u_timescale_days] * 86400) */
 } mpas_hip_config;
 
 /* Number of doubles of workspace needed for one large step. */
@@ -127,6 +129,10 @@ typedef struct mpas_hip_dyn_params {
     double h_theta_eddy_visc4;
     double del4u_div_factor;
     double prandtl_inv;
+    /* Rayleigh damping of u on the top levels (dyn_tend.py:2942-2996, coefficient :10066-10072):
+     * tend_u -= rho_edge u ((k + 1) - k0) rayleigh_coef_inverse for k >= k0 = nLevels - rayleigh_levels */
+    int32_t rayleigh_levels;
+    double rayleigh_coef_inverse;
 } mpas_hip_dyn_params;
 
 /* Scalars of atm_advance_acoustic_step_work (acoustic_step.py:1297-1340). */

@@ -415,7 +415,9 @@ def taped_step(dm: DeviceMesh, state: Mapping[str, torch.Tensor], cfg: Any, dt: 
                       "pv_edge": st["pv_edge"], "pp": st["pressure_p"], "rr_save": rr_save,
                       "cqw": moist["cqw"], "cqu": moist["cqu"], "rt_diabatic_tend": rt_diab, "qtot": moist["qtot"]}
             params = K.dyn_params(rk, dt, len_disp, c_s=cfg.c_s, visc4_2dsmag=cfg.visc4_2dsmag,
-                                  coef_3rd_order=coef3, del4u_div_factor=cfg.del4u_div_factor)
+                                  coef_3rd_order=coef3, del4u_div_factor=cfg.del4u_div_factor,
+                                  rayleigh_levels=cfg.rayleigh_levels,
+                                  rayleigh_coef_inverse=cfg.rayleigh_coef_inverse)
             if rk == 1:
                 carried_in = {k: z(*t.shape) for k, t in
                               zip(K.DYN_CARRIED, (st["rho_zz"], st["u"], st["w"], st["rho_zz"], st["rho_zz"]))}

@@ -183,6 +183,13 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
         const double gq = re * gtu;
         x_ke = re * m.invDcEdge[e] * gtu;
         g_u_own += -0.5 * hsum * gtu;
+        // Rayleigh damping of u on the top levels: tend_u -= rho_edge u damp(k)
+        const int k0 = L - P.rayleigh_levels;
+        if (P.rayleigh_levels > 0 && k >= k0) {
+            const double damp = (((double)k + 1.0) - (double)k0) * P.rayleigh_coef_inverse;
+            g_re += -(ue[k] * damp) * gtu;
+            g_u_own += -(re * damp) * gtu;
+        }
         x_hd = -ue[k] * 0.5 * gtu;
         double gpe = 0.0;
 #pragma unroll

@@ -303,7 +303,13 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_a(long long nb, mpa
         const double base = (double)((freal)rho_edge[i] * (q.value() - ke_delta * (freal)m.invDcEdge[e]) -
                                      (freal)ue[k] * (freal)0.5 * hsum);
         if (lane >= L) return;
-        const double tu = vert + base;
+        double tu = vert + base;
+        // Rayleigh damping of u on the top levels, before the Euler part (dyn_tend.py:2969-2994)
+        const int k0 = L - P.rayleigh_levels;
+        if (P.rayleigh_levels > 0 && k >= k0) {
+            const double damp = (((double)k + 1.0) - (double)k0) * P.rayleigh_coef_inverse;
+            tu = tu - (double)rho_edge[i] * ue[k] * damp;
+        }
         double eu = 0.0;
         if (P.rk_step == 1) {
             double zzs = m.zz[c2 * L + k] + m.zz[c1 * L + k];

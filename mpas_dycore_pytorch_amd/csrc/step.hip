@@ -674,6 +674,8 @@ static int srk3_step_impl(const mpas_hip_mesh *mesh, StepState *st, const mpas_h
         P.h_theta_eddy_visc4 = P.h_mom_eddy_visc4;
         P.del4u_div_factor = cfg->del4u_div_factor;
         P.prandtl_inv = 1.0 / 1.0;
+        P.rayleigh_levels = cfg->rayleigh_levels;
+        P.rayleigh_coef_inverse = cfg->rayleigh_coef_inverse;
     }
 
     for (int sub = 1; sub <= cfg->split; ++sub) {

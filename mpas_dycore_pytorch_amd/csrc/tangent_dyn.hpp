@@ -234,7 +234,14 @@ __device__ __forceinline__ void dyn_j4(const mpas_hip_mesh &m, const mpas_hip_dy
         dtue = dot_of(T, tend_u_euler_in, i);
     }
     dot_put(T, tend_u_euler, i, dtue);
-    dot_put(T, tend_u, i, dvert + dbase + dtue);
+    // Rayleigh damping of u on the top levels: tend_u -= rho_edge u damp(k)
+    double dray = 0.0;
+    const int k0 = L - P.rayleigh_levels;
+    if (P.rayleigh_levels > 0 && k >= k0) {
+        const double damp = (((double)k + 1.0) - (double)k0) * P.rayleigh_coef_inverse;
+        dray = (dre * ue + re * due) * damp;
+    }
+    dot_put(T, tend_u, i, dvert + dbase - dray + dtue);
     // per-edge advective fluxes: theta at level k, w at interface k
     const int na = m.nAdvCellsForEdge[e];
     {

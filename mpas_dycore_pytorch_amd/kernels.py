@@ -54,8 +54,11 @@ def _s(dm: DeviceMesh) -> C.c_void_p:
 
 def dyn_params(rk_step: int, dt: float, len_disp: float, c_s: float = 0.125, visc4_2dsmag: float = 0.05,
                coef_3rd_order: float = 0.25, del4u_div_factor: float = 10.0,
-               prandtl: float = 1.0) -> N.DynParamsStruct:
-    """Scalar set-up of the dynamics tendency (dyn_tend.py:3367-3400, :9616-9626)."""
+               prandtl: float = 1.0, rayleigh_levels: int = 0,
+               rayleigh_coef_inverse: float = 0.0) -> N.DynParamsStruct:
+    """Scalar set-up of the dynamics tendency (dyn_tend.py:3367-3400, :9616-9626); the
+    Rayleigh damping of u on ``rayleigh_levels`` top levels (0: off) with its
+    coefficient 1 / (levels * timescale) (dyn_tend.py:10066-10072)."""
     p = N.DynParamsStruct()
     p.rk_step = int(rk_step)
     p.dt = float(dt)
@@ -67,6 +70,8 @@ def dyn_params(rk_step: int, dt: float, len_disp: float, c_s: float = 0.125, vis
     p.h_theta_eddy_visc4 = p.h_mom_eddy_visc4
     p.del4u_div_factor = float(del4u_div_factor)
     p.prandtl_inv = 1.0 / prandtl
+    p.rayleigh_levels = int(rayleigh_levels)
+    p.rayleigh_coef_inverse = float(rayleigh_coef_inverse)
     return p
 
 

@@ -43,12 +43,18 @@ DEFAULTS = {
     "config_horiz_mixing": "2d_smagorinsky",
     "config_monotonic": True,
     "config_len_disp": 0.0,
+    "config_rayleigh_damp_u": False,
+    "config_rayleigh_damp_u_timescale_days": 5.0,
+    "config_number_rayleigh_damp_u_levels": 6,
 }
 # Short aliases the reference also accepts (config_defaults.py:149-160)
 ALIASES = {"smdiv": "config_smdiv", "len_disp": "config_len_disp", "monotonic": "config_monotonic",
            "time_integration_order": "config_time_integration_order",
            "del4u_div_factor": "config_del4u_div_factor", "visc4_2dsmag": "config_visc4_2dsmag",
-           "positive_definite": "config_positive_definite", "apply_lbcs": "config_apply_lbcs"}
+           "positive_definite": "config_positive_definite", "apply_lbcs": "config_apply_lbcs",
+           "rayleigh_damp_u": "config_rayleigh_damp_u",
+           "rayleigh_damp_u_timescale_days": "config_rayleigh_damp_u_timescale_days",
+           "number_rayleigh_damp_u_levels": "config_number_rayleigh_damp_u_levels"}
 
 # Options whose non-default values select code the GPU step does not have.
 REQUIRED_VALUES = {
@@ -60,7 +66,6 @@ REQUIRED_VALUES = {
     "config_v_theta_eddy_visc2": 0.0,
     "config_h_mom_eddy_visc4": 0.0,
     "config_h_theta_eddy_visc4": 0.0,
-    "config_rayleigh_damp_u": False,
     "config_mpas_cam_coef": 0.0,
     "config_curvature_u": False,
     "config_apply_lbcs": False,
@@ -171,7 +176,27 @@ def resolve_config(config: Mapping[str, Any] | None, state: Mapping[str, Any], m
         raise MpasConfigError(f"moist species range {s.moist_start}..{s.moist_end} outside 1..{n_scalars}")
     s.diagnostics_current = 0
     s.order = int(order)
+    s.rayleigh_levels, s.rayleigh_coef_inverse = rayleigh_damping(c, int(to_numpy(mesh["nVertLevels"]))
+                                                                  if "nVertLevels" in mesh else None)
     return s
+
+
+def rayleigh_damping(c: Mapping[str, Any], n_levels: int | None) -> tuple[int, float]:
+    """(damped top levels, 1 / (levels * timescale in s)) of the Rayleigh damping of u
+    (dyn_tend.py:10066-10072); (0, 0.0) when config_rayleigh_damp_u is off.  The reference
+    divides by the level count and the timescale, so zero or negative values are rejected."""
+    if not bool(c["config_rayleigh_damp_u"]):
+        return 0, 0.0
+    levels = c["config_number_rayleigh_damp_u_levels"]
+    days = float(c["config_rayleigh_damp_u_timescale_days"])
+    if isinstance(levels, bool) or int(levels) != levels or int(levels) < 1:
+        raise MpasConfigError(f"config_number_rayleigh_damp_u_levels must be a positive integer, got {levels!r}")
+    levels = int(levels)
+    if n_levels is not None and levels > n_levels:
+        raise MpasConfigError(f"config_number_rayleigh_damp_u_levels {levels} exceeds the {n_levels} model levels")
+    if not days > 0.0:
+        raise MpasConfigError(f"config_rayleigh_damp_u_timescale_days must be > 0, got {days}")
+    return levels, 1.0 / (float(levels) * (days * 86400.0))
 
 
 def _device_tensor(v: Any, device: torch.device, shape: tuple[int, ...], name: str) -> torch.Tensor:

@@ -240,8 +240,9 @@ void orc_solve_diagnostics(const orc_mesh *m, double dt, double apvm, int rk_ste
  * (edge advection), :4174/:4312 (w/theta horizontal advection), :4415-4657
  * (w vertical), :4803-5057 (theta vertical), :5315-5843 (w/theta mixing),
  * :5901 (theta perturbation flux), :6485 (ordered del4 subtraction).
- * Active configuration: 2d_smagorinsky, no vertical eddy viscosity, no
- * Rayleigh u damping, no curvature, no CAM damping, no physics tendencies.
+ * Active configuration: 2d_smagorinsky, no vertical eddy viscosity, optional
+ * Rayleigh damping of u on the top levels (:2942-2996, coefficient
+ * :10066-10072), no curvature, no CAM damping, no physics tendencies.
  * ---------------------------------------------------------------------- */
 typedef struct orc_dyn_params {
     int rk_step;
@@ -253,6 +254,8 @@ typedef struct orc_dyn_params {
     double h_theta_eddy_visc4;
     double del4u_div_factor;
     double prandtl_inv;
+    int rayleigh_levels;          /* damped top levels, 0 without Rayleigh damping of u */
+    double rayleigh_coef_inverse; /* 1 / (levels * timescale_days * 86400) */
 } orc_dyn_params;
 
 int orc_dyn_params_sizeof(void) { return (int)sizeof(orc_dyn_params); }
@@ -433,6 +436,17 @@ void orc_dyn_tend(const orc_mesh *m, const orc_dyn_params *P,
                 }
             }
         }
+    }
+    /* Rayleigh damping of u on the top levels, before the Euler part is added
+     * (:2969-2994): tend_u -= rho_edge u damp(k), damp(k) = ((k + 1) - k0) / (levels tau) */
+    if (P->rayleigh_levels > 0) {
+        const int k0 = L - P->rayleigh_levels;
+        for (int e = 0; e < nE; ++e)
+            for (int k = k0 < 0 ? 0 : k0; k < L; ++k) {
+                const size_t i = (size_t)e * L + k;
+                const double damp = (((double)k + 1.0) - (double)k0) * P->rayleigh_coef_inverse;
+                tend_u[i] = tend_u[i] - rho_edge[i] * u[i] * damp;
+            }
     }
     for (size_t i = 0; i < (size_t)nE * L; ++i) tend_u[i] = tend_u[i] + tend_u_euler[i];
 

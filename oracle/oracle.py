@@ -53,7 +53,8 @@ class OrcDynParams(C.Structure):
     _fields_ = [("rk_step", C.c_int), ("dt", C.c_double), ("coef_3rd_order", C.c_double),
                 ("kdiff_coef", C.c_double), ("kdiff_cap", C.c_double),
                 ("h_mom_eddy_visc4", C.c_double), ("h_theta_eddy_visc4", C.c_double),
-                ("del4u_div_factor", C.c_double), ("prandtl_inv", C.c_double)]
+                ("del4u_div_factor", C.c_double), ("prandtl_inv", C.c_double),
+                ("rayleigh_levels", C.c_int), ("rayleigh_coef_inverse", C.c_double)]
 
 
 class OrcAcousticParams(C.Structure):
@@ -183,8 +184,11 @@ DYN_INPUTS = ("u", "v", "w", "ru", "rw", "ru_save", "rw_save", "theta_m", "theta
 
 
 def dyn_params(rk_step, dt, len_disp, c_s=0.125, visc4_2dsmag=0.05, coef_3rd_order=0.25,
-               del4u_div_factor=10.0, prandtl=1.0) -> OrcDynParams:
-    """Scalar set-up of dyn_tend.py:3367-3400 and :9616-9626, evaluated in Python."""
+               del4u_div_factor=10.0, prandtl=1.0, rayleigh_levels=0, rayleigh_timescale_days=5.0
+               ) -> OrcDynParams:
+    """Scalar set-up of dyn_tend.py:3367-3400 and :9616-9626, evaluated in Python;
+    Rayleigh damping of u over ``rayleigh_levels`` top levels (0: off) with the
+    coefficient of dyn_tend.py:10066-10072."""
     p = OrcDynParams()
     p.rk_step = int(rk_step)
     p.dt = float(dt)
@@ -196,6 +200,9 @@ def dyn_params(rk_step, dt, len_disp, c_s=0.125, visc4_2dsmag=0.05, coef_3rd_ord
     p.h_theta_eddy_visc4 = p.h_mom_eddy_visc4
     p.del4u_div_factor = float(del4u_div_factor)
     p.prandtl_inv = 1.0 / prandtl
+    p.rayleigh_levels = int(rayleigh_levels)
+    p.rayleigh_coef_inverse = (1.0 / (float(rayleigh_levels) * (rayleigh_timescale_days * 86400.0))
+                               if rayleigh_levels > 0 else 0.0)
     return p
 
 

@@ -64,6 +64,13 @@ TERRAIN_CASES = {
 DEFAULT_CONFIG_CASES = {
     "l2_L20_defaults": (2, 20, "quadratic", 1, 900.0, 2, 8080),
 }
+# whole-step cases with a non-default dynamics-tendency option on top of the
+# RK3 configuration: the same tuple plus the configuration keys that select it
+OPTION_CASES = {
+    "l2_L20_rayleigh": (2, 20, "quadratic", 1, 900.0, 2, 7070,
+                        {"config_rayleigh_damp_u": True, "config_number_rayleigh_damp_u_levels": 5,
+                         "config_rayleigh_damp_u_timescale_days": 0.05}),
+}
 SUMMARY_CASES = {
     "cfg1_l4_L26": (4, 26, "uniform", 0, 1800.0, 2, 20240),
 }
@@ -123,7 +130,7 @@ def add_synthetic_terrain(m: dict, h0: float, coef_3rd_order: float = 0.25) -> d
     return m
 
 
-def reference_config(mesh: dict, dt: float) -> dict:
+def reference_config(mesh: dict, dt: float, extra: dict | None = None) -> dict:
     return R.cfg.validate_config(
         {
             "config_time_integration_order": 3,
@@ -132,6 +139,7 @@ def reference_config(mesh: dict, dt: float) -> dict:
             "config_split_dynamics_transport": True,
             "config_len_disp": float(mesh["nominalMinDc"]),
             "config_dt": float(dt),
+            **(extra or {}),
         }
     )
 
@@ -303,13 +311,13 @@ def _state_arrays(st: dict, keys=None) -> dict:
     return out
 
 
-def run_case(level, n_levels, stretch, n_tracers, dt, steps, seed, terrain=0.0, defaults=False):
+def run_case(level, n_levels, stretch, n_tracers, dt, steps, seed, terrain=0.0, defaults=False, extra=None):
     m = M.build_mesh(level, n_levels, stretch=stretch)
     if terrain:
         m = add_synthetic_terrain(m, terrain)
     raw = IS.raw_state(m, seed=seed, n_tracers=n_tracers)
     st, mesh = prepared_reference_state(m, raw, dt)
-    cfg = R.cfg.validate_config({"config_dt": float(dt)}) if defaults else reference_config(m, dt)
+    cfg = R.cfg.validate_config({"config_dt": float(dt)}) if defaults else reference_config(m, dt, extra)
     prepared = dict(st)
     outs = []
     cur = st
@@ -324,10 +332,15 @@ def run_case(level, n_levels, stretch, n_tracers, dt, steps, seed, terrain=0.0, 
 
 def write_srk3_case(name: str, out_dir: str) -> None:
     defaults = name in DEFAULT_CONFIG_CASES
-    level, L, stretch, ntr, dt, steps, seed, *terrain = (CASES.get(name) or TERRAIN_CASES.get(name)
-                                                         or DEFAULT_CONFIG_CASES[name])
+    extra = None
+    if name in OPTION_CASES:
+        level, L, stretch, ntr, dt, steps, seed, extra = OPTION_CASES[name]
+        terrain = []
+    else:
+        level, L, stretch, ntr, dt, steps, seed, *terrain = (CASES.get(name) or TERRAIN_CASES.get(name)
+                                                             or DEFAULT_CONFIG_CASES[name])
     m, raw, prepared, outs, times = run_case(level, L, stretch, ntr, dt, steps, seed, *(terrain or [0.0]),
-                                             defaults=defaults)
+                                             defaults=defaults, extra=extra)
     payload = {}
     for k, v in _state_arrays(prepared).items():
         payload["in__" + k] = v
@@ -338,7 +351,9 @@ def write_srk3_case(name: str, out_dir: str) -> None:
             "n_tracers": ntr, "dt": dt, "steps": steps, "seed": seed,
             "terrain_height": terrain[0] if terrain else 0.0,
             "reference_seconds_per_step": times,
-            "config": "reference defaults (config_defaults.py)" if defaults else "RK3, len_disp = nominalMinDc"}
+            "config": ("reference defaults (config_defaults.py)" if defaults
+                       else "RK3, len_disp = nominalMinDc" + (f", {extra}" if extra else "")),
+            "config_extra": extra or {}}
     path = os.path.join(out_dir, f"srk3_{name}.npz")
     np.savez_compressed(path, __meta__=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8),
                         **payload)
@@ -387,7 +402,7 @@ def main() -> None:
     if args.which in ("kernels", "all"):
         record_kernels(1, 10, "quadratic", 1, 900.0, 99, os.path.join(args.out, "kernels_l1_L10.npz"))
     if args.which in ("srk3", "all"):
-        for name in list(CASES) + list(TERRAIN_CASES) + list(DEFAULT_CONFIG_CASES):
+        for name in list(CASES) + list(TERRAIN_CASES) + list(DEFAULT_CONFIG_CASES) + list(OPTION_CASES):
             if args.case is None or args.case == name:
                 write_srk3_case(name, args.out)
     if args.which in ("summary", "all"):

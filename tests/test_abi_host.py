@@ -88,7 +88,19 @@ def test_config_gate():
     with pytest.raises(MpasUnsupportedCapabilityError):
         resolve_config({**ok, "config_horiz_mixing": "2d_fixed"}, st, st, meta["dt"], 2)
     with pytest.raises(MpasUnsupportedCapabilityError):
-        resolve_config({**ok, "config_rayleigh_damp_u": True}, st, st, meta["dt"], 2)
+        resolve_config({**ok, "config_curvature_u": True}, st, st, meta["dt"], 2)
+    # Rayleigh damping of u: the reference's coefficient 1 / (levels * days * 86400)
+    # (dyn_tend.py:10066-10072) and defaults (6 levels, 5 days; config_defaults.py:100-102)
+    assert resolve_config(ok, st, st, meta["dt"], 2).rayleigh_levels == 0
+    cfg = resolve_config({**ok, "config_rayleigh_damp_u": True}, st, st, meta["dt"], 2)
+    assert cfg.rayleigh_levels == 6 and cfg.rayleigh_coef_inverse == 1.0 / (6.0 * (5.0 * 86400.0))
+    cfg = resolve_config({**ok, "rayleigh_damp_u": True, "number_rayleigh_damp_u_levels": 3,
+                          "rayleigh_damp_u_timescale_days": 0.5}, st, st, meta["dt"], 2)
+    assert cfg.rayleigh_levels == 3 and cfg.rayleigh_coef_inverse == 1.0 / (3.0 * (0.5 * 86400.0))
+    for bad in ({"config_number_rayleigh_damp_u_levels": 0}, {"config_number_rayleigh_damp_u_levels": 99},
+                {"config_rayleigh_damp_u_timescale_days": 0.0}):
+        with pytest.raises(MpasConfigError):
+            resolve_config({**ok, "config_rayleigh_damp_u": True, **bad}, st, st, meta["dt"], 2)
     with pytest.raises(MpasConfigError):
         resolve_config(ok, st, st, -1.0, 2)
     # config_len_disp <= 0 (the reference default) resolves from nominalMinDc, then from

@@ -83,6 +83,28 @@ def test_srk3_whole_steps_rk2_reference_defaults():
     assert rel_err(wrong["theta_m"], outs[0]["theta_m"]) > 1e-12
 
 
+def test_srk3_whole_steps_rayleigh_damping_of_u():
+    """Rayleigh damping of u on the top 5 levels (config_rayleigh_damp_u, timescale 0.05 days)
+    over two RK3 steps: the oracle against the reference's recording; without the damping
+    the oracle misses it, so the option is exercised."""
+    st, outs, meta = srk3_case("l2_L20_rayleigh")
+    extra = meta["config_extra"]
+    assert extra["config_rayleigh_damp_u"] and extra["config_number_rayleigh_damp_u_levels"] == 5
+    mesh = O.Mesh(st)
+    kw = {"rayleigh_levels": extra["config_number_rayleigh_damp_u_levels"],
+          "rayleigh_timescale_days": extra["config_rayleigh_damp_u_timescale_days"]}
+    cur = st
+    for step, ref in enumerate(outs):
+        cur = OS.srk3(cur, mesh, meta["dt"], len_disp=float(st["nominalMinDc"]), diagnostics_current=(step > 0),
+                      **kw)
+        for k in FIELDS:
+            if k in ref:
+                err = rel_err(cur[k], ref[k])
+                assert err <= 1e-14, f"rayleigh step {step + 1} {k}: rel err {err:.3e}"
+    undamped = OS.srk3(st, mesh, meta["dt"], len_disp=float(st["nominalMinDc"]))
+    assert rel_err(undamped["u"], outs[0]["u"]) > 1e-6
+
+
 @pytest.mark.slow
 def test_level4_summary_from_own_cold_start():
     from mpas_dycore_pytorch_amd import initial_state as IS
