@@ -349,6 +349,36 @@ def layout_probe(args, lib, N) -> int:
     return 0
 
 
+def boundary_bench(args, device) -> int:
+    """``atm_srk3`` called as the reference's driver calls it, with host-resident
+    (NumPy) state and mesh dicts: every call uploads the state, runs one large
+    step and returns its fields to host memory.  The mesh's device copy is kept
+    between calls (``DeviceMesh.get``); the state crosses PCIe both ways."""
+    import torch
+
+    from mpas_dycore_pytorch_amd.time_integration import atm_srk3
+
+    m, st, _ = prepare(args.level, args.levels, "quadratic", 0, args.dt, args.seed, device)
+    cfg = {"config_time_integration_order": 3, "config_len_disp": float(m["nominalMinDc"])}
+    state = dict(st)
+    for _ in range(max(args.warmup, 1)):
+        state = atm_srk3(state, st, cfg, args.dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        state = atm_srk3(state, st, cfg, args.dt)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / args.steps
+    work = m["nCells"] * args.levels
+    log(f"atm_srk3 with host dicts: {1e3 * sec:.2f} ms per call")
+    print(json.dumps({"metric": "cell-level-steps/s of atm_srk3 with host-resident state (PCIe both ways)",
+                      "value": work / sec, "unit": "cell-level-steps/s", "ms_per_call": 1e3 * sec,
+                      "steps": args.steps, "warmup": args.warmup,
+                      "config": {"workload": f"level {args.level} ({m['nCells']} cells) x {args.levels} levels, "
+                                             f"dt {args.dt:g} s, host NumPy state in and out"}}), flush=True)
+    return 0
+
+
 def main() -> int:
     if len(sys.argv) > 1 and sys.argv[1] == "--cpu-worker":
         level, n_levels, dt, steps, seed = sys.argv[2:7]
@@ -383,6 +413,8 @@ def main() -> int:
                          "(the roofline bytes stay the FP64 model's)")
     ap.add_argument("--layout-probe", action="store_true",
                     help="measure column streams at level pitch 55, 56 and 64 over the level's edge count instead")
+    ap.add_argument("--boundary", action="store_true",
+                    help="time atm_srk3 called with host-resident state and mesh dicts (PCIe both ways) instead")
     ap.add_argument("--vjp-steps", type=int, default=24,
                     help="large steps of the reverse-mode leg (forward + VJP of an area-weighted squared "
                          "theta perturbation, one checkpoint per large step; 24 is the differentiation "
@@ -411,6 +443,8 @@ def main() -> int:
         return ensemble_bench(args, device)
     if args.layout_probe:
         return layout_probe(args, lib, N)
+    if args.boundary:
+        return boundary_bench(args, device)
 
     m, st, np = prepare(args.level, args.levels, "quadratic", 0, args.dt, args.seed, device)
     nC, nE, nV, L = m["nCells"], m["nEdges"], m["nVertices"], args.levels

@@ -17,15 +17,14 @@
 
 namespace mpas {
 
-// Storage type of the edge- and vertex-located fields (u, ru, v, pv_edge,
-// rho_edge, tendencies of u, the acoustic edge fluxes, vorticity, the
-// transport fluxes, and the edge/vertex scratch).  binary64 in the plain
-// build; binary32 in the mixed-precision build (-DMPAS_MIXED=1), which halves
-// the bytes of the arrays that dominate the step's traffic.  Cell and
-// interface fields (the vertical-implicit coefficients and column solve, rho,
-// theta, exner, pressure, w) stay binary64 in both, and all arithmetic is
-// binary64 in registers: values are widened on load and rounded once on
-// store.
+// Storage type of the edge- and vertex-located fields whose rounding stays
+// below the mixed build's accuracy bar (v, pv_edge, rho_edge, vorticity, the
+// tendencies of u, cqu, the tracer fluxes, and the edge/vertex scratch).
+// binary64 in the plain build; binary32 in the mixed-precision build
+// (-DMPAS_MIXED=1), which halves their bytes.  u and the mass fluxes (ru,
+// ruAvg, the acoustic ru_p), and every cell and interface field (the
+// vertical-implicit coefficients and column solve, rho, theta, exner,
+// pressure, w), stay binary64 in both builds.
 #if defined(MPAS_MIXED) && MPAS_MIXED
 using hreal = float;
 #else

@@ -15,7 +15,13 @@ first or a later RK stage), then:
 * from ``--pmc FETCH_SIZE`` / ``WRITE_SIZE`` runs: the counters per variant
   (KiB per dispatch, as rocprofv3 reports them), and bytes after dividing by
   what the same counter reads per byte of the plain copy kernel ``k_copy`` of
-  the same run.
+  the same run;
+* with ``--counter NAME`` (repeatable): any other counter of a ``--pmc`` run,
+  its mean per dispatch and variant, uncalibrated, beside the same counter's
+  mean for ``k_copy`` per KiB copied;
+* with ``--rate NAME``: for every kernel of a ``--pmc`` run, the counter's
+  mean per dispatch divided by the dispatch's duration, per CU and clock
+  (256 CUs at 2.4 GHz), the kernels with the most counts first.
 
 rocprofv3's rocpd databases (``*_results.db``) and its CSV output are both read.
 
@@ -82,8 +88,12 @@ def counters(directory: str, kernel: str, counter: str) -> list[float]:
                                                   "from pmc_events")]
     rows += _rows(directory, "counter_collection.csv")
     rows = [r for r in rows if hit(r.get("Kernel_Name", "")) and r.get("Counter_Name") == counter]
-    rows.sort(key=lambda r: int(r.get("Dispatch_Id") or r.get("Correlation_Id") or 0))
-    return [float(r["Counter_Value"]) for r in rows]
+    # a block counter (SQ_*) comes as one record per hardware instance: sum a dispatch's records
+    per: dict[int, float] = {}
+    for r in rows:
+        d = int(r.get("Dispatch_Id") or r.get("Correlation_Id") or 0)
+        per[d] = per.get(d, 0.0) + float(r["Counter_Value"])
+    return [per[d] for d in sorted(per)]
 
 
 def copy_calibration(directory: str, counter: str) -> float | None:
@@ -91,9 +101,10 @@ def copy_calibration(directory: str, counter: str) -> float | None:
     (each thread reads and writes one double; copies of at least 1 Mi doubles), or None."""
     grid = {d: g for n, d, g in _db_rows(directory, "select name, dispatch_id, grid_x from kernels")
             if _matches("k_copy")(n)}
-    vals = {d: v for n, d, c, v in _db_rows(directory, "select name, dispatch_id, counter_name, counter_value "
-                                                        "from pmc_events")
-            if _matches("k_copy")(n) and c == counter}
+    vals: dict[int, float] = {}
+    for n, d, c, v in _db_rows(directory, "select name, dispatch_id, counter_name, counter_value from pmc_events"):
+        if _matches("k_copy")(n) and c == counter:
+            vals[d] = vals.get(d, 0.0) + float(v)
     ratios = sorted(vals[d] * 1024.0 / (grid[d] * 8.0) for d in vals if d in grid and grid[d] >= 1 << 20)
     return ratios[len(ratios) // 2] if ratios else None
 
@@ -103,6 +114,24 @@ def label(n: int, kernel: str) -> list[str]:
     if n % len(sched):
         raise ValueError(f"{n} dispatches of {kernel} are not whole large steps of {len(sched)} launches")
     return sched * (n // len(sched))
+
+
+def raw_counters(directory: str, kernel: str, names: list[str]) -> str:
+    """Mean per dispatch and variant of each named counter (as rocprofv3 reports it)."""
+    lines = ["| counter | variant | dispatches | mean per dispatch | k_copy per KiB copied |", "|---|---|---|---|---|"]
+    for counter in names:
+        vals = counters(directory, kernel, counter)
+        if not vals:
+            lines.append(f"| {counter} | - | 0 | - | - |")
+            continue
+        labels = label(len(vals), kernel)
+        rate = copy_calibration(directory, counter)
+        rs = "-" if rate is None else f"{rate:.4g}"
+        for v in sorted(set(labels)):
+            vs = [x for x, lv in zip(vals, labels) if lv == v]
+            lines.append(f"| {counter} | {v} | {len(vs)} | {sum(vs) / len(vs):.6g} | {rs} |")
+        lines.append(f"| {counter} | all | {len(vals)} | {sum(vals) / len(vals):.6g} | {rs} |")
+    return "\n".join(lines)
 
 
 def summarize(directory: str, kernel: str, level: int, levels: int, scalars: int) -> str:
@@ -144,6 +173,32 @@ def summarize(directory: str, kernel: str, level: int, levels: int, scalars: int
     return "\n\n".join(out)
 
 
+def rates(directory: str, counter: str, n: int, cus: int = 256, ghz: float = 2.4) -> str:
+    """Per kernel: mean count per dispatch, mean duration and count per CU per clock."""
+    kern = {d: (name, (end - start) * 1e-9, grid) for name, d, start, end, grid in
+            _db_rows(directory, "select name, dispatch_id, start, end, grid_x from kernels")}
+    per: dict[int, float] = {}
+    for name, d, c, v in _db_rows(directory, "select name, dispatch_id, counter_name, counter_value from pmc_events"):
+        if c == counter:
+            per[d] = per.get(d, 0.0) + float(v)
+    groups: dict[str, list[tuple[float, float]]] = {}
+    for d, v in per.items():
+        if d not in kern:
+            continue
+        name, sec, grid = kern[d]
+        short = re.sub(r"\(.*", "", name).replace("mpas::", "")
+        if short == "k_copy" and grid < 1 << 20:
+            continue  # small copies are launch-bound
+        groups.setdefault(short, []).append((v, sec))
+    lines = [f"| kernel | dispatches | {counter} per dispatch | mean us | per CU per clock |", "|---|---|---|---|---|"]
+    order = sorted(groups.items(), key=lambda kv: -sum(v for v, _ in kv[1]))
+    for short, vals in order[:n]:
+        v = sum(a for a, _ in vals) / len(vals)
+        t = sum(b for _, b in vals) / len(vals)
+        lines.append(f"| {short} | {len(vals)} | {v:.4g} | {t * 1e6:.1f} | {v / t / cus / (ghz * 1e9):.3f} |")
+    return "\n".join(lines)
+
+
 def top_kernels(directory: str, n: int) -> str:
     """rocprofv3's kernel statistics (the ``top_kernels`` view), the ``n`` largest by total time."""
     rows = _db_rows(directory, "select name, total_calls, total_duration, average, percentage from top_kernels")
@@ -164,9 +219,18 @@ def main(argv=None) -> int:
     ap.add_argument("--levels", type=int, default=55)
     ap.add_argument("--scalars", type=int, default=1)
     ap.add_argument("--top", type=int, default=0, help="also print the N largest kernels by total time")
+    ap.add_argument("--counter", action="append", default=[],
+                    help="also print this --pmc counter per dispatch and variant (repeatable)")
+    ap.add_argument("--rate", default="", help="print this --pmc counter per CU per clock for the top kernels")
     a = ap.parse_args(argv)
+    if a.rate:
+        print(rates(a.directory, a.rate, a.top or 20))
+        print()
     if a.kernel:
         print(summarize(a.directory, a.kernel, a.level, a.levels, a.scalars))
+    if a.kernel and a.counter:
+        print()
+        print(raw_counters(a.directory, a.kernel, a.counter))
     if a.top:
         print()
         print(top_kernels(a.directory, a.top))

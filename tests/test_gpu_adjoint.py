@@ -620,11 +620,11 @@ def test_kept_tape_matches_recompute():
     g_all = run_simulation.vjp(dict(state), st, objective, track_keys=keys, config=config, dt=meta["dt"],
                                nsteps=2, device=dev, checkpoint_timesteps=False)
     for k in keys:
-        a, b = g_ck[k].cpu().numpy(), g_all[k].cpu().numpy()
-        scale = max(float(np.max(np.abs(a))), 1e-300)
-        err = float(np.max(np.abs(a - b))) / scale
-        print(f"[adjoint] kept tape vs recompute d/d{k}: {err:.2e}", flush=True)
-        assert err <= 1e-12, k
+        # the recomputed tape is bitwise the kept one and every adjoint kernel is atomic-free,
+        # so the two gradients are the same bits
+        same = torch.equal(g_ck[k], g_all[k])
+        print(f"[adjoint] kept tape vs recompute d/d{k}: bitwise equal {same}", flush=True)
+        assert same, k
 
 
 @pytest.mark.parametrize("perturbed", [False, True])
