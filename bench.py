@@ -391,6 +391,8 @@ def main() -> int:
     ap.add_argument("--level", type=int, default=7)
     ap.add_argument("--levels", type=int, default=55)
     ap.add_argument("--dt", type=float, default=300.0)
+    ap.add_argument("--tracers", type=int, default=0,
+                    help="passive tracers beside the water-vapour slot (the level-6 drop-in workload has 1)")
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--roofline-kernel", default="k_acoustic_cells")
     ap.add_argument("--cpu-level", type=int, default=5)
@@ -446,7 +448,7 @@ def main() -> int:
     if args.boundary:
         return boundary_bench(args, device)
 
-    m, st, np = prepare(args.level, args.levels, "quadratic", 0, args.dt, args.seed, device)
+    m, st, np = prepare(args.level, args.levels, "quadratic", args.tracers, args.dt, args.seed, device)
     nC, nE, nV, L = m["nCells"], m["nEdges"], m["nVertices"], args.levels
     cfg = {"config_time_integration_order": 3, "config_len_disp": float(m["nominalMinDc"])}
     stepper = Srk3Stepper(st, st, cfg, args.dt, device=device, reorder=bool(args.reorder), precision=args.precision)

@@ -408,17 +408,51 @@ __device__ __forceinline__ void dyn_a3_cells(const mpas_hip_mesh &m, const mpas_
         S.g_delsq_vort[(long long)c * L + k] = gv;
     }
     if (c >= m.nCells) return;
-    const int j0 = inv.advStart[c], j1 = inv.advStart[c + 1];
+    // The cell's (edge, slot) entries, L at a time across the lanes: lane t
+    // loads entry t of the chunk and its two stencil coefficients, and the
+    // entry loop takes them back by readlane, so its iterations depend on no
+    // load but their own column gathers.  Lanes 0..L-1 are all present here.
+    const int j0 = inv.advStart[c], cnt = inv.advStart[c + 1] - j0;
+    const double fzm_k = m.fzm[k], fzp_k = m.fzp[k];
     double gt = 0.0, gw = 0.0;
-    for (int j = j0; j < j1; ++j) {
-        const int e = inv.advEdge[j], a = inv.advSlot[j];
-        const long long i = (long long)e * L + k;
-        const double c0 = m.adv_coefs[e * MA + a], c3 = m.adv_coefs_3rd[e * MA + a];
-        const double rut = F[ru][i];
-        gt += (c0 + sign_pm(rut) * c3) * rut * S.g_tf[i];
-        if (k >= 1) {
-            const double ruw = m.fzm[k] * rut + m.fzp[k] * F[ru][i - 1];
-            gw += (c0 + sign_pm(ruw) * c3) * ruw * S.g_wf[i];
+    for (int b0 = 0; b0 < cnt; b0 += L) {
+        const int t_l = b0 + k;
+        int e_l = 0;
+        double c0_l = 0.0, c3_l = 0.0;
+        if (t_l < cnt) {
+            e_l = inv.advEdge[j0 + t_l];
+            const int a_l = inv.advSlot[j0 + t_l];
+            c0_l = m.adv_coefs[e_l * MA + a_l];
+            c3_l = m.adv_coefs_3rd[e_l * MA + a_l];
+        }
+        const int nt = (cnt - b0 < L) ? cnt - b0 : L;
+        // four entries at a time: their gathers are all issued before the
+        // first is used
+        for (int t0 = 0; t0 < nt; t0 += 4) {
+            double rv[4], gtf[4], gwf[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                rv[u] = gtf[u] = gwf[u] = 0.0;
+                if (t0 + u < nt) {  // wave-uniform
+                    const long long i = (long long)bcast(e_l, t0 + u) * L + k;
+                    rv[u] = F[ru][i];
+                    gtf[u] = S.g_tf[i];
+                    gwf[u] = S.g_wf[i];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (t0 + u >= nt) break;
+                const double c0 = bcast(c0_l, t0 + u), c3 = bcast(c3_l, t0 + u);
+                const double rut = rv[u];
+                // ru at level k - 1 of the same edge column: lane k - 1's value
+                const double rut_m1 = __shfl_up(rut, 1);
+                gt += (c0 + sign_pm(rut) * c3) * rut * gtf[u];
+                if (k >= 1) {
+                    const double ruw = fzm_k * rut + fzp_k * rut_m1;
+                    gw += (c0 + sign_pm(ruw) * c3) * ruw * gwf[u];
+                }
+            }
         }
     }
     const long long o = (long long)c * L + k;

@@ -95,6 +95,14 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_acoustic_edges(long long nb, 
 // (tridiagonal_solve.py:50-63) are the serial recurrence of lane 0 over the
 // column in LDS; Rayleigh damping and the rho''/rtheta'' back-substitution
 // are level parallel again.
+//
+// The column index is wave-uniform (readfirstlane), so the cell's
+// connectivity and metrics are scalar loads.  Each lane loads its own level
+// and interface of the column's arrays once; the value at the level or
+// interface above or below comes from the neighbouring lane by a shuffle.
+// The flux loop loads the cell's own theta once and, per live edge, the
+// edge's ru_p and the other cell's theta; slots past the cell's edge count
+// load nothing.
 constexpr int kAcWaves = 4;
 
 __global__ __launch_bounds__(64 * kAcWaves) void k_acoustic_cells(mpas_hip_mesh m, AcousticParams P,
@@ -102,27 +110,43 @@ __global__ __launch_bounds__(64 * kAcWaves) void k_acoustic_cells(mpas_hip_mesh 
     __shared__ double s_rs[kAcWaves][64], s_ts[kAcWaves][64], s_rw[kAcWaves][65];
     __shared__ double s_a[kAcWaves][64], s_al[kAcWaves][64], s_g[kAcWaves][64];
     const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
-    const int wv = threadIdx.y, k = threadIdx.x;
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.y), k = threadIdx.x;
     const int c = blockIdx.x * kAcWaves + wv;
-    const bool live = c < nC;
+    const bool live = c < nC;  // wave-uniform
     const int oc = c * L, ow = c * Lw;
+    const int kl = k < L ? k : L - 1, kw = k < Lw ? k : L;  // clamped into the column
     const bool first = (P.small_step == 1);
     const double dts = P.dts, resm = P.resm, epssm = P.epssm;
-    const double *zz = m.zz;
+    // the cell's edges, the other cell of each and the flux metric, read
+    // before the kernel's first store so that they stay scalar loads
+    int n = 0;
+    double ia = 0.0;
+    int eo[kMaxEdges], nbr[kMaxEdges];
+    bool own2[kMaxEdges];
+    double met[kMaxEdges];
+    if (live) {
+        n = m.nEdgesOnCell[c];
+        ia = m.invAreaCell[c];
+#pragma unroll
+        for (int j = 0; j < kMaxEdges; ++j) {
+            const int jj = clamp_slot(j, n);
+            const int e = m.edgesOnCell[c * ME + jj];
+            const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
+            eo[j] = e;
+            own2[j] = (c2 == c);
+            nbr[j] = (c2 == c) ? c1 : c2;
+            met[j] = m.edgesOnCell_sign[c * ME + jj] * m.dvEdge[e];
+        }
+    }
     // old values (zero on the first small step, acoustic_step.py:1702-1761)
-    double rho_old = 0.0, rt_old = 0.0, rw_k = 0.0, rw_k1 = 0.0, ww_k = 0.0;
+    double rho_old = 0.0, rt_old = 0.0, rw_k = 0.0, cz0 = 0.0;
     if (live) {
         if (!first) {
-            if (k < L) {
-                rho_old = io.rho_pp[oc + k];
-                rt_old = io.rtheta_pp[oc + k];
-            }
-            if (k <= L) {
-                rw_k = io.rw_p[ow + k];
-                ww_k = io.wwAvg[ow + k];
-            }
-            if (k < L) rw_k1 = io.rw_p[ow + k + 1];
+            rho_old = io.rho_pp[oc + kl];
+            rt_old = io.rtheta_pp[oc + kl];
+            rw_k = io.rw_p[ow + kw];
         }
+        cz0 = in.coftz[ow + kw];
         if (k < L) {
             io.rtheta_pp_old[oc + k] = first ? 0.0 : rt_old;
             // tridiagonal factors of this column, staged for the serial sweeps
@@ -131,44 +155,57 @@ __global__ __launch_bounds__(64 * kAcWaves) void k_acoustic_cells(mpas_hip_mesh 
             s_g[wv][k] = in.gamma_tri[oc + k];
         }
     }
+    // the interface above (k + 1), every lane
+    const double rw_k1 = __shfl_down(rw_k, 1), cz1 = __shfl_down(cz0, 1);
     // phase A: horizontal flux sums, rs and ts (:456-511, :700-716)
     if (live && k < L) {
-        const int n = m.nEdgesOnCell[c];
+        const double tc = in.theta_m[oc + k];
+        double rp[kMaxEdges], tn[kMaxEdges];
+#pragma unroll
+        for (int j = 0; j < kMaxEdges; ++j) {
+            rp[j] = 0.0;
+            tn[j] = 0.0;
+            if (j < n) {  // wave-uniform
+                rp[j] = io.ru_p[eo[j] * L + k];
+                tn[j] = in.theta_m[nbr[j] * L + k];
+            }
+        }
         double rsum = 0.0, tsum = 0.0;
 #pragma unroll
         for (int j = 0; j < kMaxEdges; ++j) {
-            const int jj = clamp_slot(j, n);
-            const int e = m.edgesOnCell[c * ME + jj];
-            const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-            const double rp = io.ru_p[e * L + k];
-            const double t2 = in.theta_m[c2 * L + k], t1 = in.theta_m[c1 * L + k];
-            const double metric = m.edgesOnCell_sign[c * ME + jj] * m.dvEdge[e];
-            const double flux = dts * metric * rp * m.invAreaCell[c];
+            const double flux = dts * met[j] * rp[j] * ia;
+            const double t2 = own2[j] ? tc : tn[j], t1 = own2[j] ? tn[j] : tc;
             const double thedge = 0.5 * (t2 + t1);
             const double rcon = -flux, tcon = -(flux * thedge);
             rsum = (j < n) ? ((j == 0) ? rcon : rsum + rcon) : rsum;
             tsum = (j < n) ? ((j == 0) ? tcon : tsum + tcon) : tsum;
         }
         double rs = rho_old + dts * in.tend_rho[oc + k] + rsum - in.cofrz[k] * resm * (rw_k1 - rw_k);
-        double ts = rt_old + dts * in.tend_rt[oc + k] + tsum -
-                    resm * m.rdzw[k] * (in.coftz[ow + k + 1] * rw_k1 - in.coftz[ow + k] * rw_k);
+        double ts = rt_old + dts * in.tend_rt[oc + k] + tsum - resm * m.rdzw[k] * (cz1 * rw_k1 - cz0 * rw_k);
         s_rs[wv][k] = rs;
         s_ts[wv][k] = ts;
     }
     __syncthreads();
+    // the columns phases B and D read, and the level below (k - 1), every lane
+    double z0 = 0.0, cwt0 = 0.0, rz0 = 0.0, ww_k = 0.0;
+    if (live) {
+        if (!first) ww_k = io.wwAvg[ow + kw];
+        z0 = m.zz[oc + kl];
+        cwt0 = in.cofwt[oc + kl];
+        rz0 = in.rho_zz[oc + kl];
+    }
+    const double rt_km = __shfl_up(rt_old, 1), rho_km = __shfl_up(rho_old, 1);
+    const double z_m1 = __shfl_up(z0, 1), cwt_m1 = __shfl_up(cwt0, 1), rz_m1 = __shfl_up(rz0, 1);
     // phase B: wwAvg half-update and the implicit rw_p right-hand side (:719-755)
     if (live && k <= L) {
         if (k >= 1 && k < L) {
             ww_k = ww_k + 0.5 * (1.0 - epssm) * rw_k;
-            const double rt_km = first ? 0.0 : io.rtheta_pp[oc + k - 1];
-            const double rho_km = first ? 0.0 : io.rho_pp[oc + k - 1];
             const double tsk = s_ts[wv][k], tskm = s_ts[wv][k - 1];
             const double rsk = s_rs[wv][k], rskm = s_rs[wv][k - 1];
             double r = rw_k + dts * in.tend_rw[ow + k] -
-                       in.cofwz[oc + k] * ((zz[oc + k] * tsk - zz[oc + k - 1] * tskm) +
-                                           resm * (zz[oc + k] * rt_old - zz[oc + k - 1] * rt_km)) -
+                       in.cofwz[oc + k] * ((z0 * tsk - z_m1 * tskm) + resm * (z0 * rt_old - z_m1 * rt_km)) -
                        in.cofwr[oc + k] * ((rsk + rskm) + resm * (rho_old + rho_km)) +
-                       in.cofwt[oc + k] * (tsk + resm * rt_old) + in.cofwt[oc + k - 1] * (tskm + resm * rt_km);
+                       cwt0 * (tsk + resm * rt_old) + cwt_m1 * (tskm + resm * rt_km);
             s_rw[wv][k] = r;
         } else {
             s_rw[wv][k] = rw_k;  // boundary rows pass through the elimination
@@ -198,8 +235,8 @@ __global__ __launch_bounds__(64 * kAcWaves) void k_acoustic_cells(mpas_hip_mesh 
     if (live && k <= L) {
         rwn = s_rw[wv][k];
         if (k >= 1 && k < L) {
-            double zi = m.fzm[k] * zz[oc + k] + m.fzp[k] * zz[oc + k - 1];
-            double ri = m.fzm[k] * in.rho_zz[oc + k] + m.fzp[k] * in.rho_zz[oc + k - 1];
+            double zi = m.fzm[k] * z0 + m.fzp[k] * z_m1;
+            double ri = m.fzm[k] * rz0 + m.fzp[k] * rz_m1;
             double rwd = in.rw_save[ow + k] - in.rw[ow + k];
             double dss = m.dss[oc + k];
             rwn = (rwn + rwd - dts * dss * zi * ri * in.w[ow + k]) / (1.0 + dts * dss) - rwd;
@@ -215,8 +252,7 @@ __global__ __launch_bounds__(64 * kAcWaves) void k_acoustic_cells(mpas_hip_mesh 
     if (live && k < L) {
         double rw0 = s_rw[wv][k], rw1 = s_rw[wv][k + 1];
         io.rho_pp[oc + k] = s_rs[wv][k] - in.cofrz[k] * (rw1 - rw0);
-        io.rtheta_pp[oc + k] =
-            s_ts[wv][k] - m.rdzw[k] * (in.coftz[ow + k + 1] * rw1 - in.coftz[ow + k] * rw0);
+        io.rtheta_pp[oc + k] = s_ts[wv][k] - m.rdzw[k] * (cz1 * rw1 - cz0 * rw0);
     }
 }
 

@@ -91,31 +91,52 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
     const double s1 = edge_sign_in_cell(m, c1, e), s2 = edge_sign_in_cell(m, c2, e);
     const double *ru_ = F[ru] + (long long)e * L;
     const int na = m.nAdvCellsForEdge[e];
+    const int ne = m.nEdgesOnEdge[e];
     double g_ru_own = 0.0, g_ru_dn = 0.0;  // to ru(k) and ru(k-1)
     double x_ths = 0.0;
-    // the stencil gathers of each slot loop are all issued before the first
-    // is used (a loop that loads and accumulates slot by slot waits one full
-    // memory round trip per slot)
-    int acv[kMaxAdv];
-#pragma unroll
-    for (int a = 0; a < kMaxAdv; ++a) acv[a] = m.advCellsForEdge[e * MA + (a < na ? a : na - 1)];
+    // Slot rows across the wavefront: lane j loads slot j of the edge's
+    // advection and PV-flux rows (clamped to the live count) with one vector
+    // load each, and every slot loop takes its slot's value from lane j by
+    // readlane.  The stencil gathers of each slot loop are all issued before
+    // the first is used (a loop that loads and accumulates slot by slot waits
+    // one full memory round trip per slot).
+    const int sa = lane < na ? lane : (na > 0 ? na - 1 : 0);
+    const int acv_l = m.advCellsForEdge[e * MA + sa];
+    const double c0_l = m.adv_coefs[e * MA + sa], c3_l = m.adv_coefs_3rd[e * MA + sa];
+    const int so = lane < ne ? lane : (ne > 0 ? ne - 1 : 0);
+    const int eev = m.edgesOnEdge[e * ME2 + so];
+    const double wv_l = m.weightsOnEdge[e * ME2 + so];
+    // this edge's column at the neighbouring levels, from the neighbouring
+    // lanes (lanes past the column hold the top level and feed nothing used)
+    const double rut = ru_[k];
+    const double rut_m1 = __shfl_up(rut, 1);
+    const double *ue = F[u] + (long long)e * L;
+    const double u0 = ue[k];
+    const double um1 = __shfl_up(u0, 1), um2 = __shfl_up(u0, 2), up1 = __shfl_down(u0, 1);
+    const double gtu = bar_of(B, tend_u, i);
+    const double gtu_m1 = __shfl_up(gtu, 1);
     // ---- theta edge flux
     {
         const double gf = -(s1 * S.g_thadv[(long long)c1 * L + k] + s2 * S.g_thadv[(long long)c2 * L + k]);
-        const double rut = ru_[k], srt = sign_pm(rut);
-        double qv[kMaxAdv];
-#pragma unroll
-        for (int a = 0; a < kMaxAdv; ++a) {
-            qv[a] = 0.0;
-            if (a < na) qv[a] = F[theta_m][(long long)acv[a] * L + k];
-        }
+        const double srt = sign_pm(rut);
         double ft = 0.0;
 #pragma unroll
-        for (int a = 0; a < kMaxAdv; ++a) {
-            if (a >= na) break;
-            const double wgt = m.adv_coefs[e * MA + a] + srt * m.adv_coefs_3rd[e * MA + a];
-            const double cc = wgt * qv[a];
-            ft = (a == 0) ? cc : ft + cc;
+        for (int a0 = 0; a0 < kMaxAdv; a0 += 8) {
+            if (a0 >= na) break;
+            double qv[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                qv[t] = 0.0;
+                if (a0 + t < na) qv[t] = F[theta_m][(long long)bcast(acv_l, a0 + t) * L + k];
+            }
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int a = a0 + t;
+                if (a >= na) break;
+                const double wgt = bcast(c0_l, a) + srt * bcast(c3_l, a);
+                const double cc = wgt * qv[t];
+                ft = (a == 0) ? cc : ft + cc;
+            }
         }
         if (act) S.g_tf[i] = gf;
         g_ru_own += ft * gf;
@@ -124,96 +145,98 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
             const double dv = m.dvEdge[e];
             if (act) add_own(B, ru_save, i, dv * face * gf);
             g_ru_own += -dv * face * gf;
-            x_ths = 0.5 * (dv * (F[ru_save][i] - ru_[k]) * gf);
+            x_ths = 0.5 * (dv * (F[ru_save][i] - rut) * gf);
         }
     }
     // ---- w edge flux at interface k (1..L-1)
     if (k >= 1) {
         const double gf = -(s1 * S.g_hadv[(long long)c1 * Lw + k] + s2 * S.g_hadv[(long long)c2 * Lw + k]);
-        const double ruw = m.fzm[k] * ru_[k] + m.fzp[k] * ru_[k - 1];
+        const double ruw = m.fzm[k] * rut + m.fzp[k] * rut_m1;
         const double srw = sign_pm(ruw);
-        double qv[kMaxAdv];
-#pragma unroll
-        for (int a = 0; a < kMaxAdv; ++a) {
-            qv[a] = 0.0;
-            if (a < na) qv[a] = F[w][(long long)acv[a] * Lw + k];
-        }
         double fw = 0.0;
 #pragma unroll
-        for (int a = 0; a < kMaxAdv; ++a) {
-            if (a >= na) break;
-            const double wgt = m.adv_coefs[e * MA + a] + srw * m.adv_coefs_3rd[e * MA + a];
-            const double cc = wgt * qv[a];
-            fw = (a == 0) ? cc : fw + cc;
+        for (int a0 = 0; a0 < kMaxAdv; a0 += 8) {
+            if (a0 >= na) break;
+            double qv[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                qv[t] = 0.0;
+                if (a0 + t < na) qv[t] = F[w][(long long)bcast(acv_l, a0 + t) * Lw + k];
+            }
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int a = a0 + t;
+                if (a >= na) break;
+                const double wgt = bcast(c0_l, a) + srw * bcast(c3_l, a);
+                const double cc = wgt * qv[t];
+                fw = (a == 0) ? cc : fw + cc;
+            }
         }
         if (act) S.g_wf[i] = gf;
         g_ru_own += m.fzm[k] * fw * gf;
         g_ru_dn = m.fzp[k] * fw * gf;
     }
     // ---- u tendency
-    const double gtu = bar_of(B, tend_u, i);
-    const double *ue = F[u] + (long long)e * L;
     const double re = F[rho_edge][i];
     double g_re = 0.0, g_u_own = 0.0, g_pv_own = 0.0;
     double x_ke, x_hd;
     {
-        const int ne = m.nEdgesOnEdge[e];
         const double pvs = F[pv_edge][i];
-        double uv[kMaxEdges2], pv[kMaxEdges2];
+        const double gq = re * gtu;
+        double q = 0.0, gpe = 0.0;
 #pragma unroll
-        for (int j = 0; j < kMaxEdges2; ++j) {
-            uv[j] = pv[j] = 0.0;
-            if (j < ne) {
-                const long long io = (long long)m.edgesOnEdge[e * ME2 + j] * L + k;
-                uv[j] = F[u][io];
-                pv[j] = F[pv_edge][io];
+        for (int j0 = 0; j0 < kMaxEdges2; j0 += 8) {
+            if (j0 >= ne) break;
+            double uv[8], pv[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                uv[t] = pv[t] = 0.0;
+                if (j0 + t < ne) {
+                    const long long io = (long long)bcast(eev, j0 + t) * L + k;
+                    uv[t] = F[u][io];
+                    pv[t] = F[pv_edge][io];
+                }
             }
-        }
-        double q = 0.0;
 #pragma unroll
-        for (int j = 0; j < kMaxEdges2; ++j) {
-            if (j >= ne) break;
-            const double wv = m.weightsOnEdge[e * ME2 + j];
-            const double cc = wv * uv[j] * (0.5 * (pvs + pv[j]));
-            q = (j == 0) ? cc : q + cc;
+            for (int t = 0; t < 8; ++t) {
+                const int j = j0 + t;
+                if (j >= ne) break;
+                const double wv = bcast(wv_l, j);
+                const double cc = wv * uv[t] * (0.5 * (pvs + pv[t]));
+                q = (j == 0) ? cc : q + cc;
+                gpe += wv * uv[t] * 0.5 * gq;
+            }
         }
         const double ked = F[ke][(long long)c2 * L + k] - F[ke][(long long)c1 * L + k];
         const double hsum = F[h_divergence][(long long)c1 * L + k] + F[h_divergence][(long long)c2 * L + k];
         g_re += (q - ked * m.invDcEdge[e]) * gtu;
-        const double gq = re * gtu;
         x_ke = re * m.invDcEdge[e] * gtu;
         g_u_own += -0.5 * hsum * gtu;
         // Rayleigh damping of u on the top levels: tend_u -= rho_edge u damp(k)
         const int k0 = L - P.rayleigh_levels;
         if (P.rayleigh_levels > 0 && k >= k0) {
             const double damp = (((double)k + 1.0) - (double)k0) * P.rayleigh_coef_inverse;
-            g_re += -(ue[k] * damp) * gtu;
+            g_re += -(u0 * damp) * gtu;
             g_u_own += -(re * damp) * gtu;
         }
-        x_hd = -ue[k] * 0.5 * gtu;
-        double gpe = 0.0;
-#pragma unroll
-        for (int j = 0; j < kMaxEdges2; ++j) {
-            if (j >= ne) break;
-            gpe += m.weightsOnEdge[e * ME2 + j] * uv[j] * 0.5 * gq;
-        }
+        x_hd = -u0 * 0.5 * gtu;
         g_pv_own += gpe;
         if (act) S.g_q[i] = gq;
     }
     // ---- vertical flux of u at interface k: contributions to u(k-2..k+1)
     double c_m2 = 0.0, c_m1 = 0.0, c_p1 = 0.0, x_rw = 0.0;
     if (k >= 1) {
-        const double g = -m.rdzw[k - 1] * bar_of(B, tend_u, i - 1) + m.rdzw[k] * gtu;
+        const double g = -m.rdzw[k - 1] * gtu_m1 + m.rdzw[k] * gtu;
         const double erw = 0.5 * (F[rw][(long long)c1 * Lw + k] + F[rw][(long long)c2 * Lw + k]);
         double gerw;
         if (k == 1 || k == L - 1) {
-            const double ui = m.fzm[k] * ue[k] + m.fzp[k] * ue[k - 1];
+            const double ui = m.fzm[k] * u0 + m.fzp[k] * um1;
             gerw = ui * g;
             g_u_own += erw * m.fzm[k] * g;
             c_m1 = erw * m.fzp[k] * g;
         } else {
             double d[5];
-            flux3_partials(ue[k - 2], ue[k - 1], ue[k], ue[k + 1], erw, 1.0, d);
+            flux3_partials(um2, um1, u0, up1, erw, 1.0, d);
             c_m2 = d[0] * g;
             c_m1 = d[1] * g;
             g_u_own += d[2] * g;
