@@ -576,10 +576,32 @@ __device__ __forceinline__ void diag_g4(const mpas_hip_mesh &m, double dt, doubl
         gu += t * ((pc2 - pc1) * m.invDcEdge[e]);
     }
     if (rk_step <= 0 || rk_step == 3) {
-        const int j1 = inv.eoeStart[e + 1];
-        for (int j = inv.eoeStart[e]; j < j1; ++j) {
-            const int eo = inv.eoeEdge[j];
-            gu += m.weightsOnEdge[eo * ME2 + inv.eoeSlot[j]] * S.g_kee[(long long)eo * L + k];
+        // the (edge, slot) entries naming e, L at a time across the lanes
+        // (lanes 0..L-1 are all present), taken back by readlane; four
+        // entries' gathers are issued before the first is used
+        const int j0 = inv.eoeStart[e], cnt = inv.eoeStart[e + 1] - j0;
+        for (int b0 = 0; b0 < cnt; b0 += L) {
+            const int t_l = b0 + k;
+            int eo_l = 0;
+            double w_l = 0.0;
+            if (t_l < cnt) {
+                eo_l = inv.eoeEdge[j0 + t_l];
+                w_l = m.weightsOnEdge[eo_l * ME2 + inv.eoeSlot[j0 + t_l]];
+            }
+            const int nt = (cnt - b0 < L) ? cnt - b0 : L;
+            for (int t0 = 0; t0 < nt; t0 += 4) {
+                double gk[4];
+#pragma unroll
+                for (int u4 = 0; u4 < 4; ++u4) {
+                    gk[u4] = 0.0;
+                    if (t0 + u4 < nt) gk[u4] = S.g_kee[(long long)bcast(eo_l, t0 + u4) * L + k];  // wave-uniform
+                }
+#pragma unroll
+                for (int u4 = 0; u4 < 4; ++u4) {
+                    if (t0 + u4 >= nt) break;
+                    gu += bcast(w_l, t0 + u4) * gk[u4];
+                }
+            }
         }
     }
     for (int q = 0; q < 2; ++q) {

@@ -501,15 +501,40 @@ __device__ __forceinline__ void dyn_a3_edges(const mpas_hip_mesh &m, const mpas_
     const int L = m.nLevels, ME2 = m.maxEdges2, ME = m.maxEdges;
     if (e >= m.nEdges || k >= L) return;
     const long long i = (long long)e * L + k;
-    const int j0 = inv.eoeStart[e], j1 = inv.eoeStart[e + 1];
+    const int j0 = inv.eoeStart[e], cnt = inv.eoeStart[e + 1] - j0;
     const double ue = F[u][i], pve = F[pv_edge][i];
     double gu = 0.0, gp = 0.0;
-    for (int j = j0; j < j1; ++j) {
-        const int eo = inv.eoeEdge[j], sl = inv.eoeSlot[j];
-        const long long io = (long long)eo * L + k;
-        const double wq = m.weightsOnEdge[eo * ME2 + sl] * S.g_q[io];
-        gu += wq * 0.5 * (F[pv_edge][io] + pve);
-        gp += wq * ue * 0.5;
+    // the (edge, slot) entries naming e, L at a time across the lanes (lane t
+    // holds entry t of the chunk and its weight), taken back by readlane; four
+    // entries' gathers are issued before the first is used
+    for (int b0 = 0; b0 < cnt; b0 += L) {
+        const int t_l = b0 + k;
+        int eo_l = 0;
+        double w_l = 0.0;
+        if (t_l < cnt) {
+            eo_l = inv.eoeEdge[j0 + t_l];
+            w_l = m.weightsOnEdge[eo_l * ME2 + inv.eoeSlot[j0 + t_l]];
+        }
+        const int nt = (cnt - b0 < L) ? cnt - b0 : L;
+        for (int t0 = 0; t0 < nt; t0 += 4) {
+            double gq[4], pv[4];
+#pragma unroll
+            for (int u4 = 0; u4 < 4; ++u4) {
+                gq[u4] = pv[u4] = 0.0;
+                if (t0 + u4 < nt) {  // wave-uniform
+                    const long long io = (long long)bcast(eo_l, t0 + u4) * L + k;
+                    gq[u4] = S.g_q[io];
+                    pv[u4] = F[pv_edge][io];
+                }
+            }
+#pragma unroll
+            for (int u4 = 0; u4 < 4; ++u4) {
+                if (t0 + u4 >= nt) break;
+                const double wq = bcast(w_l, t0 + u4) * gq[u4];
+                gu += wq * 0.5 * (pv[u4] + pve);
+                gp += wq * ue * 0.5;
+            }
+        }
     }
     add_own(B, u, i, gu);
     add_own(B, pv_edge, i, gp);
