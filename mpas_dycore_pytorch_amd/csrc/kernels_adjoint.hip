@@ -330,7 +330,9 @@ __global__ __launch_bounds__(64 * kColWaves) void k_adj_acoustic_colw(mpas_hip_m
     const double *const *F = S_.f;
     double *const *B = S_.b;
     const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges, Lw = L + 1;
-    const int wv = threadIdx.y, k = threadIdx.x;
+    // the column index is wave-uniform, so the cell's connectivity and
+    // metrics are scalar loads
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.y), k = threadIdx.x;
     const int c = blockIdx.x * kColWaves + wv;
     const bool live = c < nC;
     const long long oc = (long long)c * L, ow = (long long)c * Lw;
@@ -353,22 +355,25 @@ __global__ __launch_bounds__(64 * kColWaves) void k_adj_acoustic_colw(mpas_hip_m
         s_fa[wv][k] = F[a_tri][oc + k];
         s_fal[wv][k] = F[alpha_tri][oc + k];
         s_fga[wv][k] = F[gamma_tri][oc + k];
-        // indices first, then every slot's loads, then the sums in slot order
-        int eo[kMaxEdges], c1v[kMaxEdges], c2v[kMaxEdges];
+        // indices first, then every slot's loads (the cell's own theta once,
+        // the other cell's per edge), then the sums in slot order
+        int eo[kMaxEdges], nbr[kMaxEdges];
+        bool own2[kMaxEdges];
 #pragma unroll
         for (int i = 0; i < kMaxEdges; ++i) {
             eo[i] = m.edgesOnCell[c * ME + clamp_slot(i, n)];
-            c1v[i] = m.cellsOnEdge[2 * eo[i]];
-            c2v[i] = m.cellsOnEdge[2 * eo[i] + 1];
+            const int c1 = m.cellsOnEdge[2 * eo[i]], c2 = m.cellsOnEdge[2 * eo[i] + 1];
+            own2[i] = (c2 == c);
+            nbr[i] = (c2 == c) ? c1 : c2;
         }
-        double rp[kMaxEdges], t1v[kMaxEdges], t2v[kMaxEdges];
+        const double tc = F[theta_m][oc + k];
+        double rp[kMaxEdges], tn[kMaxEdges];
 #pragma unroll
         for (int i = 0; i < kMaxEdges; ++i) {
-            rp[i] = t1v[i] = t2v[i] = 0.0;
-            if (i < n) {
+            rp[i] = tn[i] = 0.0;
+            if (i < n) {  // wave-uniform
                 rp[i] = F[ru_p][(long long)eo[i] * L + k];
-                t2v[i] = F[theta_m][(long long)c2v[i] * L + k];
-                t1v[i] = F[theta_m][(long long)c1v[i] * L + k];
+                tn[i] = F[theta_m][(long long)nbr[i] * L + k];
             }
         }
         double rsum = 0.0, tsum = 0.0;
@@ -377,7 +382,8 @@ __global__ __launch_bounds__(64 * kColWaves) void k_adj_acoustic_colw(mpas_hip_m
             if (i >= n) break;
             const double metric = m.edgesOnCell_sign[c * ME + i] * m.dvEdge[eo[i]];
             const double flux = dts * metric * rp[i] * ia;
-            const double thedge = 0.5 * (t2v[i] + t1v[i]);
+            const double t2 = own2[i] ? tc : tn[i], t1 = own2[i] ? tn[i] : tc;
+            const double thedge = 0.5 * (t2 + t1);
             const double rcon = -flux, tcon = -(flux * thedge);
             rsum = (i == 0) ? rcon : rsum + rcon;
             tsum = (i == 0) ? tcon : tsum + tcon;

@@ -63,7 +63,9 @@ __global__ __launch_bounds__(64 * kTanColWaves) void k_tan_vimp_colw(mpas_hip_me
     const double *const *F = S_.f;
     double *const *T = S_.b;
     const int L = m.nLevels, nC = m.nCells, Lw = L + 1;
-    const int wv = threadIdx.y, k = threadIdx.x;
+    // the column index is wave-uniform, so the cell's connectivity and
+    // metrics are scalar loads
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.y), k = threadIdx.x;
     const int c = blockIdx.x * W + wv;  // columns past the padding one do nothing but reach the barriers
     const long long o = (long long)c * L, ow = (long long)c * Lw;
     if (c == nC) {
@@ -160,7 +162,9 @@ __global__ __launch_bounds__(64 * kTanColWaves) void k_tan_acoustic_colw(mpas_hi
     const double *const *F = S_.f;
     double *const *T = S_.b;
     const int L = m.nLevels, Lw = L + 1, ME = m.maxEdges, nC = m.nCells;
-    const int wv = threadIdx.y, k = threadIdx.x;
+    // the column index is wave-uniform, so the cell's connectivity and
+    // metrics are scalar loads
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.y), k = threadIdx.x;
     const int c = blockIdx.x * W + wv;  // columns past the padding one do nothing but reach the barriers
     const long long oc = (long long)c * L, ow = (long long)c * Lw;
     if (c == nC) {
@@ -198,21 +202,67 @@ __global__ __launch_bounds__(64 * kTanColWaves) void k_tan_acoustic_colw(mpas_hi
     if (live && k < L) {
         const int n = m.nEdgesOnCell[c];
         const double ia = m.invAreaCell[c];
+        // indices first; then the values of every slot (the cell's own theta
+        // once, the other cell's per edge) and their sums in slot order; then
+        // the tangents the same way
+        int eo[kMaxEdges], nbr[kMaxEdges];
+        bool own2[kMaxEdges];
+#pragma unroll
+        for (int i = 0; i < kMaxEdges; ++i) {
+            eo[i] = m.edgesOnCell[c * ME + clamp_slot(i, n)];
+            const int c1 = m.cellsOnEdge[2 * eo[i]], c2 = m.cellsOnEdge[2 * eo[i] + 1];
+            own2[i] = (c2 == c);
+            nbr[i] = (c2 == c) ? c1 : c2;
+        }
         double rsum = 0.0, tsum = 0.0, drsum = 0.0, dtsum = 0.0;
-        for (int i = 0; i < n; ++i) {
-            const int e = m.edgesOnCell[c * ME + i];
-            const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-            const double metric = m.edgesOnCell_sign[c * ME + i] * m.dvEdge[e];
-            const double flux = dts * metric * F[ru_p][(long long)e * L + k] * ia;
-            const double dflux = dts * metric * dot_of(T, ru_p, (long long)e * L + k) * ia;
-            const double thedge = 0.5 * (F[theta_m][(long long)c2 * L + k] + F[theta_m][(long long)c1 * L + k]);
-            const double dthedge =
-                0.5 * (dot_of(T, theta_m, (long long)c2 * L + k) + dot_of(T, theta_m, (long long)c1 * L + k));
-            const double rcon = -flux, tcon = -(flux * thedge);
-            rsum = (i == 0) ? rcon : rsum + rcon;
-            tsum = (i == 0) ? tcon : tsum + tcon;
-            drsum += -dflux;
-            dtsum += -(dflux * thedge + flux * dthedge);
+        double fl[kMaxEdges], th[kMaxEdges];
+        {
+            const double tc = F[theta_m][oc + k];
+            double rp[kMaxEdges], tn[kMaxEdges];
+#pragma unroll
+            for (int i = 0; i < kMaxEdges; ++i) {
+                rp[i] = tn[i] = 0.0;
+                if (i < n) {  // wave-uniform
+                    rp[i] = F[ru_p][(long long)eo[i] * L + k];
+                    tn[i] = F[theta_m][(long long)nbr[i] * L + k];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < kMaxEdges; ++i) {
+                fl[i] = th[i] = 0.0;
+                if (i >= n) break;
+                const double metric = m.edgesOnCell_sign[c * ME + i] * m.dvEdge[eo[i]];
+                const double flux = dts * metric * rp[i] * ia;
+                const double t2 = own2[i] ? tc : tn[i], t1 = own2[i] ? tn[i] : tc;
+                const double thedge = 0.5 * (t2 + t1);
+                const double rcon = -flux, tcon = -(flux * thedge);
+                rsum = (i == 0) ? rcon : rsum + rcon;
+                tsum = (i == 0) ? tcon : tsum + tcon;
+                fl[i] = flux;
+                th[i] = thedge;
+            }
+        }
+        {
+            const double dtc = dot_of(T, theta_m, oc + k);
+            double drp[kMaxEdges], dtn[kMaxEdges];
+#pragma unroll
+            for (int i = 0; i < kMaxEdges; ++i) {
+                drp[i] = dtn[i] = 0.0;
+                if (i < n) {  // wave-uniform
+                    drp[i] = dot_of(T, ru_p, (long long)eo[i] * L + k);
+                    dtn[i] = dot_of(T, theta_m, (long long)nbr[i] * L + k);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < kMaxEdges; ++i) {
+                if (i >= n) break;
+                const double metric = m.edgesOnCell_sign[c * ME + i] * m.dvEdge[eo[i]];
+                const double dflux = dts * metric * drp[i] * ia;
+                const double dt2 = own2[i] ? dtc : dtn[i], dt1 = own2[i] ? dtn[i] : dtc;
+                const double dthedge = 0.5 * (dt2 + dt1);
+                drsum += -dflux;
+                dtsum += -(dflux * th[i] + fl[i] * dthedge);
+            }
         }
         s_rs[wv][k] = RPP(k) + dts * F[tend_rho][oc + k] + rsum - cofrz_[k] * resm * (RWP(k + 1) - RWP(k));
         s_ts[wv][k] =

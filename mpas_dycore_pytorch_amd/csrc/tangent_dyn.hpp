@@ -185,18 +185,35 @@ __device__ __forceinline__ void dyn_j4(const mpas_hip_mesh &m, const mpas_hip_dy
         else ddelta = dyn_dwduz(m, F, T, e, c1, c2, k + 1) - dyn_dwduz(m, F, T, e, c1, c2, k);
     }
     const double dvert = -m.rdzw[k] * ddelta;
-    // PV flux q
+    // PV flux q; four slots' gathers are issued before the first is used
     double q = 0.0, dq = 0.0;
     const int ne = m.nEdgesOnEdge[e];
     const double pve = F[pv_edge][i], dpve = dot_of(T, pv_edge, i);
-    for (int j = 0; j < ne; ++j) {
-        const int ee = m.edgesOnEdge[e * ME2 + j];
-        const long long ie = (long long)ee * L + k;
-        const double wgt = m.weightsOnEdge[e * ME2 + j];
-        const double workpv = 0.5 * (pve + F[pv_edge][ie]);
-        const double cc = wgt * F[u][ie] * workpv;
-        q = (j == 0) ? cc : q + cc;
-        dq += wgt * (dot_of(T, u, ie) * workpv + F[u][ie] * 0.5 * (dpve + dot_of(T, pv_edge, ie)));
+#pragma unroll
+    for (int j0 = 0; j0 < kMaxEdges2; j0 += 4) {
+        if (j0 >= ne) break;
+        double uj[4], pj[4], duj[4], dpj[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            uj[t] = pj[t] = duj[t] = dpj[t] = 0.0;
+            if (j0 + t < ne) {  // wave-uniform
+                const long long ie = (long long)m.edgesOnEdge[e * ME2 + j0 + t] * L + k;
+                uj[t] = F[u][ie];
+                pj[t] = F[pv_edge][ie];
+                duj[t] = dot_of(T, u, ie);
+                dpj[t] = dot_of(T, pv_edge, ie);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int j = j0 + t;
+            if (j >= ne) break;
+            const double wgt = m.weightsOnEdge[e * ME2 + j];
+            const double workpv = 0.5 * (pve + pj[t]);
+            const double cc = wgt * uj[t] * workpv;
+            q = (j == 0) ? cc : q + cc;
+            dq += wgt * (duj[t] * workpv + uj[t] * 0.5 * (dpve + dpj[t]));
+        }
     }
     const double ked = F[ke][o2] - F[ke][o1], dked = dot_of(T, ke, o2) - dot_of(T, ke, o1);
     const double hsum = F[h_divergence][o1] + F[h_divergence][o2];
@@ -233,7 +250,6 @@ __device__ __forceinline__ void dyn_j4(const mpas_hip_mesh &m, const mpas_hip_dy
     } else {
         dtue = dot_of(T, tend_u_euler_in, i);
     }
-    dot_put(T, tend_u_euler, i, dtue);
     // Rayleigh damping of u on the top levels: tend_u -= rho_edge u damp(k)
     double dray = 0.0;
     const int k0 = L - P.rayleigh_levels;
@@ -241,36 +257,73 @@ __device__ __forceinline__ void dyn_j4(const mpas_hip_mesh &m, const mpas_hip_dy
         const double damp = (((double)k + 1.0) - (double)k0) * P.rayleigh_coef_inverse;
         dray = (dre * ue + re * due) * damp;
     }
-    dot_put(T, tend_u, i, dvert + dbase - dray + dtue);
-    // per-edge advective fluxes: theta at level k, w at interface k
+    // per-edge advective fluxes: theta at level k, w at interface k (the
+    // stores of this thread all come last, so the stencil rows stay scalar
+    // loads; four slots' gathers are issued before the first is used)
     const int na = m.nAdvCellsForEdge[e];
+    const double ru0 = F[ru][i], dru0 = dot_of(T, ru, i);
+    const double ru_m1 = __shfl_up(ru0, 1), dru_m1 = __shfl_up(dru0, 1);  // level k - 1, lane k - 1
+    double g_tf, g_wf = 0.0;
     {
-        const double rue = F[ru][i], drue = dot_of(T, ru, i);
-        const double sr = sign_pm(rue);
+        const double sr = sign_pm(ru0);
         double fs = 0.0, dfs = 0.0;
-        for (int a = 0; a < na; ++a) {
-            const int ac = m.advCellsForEdge[e * MA + a];
-            const double wgt = m.adv_coefs[e * MA + a] + sr * m.adv_coefs_3rd[e * MA + a];
-            fs += wgt * F[theta_m][(long long)ac * L + k];
-            dfs += wgt * dot_of(T, theta_m, (long long)ac * L + k);
+#pragma unroll
+        for (int a0 = 0; a0 < kMaxAdv; a0 += 4) {
+            if (a0 >= na) break;
+            double qv[4], dqv[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                qv[t] = dqv[t] = 0.0;
+                if (a0 + t < na) {  // wave-uniform
+                    const long long ia = (long long)m.advCellsForEdge[e * MA + a0 + t] * L + k;
+                    qv[t] = F[theta_m][ia];
+                    dqv[t] = dot_of(T, theta_m, ia);
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int a = a0 + t;
+                if (a >= na) break;
+                const double wgt = m.adv_coefs[e * MA + a] + sr * m.adv_coefs_3rd[e * MA + a];
+                fs += wgt * qv[t];
+                dfs += wgt * dqv[t];
+            }
         }
-        S.g_tf[i] = drue * fs + rue * dfs;
+        g_tf = dru0 * fs + ru0 * dfs;
     }
     if (k >= 1) {
-        const double rue = m.fzm[k] * F[ru][i] + m.fzp[k] * F[ru][i - 1];
-        const double drue = m.fzm[k] * dot_of(T, ru, i) + m.fzp[k] * dot_of(T, ru, i - 1);
+        const double rue = m.fzm[k] * ru0 + m.fzp[k] * ru_m1;
+        const double drue = m.fzm[k] * dru0 + m.fzp[k] * dru_m1;
         const double sr = sign_pm(rue);
         double fs = 0.0, dfs = 0.0;
-        for (int a = 0; a < na; ++a) {
-            const int ac = m.advCellsForEdge[e * MA + a];
-            const double wgt = m.adv_coefs[e * MA + a] + sr * m.adv_coefs_3rd[e * MA + a];
-            fs += wgt * F[w][(long long)ac * Lw + k];
-            dfs += wgt * dot_of(T, w, (long long)ac * Lw + k);
+#pragma unroll
+        for (int a0 = 0; a0 < kMaxAdv; a0 += 4) {
+            if (a0 >= na) break;
+            double qv[4], dqv[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                qv[t] = dqv[t] = 0.0;
+                if (a0 + t < na) {  // wave-uniform
+                    const long long ia = (long long)m.advCellsForEdge[e * MA + a0 + t] * Lw + k;
+                    qv[t] = F[w][ia];
+                    dqv[t] = dot_of(T, w, ia);
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int a = a0 + t;
+                if (a >= na) break;
+                const double wgt = m.adv_coefs[e * MA + a] + sr * m.adv_coefs_3rd[e * MA + a];
+                fs += wgt * qv[t];
+                dfs += wgt * dqv[t];
+            }
         }
-        S.g_wf[i] = drue * fs + rue * dfs;
-    } else {
-        S.g_wf[i] = 0.0;
+        g_wf = drue * fs + rue * dfs;
     }
+    dot_put(T, tend_u_euler, i, dtue);
+    dot_put(T, tend_u, i, dvert + dbase - dray + dtue);
+    S.g_tf[i] = g_tf;
+    S.g_wf[i] = g_wf;
 }
 
 // ---------------------------------------------------------------- J5
