@@ -151,25 +151,51 @@ __device__ __forceinline__ void divdamp_cells_vjp(const mpas_hip_mesh &m, double
     const int L = m.nLevels, ME = m.maxEdges;
     if (c >= m.nCells || k >= L) return;
     const int n = m.nEdgesOnCell[c];
+    // indices first, then every slot's loads (the cell's own columns once,
+    // the other cell's per edge), then the sums in slot order
+    int eo[kMaxEdges], nbr[kMaxEdges];
+    bool own2[kMaxEdges];  // this cell is the edge's second cell
+#pragma unroll
+    for (int j = 0; j < kMaxEdges; ++j) {
+        eo[j] = m.edgesOnCell[c * ME + clamp_slot(j, n)];
+        const int c1 = m.cellsOnEdge[2 * eo[j]], c2 = m.cellsOnEdge[2 * eo[j] + 1];
+        own2[j] = (c2 == c);
+        nbr[j] = (c2 == c) ? c1 : c2;
+    }
+    const long long o = (long long)c * L + k;
+    const double rtc = F[rtheta_pp][o], roc = F[rtheta_pp_old][o], thc = F[theta_m][o];
+    double gv[kMaxEdges], rtn[kMaxEdges], ron[kMaxEdges], thn[kMaxEdges];
+#pragma unroll
+    for (int j = 0; j < kMaxEdges; ++j) {
+        gv[j] = rtn[j] = ron[j] = thn[j] = 0.0;
+        if (j < n) {  // wave-uniform
+            const long long on = (long long)nbr[j] * L + k;
+            gv[j] = bar_of(B, ru_p_out, (long long)eo[j] * L + k);
+            rtn[j] = F[rtheta_pp][on];
+            ron[j] = F[rtheta_pp_old][on];
+            thn[j] = F[theta_m][on];
+        }
+    }
     double grt = 0.0, gold = 0.0, gth = 0.0;
-    for (int j = 0; j < n; ++j) {
-        const int e = m.edgesOnCell[c * ME + j];
-        const double g = bar_of(B, ru_p_out, (long long)e * L + k);
+#pragma unroll
+    for (int j = 0; j < kMaxEdges; ++j) {
+        if (j >= n) break;
+        const double g = gv[j];
         if (g == 0.0) continue;
-        const int c1 = m.cellsOnEdge[2 * e], c2 = m.cellsOnEdge[2 * e + 1];
-        const long long o1 = (long long)c1 * L + k, o2 = (long long)c2 * L + k;
-        const double d1 = -(F[rtheta_pp][o1] - F[rtheta_pp_old][o1]);
-        const double d2 = -(F[rtheta_pp][o2] - F[rtheta_pp_old][o2]);
-        const double ts = F[theta_m][o1] + F[theta_m][o2];
+        const double rt1 = own2[j] ? rtn[j] : rtc, rt2 = own2[j] ? rtc : rtn[j];
+        const double ro1 = own2[j] ? ron[j] : roc, ro2 = own2[j] ? roc : ron[j];
+        const double th1 = own2[j] ? thn[j] : thc, th2 = own2[j] ? thc : thn[j];
+        const double d1 = -(rt1 - ro1);
+        const double d2 = -(rt2 - ro2);
+        const double ts = th1 + th2;
         if (!(fabs(ts) >= 1.0e-30)) continue;
         const double gd = coef * g / ts;  // adjoint of (d2 - d1)
         const double gts = -(coef * (d2 - d1) / ts) / ts * g;
-        const double sg = (c == c2) ? -1.0 : 1.0;
+        const double sg = own2[j] ? -1.0 : 1.0;
         grt += sg * gd;
         gold -= sg * gd;
         gth += gts;
     }
-    const long long o = (long long)c * L + k;
     add_own(B, rtheta_pp, o, grt);
     add_own(B, rtheta_pp_old, o, gold);
     add_own(B, theta_m, o, gth);
@@ -446,11 +472,23 @@ __device__ __forceinline__ double diag_pv_cell(const mpas_hip_mesh &m, Fwd F, in
     using namespace diag;
     const int L = m.nLevels, ME = m.maxEdges;
     const int n = m.nEdgesOnCell[c];
+    // the vertices first, then every slot's vorticity, then the sum in slot order
+    int vxs[kMaxEdges];
+#pragma unroll
+    for (int i = 0; i < kMaxEdges; ++i) vxs[i] = m.verticesOnCell[c * ME + clamp_slot(i, n)];
+    double zv[kMaxEdges];
+#pragma unroll
+    for (int i = 0; i < kMaxEdges; ++i) {
+        zv[i] = 0.0;
+        if (i < n) zv[i] = F[vorticity][(long long)vxs[i] * L + k];  // wave-uniform
+    }
     double pc = 0.0;
-    for (int i = 0; i < n; ++i) {
-        const int vx = m.verticesOnCell[c * ME + i];
+#pragma unroll
+    for (int i = 0; i < kMaxEdges; ++i) {
+        if (i >= n) break;
+        const int vx = vxs[i];
         const double ka = m.kiteAreasOnVertex[3 * vx + m.kiteForCell[c * ME + i]];
-        const double pvv = m.fVertex[vx] + F[vorticity][(long long)vx * L + k];
+        const double pvv = m.fVertex[vx] + zv[i];
         const double cp = ka * pvv * m.invAreaCell[c];
         pc = (i == 0) ? cp : pc + cp;
     }
@@ -502,19 +540,38 @@ __device__ __forceinline__ void diag_g2(const mpas_hip_mesh &m, double dt, doubl
     const bool on = apvm > 0.0;
     const int n = m.nEdgesOnCell[c];
     // pv_cell of this cell, read by the edge pass for both cells of each edge
-    if (on) S.pvc[(long long)c * L + k] = diag_pv_cell(m, F, c, k);
+    // (formed here, stored with the other results at the end)
+    const double pvc = on ? diag_pv_cell(m, F, c, k) : 0.0;
+    // the edges first, then every slot's loads, then the sums in slot order
+    int eo[kMaxEdges];
+#pragma unroll
+    for (int j = 0; j < kMaxEdges; ++j) eo[j] = m.edgesOnCell[c * ME + clamp_slot(j, n)];
+    double gpe[kMaxEdges], uv[kMaxEdges], ghe[kMaxEdges];
+#pragma unroll
+    for (int j = 0; j < kMaxEdges; ++j) {
+        gpe[j] = uv[j] = ghe[j] = 0.0;
+        if (j < n) {  // wave-uniform
+            const long long ie = (long long)eo[j] * L + k;
+            gpe[j] = bar_of(B, pv_edge, ie);
+            uv[j] = F[u][ie];
+            ghe[j] = bar_of(B, h_edge, ie);
+        }
+    }
     double gpc = 0.0, gh = 0.0;
-    for (int j = 0; j < n; ++j) {
-        const int e = m.edgesOnCell[c * ME + j];
-        const long long ie = (long long)e * L + k;
-        const double t = diag_t(apvm_dt, on, B, ie);
+#pragma unroll
+    for (int j = 0; j < kMaxEdges; ++j) {
+        if (j >= n) break;
+        const int e = eo[j];
+        // t = apvm dt (-g_pv_edge), as diag_t forms it
+        const double t = (on && gpe[j] != 0.0) ? apvm_dt * (-gpe[j]) : 0.0;
         if (t != 0.0) {
-            const double ggn = t * F[u][ie] * m.invDcEdge[e];
+            const double ggn = t * uv[j] * m.invDcEdge[e];
             gpc += (m.cellsOnEdge[2 * e + 1] == c) ? ggn : -ggn;
         }
-        gh += 0.5 * bar_of(B, h_edge, ie);
+        gh += 0.5 * ghe[j];
     }
     const long long o = (long long)c * L + k;
+    if (on) S.pvc[o] = pvc;
     S.g_pvc[o] = gpc;
     add_own(B, h, o, gh);
 }
@@ -527,35 +584,81 @@ __device__ __forceinline__ void diag_g3(const mpas_hip_mesh &m, double dt, doubl
     const bool on = apvm > 0.0;
     const long long o = (long long)vx * L + k;
     const double ke_fact = 1.0 - 0.375;
-    double gz = bar_of(B, vorticity, o), gkv = 0.0;
+    // The vertex's edges and the cells of each edge, each cell taken at its
+    // first naming (the three edges name each of the three cells twice); then
+    // every load (edge adjoints, each cell's kite area by its slot for the
+    // vertex, its pv_cell and KE adjoints); then the sums in the order
+    // edge 0, its new cells, edge 1, ...
+    int ev[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) ev[j] = m.edgesOnVertex[3 * vx + j];
+    int cj[3][2];
+    bool pr[3][2];
     int seen0 = -1, seen1 = -1, seen2 = -1;
+#pragma unroll
     for (int j = 0; j < 3; ++j) {
-        const int e = m.edgesOnVertex[3 * vx + j];
-        const long long ie = (long long)e * L + k;
-        gz += 0.5 * bar_of(B, pv_edge, ie);
-        const double t = diag_t(apvm_dt, on, B, ie);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int c = m.cellsOnEdge[2 * ev[j] + q];
+            cj[j][q] = c;
+            pr[j][q] = !(c == seen0 || c == seen1 || c == seen2);
+            if (pr[j][q]) {
+                if (seen0 < 0) seen0 = c;
+                else if (seen1 < 0) seen1 = c;
+                else seen2 = c;
+            }
+        }
+    }
+    double gpe[3], vv[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const long long ie = (long long)ev[j] * L + k;
+        gpe[j] = bar_of(B, pv_edge, ie);
+        vv[j] = F[v][ie];
+    }
+    double kav[3][2], iav[3][2], gpc[3][2], gke[3][2];
+    bool hit[3][2];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            kav[j][q] = iav[j][q] = gpc[j][q] = gke[j][q] = 0.0;
+            hit[j][q] = false;
+            if (!pr[j][q]) continue;  // wave-uniform
+            const int c = cj[j][q];
+            const int n = m.nEdgesOnCell[c];
+            int h = -1;  // the cell's first slot naming the vertex
+#pragma unroll
+            for (int i = kMaxEdges - 1; i >= 0; --i) {
+                const int vc = m.verticesOnCell[c * ME + clamp_slot(i, n)];
+                if (i < n && vc == vx) h = i;
+            }
+            if (h < 0) continue;
+            const double ka = m.kiteAreasOnVertex[3 * vx + m.kiteForCell[c * ME + h]];
+            const long long oc = (long long)c * L + k;
+            kav[j][q] = ka;
+            iav[j][q] = m.invAreaCell[c];
+            gpc[j][q] = S.g_pvc[oc];
+            gke[j][q] = bar_of(B, ke, oc);
+            hit[j][q] = true;
+        }
+    }
+    double gz = bar_of(B, vorticity, o), gkv = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int e = ev[j];
+        gz += 0.5 * gpe[j];
+        // t = apvm dt (-g_pv_edge), as diag_t forms it
+        const double t = (on && gpe[j] != 0.0) ? apvm_dt * (-gpe[j]) : 0.0;
         if (t != 0.0) {
-            const double ggt = t * F[v][ie] * m.invDvEdge[e];
+            const double ggt = t * vv[j] * m.invDvEdge[e];
             gz += (m.verticesOnEdge[2 * e + 1] == vx) ? ggt : -ggt;
         }
-        // the kite terms of the cells around the vertex, each cell once (the
-        // three edges name each of the three cells twice)
+#pragma unroll
         for (int q = 0; q < 2; ++q) {
-            const int c = m.cellsOnEdge[2 * e + q];
-            if (c == seen0 || c == seen1 || c == seen2) continue;
-            if (seen0 < 0) seen0 = c;
-            else if (seen1 < 0) seen1 = c;
-            else seen2 = c;
-            const int n = m.nEdgesOnCell[c];
-            for (int i = 0; i < n; ++i) {
-                if (m.verticesOnCell[c * ME + i] != vx) continue;
-                const double ka = m.kiteAreasOnVertex[3 * vx + m.kiteForCell[c * ME + i]];
-                const long long oc = (long long)c * L + k;
-                const double ia = m.invAreaCell[c];
-                gz += ka * ia * S.g_pvc[oc];
-                gkv += (1.0 - ke_fact) * ka * ia * bar_of(B, ke, oc);
-                break;
-            }
+            if (!hit[j][q]) continue;
+            gz += kav[j][q] * iav[j][q] * gpc[j][q];
+            gkv += (1.0 - ke_fact) * kav[j][q] * iav[j][q] * gke[j][q];
         }
     }
     S.g_pvv[o] = gz;
