@@ -80,6 +80,16 @@ typedef struct mpas_hip_config {
     int32_t order;       /* [config_time_integration_order] 2 or 3 (0 is read as 3) */
     int32_t rayleigh_levels;      /* [config_number_rayleigh_damp_u_levels] when [config_rayleigh_damp_u], else 0 */
     double rayleigh_coef_inverse; /* 1 / (rayleigh_levels * [config_rayleigh_damp_u_timescale_days] * 86400) */
+    /* [config_horiz_mixing]: 0 "2d_smagorinsky", 1 "2d_fixed" (dyn_tend.py:3367-3430); the three
+     * fixed coefficients are read only in "2d_fixed" */
+    int32_t mix_fixed;
+    double h_theta_eddy_visc2;    /* [config_h_theta_eddy_visc2]: kdiff of every cell and level */
+    double h_mom_eddy_visc4;      /* [config_h_mom_eddy_visc4] */
+    double h_theta_eddy_visc4;    /* [config_h_theta_eddy_visc4] */
+    /* CAM damping (dyn_tend.py:3434-3458): on when [config_mpas_cam_coef] > 0 and
+     * [config_number_cam_damping_levels] > 0, else cam_levels = 0 */
+    int32_t cam_levels;           /* [config_number_cam_damping_levels] */
+    double cam_visc2;             /* 4 * 2.0833 * len_disp * [config_mpas_cam_coef] */
 } mpas_hip_config;
 
 /* Number of doubles of workspace needed for one large step. */
@@ -133,6 +143,13 @@ typedef struct mpas_hip_dyn_params {
      * tend_u -= rho_edge u ((k + 1) - k0) rayleigh_coef_inverse for k >= k0 = nLevels - rayleigh_levels */
     int32_t rayleigh_levels;
     double rayleigh_coef_inverse;
+    /* kdiff of the first RK stage (dyn_tend.py:3367-3458): the Smagorinsky value
+     * min(kdiff_coef def, kdiff_cap), or kdiff_fixed everywhere when mix_fixed; then on the
+     * top cam_levels levels max(kdiff, cam_visc2 (1 - (nLevels - (k + 1)) / cam_levels)) */
+    int32_t mix_fixed;
+    double kdiff_fixed;
+    int32_t cam_levels;
+    double cam_visc2;
 } mpas_hip_dyn_params;
 
 /* Scalars of atm_advance_acoustic_step_work (acoustic_step.py:1297-1340). */

@@ -60,14 +60,18 @@ class ConfigStruct(C.Structure):
                 ("del4u_div_factor", C.c_double), ("ns", C.c_int32), ("split", C.c_int32),
                 ("monotonic", C.c_int32), ("nScalars", C.c_int32), ("moist_start", C.c_int32),
                 ("moist_end", C.c_int32), ("diagnostics_current", C.c_int32), ("order", C.c_int32),
-                ("rayleigh_levels", C.c_int32), ("rayleigh_coef_inverse", C.c_double)]
+                ("rayleigh_levels", C.c_int32), ("rayleigh_coef_inverse", C.c_double),
+                ("mix_fixed", C.c_int32), ("h_theta_eddy_visc2", C.c_double), ("h_mom_eddy_visc4", C.c_double),
+                ("h_theta_eddy_visc4", C.c_double), ("cam_levels", C.c_int32), ("cam_visc2", C.c_double)]
 
 
 class DynParamsStruct(C.Structure):
     _fields_ = [("rk_step", C.c_int32), ("dt", C.c_double), ("coef_3rd_order", C.c_double),
                 ("kdiff_coef", C.c_double), ("kdiff_cap", C.c_double), ("h_mom_eddy_visc4", C.c_double),
                 ("h_theta_eddy_visc4", C.c_double), ("del4u_div_factor", C.c_double),
-                ("prandtl_inv", C.c_double), ("rayleigh_levels", C.c_int32), ("rayleigh_coef_inverse", C.c_double)]
+                ("prandtl_inv", C.c_double), ("rayleigh_levels", C.c_int32), ("rayleigh_coef_inverse", C.c_double),
+                ("mix_fixed", C.c_int32), ("kdiff_fixed", C.c_double), ("cam_levels", C.c_int32),
+                ("cam_visc2", C.c_double)]
 
 
 class AcousticParamsStruct(C.Structure):

@@ -381,7 +381,17 @@ def taped_step(dm: DeviceMesh, state: Mapping[str, torch.Tensor], cfg: Any, dt: 
     ru_save, rw_save = st["ru"], st["rw"]
     rtheta_p_save, rho_p_save = st["rtheta_p"], st["rho_p"]
     theta_m_save, rr_save = st["theta_m"], st["rho_p"]
-    z = lambda *shape: torch.zeros(shape, dtype=torch.float64, device=dm.device)  # noqa: E731
+    # Zero inputs (the acoustic state a step starts from, the carried
+    # tendencies the first RK stage records, the cold tangential velocity) are
+    # read only: one zero tensor per shape serves them all.  They depend on no
+    # step input, so the sweeps give them no adjoint or tangent.
+    zeros: dict[tuple[int, ...], torch.Tensor] = {}
+
+    def z(*shape: int) -> torch.Tensor:
+        if shape not in zeros:
+            zeros[shape] = torch.zeros(shape, dtype=torch.float64, device=dm.device)
+        return zeros[shape]
+
     L = dm.L
     acoustic = {"ru_p": z(dm.ne + 1, L), "ruAvg": z(dm.ne + 1, L), "rho_pp": z(nc + 1, L),
                 "rtheta_pp": z(nc + 1, L), "rtheta_pp_old": z(nc + 1, L), "rw_p": z(nc + 1, L + 1),
@@ -417,7 +427,10 @@ def taped_step(dm: DeviceMesh, state: Mapping[str, torch.Tensor], cfg: Any, dt: 
             params = K.dyn_params(rk, dt, len_disp, c_s=cfg.c_s, visc4_2dsmag=cfg.visc4_2dsmag,
                                   coef_3rd_order=coef3, del4u_div_factor=cfg.del4u_div_factor,
                                   rayleigh_levels=cfg.rayleigh_levels,
-                                  rayleigh_coef_inverse=cfg.rayleigh_coef_inverse)
+                                  rayleigh_coef_inverse=cfg.rayleigh_coef_inverse, mix_fixed=bool(cfg.mix_fixed),
+                                  h_theta_eddy_visc2=cfg.h_theta_eddy_visc2, h_mom_eddy_visc4=cfg.h_mom_eddy_visc4,
+                                  h_theta_eddy_visc4=cfg.h_theta_eddy_visc4, cam_levels=cfg.cam_levels,
+                                  cam_visc2=cfg.cam_visc2)
             if rk == 1:
                 carried_in = {k: z(*t.shape) for k, t in
                               zip(K.DYN_CARRIED, (st["rho_zz"], st["u"], st["w"], st["rho_zz"], st["rho_zz"]))}

@@ -316,7 +316,9 @@ def _run_differentiable(state: Mapping[str, Any], mesh: Mapping[str, Any], confi
         graph = torch.is_grad_enabled() and any(
             isinstance(v, torch.Tensor) and v.requires_grad for k, v in cur.items() if k in STEP_INPUTS)
         keep = graph and (keep_all or step == nsteps)
-        cur = _atm_srk3_differentiable(cur, mesh, config, dt, strict=True, keep_tape=keep)
+        # the base state is bound by the first step; later steps of the run reuse it
+        cur = _atm_srk3_differentiable(cur, mesh, config, dt, strict=True, keep_tape=keep,
+                                       rebind_base=step == 1)
         if on_step is not None:
             on_step(step, cur)
     if on_phase is not None:

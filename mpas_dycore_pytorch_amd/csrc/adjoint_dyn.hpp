@@ -605,11 +605,12 @@ __device__ __forceinline__ void dyn_a6(const mpas_hip_mesh &m, const mpas_hip_dy
     const double gdp = S.g_dpdz[o];
     add_own(B, qtot, o, -MPAS_GRAVITY * (m.rho_base[o] + F[rr_save][o]) * gdp);
     add_own(B, rr_save, o, -MPAS_GRAVITY * (1.0 + F[qtot][o]) * gdp);
-    // kdiff = min(coef def, cap), def = sqrt(dd^2 + dof^2): the adjoints of dd
-    // and dof, spread over the cell's edges by A7
+    // kdiff = min(coef def, cap), def = sqrt(dd^2 + dof^2), floored on the CAM
+    // levels by max(kdiff, cam): the adjoints of dd and dof, spread over the
+    // cell's edges by A7.  A fixed kdiff depends on nothing.
     double gdd = 0.0, gdof = 0.0;
-    const double gkd = B[kdiff][o];
-    if (gkd != 0.0) {
+    double gkd = B[kdiff][o];
+    if (gkd != 0.0 && !P.mix_fixed) {
         double dd = 0.0, dof = 0.0;
         for (int j = 0; j < n; ++j) {
             const int e = m.edgesOnCell[c * ME + j];
@@ -619,6 +620,12 @@ __device__ __forceinline__ void dyn_a6(const mpas_hip_mesh &m, const mpas_hip_dy
             dof = (dof + b * us) + a * vs;
         }
         const double def = sqrt(dd * dd + dof * dof);
+        const double smag = dmin(P.kdiff_coef * def, P.kdiff_cap);
+        if (k >= L - P.cam_levels) {
+            double dk, dcam;
+            max_partials(smag, cam_kdiff(P, L, k), dk, dcam);
+            gkd *= dk;
+        }
         double dsel, dcap;
         min_partials(P.kdiff_coef * def, P.kdiff_cap, dsel, dcap);
         if (dsel != 0.0 && def != 0.0) {

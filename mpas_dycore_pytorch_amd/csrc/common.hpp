@@ -87,6 +87,12 @@ struct SlotSum {
 using DynParams = mpas_hip_dyn_params;
 using AcousticParams = mpas_hip_acoustic_params;
 
+// CAM damping floor of kdiff at level k (one of the top P.cam_levels levels):
+// cam_visc2 (1 - (nLevels - (k + 1)) / cam_levels), dyn_tend.py:3442-3446
+__device__ __forceinline__ double cam_kdiff(const DynParams &P, int L, int k) {
+    return P.cam_visc2 * (1.0 - (double)(L - (k + 1)) / (double)P.cam_levels);
+}
+
 constexpr int kBlock = 256;
 
 // Compile-time caps on the slot counts, so slot loops unroll fully: every

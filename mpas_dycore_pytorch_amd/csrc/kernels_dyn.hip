@@ -49,11 +49,46 @@ static DynScratch carve(const mpas_hip_mesh &m, double *base) {
     return s;
 }
 
-// P1: Smagorinsky eddy viscosity (first RK stage only)
+// Smagorinsky eddy viscosity of cell c at level k: min(coef def, cap), def the
+// horizontal deformation summed over the cell's edges in slot order
+__device__ __forceinline__ double smagorinsky_kdiff(const mpas_hip_mesh &m, const DynParams &P,
+                                                    const double *__restrict__ u, const hreal *__restrict__ v,
+                                                    int c, int k) {
+    const int L = m.nLevels, ME = m.maxEdges;
+    const int n = m.nEdgesOnCell[c];
+    int eo[kMaxEdges];
+#pragma unroll
+    for (int j = 0; j < kMaxEdges; ++j) eo[j] = m.edgesOnCell[c * ME + clamp_slot(j, n)];
+    double uv[kMaxEdges], vv[kMaxEdges];
+#pragma unroll
+    for (int j = 0; j < kMaxEdges; ++j) {
+        uv[j] = vv[j] = 0.0;
+        if (j < n) {  // wave-uniform: slots past the live count skip the loads
+            uv[j] = u[eo[j] * L + k];
+            vv[j] = v[eo[j] * L + k];
+        }
+    }
+    double dd = 0.0, dof = 0.0;
+#pragma unroll
+    for (int j = 0; j < kMaxEdges; ++j) {
+        const int jj = clamp_slot(j, n);
+        const double a = m.defc_a[c * ME + jj], b = m.defc_b[c * ME + jj];
+        const double us = uv[j], vs = vv[j];
+        const double ndd = (dd + a * us) - b * vs;
+        const double ndof = (dof + b * us) + a * vs;
+        dd = (j < n) ? ndd : dd;
+        dof = (j < n) ? ndof : dof;
+    }
+    double def = sqrt(dd * dd + dof * dof);
+    return dmin(P.kdiff_coef * def, P.kdiff_cap);
+}
+
+// P1: eddy viscosity of the first RK stage: Smagorinsky or fixed, then the
+// CAM floor on the top levels
 __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_kdiff(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ u,
                             const hreal *__restrict__ v, double *__restrict__ kdiff) {
     auto body_ = [&](const int c) {
-        const int L = m.nLevels, nC = m.nCells, ME = m.maxEdges;
+        const int L = m.nLevels, nC = m.nCells;
         const int k = wave_lane();
         if (c > nC || k >= L) return;
         const int i = c * L + k;
@@ -61,32 +96,14 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_kdiff(long long nb, mpas_
             kdiff[i] = 0.0;
             return;
         }
-        const int n = m.nEdgesOnCell[c];
-        int eo[kMaxEdges];
-    #pragma unroll
-        for (int j = 0; j < kMaxEdges; ++j) eo[j] = m.edgesOnCell[c * ME + clamp_slot(j, n)];
-        double uv[kMaxEdges], vv[kMaxEdges];
-    #pragma unroll
-        for (int j = 0; j < kMaxEdges; ++j) {
-            uv[j] = vv[j] = 0.0;
-            if (j < n) {  // wave-uniform: slots past the live count skip the loads
-                uv[j] = u[eo[j] * L + k];
-                vv[j] = v[eo[j] * L + k];
-            }
+        double kd;
+        if (P.mix_fixed) {
+            kd = P.kdiff_fixed;
+        } else {
+            kd = smagorinsky_kdiff(m, P, u, v, c, k);
         }
-        double dd = 0.0, dof = 0.0;
-    #pragma unroll
-        for (int j = 0; j < kMaxEdges; ++j) {
-            const int jj = clamp_slot(j, n);
-            const double a = m.defc_a[c * ME + jj], b = m.defc_b[c * ME + jj];
-            const double us = uv[j], vs = vv[j];
-            const double ndd = (dd + a * us) - b * vs;
-            const double ndof = (dof + b * us) + a * vs;
-            dd = (j < n) ? ndd : dd;
-            dof = (j < n) ? ndof : dof;
-        }
-        double def = sqrt(dd * dd + dof * dof);
-        kdiff[i] = dmin(P.kdiff_coef * def, P.kdiff_cap);
+        if (k >= L - P.cam_levels) kd = dmax(kd, cam_kdiff(P, L, k));
+        kdiff[i] = kd;
     };
     const int base_ = wave_entity_base(nb);
 #pragma unroll 1

@@ -670,12 +670,21 @@ static int srk3_step_impl(const mpas_hip_mesh *mesh, StepState *st, const mpas_h
         const double cl = cfg->c_s * cfg->len_disp;
         P.kdiff_coef = cl * cl;
         P.kdiff_cap = (0.01 * (cfg->len_disp * cfg->len_disp)) * inv_dt;
-        P.h_mom_eddy_visc4 = cfg->visc4_2dsmag * std::pow(cfg->len_disp, 3.0);
-        P.h_theta_eddy_visc4 = P.h_mom_eddy_visc4;
+        if (cfg->mix_fixed) {
+            P.h_mom_eddy_visc4 = cfg->h_mom_eddy_visc4;
+            P.h_theta_eddy_visc4 = cfg->h_theta_eddy_visc4;
+        } else {
+            P.h_mom_eddy_visc4 = cfg->visc4_2dsmag * std::pow(cfg->len_disp, 3.0);
+            P.h_theta_eddy_visc4 = P.h_mom_eddy_visc4;
+        }
         P.del4u_div_factor = cfg->del4u_div_factor;
         P.prandtl_inv = 1.0 / 1.0;
         P.rayleigh_levels = cfg->rayleigh_levels;
         P.rayleigh_coef_inverse = cfg->rayleigh_coef_inverse;
+        P.mix_fixed = cfg->mix_fixed ? 1 : 0;
+        P.kdiff_fixed = cfg->h_theta_eddy_visc2;
+        P.cam_levels = cfg->cam_levels;
+        P.cam_visc2 = cfg->cam_visc2;
     }
 
     for (int sub = 1; sub <= cfg->split; ++sub) {

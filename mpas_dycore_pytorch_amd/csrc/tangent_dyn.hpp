@@ -54,23 +54,31 @@ __device__ __forceinline__ void dyn_j1(const mpas_hip_mesh &m, const mpas_hip_dy
             const double q = F[qtot][o];
             S.g_dpdz[o] = -MPAS_GRAVITY * ((m.rho_base[o] + F[rr_save][o]) * dot_of(T, qtot, o) +
                                            (1.0 + q) * dot_of(T, rr_save, o));
-            // kdiff = min(coef def, cap), def = sqrt(dd^2 + dof^2)
-            double dd = 0.0, dof = 0.0, ddd = 0.0, ddof = 0.0;
-            for (int j = 0; j < n; ++j) {
-                const int e = m.edgesOnCell[c * ME + j];
-                const double a = m.defc_a[c * ME + j], b = m.defc_b[c * ME + j];
-                const long long ie = (long long)e * L + k;
-                const double us = F[u][ie], vs = F[v][ie], dus = dot_of(T, u, ie), dvs = dot_of(T, v, ie);
-                dd = (dd + a * us) - b * vs;
-                dof = (dof + b * us) + a * vs;
-                ddd += a * dus - b * dvs;
-                ddof += b * dus + a * dvs;
-            }
-            const double def = sqrt(dd * dd + dof * dof);
-            double dsel, dcap;
-            min_partials(P.kdiff_coef * def, P.kdiff_cap, dsel, dcap);
+            // kdiff = min(coef def, cap), def = sqrt(dd^2 + dof^2), floored on the
+            // CAM levels by max(kdiff, cam); a fixed kdiff has no tangent
             double dk = 0.0;
-            if (dsel != 0.0 && def != 0.0) dk = P.kdiff_coef * dsel * (dd * ddd + dof * ddof) / def;
+            if (!P.mix_fixed) {
+                double dd = 0.0, dof = 0.0, ddd = 0.0, ddof = 0.0;
+                for (int j = 0; j < n; ++j) {
+                    const int e = m.edgesOnCell[c * ME + j];
+                    const double a = m.defc_a[c * ME + j], b = m.defc_b[c * ME + j];
+                    const long long ie = (long long)e * L + k;
+                    const double us = F[u][ie], vs = F[v][ie], dus = dot_of(T, u, ie), dvs = dot_of(T, v, ie);
+                    dd = (dd + a * us) - b * vs;
+                    dof = (dof + b * us) + a * vs;
+                    ddd += a * dus - b * dvs;
+                    ddof += b * dus + a * dvs;
+                }
+                const double def = sqrt(dd * dd + dof * dof);
+                double dsel, dcap;
+                min_partials(P.kdiff_coef * def, P.kdiff_cap, dsel, dcap);
+                if (dsel != 0.0 && def != 0.0) dk = P.kdiff_coef * dsel * (dd * ddd + dof * ddof) / def;
+                if (k >= L - P.cam_levels) {
+                    double dsm, dcam;
+                    max_partials(dmin(P.kdiff_coef * def, P.kdiff_cap), cam_kdiff(P, L, k), dsm, dcam);
+                    dk *= dsm;
+                }
+            }
             dot_put(T, kdiff, o, dk);
             // delsq_theta = sum es (th2 - th1) rho_edge
             double st = 0.0;

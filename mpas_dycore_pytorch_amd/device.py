@@ -185,7 +185,14 @@ class DeviceMesh:
         setattr(self.s, name, t.data_ptr())
 
     @classmethod
-    def get(cls, mesh: Mapping, device: torch.device, state: Mapping | None = None) -> "DeviceMesh":
+    def get(cls, mesh: Mapping, device: torch.device, state: Mapping | None = None,
+            rebind_base: bool = True) -> "DeviceMesh":
+        """The cached device mesh of ``mesh`` on ``device``.  A cached mesh
+        binds its base state again from ``mesh``/``state`` unless
+        ``rebind_base`` is False, which a caller passes only when it bound
+        that same base state earlier in the same run (the large steps after
+        the first of a differentiable run): a NumPy base field is uploaded
+        again at every binding."""
         key = (id(mesh), str(device))
         dm = cls._cache.get(key)
         if dm is None or dm._mesh_ref is not mesh:
@@ -194,7 +201,7 @@ class DeviceMesh:
             if len(cls._cache) > 8:
                 cls._cache.clear()
             cls._cache[key] = dm
-        else:
+        elif rebind_base:
             dm.set_base(mesh, state)
         return dm
 

@@ -324,7 +324,8 @@ def ensemble_vjp(members: Sequence[tuple], objective, nsteps: int = 1,
             cur = dict(ustate)
             cur.update(leaves)
             for step in range(1, nsteps + 1):
-                cur = _atm_srk3_differentiable(cur, umesh, cfgs[cfg_id], dt, strict=True, keep_tape=step == nsteps)
+                cur = _atm_srk3_differentiable(cur, umesh, cfgs[cfg_id], dt, strict=True, keep_tape=step == nsteps,
+                                               rebind_base=step == 1)
             outs = split({k: v for k, v in cur.items() if isinstance(v, torch.Tensor)}, meshes)
             loss = None
             for i, out in zip(batch, outs):

@@ -48,6 +48,19 @@ def _z(dm: DeviceMesh, *shape: int) -> torch.Tensor:
     return torch.zeros(shape, dtype=torch.float64, device=dm.device)
 
 
+def _e(dm: DeviceMesh, *shape: int) -> torch.Tensor:
+    """An output buffer its kernel writes in full (every row and level)."""
+    return torch.empty(shape, dtype=torch.float64, device=dm.device)
+
+
+def _ep(dm: DeviceMesh, *shape: int) -> torch.Tensor:
+    """An output buffer its kernel writes in full except the padding row,
+    which is zero."""
+    t = torch.empty(shape, dtype=torch.float64, device=dm.device)
+    t[shape[0] - 1].zero_()
+    return t
+
+
 def _s(dm: DeviceMesh) -> C.c_void_p:
     return C.c_void_p(stream_handle(dm.device))
 
@@ -55,9 +68,17 @@ def _s(dm: DeviceMesh) -> C.c_void_p:
 def dyn_params(rk_step: int, dt: float, len_disp: float, c_s: float = 0.125, visc4_2dsmag: float = 0.05,
                coef_3rd_order: float = 0.25, del4u_div_factor: float = 10.0,
                prandtl: float = 1.0, rayleigh_levels: int = 0,
-               rayleigh_coef_inverse: float = 0.0) -> N.DynParamsStruct:
-    """Scalar set-up of the dynamics tendency (dyn_tend.py:3367-3400, :9616-9626); the
-    Rayleigh damping of u on ``rayleigh_levels`` top levels (0: off) with its
+               rayleigh_coef_inverse: float = 0.0, mix_fixed: bool = False, h_theta_eddy_visc2: float = 0.0,
+               h_mom_eddy_visc4: float = 0.0, h_theta_eddy_visc4: float = 0.0, cam_levels: int = 0,
+               cam_visc2: float = 0.0) -> N.DynParamsStruct:
+    """Scalar set-up of the dynamics tendency (dyn_tend.py:3367-3458, :9616-9626).
+
+    ``mix_fixed`` selects the "2d_fixed" horizontal mixing: kdiff is
+    ``h_theta_eddy_visc2`` everywhere and the del4 coefficients are the given
+    ones; otherwise Smagorinsky kdiff and ``visc4_2dsmag * len_disp**3``.  The
+    CAM damping floors kdiff on the top ``cam_levels`` levels (0: off) at
+    ``cam_visc2 * (1 - (nLevels - (k + 1)) / cam_levels)``.  The Rayleigh
+    damping of u acts on ``rayleigh_levels`` top levels (0: off) with its
     coefficient 1 / (levels * timescale) (dyn_tend.py:10066-10072)."""
     p = N.DynParamsStruct()
     p.rk_step = int(rk_step)
@@ -66,12 +87,20 @@ def dyn_params(rk_step: int, dt: float, len_disp: float, c_s: float = 0.125, vis
     inv_dt = 1.0 / float(dt)
     p.kdiff_coef = (c_s * len_disp) ** 2
     p.kdiff_cap = (0.01 * len_disp ** 2) * inv_dt
-    p.h_mom_eddy_visc4 = visc4_2dsmag * len_disp ** 3
-    p.h_theta_eddy_visc4 = p.h_mom_eddy_visc4
+    if mix_fixed:
+        p.h_mom_eddy_visc4 = float(h_mom_eddy_visc4)
+        p.h_theta_eddy_visc4 = float(h_theta_eddy_visc4)
+    else:
+        p.h_mom_eddy_visc4 = visc4_2dsmag * len_disp ** 3
+        p.h_theta_eddy_visc4 = p.h_mom_eddy_visc4
     p.del4u_div_factor = float(del4u_div_factor)
     p.prandtl_inv = 1.0 / prandtl
     p.rayleigh_levels = int(rayleigh_levels)
     p.rayleigh_coef_inverse = float(rayleigh_coef_inverse)
+    p.mix_fixed = 1 if mix_fixed else 0
+    p.kdiff_fixed = float(h_theta_eddy_visc2)
+    p.cam_levels = int(cam_levels)
+    p.cam_visc2 = float(cam_visc2)
     return p
 
 
@@ -90,7 +119,7 @@ def acoustic_params(small_step: int, dts: float, epssm: float = 0.1) -> N.Acoust
 def moist_coefficients(dm: DeviceMesh, scalars, moist_start: int, moist_end: int) -> dict:
     sc = _d(scalars, dm)
     S = sc.shape[2]
-    out = {"cqw": _z(dm, dm.nc + 1, dm.L + 1), "cqu": _z(dm, dm.ne + 1, dm.L), "qtot": _z(dm, dm.nc + 1, dm.L)}
+    out = {"cqw": _e(dm, dm.nc + 1, dm.L + 1), "cqu": _e(dm, dm.ne + 1, dm.L), "qtot": _e(dm, dm.nc + 1, dm.L)}
     N.check(N.load().mpas_hip_moist_coefficients(dm.ref, _p(sc), S, int(moist_start), int(moist_end),
                                                  _p(out["cqw"]), _p(out["cqu"]), _p(out["qtot"]), _s(dm)),
             "moist_coefficients")
@@ -100,9 +129,9 @@ def moist_coefficients(dm: DeviceMesh, scalars, moist_start: int, moist_end: int
 def vert_imp_coefs(dm: DeviceMesh, dts: float, epssm: float, cqw, p, t, rt, qtot) -> dict:
     ins = [_d(x, dm) for x in (cqw, p, t, rt, qtot)]
     nc, L = dm.nc, dm.L
-    out = {"cofrz": _z(dm, L), "cofwr": _z(dm, nc + 1, L), "cofwz": _z(dm, nc + 1, L),
-           "coftz": _z(dm, nc + 1, L + 1), "cofwt": _z(dm, nc + 1, L), "a_tri": _z(dm, nc + 1, L),
-           "alpha_tri": _z(dm, nc + 1, L), "gamma_tri": _z(dm, nc + 1, L)}
+    out = {"cofrz": _e(dm, L), "cofwr": _e(dm, nc + 1, L), "cofwz": _e(dm, nc + 1, L),
+           "coftz": _e(dm, nc + 1, L + 1), "cofwt": _e(dm, nc + 1, L), "a_tri": _e(dm, nc + 1, L),
+           "alpha_tri": _e(dm, nc + 1, L), "gamma_tri": _e(dm, nc + 1, L)}
     N.check(N.load().mpas_hip_vert_imp_coefs(dm.ref, float(dts), float(epssm), *[_p(x) for x in ins],
                                              *[_p(out[k]) for k in VIMP_OUT], _s(dm)), "vert_imp_coefs")
     return out
@@ -111,8 +140,8 @@ def vert_imp_coefs(dm: DeviceMesh, dts: float, epssm: float, cqw, p, t, rt, qtot
 def solve_diagnostics(dm: DeviceMesh, dt: float, apvm: float, rk_step, u, h, v_in=None) -> dict:
     u, h = _d(u, dm), _d(h, dm)
     v_in = _d(v_in, dm) if v_in is not None else _z(dm, dm.ne + 1, dm.L)
-    out = {"h_edge": _z(dm, dm.ne + 1, dm.L), "v": _z(dm, dm.ne + 1, dm.L), "divergence": _z(dm, dm.nc + 1, dm.L),
-           "vorticity": _z(dm, dm.nv + 1, dm.L), "ke": _z(dm, dm.nc + 1, dm.L), "pv_edge": _z(dm, dm.ne + 1, dm.L)}
+    out = {"h_edge": _e(dm, dm.ne + 1, dm.L), "v": _e(dm, dm.ne + 1, dm.L), "divergence": _e(dm, dm.nc + 1, dm.L),
+           "vorticity": _e(dm, dm.nv + 1, dm.L), "ke": _e(dm, dm.nc + 1, dm.L), "pv_edge": _e(dm, dm.ne + 1, dm.L)}
     rk = -1 if rk_step is None else int(rk_step)
     N.check(N.load().mpas_hip_solve_diagnostics(dm.ref, float(dt), float(apvm), rk, _p(u), _p(h), _p(v_in),
                                                 *[_p(out[k]) for k in DIAG_OUT], _p(dm.kernel_scratch()), _s(dm)),
@@ -126,9 +155,12 @@ def dyn_tend(dm: DeviceMesh, params: N.DynParamsStruct, fields: Mapping, carried
     carried = carried or {}
     shapes = {"tend_rho": (nc + 1, L), "tend_u_euler": (ne + 1, L), "tend_w_euler": (nc + 1, L + 1),
               "tend_theta_euler": (nc + 1, L), "kdiff": (nc + 1, L)}
-    io = {k: (_d(carried[k], dm).clone() if k in carried else _z(dm, *shapes[k])) for k in DYN_CARRIED}
-    out = {"tend_u": _z(dm, ne + 1, L), "tend_w": _z(dm, nc + 1, L + 1), "tend_theta": _z(dm, nc + 1, L),
-           "h_divergence": _z(dm, nc + 1, L), "rthdynten": _z(dm, nc + 1, L)}
+    # without carried inputs (the first RK stage) the kernels write the carried
+    # tendencies in full but for their padding rows, and kdiff in full
+    io = {k: (_d(carried[k], dm).clone() if k in carried else (_e if k == "kdiff" else _ep)(dm, *shapes[k]))
+          for k in DYN_CARRIED}
+    out = {"tend_u": _e(dm, ne + 1, L), "tend_w": _e(dm, nc + 1, L + 1), "tend_theta": _e(dm, nc + 1, L),
+           "h_divergence": _e(dm, nc + 1, L), "rthdynten": _e(dm, nc + 1, L)}
     N.check(N.load().mpas_hip_dyn_tend(dm.ref, C.byref(params), *[_p(x) for x in ins],
                                        *[_p(io[k]) for k in DYN_CARRIED], *[_p(out[k]) for k in DYN_OUTPUTS],
                                        _p(dm.kernel_scratch()), _s(dm)), "dyn_tend")
@@ -165,13 +197,16 @@ def recover_large_step(dm: DeviceMesh, rk_step: int, dt: float, ns: int, fields:
     nc, ne, L = dm.nc, dm.ne, dm.L
     ww = _d(fields["wwAvg"], dm).clone()
     ra = _d(fields["ruAvg"], dm).clone()
-    out = {k: _z(dm, nc + 1, L) for k in ("rho_p", "rho_zz", "rtheta_p", "theta_m")}
+    # written in full but for the padding rows (rho_zz in full); rw leaves its
+    # boundary interfaces to the zero fill
+    out = {k: _ep(dm, nc + 1, L) for k in ("rho_p", "rtheta_p", "theta_m")}
+    out["rho_zz"] = _e(dm, nc + 1, L)
     out["exner"] = _d(exner_in, dm).clone() if exner_in is not None else _z(dm, nc + 1, L)
     out["pressure_p"] = _d(pressure_p_in, dm).clone() if pressure_p_in is not None else _z(dm, nc + 1, L)
     out["rw"] = _z(dm, nc + 1, L + 1)
-    out["w"] = _z(dm, nc + 1, L + 1)
-    out["ru"] = _z(dm, ne + 1, L)
-    out["u"] = _z(dm, ne + 1, L)
+    out["w"] = _ep(dm, nc + 1, L + 1)
+    out["ru"] = _ep(dm, ne + 1, L)
+    out["u"] = _ep(dm, ne + 1, L)
     names = ("rho_p", "rho_zz", "rtheta_p", "theta_m", "exner", "pressure_p", "rw", "w", "ru", "u")
     N.check(N.load().mpas_hip_recover_large_step(dm.ref, int(rk_step), float(dt), int(ns), *[_p(x) for x in ins],
                                                  _p(ww), _p(ra), *[_p(out[k]) for k in names], _s(dm)),
@@ -186,7 +221,7 @@ def advance_scalars(dm: DeviceMesh, dt, coef3, weight_old, weight_new, scalar_ol
     so, sn = _d(scalar_old, dm), _d(scalar_new, dm)
     S = so.shape[2]
     out = torch.empty_like(sn)
-    hf = _z(dm, dm.ne + 1, dm.L, S)
+    hf = _ep(dm, dm.ne + 1, dm.L, S)
     ins = [_d(x, dm) for x in (rho_zz_old, rho_zz_new, uhAvg, wwAvg)]
     N.check(N.load().mpas_hip_advance_scalars(dm.ref, S, float(dt), float(coef3), float(weight_old),
                                               float(weight_new), _p(so), _p(sn), *[_p(x) for x in ins],
@@ -200,7 +235,7 @@ def advance_scalars_mono(dm: DeviceMesh, dt, coef3, scalars_old, scalars_new, rh
     so, sn = _d(scalars_old, dm), _d(scalars_new, dm)
     S = so.shape[2]
     out = sn.clone()
-    rzi = _z(dm, dm.nc + 1, dm.L)
+    rzi = _ep(dm, dm.nc + 1, dm.L)
     ins = [_d(x, dm) for x in (rho_zz_old, uhAvg, wwAvg)]
     N.check(N.load().mpas_hip_advance_scalars_mono(dm.ref, S, float(dt), float(coef3), _p(so), _p(sn),
                                                    *[_p(x) for x in ins],
@@ -240,7 +275,7 @@ def init_coupled_diagnostics(dm: DeviceMesh, theta, rho, qv, u, w, theta_base, t
 
 
 def reconstruct(dm: DeviceMesh, u) -> dict:
-    out = {k: _z(dm, dm.nc + 1, dm.L) for k in RECON_OUT}
+    out = {k: _e(dm, dm.nc + 1, dm.L) for k in RECON_OUT}
     ud = _d(u, dm)
     N.check(N.load().mpas_hip_reconstruct(dm.ref, _p(ud), *[_p(out[k]) for k in RECON_OUT], _s(dm)),
             "reconstruct")

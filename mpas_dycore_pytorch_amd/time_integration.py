@@ -46,6 +46,11 @@ DEFAULTS = {
     "config_rayleigh_damp_u": False,
     "config_rayleigh_damp_u_timescale_days": 5.0,
     "config_number_rayleigh_damp_u_levels": 6,
+    "config_h_theta_eddy_visc2": 0.0,
+    "config_h_mom_eddy_visc4": 0.0,
+    "config_h_theta_eddy_visc4": 0.0,
+    "config_mpas_cam_coef": 0.0,
+    "config_number_cam_damping_levels": 4,
 }
 # Short aliases the reference also accepts (config_defaults.py:149-160)
 ALIASES = {"smdiv": "config_smdiv", "len_disp": "config_len_disp", "monotonic": "config_monotonic",
@@ -54,19 +59,23 @@ ALIASES = {"smdiv": "config_smdiv", "len_disp": "config_len_disp", "monotonic": 
            "positive_definite": "config_positive_definite", "apply_lbcs": "config_apply_lbcs",
            "rayleigh_damp_u": "config_rayleigh_damp_u",
            "rayleigh_damp_u_timescale_days": "config_rayleigh_damp_u_timescale_days",
-           "number_rayleigh_damp_u_levels": "config_number_rayleigh_damp_u_levels"}
+           "number_rayleigh_damp_u_levels": "config_number_rayleigh_damp_u_levels",
+           "horiz_mixing": "config_horiz_mixing", "mpas_cam_coef": "config_mpas_cam_coef",
+           "number_cam_damping_levels": "config_number_cam_damping_levels",
+           "apvm_upwinding": "config_apvm_upwinding", "epssm": "config_epssm",
+           "coef_3rd_order": "config_coef_3rd_order", "smagorinsky_coef": "config_smagorinsky_coef",
+           "c_s": "config_smagorinsky_coef", "nAcousticSteps": "config_number_of_sub_steps",
+           "number_of_sub_steps": "config_number_of_sub_steps",
+           "split_dynamics_transport": "config_split_dynamics_transport",
+           "dynamics_split_steps": "config_dynamics_split_steps", "mix_full": "config_mix_full",
+           "curvature_u": "config_curvature_u", "scalar_advection": "config_scalar_advection",
+           "advect_scalars": "config_scalar_advection"}
 
 # Options whose non-default values select code the GPU step does not have.
 REQUIRED_VALUES = {
     "config_split_dynamics_transport": True,
-    "config_horiz_mixing": "2d_smagorinsky",
-    "config_h_mom_eddy_visc2": 0.0,
-    "config_h_theta_eddy_visc2": 0.0,
     "config_v_mom_eddy_visc2": 0.0,
     "config_v_theta_eddy_visc2": 0.0,
-    "config_h_mom_eddy_visc4": 0.0,
-    "config_h_theta_eddy_visc4": 0.0,
-    "config_mpas_cam_coef": 0.0,
     "config_curvature_u": False,
     "config_apply_lbcs": False,
     "config_positive_definite": False,
@@ -85,9 +94,16 @@ OUTPUT_ONLY_KEYS = ("ur_cell", "vr_cell", "uReconstructX", "uReconstructY", "uRe
 
 
 def _cfg(config: Mapping[str, Any] | None) -> dict[str, Any]:
+    """The defaults overlaid with ``config``; a full key name takes precedence
+    over its short alias when both are given (config_defaults.py:745-774)."""
     out = dict(DEFAULTS)
-    for k, v in (config or {}).items():
-        out[ALIASES.get(k, k)] = v
+    given = dict(config or {})
+    for k, v in given.items():
+        if k in ALIASES and ALIASES[k] not in given:
+            out[ALIASES[k]] = v
+    for k, v in given.items():
+        if k not in ALIASES:
+            out[k] = v
     return out
 
 
@@ -176,9 +192,35 @@ def resolve_config(config: Mapping[str, Any] | None, state: Mapping[str, Any], m
         raise MpasConfigError(f"moist species range {s.moist_start}..{s.moist_end} outside 1..{n_scalars}")
     s.diagnostics_current = 0
     s.order = int(order)
-    s.rayleigh_levels, s.rayleigh_coef_inverse = rayleigh_damping(c, int(to_numpy(mesh["nVertLevels"]))
-                                                                  if "nVertLevels" in mesh else None)
+    n_levels = int(to_numpy(mesh["nVertLevels"])) if "nVertLevels" in mesh else None
+    s.rayleigh_levels, s.rayleigh_coef_inverse = rayleigh_damping(c, n_levels)
+    mixing = str(c["config_horiz_mixing"]).strip()
+    if mixing not in ("2d_smagorinsky", "2d_fixed"):
+        raise MpasConfigError(f"Unsupported config_horiz_mixing={c['config_horiz_mixing']!r}. Supported modes are "
+                              "'2d_smagorinsky' and '2d_fixed'.")
+    s.mix_fixed = 1 if mixing == "2d_fixed" else 0
+    s.h_theta_eddy_visc2 = float(c["config_h_theta_eddy_visc2"])
+    s.h_mom_eddy_visc4 = float(c["config_h_mom_eddy_visc4"])
+    s.h_theta_eddy_visc4 = float(c["config_h_theta_eddy_visc4"])
+    s.cam_levels, s.cam_visc2 = cam_damping(c, len_disp, n_levels)
     return s
+
+
+def cam_damping(c: Mapping[str, Any], len_disp: float, n_levels: int | None) -> tuple[int, float]:
+    """(damped top levels, 4 * 2.0833 * len_disp * coef) of the CAM damping of kdiff
+    (dyn_tend.py:3434-3458); (0, 0.0) unless config_mpas_cam_coef > 0 and
+    config_number_cam_damping_levels > 0, as the reference gates it.  More damped
+    levels than the model has are rejected."""
+    coef = float(c["config_mpas_cam_coef"])
+    levels = c["config_number_cam_damping_levels"]
+    if isinstance(levels, bool) or int(levels) != levels:
+        raise MpasConfigError(f"config_number_cam_damping_levels must be an integer, got {levels!r}")
+    levels = int(levels)
+    if not (coef > 0.0 and levels > 0):
+        return 0, 0.0
+    if n_levels is not None and levels > n_levels:
+        raise MpasConfigError(f"config_number_cam_damping_levels {levels} exceeds the {n_levels} model levels")
+    return levels, 4.0 * 2.0833 * len_disp * coef
 
 
 def rayleigh_damping(c: Mapping[str, Any], n_levels: int | None) -> tuple[int, float]:
@@ -248,7 +290,7 @@ class Srk3Stepper:
 
     def __init__(self, state: Mapping[str, Any], mesh: Mapping[str, Any], config: Mapping[str, Any] | None,
                  dt: float, device: torch.device | str | None = None, strict: bool = True,
-                 reorder: bool = False, precision: str | None = None):
+                 reorder: bool = False, precision: str | None = None, rebind_base: bool = True):
         if precision is None:
             # the build's own configuration key: "fp64" (default) or "mixed"
             precision = str(_cfg(config).get("config_precision", "fp64"))
@@ -276,7 +318,7 @@ class Srk3Stepper:
                 "edge": torch.as_tensor(np.concatenate([perm.edge_old_to_new, [ne]]), device=self.device),
                 "vertex": torch.as_tensor(np.concatenate([perm.vertex_old_to_new, [nv]]), device=self.device),
             }
-        self.mesh = DeviceMesh.get(mesh, self.device, state)
+        self.mesh = DeviceMesh.get(mesh, self.device, state, rebind_base=rebind_base)
         self.cfg = resolve_config(config, state, mesh, dt, n_scalars, strict=strict)
         dm = self.mesh
         missing = [k for k in PROGNOSTIC_KEYS[:5] + CARRIED_KEYS if k not in state]
@@ -390,7 +432,8 @@ class _Srk3Step(torch.autograd.Function):
         u = state["u"]
         out_device = u.device if isinstance(u, torch.Tensor) else torch.device("cpu")
         gpu = out_device if out_device.type == "cuda" else None
-        stepper = Srk3Stepper(state, box["mesh"], box["config"], box["dt"], device=gpu, strict=box["strict"])
+        stepper = Srk3Stepper(state, box["mesh"], box["config"], box["dt"], device=gpu, strict=box["strict"],
+                              rebind_base=box.get("rebind_base", True))
         # the checkpoint is binary64 (widened from the mixed step's binary32 fields): the
         # backward recomputes the step in binary64 around the trajectory the forward took
         ckpt = {k: stepper.t[k].to(torch.float64).clone() for k in STEP_INPUTS}
@@ -465,11 +508,14 @@ class _Srk3Step(torch.autograd.Function):
 
 
 def _atm_srk3_differentiable(state: Mapping[str, Any], mesh: Mapping[str, Any], config: Mapping[str, Any],
-                             dt: float, strict: bool, keep_tape: bool = False) -> dict[str, Any]:
+                             dt: float, strict: bool, keep_tape: bool = False,
+                             rebind_base: bool = True) -> dict[str, Any]:
     """The differentiable step.  The forward records the step's tape when
     ``keep_tape`` is set (the caller knows a backward follows) or when an
     input carries a forward-mode tangent (the jvp runs right after the
-    forward); otherwise the backward recomputes it from the checkpoint."""
+    forward); otherwise the backward recomputes it from the checkpoint.
+    ``rebind_base=False`` keeps the base state the device mesh holds (a run
+    passes it for every step after its first)."""
     from torch.autograd import forward_ad
 
     from .adjoint import STEP_INPUTS
@@ -477,7 +523,8 @@ def _atm_srk3_differentiable(state: Mapping[str, Any], mesh: Mapping[str, Any], 
     in_keys = [k for k in STEP_INPUTS if isinstance(state.get(k), torch.Tensor)]
     dual = any(forward_ad.unpack_dual(state[k]).tangent is not None for k in in_keys)
     box = {"state": {k: v for k, v in state.items() if k not in in_keys}, "mesh": mesh, "config": config,
-           "dt": dt, "strict": strict, "in_keys": in_keys, "keep_tape": bool(keep_tape or dual)}
+           "dt": dt, "strict": strict, "in_keys": in_keys, "keep_tape": bool(keep_tape or dual),
+           "rebind_base": bool(rebind_base)}
     outs = _Srk3Step.apply(box, *[state[k] for k in in_keys])
     result = dict(state)
     result.update(zip(box["out_keys"], outs))

@@ -240,9 +240,10 @@ void orc_solve_diagnostics(const orc_mesh *m, double dt, double apvm, int rk_ste
  * (edge advection), :4174/:4312 (w/theta horizontal advection), :4415-4657
  * (w vertical), :4803-5057 (theta vertical), :5315-5843 (w/theta mixing),
  * :5901 (theta perturbation flux), :6485 (ordered del4 subtraction).
- * Active configuration: 2d_smagorinsky, no vertical eddy viscosity, optional
- * Rayleigh damping of u on the top levels (:2942-2996, coefficient
- * :10066-10072), no curvature, no CAM damping, no physics tendencies.
+ * Active configuration: 2d_smagorinsky or 2d_fixed horizontal mixing
+ * (:3367-3430), optional CAM damping of kdiff (:3434-3458), no vertical eddy
+ * viscosity, optional Rayleigh damping of u on the top levels (:2942-2996,
+ * coefficient :10066-10072), no curvature, no physics tendencies.
  * ---------------------------------------------------------------------- */
 typedef struct orc_dyn_params {
     int rk_step;
@@ -256,6 +257,10 @@ typedef struct orc_dyn_params {
     double prandtl_inv;
     int rayleigh_levels;          /* damped top levels, 0 without Rayleigh damping of u */
     double rayleigh_coef_inverse; /* 1 / (levels * timescale_days * 86400) */
+    int mix_fixed;                /* 1: "2d_fixed", kdiff = kdiff_fixed (config_h_theta_eddy_visc2) */
+    double kdiff_fixed;
+    int cam_levels;               /* CAM-damped top levels, 0 without CAM damping */
+    double cam_visc2;             /* 4 * 2.0833 * len_disp * config_mpas_cam_coef */
 } orc_dyn_params;
 
 int orc_dyn_params_sizeof(void) { return (int)sizeof(orc_dyn_params); }
@@ -293,7 +298,9 @@ void orc_dyn_tend(const orc_mesh *m, const orc_dyn_params *P,
     memset(h_divergence, 0, sizeof(double) * CL);
     memset(rthdynten, 0, sizeof(double) * CL);
 
-    /* mixing initialisation: Smagorinsky kdiff (dyn_tend.py:3367-3465) */
+    /* mixing initialisation (dyn_tend.py:3367-3465): Smagorinsky kdiff, or the
+     * fixed h_theta_eddy_visc2; then max(kdiff, visc2cam(k)) on the CAM levels with
+     * visc2cam(k) = base * (1 - (L - (k + 1)) / levels) */
     if (rk1) {
         memset(kdiff, 0, sizeof(double) * CL);
         memset(tend_u_euler, 0, sizeof(double) * EL);
@@ -303,16 +310,26 @@ void orc_dyn_tend(const orc_mesh *m, const orc_dyn_params *P,
         for (int c = 0; c < nC; ++c) {
             int n = m->nEdgesOnCell[c];
             for (int k = 0; k < L; ++k) {
-                double dd = 0.0, dof = 0.0;
-                for (int i = 0; i < n; ++i) {
-                    int e = m->edgesOnCell[c * ME + i];
-                    double a = m->defc_a[c * ME + i], b = m->defc_b[c * ME + i];
-                    double us = u[(size_t)e * L + k], vs = v[(size_t)e * L + k];
-                    dd = (dd + a * us) - b * vs;
-                    dof = (dof + b * us) + a * vs;
+                double kd;
+                if (P->mix_fixed) {
+                    kd = P->kdiff_fixed;
+                } else {
+                    double dd = 0.0, dof = 0.0;
+                    for (int i = 0; i < n; ++i) {
+                        int e = m->edgesOnCell[c * ME + i];
+                        double a = m->defc_a[c * ME + i], b = m->defc_b[c * ME + i];
+                        double us = u[(size_t)e * L + k], vs = v[(size_t)e * L + k];
+                        dd = (dd + a * us) - b * vs;
+                        dof = (dof + b * us) + a * vs;
+                    }
+                    double def = sqrt(dd * dd + dof * dof);
+                    kd = dmin(P->kdiff_coef * def, P->kdiff_cap);
                 }
-                double def = sqrt(dd * dd + dof * dof);
-                kdiff[(size_t)c * L + k] = dmin(P->kdiff_coef * def, P->kdiff_cap);
+                if (P->cam_levels > 0 && k >= L - P->cam_levels) {
+                    double factor = 1.0 - (double)(L - (k + 1)) / (double)P->cam_levels;
+                    kd = dmax(kd, P->cam_visc2 * factor);
+                }
+                kdiff[(size_t)c * L + k] = kd;
             }
         }
     }

@@ -54,7 +54,9 @@ class OrcDynParams(C.Structure):
                 ("kdiff_coef", C.c_double), ("kdiff_cap", C.c_double),
                 ("h_mom_eddy_visc4", C.c_double), ("h_theta_eddy_visc4", C.c_double),
                 ("del4u_div_factor", C.c_double), ("prandtl_inv", C.c_double),
-                ("rayleigh_levels", C.c_int), ("rayleigh_coef_inverse", C.c_double)]
+                ("rayleigh_levels", C.c_int), ("rayleigh_coef_inverse", C.c_double),
+                ("mix_fixed", C.c_int), ("kdiff_fixed", C.c_double), ("cam_levels", C.c_int),
+                ("cam_visc2", C.c_double)]
 
 
 class OrcAcousticParams(C.Structure):
@@ -184,11 +186,14 @@ DYN_INPUTS = ("u", "v", "w", "ru", "rw", "ru_save", "rw_save", "theta_m", "theta
 
 
 def dyn_params(rk_step, dt, len_disp, c_s=0.125, visc4_2dsmag=0.05, coef_3rd_order=0.25,
-               del4u_div_factor=10.0, prandtl=1.0, rayleigh_levels=0, rayleigh_timescale_days=5.0
-               ) -> OrcDynParams:
-    """Scalar set-up of dyn_tend.py:3367-3400 and :9616-9626, evaluated in Python;
-    Rayleigh damping of u over ``rayleigh_levels`` top levels (0: off) with the
-    coefficient of dyn_tend.py:10066-10072."""
+               del4u_div_factor=10.0, prandtl=1.0, rayleigh_levels=0, rayleigh_timescale_days=5.0,
+               horiz_mixing="2d_smagorinsky", h_theta_eddy_visc2=0.0, h_mom_eddy_visc4=0.0,
+               h_theta_eddy_visc4=0.0, cam_coef=0.0, cam_levels=4) -> OrcDynParams:
+    """Scalar set-up of dyn_tend.py:3367-3458 and :9616-9626, evaluated in Python:
+    Smagorinsky or "2d_fixed" mixing coefficients, the CAM damping of kdiff when
+    ``cam_coef`` > 0 and ``cam_levels`` > 0, Rayleigh damping of u over
+    ``rayleigh_levels`` top levels (0: off) with the coefficient of
+    dyn_tend.py:10066-10072."""
     p = OrcDynParams()
     p.rk_step = int(rk_step)
     p.dt = float(dt)
@@ -196,8 +201,21 @@ def dyn_params(rk_step, dt, len_disp, c_s=0.125, visc4_2dsmag=0.05, coef_3rd_ord
     inv_dt = 1.0 / float(dt)
     p.kdiff_coef = (c_s * len_disp) ** 2
     p.kdiff_cap = (0.01 * len_disp ** 2) * inv_dt
-    p.h_mom_eddy_visc4 = visc4_2dsmag * len_disp ** 3
-    p.h_theta_eddy_visc4 = p.h_mom_eddy_visc4
+    if horiz_mixing == "2d_smagorinsky":
+        p.h_mom_eddy_visc4 = visc4_2dsmag * len_disp ** 3
+        p.h_theta_eddy_visc4 = p.h_mom_eddy_visc4
+    elif horiz_mixing == "2d_fixed":
+        p.h_mom_eddy_visc4 = float(h_mom_eddy_visc4)
+        p.h_theta_eddy_visc4 = float(h_theta_eddy_visc4)
+    else:
+        raise ValueError(f"config_horiz_mixing {horiz_mixing!r}")
+    p.mix_fixed = 1 if horiz_mixing == "2d_fixed" else 0
+    p.kdiff_fixed = float(h_theta_eddy_visc2)
+    if cam_coef > 0.0 and cam_levels > 0:
+        p.cam_levels = int(cam_levels)
+        p.cam_visc2 = 4.0 * 2.0833 * len_disp * cam_coef
+    else:
+        p.cam_levels, p.cam_visc2 = 0, 0.0
     p.del4u_div_factor = float(del4u_div_factor)
     p.prandtl_inv = 1.0 / prandtl
     p.rayleigh_levels = int(rayleigh_levels)

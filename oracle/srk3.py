@@ -56,9 +56,10 @@ DIAG_KEYS = ("h_edge", "v", "divergence", "vorticity", "ke", "pv_edge")
 def srk3(state: dict, mesh: "O.Mesh", dt: float, *, len_disp: float, ns: int = 2, split: int = 3,
          epssm: float = 0.1, smdiv: float = 0.1, apvm: float = 0.5, coef3: float = 0.25,
          monotonic: bool = True, diagnostics_current: bool = False, order: int = 3,
-         rayleigh_levels: int = 0, rayleigh_timescale_days: float = 5.0) -> dict:
+         rayleigh_levels: int = 0, rayleigh_timescale_days: float = 5.0, mixing: dict | None = None) -> dict:
     """One large step on NumPy arrays in the reference layout; returns a new dict.
-    ``rayleigh_levels`` > 0 damps u on that many top levels (config_rayleigh_damp_u)."""
+    ``rayleigh_levels`` > 0 damps u on that many top levels (config_rayleigh_damp_u);
+    ``mixing`` passes the horizontal-mixing and CAM keywords of ``oracle.dyn_params``."""
     st = {k: (np.array(v, dtype=np.float64, copy=True) if isinstance(v, np.ndarray) and v.dtype.kind == "f" else v)
           for k, v in state.items()}
     nc, ne, L = mesh.nc, mesh.ne, mesh.L
@@ -111,7 +112,7 @@ def srk3(state: dict, mesh: "O.Mesh", dt: float, *, len_disp: float, ns: int = 2
                 "qtot": moist["qtot"],
             }
             params = O.dyn_params(rk, dt, len_disp, coef_3rd_order=coef3, rayleigh_levels=rayleigh_levels,
-                                  rayleigh_timescale_days=rayleigh_timescale_days)
+                                  rayleigh_timescale_days=rayleigh_timescale_days, **(mixing or {}))
             tend = O.dyn_tend(mesh, params, fields, carried if rk > 1 else None)
             carried = {k: tend[k] for k in ("tend_rho", "tend_u_euler", "tend_w_euler", "tend_theta_euler", "kdiff")}
             tend_rw = O.smlstep_pert(mesh, tend["tend_u"], tend["tend_w"])

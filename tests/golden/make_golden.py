@@ -70,6 +70,15 @@ OPTION_CASES = {
     "l2_L20_rayleigh": (2, 20, "quadratic", 1, 900.0, 2, 7070,
                         {"config_rayleigh_damp_u": True, "config_number_rayleigh_damp_u_levels": 5,
                          "config_rayleigh_damp_u_timescale_days": 0.05}),
+    # fixed horizontal mixing: constant kdiff and del4 coefficients of their own
+    # (different for momentum and theta), each strong enough to move the fields
+    "l2_L20_fixedmix": (2, 20, "quadratic", 1, 900.0, 2, 9090,
+                        {"config_horiz_mixing": "2d_fixed", "config_h_theta_eddy_visc2": 1.0e8,
+                         "config_h_mom_eddy_visc4": 5.0e19, "config_h_theta_eddy_visc4": 2.0e19}),
+    # CAM damping over the top 6 levels, its floor comparable to the Smagorinsky
+    # kdiff so that both sides of the max are taken
+    "l2_L20_cam": (2, 20, "quadratic", 1, 900.0, 2, 9191,
+                   {"config_mpas_cam_coef": 0.02, "config_number_cam_damping_levels": 6}),
 }
 SUMMARY_CASES = {
     "cfg1_l4_L26": (4, 26, "uniform", 0, 1800.0, 2, 20240),

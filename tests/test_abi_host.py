@@ -85,10 +85,31 @@ def test_config_gate():
     for bad in (1, 4, 2.5, "three", True):
         with pytest.raises(MpasConfigError):
             resolve_config({**ok, "config_time_integration_order": bad}, st, st, meta["dt"], 2)
-    with pytest.raises(MpasUnsupportedCapabilityError):
-        resolve_config({**ok, "config_horiz_mixing": "2d_fixed"}, st, st, meta["dt"], 2)
-    with pytest.raises(MpasUnsupportedCapabilityError):
-        resolve_config({**ok, "config_curvature_u": True}, st, st, meta["dt"], 2)
+    # horizontal mixing (dyn_tend.py:3367-3430): "2d_fixed" takes its own coefficients, an
+    # unknown mode is a configuration error; the fixed coefficients are ignored under Smagorinsky
+    cfg = resolve_config({**ok, "horiz_mixing": "2d_fixed", "config_h_theta_eddy_visc2": 3.0,
+                          "config_h_mom_eddy_visc4": 5.0, "config_h_theta_eddy_visc4": 7.0}, st, st, meta["dt"], 2)
+    assert (cfg.mix_fixed, cfg.h_theta_eddy_visc2, cfg.h_mom_eddy_visc4, cfg.h_theta_eddy_visc4) == (1, 3.0, 5.0, 7.0)
+    assert resolve_config({**ok, "config_h_mom_eddy_visc4": 5.0}, st, st, meta["dt"], 2).mix_fixed == 0
+    with pytest.raises(MpasConfigError):
+        resolve_config({**ok, "config_horiz_mixing": "3d_smagorinsky"}, st, st, meta["dt"], 2)
+    # CAM damping (dyn_tend.py:3434-3458): on only for a positive coefficient and level count;
+    # 4 levels by default (config_defaults.py:104-105)
+    ld = float(st["nominalMinDc"])
+    cfg = resolve_config({**ok, "config_mpas_cam_coef": 0.5}, st, st, meta["dt"], 2)
+    assert cfg.cam_levels == 4 and cfg.cam_visc2 == 4.0 * 2.0833 * ld * 0.5
+    assert resolve_config({**ok, "mpas_cam_coef": 0.5, "number_cam_damping_levels": 0}, st, st,
+                          meta["dt"], 2).cam_levels == 0
+    assert resolve_config(ok, st, st, meta["dt"], 2).cam_levels == 0
+    with pytest.raises(MpasConfigError):
+        resolve_config({**ok, "config_mpas_cam_coef": 0.5, "config_number_cam_damping_levels": 99}, st, st,
+                       meta["dt"], 2)
+    # a full key name takes precedence over its short alias (config_defaults.py:745-774)
+    assert resolve_config({**ok, "epssm": 0.3, "config_epssm": 0.2}, st, st, meta["dt"], 2).epssm == 0.2
+    for key, value in (("config_curvature_u", True), ("config_v_mom_eddy_visc2", 1.0),
+                       ("config_v_theta_eddy_visc2", 1.0), ("config_mix_full", False)):
+        with pytest.raises(MpasUnsupportedCapabilityError):
+            resolve_config({**ok, key: value}, st, st, meta["dt"], 2)
     # Rayleigh damping of u: the reference's coefficient 1 / (levels * days * 86400)
     # (dyn_tend.py:10066-10072) and defaults (6 levels, 5 days; config_defaults.py:100-102)
     assert resolve_config(ok, st, st, meta["dt"], 2).rayleigh_levels == 0

@@ -1,14 +1,19 @@
 """Non-default dynamics-tendency options on the GPU.
 
-Rayleigh damping of u (``config_rayleigh_damp_u``: the top 5 levels, timescale
-0.05 days; reference ``dyn_tend.py:2942-2996``, coefficient ``:10066-10072``):
+* Rayleigh damping of u (``config_rayleigh_damp_u``: the top 5 levels, timescale
+  0.05 days; reference ``dyn_tend.py:2942-2996``, coefficient ``:10066-10072``);
+* "2d_fixed" horizontal mixing (``config_horiz_mixing``: kdiff =
+  ``config_h_theta_eddy_visc2`` everywhere, del4 coefficients
+  ``config_h_mom_eddy_visc4`` / ``config_h_theta_eddy_visc4``;
+  ``dyn_tend.py:3367-3430``);
+* CAM damping of kdiff (``config_mpas_cam_coef`` 0.02 on the top 6 levels;
+  ``dyn_tend.py:3434-3458``).
 
-* two RK3 steps against the reference's own recording of the same case
-  (``tests/golden/srk3_l2_L20_rayleigh.npz``), per field at the whole-step bar,
-  and the undamped step missing it;
-* the step's derivatives: the taped step equals the fused one bit for bit, the
-  JVP/VJP identity, and the JVP against central differences of the oracle with
-  the damping on.
+For each: two RK3 steps against the reference's own recording of the same case
+(``tests/golden/srk3_l2_L20_<option>.npz``), per field at the whole-step bar,
+and the step without the option missing it; then the step's derivatives: the
+taped step equals the fused one bit for bit, the JVP/VJP identity, and the JVP
+against central differences of the oracle with the option on.
 """
 
 from __future__ import annotations
@@ -32,13 +37,45 @@ def _config(st, meta, damp=True):
     return cfg
 
 
-def test_rayleigh_damping_whole_steps_match_reference():
+CASES = ("l2_L20_rayleigh", "l2_L20_fixedmix", "l2_L20_cam")
+# how far the step without the option lands from the recording after one step (u)
+MISS = {"l2_L20_rayleigh": 1e-6, "l2_L20_fixedmix": 1e-3, "l2_L20_cam": 1e-6}
+
+
+def _oracle_options(extra: dict) -> dict:
+    """oracle.srk3 keywords of a recorded configuration's option keys."""
+    kw, mixing = {}, {}
+    if extra.get("config_rayleigh_damp_u"):
+        kw["rayleigh_levels"] = extra["config_number_rayleigh_damp_u_levels"]
+        kw["rayleigh_timescale_days"] = extra["config_rayleigh_damp_u_timescale_days"]
+    for key, name in (("config_horiz_mixing", "horiz_mixing"), ("config_h_theta_eddy_visc2", "h_theta_eddy_visc2"),
+                      ("config_h_mom_eddy_visc4", "h_mom_eddy_visc4"),
+                      ("config_h_theta_eddy_visc4", "h_theta_eddy_visc4"),
+                      ("config_mpas_cam_coef", "cam_coef"), ("config_number_cam_damping_levels", "cam_levels")):
+        if key in extra:
+            mixing[name] = extra[key]
+    if mixing:
+        kw["mixing"] = mixing
+    return kw
+
+
+def _check_option(cfg, case: str) -> None:
+    if case == "l2_L20_rayleigh":
+        assert cfg.rayleigh_levels == 5
+    elif case == "l2_L20_fixedmix":
+        assert cfg.mix_fixed == 1 and cfg.h_theta_eddy_visc2 == 1.0e8 and cfg.cam_levels == 0
+    else:
+        assert cfg.mix_fixed == 0 and cfg.cam_levels == 6 and cfg.cam_visc2 > 0.0
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_option_whole_steps_match_reference(case):
     from mpas_dycore_pytorch_amd.time_integration import Srk3Stepper
 
-    st, refs, meta = srk3_case("l2_L20_rayleigh")
-    print(f"[options] Rayleigh damping of u: {meta['config_extra']}", flush=True)
+    st, refs, meta = srk3_case(case)
+    print(f"[options] {case}: {meta['config_extra']}", flush=True)
     stepper = Srk3Stepper(st, st, _config(st, meta), meta["dt"], device=torch.device("cuda"))
-    assert stepper.cfg.rayleigh_levels == 5
+    _check_option(stepper.cfg, case)
     worst = 0.0
     for step, ref in enumerate(refs):
         stepper.step(1)
@@ -47,21 +84,22 @@ def test_rayleigh_damping_whole_steps_match_reference():
             if k in ref:
                 err = rel_err(got[k], ref[k])
                 worst = max(worst, err)
-                assert err <= STEP_TOL, f"rayleigh step {step + 1} {k}: rel err {err:.3e}"
+                assert err <= STEP_TOL, f"{case} step {step + 1} {k}: rel err {err:.3e}"
     plain = Srk3Stepper(st, st, _config(st, meta, damp=False), meta["dt"], device=torch.device("cuda"))
     plain.step(1)
     miss = rel_err(plain.state_dict()["u"].cpu().numpy(), refs[0]["u"])
-    print(f"[options] Rayleigh: worst field error {worst:.2e}; undamped u misses the reference by {miss:.2e}",
-          flush=True)
-    assert miss > 1e-6
+    print(f"[options] {case}: worst field error {worst:.2e}; without the option u misses the reference by "
+          f"{miss:.2e}", flush=True)
+    assert miss > MISS[case]
 
 
-def test_rayleigh_damping_step_derivatives():
+@pytest.mark.parametrize("case", CASES)
+def test_option_step_derivatives(case):
     from oracle import oracle as O, srk3 as OS
     from mpas_dycore_pytorch_amd.adjoint import STEP_INPUTS, Tape, step_jvp, step_vjp, taped_step
     from mpas_dycore_pytorch_amd.time_integration import Srk3Stepper, resolve_config
 
-    st0, _, meta = srk3_case("l2_L20_rayleigh")
+    st0, _, meta = srk3_case(case)
     st = perturbed_state(st0, 31)
     dev = torch.device("cuda")
     config = _config(st, meta)
@@ -97,9 +135,7 @@ def test_rayleigh_damping_step_derivatives():
     om = O.Mesh(st)
     tn = {k: t[k].cpu().numpy() for k in keys}
     yb = {k: ybar[k].cpu().numpy() for k in outk}
-    extra = meta["config_extra"]
-    kw = {"rayleigh_levels": extra["config_number_rayleigh_damp_u_levels"],
-          "rayleigh_timescale_days": extra["config_rayleigh_damp_u_timescale_days"]}
+    kw = _oracle_options(meta["config_extra"])
 
     def f(sign, h):
         s2 = dict(st)
@@ -107,8 +143,10 @@ def test_rayleigh_damping_step_derivatives():
         y = OS.srk3(s2, om, meta["dt"], len_disp=float(st["nominalMinDc"]), **kw)
         return sum(float(np.sum(yb[k] * np.asarray(y[k]))) for k in outk)
 
-    fds = [(f(1, h) - f(-1, h)) / (2 * h) for h in (1e-6, 1e-7)]
+    # the FD noise floor is eps |f| / h: a small directional derivative (the CAM case's is
+    # ~1e-3 of the others') needs the larger steps
+    fds = [(f(1, h) - f(-1, h)) / (2 * h) for h in (1e-4, 1e-5, 1e-6, 1e-7)]
     rf = min(abs(fd - jv) / max(abs(fd), abs(jv)) for fd in fds)
-    print(f"[options] Rayleigh step: jvp {jv:.15e} vjp {vj:.15e} rel {rel:.1e}; fd {fds} rel {rf:.1e}", flush=True)
+    print(f"[options] {case} step: jvp {jv:.15e} vjp {vj:.15e} rel {rel:.1e}; fd {fds} rel {rf:.1e}", flush=True)
     assert rel <= 1e-11
     assert rf < 1e-5

@@ -105,6 +105,43 @@ def test_srk3_whole_steps_rayleigh_damping_of_u():
     assert rel_err(undamped["u"], outs[0]["u"]) > 1e-6
 
 
+def _mixing_kwargs(extra: dict) -> dict:
+    """oracle.dyn_params keywords of a recorded configuration's mixing keys."""
+    kw = {}
+    if "config_horiz_mixing" in extra:
+        kw["horiz_mixing"] = extra["config_horiz_mixing"]
+    for key, name in (("config_h_theta_eddy_visc2", "h_theta_eddy_visc2"),
+                      ("config_h_mom_eddy_visc4", "h_mom_eddy_visc4"),
+                      ("config_h_theta_eddy_visc4", "h_theta_eddy_visc4"),
+                      ("config_mpas_cam_coef", "cam_coef"), ("config_number_cam_damping_levels", "cam_levels")):
+        if key in extra:
+            kw[name] = extra[key]
+    return kw
+
+
+@pytest.mark.parametrize("case", ["l2_L20_fixedmix", "l2_L20_cam"])
+def test_srk3_whole_steps_mixing_options(case):
+    """"2d_fixed" horizontal mixing (constant kdiff, separate del4 coefficients for
+    momentum and theta) and the CAM damping of kdiff on the top 6 levels, over two
+    RK3 steps: the oracle against the reference's recordings at 1e-14.  Without the
+    option the oracle misses the recording, so each option is exercised."""
+    st, outs, meta = srk3_case(case)
+    kw = _mixing_kwargs(meta["config_extra"])
+    assert kw, meta["config_extra"]
+    mesh = O.Mesh(st)
+    cur = st
+    for step, ref in enumerate(outs):
+        cur = OS.srk3(cur, mesh, meta["dt"], len_disp=float(st["nominalMinDc"]), diagnostics_current=(step > 0),
+                      mixing=kw)
+        for k in FIELDS:
+            if k in ref:
+                err = rel_err(cur[k], ref[k])
+                assert err <= 1e-14, f"{case} step {step + 1} {k}: rel err {err:.3e}"
+    plain = OS.srk3(st, mesh, meta["dt"], len_disp=float(st["nominalMinDc"]))
+    assert rel_err(plain["u"], outs[0]["u"]) > 1e-9
+    assert rel_err(plain["theta_m"], outs[0]["theta_m"]) > 1e-12
+
+
 @pytest.mark.slow
 def test_level4_summary_from_own_cold_start():
     from mpas_dycore_pytorch_amd import initial_state as IS
