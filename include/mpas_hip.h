@@ -39,6 +39,10 @@ typedef struct mpas_hip_mesh {
     const double *rdzw, *rdzu, *fzm, *fzp, *rdnw, *fnm, *fnp;
     double cf1, cf2, cf3, reserved1;
     const double *rho_base, *rtheta_base, *exner_base;
+    /* Coriolis factors of the curvature terms (dyn_tend.py:2494-2502, :4258-4265), read only
+     * with curvature on: (2 omega cos angleEdge) cos latEdge per edge and 2 omega cos latCell
+     * per cell, omega = 7.29212e-5 */
+    const double *curvEdge, *curvCell;
 } mpas_hip_mesh;
 
 /* Prognostic state and the diagnostics carried between large steps
@@ -90,6 +94,7 @@ typedef struct mpas_hip_config {
      * [config_number_cam_damping_levels] > 0, else cam_levels = 0 */
     int32_t cam_levels;           /* [config_number_cam_damping_levels] */
     double cam_visc2;             /* 4 * 2.0833 * len_disp * [config_mpas_cam_coef] */
+    int32_t curvature;            /* [config_curvature_u]: curvature terms of u and w */
 } mpas_hip_config;
 
 /* Number of doubles of workspace needed for one large step. */
@@ -150,6 +155,10 @@ typedef struct mpas_hip_dyn_params {
     double kdiff_fixed;
     int32_t cam_levels;
     double cam_visc2;
+    /* curvature terms (dyn_tend.py:2471-2505, :4241-4267): tend_u -= curvEdge rho_edge w_avg +
+     * u w_avg rho_edge / r_earth; the w advective sum gains rho (ur^2 + vr^2) / r_earth +
+     * curvCell ur rho at interfaces (cell winds of the step's input state), r_earth = 6371229 m */
+    int32_t curvature;
 } mpas_hip_dyn_params;
 
 /* Scalars of atm_advance_acoustic_step_work (acoustic_step.py:1297-1340). */
@@ -179,7 +188,8 @@ int mpas_hip_solve_diagnostics(const mpas_hip_mesh *m, double dt, double apvm, i
                                double *pv_edge, double *scratch, void *stream);
 /* atm_compute_dyn_tend_work, dyn_tend.py:10836.  inputs: u, v, w, ru, rw,
  * ru_save, rw_save, theta_m, theta_m_save, rho_zz, rho_edge, divergence,
- * vorticity, ke, pv_edge, pp, rr_save, cqw, cqu, rt_diabatic_tend, qtot.
+ * vorticity, ke, pv_edge, pp, rr_save, cqw, cqu, rt_diabatic_tend, qtot, ur_cell,
+ * vr_cell (the last two read only with p->curvature).
  * carried (written at rk_step 1, read later): tend_rho, tend_u_euler,
  * tend_w_euler, tend_theta_euler, kdiff. */
 int mpas_hip_dyn_tend(const mpas_hip_mesh *m, const mpas_hip_dyn_params *p,
@@ -190,6 +200,7 @@ int mpas_hip_dyn_tend(const mpas_hip_mesh *m, const mpas_hip_dyn_params *p,
                       const double *ke, const double *pv_edge, const double *pp,
                       const double *rr_save, const double *cqw, const double *cqu,
                       const double *rt_diabatic_tend, const double *qtot,
+                      const double *ur_cell, const double *vr_cell,
                       double *tend_rho, double *tend_u_euler, double *tend_w_euler,
                       double *tend_theta_euler, double *kdiff,
                       double *tend_u, double *tend_w, double *tend_theta, double *h_divergence,

@@ -30,6 +30,7 @@ MESH_REAL_FIELDS_A = ("edgesOnCell_sign", "edgesOnVertex_sign", "weightsOnEdge",
                       "zb_cell", "zb3_cell", "rdzw", "rdzu", "fzm", "fzp", "rdnw", "fnm", "fnp")
 MESH_SCALAR_FIELDS = ("cf1", "cf2", "cf3", "reserved1")
 MESH_REAL_FIELDS_B = ("rho_base", "rtheta_base", "exner_base")
+MESH_REAL_FIELDS_C = ("curvEdge", "curvCell")
 
 STATE_FIELDS = ("u", "w", "theta_m", "rho_zz", "scalars", "rt_diabatic_tend",
                 "rho_p", "rtheta_p", "exner", "pressure_p", "ru", "rw",
@@ -43,7 +44,7 @@ STATE_FIELDS = ("u", "w", "theta_m", "rho_zz", "scalars", "rt_diabatic_tend",
 class MeshStruct(C.Structure):
     _fields_ = ([(n, C.c_int32) for n in MESH_INT_FIELDS] + [(n, _dp) for n in MESH_INDEX_FIELDS]
                 + [(n, _dp) for n in MESH_REAL_FIELDS_A] + [(n, C.c_double) for n in MESH_SCALAR_FIELDS]
-                + [(n, _dp) for n in MESH_REAL_FIELDS_B])
+                + [(n, _dp) for n in MESH_REAL_FIELDS_B] + [(n, _dp) for n in MESH_REAL_FIELDS_C])
 
 
 class StateStruct(C.Structure):
@@ -62,7 +63,8 @@ class ConfigStruct(C.Structure):
                 ("moist_end", C.c_int32), ("diagnostics_current", C.c_int32), ("order", C.c_int32),
                 ("rayleigh_levels", C.c_int32), ("rayleigh_coef_inverse", C.c_double),
                 ("mix_fixed", C.c_int32), ("h_theta_eddy_visc2", C.c_double), ("h_mom_eddy_visc4", C.c_double),
-                ("h_theta_eddy_visc4", C.c_double), ("cam_levels", C.c_int32), ("cam_visc2", C.c_double)]
+                ("h_theta_eddy_visc4", C.c_double), ("cam_levels", C.c_int32), ("cam_visc2", C.c_double),
+                ("curvature", C.c_int32)]
 
 
 class DynParamsStruct(C.Structure):
@@ -71,7 +73,7 @@ class DynParamsStruct(C.Structure):
                 ("h_theta_eddy_visc4", C.c_double), ("del4u_div_factor", C.c_double),
                 ("prandtl_inv", C.c_double), ("rayleigh_levels", C.c_int32), ("rayleigh_coef_inverse", C.c_double),
                 ("mix_fixed", C.c_int32), ("kdiff_fixed", C.c_double), ("cam_levels", C.c_int32),
-                ("cam_visc2", C.c_double)]
+                ("cam_visc2", C.c_double), ("curvature", C.c_int32)]
 
 
 class AcousticParamsStruct(C.Structure):
@@ -99,7 +101,7 @@ SIGNATURES = {
     "mpas_hip_moist_coefficients": (C.c_int, [_PM, _dp, _I32, _I32, _I32, _dp, _dp, _dp, _vp]),
     "mpas_hip_vert_imp_coefs": (C.c_int, [_PM, _F64, _F64] + [_dp] * 13 + [_vp]),
     "mpas_hip_solve_diagnostics": (C.c_int, [_PM, _F64, _F64, _I32] + [_dp] * 10 + [_vp]),
-    "mpas_hip_dyn_tend": (C.c_int, [_PM, C.POINTER(DynParamsStruct)] + [_dp] * 32 + [_vp]),
+    "mpas_hip_dyn_tend": (C.c_int, [_PM, C.POINTER(DynParamsStruct)] + [_dp] * 34 + [_vp]),
     "mpas_hip_smlstep_pert": (C.c_int, [_PM, _dp, _dp, _vp]),
     "mpas_hip_acoustic_step": (C.c_int, [_PM, C.POINTER(AcousticParamsStruct)] + [_dp] * 26 + [_vp]),
     "mpas_hip_divergence_damping": (C.c_int, [_PM, _F64, _F64, _F64, _dp, _dp, _dp, _dp, _vp]),

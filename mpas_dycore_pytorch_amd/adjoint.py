@@ -342,7 +342,8 @@ def vjp_scratch(dm: DeviceMesh, n_scalars: int) -> torch.Tensor:
 # ---------------------------------------------------------------------------
 DIAG_KEYS = ("h_edge", "v", "divergence", "vorticity", "ke", "pv_edge")
 STEP_INPUTS = ("u", "v", "w", "ru", "rw", "theta_m", "rho_zz", "rho_edge", "divergence", "vorticity", "ke",
-               "pv_edge", "pressure_p", "exner", "rtheta_p", "rho_p", "scalars", "rt_diabatic_tend")
+               "pv_edge", "pressure_p", "exner", "rtheta_p", "rho_p", "scalars", "rt_diabatic_tend", "ur_cell",
+               "vr_cell")
 
 
 def _explicit_weights(rk_step: int, order: int = 3) -> tuple[float, float]:
@@ -423,14 +424,15 @@ def taped_step(dm: DeviceMesh, state: Mapping[str, torch.Tensor], cfg: Any, dt: 
                       "theta_m_save": theta_m_save, "rho_zz": st["rho_zz"], "rho_edge": st["rho_edge"],
                       "divergence": st["divergence"], "vorticity": st["vorticity"], "ke": st["ke"],
                       "pv_edge": st["pv_edge"], "pp": st["pressure_p"], "rr_save": rr_save,
-                      "cqw": moist["cqw"], "cqu": moist["cqu"], "rt_diabatic_tend": rt_diab, "qtot": moist["qtot"]}
+                      "cqw": moist["cqw"], "cqu": moist["cqu"], "rt_diabatic_tend": rt_diab, "qtot": moist["qtot"],
+                      "ur_cell": st["ur_cell"], "vr_cell": st["vr_cell"]}
             params = K.dyn_params(rk, dt, len_disp, c_s=cfg.c_s, visc4_2dsmag=cfg.visc4_2dsmag,
                                   coef_3rd_order=coef3, del4u_div_factor=cfg.del4u_div_factor,
                                   rayleigh_levels=cfg.rayleigh_levels,
                                   rayleigh_coef_inverse=cfg.rayleigh_coef_inverse, mix_fixed=bool(cfg.mix_fixed),
                                   h_theta_eddy_visc2=cfg.h_theta_eddy_visc2, h_mom_eddy_visc4=cfg.h_mom_eddy_visc4,
                                   h_theta_eddy_visc4=cfg.h_theta_eddy_visc4, cam_levels=cfg.cam_levels,
-                                  cam_visc2=cfg.cam_visc2)
+                                  cam_visc2=cfg.cam_visc2, curvature=bool(cfg.curvature))
             if rk == 1:
                 carried_in = {k: z(*t.shape) for k, t in
                               zip(K.DYN_CARRIED, (st["rho_zz"], st["u"], st["w"], st["rho_zz"], st["rho_zz"]))}

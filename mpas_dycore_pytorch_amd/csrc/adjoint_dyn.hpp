@@ -30,7 +30,7 @@ namespace adj {
 namespace dyn {
 enum {
     u, v, w, ru, rw, ru_save, rw_save, theta_m, theta_m_save, rho_zz, rho_edge, divergence, vorticity, ke,
-    pv_edge, pp, rr_save, cqw, cqu, rt_diabatic_tend, qtot,
+    pv_edge, pp, rr_save, cqw, cqu, rt_diabatic_tend, qtot, ur_cell, vr_cell,
     tend_rho_in, tend_u_euler_in, tend_w_euler_in, tend_theta_euler_in, kdiff_in,
     tend_rho, tend_u_euler, tend_w_euler, tend_theta_euler, kdiff, tend_u, tend_w, tend_theta, h_divergence,
     rthdynten, N
@@ -47,8 +47,9 @@ struct DynAdjScratch {
     // what edge e owes its cells and vertices (A3 -> A3c): x[0] ke (c1 +, c2 -),
     // x[1] h_divergence, x[2] rw (interface k), x[3] pp (c1 +, c2 -; rk 1) or
     // theta_m_save (rk > 1), x[4] dpdz, x[5] kdiff, x[6] delsq_div (c1 +, c2 -),
-    // x[7] delsq_vort (v1 -, v2 +)
-    double *x[8];
+    // x[7] delsq_vort (v1 -, v2 +), x[8] w at interfaces k and k + 1 of both
+    // cells (the curvature term's w_avg / 4)
+    double *x[9];
     double *g_dd, *g_dof;  // deformation adjoints of each cell (A6 -> A7)
 };
 
@@ -225,7 +226,34 @@ __device__ __forceinline__ void dyn_a1(const mpas_hip_mesh &m, const mpas_hip_dy
             r_0 += 0.5 * d[4] * g;
         }
     }
+    // curvature of w at interface k (1..L-1), added to the advective sum before
+    // its 1/area scaling: rh (ur^2 + vr^2) / r_earth + curvCell ur rh, with ur, vr
+    // and rh interpolated from levels k and k-1
+    double cz_0 = 0.0, cz_m1 = 0.0, cu_0 = 0.0, cu_m1 = 0.0, cv_0 = 0.0, cv_m1 = 0.0;
+    if (P.curvature && k >= 1 && k < L) {
+        const double gh = ia * bar_of(B, tend_w, i);
+        const double *ur = F[ur_cell] + oc, *vr = F[vr_cell] + oc, *rz = F[rho_zz] + oc;
+        const double uri = fzm[k] * ur[k] + fzp[k] * ur[k - 1], vri = fzm[k] * vr[k] + fzp[k] * vr[k - 1];
+        const double rh = rz[k] * fzm[k] + rz[k - 1] * fzp[k];
+        const double grh = gh * ((uri * uri + vri * vri) / kEarthRadius + m.curvCell[c] * uri);
+        const double gur = gh * (2.0 * uri * rh / kEarthRadius + m.curvCell[c] * rh);
+        const double gvr = gh * (2.0 * vri * rh / kEarthRadius);
+        cz_0 = fzm[k] * grh;
+        cz_m1 = fzp[k] * grh;
+        cu_0 = fzm[k] * gur;
+        cu_m1 = fzp[k] * gur;
+        cv_0 = fzm[k] * gvr;
+        cv_m1 = fzp[k] * gvr;
+    }
     // ------------------------------------------------ exchange, then one write per element
+    const double g_cz = cz_0 + from_lane(cz_m1, 1, k, L);
+    const double g_cu = cu_0 + from_lane(cu_m1, 1, k, L);
+    const double g_cv = cv_0 + from_lane(cv_m1, 1, k, L);
+    if (P.curvature && k < L) {
+        add_own(B, rho_zz, oc + k, g_cz);
+        add_own(B, ur_cell, oc + k, g_cu);
+        add_own(B, vr_cell, oc + k, g_cv);
+    }
     const double g_th = th_0 + from_lane(th_m1, 1, k, L) + from_lane(th_m2, 2, k, L) + from_lane(th_p1, -1, k, L);
     const double g_ts = ts_0 + from_lane(ts_m1, 1, k, L);
     const double g_pp = pp_0 + from_lane(pp_m1, 1, k, L);
@@ -479,6 +507,20 @@ __device__ __forceinline__ void dyn_a3_cells(const mpas_hip_mesh &m, const mpas_
     add_own(B, ke, o, gke);
     add_own(B, h_divergence, o, ghd);
     if (k >= 1) add_own(B, rw, (long long)c * Lw + k, grw);
+    if (P.curvature) {
+        // w_avg of each edge reads interfaces k and k + 1 of both its cells:
+        // interface k collects the shares of levels k and k - 1, lane L - 1 also
+        // interface L's
+        double gcw = 0.0, gtop = 0.0;
+        for (int j = 0; j < n; ++j) {
+            const long long i = (long long)m.edgesOnCell[c * ME + j] * L + k;
+            gcw += S.x[8][i];
+            if (k >= 1) gcw += S.x[8][i - 1];
+            if (k == L - 1) gtop += S.x[8][i];
+        }
+        add_own(B, w, (long long)c * Lw + k, gcw);
+        if (k == L - 1) add_own(B, w, (long long)c * Lw + L, gtop);
+    }
     if (rk1) {
         add_own(B, pp, o, g3);
         S.g_dpdz[o] += gdp;

@@ -75,7 +75,7 @@ struct KArgs {
 inline size_t dyn_adj_doubles(const mpas_hip_mesh &m) {
     const size_t L = m.nLevels, C = (m.nCells + 1) * L, E = (m.nEdges + 1) * L, V = (m.nVertices + 1) * L,
                  W = (m.nCells + 1) * (L + 1);
-    return C + E + C + V + C + W + C + W + C + C + V + E + 3 * E + 8 * E + 2 * C;
+    return C + E + C + V + C + W + C + W + C + C + V + E + 3 * E + 9 * E + 2 * C;
 }
 inline DynAdjScratch carve_dyn(const mpas_hip_mesh &m, double *p) {
     const size_t L = m.nLevels, C = (m.nCells + 1) * L, E = (m.nEdges + 1) * L, V = (m.nVertices + 1) * L,
@@ -96,7 +96,7 @@ inline DynAdjScratch carve_dyn(const mpas_hip_mesh &m, double *p) {
     S.g_tf = p; p += E;
     S.g_wf = p; p += E;
     S.g_q = p; p += E;
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < 9; ++j) {
         S.x[j] = p;
         p += E;
     }

@@ -87,6 +87,9 @@ struct SlotSum {
 using DynParams = mpas_hip_dyn_params;
 using AcousticParams = mpas_hip_acoustic_params;
 
+// Earth radius of the curvature terms (the reference's constants.py:84)
+constexpr double kEarthRadius = 6371229.0;
+
 // CAM damping floor of kdiff at level k (one of the top P.cam_levels levels):
 // cam_visc2 (1 - (nLevels - (k + 1)) / cam_levels), dyn_tend.py:3442-3446
 __device__ __forceinline__ double cam_kdiff(const DynParams &P, int L, int k) {
@@ -239,6 +242,7 @@ struct DynInputs {
     const double *pp, *rr_save, *cqw;
     const hreal *cqu;
     const double *rt_diabatic_tend, *qtot;
+    const double *ur_cell, *vr_cell;  // read only with the curvature terms on
 };
 struct DynCarried {
     double *tend_rho;

@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
     // ---- u tendency
     const double re = F[rho_edge][i];
     double g_re = 0.0, g_u_own = 0.0, g_pv_own = 0.0;
-    double x_ke, x_hd;
+    double x_ke, x_hd, x_cw = 0.0;
     {
         const double pvs = F[pv_edge][i];
         const double gq = re * gtu;
@@ -218,6 +218,15 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
             const double damp = (((double)k + 1.0) - (double)k0) * P.rayleigh_coef_inverse;
             g_re += -(u0 * damp) * gtu;
             g_u_own += -(re * damp) * gtu;
+        }
+        // curvature: tend_u -= curvEdge re w_avg + u w_avg re / r_earth
+        if (P.curvature) {
+            const double *w1 = F[w] + (long long)c1 * Lw, *w2 = F[w] + (long long)c2 * Lw;
+            const double wa = 0.25 * (((w1[k] + w1[k + 1]) + w2[k]) + w2[k + 1]);
+            const double ce = m.curvEdge[e], ir = 1.0 / kEarthRadius;
+            g_re += -(ce * wa + u0 * wa * ir) * gtu;
+            g_u_own += -(wa * re * ir) * gtu;
+            x_cw = -0.25 * (ce * re + u0 * re * ir) * gtu;
         }
         x_hd = -u0 * 0.5 * gtu;
         g_pv_own += gpe;
@@ -299,6 +308,7 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
     S.x[0][i] = x_ke;
     S.x[1][i] = x_hd;
     S.x[2][i] = (k >= 1) ? x_rw : 0.0;
+    if (P.curvature) S.x[8][i] = x_cw;
     if (rk1) {
         S.x[3][i] = x_pp;
         S.x[4][i] = x_dp;

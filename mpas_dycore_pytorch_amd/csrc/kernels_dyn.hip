@@ -246,7 +246,8 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_a(long long nb, mpa
 // P4: normal-velocity tendency (vertical flux, PV flux, KE gradient,
 // divergence coupling); first stage also PGF and del2 mixing into tend_u_euler.
 __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_a(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ u,
-                              const double *__restrict__ rw, const hreal *__restrict__ pv_edge,
+                              const double *__restrict__ rw, const double *__restrict__ w,
+                              const hreal *__restrict__ pv_edge,
                               const double *__restrict__ ke, const double *__restrict__ h_div,
                               const hreal *__restrict__ rho_edge, const double *__restrict__ pp,
                               const double *__restrict__ dpdz, const hreal *__restrict__ cqu,
@@ -321,6 +322,13 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_a(long long nb, mpa
                                      (freal)ue[k] * (freal)0.5 * hsum);
         if (lane >= L) return;
         double tu = vert + base;
+        if (P.curvature) {
+            // curvature (dyn_tend.py:2471-2505): tu -= curvEdge rho_edge w_avg + u w_avg rho_edge / r_earth
+            const double *w1 = w + c1 * Lw, *w2 = w + c2 * Lw;
+            const double w_avg = 0.25 * (((w1[k] + w1[k + 1]) + w2[k]) + w2[k + 1]);
+            const double re = (double)rho_edge[i];
+            tu = tu - (m.curvEdge[e] * re * w_avg + ue[k] * w_avg * re * (1.0 / kEarthRadius));
+        }
         // Rayleigh damping of u on the top levels, before the Euler part (dyn_tend.py:2969-2994)
         const int k0 = L - P.rayleigh_levels;
         if (P.rayleigh_levels > 0 && k >= k0) {
@@ -502,7 +510,8 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_b(long long nb, mpa
                               const double *__restrict__ rw_save, const double *__restrict__ tpert,
                               const double *__restrict__ w,
                               const double *__restrict__ theta_m, const double *__restrict__ theta_m_save,
-                              const double *__restrict__ rho_zz, const double *__restrict__ pp,
+                              const double *__restrict__ rho_zz, const double *__restrict__ ur_cell,
+                              const double *__restrict__ vr_cell, const double *__restrict__ pp,
                               const double *__restrict__ cqw, const double *__restrict__ rt_diab,
                               const double *__restrict__ dpdz, const double *__restrict__ delsq_w,
                               const double *__restrict__ delsq_theta, const double *__restrict__ tend_rho,
@@ -589,6 +598,15 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_b(long long nb, mpa
             for (int j = 0; j < kMaxEdges; ++j) {
                 const double cc = -sg[j] * fv[j];
                 hadv = (j < n) ? ((j == 0) ? cc : hadv + cc) : hadv;
+            }
+            if (P.curvature) {
+                // curvature (dyn_tend.py:4241-4267), added to the advective sum
+                // before its 1/area scaling
+                const double ur = fzm[k] * ur_cell[oc + k] + fzp[k] * ur_cell[oc + k - 1];
+                const double vr = fzm[k] * vr_cell[oc + k] + fzp[k] * vr_cell[oc + k - 1];
+                const double rh = rho_zz[oc + k] * fzm[k] + rho_zz[oc + k - 1] * fzp[k];
+                hadv = hadv + rh * (ur * ur + vr * vr) / kEarthRadius;
+                hadv = hadv + m.curvCell[c] * ur * rh;
             }
             double eu = tend_w_euler[i];
             if (rk1) {
@@ -698,7 +716,7 @@ void launch_dyn_tend(const mpas_hip_mesh &m, const DynParams &P, const DynInputs
     // pass completes tend_u itself and the del4/finish pass is skipped.
     const int do_del4 = (rk1 && P.h_mom_eddy_visc4 > 0.0) ? 1 : 0;
     MPAS_WAVE_LAUNCH(k_dyn_edges_a, nE + 1, s, m, P, in.u,
-                       in.rw, in.pv_edge, in.ke, out.h_divergence, in.rho_edge, in.pp, sc.dpdz, in.cqu, in.divergence,
+                       in.rw, in.w, in.pv_edge, in.ke, out.h_divergence, in.rho_edge, in.pp, sc.dpdz, in.cqu, in.divergence,
                        in.vorticity, io.kdiff, out.tend_u, io.tend_u_euler, sc.delsq_u, do_del4 ? 0 : 1);
     if (do_del4) {
         MPAS_WAVE_LAUNCH(k_dyn_del4_vc, nC > nV ? nC : nV, s, m, sc.delsq_u, sc.delsq_vort, sc.delsq_div);
@@ -708,7 +726,8 @@ void launch_dyn_tend(const mpas_hip_mesh &m, const DynParams &P, const DynInputs
     MPAS_WAVE_LAUNCH(k_dyn_edge_adv, nE, s, m, P, in.ru, in.ru_save,
                 in.w, in.theta_m, in.theta_m_save, sc.wflux, sc.tflux, sc.tpert);
     MPAS_WAVE_LAUNCH(k_dyn_cells_b, nC + 1, s, m, P,
-                       sc.wflux, in.rw, sc.tflux, in.rw_save, sc.tpert, in.w, in.theta_m, in.theta_m_save, in.rho_zz, in.pp,
+                       sc.wflux, in.rw, sc.tflux, in.rw_save, sc.tpert, in.w, in.theta_m, in.theta_m_save, in.rho_zz,
+                       in.ur_cell, in.vr_cell, in.pp,
                        in.cqw, in.rt_diabatic_tend, sc.dpdz, sc.delsq_w, sc.delsq_theta, io.tend_rho,
                        io.tend_w_euler, io.tend_theta_euler, out.tend_w, out.tend_theta, out.rthdynten);
 }

@@ -251,6 +251,15 @@ static StepState step_state(St *p) {
                      p->uReconstructZonal, p->uReconstructMeridional, p->ruAvg, p->wwAvg};
 }
 
+// The curvature terms read the mesh's Coriolis factors and the cell winds.
+static int validate_curvature(const mpas_hip_mesh *m, int curvature, const void *ur_cell, const void *vr_cell) {
+    if (curvature && (!m->curvEdge || !m->curvCell || !ur_cell || !vr_cell)) {
+        set_error("curvature terms need the mesh's curvEdge/curvCell and the state's ur_cell/vr_cell");
+        return 1;
+    }
+    return 0;
+}
+
 static int validate_mesh(const mpas_hip_mesh *m) {
     if (!m) {
         set_error("null mesh");
@@ -494,12 +503,13 @@ int mpas_hip_dyn_tend(const mpas_hip_mesh *m, const mpas_hip_dyn_params *p, cons
                       const double *theta_m, const double *theta_m_save, const double *rho_zz, const double *rho_edge,
                       const double *divergence, const double *vorticity, const double *ke, const double *pv_edge,
                       const double *pp, const double *rr_save, const double *cqw, const double *cqu,
-                      const double *rt_diabatic_tend, const double *qtot, double *tend_rho, double *tend_u_euler,
+                      const double *rt_diabatic_tend, const double *qtot, const double *ur_cell,
+                      const double *vr_cell, double *tend_rho, double *tend_u_euler,
                       double *tend_w_euler, double *tend_theta_euler, double *kdiff, double *tend_u, double *tend_w,
                       double *tend_theta, double *h_divergence, double *rthdynten, double *scratch, void *stream) {
-    if (validate_mesh(m)) return 1;
+    if (validate_mesh(m) || validate_curvature(m, p->curvature, ur_cell, vr_cell)) return 1;
     DynInputs in{u, v, w, ru, rw, ru_save, rw_save, theta_m, theta_m_save, rho_zz, rho_edge, divergence, vorticity,
-                 ke, pv_edge, pp, rr_save, cqw, cqu, rt_diabatic_tend, qtot};
+                 ke, pv_edge, pp, rr_save, cqw, cqu, rt_diabatic_tend, qtot, ur_cell, vr_cell};
     DynCarried io{tend_rho, tend_u_euler, tend_w_euler, tend_theta_euler, kdiff};
     DynOutputs out{tend_u, tend_w, tend_theta, h_divergence, rthdynten};
     launch_dyn_tend(*m, *p, in, io, out, scratch, STREAM(stream));
@@ -617,7 +627,7 @@ int mpas_hip_history_fields(int32_t nCells, int32_t nLevels, int32_t nScalars, c
 // ---------------------------------------------------------------------------
 static int srk3_step_impl(const mpas_hip_mesh *mesh, StepState *st, const mpas_hip_workspace *work,
                           const mpas_hip_config *cfg, hipStream_t s) {
-    if (validate_mesh(mesh)) return 1;
+    if (validate_mesh(mesh) || validate_curvature(mesh, cfg->curvature, st->ur_cell, st->vr_cell)) return 1;
     const mpas_hip_mesh &m = *mesh;
     const int S = cfg->nScalars;
     if (validate_moist(S, cfg->moist_start, cfg->moist_end)) return 1;
@@ -685,6 +695,7 @@ static int srk3_step_impl(const mpas_hip_mesh *mesh, StepState *st, const mpas_h
         P.kdiff_fixed = cfg->h_theta_eddy_visc2;
         P.cam_levels = cfg->cam_levels;
         P.cam_visc2 = cfg->cam_visc2;
+        P.curvature = cfg->curvature ? 1 : 0;
     }
 
     for (int sub = 1; sub <= cfg->split; ++sub) {
@@ -707,7 +718,7 @@ static int srk3_step_impl(const mpas_hip_mesh *mesh, StepState *st, const mpas_h
                           w.ru_save,      w.rw_save,      st->theta_m,    w.theta_m_init, st->rho_zz,
                           st->rho_edge,   st->divergence, st->vorticity,  st->ke,      st->pv_edge,
                           st->pressure_p, w.rho_p_save,   w.cqw,          w.cqu,       st->rt_diabatic_tend,
-                          w.qtot};
+                          w.qtot,         st->ur_cell,    st->vr_cell};
             DynCarried dio{w.tend_rho, w.tend_u_euler, w.tend_w_euler, w.tend_theta_euler, w.kdiff};
             DynOutputs dout{w.tend_u, w.tend_w, w.tend_theta, w.h_divergence, w.rthdynten};
             launch_dyn_tend(m, P, din, dio, dout, w.scratch, s);

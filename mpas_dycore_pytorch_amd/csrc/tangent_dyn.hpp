@@ -258,6 +258,16 @@ __device__ __forceinline__ void dyn_j4(const mpas_hip_mesh &m, const mpas_hip_dy
     } else {
         dtue = dot_of(T, tend_u_euler_in, i);
     }
+    // curvature: tend_u -= curvEdge re w_avg + u w_avg re / r_earth
+    double dcurv = 0.0;
+    if (P.curvature) {
+        const long long w1 = (long long)c1 * Lw + k, w2 = (long long)c2 * Lw + k;
+        const double wa = 0.25 * (((F[w][w1] + F[w][w1 + 1]) + F[w][w2]) + F[w][w2 + 1]);
+        const double dwa =
+            0.25 * (((dot_of(T, w, w1) + dot_of(T, w, w1 + 1)) + dot_of(T, w, w2)) + dot_of(T, w, w2 + 1));
+        const double ce = m.curvEdge[e], ir = 1.0 / kEarthRadius;
+        dcurv = ce * (dre * wa + re * dwa) + ((due * wa + ue * dwa) * re + ue * wa * dre) * ir;
+    }
     // Rayleigh damping of u on the top levels: tend_u -= rho_edge u damp(k)
     double dray = 0.0;
     const int k0 = L - P.rayleigh_levels;
@@ -329,7 +339,7 @@ __device__ __forceinline__ void dyn_j4(const mpas_hip_mesh &m, const mpas_hip_dy
         g_wf = drue * fs + rue * dfs;
     }
     dot_put(T, tend_u_euler, i, dtue);
-    dot_put(T, tend_u, i, dvert + dbase - dray + dtue);
+    dot_put(T, tend_u, i, dvert + dbase - dcurv - dray + dtue);
     S.g_tf[i] = g_tf;
     S.g_wf[i] = g_wf;
 }
@@ -430,6 +440,19 @@ __device__ __forceinline__ void dyn_j5(const mpas_hip_mesh &m, const mpas_hip_dy
         for (int j = 0; j < n; ++j) {
             const int e = m.edgesOnCell[c * ME + j];
             dh -= m.edgesOnCell_sign[c * ME + j] * S.g_wf[(long long)e * L + k];
+        }
+        if (P.curvature) {
+            // curvature: rh (ur^2 + vr^2) / r_earth + curvCell ur rh in the advective sum
+            const double fk = m.fzm[k], gk = m.fzp[k];
+            const long long a0 = oc + k, a1 = oc + k - 1;
+            const double ur = fk * F[ur_cell][a0] + gk * F[ur_cell][a1];
+            const double vr = fk * F[vr_cell][a0] + gk * F[vr_cell][a1];
+            const double rh = F[rho_zz][a0] * fk + F[rho_zz][a1] * gk;
+            const double dur = fk * dot_of(T, ur_cell, a0) + gk * dot_of(T, ur_cell, a1);
+            const double dvr = fk * dot_of(T, vr_cell, a0) + gk * dot_of(T, vr_cell, a1);
+            const double drh = dot_of(T, rho_zz, a0) * fk + dot_of(T, rho_zz, a1) * gk;
+            dh += (drh * (ur * ur + vr * vr) + rh * 2.0 * (ur * dur + vr * dvr)) / kEarthRadius +
+                  m.curvCell[c] * (dur * rh + ur * drh);
         }
         const double ddelta =
             (k == L - 1) ? -dyn_dwdwz(m, F, T, c, L - 2) : dyn_dwdwz(m, F, T, c, k) - dyn_dwdwz(m, F, T, c, k - 1);

@@ -29,6 +29,7 @@ INDEX_TARGET = {
 }
 COUNT_FIELDS = ("nEdgesOnCell", "nEdgesOnEdge", "nAdvCellsForEdge")
 BASE_FIELDS = ("rho_base", "rtheta_base", "exner_base")
+OMEGA = 7.29212e-5  # rad/s, the reference's rotation rate (constants.py:88)
 
 
 def to_numpy(v: Any) -> np.ndarray:
@@ -130,6 +131,21 @@ class DeviceMesh:
         s.flatTerrain = 1 if flat else 0
         for name in ("cf1", "cf2", "cf3"):
             setattr(s, name, float(to_numpy(_get(mesh, state, name))))
+        # Coriolis factors of the curvature terms, formed as the reference forms them
+        # ((2 omega cos angleEdge) cos latEdge; 2 omega cos latCell: dyn_tend.py:2494-2502,
+        # :4258-4265); a mesh without angleEdge/latEdge cannot run with curvature on
+        two_omega = 2.0 * OMEGA
+        self.has_edge_angles = "angleEdge" in mesh and "latEdge" in mesh
+        if self.has_edge_angles:
+            ce = two_omega * np.cos(to_numpy(mesh["angleEdge"]).astype(np.float64)) * np.cos(
+                to_numpy(mesh["latEdge"]).astype(np.float64))
+        else:
+            ce = np.zeros(ne + 1)
+        cc = two_omega * np.cos(to_numpy(_get(mesh, state, "latCell")).astype(np.float64))
+        for name, a in (("curvEdge", ce), ("curvCell", cc)):
+            t = torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=device)
+            self.t[name] = t
+            setattr(s, name, t.data_ptr())
         self.s = s
         self._base_src: dict[str, Any] = {}
         self.set_base(mesh, state)

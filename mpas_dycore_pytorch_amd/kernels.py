@@ -20,7 +20,7 @@ from .device import DeviceMesh, stream_handle
 
 DYN_INPUTS = ("u", "v", "w", "ru", "rw", "ru_save", "rw_save", "theta_m", "theta_m_save", "rho_zz",
               "rho_edge", "divergence", "vorticity", "ke", "pv_edge", "pp", "rr_save", "cqw", "cqu",
-              "rt_diabatic_tend", "qtot")
+              "rt_diabatic_tend", "qtot", "ur_cell", "vr_cell")
 DYN_CARRIED = ("tend_rho", "tend_u_euler", "tend_w_euler", "tend_theta_euler", "kdiff")
 DYN_OUTPUTS = ("tend_u", "tend_w", "tend_theta", "h_divergence", "rthdynten")
 ACOUSTIC_IN = ("cofrz", "cofwr", "cofwz", "coftz", "cofwt", "a_tri", "alpha_tri", "gamma_tri",
@@ -70,7 +70,7 @@ def dyn_params(rk_step: int, dt: float, len_disp: float, c_s: float = 0.125, vis
                prandtl: float = 1.0, rayleigh_levels: int = 0,
                rayleigh_coef_inverse: float = 0.0, mix_fixed: bool = False, h_theta_eddy_visc2: float = 0.0,
                h_mom_eddy_visc4: float = 0.0, h_theta_eddy_visc4: float = 0.0, cam_levels: int = 0,
-               cam_visc2: float = 0.0) -> N.DynParamsStruct:
+               cam_visc2: float = 0.0, curvature: bool = False) -> N.DynParamsStruct:
     """Scalar set-up of the dynamics tendency (dyn_tend.py:3367-3458, :9616-9626).
 
     ``mix_fixed`` selects the "2d_fixed" horizontal mixing: kdiff is
@@ -79,8 +79,11 @@ def dyn_params(rk_step: int, dt: float, len_disp: float, c_s: float = 0.125, vis
     CAM damping floors kdiff on the top ``cam_levels`` levels (0: off) at
     ``cam_visc2 * (1 - (nLevels - (k + 1)) / cam_levels)``.  The Rayleigh
     damping of u acts on ``rayleigh_levels`` top levels (0: off) with its
-    coefficient 1 / (levels * timescale) (dyn_tend.py:10066-10072)."""
+    coefficient 1 / (levels * timescale) (dyn_tend.py:10066-10072).  ``curvature``
+    adds the curvature terms of u and w (config_curvature_u, dyn_tend.py:2471-2505,
+    :4241-4267), which read the ``ur_cell`` / ``vr_cell`` inputs."""
     p = N.DynParamsStruct()
+    p.curvature = 1 if curvature else 0
     p.rk_step = int(rk_step)
     p.dt = float(dt)
     p.coef_3rd_order = float(coef_3rd_order)
@@ -150,8 +153,10 @@ def solve_diagnostics(dm: DeviceMesh, dt: float, apvm: float, rk_step, u, h, v_i
 
 
 def dyn_tend(dm: DeviceMesh, params: N.DynParamsStruct, fields: Mapping, carried: Mapping | None = None) -> dict:
-    ins = [_d(fields[k], dm) for k in DYN_INPUTS]
     nc, ne, L = dm.nc, dm.ne, dm.L
+    # the cell winds are read only by the curvature terms
+    ins = [_d(fields[k], dm) if (k in fields or k not in ("ur_cell", "vr_cell")) else _z(dm, nc + 1, L)
+           for k in DYN_INPUTS]
     carried = carried or {}
     shapes = {"tend_rho": (nc + 1, L), "tend_u_euler": (ne + 1, L), "tend_w_euler": (nc + 1, L + 1),
               "tend_theta_euler": (nc + 1, L), "kdiff": (nc + 1, L)}

@@ -51,6 +51,7 @@ DEFAULTS = {
     "config_h_theta_eddy_visc4": 0.0,
     "config_mpas_cam_coef": 0.0,
     "config_number_cam_damping_levels": 4,
+    "config_curvature_u": False,
 }
 # Short aliases the reference also accepts (config_defaults.py:149-160)
 ALIASES = {"smdiv": "config_smdiv", "len_disp": "config_len_disp", "monotonic": "config_monotonic",
@@ -76,7 +77,6 @@ REQUIRED_VALUES = {
     "config_split_dynamics_transport": True,
     "config_v_mom_eddy_visc2": 0.0,
     "config_v_theta_eddy_visc2": 0.0,
-    "config_curvature_u": False,
     "config_apply_lbcs": False,
     "config_positive_definite": False,
     "config_mix_full": True,
@@ -203,6 +203,9 @@ def resolve_config(config: Mapping[str, Any] | None, state: Mapping[str, Any], m
     s.h_mom_eddy_visc4 = float(c["config_h_mom_eddy_visc4"])
     s.h_theta_eddy_visc4 = float(c["config_h_theta_eddy_visc4"])
     s.cam_levels, s.cam_visc2 = cam_damping(c, len_disp, n_levels)
+    s.curvature = 1 if bool(c["config_curvature_u"]) else 0
+    if s.curvature and not ("angleEdge" in mesh and "latEdge" in mesh):
+        raise MpasMissingFieldError(fields=[k for k in ("angleEdge", "latEdge") if k not in mesh], source="mesh")
     return s
 
 

@@ -21,6 +21,7 @@
 #define RGAS 287.0
 #define CP 1004.5
 #define P0 100000.0
+#define R_EARTH 6371229.0 /* constants.py:84, the curvature terms' radius */
 
 int orc_mesh_sizeof(void) { return (int)sizeof(orc_mesh); }
 
@@ -243,7 +244,8 @@ void orc_solve_diagnostics(const orc_mesh *m, double dt, double apvm, int rk_ste
  * Active configuration: 2d_smagorinsky or 2d_fixed horizontal mixing
  * (:3367-3430), optional CAM damping of kdiff (:3434-3458), no vertical eddy
  * viscosity, optional Rayleigh damping of u on the top levels (:2942-2996,
- * coefficient :10066-10072), no curvature, no physics tendencies.
+ * coefficient :10066-10072), optional curvature terms of u (:2471-2505) and w
+ * (:4241-4267; config_curvature_u), no physics tendencies.
  * ---------------------------------------------------------------------- */
 typedef struct orc_dyn_params {
     int rk_step;
@@ -261,6 +263,7 @@ typedef struct orc_dyn_params {
     double kdiff_fixed;
     int cam_levels;               /* CAM-damped top levels, 0 without CAM damping */
     double cam_visc2;             /* 4 * 2.0833 * len_disp * config_mpas_cam_coef */
+    int curvature;                /* config_curvature_u */
 } orc_dyn_params;
 
 int orc_dyn_params_sizeof(void) { return (int)sizeof(orc_dyn_params); }
@@ -273,6 +276,10 @@ void orc_dyn_tend(const orc_mesh *m, const orc_dyn_params *P,
                   const double *ke, const double *pv_edge, const double *pp,
                   const double *rr_save, const double *cqw, const double *cqu,
                   const double *rt_diabatic_tend, const double *qtot,
+                  /* curvature: reconstructed cell winds of the step's input state, and
+                   * (2 omega cos angleEdge) cos latEdge per edge, 2 omega cos latCell per cell */
+                  const double *ur_cell, const double *vr_cell, const double *curv_edge,
+                  const double *curv_cell,
                   /* in (rk>1) / out (rk==1) */
                   double *tend_rho, double *tend_u_euler, double *tend_w_euler,
                   double *tend_theta_euler, double *kdiff,
@@ -388,6 +395,14 @@ void orc_dyn_tend(const orc_mesh *m, const orc_dyn_params *P,
             double hsum = h_divergence[(size_t)c1 * L + k] + h_divergence[(size_t)c2 * L + k];
             double base = rho_edge[(size_t)e * L + k] * (q - ke_delta * m->invDcEdge[e]) - ue[k] * 0.5 * hsum;
             tend_u[(size_t)e * L + k] = vert + base;
+            if (P->curvature) {
+                /* (:2471-2505) tend_u -= curv_edge rho_edge w_avg + u w_avg rho_edge / r_earth */
+                const double *w1 = w + (size_t)c1 * (L + 1), *w2 = w + (size_t)c2 * (L + 1);
+                const double w_avg = 0.25 * (((w1[k] + w1[k + 1]) + w2[k]) + w2[k + 1]);
+                const double re = rho_edge[(size_t)e * L + k];
+                const double curv = curv_edge[e] * re * w_avg + ue[k] * w_avg * re * (1.0 / R_EARTH);
+                tend_u[(size_t)e * L + k] = tend_u[(size_t)e * L + k] - curv;
+            }
             if (rk1) {
                 double zzs = m->zz[(size_t)c2 * L + k] + m->zz[(size_t)c1 * L + k];
                 double ppd = pp[(size_t)c2 * L + k] - pp[(size_t)c1 * L + k];
@@ -491,6 +506,17 @@ void orc_dyn_tend(const orc_mesh *m, const orc_dyn_params *P,
                 *acc = (i == 0) ? cc : *acc + cc;
             }
         }
+        if (P->curvature)
+            /* (:4241-4267) added to the advective sum before its 1/area scaling */
+            for (int k = 1; k < L; ++k) {
+                const size_t o = (size_t)c * L + k;
+                const double ur = fzm[k] * ur_cell[o] + fzp[k] * ur_cell[o - 1];
+                const double vr = fzm[k] * vr_cell[o] + fzp[k] * vr_cell[o - 1];
+                const double rh = rho_zz[o] * fzm[k] + rho_zz[o - 1] * fzp[k];
+                double *acc = &hadv[(size_t)c * (L + 1) + k];
+                *acc = *acc + rh * (ur * ur + vr * vr) / R_EARTH;
+                *acc = *acc + curv_cell[c] * ur * rh;
+            }
     }
     /* w: horizontal mixing, first RK stage (:5315-5842, :6485) */
     if (rk1) {

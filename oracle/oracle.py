@@ -56,7 +56,7 @@ class OrcDynParams(C.Structure):
                 ("del4u_div_factor", C.c_double), ("prandtl_inv", C.c_double),
                 ("rayleigh_levels", C.c_int), ("rayleigh_coef_inverse", C.c_double),
                 ("mix_fixed", C.c_int), ("kdiff_fixed", C.c_double), ("cam_levels", C.c_int),
-                ("cam_visc2", C.c_double)]
+                ("cam_visc2", C.c_double), ("curvature", C.c_int)]
 
 
 class OrcAcousticParams(C.Structure):
@@ -133,6 +133,15 @@ class Mesh:
             setattr(s, name, _ptr(a))
         self.s = s
         self.nc, self.ne, self.nv, self.L = nc, ne, nv, L
+        # Coriolis factors of the curvature terms as the reference forms them
+        # (dyn_tend.py:2494-2502, :4258-4265): (2 omega cos angleEdge) cos latEdge per
+        # edge and 2 omega cos latCell per cell, omega = 7.29212e-5 (constants.py:88)
+        two_omega = 2.0 * 7.29212e-5
+        if "angleEdge" in d and "latEdge" in d:
+            self.curv_edge = _f64(two_omega * np.cos(_f64(d["angleEdge"])) * np.cos(_f64(d["latEdge"])))
+        else:
+            self.curv_edge = np.zeros(ne + 1)
+        self.curv_cell = _f64(two_omega * np.cos(_f64(d["latCell"])))
 
     @property
     def ref(self):
@@ -188,13 +197,15 @@ DYN_INPUTS = ("u", "v", "w", "ru", "rw", "ru_save", "rw_save", "theta_m", "theta
 def dyn_params(rk_step, dt, len_disp, c_s=0.125, visc4_2dsmag=0.05, coef_3rd_order=0.25,
                del4u_div_factor=10.0, prandtl=1.0, rayleigh_levels=0, rayleigh_timescale_days=5.0,
                horiz_mixing="2d_smagorinsky", h_theta_eddy_visc2=0.0, h_mom_eddy_visc4=0.0,
-               h_theta_eddy_visc4=0.0, cam_coef=0.0, cam_levels=4) -> OrcDynParams:
+               h_theta_eddy_visc4=0.0, cam_coef=0.0, cam_levels=4, curvature=False) -> OrcDynParams:
     """Scalar set-up of dyn_tend.py:3367-3458 and :9616-9626, evaluated in Python:
     Smagorinsky or "2d_fixed" mixing coefficients, the CAM damping of kdiff when
     ``cam_coef`` > 0 and ``cam_levels`` > 0, Rayleigh damping of u over
     ``rayleigh_levels`` top levels (0: off) with the coefficient of
-    dyn_tend.py:10066-10072."""
+    dyn_tend.py:10066-10072; ``curvature`` adds the curvature terms of u and w
+    (config_curvature_u, dyn_tend.py:2471-2505, :4241-4267)."""
     p = OrcDynParams()
+    p.curvature = 1 if curvature else 0
     p.rk_step = int(rk_step)
     p.dt = float(dt)
     p.coef_3rd_order = float(coef_3rd_order)
@@ -239,7 +250,11 @@ def dyn_tend(m: Mesh, params: OrcDynParams, fields: dict, carried: dict | None =
     }
     out = {"tend_u": _out((ne + 1, L)), "tend_w": _out((nc + 1, L + 1)), "tend_theta": _out((nc + 1, L)),
            "h_divergence": _out((nc + 1, L)), "rthdynten": _out((nc + 1, L))}
-    lib().orc_dyn_tend(m.ref, C.byref(params), *[_ptr(x) for x in ins],
+    # the step input's reconstructed winds (read only with curvature on)
+    ur = _f64(fields.get("ur_cell", np.zeros((nc + 1, L))))
+    vr = _f64(fields.get("vr_cell", np.zeros((nc + 1, L))))
+    lib().orc_dyn_tend(m.ref, C.byref(params), *[_ptr(x) for x in ins], _ptr(ur), _ptr(vr),
+                       _ptr(m.curv_edge), _ptr(m.curv_cell),
                        *[_ptr(io[k]) for k in ("tend_rho", "tend_u_euler", "tend_w_euler", "tend_theta_euler", "kdiff")],
                        *[_ptr(out[k]) for k in ("tend_u", "tend_w", "tend_theta", "h_divergence", "rthdynten")])
     out.update(io)

@@ -56,10 +56,13 @@ DIAG_KEYS = ("h_edge", "v", "divergence", "vorticity", "ke", "pv_edge")
 def srk3(state: dict, mesh: "O.Mesh", dt: float, *, len_disp: float, ns: int = 2, split: int = 3,
          epssm: float = 0.1, smdiv: float = 0.1, apvm: float = 0.5, coef3: float = 0.25,
          monotonic: bool = True, diagnostics_current: bool = False, order: int = 3,
-         rayleigh_levels: int = 0, rayleigh_timescale_days: float = 5.0, mixing: dict | None = None) -> dict:
+         rayleigh_levels: int = 0, rayleigh_timescale_days: float = 5.0, mixing: dict | None = None,
+         curvature: bool = False) -> dict:
     """One large step on NumPy arrays in the reference layout; returns a new dict.
     ``rayleigh_levels`` > 0 damps u on that many top levels (config_rayleigh_damp_u);
-    ``mixing`` passes the horizontal-mixing and CAM keywords of ``oracle.dyn_params``."""
+    ``mixing`` passes the horizontal-mixing and CAM keywords of ``oracle.dyn_params``;
+    ``curvature`` adds the curvature terms (config_curvature_u), which read the input
+    state's ``ur_cell`` / ``vr_cell`` all step long."""
     st = {k: (np.array(v, dtype=np.float64, copy=True) if isinstance(v, np.ndarray) and v.dtype.kind == "f" else v)
           for k, v in state.items()}
     nc, ne, L = mesh.nc, mesh.ne, mesh.L
@@ -109,10 +112,12 @@ def srk3(state: dict, mesh: "O.Mesh", dt: float, *, len_disp: float, ns: int = 2
                 "divergence": st["divergence"], "vorticity": st["vorticity"], "ke": st["ke"],
                 "pv_edge": st["pv_edge"], "pp": st["pressure_p"], "rr_save": rr_save,
                 "cqw": moist["cqw"], "cqu": moist["cqu"], "rt_diabatic_tend": rt_diab,
-                "qtot": moist["qtot"],
+                "qtot": moist["qtot"], "ur_cell": st.get("ur_cell", zeros_c()),
+                "vr_cell": st.get("vr_cell", zeros_c()),
             }
             params = O.dyn_params(rk, dt, len_disp, coef_3rd_order=coef3, rayleigh_levels=rayleigh_levels,
-                                  rayleigh_timescale_days=rayleigh_timescale_days, **(mixing or {}))
+                                  rayleigh_timescale_days=rayleigh_timescale_days, curvature=curvature,
+                                  **(mixing or {}))
             tend = O.dyn_tend(mesh, params, fields, carried if rk > 1 else None)
             carried = {k: tend[k] for k in ("tend_rho", "tend_u_euler", "tend_w_euler", "tend_theta_euler", "kdiff")}
             tend_rw = O.smlstep_pert(mesh, tend["tend_u"], tend["tend_w"])

@@ -79,6 +79,8 @@ OPTION_CASES = {
     # kdiff so that both sides of the max are taken
     "l2_L20_cam": (2, 20, "quadratic", 1, 900.0, 2, 9191,
                    {"config_mpas_cam_coef": 0.02, "config_number_cam_damping_levels": 6}),
+    # curvature terms of u and w (config_curvature_u)
+    "l2_L20_curvature": (2, 20, "quadratic", 1, 900.0, 2, 9292, {"config_curvature_u": True}),
 }
 SUMMARY_CASES = {
     "cfg1_l4_L26": (4, 26, "uniform", 0, 1800.0, 2, 20240),

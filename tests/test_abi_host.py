@@ -22,7 +22,7 @@ from conftest import ROOT, srk3_case
 
 from mpas_dycore_pytorch_amd import _native as N
 from mpas_dycore_pytorch_amd.device import DeviceMesh
-from mpas_dycore_pytorch_amd.errors import (MpasConfigError, MpasMeshMismatchError,
+from mpas_dycore_pytorch_amd.errors import (MpasConfigError, MpasMeshMismatchError, MpasMissingFieldError,
                                             MpasUnsupportedCapabilityError)
 from mpas_dycore_pytorch_amd.time_integration import resolve_config
 
@@ -106,8 +106,13 @@ def test_config_gate():
                        meta["dt"], 2)
     # a full key name takes precedence over its short alias (config_defaults.py:745-774)
     assert resolve_config({**ok, "epssm": 0.3, "config_epssm": 0.2}, st, st, meta["dt"], 2).epssm == 0.2
-    for key, value in (("config_curvature_u", True), ("config_v_mom_eddy_visc2", 1.0),
-                       ("config_v_theta_eddy_visc2", 1.0), ("config_mix_full", False)):
+    # curvature terms (dyn_tend.py:2471-2505, :4241-4267) need the edge angles and latitudes
+    assert resolve_config({**ok, "curvature_u": True}, st, st, meta["dt"], 2).curvature == 1
+    no_angles = {k: v for k, v in st.items() if k not in ("angleEdge", "latEdge")}
+    with pytest.raises(MpasMissingFieldError):
+        resolve_config({**ok, "config_curvature_u": True}, no_angles, no_angles, meta["dt"], 2)
+    for key, value in (("config_v_mom_eddy_visc2", 1.0), ("config_v_theta_eddy_visc2", 1.0),
+                       ("config_mix_full", False)):
         with pytest.raises(MpasUnsupportedCapabilityError):
             resolve_config({**ok, key: value}, st, st, meta["dt"], 2)
     # Rayleigh damping of u: the reference's coefficient 1 / (levels * days * 86400)

@@ -7,7 +7,10 @@
   ``config_h_mom_eddy_visc4`` / ``config_h_theta_eddy_visc4``;
   ``dyn_tend.py:3367-3430``);
 * CAM damping of kdiff (``config_mpas_cam_coef`` 0.02 on the top 6 levels;
-  ``dyn_tend.py:3434-3458``).
+  ``dyn_tend.py:3434-3458``);
+* the curvature terms of u and w (``config_curvature_u``; ``dyn_tend.py:2471-2505``,
+  ``:4241-4267``), which also make the step differentiable in the input state's
+  reconstructed winds ``ur_cell`` / ``vr_cell``.
 
 For each: two RK3 steps against the reference's own recording of the same case
 (``tests/golden/srk3_l2_L20_<option>.npz``), per field at the whole-step bar,
@@ -37,9 +40,9 @@ def _config(st, meta, damp=True):
     return cfg
 
 
-CASES = ("l2_L20_rayleigh", "l2_L20_fixedmix", "l2_L20_cam")
+CASES = ("l2_L20_rayleigh", "l2_L20_fixedmix", "l2_L20_cam", "l2_L20_curvature")
 # how far the step without the option lands from the recording after one step (u)
-MISS = {"l2_L20_rayleigh": 1e-6, "l2_L20_fixedmix": 1e-3, "l2_L20_cam": 1e-6}
+MISS = {"l2_L20_rayleigh": 1e-6, "l2_L20_fixedmix": 1e-3, "l2_L20_cam": 1e-6, "l2_L20_curvature": 1e-9}
 
 
 def _oracle_options(extra: dict) -> dict:
@@ -56,6 +59,8 @@ def _oracle_options(extra: dict) -> dict:
             mixing[name] = extra[key]
     if mixing:
         kw["mixing"] = mixing
+    if extra.get("config_curvature_u"):
+        kw["curvature"] = True
     return kw
 
 
@@ -64,8 +69,10 @@ def _check_option(cfg, case: str) -> None:
         assert cfg.rayleigh_levels == 5
     elif case == "l2_L20_fixedmix":
         assert cfg.mix_fixed == 1 and cfg.h_theta_eddy_visc2 == 1.0e8 and cfg.cam_levels == 0
-    else:
+    elif case == "l2_L20_cam":
         assert cfg.mix_fixed == 0 and cfg.cam_levels == 6 and cfg.cam_visc2 > 0.0
+    else:
+        assert cfg.curvature == 1 and cfg.cam_levels == 0 and cfg.mix_fixed == 0
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -114,6 +121,9 @@ def test_option_step_derivatives(case):
         assert torch.equal(taped[k], fused[k]), k
     gen = torch.Generator().manual_seed(5)
     keys = ("u", "w", "theta_m")
+    if cfg.curvature:
+        # the curvature terms read the input state's reconstructed winds
+        keys = keys + ("ur_cell", "vr_cell")
     pad = {dm.nc + 1, dm.ne + 1, dm.nv + 1}
 
     def rand(like):

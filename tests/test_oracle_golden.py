@@ -119,20 +119,23 @@ def _mixing_kwargs(extra: dict) -> dict:
     return kw
 
 
-@pytest.mark.parametrize("case", ["l2_L20_fixedmix", "l2_L20_cam"])
+@pytest.mark.parametrize("case", ["l2_L20_fixedmix", "l2_L20_cam", "l2_L20_curvature"])
 def test_srk3_whole_steps_mixing_options(case):
     """"2d_fixed" horizontal mixing (constant kdiff, separate del4 coefficients for
-    momentum and theta) and the CAM damping of kdiff on the top 6 levels, over two
-    RK3 steps: the oracle against the reference's recordings at 1e-14.  Without the
-    option the oracle misses the recording, so each option is exercised."""
+    momentum and theta), the CAM damping of kdiff on the top 6 levels and the
+    curvature terms of u and w, over two RK3 steps: the oracle against the
+    reference's recordings at 1e-14.  Without the option the oracle misses the
+    recording, so each option is exercised."""
     st, outs, meta = srk3_case(case)
-    kw = _mixing_kwargs(meta["config_extra"])
-    assert kw, meta["config_extra"]
+    extra = meta["config_extra"]
+    kw = _mixing_kwargs(extra)
+    curvature = bool(extra.get("config_curvature_u", False))
+    assert kw or curvature, extra
     mesh = O.Mesh(st)
     cur = st
     for step, ref in enumerate(outs):
         cur = OS.srk3(cur, mesh, meta["dt"], len_disp=float(st["nominalMinDc"]), diagnostics_current=(step > 0),
-                      mixing=kw)
+                      mixing=kw, curvature=curvature)
         for k in FIELDS:
             if k in ref:
                 err = rel_err(cur[k], ref[k])
