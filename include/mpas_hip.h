@@ -43,6 +43,11 @@ typedef struct mpas_hip_mesh {
      * with curvature on: (2 omega cos angleEdge) cos latEdge per edge and 2 omega cos latCell
      * per cell, omega = 7.29212e-5 */
     const double *curvEdge, *curvCell;
+    /* vertical eddy viscosities (dyn_tend.py:2731-2781, :4633-4647, :5060-5097), read only with
+     * them on: interface heights zgrid (cells x interfaces); without config_mix_full the
+     * reference profiles t_init (cells x levels), u_init and v_init (levels) and cos/sin of
+     * angleEdge (edges) */
+    const double *zgrid, *t_init, *u_init, *v_init, *cosAngleEdge, *sinAngleEdge;
 } mpas_hip_mesh;
 
 /* Prognostic state and the diagnostics carried between large steps
@@ -95,6 +100,9 @@ typedef struct mpas_hip_config {
     int32_t cam_levels;           /* [config_number_cam_damping_levels] */
     double cam_visc2;             /* 4 * 2.0833 * len_disp * [config_mpas_cam_coef] */
     int32_t curvature;            /* [config_curvature_u]: curvature terms of u and w */
+    double v_mom_eddy_visc2;      /* [config_v_mom_eddy_visc2]: vertical mixing of u and w */
+    double v_theta_eddy_visc2;    /* [config_v_theta_eddy_visc2]: vertical mixing of theta */
+    int32_t mix_full;             /* [config_mix_full] 1; 0 mixes departures from u_init/v_init/t_init */
 } mpas_hip_config;
 
 /* Number of doubles of workspace needed for one large step. */
@@ -159,6 +167,11 @@ typedef struct mpas_hip_dyn_params {
      * u w_avg rho_edge / r_earth; the w advective sum gains rho (ur^2 + vr^2) / r_earth +
      * curvCell ur rho at interfaces (cell winds of the step's input state), r_earth = 6371229 m */
     int32_t curvature;
+    /* vertical eddy viscosities of the first RK stage, added to the Euler tendencies of levels
+     * (u, theta: 1..L-2) and interfaces (w: 1..L-1); mix_full = 0 mixes u - u_init cos(angleEdge)
+     * - v_init sin(angleEdge) and theta_m - t_init */
+    double v_mom_eddy_visc2, v_theta_eddy_visc2;
+    int32_t mix_full;
 } mpas_hip_dyn_params;
 
 /* Scalars of atm_advance_acoustic_step_work (acoustic_step.py:1297-1340). */

@@ -30,7 +30,7 @@ MESH_REAL_FIELDS_A = ("edgesOnCell_sign", "edgesOnVertex_sign", "weightsOnEdge",
                       "zb_cell", "zb3_cell", "rdzw", "rdzu", "fzm", "fzp", "rdnw", "fnm", "fnp")
 MESH_SCALAR_FIELDS = ("cf1", "cf2", "cf3", "reserved1")
 MESH_REAL_FIELDS_B = ("rho_base", "rtheta_base", "exner_base")
-MESH_REAL_FIELDS_C = ("curvEdge", "curvCell")
+MESH_REAL_FIELDS_C = ("curvEdge", "curvCell", "zgrid", "t_init", "u_init", "v_init", "cosAngleEdge", "sinAngleEdge")
 
 STATE_FIELDS = ("u", "w", "theta_m", "rho_zz", "scalars", "rt_diabatic_tend",
                 "rho_p", "rtheta_p", "exner", "pressure_p", "ru", "rw",
@@ -64,7 +64,8 @@ class ConfigStruct(C.Structure):
                 ("rayleigh_levels", C.c_int32), ("rayleigh_coef_inverse", C.c_double),
                 ("mix_fixed", C.c_int32), ("h_theta_eddy_visc2", C.c_double), ("h_mom_eddy_visc4", C.c_double),
                 ("h_theta_eddy_visc4", C.c_double), ("cam_levels", C.c_int32), ("cam_visc2", C.c_double),
-                ("curvature", C.c_int32)]
+                ("curvature", C.c_int32), ("v_mom_eddy_visc2", C.c_double), ("v_theta_eddy_visc2", C.c_double),
+                ("mix_full", C.c_int32)]
 
 
 class DynParamsStruct(C.Structure):
@@ -73,7 +74,8 @@ class DynParamsStruct(C.Structure):
                 ("h_theta_eddy_visc4", C.c_double), ("del4u_div_factor", C.c_double),
                 ("prandtl_inv", C.c_double), ("rayleigh_levels", C.c_int32), ("rayleigh_coef_inverse", C.c_double),
                 ("mix_fixed", C.c_int32), ("kdiff_fixed", C.c_double), ("cam_levels", C.c_int32),
-                ("cam_visc2", C.c_double), ("curvature", C.c_int32)]
+                ("cam_visc2", C.c_double), ("curvature", C.c_int32), ("v_mom_eddy_visc2", C.c_double),
+                ("v_theta_eddy_visc2", C.c_double), ("mix_full", C.c_int32)]
 
 
 class AcousticParamsStruct(C.Structure):

@@ -432,7 +432,9 @@ def taped_step(dm: DeviceMesh, state: Mapping[str, torch.Tensor], cfg: Any, dt: 
                                   rayleigh_coef_inverse=cfg.rayleigh_coef_inverse, mix_fixed=bool(cfg.mix_fixed),
                                   h_theta_eddy_visc2=cfg.h_theta_eddy_visc2, h_mom_eddy_visc4=cfg.h_mom_eddy_visc4,
                                   h_theta_eddy_visc4=cfg.h_theta_eddy_visc4, cam_levels=cfg.cam_levels,
-                                  cam_visc2=cfg.cam_visc2, curvature=bool(cfg.curvature))
+                                  cam_visc2=cfg.cam_visc2, curvature=bool(cfg.curvature),
+                                  v_mom_eddy_visc2=cfg.v_mom_eddy_visc2, v_theta_eddy_visc2=cfg.v_theta_eddy_visc2,
+                                  mix_full=bool(cfg.mix_full))
             if rk == 1:
                 carried_in = {k: z(*t.shape) for k, t in
                               zip(K.DYN_CARRIED, (st["rho_zz"], st["u"], st["w"], st["rho_zz"], st["rho_zz"]))}

@@ -123,6 +123,9 @@ __device__ __forceinline__ void dyn_a1(const mpas_hip_mesh &m, const mpas_hip_dy
     double th_m2 = 0.0, th_m1 = 0.0, th_0 = 0.0, th_p1 = 0.0, ts_m1 = 0.0, ts_0 = 0.0;
     double pp_m1 = 0.0, pp_0 = 0.0, dp_m1 = 0.0, dp_0 = 0.0;
     double r_0 = 0.0, r_p1 = 0.0, rs_0 = 0.0, w_m1 = 0.0, w_0 = 0.0, w_p1 = 0.0, w_p2 = 0.0;
+    // rho_zz at levels k and k-1 (curvature, vertical mixing); ur_cell, vr_cell (curvature)
+    double cz_0 = 0.0, cz_m1 = 0.0, cu_0 = 0.0, cu_m1 = 0.0, cv_0 = 0.0, cv_m1 = 0.0;
+    const bool vmix_w = rk1 && P.v_mom_eddy_visc2 > 0.0, vmix_t = rk1 && P.v_theta_eddy_visc2 > 0.0;
     // ------------------------------------------------ theta, level k
     if (k < L) {
         const long long o = oc + k;
@@ -136,6 +139,22 @@ __device__ __forceinline__ void dyn_a1(const mpas_hip_mesh &m, const mpas_hip_dy
         add_own(B, rho_zz, o, -(rth / rz) * grth + F[rt_diabatic_tend][o] * gtt);
         add_own(B, rt_diabatic_tend, o, rz * gtt);
         S.g_thadv[o] = ia * gbase;
+        if (vmix_t && k >= 1 && k <= L - 2) {
+            // vertical mixing of theta: vt pr rho_zz ((c - b)/(zp - z0) - (b - a)/(z0 - zm)) / h
+            const VmixZ z = vmix_z_cell(m.zgrid + c * Lw, k);
+            const double ip = 1.0 / (z.zp - z.z0), im = 1.0 / (z.z0 - z.zm);
+            const double *thc = F[theta_m] + oc, *ti = m.t_init + oc;
+            const double a = P.mix_full ? thc[k - 1] : thc[k - 1] - ti[k - 1];
+            const double b = P.mix_full ? thc[k] : thc[k] - ti[k];
+            const double cc = P.mix_full ? thc[k + 1] : thc[k + 1] - ti[k + 1];
+            const double sv = P.v_theta_eddy_visc2 * P.prandtl_inv / (0.5 * (z.zp - z.zm));
+            const double gt = bar_of(B, tend_theta_euler, o) + gtt;
+            cz_0 += sv * ((cc - b) * ip - (b - a) * im) * gt;
+            const double gr = rz * sv * gt;
+            th_p1 += gr * ip;
+            th_0 += -gr * (ip + im);
+            th_m1 += gr * im;
+        }
         // tend_rho: computed here on the first stage, carried otherwise
         if (rk1) {
             r_p1 += -m.rdzw[k] * g_trho;
@@ -200,6 +219,18 @@ __device__ __forceinline__ void dyn_a1(const mpas_hip_mesh &m, const mpas_hip_dy
             pp_m1 += cq * m.rdzu[k] * gtwe;
             dp_0 += cq * fzm[k] * gtwe;
             dp_m1 += cq * fzp[k] * gtwe;
+            if (vmix_w) {
+                // vertical mixing of w: visc 0.5 (rho_zz(k) + rho_zz(k-1)) (w_up - w_down) rdzu
+                const double *wc = F[w] + ow, *rz = F[rho_zz] + oc;
+                const double rho_half = 0.5 * (rz[k] + rz[k - 1]);
+                const double w_up = (wc[k + 1] - wc[k]) * m.rdzw[k], w_down = (wc[k] - wc[k - 1]) * m.rdzw[k - 1];
+                const double g = P.v_mom_eddy_visc2 * m.rdzu[k] * gtwe;
+                cz_0 += 0.5 * (w_up - w_down) * g;
+                cz_m1 += 0.5 * (w_up - w_down) * g;
+                w_p1 += rho_half * m.rdzw[k] * g;
+                w_0 += -rho_half * (m.rdzw[k] + m.rdzw[k - 1]) * g;
+                w_m1 += rho_half * m.rdzw[k - 1] * g;
+            }
         }
     }
     // w vertical flux at mass level j = k (0..L-2): g_wdwz(j) = g_delta(j) [j>=1] - g_delta(j+1)
@@ -229,7 +260,6 @@ __device__ __forceinline__ void dyn_a1(const mpas_hip_mesh &m, const mpas_hip_dy
     // curvature of w at interface k (1..L-1), added to the advective sum before
     // its 1/area scaling: rh (ur^2 + vr^2) / r_earth + curvCell ur rh, with ur, vr
     // and rh interpolated from levels k and k-1
-    double cz_0 = 0.0, cz_m1 = 0.0, cu_0 = 0.0, cu_m1 = 0.0, cv_0 = 0.0, cv_m1 = 0.0;
     if (P.curvature && k >= 1 && k < L) {
         const double gh = ia * bar_of(B, tend_w, i);
         const double *ur = F[ur_cell] + oc, *vr = F[vr_cell] + oc, *rz = F[rho_zz] + oc;
@@ -238,8 +268,8 @@ __device__ __forceinline__ void dyn_a1(const mpas_hip_mesh &m, const mpas_hip_dy
         const double grh = gh * ((uri * uri + vri * vri) / kEarthRadius + m.curvCell[c] * uri);
         const double gur = gh * (2.0 * uri * rh / kEarthRadius + m.curvCell[c] * rh);
         const double gvr = gh * (2.0 * vri * rh / kEarthRadius);
-        cz_0 = fzm[k] * grh;
-        cz_m1 = fzp[k] * grh;
+        cz_0 += fzm[k] * grh;
+        cz_m1 += fzp[k] * grh;
         cu_0 = fzm[k] * gur;
         cu_m1 = fzp[k] * gur;
         cv_0 = fzm[k] * gvr;
@@ -250,10 +280,10 @@ __device__ __forceinline__ void dyn_a1(const mpas_hip_mesh &m, const mpas_hip_dy
     const double g_cu = cu_0 + from_lane(cu_m1, 1, k, L);
     const double g_cv = cv_0 + from_lane(cv_m1, 1, k, L);
     if (P.curvature && k < L) {
-        add_own(B, rho_zz, oc + k, g_cz);
         add_own(B, ur_cell, oc + k, g_cu);
         add_own(B, vr_cell, oc + k, g_cv);
     }
+    if ((P.curvature || vmix_w || vmix_t) && k < L) add_own(B, rho_zz, oc + k, g_cz);
     const double g_th = th_0 + from_lane(th_m1, 1, k, L) + from_lane(th_m2, 2, k, L) + from_lane(th_p1, -1, k, L);
     const double g_ts = ts_0 + from_lane(ts_m1, 1, k, L);
     const double g_pp = pp_0 + from_lane(pp_m1, 1, k, L);

@@ -90,6 +90,27 @@ using AcousticParams = mpas_hip_acoustic_params;
 // Earth radius of the curvature terms (the reference's constants.py:84)
 constexpr double kEarthRadius = 6371229.0;
 
+// Heights of the vertical mixing stencil at level k (1..L-2): zm, z0, zp are the
+// midpoints of interfaces (k-1, k), (k, k+1), (k+1, k+2) of a cell column
+// (dyn_tend.py:5073-5080), or of the mean of an edge's two cell columns
+// (:2750-2762).  The mixing is ((f(k+1) - f(k)) / (zp - z0) - (f(k) - f(k-1)) /
+// (z0 - zm)) / (0.5 (zp - zm)) times its coefficient.
+struct VmixZ {
+    double zm, z0, zp;
+};
+__device__ __forceinline__ VmixZ vmix_z_cell(const double *zg, int k) {
+    return {0.5 * (zg[k - 1] + zg[k]), 0.5 * (zg[k] + zg[k + 1]), 0.5 * (zg[k + 1] + zg[k + 2])};
+}
+__device__ __forceinline__ VmixZ vmix_z_edge(const double *g1, const double *g2, int k) {
+    const double z1 = 0.5 * (g1[k - 1] + g2[k - 1]), z2 = 0.5 * (g1[k] + g2[k]);
+    const double z3 = 0.5 * (g1[k + 1] + g2[k + 1]), z4 = 0.5 * (g1[k + 2] + g2[k + 2]);
+    return {0.5 * (z1 + z2), 0.5 * (z2 + z3), 0.5 * (z3 + z4)};
+}
+// the mixed u of level k: u itself, or its departure from the reference profile
+__device__ __forceinline__ double vmix_u(const mpas_hip_mesh &m, const DynParams &P, double u, int e, int k) {
+    return P.mix_full ? u : (u - m.u_init[k] * m.cosAngleEdge[e]) - m.v_init[k] * m.sinAngleEdge[e];
+}
+
 // CAM damping floor of kdiff at level k (one of the top P.cam_levels levels):
 // cam_visc2 (1 - (nLevels - (k + 1)) / cam_levels), dyn_tend.py:3442-3446
 __device__ __forceinline__ double cam_kdiff(const DynParams &P, int L, int k) {

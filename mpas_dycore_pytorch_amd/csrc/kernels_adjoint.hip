@@ -284,6 +284,19 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
             x_dd = re * r_dc * gtue;
             x_dv = re * r_dv * gtue;
         }
+        if (P.v_mom_eddy_visc2 > 0.0 && k >= 1 && k <= L - 2) {
+            // vertical mixing of u: re visc ((c - b)/(zp - z0) - (b - a)/(z0 - zm)) / h,
+            // a, b, c the mixed u of levels k-1, k, k+1
+            const VmixZ z = vmix_z_edge(m.zgrid + c1 * Lw, m.zgrid + c2 * Lw, k);
+            const double ip = 1.0 / (z.zp - z.z0), im = 1.0 / (z.z0 - z.zm);
+            const double a = vmix_u(m, P, um1, e, k - 1), b = vmix_u(m, P, u0, e, k), cc = vmix_u(m, P, up1, e, k + 1);
+            const double sv = P.v_mom_eddy_visc2 / (0.5 * (z.zp - z.zm));
+            g_re += sv * ((cc - b) * ip - (b - a) * im) * gtue;
+            const double gr = re * sv * gtue;
+            c_p1 += gr * ip;
+            g_u_own += -gr * (ip + im);
+            c_m1 += gr * im;
+        }
     }
     // ---- lanes exchange what they owe the neighbouring levels
     c_m2 *= on;

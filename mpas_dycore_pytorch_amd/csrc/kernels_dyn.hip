@@ -243,6 +243,16 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_a(long long nb, mpa
     for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
 
+// Vertical mixing of u at level k (1..L-2) of edge e (dyn_tend.py:2731-2781),
+// from u at levels k-1, k, k+1; added to tend_u_euler after the del4 mixing
+__device__ __forceinline__ double u_vertical_mix(const mpas_hip_mesh &m, const DynParams &P, int e, int k, double re,
+                                                 double u_m1, double u_0, double u_p1) {
+    const int Lw = m.nLevels + 1;
+    const VmixZ z = vmix_z_edge(m.zgrid + m.cellsOnEdge[2 * e] * Lw, m.zgrid + m.cellsOnEdge[2 * e + 1] * Lw, k);
+    const double a = vmix_u(m, P, u_m1, e, k - 1), b = vmix_u(m, P, u_0, e, k), c = vmix_u(m, P, u_p1, e, k + 1);
+    return re * P.v_mom_eddy_visc2 * ((c - b) / (z.zp - z.z0) - (b - a) / (z.z0 - z.zm)) / (0.5 * (z.zp - z.zm));
+}
+
 // P4: normal-velocity tendency (vertical flux, PV flux, KE gradient,
 // divergence coupling); first stage also PGF and del2 mixing into tend_u_euler.
 __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_a(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ u,
@@ -349,6 +359,9 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_a(long long nb, mpa
             delsq_u[i] = ud;
             double kdiffu = 0.5 * (kdiff[c1 * L + k] + kdiff[c2 * L + k]);
             eu = eu + rho_edge[i] * kdiffu * ud * m.meshScalingDel2[e];
+            // without a del4 pass the vertical mixing closes tend_u_euler here
+            if (add_euler && P.v_mom_eddy_visc2 > 0.0 && k >= 1 && k <= L - 2)
+                eu = eu + u_vertical_mix(m, P, e, k, (double)rho_edge[i], um1, u0, up1);
             tend_u_euler[i] = eu;
         } else if (add_euler) {
             eu = tend_u_euler[i];
@@ -410,7 +423,8 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_del4_vc(long long nb, mpa
 // P6: del4 u mixing (first stage) and the final u tendency; one wavefront per
 // edge, lane = level
 __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_b(long long nb, mpas_hip_mesh m, DynParams P,
-                                                               int do_del4, const hreal *__restrict__ rho_edge,
+                                                               int do_del4, const double *__restrict__ u,
+                                                               const hreal *__restrict__ rho_edge,
                                                                const double *__restrict__ delsq_div,
                                                                const hreal *__restrict__ delsq_vort,
                                                                hreal *__restrict__ tend_u_euler,
@@ -430,6 +444,8 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_b(long long nb, mpa
         double ud = rho_edge[i] * ((delsq_div[c2 * L + k] - delsq_div[c1 * L + k]) * r_dc -
                                    (delsq_vort[v2 * L + k] - delsq_vort[v1 * L + k]) * r_dv);
         eu = eu - ud;
+        if (P.v_mom_eddy_visc2 > 0.0 && k >= 1 && k <= L - 2)
+            eu = eu + u_vertical_mix(m, P, e, k, (double)rho_edge[i], u[i - 1], u[i], u[i + 1]);
         tend_u_euler[i] = eu;
     }
     tend_u[i] = tend_u[i] + eu;
@@ -630,6 +646,12 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_b(long long nb, mpa
                     }
                 }
                 eu = eu - cqw[i] * (m.rdzu[k] * (pp0 - pp_m1) - (fzm[k] * dp0 + fzp[k] * dp_m1));
+                if (P.v_mom_eddy_visc2 > 0.0) {
+                    // vertical mixing of w (dyn_tend.py:4633-4647)
+                    const double rho_half = 0.5 * (rho_zz[oc + k] + rho_zz[oc + k - 1]);
+                    const double w_up = (w_p1 - w0) * m.rdzw[k], w_down = (w0 - w_m1) * m.rdzw[k - 1];
+                    eu = eu + P.v_mom_eddy_visc2 * rho_half * (w_up - w_down) * m.rdzu[k];
+                }
                 tend_w_euler[i] = eu;
             }
             const double delta = (k == L - 1) ? -wz_m1 : wz - wz_m1;
@@ -686,6 +708,16 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_b(long long nb, mpa
                 }
                 tend_theta_euler[o] = tte;
             }
+            if (rk1 && P.v_theta_eddy_visc2 > 0.0 && k >= 1 && k <= L - 2) {
+                // vertical mixing of theta_m, or of theta_m - t_init (dyn_tend.py:5060-5097)
+                const VmixZ z = vmix_z_cell(m.zgrid + c * Lw, k);
+                const double *ti = m.t_init + oc;
+                const double a = P.mix_full ? th_m1 : th_m1 - ti[k - 1], b = P.mix_full ? th0 : th0 - ti[k];
+                const double cth = P.mix_full ? th_p1 : th_p1 - ti[k + 1];
+                tte = tte + P.v_theta_eddy_visc2 * P.prandtl_inv * rho_zz[o] *
+                                ((cth - b) / (z.zp - z.z0) - (b - a) / (z.z0 - z.zm)) / (0.5 * (z.zp - z.zm));
+                tend_theta_euler[o] = tte;
+            }
             double delta;
             if (k == 0) delta = tz_p1;
             else if (k == L - 1) delta = -tz;
@@ -720,7 +752,7 @@ void launch_dyn_tend(const mpas_hip_mesh &m, const DynParams &P, const DynInputs
                        in.vorticity, io.kdiff, out.tend_u, io.tend_u_euler, sc.delsq_u, do_del4 ? 0 : 1);
     if (do_del4) {
         MPAS_WAVE_LAUNCH(k_dyn_del4_vc, nC > nV ? nC : nV, s, m, sc.delsq_u, sc.delsq_vort, sc.delsq_div);
-        MPAS_WAVE_LAUNCH(k_dyn_edges_b, nE, s, m, P, do_del4, in.rho_edge, sc.delsq_div, sc.delsq_vort,
+        MPAS_WAVE_LAUNCH(k_dyn_edges_b, nE, s, m, P, do_del4, in.u, in.rho_edge, sc.delsq_div, sc.delsq_vort,
                          io.tend_u_euler, out.tend_u);
     }
     MPAS_WAVE_LAUNCH(k_dyn_edge_adv, nE, s, m, P, in.ru, in.ru_save,

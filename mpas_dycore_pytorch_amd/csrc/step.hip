@@ -696,6 +696,9 @@ static int srk3_step_impl(const mpas_hip_mesh *mesh, StepState *st, const mpas_h
         P.cam_levels = cfg->cam_levels;
         P.cam_visc2 = cfg->cam_visc2;
         P.curvature = cfg->curvature ? 1 : 0;
+        P.v_mom_eddy_visc2 = cfg->v_mom_eddy_visc2;
+        P.v_theta_eddy_visc2 = cfg->v_theta_eddy_visc2;
+        P.mix_full = cfg->mix_full ? 1 : 0;
     }
 
     for (int sub = 1; sub <= cfg->split; ++sub) {

@@ -255,6 +255,16 @@ __device__ __forceinline__ void dyn_j4(const mpas_hip_mesh &m, const mpas_hip_dy
                 (S.g_delsq_div[o2] - S.g_delsq_div[o1]) * r_dc - (S.g_delsq_vort[w2] - S.g_delsq_vort[w1]) * r_dv;
             dtue -= dre * Y + re * dY;
         }
+        if (P.v_mom_eddy_visc2 > 0.0 && k >= 1 && k <= L - 2) {
+            // vertical mixing of u (the mixed u's departures are constant shifts)
+            const VmixZ z = vmix_z_edge(m.zgrid + c1 * Lw, m.zgrid + c2 * Lw, k);
+            const double ip = 1.0 / (z.zp - z.z0), im = 1.0 / (z.z0 - z.zm);
+            const double a = vmix_u(m, P, F[u][i - 1], e, k - 1), b = vmix_u(m, P, ue, e, k);
+            const double cc = vmix_u(m, P, F[u][i + 1], e, k + 1);
+            const double da = dot_of(T, u, i - 1), db = due, dc = dot_of(T, u, i + 1);
+            const double sv = P.v_mom_eddy_visc2 / (0.5 * (z.zp - z.zm));
+            dtue += sv * (dre * ((cc - b) * ip - (b - a) * im) + re * ((dc - db) * ip - (db - da) * im));
+        }
     } else {
         dtue = dot_of(T, tend_u_euler_in, i);
     }
@@ -432,6 +442,16 @@ __device__ __forceinline__ void dyn_j5(const mpas_hip_mesh &m, const mpas_hip_dy
             const double dY = m.rdzu[k] * (dot_of(T, pp, oc + k) - dot_of(T, pp, oc + k - 1)) -
                               (m.fzm[k] * S.g_dpdz[oc + k] + m.fzp[k] * S.g_dpdz[oc + k - 1]);
             dtwe -= dot_of(T, cqw, ow + k) * Y + F[cqw][ow + k] * dY;
+            if (P.v_mom_eddy_visc2 > 0.0) {
+                // vertical mixing of w
+                const double *wc = F[w] + ow, *rz = F[rho_zz] + oc;
+                const double rho_half = 0.5 * (rz[k] + rz[k - 1]);
+                const double drho_half = 0.5 * (dot_of(T, rho_zz, oc + k) + dot_of(T, rho_zz, oc + k - 1));
+                const double wud = (wc[k + 1] - wc[k]) * m.rdzw[k] - (wc[k] - wc[k - 1]) * m.rdzw[k - 1];
+                const double dwud = (dot_of(T, w, ow + k + 1) - dot_of(T, w, ow + k)) * m.rdzw[k] -
+                                    (dot_of(T, w, ow + k) - dot_of(T, w, ow + k - 1)) * m.rdzw[k - 1];
+                dtwe += P.v_mom_eddy_visc2 * (drho_half * wud + rho_half * dwud) * m.rdzu[k];
+            }
         } else {
             dtwe = dot_of(T, tend_w_euler_in, ow + k);
         }
@@ -498,6 +518,19 @@ __device__ __forceinline__ void dyn_j5(const mpas_hip_mesh &m, const mpas_hip_dy
                                    m.invDcEdge[e];
                 dtte -= es4 * (S.g_delsq_theta[a2] - S.g_delsq_theta[a1]);
             }
+        }
+        if (P.v_theta_eddy_visc2 > 0.0 && k >= 1 && k <= L - 2) {
+            // vertical mixing of theta_m (or of its departure from t_init, a constant shift)
+            const VmixZ z = vmix_z_cell(m.zgrid + c * Lw, k);
+            const double ip = 1.0 / (z.zp - z.z0), im = 1.0 / (z.z0 - z.zm);
+            const double *thc = F[theta_m] + oc, *ti = m.t_init + oc;
+            const double a = P.mix_full ? thc[k - 1] : thc[k - 1] - ti[k - 1];
+            const double b = P.mix_full ? thc[k] : thc[k] - ti[k];
+            const double cc = P.mix_full ? thc[k + 1] : thc[k + 1] - ti[k + 1];
+            const double da = dot_of(T, theta_m, o - 1), db = dot_of(T, theta_m, o), dc = dot_of(T, theta_m, o + 1);
+            const double sv = P.v_theta_eddy_visc2 * P.prandtl_inv / (0.5 * (z.zp - z.zm));
+            dtte += sv * (dot_of(T, rho_zz, o) * ((cc - b) * ip - (b - a) * im) +
+                          F[rho_zz][o] * ((dc - db) * ip - (db - da) * im));
         }
     } else {
         dtte = dot_of(T, tend_theta_euler_in, o);

@@ -142,7 +142,13 @@ class DeviceMesh:
         else:
             ce = np.zeros(ne + 1)
         cc = two_omega * np.cos(to_numpy(_get(mesh, state, "latCell")).astype(np.float64))
-        for name, a in (("curvEdge", ce), ("curvCell", cc)):
+        # vertical mixing: interface heights, and cos/sin of angleEdge for the
+        # departures of u from its reference profile
+        zg = to_numpy(_get(mesh, state, "zgrid")).astype(np.float64) if ("zgrid" in mesh or (
+            state is not None and "zgrid" in state)) else np.zeros((nc + 1, L + 1))
+        ang = to_numpy(mesh["angleEdge"]).astype(np.float64) if "angleEdge" in mesh else np.zeros(ne + 1)
+        for name, a in (("curvEdge", ce), ("curvCell", cc), ("zgrid", zg), ("cosAngleEdge", np.cos(ang)),
+                        ("sinAngleEdge", np.sin(ang))):
             t = torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=device)
             self.t[name] = t
             setattr(s, name, t.data_ptr())
@@ -153,11 +159,20 @@ class DeviceMesh:
         self._kscratch: torch.Tensor | None = None
 
     def set_base(self, mesh: Mapping, state: Mapping | None) -> None:
-        """(Re)bind the hydrostatic base state, looked up in the mesh then the state.
+        """(Re)bind the hydrostatic base state and the reference profiles of the
+        vertical mixing (``t_init``, ``u_init``, ``v_init``), looked up in the mesh
+        then the state.
 
         ``rtheta_base`` and ``exner_base`` are outputs of the cold start; until
         it has run they are bound to zeros (``missing_base`` lists them)."""
         self.missing_base = []
+        # reference profiles of the vertical mixing departures (zeros where absent)
+        for name, shape in (("t_init", (self.nc + 1, self.L)), ("u_init", (self.L,)), ("v_init", (self.L,))):
+            src = mesh.get(name, None) if name in mesh else (state or {}).get(name, None)
+            a = np.zeros(shape) if src is None else to_numpy(src).astype(np.float64).reshape(shape)
+            t = torch.as_tensor(np.ascontiguousarray(a), device=self.device)
+            self.t[name] = t
+            setattr(self.s, name, t.data_ptr())
         for name in BASE_FIELDS:
             try:
                 src = _get(mesh, state, name)

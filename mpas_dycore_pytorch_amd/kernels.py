@@ -70,7 +70,8 @@ def dyn_params(rk_step: int, dt: float, len_disp: float, c_s: float = 0.125, vis
                prandtl: float = 1.0, rayleigh_levels: int = 0,
                rayleigh_coef_inverse: float = 0.0, mix_fixed: bool = False, h_theta_eddy_visc2: float = 0.0,
                h_mom_eddy_visc4: float = 0.0, h_theta_eddy_visc4: float = 0.0, cam_levels: int = 0,
-               cam_visc2: float = 0.0, curvature: bool = False) -> N.DynParamsStruct:
+               cam_visc2: float = 0.0, curvature: bool = False, v_mom_eddy_visc2: float = 0.0,
+               v_theta_eddy_visc2: float = 0.0, mix_full: bool = True) -> N.DynParamsStruct:
     """Scalar set-up of the dynamics tendency (dyn_tend.py:3367-3458, :9616-9626).
 
     ``mix_fixed`` selects the "2d_fixed" horizontal mixing: kdiff is
@@ -81,9 +82,15 @@ def dyn_params(rk_step: int, dt: float, len_disp: float, c_s: float = 0.125, vis
     damping of u acts on ``rayleigh_levels`` top levels (0: off) with its
     coefficient 1 / (levels * timescale) (dyn_tend.py:10066-10072).  ``curvature``
     adds the curvature terms of u and w (config_curvature_u, dyn_tend.py:2471-2505,
-    :4241-4267), which read the ``ur_cell`` / ``vr_cell`` inputs."""
+    :4241-4267), which read the ``ur_cell`` / ``vr_cell`` inputs.  The vertical
+    eddy viscosities mix u, w and theta on the first RK stage, without
+    ``mix_full`` as departures from the reference profiles (dyn_tend.py:2731-2781,
+    :4633-4647, :5060-5097)."""
     p = N.DynParamsStruct()
     p.curvature = 1 if curvature else 0
+    p.v_mom_eddy_visc2 = float(v_mom_eddy_visc2)
+    p.v_theta_eddy_visc2 = float(v_theta_eddy_visc2)
+    p.mix_full = 1 if mix_full else 0
     p.rk_step = int(rk_step)
     p.dt = float(dt)
     p.coef_3rd_order = float(coef_3rd_order)

@@ -52,6 +52,9 @@ DEFAULTS = {
     "config_mpas_cam_coef": 0.0,
     "config_number_cam_damping_levels": 4,
     "config_curvature_u": False,
+    "config_v_mom_eddy_visc2": 0.0,
+    "config_v_theta_eddy_visc2": 0.0,
+    "config_mix_full": True,
 }
 # Short aliases the reference also accepts (config_defaults.py:149-160)
 ALIASES = {"smdiv": "config_smdiv", "len_disp": "config_len_disp", "monotonic": "config_monotonic",
@@ -75,11 +78,8 @@ ALIASES = {"smdiv": "config_smdiv", "len_disp": "config_len_disp", "monotonic": 
 # Options whose non-default values select code the GPU step does not have.
 REQUIRED_VALUES = {
     "config_split_dynamics_transport": True,
-    "config_v_mom_eddy_visc2": 0.0,
-    "config_v_theta_eddy_visc2": 0.0,
     "config_apply_lbcs": False,
     "config_positive_definite": False,
-    "config_mix_full": True,
     "config_scalar_advection": True,
     "config_w_adv_order": 3, "config_theta_adv_order": 3, "config_scalar_adv_order": 3,
     "config_u_vadv_order": 3, "config_w_vadv_order": 3, "config_theta_vadv_order": 3,
@@ -204,6 +204,11 @@ def resolve_config(config: Mapping[str, Any] | None, state: Mapping[str, Any], m
     s.h_theta_eddy_visc4 = float(c["config_h_theta_eddy_visc4"])
     s.cam_levels, s.cam_visc2 = cam_damping(c, len_disp, n_levels)
     s.curvature = 1 if bool(c["config_curvature_u"]) else 0
+    s.v_mom_eddy_visc2 = float(c["config_v_mom_eddy_visc2"])
+    s.v_theta_eddy_visc2 = float(c["config_v_theta_eddy_visc2"])
+    s.mix_full = 1 if bool(c["config_mix_full"]) else 0
+    if (s.v_mom_eddy_visc2 > 0.0 or s.v_theta_eddy_visc2 > 0.0) and "zgrid" not in mesh and "zgrid" not in state:
+        raise MpasMissingFieldError(fields=["zgrid"], source="mesh")
     if s.curvature and not ("angleEdge" in mesh and "latEdge" in mesh):
         raise MpasMissingFieldError(fields=[k for k in ("angleEdge", "latEdge") if k not in mesh], source="mesh")
     return s

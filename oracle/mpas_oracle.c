@@ -242,8 +242,9 @@ void orc_solve_diagnostics(const orc_mesh *m, double dt, double apvm, int rk_ste
  * (w vertical), :4803-5057 (theta vertical), :5315-5843 (w/theta mixing),
  * :5901 (theta perturbation flux), :6485 (ordered del4 subtraction).
  * Active configuration: 2d_smagorinsky or 2d_fixed horizontal mixing
- * (:3367-3430), optional CAM damping of kdiff (:3434-3458), no vertical eddy
- * viscosity, optional Rayleigh damping of u on the top levels (:2942-2996,
+ * (:3367-3430), optional CAM damping of kdiff (:3434-3458), optional vertical
+ * eddy viscosities of u, w and theta with config_mix_full either way (:2731-2781,
+ * :4633-4647, :5060-5097), optional Rayleigh damping of u on the top levels (:2942-2996,
  * coefficient :10066-10072), optional curvature terms of u (:2471-2505) and w
  * (:4241-4267; config_curvature_u), no physics tendencies.
  * ---------------------------------------------------------------------- */
@@ -264,6 +265,9 @@ typedef struct orc_dyn_params {
     int cam_levels;               /* CAM-damped top levels, 0 without CAM damping */
     double cam_visc2;             /* 4 * 2.0833 * len_disp * config_mpas_cam_coef */
     int curvature;                /* config_curvature_u */
+    double v_mom_eddy_visc2;      /* vertical mixing of u and w, first RK stage */
+    double v_theta_eddy_visc2;    /* vertical mixing of theta, first RK stage */
+    int mix_full;                 /* 0: mix u and theta as departures from u_init/v_init/t_init */
 } orc_dyn_params;
 
 int orc_dyn_params_sizeof(void) { return (int)sizeof(orc_dyn_params); }
@@ -280,6 +284,10 @@ void orc_dyn_tend(const orc_mesh *m, const orc_dyn_params *P,
                    * (2 omega cos angleEdge) cos latEdge per edge, 2 omega cos latCell per cell */
                   const double *ur_cell, const double *vr_cell, const double *curv_edge,
                   const double *curv_cell,
+                  /* vertical mixing: zgrid (cells x interfaces), t_init (cells x levels),
+                   * u_init, v_init (levels), cos and sin of angleEdge (edges) */
+                  const double *zgrid, const double *t_init, const double *u_init,
+                  const double *v_init, const double *cos_edge, const double *sin_edge,
                   /* in (rk>1) / out (rk==1) */
                   double *tend_rho, double *tend_u_euler, double *tend_w_euler,
                   double *tend_theta_euler, double *kdiff,
@@ -468,6 +476,29 @@ void orc_dyn_tend(const orc_mesh *m, const orc_dyn_params *P,
                 }
             }
         }
+        /* vertical mixing of u (:2731-2781), levels 1..L-2, on the full u or on its
+         * departure u - u_init cos(angleEdge) - v_init sin(angleEdge); heights are
+         * the means of the edge's two cell columns */
+        if (P->v_mom_eddy_visc2 > 0.0 && L > 2) {
+            double *um = colbuf;
+            for (int e = 0; e < nE; ++e) {
+                const int c1 = m->cellsOnEdge[2 * e], c2 = m->cellsOnEdge[2 * e + 1];
+                const double *g1 = zgrid + (size_t)c1 * (L + 1), *g2 = zgrid + (size_t)c2 * (L + 1);
+                for (int k = 0; k < L; ++k) {
+                    const double uk = u[(size_t)e * L + k];
+                    um[k] = P->mix_full ? uk : (uk - u_init[k] * cos_edge[e]) - v_init[k] * sin_edge[e];
+                }
+                for (int k = 1; k + 1 < L; ++k) {
+                    const double z1 = 0.5 * (g1[k - 1] + g2[k - 1]), z2 = 0.5 * (g1[k] + g2[k]);
+                    const double z3 = 0.5 * (g1[k + 1] + g2[k + 1]), z4 = 0.5 * (g1[k + 2] + g2[k + 2]);
+                    const double zm = 0.5 * (z1 + z2), z0 = 0.5 * (z2 + z3), zp = 0.5 * (z3 + z4);
+                    const double vm = rho_edge[(size_t)e * L + k] * P->v_mom_eddy_visc2 *
+                                      ((um[k + 1] - um[k]) / (zp - z0) - (um[k] - um[k - 1]) / (z0 - zm)) /
+                                      (0.5 * (zp - zm));
+                    tend_u_euler[(size_t)e * L + k] = tend_u_euler[(size_t)e * L + k] + vm;
+                }
+            }
+        }
     }
     /* Rayleigh damping of u on the top levels, before the Euler part is added
      * (:2969-2994): tend_u -= rho_edge u damp(k), damp(k) = ((k + 1) - k0) / (levels tau) */
@@ -558,6 +589,13 @@ void orc_dyn_tend(const orc_mesh *m, const orc_dyn_params *P,
                 size_t o = (size_t)c * (L + 1) + k, oc = (size_t)c * L;
                 double e0 = wturb[o];
                 e0 = e0 - cqw[o] * (rdzu[k] * (pp[oc + k] - pp[oc + k - 1]) - (fzm[k] * dpdz[oc + k] + fzp[k] * dpdz[oc + k - 1]));
+                if (P->v_mom_eddy_visc2 > 0.0) {
+                    /* vertical mixing of w (:4633-4647) */
+                    const double *wc = w + (size_t)c * (L + 1);
+                    const double rho_half = 0.5 * (rho_zz[oc + k] + rho_zz[oc + k - 1]);
+                    const double w_up = (wc[k + 1] - wc[k]) * rdzw[k], w_down = (wc[k] - wc[k - 1]) * rdzw[k - 1];
+                    e0 = e0 + P->v_mom_eddy_visc2 * rho_half * (w_up - w_down) * rdzu[k];
+                }
                 tend_w_euler[o] = e0;
             }
         free(wturb);
@@ -648,6 +686,23 @@ void orc_dyn_tend(const orc_mesh *m, const orc_dyn_params *P,
                     }
                 }
             }
+        /* vertical mixing of theta (:5060-5097), levels 1..L-2, on theta_m or on
+         * theta_m - t_init */
+        if (P->v_theta_eddy_visc2 > 0.0 && L > 2) {
+            double *tw = colbuf;
+            for (int c = 0; c < nC; ++c) {
+                const size_t oc = (size_t)c * L;
+                const double *zg = zgrid + (size_t)c * (L + 1);
+                for (int k = 0; k < L; ++k) tw[k] = P->mix_full ? theta_m[oc + k] : theta_m[oc + k] - t_init[oc + k];
+                for (int k = 1; k + 1 < L; ++k) {
+                    const double zm = 0.5 * (zg[k - 1] + zg[k]), z0 = 0.5 * (zg[k] + zg[k + 1]);
+                    const double zp = 0.5 * (zg[k + 1] + zg[k + 2]);
+                    const double diff = (tw[k + 1] - tw[k]) / (zp - z0) - (tw[k] - tw[k - 1]) / (z0 - zm);
+                    tend_theta_euler[oc + k] = tend_theta_euler[oc + k] + P->v_theta_eddy_visc2 * P->prandtl_inv *
+                                                                              rho_zz[oc + k] * diff / (0.5 * (zp - zm));
+                }
+            }
+        }
     }
     /* theta: vertical transport, diabatic forcing and final (:4803-5057, :5097) */
     for (int c = 0; c < nC; ++c) {

@@ -56,7 +56,8 @@ class OrcDynParams(C.Structure):
                 ("del4u_div_factor", C.c_double), ("prandtl_inv", C.c_double),
                 ("rayleigh_levels", C.c_int), ("rayleigh_coef_inverse", C.c_double),
                 ("mix_fixed", C.c_int), ("kdiff_fixed", C.c_double), ("cam_levels", C.c_int),
-                ("cam_visc2", C.c_double), ("curvature", C.c_int)]
+                ("cam_visc2", C.c_double), ("curvature", C.c_int), ("v_mom_eddy_visc2", C.c_double),
+                ("v_theta_eddy_visc2", C.c_double), ("mix_full", C.c_int)]
 
 
 class OrcAcousticParams(C.Structure):
@@ -142,6 +143,14 @@ class Mesh:
         else:
             self.curv_edge = np.zeros(ne + 1)
         self.curv_cell = _f64(two_omega * np.cos(_f64(d["latCell"])))
+        # vertical mixing: interface heights and the reference profiles of the
+        # departures (zeros where the mapping has none)
+        self.zgrid = _f64(d["zgrid"]) if "zgrid" in d else np.zeros((nc + 1, L + 1))
+        self.t_init = _f64(d["t_init"]) if "t_init" in d else np.zeros((nc + 1, L))
+        self.u_init = _f64(d["u_init"]) if "u_init" in d else np.zeros(L)
+        self.v_init = _f64(d["v_init"]) if "v_init" in d else np.zeros(L)
+        ang = _f64(d["angleEdge"]) if "angleEdge" in d else np.zeros(ne + 1)
+        self.cos_edge, self.sin_edge = _f64(np.cos(ang)), _f64(np.sin(ang))
 
     @property
     def ref(self):
@@ -197,15 +206,22 @@ DYN_INPUTS = ("u", "v", "w", "ru", "rw", "ru_save", "rw_save", "theta_m", "theta
 def dyn_params(rk_step, dt, len_disp, c_s=0.125, visc4_2dsmag=0.05, coef_3rd_order=0.25,
                del4u_div_factor=10.0, prandtl=1.0, rayleigh_levels=0, rayleigh_timescale_days=5.0,
                horiz_mixing="2d_smagorinsky", h_theta_eddy_visc2=0.0, h_mom_eddy_visc4=0.0,
-               h_theta_eddy_visc4=0.0, cam_coef=0.0, cam_levels=4, curvature=False) -> OrcDynParams:
+               h_theta_eddy_visc4=0.0, cam_coef=0.0, cam_levels=4, curvature=False, v_mom_eddy_visc2=0.0,
+               v_theta_eddy_visc2=0.0, mix_full=True) -> OrcDynParams:
     """Scalar set-up of dyn_tend.py:3367-3458 and :9616-9626, evaluated in Python:
     Smagorinsky or "2d_fixed" mixing coefficients, the CAM damping of kdiff when
     ``cam_coef`` > 0 and ``cam_levels`` > 0, Rayleigh damping of u over
     ``rayleigh_levels`` top levels (0: off) with the coefficient of
     dyn_tend.py:10066-10072; ``curvature`` adds the curvature terms of u and w
-    (config_curvature_u, dyn_tend.py:2471-2505, :4241-4267)."""
+    (config_curvature_u, dyn_tend.py:2471-2505, :4241-4267); the vertical eddy
+    viscosities mix u, w and theta on the first RK stage, with ``mix_full`` False
+    on the departures from u_init/v_init/t_init (dyn_tend.py:2731-2781, :4633-4647,
+    :5060-5097, :10178-10197)."""
     p = OrcDynParams()
     p.curvature = 1 if curvature else 0
+    p.v_mom_eddy_visc2 = float(v_mom_eddy_visc2)
+    p.v_theta_eddy_visc2 = float(v_theta_eddy_visc2)
+    p.mix_full = 1 if mix_full else 0
     p.rk_step = int(rk_step)
     p.dt = float(dt)
     p.coef_3rd_order = float(coef_3rd_order)
@@ -254,7 +270,8 @@ def dyn_tend(m: Mesh, params: OrcDynParams, fields: dict, carried: dict | None =
     ur = _f64(fields.get("ur_cell", np.zeros((nc + 1, L))))
     vr = _f64(fields.get("vr_cell", np.zeros((nc + 1, L))))
     lib().orc_dyn_tend(m.ref, C.byref(params), *[_ptr(x) for x in ins], _ptr(ur), _ptr(vr),
-                       _ptr(m.curv_edge), _ptr(m.curv_cell),
+                       _ptr(m.curv_edge), _ptr(m.curv_cell), _ptr(m.zgrid), _ptr(m.t_init), _ptr(m.u_init),
+                       _ptr(m.v_init), _ptr(m.cos_edge), _ptr(m.sin_edge),
                        *[_ptr(io[k]) for k in ("tend_rho", "tend_u_euler", "tend_w_euler", "tend_theta_euler", "kdiff")],
                        *[_ptr(out[k]) for k in ("tend_u", "tend_w", "tend_theta", "h_divergence", "rthdynten")])
     out.update(io)

@@ -81,6 +81,18 @@ OPTION_CASES = {
                    {"config_mpas_cam_coef": 0.02, "config_number_cam_damping_levels": 6}),
     # curvature terms of u and w (config_curvature_u)
     "l2_L20_curvature": (2, 20, "quadratic", 1, 900.0, 2, 9292, {"config_curvature_u": True}),
+    # vertical eddy viscosities of u, w and theta on the full fields, and on their
+    # departures from the reference profiles (config_mix_full = False)
+    "l2_L20_vmix": (2, 20, "quadratic", 1, 900.0, 2, 9393,
+                    {"config_v_mom_eddy_visc2": 4.0, "config_v_theta_eddy_visc2": 2.0}),
+    "l2_L20_vmix_dep": (2, 20, "quadratic", 1, 900.0, 2, 9494,
+                        {"config_v_mom_eddy_visc2": 4.0, "config_v_theta_eddy_visc2": 2.0,
+                         "config_mix_full": False}),
+}
+# reference profiles the departure case mixes against: u_init and v_init are zero
+# in the synthetic initial state, so that case gives them a shape of their own
+STATE_EDITS = {
+    "l2_L20_vmix_dep": lambda L: {"u_init": np.linspace(5.0, 25.0, L), "v_init": np.linspace(-4.0, 4.0, L)},
 }
 SUMMARY_CASES = {
     "cfg1_l4_L26": (4, 26, "uniform", 0, 1800.0, 2, 20240),
@@ -322,11 +334,14 @@ def _state_arrays(st: dict, keys=None) -> dict:
     return out
 
 
-def run_case(level, n_levels, stretch, n_tracers, dt, steps, seed, terrain=0.0, defaults=False, extra=None):
+def run_case(level, n_levels, stretch, n_tracers, dt, steps, seed, terrain=0.0, defaults=False, extra=None,
+             state_edit=None):
     m = M.build_mesh(level, n_levels, stretch=stretch)
     if terrain:
         m = add_synthetic_terrain(m, terrain)
     raw = IS.raw_state(m, seed=seed, n_tracers=n_tracers)
+    if state_edit is not None:
+        raw.update(state_edit(n_levels))
     st, mesh = prepared_reference_state(m, raw, dt)
     cfg = R.cfg.validate_config({"config_dt": float(dt)}) if defaults else reference_config(m, dt, extra)
     prepared = dict(st)
@@ -351,7 +366,7 @@ def write_srk3_case(name: str, out_dir: str) -> None:
         level, L, stretch, ntr, dt, steps, seed, *terrain = (CASES.get(name) or TERRAIN_CASES.get(name)
                                                              or DEFAULT_CONFIG_CASES[name])
     m, raw, prepared, outs, times = run_case(level, L, stretch, ntr, dt, steps, seed, *(terrain or [0.0]),
-                                             defaults=defaults, extra=extra)
+                                             defaults=defaults, extra=extra, state_edit=STATE_EDITS.get(name))
     payload = {}
     for k, v in _state_arrays(prepared).items():
         payload["in__" + k] = v

@@ -111,8 +111,13 @@ def test_config_gate():
     no_angles = {k: v for k, v in st.items() if k not in ("angleEdge", "latEdge")}
     with pytest.raises(MpasMissingFieldError):
         resolve_config({**ok, "config_curvature_u": True}, no_angles, no_angles, meta["dt"], 2)
-    for key, value in (("config_v_mom_eddy_visc2", 1.0), ("config_v_theta_eddy_visc2", 1.0),
-                       ("config_mix_full", False)):
+    # vertical eddy viscosities (dyn_tend.py:2731-2781, :4633-4647, :5060-5097)
+    cfg = resolve_config({**ok, "config_v_mom_eddy_visc2": 4.0, "config_v_theta_eddy_visc2": 2.0, "mix_full": False},
+                         st, st, meta["dt"], 2)
+    assert (cfg.v_mom_eddy_visc2, cfg.v_theta_eddy_visc2, cfg.mix_full) == (4.0, 2.0, 0)
+    assert resolve_config(ok, st, st, meta["dt"], 2).mix_full == 1
+    for key, value in (("config_split_dynamics_transport", False), ("config_positive_definite", True),
+                       ("config_apply_lbcs", True)):
         with pytest.raises(MpasUnsupportedCapabilityError):
             resolve_config({**ok, key: value}, st, st, meta["dt"], 2)
     # Rayleigh damping of u: the reference's coefficient 1 / (levels * days * 86400)

@@ -10,7 +10,11 @@
   ``dyn_tend.py:3434-3458``);
 * the curvature terms of u and w (``config_curvature_u``; ``dyn_tend.py:2471-2505``,
   ``:4241-4267``), which also make the step differentiable in the input state's
-  reconstructed winds ``ur_cell`` / ``vr_cell``.
+  reconstructed winds ``ur_cell`` / ``vr_cell``;
+* the vertical eddy viscosities of u, w and theta (``config_v_mom_eddy_visc2`` 4,
+  ``config_v_theta_eddy_visc2`` 2; ``dyn_tend.py:2731-2781``, ``:4633-4647``,
+  ``:5060-5097``) on the full fields and, with ``config_mix_full=False``, on their
+  departures from the reference profiles u_init / v_init / t_init.
 
 For each: two RK3 steps against the reference's own recording of the same case
 (``tests/golden/srk3_l2_L20_<option>.npz``), per field at the whole-step bar,
@@ -40,9 +44,10 @@ def _config(st, meta, damp=True):
     return cfg
 
 
-CASES = ("l2_L20_rayleigh", "l2_L20_fixedmix", "l2_L20_cam", "l2_L20_curvature")
+CASES = ("l2_L20_rayleigh", "l2_L20_fixedmix", "l2_L20_cam", "l2_L20_curvature", "l2_L20_vmix", "l2_L20_vmix_dep")
 # how far the step without the option lands from the recording after one step (u)
-MISS = {"l2_L20_rayleigh": 1e-6, "l2_L20_fixedmix": 1e-3, "l2_L20_cam": 1e-6, "l2_L20_curvature": 1e-9}
+MISS = {"l2_L20_rayleigh": 1e-6, "l2_L20_fixedmix": 1e-3, "l2_L20_cam": 1e-6, "l2_L20_curvature": 1e-9,
+        "l2_L20_vmix": 1e-5, "l2_L20_vmix_dep": 1e-4}
 
 
 def _oracle_options(extra: dict) -> dict:
@@ -54,7 +59,9 @@ def _oracle_options(extra: dict) -> dict:
     for key, name in (("config_horiz_mixing", "horiz_mixing"), ("config_h_theta_eddy_visc2", "h_theta_eddy_visc2"),
                       ("config_h_mom_eddy_visc4", "h_mom_eddy_visc4"),
                       ("config_h_theta_eddy_visc4", "h_theta_eddy_visc4"),
-                      ("config_mpas_cam_coef", "cam_coef"), ("config_number_cam_damping_levels", "cam_levels")):
+                      ("config_mpas_cam_coef", "cam_coef"), ("config_number_cam_damping_levels", "cam_levels"),
+                      ("config_v_mom_eddy_visc2", "v_mom_eddy_visc2"),
+                      ("config_v_theta_eddy_visc2", "v_theta_eddy_visc2"), ("config_mix_full", "mix_full")):
         if key in extra:
             mixing[name] = extra[key]
     if mixing:
@@ -71,8 +78,11 @@ def _check_option(cfg, case: str) -> None:
         assert cfg.mix_fixed == 1 and cfg.h_theta_eddy_visc2 == 1.0e8 and cfg.cam_levels == 0
     elif case == "l2_L20_cam":
         assert cfg.mix_fixed == 0 and cfg.cam_levels == 6 and cfg.cam_visc2 > 0.0
-    else:
+    elif case == "l2_L20_curvature":
         assert cfg.curvature == 1 and cfg.cam_levels == 0 and cfg.mix_fixed == 0
+    else:
+        assert cfg.v_mom_eddy_visc2 == 4.0 and cfg.v_theta_eddy_visc2 == 2.0
+        assert cfg.mix_full == (0 if case == "l2_L20_vmix_dep" else 1)
 
 
 @pytest.mark.parametrize("case", CASES)

@@ -113,18 +113,22 @@ def _mixing_kwargs(extra: dict) -> dict:
     for key, name in (("config_h_theta_eddy_visc2", "h_theta_eddy_visc2"),
                       ("config_h_mom_eddy_visc4", "h_mom_eddy_visc4"),
                       ("config_h_theta_eddy_visc4", "h_theta_eddy_visc4"),
-                      ("config_mpas_cam_coef", "cam_coef"), ("config_number_cam_damping_levels", "cam_levels")):
+                      ("config_mpas_cam_coef", "cam_coef"), ("config_number_cam_damping_levels", "cam_levels"),
+                      ("config_v_mom_eddy_visc2", "v_mom_eddy_visc2"),
+                      ("config_v_theta_eddy_visc2", "v_theta_eddy_visc2"), ("config_mix_full", "mix_full")):
         if key in extra:
             kw[name] = extra[key]
     return kw
 
 
-@pytest.mark.parametrize("case", ["l2_L20_fixedmix", "l2_L20_cam", "l2_L20_curvature"])
+@pytest.mark.parametrize("case", ["l2_L20_fixedmix", "l2_L20_cam", "l2_L20_curvature", "l2_L20_vmix",
+                                  "l2_L20_vmix_dep"])
 def test_srk3_whole_steps_mixing_options(case):
     """"2d_fixed" horizontal mixing (constant kdiff, separate del4 coefficients for
-    momentum and theta), the CAM damping of kdiff on the top 6 levels and the
-    curvature terms of u and w, over two RK3 steps: the oracle against the
-    reference's recordings at 1e-14.  Without the option the oracle misses the
+    momentum and theta), the CAM damping of kdiff on the top 6 levels, the
+    curvature terms of u and w, and the vertical eddy viscosities on the full
+    fields and on their departures from the reference profiles, over two RK3
+    steps: the oracle against the reference's recordings at 1e-14.  Without the option the oracle misses the
     recording, so each option is exercised."""
     st, outs, meta = srk3_case(case)
     extra = meta["config_extra"]
