@@ -110,6 +110,8 @@ __device__ __forceinline__ double dyn_g_theta_base(Fwd F, Bar B, long long o) {
 // One wavefront per cell c; every lane runs (lanes past interface L hold
 // nothing).  Lane k forms the shares its terms owe levels / interfaces k-2 ..
 // k+2 of the column; after the exchange each lane adds its own sums once.
+// kOpt builds in the curvature and vertical-mixing terms.
+template <bool kOpt>
 __device__ __forceinline__ void dyn_a1(const mpas_hip_mesh &m, const mpas_hip_dyn_params &P, Fwd F, Bar B,
                                        const DynAdjScratch &S, int c, int k) {
     using namespace dyn;
@@ -125,7 +127,8 @@ __device__ __forceinline__ void dyn_a1(const mpas_hip_mesh &m, const mpas_hip_dy
     double r_0 = 0.0, r_p1 = 0.0, rs_0 = 0.0, w_m1 = 0.0, w_0 = 0.0, w_p1 = 0.0, w_p2 = 0.0;
     // rho_zz at levels k and k-1 (curvature, vertical mixing); ur_cell, vr_cell (curvature)
     double cz_0 = 0.0, cz_m1 = 0.0, cu_0 = 0.0, cu_m1 = 0.0, cv_0 = 0.0, cv_m1 = 0.0;
-    const bool vmix_w = rk1 && P.v_mom_eddy_visc2 > 0.0, vmix_t = rk1 && P.v_theta_eddy_visc2 > 0.0;
+    const bool vmix_w = kOpt && rk1 && P.v_mom_eddy_visc2 > 0.0;
+    const bool vmix_t = kOpt && rk1 && P.v_theta_eddy_visc2 > 0.0;
     // ------------------------------------------------ theta, level k
     if (k < L) {
         const long long o = oc + k;
@@ -260,7 +263,7 @@ __device__ __forceinline__ void dyn_a1(const mpas_hip_mesh &m, const mpas_hip_dy
     // curvature of w at interface k (1..L-1), added to the advective sum before
     // its 1/area scaling: rh (ur^2 + vr^2) / r_earth + curvCell ur rh, with ur, vr
     // and rh interpolated from levels k and k-1
-    if (P.curvature && k >= 1 && k < L) {
+    if (kOpt && P.curvature && k >= 1 && k < L) {
         const double gh = ia * bar_of(B, tend_w, i);
         const double *ur = F[ur_cell] + oc, *vr = F[vr_cell] + oc, *rz = F[rho_zz] + oc;
         const double uri = fzm[k] * ur[k] + fzp[k] * ur[k - 1], vri = fzm[k] * vr[k] + fzp[k] * vr[k - 1];
@@ -276,14 +279,16 @@ __device__ __forceinline__ void dyn_a1(const mpas_hip_mesh &m, const mpas_hip_dy
         cv_m1 = fzp[k] * gvr;
     }
     // ------------------------------------------------ exchange, then one write per element
-    const double g_cz = cz_0 + from_lane(cz_m1, 1, k, L);
-    const double g_cu = cu_0 + from_lane(cu_m1, 1, k, L);
-    const double g_cv = cv_0 + from_lane(cv_m1, 1, k, L);
-    if (P.curvature && k < L) {
-        add_own(B, ur_cell, oc + k, g_cu);
-        add_own(B, vr_cell, oc + k, g_cv);
+    if (kOpt) {
+        const double g_cz = cz_0 + from_lane(cz_m1, 1, k, L);
+        const double g_cu = cu_0 + from_lane(cu_m1, 1, k, L);
+        const double g_cv = cv_0 + from_lane(cv_m1, 1, k, L);
+        if (P.curvature && k < L) {
+            add_own(B, ur_cell, oc + k, g_cu);
+            add_own(B, vr_cell, oc + k, g_cv);
+        }
+        if ((P.curvature || vmix_w || vmix_t) && k < L) add_own(B, rho_zz, oc + k, g_cz);
     }
-    if ((P.curvature || vmix_w || vmix_t) && k < L) add_own(B, rho_zz, oc + k, g_cz);
     const double g_th = th_0 + from_lane(th_m1, 1, k, L) + from_lane(th_m2, 2, k, L) + from_lane(th_p1, -1, k, L);
     const double g_ts = ts_0 + from_lane(ts_m1, 1, k, L);
     const double g_pp = pp_0 + from_lane(pp_m1, 1, k, L);

@@ -90,6 +90,13 @@ using AcousticParams = mpas_hip_acoustic_params;
 // Earth radius of the curvature terms (the reference's constants.py:84)
 constexpr double kEarthRadius = 6371229.0;
 
+// Whether a dynamics tendency runs with the curvature terms or the vertical
+// eddy viscosities: the passes that carry them are built twice, and the plain
+// build of each (without their code, so with fewer registers) runs otherwise.
+inline bool dyn_options(const DynParams &P) {
+    return P.curvature || P.v_mom_eddy_visc2 > 0.0 || P.v_theta_eddy_visc2 > 0.0;
+}
+
 // Heights of the vertical mixing stencil at level k (1..L-2): zm, z0, zp are the
 // midpoints of interfaces (k-1, k), (k, k+1), (k+1, k+2) of a cell column
 // (dyn_tend.py:5073-5080), or of the mean of an edge's two cell columns

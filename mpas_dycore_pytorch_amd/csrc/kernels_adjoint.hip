@@ -57,7 +57,8 @@ ADJ_WAVE_KERNEL(k_adj_mono_r7g, mono_r7g(m, A.inv, A.S, A.s, ADJ_F, ADJ_B, A.mo,
 ADJ_WAVE_KERNEL(k_adj_mono_r8, mono_r8(m, A.dt, ADJ_F, ADJ_B, ent, k))
 ADJ_WAVE_KERNEL(k_adj_mono_r8e, mono_r8e(m, A.dt, ADJ_B, ent, k))
 
-ADJ_COLUMN_KERNEL(k_adj_dyn_a1, dyn_a1(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
+ADJ_COLUMN_KERNEL(k_adj_dyn_a1, dyn_a1<false>(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
+ADJ_COLUMN_KERNEL(k_adj_dyn_a1_opt, dyn_a1<true>(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
 
 // ---------------------------------------------------------------------------
 // Dynamics-tendency edge adjoint (A3), one wavefront per edge (lane = level):
@@ -71,8 +72,9 @@ ADJ_COLUMN_KERNEL(k_adj_dyn_a1, dyn_a1(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
 // All 64 lanes run through the shuffles; lanes past the column compute on a
 // clamped level and contribute nothing.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_ent, int n_lev, mpas_hip_mesh m,
-                                                     Slots S_, KArgs A) {
+// kOpt builds in the curvature and vertical-mixing terms
+template <bool kOpt>
+__device__ __forceinline__ void adj_dyn_a3w(long long nb, mpas_hip_mesh m, Slots S_, KArgs A) {
     using namespace dyn;
     const int e = wave_entity_base(nb);
     if (e >= m.nEdges) return;  // wave-uniform
@@ -220,7 +222,7 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
             g_u_own += -(re * damp) * gtu;
         }
         // curvature: tend_u -= curvEdge re w_avg + u w_avg re / r_earth
-        if (P.curvature) {
+        if (kOpt && P.curvature) {
             const double *w1 = F[w] + (long long)c1 * Lw, *w2 = F[w] + (long long)c2 * Lw;
             const double wa = 0.25 * (((w1[k] + w1[k + 1]) + w2[k]) + w2[k + 1]);
             const double ce = m.curvEdge[e], ir = 1.0 / kEarthRadius;
@@ -284,7 +286,7 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
             x_dd = re * r_dc * gtue;
             x_dv = re * r_dv * gtue;
         }
-        if (P.v_mom_eddy_visc2 > 0.0 && k >= 1 && k <= L - 2) {
+        if (kOpt && P.v_mom_eddy_visc2 > 0.0 && k >= 1 && k <= L - 2) {
             // vertical mixing of u: re visc ((c - b)/(zp - z0) - (b - a)/(z0 - zm)) / h,
             // a, b, c the mixed u of levels k-1, k, k+1
             const VmixZ z = vmix_z_edge(m.zgrid + c1 * Lw, m.zgrid + c2 * Lw, k);
@@ -321,7 +323,7 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
     S.x[0][i] = x_ke;
     S.x[1][i] = x_hd;
     S.x[2][i] = (k >= 1) ? x_rw : 0.0;
-    if (P.curvature) S.x[8][i] = x_cw;
+    if (kOpt && P.curvature) S.x[8][i] = x_cw;
     if (rk1) {
         S.x[3][i] = x_pp;
         S.x[4][i] = x_dp;
@@ -334,6 +336,14 @@ __global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_e
     } else {
         S.x[3][i] = x_ths;
     }
+}
+__global__ __launch_bounds__(256) void k_adj_dyn_a3w(long long nb, long long n_ent, int n_lev, mpas_hip_mesh m,
+                                                     Slots S_, KArgs A) {
+    adj_dyn_a3w<false>(nb, m, S_, A);
+}
+__global__ __launch_bounds__(256) void k_adj_dyn_a3w_opt(long long nb, long long n_ent, int n_lev, mpas_hip_mesh m,
+                                                         Slots S_, KArgs A) {
+    adj_dyn_a3w<true>(nb, m, S_, A);
 }
 
 // ---------------------------------------------------------------------------
@@ -962,13 +972,19 @@ int mpas_hip_dyn_tend_vjp(const mpas_hip_mesh *mesh, const mpas_hip_dyn_params *
         ADJ_LAUNCH(k_adj_dyn_fwd_edges, nE, L, s);
         ADJ_LAUNCH(k_adj_dyn_fwd_del4, ncv, L, s);
     }
-    ADJ_LAUNCH(k_adj_dyn_a1, nC, L + 1, s);
+    if (dyn_options(*p))
+        ADJ_LAUNCH(k_adj_dyn_a1_opt, nC, L + 1, s);
+    else
+        ADJ_LAUNCH(k_adj_dyn_a1, nC, L + 1, s);
     if (rk1) {
         ADJ_LAUNCH(k_adj_dyn_a1_del4, nC, L + 1, s);
         ADJ_LAUNCH(k_adj_dyn_a2_edges, nE, L, s);
         ADJ_LAUNCH(k_adj_dyn_a2_cells, nC, L, s);
     }
-    ADJ_LAUNCH(k_adj_dyn_a3w, nE, L, s);
+    if (dyn_options(*p))
+        ADJ_LAUNCH(k_adj_dyn_a3w_opt, nE, L, s);
+    else
+        ADJ_LAUNCH(k_adj_dyn_a3w, nE, L, s);
     ADJ_LAUNCH(k_adj_dyn_a3_cells, del4 ? ncv : nC, L, s);
     ADJ_LAUNCH(k_adj_dyn_a3_edges, nE, L, s);
     ADJ_LAUNCH(k_adj_dyn_a6, rk1 ? ncv : nC, L, s);

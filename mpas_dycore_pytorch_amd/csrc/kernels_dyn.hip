@@ -51,7 +51,7 @@ static DynScratch carve(const mpas_hip_mesh &m, double *base) {
 
 // Smagorinsky eddy viscosity of cell c at level k: min(coef def, cap), def the
 // horizontal deformation summed over the cell's edges in slot order
-__device__ __forceinline__ double smagorinsky_kdiff(const mpas_hip_mesh &m, const DynParams &P,
+__device__ __forceinline__ double smagorinsky_kdiff(mpas_hip_mesh m, DynParams P,
                                                     const double *__restrict__ u, const hreal *__restrict__ v,
                                                     int c, int k) {
     const int L = m.nLevels, ME = m.maxEdges;
@@ -245,7 +245,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_a(long long nb, mpa
 
 // Vertical mixing of u at level k (1..L-2) of edge e (dyn_tend.py:2731-2781),
 // from u at levels k-1, k, k+1; added to tend_u_euler after the del4 mixing
-__device__ __forceinline__ double u_vertical_mix(const mpas_hip_mesh &m, const DynParams &P, int e, int k, double re,
+__device__ __forceinline__ double u_vertical_mix(mpas_hip_mesh m, DynParams P, int e, int k, double re,
                                                  double u_m1, double u_0, double u_p1) {
     const int Lw = m.nLevels + 1;
     const VmixZ z = vmix_z_edge(m.zgrid + m.cellsOnEdge[2 * e] * Lw, m.zgrid + m.cellsOnEdge[2 * e + 1] * Lw, k);
@@ -255,7 +255,9 @@ __device__ __forceinline__ double u_vertical_mix(const mpas_hip_mesh &m, const D
 
 // P4: normal-velocity tendency (vertical flux, PV flux, KE gradient,
 // divergence coupling); first stage also PGF and del2 mixing into tend_u_euler.
-__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_a(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ u,
+// kOpt builds in the curvature and vertical-mixing terms
+template <bool kOpt>
+__device__ __forceinline__ void dyn_edges_a_body(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ u,
                               const double *__restrict__ rw, const double *__restrict__ w,
                               const hreal *__restrict__ pv_edge,
                               const double *__restrict__ ke, const double *__restrict__ h_div,
@@ -332,7 +334,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_a(long long nb, mpa
                                      (freal)ue[k] * (freal)0.5 * hsum);
         if (lane >= L) return;
         double tu = vert + base;
-        if (P.curvature) {
+        if (kOpt && P.curvature) {
             // curvature (dyn_tend.py:2471-2505): tu -= curvEdge rho_edge w_avg + u w_avg rho_edge / r_earth
             const double *w1 = w + c1 * Lw, *w2 = w + c2 * Lw;
             const double w_avg = 0.25 * (((w1[k] + w1[k + 1]) + w2[k]) + w2[k + 1]);
@@ -360,7 +362,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_a(long long nb, mpa
             double kdiffu = 0.5 * (kdiff[c1 * L + k] + kdiff[c2 * L + k]);
             eu = eu + rho_edge[i] * kdiffu * ud * m.meshScalingDel2[e];
             // without a del4 pass the vertical mixing closes tend_u_euler here
-            if (add_euler && P.v_mom_eddy_visc2 > 0.0 && k >= 1 && k <= L - 2)
+            if (kOpt && add_euler && P.v_mom_eddy_visc2 > 0.0 && k >= 1 && k <= L - 2)
                 eu = eu + u_vertical_mix(m, P, e, k, (double)rho_edge[i], um1, u0, up1);
             tend_u_euler[i] = eu;
         } else if (add_euler) {
@@ -373,6 +375,30 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_a(long long nb, mpa
     const int base_ = wave_entity_base(nb);
 #pragma unroll 1
     for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
+}
+__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_a(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ u,
+                              const double *__restrict__ rw, const double *__restrict__ w,
+                              const hreal *__restrict__ pv_edge,
+                              const double *__restrict__ ke, const double *__restrict__ h_div,
+                              const hreal *__restrict__ rho_edge, const double *__restrict__ pp,
+                              const double *__restrict__ dpdz, const hreal *__restrict__ cqu,
+                              const double *__restrict__ divergence, const hreal *__restrict__ vorticity,
+                              const double *__restrict__ kdiff, hreal *__restrict__ tend_u,
+                              hreal *__restrict__ tend_u_euler, hreal *__restrict__ delsq_u, int add_euler) {
+    dyn_edges_a_body<false>(nb, m, P, u, rw, w, pv_edge, ke, h_div, rho_edge, pp, dpdz, cqu, divergence,
+                            vorticity, kdiff, tend_u, tend_u_euler, delsq_u, add_euler);
+}
+__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edges_a_opt(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ u,
+                              const double *__restrict__ rw, const double *__restrict__ w,
+                              const hreal *__restrict__ pv_edge,
+                              const double *__restrict__ ke, const double *__restrict__ h_div,
+                              const hreal *__restrict__ rho_edge, const double *__restrict__ pp,
+                              const double *__restrict__ dpdz, const hreal *__restrict__ cqu,
+                              const double *__restrict__ divergence, const hreal *__restrict__ vorticity,
+                              const double *__restrict__ kdiff, hreal *__restrict__ tend_u,
+                              hreal *__restrict__ tend_u_euler, hreal *__restrict__ delsq_u, int add_euler) {
+    dyn_edges_a_body<true>(nb, m, P, u, rw, w, pv_edge, ke, h_div, rho_edge, pp, dpdz, cqu, divergence,
+                           vorticity, kdiff, tend_u, tend_u_euler, delsq_u, add_euler);
 }
 
 // P5: del2 of the u Laplacian at vertices and cells (del4 first half); one
@@ -521,7 +547,9 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_edge_adv(long long nb, mp
 
 // P7: w and theta tendencies (advection, mixing finish, PGF/buoyancy,
 // vertical transport, diabatic forcing), one thread per (cell, w-level).
-__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_b(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ wflux,
+// kOpt builds in the curvature and vertical-mixing terms
+template <bool kOpt>
+__device__ __forceinline__ void dyn_cells_b_body(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ wflux,
                               const double *__restrict__ rw, const double *__restrict__ tflux,
                               const double *__restrict__ rw_save, const double *__restrict__ tpert,
                               const double *__restrict__ w,
@@ -615,7 +643,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_b(long long nb, mpa
                 const double cc = -sg[j] * fv[j];
                 hadv = (j < n) ? ((j == 0) ? cc : hadv + cc) : hadv;
             }
-            if (P.curvature) {
+            if (kOpt && P.curvature) {
                 // curvature (dyn_tend.py:4241-4267), added to the advective sum
                 // before its 1/area scaling
                 const double ur = fzm[k] * ur_cell[oc + k] + fzp[k] * ur_cell[oc + k - 1];
@@ -646,7 +674,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_b(long long nb, mpa
                     }
                 }
                 eu = eu - cqw[i] * (m.rdzu[k] * (pp0 - pp_m1) - (fzm[k] * dp0 + fzp[k] * dp_m1));
-                if (P.v_mom_eddy_visc2 > 0.0) {
+                if (kOpt && P.v_mom_eddy_visc2 > 0.0) {
                     // vertical mixing of w (dyn_tend.py:4633-4647)
                     const double rho_half = 0.5 * (rho_zz[oc + k] + rho_zz[oc + k - 1]);
                     const double w_up = (w_p1 - w0) * m.rdzw[k], w_down = (w0 - w_m1) * m.rdzw[k - 1];
@@ -708,7 +736,7 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_b(long long nb, mpa
                 }
                 tend_theta_euler[o] = tte;
             }
-            if (rk1 && P.v_theta_eddy_visc2 > 0.0 && k >= 1 && k <= L - 2) {
+            if (kOpt && rk1 && P.v_theta_eddy_visc2 > 0.0 && k >= 1 && k <= L - 2) {
                 // vertical mixing of theta_m, or of theta_m - t_init (dyn_tend.py:5060-5097)
                 const VmixZ z = vmix_z_cell(m.zgrid + c * Lw, k);
                 const double *ti = m.t_init + oc;
@@ -732,6 +760,40 @@ __global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_b(long long nb, mpa
 #pragma unroll 1
     for (int t_ = 0; t_ < kEntPerWave; ++t_) body_(base_ + t_);
 }
+__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_b(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ wflux,
+                              const double *__restrict__ rw, const double *__restrict__ tflux,
+                              const double *__restrict__ rw_save, const double *__restrict__ tpert,
+                              const double *__restrict__ w,
+                              const double *__restrict__ theta_m, const double *__restrict__ theta_m_save,
+                              const double *__restrict__ rho_zz, const double *__restrict__ ur_cell,
+                              const double *__restrict__ vr_cell, const double *__restrict__ pp,
+                              const double *__restrict__ cqw, const double *__restrict__ rt_diab,
+                              const double *__restrict__ dpdz, const double *__restrict__ delsq_w,
+                              const double *__restrict__ delsq_theta, const double *__restrict__ tend_rho,
+                              double *__restrict__ tend_w_euler, double *__restrict__ tend_theta_euler,
+                              double *__restrict__ tend_w, double *__restrict__ tend_theta,
+                              double *__restrict__ rthdynten) {
+    dyn_cells_b_body<false>(nb, m, P, wflux, rw, tflux, rw_save, tpert, w, theta_m, theta_m_save, rho_zz,
+                            ur_cell, vr_cell, pp, cqw, rt_diab, dpdz, delsq_w, delsq_theta, tend_rho,
+                            tend_w_euler, tend_theta_euler, tend_w, tend_theta, rthdynten);
+}
+__global__ __launch_bounds__(kMaxWaveBlock) void k_dyn_cells_b_opt(long long nb, mpas_hip_mesh m, DynParams P, const double *__restrict__ wflux,
+                              const double *__restrict__ rw, const double *__restrict__ tflux,
+                              const double *__restrict__ rw_save, const double *__restrict__ tpert,
+                              const double *__restrict__ w,
+                              const double *__restrict__ theta_m, const double *__restrict__ theta_m_save,
+                              const double *__restrict__ rho_zz, const double *__restrict__ ur_cell,
+                              const double *__restrict__ vr_cell, const double *__restrict__ pp,
+                              const double *__restrict__ cqw, const double *__restrict__ rt_diab,
+                              const double *__restrict__ dpdz, const double *__restrict__ delsq_w,
+                              const double *__restrict__ delsq_theta, const double *__restrict__ tend_rho,
+                              double *__restrict__ tend_w_euler, double *__restrict__ tend_theta_euler,
+                              double *__restrict__ tend_w, double *__restrict__ tend_theta,
+                              double *__restrict__ rthdynten) {
+    dyn_cells_b_body<true>(nb, m, P, wflux, rw, tflux, rw_save, tpert, w, theta_m, theta_m_save, rho_zz,
+                           ur_cell, vr_cell, pp, cqw, rt_diab, dpdz, delsq_w, delsq_theta, tend_rho,
+                           tend_w_euler, tend_theta_euler, tend_w, tend_theta, rthdynten);
+}
 
 void launch_dyn_tend(const mpas_hip_mesh &m, const DynParams &P, const DynInputs &in, const DynCarried &io,
                      const DynOutputs &out, double *scratch, hipStream_t s) {
@@ -747,9 +809,15 @@ void launch_dyn_tend(const mpas_hip_mesh &m, const DynParams &P, const DynInputs
     // Without a del4 pass (rk_step > 1, or no fourth-order mixing) the edge
     // pass completes tend_u itself and the del4/finish pass is skipped.
     const int do_del4 = (rk1 && P.h_mom_eddy_visc4 > 0.0) ? 1 : 0;
-    MPAS_WAVE_LAUNCH(k_dyn_edges_a, nE + 1, s, m, P, in.u,
-                       in.rw, in.w, in.pv_edge, in.ke, out.h_divergence, in.rho_edge, in.pp, sc.dpdz, in.cqu, in.divergence,
-                       in.vorticity, io.kdiff, out.tend_u, io.tend_u_euler, sc.delsq_u, do_del4 ? 0 : 1);
+    const bool opt = dyn_options(P);
+    if (opt)
+        MPAS_WAVE_LAUNCH(k_dyn_edges_a_opt, nE + 1, s, m, P, in.u, in.rw, in.w, in.pv_edge, in.ke, out.h_divergence,
+                         in.rho_edge, in.pp, sc.dpdz, in.cqu, in.divergence, in.vorticity, io.kdiff, out.tend_u,
+                         io.tend_u_euler, sc.delsq_u, do_del4 ? 0 : 1);
+    else
+        MPAS_WAVE_LAUNCH(k_dyn_edges_a, nE + 1, s, m, P, in.u, in.rw, in.w, in.pv_edge, in.ke, out.h_divergence,
+                         in.rho_edge, in.pp, sc.dpdz, in.cqu, in.divergence, in.vorticity, io.kdiff, out.tend_u,
+                         io.tend_u_euler, sc.delsq_u, do_del4 ? 0 : 1);
     if (do_del4) {
         MPAS_WAVE_LAUNCH(k_dyn_del4_vc, nC > nV ? nC : nV, s, m, sc.delsq_u, sc.delsq_vort, sc.delsq_div);
         MPAS_WAVE_LAUNCH(k_dyn_edges_b, nE, s, m, P, do_del4, in.u, in.rho_edge, sc.delsq_div, sc.delsq_vort,
@@ -757,11 +825,16 @@ void launch_dyn_tend(const mpas_hip_mesh &m, const DynParams &P, const DynInputs
     }
     MPAS_WAVE_LAUNCH(k_dyn_edge_adv, nE, s, m, P, in.ru, in.ru_save,
                 in.w, in.theta_m, in.theta_m_save, sc.wflux, sc.tflux, sc.tpert);
-    MPAS_WAVE_LAUNCH(k_dyn_cells_b, nC + 1, s, m, P,
-                       sc.wflux, in.rw, sc.tflux, in.rw_save, sc.tpert, in.w, in.theta_m, in.theta_m_save, in.rho_zz,
-                       in.ur_cell, in.vr_cell, in.pp,
-                       in.cqw, in.rt_diabatic_tend, sc.dpdz, sc.delsq_w, sc.delsq_theta, io.tend_rho,
-                       io.tend_w_euler, io.tend_theta_euler, out.tend_w, out.tend_theta, out.rthdynten);
+    if (opt)
+        MPAS_WAVE_LAUNCH(k_dyn_cells_b_opt, nC + 1, s, m, P, sc.wflux, in.rw, sc.tflux, in.rw_save, sc.tpert, in.w,
+                         in.theta_m, in.theta_m_save, in.rho_zz, in.ur_cell, in.vr_cell, in.pp, in.cqw,
+                         in.rt_diabatic_tend, sc.dpdz, sc.delsq_w, sc.delsq_theta, io.tend_rho, io.tend_w_euler,
+                         io.tend_theta_euler, out.tend_w, out.tend_theta, out.rthdynten);
+    else
+        MPAS_WAVE_LAUNCH(k_dyn_cells_b, nC + 1, s, m, P, sc.wflux, in.rw, sc.tflux, in.rw_save, sc.tpert, in.w,
+                         in.theta_m, in.theta_m_save, in.rho_zz, in.ur_cell, in.vr_cell, in.pp, in.cqw,
+                         in.rt_diabatic_tend, sc.dpdz, sc.delsq_w, sc.delsq_theta, io.tend_rho, io.tend_w_euler,
+                         io.tend_theta_euler, out.tend_w, out.tend_theta, out.rthdynten);
 }
 
 }  // namespace mpas

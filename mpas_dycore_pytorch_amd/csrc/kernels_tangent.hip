@@ -28,8 +28,10 @@ ADJ_WAVE_KERNEL(k_tan_acoustic_edge, acoustic_edge_jvp(m, A.ap, ADJ_F, ADJ_B, en
 ADJ_WAVE_KERNEL(k_tan_dyn_j1, dyn_j1(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_tan_dyn_j2, dyn_j2(m, ADJ_B, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_tan_dyn_j3, dyn_j3(m, A.dy, ent, k))
-ADJ_WAVE_KERNEL(k_tan_dyn_j4, dyn_j4(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
-ADJ_WAVE_KERNEL(k_tan_dyn_j5, dyn_j5(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
+ADJ_WAVE_KERNEL(k_tan_dyn_j4, dyn_j4<false>(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
+ADJ_WAVE_KERNEL(k_tan_dyn_j5, dyn_j5<false>(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
+ADJ_WAVE_KERNEL(k_tan_dyn_j4_opt, dyn_j4<true>(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
+ADJ_WAVE_KERNEL(k_tan_dyn_j5_opt, dyn_j5<true>(m, A.dp, ADJ_F, ADJ_B, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_tan_dyn_fwd_cells, dyn_fwd_cells(m, ADJ_F, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_tan_dyn_fwd_edges, dyn_fwd_edges(m, ADJ_F, A.dy, ent, k))
 ADJ_WAVE_KERNEL(k_tan_dyn_fwd_del4, dyn_fwd_del4(m, A.dy, ent, k))
@@ -520,8 +522,13 @@ int mpas_hip_dyn_tend_jvp(const mpas_hip_mesh *mesh, const mpas_hip_dyn_params *
     ADJ_LAUNCH(k_tan_dyn_j1, nC + 1, L + 1, s);
     if (rk1) ADJ_LAUNCH(k_tan_dyn_j2, nE + 1, L, s);
     if (rk1 && p->h_mom_eddy_visc4 > 0.0) ADJ_LAUNCH(k_tan_dyn_j3, ncv, L, s);
-    ADJ_LAUNCH(k_tan_dyn_j4, nE + 1, L, s);
-    ADJ_LAUNCH(k_tan_dyn_j5, nC + 1, L + 1, s);
+    if (dyn_options(*p)) {
+        ADJ_LAUNCH(k_tan_dyn_j4_opt, nE + 1, L, s);
+        ADJ_LAUNCH(k_tan_dyn_j5_opt, nC + 1, L + 1, s);
+    } else {
+        ADJ_LAUNCH(k_tan_dyn_j4, nE + 1, L, s);
+        ADJ_LAUNCH(k_tan_dyn_j5, nC + 1, L + 1, s);
+    }
     return check_launch("dyn_tend_jvp");
 }
 

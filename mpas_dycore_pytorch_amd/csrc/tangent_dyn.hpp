@@ -169,6 +169,8 @@ __device__ __forceinline__ double dyn_dwduz(const mpas_hip_mesh &m, Fwd F, Dot T
                      dot_of(T, u, eu + kk + 1), derw);
 }
 
+// kOpt builds in the curvature and vertical-mixing terms (J4 and J5)
+template <bool kOpt>
 __device__ __forceinline__ void dyn_j4(const mpas_hip_mesh &m, const mpas_hip_dyn_params &P, Fwd F, Dot T,
                                        const DynAdjScratch &S, int e, int k) {
     using namespace dyn;
@@ -255,7 +257,7 @@ __device__ __forceinline__ void dyn_j4(const mpas_hip_mesh &m, const mpas_hip_dy
                 (S.g_delsq_div[o2] - S.g_delsq_div[o1]) * r_dc - (S.g_delsq_vort[w2] - S.g_delsq_vort[w1]) * r_dv;
             dtue -= dre * Y + re * dY;
         }
-        if (P.v_mom_eddy_visc2 > 0.0 && k >= 1 && k <= L - 2) {
+        if (kOpt && P.v_mom_eddy_visc2 > 0.0 && k >= 1 && k <= L - 2) {
             // vertical mixing of u (the mixed u's departures are constant shifts)
             const VmixZ z = vmix_z_edge(m.zgrid + c1 * Lw, m.zgrid + c2 * Lw, k);
             const double ip = 1.0 / (z.zp - z.z0), im = 1.0 / (z.z0 - z.zm);
@@ -270,7 +272,7 @@ __device__ __forceinline__ void dyn_j4(const mpas_hip_mesh &m, const mpas_hip_dy
     }
     // curvature: tend_u -= curvEdge re w_avg + u w_avg re / r_earth
     double dcurv = 0.0;
-    if (P.curvature) {
+    if (kOpt && P.curvature) {
         const long long w1 = (long long)c1 * Lw + k, w2 = (long long)c2 * Lw + k;
         const double wa = 0.25 * (((F[w][w1] + F[w][w1 + 1]) + F[w][w2]) + F[w][w2 + 1]);
         const double dwa =
@@ -389,6 +391,7 @@ __device__ __forceinline__ double dyn_dwdtz(const mpas_hip_mesh &m, const mpas_h
            (drws - drwc) * tsi + (rws[kk] - rwc[kk]) * dtsi;
 }
 
+template <bool kOpt>
 __device__ __forceinline__ void dyn_j5(const mpas_hip_mesh &m, const mpas_hip_dyn_params &P, Fwd F, Dot T,
                                        const DynAdjScratch &S, int c, int k) {
     using namespace dyn;
@@ -442,7 +445,7 @@ __device__ __forceinline__ void dyn_j5(const mpas_hip_mesh &m, const mpas_hip_dy
             const double dY = m.rdzu[k] * (dot_of(T, pp, oc + k) - dot_of(T, pp, oc + k - 1)) -
                               (m.fzm[k] * S.g_dpdz[oc + k] + m.fzp[k] * S.g_dpdz[oc + k - 1]);
             dtwe -= dot_of(T, cqw, ow + k) * Y + F[cqw][ow + k] * dY;
-            if (P.v_mom_eddy_visc2 > 0.0) {
+            if (kOpt && P.v_mom_eddy_visc2 > 0.0) {
                 // vertical mixing of w
                 const double *wc = F[w] + ow, *rz = F[rho_zz] + oc;
                 const double rho_half = 0.5 * (rz[k] + rz[k - 1]);
@@ -461,7 +464,7 @@ __device__ __forceinline__ void dyn_j5(const mpas_hip_mesh &m, const mpas_hip_dy
             const int e = m.edgesOnCell[c * ME + j];
             dh -= m.edgesOnCell_sign[c * ME + j] * S.g_wf[(long long)e * L + k];
         }
-        if (P.curvature) {
+        if (kOpt && P.curvature) {
             // curvature: rh (ur^2 + vr^2) / r_earth + curvCell ur rh in the advective sum
             const double fk = m.fzm[k], gk = m.fzp[k];
             const long long a0 = oc + k, a1 = oc + k - 1;
@@ -519,7 +522,7 @@ __device__ __forceinline__ void dyn_j5(const mpas_hip_mesh &m, const mpas_hip_dy
                 dtte -= es4 * (S.g_delsq_theta[a2] - S.g_delsq_theta[a1]);
             }
         }
-        if (P.v_theta_eddy_visc2 > 0.0 && k >= 1 && k <= L - 2) {
+        if (kOpt && P.v_theta_eddy_visc2 > 0.0 && k >= 1 && k <= L - 2) {
             // vertical mixing of theta_m (or of its departure from t_init, a constant shift)
             const VmixZ z = vmix_z_cell(m.zgrid + c * Lw, k);
             const double ip = 1.0 / (z.zp - z.z0), im = 1.0 / (z.z0 - z.zm);

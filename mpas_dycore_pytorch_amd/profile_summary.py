@@ -200,14 +200,21 @@ def rates(directory: str, counter: str, n: int, cus: int = 256, ghz: float = 2.4
 
 
 def top_kernels(directory: str, n: int) -> str:
-    """rocprofv3's kernel statistics (the ``top_kernels`` view), the ``n`` largest by total time."""
-    rows = _db_rows(directory, "select name, total_calls, total_duration, average, percentage from top_kernels")
-    rows.sort(key=lambda r: -float(r[2]))
+    """rocprofv3's kernel statistics, the ``n`` largest by total time: the
+    ``top_kernels`` view of a rocpd database (microseconds), or the
+    ``*kernel_stats.csv`` of ``--output-format csv`` (nanoseconds)."""
+    rows = [(name, calls, float(total), float(avg), pct) for name, calls, total, avg, pct in
+            _db_rows(directory, "select name, total_calls, total_duration, average, percentage from top_kernels")]
+    for path in sorted(glob.glob(os.path.join(directory, "**", "*kernel_stats.csv"), recursive=True)):
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                rows.append((r["Name"], r["Calls"], float(r["TotalDurationNs"]) / 1e3, float(r["AverageNs"]) / 1e3,
+                             r["Percentage"]))
+    rows.sort(key=lambda r: -r[2])
     lines = ["| kernel | calls | total ms | mean us | % |", "|---|---|---|---|---|"]
     for name, calls, total, avg, pct in rows[:n]:
-        short = re.sub(r"\(.*", "", name).replace("mpas::", "").replace("adj::", "")
-        # the view reports microseconds
-        lines.append(f"| {short} | {calls} | {float(total) / 1e3:.2f} | {float(avg):.1f} | {float(pct):.1f} |")
+        short = re.sub(r"\(.*", "", name).replace("mpas::", "").replace("adj::", "").replace("tan::", "")
+        lines.append(f"| {short} | {calls} | {total / 1e3:.2f} | {avg:.1f} | {float(pct):.1f} |")
     return "\n".join(lines)
 
 
