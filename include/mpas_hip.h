@@ -82,7 +82,7 @@ typedef struct mpas_hip_config {
     double del4u_div_factor; /* [config_del4u_div_factor] 10 */
     int32_t ns;          /* [config_number_of_sub_steps] 2 */
     int32_t split;       /* [config_dynamics_split_steps] 3 */
-    int32_t monotonic;   /* [config_monotonic] 1 */
+    int32_t monotonic;   /* [config_monotonic] or [config_positive_definite]: FCT in the last transport stage */
     int32_t nScalars;
     int32_t moist_start, moist_end; /* 1-based inclusive species range */
     int32_t diagnostics_current; /* 1: skip the first-stage diagnostic refresh */
@@ -103,6 +103,7 @@ typedef struct mpas_hip_config {
     double v_mom_eddy_visc2;      /* [config_v_mom_eddy_visc2]: vertical mixing of u and w */
     double v_theta_eddy_visc2;    /* [config_v_theta_eddy_visc2]: vertical mixing of theta */
     int32_t mix_full;             /* [config_mix_full] 1; 0 mixes departures from u_init/v_init/t_init */
+    int32_t scalar_advection;     /* [config_scalar_advection] 1; 0 leaves the scalars untransported */
 } mpas_hip_config;
 
 /* Number of doubles of workspace needed for one large step. */

@@ -65,7 +65,7 @@ class ConfigStruct(C.Structure):
                 ("mix_fixed", C.c_int32), ("h_theta_eddy_visc2", C.c_double), ("h_mom_eddy_visc4", C.c_double),
                 ("h_theta_eddy_visc4", C.c_double), ("cam_levels", C.c_int32), ("cam_visc2", C.c_double),
                 ("curvature", C.c_int32), ("v_mom_eddy_visc2", C.c_double), ("v_theta_eddy_visc2", C.c_double),
-                ("mix_full", C.c_int32)]
+                ("mix_full", C.c_int32), ("scalar_advection", C.c_int32)]
 
 
 class DynParamsStruct(C.Structure):

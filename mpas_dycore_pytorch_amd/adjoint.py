@@ -502,7 +502,7 @@ def taped_step(dm: DeviceMesh, state: Mapping[str, torch.Tensor], cfg: Any, dt: 
     ruAvg = tape.scale(ruAvg_split, inv_split)
     wwAvg = tape.scale(wwAvg_split, inv_split)
     rho_zz_new = st["rho_zz"]
-    if S > 0:
+    if S > 0 and cfg.scalar_advection:
         for rk_t in (1, 2, 3):
             dt_t = [dt / 2.0 if order == 2 else dt / 3.0, dt / 2.0, dt][rk_t - 1]
             if rk_t == 3 and cfg.monotonic:

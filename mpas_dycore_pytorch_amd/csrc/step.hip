@@ -775,8 +775,9 @@ static int srk3_step_impl(const mpas_hip_mesh *mesh, StepState *st, const mpas_h
     scale_d(st->ruAvg, w.ruAvg_split, inv_split, E, s);
     scale_d(st->wwAvg, w.wwAvg_split, inv_split, Wd, s);
     // split transport RK3 (time_integration.py:3976-4675); rho_zz_new is the
-    // post-dynamics rho_zz already in the state
-    if (S > 0) {
+    // post-dynamics rho_zz already in the state.  Without scalar advection the
+    // scalars keep their values (:3975)
+    if (S > 0 && cfg->scalar_advection) {
         // transport stage lengths and density weights (time_integration.py:3367-3371,
         // advance_scalars.py:271-286)
         const double dts[3] = {rk2 ? dt / 2.0 : dt / 3.0, dt / 2.0, dt};

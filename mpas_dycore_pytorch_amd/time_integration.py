@@ -55,6 +55,8 @@ DEFAULTS = {
     "config_v_mom_eddy_visc2": 0.0,
     "config_v_theta_eddy_visc2": 0.0,
     "config_mix_full": True,
+    "config_scalar_advection": True,
+    "config_positive_definite": False,
 }
 # Short aliases the reference also accepts (config_defaults.py:149-160)
 ALIASES = {"smdiv": "config_smdiv", "len_disp": "config_len_disp", "monotonic": "config_monotonic",
@@ -79,12 +81,12 @@ ALIASES = {"smdiv": "config_smdiv", "len_disp": "config_len_disp", "monotonic": 
 REQUIRED_VALUES = {
     "config_split_dynamics_transport": True,
     "config_apply_lbcs": False,
-    "config_positive_definite": False,
-    "config_scalar_advection": True,
-    "config_w_adv_order": 3, "config_theta_adv_order": 3, "config_scalar_adv_order": 3,
-    "config_u_vadv_order": 3, "config_w_vadv_order": 3, "config_theta_vadv_order": 3,
-    "config_scalar_vadv_order": 3,
 }
+# Advection-order keys the reference validates as integers and otherwise ignores
+# (config_defaults.py:138-146, :604-614): its numerics are third order throughout.
+INERT_ORDER_KEYS = ("config_w_adv_order", "config_theta_adv_order", "config_scalar_adv_order",
+                    "config_u_vadv_order", "config_w_vadv_order", "config_theta_vadv_order",
+                    "config_scalar_vadv_order")
 
 PROGNOSTIC_KEYS = ("u", "w", "theta_m", "rho_zz", "scalars", "rt_diabatic_tend")
 CARRIED_KEYS = ("rho_p", "rtheta_p", "exner", "pressure_p", "ru", "rw")
@@ -159,6 +161,12 @@ def resolve_config(config: Mapping[str, Any] | None, state: Mapping[str, Any], m
     for key, want in REQUIRED_VALUES.items():
         if key in c and not _same(c[key], want):
             raise MpasUnsupportedCapabilityError(capability=key, requested=c[key])
+    for key in INERT_ORDER_KEYS:
+        if key in c:
+            try:
+                int(c[key])
+            except (TypeError, ValueError):
+                raise MpasConfigError(f"Config key '{key}' must be integer-like, got {c[key]!r}.") from None
     order = c["config_time_integration_order"]
     try:
         order_ok = not isinstance(order, bool) and float(order) == int(order) and int(order) in (2, 3)
@@ -172,6 +180,13 @@ def resolve_config(config: Mapping[str, Any] | None, state: Mapping[str, Any], m
     ns, split = int(c["config_number_of_sub_steps"]), int(c["config_dynamics_split_steps"])
     if ns < 1 or split < 1:
         raise MpasConfigError("config_number_of_sub_steps and config_dynamics_split_steps must be >= 1")
+    if split == 1 and n_scalars > 0 and bool(c["config_scalar_advection"]):
+        # with one dynamics substep the reference skips its split transport loop
+        # (time_integration.py:3953) and transports in every RK stage instead
+        raise MpasUnsupportedCapabilityError(
+            "config_dynamics_split_steps=1 transports the scalars inside every RK stage (the reference's "
+            "coupled transport), which the GPU step does not implement", capability="config_dynamics_split_steps",
+            requested=split)
     len_disp = resolve_len_disp(float(c["config_len_disp"]), mesh, state, strict=strict)
     coef3 = float(c["config_coef_3rd_order"])
     if "coef_3rd_order" in state and float(to_numpy(state["coef_3rd_order"])) != coef3:
@@ -185,7 +200,11 @@ def resolve_config(config: Mapping[str, Any] | None, state: Mapping[str, Any], m
     s.dt, s.len_disp, s.epssm, s.smdiv = dt, len_disp, float(c["config_epssm"]), float(c["config_smdiv"])
     s.apvm, s.coef3, s.c_s = float(c["config_apvm_upwinding"]), coef3, c_s
     s.visc4_2dsmag, s.del4u_div_factor = float(c["config_visc4_2dsmag"]), float(c["config_del4u_div_factor"])
-    s.ns, s.split, s.monotonic, s.nScalars = ns, split, 1 if bool(c["config_monotonic"]) else 0, n_scalars
+    # positive-definite transport selects the same limited final stage as the monotonic one
+    # (time_integration.py:3980-3987, :7942-7945)
+    limited = bool(c["config_monotonic"]) or bool(c["config_positive_definite"])
+    s.ns, s.split, s.monotonic, s.nScalars = ns, split, 1 if limited else 0, n_scalars
+    s.scalar_advection = 1 if bool(c["config_scalar_advection"]) else 0
     s.moist_start = int(to_numpy(state.get("moist_start", 1)))
     s.moist_end = int(to_numpy(state.get("moist_end", n_scalars)))
     if s.moist_start < 1 or s.moist_end > n_scalars:

@@ -57,12 +57,13 @@ def srk3(state: dict, mesh: "O.Mesh", dt: float, *, len_disp: float, ns: int = 2
          epssm: float = 0.1, smdiv: float = 0.1, apvm: float = 0.5, coef3: float = 0.25,
          monotonic: bool = True, diagnostics_current: bool = False, order: int = 3,
          rayleigh_levels: int = 0, rayleigh_timescale_days: float = 5.0, mixing: dict | None = None,
-         curvature: bool = False) -> dict:
+         curvature: bool = False, advect_scalars: bool = True) -> dict:
     """One large step on NumPy arrays in the reference layout; returns a new dict.
     ``rayleigh_levels`` > 0 damps u on that many top levels (config_rayleigh_damp_u);
     ``mixing`` passes the horizontal-mixing and CAM keywords of ``oracle.dyn_params``;
     ``curvature`` adds the curvature terms (config_curvature_u), which read the input
-    state's ``ur_cell`` / ``vr_cell`` all step long."""
+    state's ``ur_cell`` / ``vr_cell`` all step long.  ``advect_scalars=False``
+    (config_scalar_advection) leaves the scalars as they are."""
     st = {k: (np.array(v, dtype=np.float64, copy=True) if isinstance(v, np.ndarray) and v.dtype.kind == "f" else v)
           for k, v in state.items()}
     nc, ne, L = mesh.nc, mesh.ne, mesh.L
@@ -163,8 +164,8 @@ def srk3(state: dict, mesh: "O.Mesh", dt: float, *, len_disp: float, ns: int = 2
     ruAvg = ruAvg_split * inv_split
     wwAvg = wwAvg_split * inv_split
     rho_zz_new = st["rho_zz"].copy()
-    # split transport (time_integration.py:3976-4675)
-    for rk_t in (1, 2, 3):
+    # split transport (time_integration.py:3976-4675), skipped without scalar advection (:3975)
+    for rk_t in ((1, 2, 3) if advect_scalars else ()):
         # stage lengths (time_integration.py:3367-3371)
         dt_t = [dt / 2.0 if order == 2 else dt / 3.0, dt / 2.0, dt][rk_t - 1]
         if rk_t == 3 and monotonic:

@@ -88,6 +88,15 @@ OPTION_CASES = {
     "l2_L20_vmix_dep": (2, 20, "quadratic", 1, 900.0, 2, 9494,
                         {"config_v_mom_eddy_visc2": 4.0, "config_v_theta_eddy_visc2": 2.0,
                          "config_mix_full": False}),
+    # transport switches: the explicit final stage (no limiter), the positive-definite
+    # flag (which selects the limited final stage), and no scalar advection at all
+    "l2_L20_nonmono": (2, 20, "quadratic", 1, 900.0, 2, 9595, {"config_monotonic": False}),
+    "l2_L20_posdef": (2, 20, "quadratic", 1, 900.0, 2, 9696,
+                      {"config_monotonic": False, "config_positive_definite": True}),
+    "l2_L20_noadv": (2, 20, "quadratic", 1, 900.0, 2, 9797, {"config_scalar_advection": False}),
+    # one dynamics substep: the reference's split transport loop runs only for more than one
+    # (time_integration.py:3953), and its in-stage transport is off under split transport
+    "l2_L20_split1": (2, 20, "quadratic", 1, 900.0, 2, 9898, {"config_dynamics_split_steps": 1}),
 }
 # reference profiles the departure case mixes against: u_init and v_init are zero
 # in the synthetic initial state, so that case gives them a shape of their own

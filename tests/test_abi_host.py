@@ -116,8 +116,23 @@ def test_config_gate():
                          st, st, meta["dt"], 2)
     assert (cfg.v_mom_eddy_visc2, cfg.v_theta_eddy_visc2, cfg.mix_full) == (4.0, 2.0, 0)
     assert resolve_config(ok, st, st, meta["dt"], 2).mix_full == 1
-    for key, value in (("config_split_dynamics_transport", False), ("config_positive_definite", True),
-                       ("config_apply_lbcs", True)):
+    # positive-definite transport selects the limited final stage (time_integration.py:3980-3987);
+    # scalar advection can be switched off; the advection-order keys are inert integers
+    # (config_defaults.py:604-614)
+    pd = resolve_config({**ok, "monotonic": False, "positive_definite": True}, st, st, meta["dt"], 2)
+    assert pd.monotonic == 1 and pd.scalar_advection == 1
+    assert resolve_config({**ok, "config_monotonic": False}, st, st, meta["dt"], 2).monotonic == 0
+    assert resolve_config({**ok, "advect_scalars": False}, st, st, meta["dt"], 2).scalar_advection == 0
+    assert resolve_config({**ok, "config_scalar_adv_order": 2, "config_w_adv_order": "4"}, st, st, meta["dt"],
+                          2).monotonic == 1
+    with pytest.raises(MpasConfigError):
+        resolve_config({**ok, "config_theta_adv_order": "third"}, st, st, meta["dt"], 2)
+    # one dynamics substep means the reference's coupled (in-stage) transport
+    with pytest.raises(MpasUnsupportedCapabilityError):
+        resolve_config({**ok, "config_dynamics_split_steps": 1}, st, st, meta["dt"], 2)
+    assert resolve_config({**ok, "config_dynamics_split_steps": 1, "config_scalar_advection": False}, st, st,
+                          meta["dt"], 2).split == 1
+    for key, value in (("config_split_dynamics_transport", False), ("config_apply_lbcs", True)):
         with pytest.raises(MpasUnsupportedCapabilityError):
             resolve_config({**ok, key: value}, st, st, meta["dt"], 2)
     # Rayleigh damping of u: the reference's coefficient 1 / (levels * days * 86400)

@@ -14,7 +14,10 @@
 * the vertical eddy viscosities of u, w and theta (``config_v_mom_eddy_visc2`` 4,
   ``config_v_theta_eddy_visc2`` 2; ``dyn_tend.py:2731-2781``, ``:4633-4647``,
   ``:5060-5097``) on the full fields and, with ``config_mix_full=False``, on their
-  departures from the reference profiles u_init / v_init / t_init.
+  departures from the reference profiles u_init / v_init / t_init;
+* the transport switches: ``config_monotonic=False`` (an explicit final stage),
+  ``config_positive_definite=True`` (which selects the limited final stage,
+  ``time_integration.py:3980-3987``) and ``config_scalar_advection=False``.
 
 For each: two RK3 steps against the reference's own recording of the same case
 (``tests/golden/srk3_l2_L20_<option>.npz``), per field at the whole-step bar,
@@ -44,10 +47,18 @@ def _config(st, meta, damp=True):
     return cfg
 
 
-CASES = ("l2_L20_rayleigh", "l2_L20_fixedmix", "l2_L20_cam", "l2_L20_curvature", "l2_L20_vmix", "l2_L20_vmix_dep")
-# how far the step without the option lands from the recording after one step (u)
-MISS = {"l2_L20_rayleigh": 1e-6, "l2_L20_fixedmix": 1e-3, "l2_L20_cam": 1e-6, "l2_L20_curvature": 1e-9,
-        "l2_L20_vmix": 1e-5, "l2_L20_vmix_dep": 1e-4}
+CASES = ("l2_L20_rayleigh", "l2_L20_fixedmix", "l2_L20_cam", "l2_L20_curvature", "l2_L20_vmix", "l2_L20_vmix_dep",
+         "l2_L20_nonmono", "l2_L20_posdef", "l2_L20_noadv")
+# how far the step without the option lands from the recording after one step, in
+# the field the option moves first
+MISS = {"l2_L20_rayleigh": ("u", 1e-6), "l2_L20_fixedmix": ("u", 1e-3), "l2_L20_cam": ("u", 1e-6),
+        "l2_L20_curvature": ("u", 1e-9), "l2_L20_vmix": ("u", 1e-5), "l2_L20_vmix_dep": ("u", 1e-4),
+        "l2_L20_nonmono": ("scalars", 1e-12), "l2_L20_posdef": ("scalars", 1e-12),
+        "l2_L20_noadv": ("scalars", 1e-9)}
+# the option keys of each transport case, and what "without the option" means there
+WITHOUT = {"l2_L20_nonmono": {"config_monotonic": True}, "l2_L20_posdef": {"config_positive_definite": False,
+                                                                           "config_monotonic": False},
+           "l2_L20_noadv": {"config_scalar_advection": True}}
 
 
 def _oracle_options(extra: dict) -> dict:
@@ -68,6 +79,11 @@ def _oracle_options(extra: dict) -> dict:
         kw["mixing"] = mixing
     if extra.get("config_curvature_u"):
         kw["curvature"] = True
+    if "config_monotonic" in extra or "config_positive_definite" in extra:
+        # the positive-definite flag selects the limited final stage (time_integration.py:3980-3987)
+        kw["monotonic"] = bool(extra.get("config_monotonic", True)) or bool(extra.get("config_positive_definite"))
+    if "config_scalar_advection" in extra:
+        kw["advect_scalars"] = bool(extra["config_scalar_advection"])
     return kw
 
 
@@ -80,6 +96,12 @@ def _check_option(cfg, case: str) -> None:
         assert cfg.mix_fixed == 0 and cfg.cam_levels == 6 and cfg.cam_visc2 > 0.0
     elif case == "l2_L20_curvature":
         assert cfg.curvature == 1 and cfg.cam_levels == 0 and cfg.mix_fixed == 0
+    elif case == "l2_L20_nonmono":
+        assert cfg.monotonic == 0 and cfg.scalar_advection == 1
+    elif case == "l2_L20_posdef":
+        assert cfg.monotonic == 1 and cfg.scalar_advection == 1
+    elif case == "l2_L20_noadv":
+        assert cfg.scalar_advection == 0
     else:
         assert cfg.v_mom_eddy_visc2 == 4.0 and cfg.v_theta_eddy_visc2 == 2.0
         assert cfg.mix_full == (0 if case == "l2_L20_vmix_dep" else 1)
@@ -102,12 +124,15 @@ def test_option_whole_steps_match_reference(case):
                 err = rel_err(got[k], ref[k])
                 worst = max(worst, err)
                 assert err <= STEP_TOL, f"{case} step {step + 1} {k}: rel err {err:.3e}"
-    plain = Srk3Stepper(st, st, _config(st, meta, damp=False), meta["dt"], device=torch.device("cuda"))
+    without = _config(st, meta, damp=False)
+    without.update(WITHOUT.get(case, {}))
+    plain = Srk3Stepper(st, st, without, meta["dt"], device=torch.device("cuda"))
     plain.step(1)
-    miss = rel_err(plain.state_dict()["u"].cpu().numpy(), refs[0]["u"])
-    print(f"[options] {case}: worst field error {worst:.2e}; without the option u misses the reference by "
+    field, bar = MISS[case]
+    miss = rel_err(plain.state_dict()[field].cpu().numpy(), refs[0][field])
+    print(f"[options] {case}: worst field error {worst:.2e}; without the option {field} misses the reference by "
           f"{miss:.2e}", flush=True)
-    assert miss > MISS[case]
+    assert miss > bar
 
 
 @pytest.mark.parametrize("case", CASES)

@@ -149,6 +149,46 @@ def test_srk3_whole_steps_mixing_options(case):
     assert rel_err(plain["theta_m"], outs[0]["theta_m"]) > 1e-12
 
 
+@pytest.mark.parametrize("case,kw", [("l2_L20_nonmono", {"monotonic": False}),
+                                     ("l2_L20_posdef", {"monotonic": True}),
+                                     ("l2_L20_noadv", {"advect_scalars": False})])
+def test_srk3_whole_steps_transport_switches(case, kw):
+    """The transport switches over two RK3 steps against the reference's recordings:
+    config_monotonic=False (an explicit final stage), config_positive_definite=True
+    with config_monotonic=False (the reference runs the limited final stage, so the
+    oracle's monotonic step must match it) and config_scalar_advection=False (the
+    scalars stay as they are)."""
+    st, outs, meta = srk3_case(case)
+    mesh = O.Mesh(st)
+    cur = st
+    for step, ref in enumerate(outs):
+        cur = OS.srk3(cur, mesh, meta["dt"], len_disp=float(st["nominalMinDc"]), diagnostics_current=(step > 0), **kw)
+        for k in FIELDS:
+            if k in ref:
+                err = rel_err(cur[k], ref[k])
+                assert err <= 1e-14, f"{case} step {step + 1} {k}: rel err {err:.3e}"
+    if case == "l2_L20_noadv":
+        assert np.array_equal(np.asarray(outs[-1]["scalars"]), np.asarray(st["scalars"]))
+    else:
+        other = OS.srk3(st, mesh, meta["dt"], len_disp=float(st["nominalMinDc"]), monotonic=not kw["monotonic"])
+        assert rel_err(other["scalars"], outs[0]["scalars"]) > 1e-12
+
+
+def test_one_dynamics_substep_transports_in_stage():
+    """With config_dynamics_split_steps=1 the reference skips its split transport loop
+    (time_integration.py:3953) and transports the scalars inside every RK stage.  The
+    recording pins both halves of that: the dynamics of one substep match the
+    oracle's at 1e-14, and its scalars differ from the split transport's, which is
+    why the GPU step rejects the configuration instead of running it."""
+    st, outs, meta = srk3_case("l2_L20_split1")
+    mesh = O.Mesh(st)
+    cur = OS.srk3(st, mesh, meta["dt"], len_disp=float(st["nominalMinDc"]), split=1)
+    for k in FIELDS:
+        if k in outs[0] and k != "scalars":
+            assert rel_err(cur[k], outs[0][k]) <= 1e-14, k
+    assert rel_err(cur["scalars"], outs[0]["scalars"]) > 1e-9
+
+
 @pytest.mark.slow
 def test_level4_summary_from_own_cold_start():
     from mpas_dycore_pytorch_amd import initial_state as IS
