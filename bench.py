@@ -463,7 +463,10 @@ def main() -> int:
         t_tune = time.time()
         tuning = AT.autotune(stepper)
         log(f"autotuned {len(tuning)} kernels in {time.time() - t_tune:.1f} s: "
-            + ", ".join(f"{k}={v['waves']}" for k, v in tuning.items()))
+            + ", ".join(f"{k}={v['waves']}{'L' if v['order'] else ''}" for k, v in tuning.items()))
+        log("block order, ms per launch (xcd chunks / linear): "
+            + ", ".join(f"{k} {v['order_ms'].get('xcd', 0):.4f}/{v['order_ms'].get('linear', 0):.4f}"
+                        for k, v in tuning.items()))
     if args.graph:
         stepper.capture_graph()
     torch.cuda.synchronize()
@@ -558,7 +561,8 @@ def main() -> int:
                    "cells": nC, "levels": L, "dt": args.dt, "parallelism": "single GPU",
                    "mesh_order": "hilbert" if args.reorder else "generator",
                    "launch": "hip graph" if args.graph else "stream",
-                   "waves_per_block": ({k: v["waves"] for k, v in tuning.items()} if tuning else "default 4")},
+                   "waves_per_block": ({k: v["waves"] for k, v in tuning.items()} if tuning else "default 4"),
+                   "linear_block_order": ([k for k, v in tuning.items() if v["order"]] if tuning else [])},
         "roofline": roof, "step_roofline": step_roof, "vjp_fwd_ratio": (vjp or {}).get("ratio"), "vjp": vjp, "mixed_precision": mixed, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

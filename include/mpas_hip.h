@@ -465,6 +465,11 @@ int mpas_hip_tridiagonal_solve_jvp(int64_t nColumns, int32_t nLevels, const doub
  * Written by the autotuner (autotune.py) and read at every launch. */
 int mpas_hip_tuning_set(const char *kernel_name, int32_t waves_per_block);
 int32_t mpas_hip_tuning_get(const char *kernel_name);
+/* Block order of a wavefront-per-entity kernel: 0 (default) deals contiguous chunks of entities
+ * to the 8 XCDs, so neighbouring entities share an L2; 1 keeps the hardware's round-robin
+ * order.  No counterpart in the reference (launch tuning). */
+int mpas_hip_tuning_set_order(const char *kernel_name, int32_t order);
+int32_t mpas_hip_tuning_get_order(const char *kernel_name);
 
 const char *mpas_hip_last_error(void);
 

@@ -24,7 +24,8 @@
  * The reference has no such variant (it runs float64 on the CPU only); this
  * is the reduced-precision form of mpas_hip_srk3_step
  * (include/mpas_hip.h).  The library also exports mpas_hip_tuning_set/get,
- * mpas_hip_timer_arm/read and mpas_hip_last_error with the meanings given in
+ * mpas_hip_tuning_set_order/get_order, mpas_hip_timer_arm/read and
+ * mpas_hip_last_error with the meanings given in
  * mpas_hip.h, acting on its own kernels. */
 #ifndef MPAS_HIP_MIXED_H
 #define MPAS_HIP_MIXED_H

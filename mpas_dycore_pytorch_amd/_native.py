@@ -152,6 +152,8 @@ SIGNATURES = {
     "mpas_hip_history_fields": (C.c_int, [_I32, _I32, _I32, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _vp]),
     "mpas_hip_tuning_set": (C.c_int, [C.c_char_p, _I32]),
     "mpas_hip_tuning_get": (_I32, [C.c_char_p]),
+    "mpas_hip_tuning_set_order": (C.c_int, [C.c_char_p, _I32]),
+    "mpas_hip_tuning_get_order": (_I32, [C.c_char_p]),
     "mpas_hip_last_error": (C.c_char_p, []),
     "mpas_hip_timer_arm": (C.c_int, [C.c_char_p, _I32]),
     "mpas_hip_timer_read": (C.c_int, [C.POINTER(C.c_double), C.POINTER(_I32)]),
@@ -207,6 +209,8 @@ MIXED_SIGNATURES = {
                                            C.POINTER(ConfigStruct), _vp]),
     "mpas_hip_tuning_set": (C.c_int, [C.c_char_p, _I32]),
     "mpas_hip_tuning_get": (_I32, [C.c_char_p]),
+    "mpas_hip_tuning_set_order": (C.c_int, [C.c_char_p, _I32]),
+    "mpas_hip_tuning_get_order": (_I32, [C.c_char_p]),
     "mpas_hip_last_error": (C.c_char_p, []),
     "mpas_hip_timer_arm": (C.c_int, [C.c_char_p, _I32]),
     "mpas_hip_timer_read": (C.c_int, [C.POINTER(C.c_double), C.POINTER(_I32)]),

@@ -1,14 +1,17 @@
 """Launch-shape autotuner for the wavefront-per-entity kernels.
 
 Each wavefront-per-entity kernel (one entity per wavefront, lane = level)
-can be launched with 1, 2, 4, 8 or 16 wavefronts per block; the best choice depends
-on the kernel's register and LDS footprint, on the mesh size and on how the
-launch spreads over the 8 XCDs.  :func:`autotune` times every candidate of
-every kernel on the real workload with HIP events around that kernel's
-launches (``mpas_hip_timer_arm``), one kernel at a time (coordinate search),
-and installs the fastest in the library's launch table
-(``mpas_hip_tuning_set``).  Tuning runs whole large steps, so it advances the
-stepper's state: run it as part of the warm-up.
+has two launch choices: 1, 2, 4, 8 or 16 wavefronts per block, and the block
+order -- contiguous chunks of entities dealt to the 8 XCDs (neighbours share an
+L2; the default) or the hardware's round-robin order.  The best choice depends
+on the kernel's register and LDS footprint, its gather pattern and the mesh
+size.  :func:`autotune` times every candidate of every kernel on the real
+workload with HIP events around that kernel's launches
+(``mpas_hip_timer_arm``), one kernel and one dimension at a time (coordinate
+search: the block shape first, then the order under the best shape), and
+installs the fastest in the library's launch table (``mpas_hip_tuning_set``,
+``mpas_hip_tuning_set_order``).  Tuning runs whole large steps, so it advances
+the stepper's state: run it as part of the warm-up.
 """
 
 from __future__ import annotations
@@ -28,52 +31,75 @@ WAVE_KERNELS: dict[str, int] = {
     "k_diag_vertices": 9,
 }
 CANDIDATES = (1, 2, 4, 8, 16)
+ORDERS = {0: "xcd", 1: "linear"}
 
 
-def current() -> dict[str, int]:
+def current() -> dict[str, dict[str, int]]:
     lib = N.load()
-    return {k: int(lib.mpas_hip_tuning_get(k.encode())) for k in WAVE_KERNELS}
+    return {k: {"waves": int(lib.mpas_hip_tuning_get(k.encode())),
+                "order": int(lib.mpas_hip_tuning_get_order(k.encode()))} for k in WAVE_KERNELS}
 
 
-def install(table: dict[str, int]) -> None:
+def install(table: dict) -> None:
+    """Install ``{kernel: waves}`` or ``{kernel: {"waves": w, "order": o}}``."""
     lib = N.load()
-    for k, w in table.items():
-        N.check(lib.mpas_hip_tuning_set(k.encode(), int(w)), "tuning_set")
+    for k, v in table.items():
+        entry = v if isinstance(v, dict) else {"waves": v}
+        N.check(lib.mpas_hip_tuning_set(k.encode(), int(entry.get("waves", 0))), "tuning_set")
+        N.check(lib.mpas_hip_tuning_set_order(k.encode(), int(entry.get("order", 0))), "tuning_set_order")
 
 
-def autotune(stepper, kernels: Iterable[str] | None = None, candidates=CANDIDATES, steps: int = 2) -> dict:
-    """Tune the launch shape of each kernel on ``stepper``'s workload.
+def _time_kernel(lib, stepper, name: str, steps: int) -> float | None:
+    """Average duration (ms) of ``name``'s launches over ``steps`` large steps."""
+    torch.cuda.synchronize(stepper.device)
+    N.check(lib.mpas_hip_timer_arm(name.encode(), WAVE_KERNELS.get(name, 16) * steps + 16), "timer_arm")
+    stepper.step(steps)
+    tot, n = C.c_double(), C.c_int32()
+    N.check(lib.mpas_hip_timer_read(C.byref(tot), C.byref(n)), "timer_read")
+    return tot.value / n.value if n.value > 0 else None
 
-    Returns ``{kernel: {"waves": best, "avg_ms": {waves: ms}}}`` and leaves
-    the best shapes installed."""
+
+def autotune(stepper, kernels: Iterable[str] | None = None, candidates=CANDIDATES, steps: int = 2,
+             orders=tuple(ORDERS)) -> dict:
+    """Tune the launch shape, then the block order, of each kernel on ``stepper``'s workload.
+
+    Returns ``{kernel: {"waves": best, "order": best, "avg_ms": {waves: ms},
+    "order_ms": {order: ms}}}`` and leaves the best choices installed."""
     lib = N.load()
-    device = stepper.device
     result = {}
     for name in (kernels or WAVE_KERNELS):
-        per_step = WAVE_KERNELS.get(name, 16)
+        N.check(lib.mpas_hip_tuning_set_order(name.encode(), 0), "tuning_set_order")
         times = {}
         for w in candidates:
             N.check(lib.mpas_hip_tuning_set(name.encode(), int(w)), "tuning_set")
-            torch.cuda.synchronize(device)
-            N.check(lib.mpas_hip_timer_arm(name.encode(), per_step * steps + 16), "timer_arm")
-            stepper.step(steps)
-            tot, n = C.c_double(), C.c_int32()
-            N.check(lib.mpas_hip_timer_read(C.byref(tot), C.byref(n)), "timer_read")
-            if n.value > 0:
-                times[w] = tot.value / n.value
+            t = _time_kernel(lib, stepper, name, steps)
+            if t is not None:
+                times[w] = t
         best = min(times, key=times.get) if times else 4
         N.check(lib.mpas_hip_tuning_set(name.encode(), int(best)), "tuning_set")
-        result[name] = {"waves": best, "avg_ms": {str(k): round(v, 5) for k, v in times.items()}}
+        order_times = {0: times[best]} if best in times else {}
+        for o in orders:
+            if o in order_times:
+                continue
+            N.check(lib.mpas_hip_tuning_set_order(name.encode(), int(o)), "tuning_set_order")
+            t = _time_kernel(lib, stepper, name, steps)
+            if t is not None:
+                order_times[o] = t
+        best_order = min(order_times, key=order_times.get) if order_times else 0
+        N.check(lib.mpas_hip_tuning_set_order(name.encode(), int(best_order)), "tuning_set_order")
+        result[name] = {"waves": best, "order": best_order,
+                        "avg_ms": {str(k): round(v, 5) for k, v in times.items()},
+                        "order_ms": {ORDERS[k]: round(v, 5) for k, v in order_times.items()}}
     return result
 
 
-def save(path: str, table: dict[str, int]) -> None:
+def save(path: str, table: dict) -> None:
     with open(path, "w") as f:
         json.dump(table, f, indent=1, sort_keys=True)
 
 
-def load(path: str) -> dict[str, int]:
+def load(path: str) -> dict:
     with open(path) as f:
-        table = {k: int(v) for k, v in json.load(f).items()}
+        table = json.load(f)
     install(table)
     return table

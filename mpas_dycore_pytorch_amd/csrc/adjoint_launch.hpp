@@ -133,8 +133,9 @@ inline MonoScratch carve_mono(const mpas_hip_mesh &m, double *p) {
     do {                                                                                              \
         const int wpb_ = ::mpas::tuned_waves(#kernel) > 4 ? 4 : ::mpas::tuned_waves(#kernel);         \
         const long long nb_ = ::mpas::wave_blocks(n_ent, wpb_);                                       \
-        MPAS_LAUNCH(kernel, dim3(::mpas::xcd_grid(nb_)), dim3(64 * wpb_), 0, stream, nb_,             \
-                    (long long)(n_ent), (int)(n_lev), m, S_, A);                                      \
+        MPAS_LAUNCH(kernel, dim3(::mpas::xcd_grid(nb_)), dim3(64 * wpb_), 0, stream,                  \
+                    nb_ | (::mpas::tuned_order(#kernel) ? ::mpas::kLinearOrder : 0LL), (long long)(n_ent), \
+                    (int)(n_lev), m, S_, A);                                                          \
     } while (0)
 
 

@@ -157,10 +157,18 @@ inline unsigned blocks_for(long long n, int block = kBlock) {
 // entities.  The grid is padded to a multiple of 8; blocks mapped past the
 // end exit.
 constexpr int kXcds = 8;
-inline unsigned xcd_grid(long long nblocks) { return (unsigned)(((nblocks + kXcds - 1) / kXcds) * kXcds); }
+// A launch whose block count carries kLinearOrder keeps the hardware's
+// round-robin block order instead (block b takes the b-th group of entities):
+// a per-kernel choice of the autotuner (tuned_order).
+constexpr long long kLinearOrder = 1LL << 62;
+inline unsigned xcd_grid(long long nblocks) {
+    nblocks &= ~kLinearOrder;
+    return (unsigned)(((nblocks + kXcds - 1) / kXcds) * kXcds);
+}
 __device__ __forceinline__ long long xcd_block(long long nblocks) {
-    const long long per = (nblocks + kXcds - 1) / kXcds;
     const long long b = blockIdx.x;
+    if (nblocks & kLinearOrder) return b;
+    const long long per = (nblocks + kXcds - 1) / kXcds;
     return (b % kXcds) * per + b / kXcds;
 }
 
@@ -180,6 +188,7 @@ constexpr int kWaves = 4;
 constexpr int kMaxWaveBlock = 1024;  // the forward wavefront kernels take up to 16 wavefronts per block
 constexpr int kEntPerWave = 1;
 int tuned_waves(const char *kernel_name);
+int tuned_order(const char *kernel_name);  // 0: blocks dealt to the XCDs in chunks, 1: linear
 inline long long wave_blocks(long long n_entities, int waves = kWaves) {
     return (n_entities + waves * kEntPerWave - 1) / (waves * kEntPerWave);
 }
@@ -194,8 +203,8 @@ __device__ __forceinline__ int wave_lane() { return (int)(threadIdx.x & 63); }
     do {                                                                                           \
         const int wpb_ = ::mpas::tuned_waves(#kernel);                                             \
         const long long nb_ = ::mpas::wave_blocks(n_entities, wpb_);                               \
-        MPAS_LAUNCH(kernel, dim3(::mpas::xcd_grid(nb_)), dim3(64 * wpb_), 0, stream, nb_,          \
-                    __VA_ARGS__);                                                                  \
+        MPAS_LAUNCH(kernel, dim3(::mpas::xcd_grid(nb_)), dim3(64 * wpb_), 0, stream,               \
+                    nb_ | (::mpas::tuned_order(#kernel) ? ::mpas::kLinearOrder : 0LL), __VA_ARGS__);   \
     } while (0)
 
 // Zero-fill by kernel rather than hipMemsetAsync so launch sequences stay

@@ -54,7 +54,8 @@ static void timer_release() {
 // ---------------------------------------------------------------------------
 struct TuneEntry {
     char name[64];
-    int waves;
+    int waves;  // 0: the default
+    int order;  // 0: chunks per XCD (the default), 1: linear
 };
 constexpr int kTuneCapacity = 128;
 static TuneEntry g_tune[kTuneCapacity];
@@ -64,6 +65,25 @@ int tuned_waves(const char *kernel_name) {
     for (int i = 0; i < g_tune_n; ++i)
         if (g_tune[i].waves > 0 && std::strcmp(g_tune[i].name, kernel_name) == 0) return g_tune[i].waves;
     return kWaves;
+}
+
+int tuned_order(const char *kernel_name) {
+    for (int i = 0; i < g_tune_n; ++i)
+        if (std::strcmp(g_tune[i].name, kernel_name) == 0) return g_tune[i].order;
+    return 0;
+}
+
+// the table entry of a kernel, added if absent (nullptr when the table is full)
+static TuneEntry *tune_entry(const char *kernel_name) {
+    for (int i = 0; i < g_tune_n; ++i)
+        if (std::strcmp(g_tune[i].name, kernel_name) == 0) return &g_tune[i];
+    if (g_tune_n == kTuneCapacity) return nullptr;
+    TuneEntry &t = g_tune[g_tune_n];
+    std::strcpy(t.name, kernel_name);
+    t.waves = 0;
+    t.order = 0;
+    ++g_tune_n;
+    return &t;
 }
 
 int check_launch(const char *what) {
@@ -345,23 +365,36 @@ int mpas_hip_tuning_set(const char *kernel_name, int32_t waves_per_block) {
         set_error("tuning_set: waves per block must be 1, 2, 4, 8 or 16 (0 restores the default)");
         return 1;
     }
-    for (int i = 0; i < g_tune_n; ++i)
-        if (std::strcmp(g_tune[i].name, kernel_name) == 0) {
-            g_tune[i].waves = waves_per_block;
-            return 0;
-        }
-    if (waves_per_block == 0) return 0;
-    if (g_tune_n == kTuneCapacity) {
+    TuneEntry *t = tune_entry(kernel_name);
+    if (!t) {
         set_error("tuning_set: table full");
         return 1;
     }
-    std::strcpy(g_tune[g_tune_n].name, kernel_name);
-    g_tune[g_tune_n].waves = waves_per_block;
-    ++g_tune_n;
+    t->waves = waves_per_block;
     return 0;
 }
 
 int32_t mpas_hip_tuning_get(const char *kernel_name) { return kernel_name ? tuned_waves(kernel_name) : kWaves; }
+
+int mpas_hip_tuning_set_order(const char *kernel_name, int32_t order) {
+    if (!kernel_name || std::strlen(kernel_name) >= sizeof(g_tune[0].name)) {
+        set_error("tuning_set_order: kernel name missing or too long");
+        return 1;
+    }
+    if (order != 0 && order != 1) {
+        set_error("tuning_set_order: the block order is 0 (chunks per XCD) or 1 (linear)");
+        return 1;
+    }
+    TuneEntry *t = tune_entry(kernel_name);
+    if (!t) {
+        set_error("tuning_set_order: table full");
+        return 1;
+    }
+    t->order = order;
+    return 0;
+}
+
+int32_t mpas_hip_tuning_get_order(const char *kernel_name) { return kernel_name ? tuned_order(kernel_name) : 0; }
 
 const char *mpas_hip_last_error(void) {
     return g_last_error.c_str();

@@ -240,6 +240,13 @@ def test_launch_shape_table():
     assert lib.mpas_hip_tuning_set(b"k_dyn_edges_a", 3) != 0  # not 1, 2, 4, 8 or 16
     assert lib.mpas_hip_tuning_set(b"k_dyn_edges_a", 0) == 0
     assert lib.mpas_hip_tuning_get(b"k_dyn_edges_a") == 4
+    assert lib.mpas_hip_tuning_get_order(b"k_dyn_edges_a") == 0
+    assert lib.mpas_hip_tuning_set_order(b"k_dyn_edges_a", 1) == 0
+    assert lib.mpas_hip_tuning_get_order(b"k_dyn_edges_a") == 1
+    assert lib.mpas_hip_tuning_get(b"k_dyn_edges_a") == 4  # the order leaves the shape alone
+    assert lib.mpas_hip_tuning_set_order(b"k_dyn_edges_a", 2) != 0  # 0 or 1
+    assert lib.mpas_hip_tuning_set_order(b"k_dyn_edges_a", 0) == 0
+    assert lib.mpas_hip_tuning_get_order(b"k_dyn_edges_a") == 0
 
 
 def test_run_simulation_jvp_validates_the_perturbation():

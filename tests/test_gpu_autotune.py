@@ -1,4 +1,4 @@
-"""Launch shapes from the autotuner change timing only: results stay bitwise equal."""
+"""Launch shapes and block orders from the autotuner change timing only: results stay bitwise equal."""
 
 from __future__ import annotations
 
@@ -19,14 +19,15 @@ def test_launch_shapes_do_not_change_results():
     cfg = {"config_time_integration_order": 3, "config_len_disp": float(st["nominalMinDc"])}
     outs = []
     try:
-        for waves in (4, 1, 2, 16):
-            AT.install({k: waves for k in AT.WAVE_KERNELS})
+        for waves, order in ((4, 0), (1, 0), (2, 1), (16, 0), (4, 1)):
+            AT.install({k: {"waves": waves, "order": order} for k in AT.WAVE_KERNELS})
             sp = Srk3Stepper(st, st, cfg, meta["dt"])
             sp.step(2)
             outs.append({k: v.cpu().numpy() for k, v in sp.state_dict().items()})
         sp = Srk3Stepper(st, st, cfg, meta["dt"])
         table = AT.autotune(sp, steps=1)
         assert set(table) == set(AT.WAVE_KERNELS) and all(v["waves"] in AT.CANDIDATES for v in table.values())
+        assert all(v["order"] in AT.ORDERS for v in table.values())
     finally:
         AT.install({k: 0 for k in AT.WAVE_KERNELS})
     for o in outs[1:]:
