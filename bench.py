@@ -147,7 +147,8 @@ def reference_cpu_timing() -> dict | None:
             "where": "reference atm_srk3 on the build container's CPU (Intel Xeon 2.7 GHz), torch float64"}
 
 
-def vjp_leg(m, stepper, config, dt: float, steps: int, fwd_ms_per_step: float, device, jvp: bool = True) -> dict:
+def vjp_leg(m, stepper, config, dt: float, steps: int, fwd_ms_per_step: float, device, jvp: bool = True,
+            tune: bool = False) -> dict:
     # m: the prepared mapping (mesh, base state and initial fields, host arrays)
     """Forward + VJP of ``steps`` large steps from the stepper's current state.
 
@@ -176,6 +177,16 @@ def vjp_leg(m, stepper, config, dt: float, steps: int, fwd_ms_per_step: float, d
     keys = ("u", "w", "theta_m", "rho_zz")
     # untimed warm-up: device mesh, inverse tables and allocator pools
     run_simulation.vjp(state, m, objective, track_keys=keys, config=config, dt=dt, nsteps=1, device=device)
+    tuning = None
+    if tune:
+        # the adjoint kernels' launch shapes and block orders, timed on 1-step VJPs (untimed)
+        from mpas_dycore_pytorch_amd import autotune as AT
+
+        t_tune = time.time()
+        tuning = AT.autotune_reverse(lambda: run_simulation.vjp(state, m, objective, track_keys=keys, config=config,
+                                                                dt=dt, nsteps=1, device=device))
+        log(f"autotuned {len(tuning)} adjoint kernels in {time.time() - t_tune:.1f} s: "
+            + ", ".join(f"{k}={v['waves']}{'L' if v['order'] else ''}" for k, v in tuning.items()))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     grads = run_simulation.vjp(state, m, objective, track_keys=keys, config=config, dt=dt, nsteps=steps,
@@ -188,7 +199,9 @@ def vjp_leg(m, stepper, config, dt: float, steps: int, fwd_ms_per_step: float, d
     peak = torch.cuda.max_memory_allocated(device) / 2**30
     out = {"steps": steps, "fwd_plus_vjp_ms_per_step": round(1e3 * total_s / steps, 2),
            "ratio": round(1e3 * total_s / steps / fwd_ms_per_step, 3), "peak_hbm_gib": round(peak, 1),
-           "checkpoint": "per large step (acoustic substeps recomputed); the last step keeps its forward tape"}
+           "checkpoint": "per large step (acoustic substeps recomputed); the last step keeps its forward tape",
+           "adjoint_launch": ({k: f"{v['waves']}{' linear' if v['order'] else ''}" for k, v in tuning.items()}
+                              if tuning else "default 4 waves")}
     if not jvp:
         return out
     # forward mode on the same state: seeded tangents N(0, 0.1) on u and N(0, 0.01) on w (interior
@@ -201,6 +214,15 @@ def vjp_leg(m, stepper, config, dt: float, steps: int, fwd_ms_per_step: float, d
     tw[:, 0] = 0.0
     tw[:, -1] = 0.0
     run_simulation.jvp(state, m, {"u": tu, "w": tw}, config=config, dt=dt, nsteps=1, device=device)
+    tan_tuning = None
+    if tune:
+        from mpas_dycore_pytorch_amd import autotune as AT
+
+        t_tune = time.time()
+        tan_tuning = AT.autotune_tangent(lambda: run_simulation.jvp(state, m, {"u": tu, "w": tw}, config=config,
+                                                                    dt=dt, nsteps=1, device=device))
+        log(f"autotuned {len(tan_tuning)} tangent kernels in {time.time() - t_tune:.1f} s: "
+            + ", ".join(f"{k}={v['waves']}{'L' if v['order'] else ''}" for k, v in tan_tuning.items()))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     tan = run_simulation.jvp(state, m, {"u": tu, "w": tw}, config=config, dt=dt, nsteps=steps, device=device)
@@ -209,7 +231,9 @@ def vjp_leg(m, stepper, config, dt: float, steps: int, fwd_ms_per_step: float, d
     if not bool(torch.isfinite(tan["theta_m"]).all()):
         raise RuntimeError("non-finite theta_m tangent")
     out.update({"fwd_plus_jvp_ms_per_step": round(1e3 * jvp_s / steps, 2),
-                "jvp_ratio": round(1e3 * jvp_s / steps / fwd_ms_per_step, 3)})
+                "jvp_ratio": round(1e3 * jvp_s / steps / fwd_ms_per_step, 3),
+                "tangent_launch": ({k: f"{v['waves']}{' linear' if v['order'] else ''}" for k, v in tan_tuning.items()}
+                                   if tan_tuning else "default 4 waves")})
     return out
 
 
@@ -539,7 +563,8 @@ def main() -> int:
 
     vjp = None
     if args.vjp_steps > 0:
-        vjp = vjp_leg(st, stepper, cfg, args.dt, args.vjp_steps, ms_per_step, device, jvp=not args.no_jvp)
+        vjp = vjp_leg(st, stepper, cfg, args.dt, args.vjp_steps, ms_per_step, device, jvp=not args.no_jvp,
+                      tune=bool(args.autotune))
         log(f"reverse mode: {vjp}")
 
     cpu = None

@@ -35,6 +35,24 @@ WAVE_KERNELS: dict[str, int] = {
     # the variants with the option terms: launched only when an option is on
     "k_dyn_edges_a_opt": 9, "k_dyn_cells_b_opt": 9,
 }
+# adjoint kernels launched one wavefront per entity (at most 4 per block), with their
+# launches per large step of a VJP
+ADJOINT_KERNELS: dict[str, int] = {
+    "k_adj_dyn_a3w": 9, "k_adj_dyn_a3_cells": 9, "k_adj_dyn_a3_edges": 9, "k_adj_dyn_a1": 9,
+    "k_adj_dyn_a2_cells": 3, "k_adj_dyn_a2_edges": 3, "k_adj_dyn_a6": 3, "k_adj_dyn_a7": 9,
+    "k_adj_dyn_fwd_edges": 9, "k_adj_dyn_fwd_cells": 9, "k_adj_diag_g1": 10, "k_adj_diag_g2": 10,
+    "k_adj_diag_g3": 10, "k_adj_diag_g4": 10, "k_adj_acoustic_edge_flux": 12, "k_adj_acoustic_theta": 12,
+    "k_adj_recover_cell": 9, "k_adj_recover_edge": 9, "k_adj_recover_w": 9, "k_adj_smlstep_edges": 9,
+    "k_adj_smlstep_cells": 9,
+}
+# tangent kernels launched the same way, with their launches per large step of a JVP
+TANGENT_KERNELS: dict[str, int] = {
+    "k_tan_dyn_j1": 9, "k_tan_dyn_j2": 9, "k_tan_dyn_j3": 9, "k_tan_dyn_j4": 9, "k_tan_dyn_j5": 9,
+    "k_tan_dyn_fwd_edges": 9, "k_tan_dyn_fwd_cells": 9, "k_tan_diag_edge": 10, "k_tan_diag_cell": 10,
+    "k_tan_diag_vertex": 10, "k_tan_acoustic_edge": 12, "k_tan_recover_cell": 9, "k_tan_recover_edge": 9,
+    "k_tan_recover_w": 9, "k_tan_smlstep": 9, "k_tan_scal_edge": 2, "k_tan_scal_cell": 2,
+}
+ADJOINT_CANDIDATES = (1, 2, 4)
 CANDIDATES = (1, 2, 4, 8, 16)
 ORDERS = {0: "xcd", 1: "linear"}
 
@@ -54,11 +72,11 @@ def install(table: dict) -> None:
         N.check(lib.mpas_hip_tuning_set_order(k.encode(), int(entry.get("order", 0))), "tuning_set_order")
 
 
-def _time_kernel(lib, stepper, name: str, steps: int) -> float | None:
-    """Average duration (ms) of ``name``'s launches over ``steps`` large steps."""
-    torch.cuda.synchronize(stepper.device)
-    N.check(lib.mpas_hip_timer_arm(name.encode(), WAVE_KERNELS.get(name, 16) * steps + 16), "timer_arm")
-    stepper.step(steps)
+def _time_kernel(lib, run, name: str, capacity: int) -> float | None:
+    """Average duration (ms) of ``name``'s launches during ``run()``."""
+    torch.cuda.synchronize()
+    N.check(lib.mpas_hip_timer_arm(name.encode(), capacity), "timer_arm")
+    run()
     tot, n = C.c_double(), C.c_int32()
     N.check(lib.mpas_hip_timer_read(C.byref(tot), C.byref(n)), "timer_read")
     return tot.value / n.value if n.value > 0 else None
@@ -70,14 +88,38 @@ def autotune(stepper, kernels: Iterable[str] | None = None, candidates=CANDIDATE
 
     Returns ``{kernel: {"waves": best, "order": best, "avg_ms": {waves: ms},
     "order_ms": {order: ms}}}`` and leaves the best choices installed."""
+    return tune(lambda: stepper.step(steps), {k: WAVE_KERNELS.get(k, 16) * steps for k in (kernels or WAVE_KERNELS)},
+                candidates, orders)
+
+
+def autotune_reverse(run_vjp, kernels: Iterable[str] | None = None, candidates=ADJOINT_CANDIDATES,
+                     orders=tuple(ORDERS), steps: int = 1) -> dict:
+    """The same search over the adjoint kernels, timed on ``run_vjp()`` (a VJP of
+    ``steps`` large steps); as :func:`autotune`, it leaves the best choices installed."""
+    return tune(run_vjp, {k: ADJOINT_KERNELS.get(k, 16) * steps for k in (kernels or ADJOINT_KERNELS)},
+                candidates, orders)
+
+
+def autotune_tangent(run_jvp, kernels: Iterable[str] | None = None, candidates=ADJOINT_CANDIDATES,
+                     orders=tuple(ORDERS), steps: int = 1) -> dict:
+    """The same search over the tangent kernels, timed on ``run_jvp()`` (a JVP of
+    ``steps`` large steps)."""
+    return tune(run_jvp, {k: TANGENT_KERNELS.get(k, 16) * steps for k in (kernels or TANGENT_KERNELS)},
+                candidates, orders)
+
+
+def tune(run, kernels: dict[str, int], candidates, orders=tuple(ORDERS)) -> dict:
+    """Coordinate search over ``kernels`` (name: launches per ``run()``): each kernel's
+    shape under the default order, then its order under the best shape."""
     lib = N.load()
     result = {}
-    for name in (kernels or WAVE_KERNELS):
+    for name, launches in kernels.items():
+        capacity = launches + 16
         N.check(lib.mpas_hip_tuning_set_order(name.encode(), 0), "tuning_set_order")
         times = {}
         for w in candidates:
             N.check(lib.mpas_hip_tuning_set(name.encode(), int(w)), "tuning_set")
-            t = _time_kernel(lib, stepper, name, steps)
+            t = _time_kernel(lib, run, name, capacity)
             if t is not None:
                 times[w] = t
         if not times:  # not launched on this workload: keep the defaults
@@ -90,7 +132,7 @@ def autotune(stepper, kernels: Iterable[str] | None = None, candidates=CANDIDATE
             if o in order_times:
                 continue
             N.check(lib.mpas_hip_tuning_set_order(name.encode(), int(o)), "tuning_set_order")
-            t = _time_kernel(lib, stepper, name, steps)
+            t = _time_kernel(lib, run, name, capacity)
             if t is not None:
                 order_times[o] = t
         best_order = min(order_times, key=order_times.get) if order_times else 0
