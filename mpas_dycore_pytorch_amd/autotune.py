@@ -89,7 +89,7 @@ def autotune(stepper, kernels: Iterable[str] | None = None, candidates=CANDIDATE
     Returns ``{kernel: {"waves": best, "order": best, "avg_ms": {waves: ms},
     "order_ms": {order: ms}}}`` and leaves the best choices installed."""
     return tune(lambda: stepper.step(steps), {k: WAVE_KERNELS.get(k, 16) * steps for k in (kernels or WAVE_KERNELS)},
-                candidates, orders)
+                candidates, orders, lib=getattr(stepper, "lib", None))
 
 
 def autotune_reverse(run_vjp, kernels: Iterable[str] | None = None, candidates=ADJOINT_CANDIDATES,
@@ -108,10 +108,12 @@ def autotune_tangent(run_jvp, kernels: Iterable[str] | None = None, candidates=A
                 candidates, orders)
 
 
-def tune(run, kernels: dict[str, int], candidates, orders=tuple(ORDERS)) -> dict:
+def tune(run, kernels: dict[str, int], candidates, orders=tuple(ORDERS), lib=None) -> dict:
     """Coordinate search over ``kernels`` (name: launches per ``run()``): each kernel's
-    shape under the default order, then its order under the best shape."""
-    lib = N.load()
+    shape under the default order, then its order under the best shape.  ``lib`` is the
+    library whose kernels ``run()`` launches (the FP64 one by default; the mixed-precision
+    library keeps a table of its own)."""
+    lib = lib or N.load()
     result = {}
     for name, launches in kernels.items():
         capacity = launches + 16
